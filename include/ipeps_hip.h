@@ -1,0 +1,127 @@
+/*
+ * ipeps_hip -- C ABI of the MI355X (gfx950) CTMRG engine.
+ *
+ * Conventions
+ *  - Tensors are dense, column-major (Julia layout), with explicit int64 dims.
+ *  - dtype codes: IPEPS_F64 (double), IPEPS_C128 (interleaved re,im doubles =
+ *    ComplexF64 = C double _Complex), IPEPS_MIXED (environment stored in FP32,
+ *    contractions on split-bf16 MFMA with FP32 accumulation, FP64 projector
+ *    SVD / orthogonalisation / energy reduction; host buffers are FP64).
+ *  - Hamiltonian terms H1 (horizontal bond) and H2 (vertical bond) are
+ *    hdim x hdim ComplexF64 matrices (hdim = d^2), site-1 index fastest as
+ *    tout() builds them (src/model/basis.jl:16-21).
+ *  - Host pointers are borrowed for the duration of a call only.  Device state
+ *    lives behind the opaque handles and is freed by the destroy calls.
+ *  - Every entry point returns 0 on success or a nonzero status; the message
+ *    is available from ipeps_last_error() on the calling thread.
+ *  - One HIP stream per context; calls on one handle are not re-entrant,
+ *    separate contexts may be used from separate threads.
+ *
+ * Each entry point names the reference (yitan1/iPEPS.jl) function it
+ * replaces.
+ */
+#ifndef IPEPS_HIP_H
+#define IPEPS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPEPS_F64 0
+#define IPEPS_C128 1
+#define IPEPS_MIXED 2
+
+typedef struct ipeps_ctx ipeps_ctx;
+typedef struct ipeps_env ipeps_env;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int ipeps_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* ipeps_last_error(void);
+
+/* Context: device, stream, caching allocator, solver handle. */
+int ipeps_ctx_create(int device, ipeps_ctx** out);
+int ipeps_ctx_destroy(ipeps_ctx* ctx);
+int ipeps_ctx_sync(ipeps_ctx* ctx);
+/* Live / peak device bytes of the context allocator. */
+int ipeps_ctx_mem(ipeps_ctx* ctx, int64_t* live, int64_t* peak);
+
+/* CTMTensors(A, cfg) -> init_ctm1 (src/ctm/ctm_tensor.jl:63-69, 97-130).
+ * dims = (D, D, D, D, d); A is F64 or C128 host data (MIXED takes F64). */
+int ipeps_env_create(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const void* A, ipeps_env** out);
+int ipeps_env_destroy(ipeps_env* env);
+/* Engine options: "svd_mode" (0 full SVD of M = R1 R2 as the reference,
+ * 1 randomized range finder), "oversample", "power_iters", "full_below",
+ * "seed", "canonical_edge" (0/1), "mixed" (1, 3 or 6 bf16 products),
+ * "energy_fp64" (0/1). */
+int ipeps_env_set_option(ipeps_env* env, const char* key, double value);
+/* which: 0..3 = C1..C4, 4..7 = E1..E4, 8 = A.  dims has room for 5. */
+int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd);
+/* Copy out / in (host data in the env's host dtype: F64 or C128). */
+int ipeps_env_get_tensor(ipeps_env* env, int which, void* host);
+int ipeps_env_set_tensor(ipeps_env* env, int which, int nd, const int64_t* dims, const void* host);
+/* setproperties(ts, A=A, Ad=conj(A)) keeping the environment
+ * (src/optimize/optim_gs.jl:134). */
+int ipeps_env_set_A(ipeps_env* env, const void* A);
+
+/* rg_step (src/ctm/ctmrg.jl:57-64): one left/right/top/bottom step; writes
+ * the left move's truncated spectrum (s_len values, at most s_cap). */
+int ipeps_rg_step(ipeps_env* env, int64_t chi, double* s_out, int64_t s_cap, int64_t* s_len);
+
+/* run_ctm (src/ctm/ctmrg.jl:1-55) with its stopping rule; conv_mode 0 uses
+ * get_gs_energy as conv_fun, 1 the left spectrum.  E_out = (re, im) of the
+ * last conv energy; diffs_out receives the diff trace. */
+int ipeps_run_ctm(ipeps_env* env, int64_t chi, int min_iter, int max_iter, double tol, int conv_mode,
+                  const double* H1, const double* H2, int64_t hdim, int* iters, double* E_out,
+                  double* diffs_out, int64_t diffs_cap, int64_t* n_diffs);
+
+/* get_gs_energy (src/graph/evaluation.jl:15-29): E and Na as (re, im). */
+int ipeps_gs_energy(ipeps_env* env, const double* H1, const double* H2, int64_t hdim, double* E_out,
+                    double* N_out);
+/* get_gs_norm (src/graph/evaluation.jl:47-59): (re, im). */
+int ipeps_gs_norm(ipeps_env* env, double* out);
+
+/* One CTMRG step plus its adjoint: forward rg_step, the energy and its VJP,
+ * then the step VJP (the hand-written adjoint used by the gradient).  The
+ * environment advances by one step.  gradA_host gets dRe(E)/dA (A's dtype;
+ * MIXED returns F64). */
+int ipeps_step_adjoint(ipeps_env* env, int64_t chi, const double* H1, const double* H2, int64_t hdim,
+                       double* E_out, void* gradA_host);
+
+/* fg! of optim_gs (src/optimize/optim_gs.jl:53-91, 120-143):
+ * x = renormalize(A0); converge CTMRG from init_ctm1(x) (min/max_iter, tol,
+ * energy conv); then ad_max_iter differentiated steps from the converged
+ * environment held constant; E = get_gs_energy.  grad_out = dRe(E)/dRe(x) +
+ * i dRe(E)/dIm(x) (A's dtype; MIXED returns F64).  options: "key=value;..."
+ * as ipeps_env_set_option.  iters_out = (converging iterations, ad steps). */
+int ipeps_gs_energy_grad(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const void* A0, const double* H1,
+                         const double* H2, int64_t hdim, int64_t chi, int min_iter, int max_iter,
+                         int ad_max_iter, double tol, const char* options, double* E_out, void* grad_out,
+                         int* iters_out);
+
+/* tcon (src/utility/wrapper.jl:15-23): ncon-style pairwise contraction on the
+ * device.  Positive labels are summed, negative ones are output legs -1, -2,
+ * ...  dimsC receives the output dims (room for 16), C_host the result
+ * (capacity C_cap elements). */
+int ipeps_tcon(ipeps_ctx* ctx, int dtype, int ndA, const int64_t* dimsA, const void* A, const int* la, int ndB,
+               const int64_t* dimsB, const void* B, const int* lb, int64_t* dimsC, int* ndC, void* C_host,
+               int64_t C_cap);
+
+/* svd1 (src/utility/autodiff.jl:22-25): full SVD A = U diag(S) V^H of an
+ * m x n matrix; U is m x m, V is n x n (V, not V^H), S has min(m,n). */
+int ipeps_svd(ipeps_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A, void* U, double* S, void* V);
+
+/* svd_back (src/utility/autodiff.jl:41-75) for a square N x N matrix with
+ * full U, S, V and cotangents of the first n columns dU (N x n), dS (n),
+ * dV (N x n) (any may be NULL); eta is the Lorentzian broadening (1e-12 in
+ * the reference).  dA_out is N x N. */
+int ipeps_svd_back(ipeps_ctx* ctx, int dtype, int64_t N, int64_t n, const void* U, const double* S, const void* V,
+                   const void* dU, const double* dS, const void* dV, double eta, void* dA_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

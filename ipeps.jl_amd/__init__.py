@@ -1,0 +1,14 @@
+"""ipeps_amd -- MI355X (gfx950) CTMRG engine for iPEPS with a host API that
+mirrors yitan1/iPEPS.jl.
+
+The numerics run in libipeps_hip.so (hand-written HIP kernels behind a C
+ABI, include/ipeps_hip.h); this package is the ctypes host side.
+"""
+from ._lib import C128, F64, MIXED, LIB_PATH, IpepsError, lib
+from .ctm import (ENV_OPTIONS, Context, CTMTensors, compute_gs_energy, default_context, get_gs_energy,
+                  get_gs_norm, gs_energy_grad, rg_step, run_ctm, step_adjoint, svd1, svd_back, tcon)
+from . import models
+
+__all__ = ["C128", "F64", "MIXED", "LIB_PATH", "IpepsError", "lib", "ENV_OPTIONS", "Context", "CTMTensors",
+           "compute_gs_energy", "default_context", "get_gs_energy", "get_gs_norm", "gs_energy_grad", "rg_step",
+           "run_ctm", "step_adjoint", "svd1", "svd_back", "tcon", "models"]

@@ -1,0 +1,372 @@
+// extern "C" entry points (declared in include/ipeps_hip.h).
+#include <cstring>
+#include <sstream>
+#include <string>
+
+#include "../../include/ipeps_hip.h"
+#include "ctmrg.h"
+
+using namespace ipeps;
+
+struct ipeps_ctx {
+  Ctx* c = nullptr;
+};
+
+struct ipeps_env {
+  ipeps_ctx* ctx = nullptr;
+  int dtype = IPEPS_F64;
+  ctm::Env env;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return IPEPS_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code ? e.code : IPEPS_ERR_INTERNAL;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return IPEPS_ERR_INTERNAL;
+  } catch (...) {
+    g_err = "unknown error";
+    return IPEPS_ERR_INTERNAL;
+  }
+}
+
+DType host_dt(int dtype) {
+  if (dtype == IPEPS_F64 || dtype == IPEPS_MIXED) return DType::F64;
+  if (dtype == IPEPS_C128) return DType::C128;
+  throw Error(IPEPS_ERR_ARG, "unknown dtype code " + std::to_string(dtype));
+}
+
+DType dev_dt(int dtype) {
+  if (dtype == IPEPS_MIXED) return DType::F32;
+  return host_dt(dtype);
+}
+
+Tensor upload(Ctx& c, DType hdt, DType ddt, const std::vector<int64_t>& dims, const void* host) {
+  IPEPS_CHECK(host != nullptr, "null host pointer");
+  Tensor t = c.empty(hdt, dims);
+  to_device(c, t, host);
+  if (ddt != hdt) t = op::to_dtype(c, t, ddt);
+  return t;
+}
+
+void download(Ctx& c, const Tensor& t, DType hdt, void* host) {
+  IPEPS_CHECK(host != nullptr, "null host pointer");
+  Tensor h = t.dt == hdt ? t : op::to_dtype(c, t, hdt);
+  to_host(c, h, host);
+}
+
+Tensor upload_H(Ctx& c, const double* H, int64_t hdim) {
+  IPEPS_CHECK(H != nullptr && hdim > 0, "null Hamiltonian");
+  Tensor t = c.empty(DType::C128, {hdim, hdim});
+  to_device(c, t, H);
+  return t;
+}
+
+void set_opt(ctm::Options& o, const std::string& key, double v) {
+  if (key == "svd_mode") o.svd.mode = (int)v;
+  else if (key == "oversample") o.svd.oversample = (int)v;
+  else if (key == "power_iters") o.svd.power_iters = (int)v;
+  else if (key == "full_below") o.svd.full_below = (int)v;
+  else if (key == "seed") o.svd.seed = (uint64_t)v;
+  else if (key == "eta") o.svd.eta = v;
+  else if (key == "canonical_edge") o.canonical_edge = v != 0.0;
+  else if (key == "energy_fp64") o.energy_fp64 = v != 0.0;
+  else if (key == "mixed") {
+    int m = (int)v;
+    IPEPS_CHECK(m == 1 || m == 3 || m == 6, "mixed must be 1, 3 or 6");
+    o.mixed = (k::Mixed)m;
+  } else
+    throw Error(IPEPS_ERR_ARG, "unknown option '" + key + "'");
+}
+
+void parse_opts(ctm::Options& o, const char* s) {
+  if (!s) return;
+  std::stringstream ss(s);
+  std::string item;
+  while (std::getline(ss, item, ';')) {
+    if (item.empty()) continue;
+    auto eq = item.find('=');
+    IPEPS_CHECK(eq != std::string::npos, "options: expected key=value");
+    set_opt(o, item.substr(0, eq), std::stod(item.substr(eq + 1)));
+  }
+}
+
+Tensor& env_slot(ctm::Env& e, int which) {
+  if (which >= 0 && which < 4) return e.C[which];
+  if (which >= 4 && which < 8) return e.E[which - 4];
+  if (which == 8) return e.A;
+  throw Error(IPEPS_ERR_ARG, "env tensor index must be 0..8");
+}
+
+}  // namespace
+
+extern "C" {
+
+int ipeps_version(void) { return 10000; }
+
+const char* ipeps_last_error(void) { return g_err.c_str(); }
+
+int ipeps_ctx_create(int device, ipeps_ctx** out) {
+  return guard([&] {
+    IPEPS_CHECK(out, "null out");
+    auto* h = new ipeps_ctx();
+    try {
+      h->c = ctx_create(device);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int ipeps_ctx_destroy(ipeps_ctx* ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    ctx_destroy(ctx->c);
+    delete ctx;
+  });
+}
+
+int ipeps_ctx_sync(ipeps_ctx* ctx) {
+  return guard([&] {
+    IPEPS_CHECK(ctx, "null ctx");
+    ctx->c->sync();
+  });
+}
+
+int ipeps_ctx_mem(ipeps_ctx* ctx, int64_t* live, int64_t* peak) {
+  return guard([&] {
+    IPEPS_CHECK(ctx, "null ctx");
+    if (live) *live = (int64_t)ctx->c->pool->live_bytes();
+    if (peak) *peak = (int64_t)ctx->c->pool->peak_bytes();
+  });
+}
+
+int ipeps_env_create(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const void* A, ipeps_env** out) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && out && dims, "null argument");
+    for (int i = 0; i < 5; ++i) IPEPS_CHECK(dims[i] > 0, "A dims must be positive");
+    IPEPS_CHECK(dims[0] == dims[1] && dims[1] == dims[2] && dims[2] == dims[3], "A must be (D,D,D,D,d)");
+    Ctx& c = *ctx->c;
+    auto* e = new ipeps_env();
+    e->ctx = ctx;
+    e->dtype = dtype;
+    e->env.dt = dev_dt(dtype);
+    e->env.opt.mixed = k::Mixed::BF16X3;
+    try {
+      TapeScope ts(nullptr);
+      e->env.A = upload(c, host_dt(dtype), dev_dt(dtype), {dims[0], dims[1], dims[2], dims[3], dims[4]}, A);
+      e->env.Ad = op::conj(c, e->env.A);
+      ctm::init_ctm1(c, e->env);
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    *out = e;
+  });
+}
+
+int ipeps_env_destroy(ipeps_env* env) {
+  return guard([&] { delete env; });
+}
+
+int ipeps_env_set_option(ipeps_env* env, const char* key, double value) {
+  return guard([&] {
+    IPEPS_CHECK(env && key, "null argument");
+    set_opt(env->env.opt, key, value);
+  });
+}
+
+int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd) {
+  return guard([&] {
+    IPEPS_CHECK(env && dims && nd, "null argument");
+    Tensor& t = env_slot(env->env, which);
+    *nd = t.nd();
+    for (int i = 0; i < t.nd(); ++i) dims[i] = t.dims[i];
+  });
+}
+
+int ipeps_env_get_tensor(ipeps_env* env, int which, void* host) {
+  return guard([&] {
+    IPEPS_CHECK(env, "null env");
+    download(*env->ctx->c, env_slot(env->env, which), host_dt(env->dtype), host);
+  });
+}
+
+int ipeps_env_set_tensor(ipeps_env* env, int which, int nd, const int64_t* dims, const void* host) {
+  return guard([&] {
+    IPEPS_CHECK(env && dims && nd > 0 && nd <= 5, "bad arguments");
+    std::vector<int64_t> d(dims, dims + nd);
+    Ctx& c = *env->ctx->c;
+    Tensor t = upload(c, host_dt(env->dtype), dev_dt(env->dtype), d, host);
+    env_slot(env->env, which) = t;
+    if (which == 8) env->env.Ad = op::conj(c, t);
+  });
+}
+
+int ipeps_env_set_A(ipeps_env* env, const void* A) {
+  return guard([&] {
+    IPEPS_CHECK(env, "null env");
+    Ctx& c = *env->ctx->c;
+    Tensor t = upload(c, host_dt(env->dtype), dev_dt(env->dtype), env->env.A.dims, A);
+    env->env.A = t;
+    env->env.Ad = op::conj(c, t);
+  });
+}
+
+int ipeps_rg_step(ipeps_env* env, int64_t chi, double* s_out, int64_t s_cap, int64_t* s_len) {
+  return guard([&] {
+    IPEPS_CHECK(env && chi > 0, "bad arguments");
+    TapeScope ts(nullptr);
+    auto s = ctm::rg_step(*env->ctx->c, env->env, chi);
+    if (s_len) *s_len = (int64_t)s.size();
+    if (s_out) {
+      IPEPS_CHECK(s_cap >= (int64_t)s.size(), "rg_step: s_out too small");
+      std::memcpy(s_out, s.data(), s.size() * sizeof(double));
+    }
+  });
+}
+
+int ipeps_run_ctm(ipeps_env* env, int64_t chi, int min_iter, int max_iter, double tol, int conv_mode,
+                  const double* H1, const double* H2, int64_t hdim, int* iters, double* E_out, double* diffs_out,
+                  int64_t diffs_cap, int64_t* n_diffs) {
+  return guard([&] {
+    IPEPS_CHECK(env && chi > 0 && max_iter >= 1, "bad arguments");
+    Ctx& c = *env->ctx->c;
+    TapeScope ts(nullptr);
+    Tensor h1, h2;
+    if (conv_mode == 0) {
+      h1 = upload_H(c, H1, hdim);
+      h2 = upload_H(c, H2, hdim);
+    }
+    auto r = ctm::run_ctm(c, env->env, chi, min_iter, max_iter, tol, conv_mode, h1, h2);
+    if (iters) *iters = r.iters;
+    if (E_out) { E_out[0] = r.E[0]; E_out[1] = r.E[1]; }
+    if (n_diffs) *n_diffs = (int64_t)r.diffs.size();
+    if (diffs_out) {
+      int64_t n = std::min<int64_t>(diffs_cap, (int64_t)r.diffs.size());
+      std::memcpy(diffs_out, r.diffs.data(), n * sizeof(double));
+    }
+  });
+}
+
+int ipeps_gs_energy(ipeps_env* env, const double* H1, const double* H2, int64_t hdim, double* E_out, double* N_out) {
+  return guard([&] {
+    IPEPS_CHECK(env, "null env");
+    Ctx& c = *env->ctx->c;
+    TapeScope ts(nullptr);
+    Tensor out = ctm::energy(c, env->env, upload_H(c, H1, hdim), upload_H(c, H2, hdim));
+    double h[8];
+    to_host(c, out, h);
+    if (E_out) { E_out[0] = h[0]; E_out[1] = h[1]; }
+    if (N_out) { N_out[0] = h[2]; N_out[1] = h[3]; }
+  });
+}
+
+int ipeps_gs_norm(ipeps_env* env, double* out) {
+  return guard([&] {
+    IPEPS_CHECK(env && out, "null argument");
+    TapeScope ts(nullptr);
+    ctm::gs_norm(*env->ctx->c, env->env, out);
+  });
+}
+
+int ipeps_step_adjoint(ipeps_env* env, int64_t chi, const double* H1, const double* H2, int64_t hdim, double* E_out,
+                       void* gradA_host) {
+  return guard([&] {
+    IPEPS_CHECK(env && chi > 0, "bad arguments");
+    Ctx& c = *env->ctx->c;
+    double E[2];
+    Tensor g = ctm::step_and_adjoint(c, env->env, chi, upload_H(c, H1, hdim), upload_H(c, H2, hdim), E);
+    if (E_out) { E_out[0] = E[0]; E_out[1] = E[1]; }
+    if (gradA_host) download(c, g, host_dt(env->dtype), gradA_host);
+    else c.sync();
+  });
+}
+
+int ipeps_gs_energy_grad(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const void* A0, const double* H1,
+                         const double* H2, int64_t hdim, int64_t chi, int min_iter, int max_iter, int ad_max_iter,
+                         double tol, const char* options, double* E_out, void* grad_out, int* iters_out) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && dims, "null argument");
+    Ctx& c = *ctx->c;
+    ctm::Options o;
+    parse_opts(o, options);
+    Tensor A = upload(c, host_dt(dtype), host_dt(dtype), {dims[0], dims[1], dims[2], dims[3], dims[4]}, A0);
+    auto r = ctm::gs_energy_grad(c, A, dev_dt(dtype), o, upload_H(c, H1, hdim), upload_H(c, H2, hdim), chi,
+                                 min_iter, max_iter, ad_max_iter, tol);
+    if (E_out) { E_out[0] = r.E[0]; E_out[1] = r.E[1]; }
+    if (grad_out) download(c, r.grad, host_dt(dtype), grad_out);
+    if (iters_out) { iters_out[0] = r.conv_iters; iters_out[1] = r.ad_iters; }
+  });
+}
+
+int ipeps_tcon(ipeps_ctx* ctx, int dtype, int ndA, const int64_t* dimsA, const void* A, const int* la, int ndB,
+               const int64_t* dimsB, const void* B, const int* lb, int64_t* dimsC, int* ndC, void* C_host,
+               int64_t C_cap) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && dimsA && dimsB && la && lb && ndA >= 0 && ndB >= 0, "bad arguments");
+    IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_C128, "tcon: F64 or C128");
+    Ctx& c = *ctx->c;
+    DType dt = host_dt(dtype);
+    Tensor tA = upload(c, dt, dt, std::vector<int64_t>(dimsA, dimsA + ndA), A);
+    Tensor tB = upload(c, dt, dt, std::vector<int64_t>(dimsB, dimsB + ndB), B);
+    Tensor C = op::tcon_raw(c, tA, std::vector<int>(la, la + ndA), false, tB, std::vector<int>(lb, lb + ndB), false);
+    IPEPS_CHECK(C.nd() <= 16, "tcon: output rank > 16");
+    if (ndC) *ndC = C.nd();
+    if (dimsC) for (int i = 0; i < C.nd(); ++i) dimsC[i] = C.dims[i];
+    if (C_host) {
+      IPEPS_CHECK(C_cap >= C.numel(), "tcon: output buffer too small");
+      to_host(c, C, C_host);
+    }
+  });
+}
+
+int ipeps_svd(ipeps_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A, void* U, double* S, void* V) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && m > 0 && n > 0, "bad arguments");
+    IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_C128, "svd: F64 or C128");
+    Ctx& c = *ctx->c;
+    DType dt = host_dt(dtype);
+    Tensor tA = upload(c, dt, dt, {m, n}, A);
+    Tensor tU, tS, tV;
+    op::svd_full_raw(c, tA, tU, tS, tV);
+    download(c, tU, dt, U);
+    to_host(c, tS, S);
+    download(c, tV, dt, V);
+  });
+}
+
+int ipeps_svd_back(ipeps_ctx* ctx, int dtype, int64_t N, int64_t n, const void* U, const double* S, const void* V,
+                   const void* dU, const double* dS, const void* dV, double eta, void* dA_out) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && N > 0 && n > 0 && n <= N && U && S && V && dA_out, "bad arguments");
+    IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_C128, "svd_back: F64 or C128");
+    Ctx& c = *ctx->c;
+    DType dt = host_dt(dtype);
+    Tensor tU = upload(c, dt, dt, {N, N}, U), tV = upload(c, dt, dt, {N, N}, V);
+    Tensor tS = upload(c, DType::F64, DType::F64, {N}, S);
+    Tensor UdU, VdV, tdS;
+    if (dU) UdU = op::matmul_raw(c, tU, true, true, upload(c, dt, dt, {N, n}, dU), false, false);
+    if (dV) VdV = op::matmul_raw(c, tV, true, true, upload(c, dt, dt, {N, n}, dV), false, false);
+    if (dS) tdS = upload(c, DType::F64, DType::F64, {n}, dS);
+    Tensor G = c.empty(dt, {N, N});
+    k::svd_gamma(c, dt, N, n, tS.ptr<double>(), UdU.defined() ? UdU.data() : nullptr,
+                 VdV.defined() ? VdV.data() : nullptr, tdS.defined() ? tdS.ptr<double>() : nullptr, eta, G.data());
+    Tensor R = op::matmul_raw(c, op::matmul_raw(c, tU, false, false, G, false, false), false, false, tV, true, true);
+    download(c, R, dt, dA_out);
+  });
+}
+
+}  // extern "C"

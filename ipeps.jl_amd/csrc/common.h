@@ -1,0 +1,154 @@
+// Core runtime types of the CTMRG engine: dtypes, device buffers, tensors,
+// the per-context stream / allocator, and error reporting.
+//
+// Tensors are dense and column-major (first index fastest), the layout of
+// Julia arrays, so a Julia caller can hand its arrays over unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace ipeps {
+
+enum class DType : int { F64 = 0, C128 = 1, F32 = 2 };
+
+inline size_t dtype_size(DType t) {
+  switch (t) {
+    case DType::F64: return 8;
+    case DType::C128: return 16;
+    case DType::F32: return 4;
+  }
+  return 0;
+}
+inline bool is_complex(DType t) { return t == DType::C128; }
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Status codes returned through the C ABI.
+enum Status : int {
+  IPEPS_OK = 0,
+  IPEPS_ERR_ARG = 1,
+  IPEPS_ERR_HIP = 2,
+  IPEPS_ERR_SOLVER = 3,
+  IPEPS_ERR_STATE = 4,
+  IPEPS_ERR_INTERNAL = 5,
+};
+
+#define IPEPS_CHECK(cond, msg)                                                     \
+  do {                                                                             \
+    if (!(cond)) throw ::ipeps::Error(::ipeps::IPEPS_ERR_ARG, std::string(msg));   \
+  } while (0)
+
+#define HIP_CHECK(expr)                                                            \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      throw ::ipeps::Error(::ipeps::IPEPS_ERR_HIP,                                 \
+                           std::string(#expr) + ": " + hipGetErrorString(_e));     \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Caching device allocator.  hipMalloc synchronises the device, so freed
+// blocks are kept per exact size class and reused.
+// ---------------------------------------------------------------------------
+class Pool {
+ public:
+  explicit Pool(int device) : device_(device) {}
+  ~Pool();
+  void* alloc(size_t bytes);
+  void release(void* p, size_t bytes);
+  void trim();
+  size_t live_bytes() const { return live_; }
+  size_t peak_bytes() const { return peak_; }
+
+ private:
+  static size_t round(size_t b) {
+    if (b < 256) return 256;
+    // 1/8-octave size classes bound the waste at 12.5%.
+    size_t p = 1;
+    while (p < b) p <<= 1;
+    size_t step = p >> 3;
+    if (step < 256) step = 256;
+    return (b + step - 1) / step * step;
+  }
+  int device_;
+  std::multimap<size_t, void*> free_;
+  size_t live_ = 0, peak_ = 0, cached_ = 0;
+};
+
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  Pool* pool = nullptr;
+  ~Buf() {
+    if (p && pool) pool->release(p, bytes);
+  }
+};
+
+struct Node;  // reverse-mode record, see ad.h
+
+struct Tensor {
+  std::shared_ptr<Buf> buf;
+  size_t offset = 0;  // in bytes
+  DType dt = DType::F64;
+  std::vector<int64_t> dims;
+  std::shared_ptr<Node> node;  // set when the tensor is on an active tape
+
+  int nd() const { return (int)dims.size(); }
+  int64_t numel() const {
+    int64_t n = 1;
+    for (auto d : dims) n *= d;
+    return n;
+  }
+  size_t bytes() const { return (size_t)numel() * dtype_size(dt); }
+  void* data() const { return buf ? (char*)buf->p + offset : nullptr; }
+  template <typename T>
+  T* ptr() const { return reinterpret_cast<T*>(data()); }
+  int64_t dim(int i) const { return dims.at(i); }
+  bool defined() const { return (bool)buf; }
+};
+
+struct Ctx;
+
+// A GPU context: device, one stream, allocator, solver handle and the
+// last-error text of its thread.
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Pool> pool;
+  void* solver = nullptr;  // rocblas_handle
+  uint64_t rng_counter = 0x9e3779b97f4a7c15ull;
+  // kernel-time instrumentation (hipEvents around the named phases)
+  bool timing = false;
+  std::map<std::string, double> phase_ms;
+  std::map<std::string, long> phase_calls;
+
+  Tensor empty(DType dt, const std::vector<int64_t>& dims);
+  Tensor zeros(DType dt, const std::vector<int64_t>& dims);
+  void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+Ctx* ctx_create(int device);
+void ctx_destroy(Ctx* c);
+
+// Copy helpers (host <-> device, on the context stream, synchronous).
+void to_device(Ctx& c, Tensor& t, const void* host);
+void to_host(Ctx& c, const Tensor& t, void* host);
+
+// Tensor with the same buffer and new dims (column-major reshape).
+Tensor reshape(const Tensor& t, const std::vector<int64_t>& dims);
+
+std::string dims_str(const std::vector<int64_t>& d);
+
+}  // namespace ipeps

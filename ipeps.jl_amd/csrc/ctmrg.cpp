@@ -1,0 +1,571 @@
+// CTMRG on the device: the reference's contraction networks written with the
+// engine's differentiable tcon (label lists follow src/ctm/ctmrg.jl and
+// src/graph/evaluation.jl), plus the step-checkpointed gradient driver.
+#include "ctmrg.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace ipeps {
+namespace ctm {
+
+using op::tcon;
+using L = std::vector<int>;
+
+namespace {
+
+Tensor mat(Ctx& c, const Tensor& x, int rows_nd) {
+  int64_t r = 1, q = 1;
+  for (int i = 0; i < x.nd(); ++i) (i < rows_nd ? r : q) *= x.dims[i];
+  return op::reshape(c, x, {r, q});
+}
+
+Tensor tr2(Ctx& c, const Tensor& x) { return op::permute(c, x, {1, 0}); }
+
+struct Proj {
+  Tensor P1, P2;
+  std::vector<double> s;
+};
+
+// get_projector (src/ctm/ctmrg.jl:254-265): M = R1 R2, truncated SVD,
+// P1 = R2 V S^-1/2, P2 = S^-1/2 U^H R1.
+Proj get_projector(Ctx& c, const Env& e, const Tensor& R1, const Tensor& R2, int64_t chi) {
+  op::SvdOpts so = e.opt.svd;
+  auto r = op::svd_trunc(c, R1, R2, chi, so);
+  Tensor isq = op::inv_sqrt(c, r.S);
+  Proj p;
+  p.P1 = op::scale_vec(c, op::matmul(c, R2, r.V), isq, false);
+  p.P2 = op::scale_vec(c, op::adjoint_matmul(c, r.U, R1), isq, true);
+  p.s.assign(r.s_host.begin(), r.s_host.begin() + r.n);
+  return p;
+}
+
+// Edge renormalisation.  ia / iad are the reference label lists of A and Ad
+// in E*A and (E*A)*Ad (src/ctm/ctmrg.jl:285-289 for the left move).
+Tensor edge_update(Ctx& c, const Env& e, const Tensor& E, const L& ia, const L& iad, const Tensor& P, const Tensor& Pd) {
+  const Tensor& A = e.A;
+  const Tensor& Ad = e.Ad;
+  if (!e.opt.canonical_edge) {
+    // reference order: chi^2 D^6 intermediate, then Pd and P
+    Tensor EA = tcon(c, E, {-1, -3, 1, -6}, A, ia);
+    Tensor EAAd = tcon(c, EA, {-1, -2, -4, -5, -7, 1, 2}, Ad, iad);
+    auto s = EAAd.dims;
+    EAAd = op::reshape(c, EAAd, {s[0] * s[1] * s[2], s[3] * s[4] * s[5], s[6], s[7]});
+    Tensor EPd = tcon(c, Pd, {-1, 1}, EAAd, {1, -2, -3, -4});
+    return tcon(c, EPd, {-1, 1, -3, -4}, P, {1, -2});
+  }
+  // canonical order: Pd first (chi^3 D^4 + 2 chi^2 D^6 d + chi^3 D^4)
+  auto pos = [](const L& l, int lab) {
+    for (size_t i = 0; i < l.size(); ++i)
+      if (l[i] == lab) return (int)i;
+    throw Error(IPEPS_ERR_INTERNAL, "edge_update: label missing");
+  };
+  const int a_p2 = pos(ia, -2), a_E = pos(ia, 1), a_p1 = pos(ia, -4), a_open = pos(ia, -5), a_ph = pos(ia, -7);
+  const int d_p2 = pos(iad, -3), d_E = pos(iad, 1), d_p1 = pos(iad, -6), d_open = pos(iad, -8), d_ph = pos(iad, 2);
+  const int64_t chi_e = E.dims[0], D = A.dims[0];
+  const int64_t chip = Pd.dims[0], chin = P.dims[1];
+  Tensor P2r = op::reshape(c, Pd, {chip, chi_e, D, D});
+  Tensor X1 = tcon(c, P2r, {-1, 1, -2, -3}, E, {1, -4, -5, -6});  // (i, ap2, adp2, E2, Eket, Ebra)
+  L la(5);
+  la[a_p2] = 10; la[a_E] = 11; la[a_p1] = -5; la[a_open] = -6; la[a_ph] = -7;
+  Tensor X2 = tcon(c, X1, {-1, 10, -2, -3, 11, -4}, A, la);       // (i, adp2, E2, Ebra, Ap1, Aopen, ph)
+  L ld(5);
+  ld[d_p2] = 20; ld[d_E] = 21; ld[d_ph] = 22; ld[d_p1] = -5; ld[d_open] = -6;
+  Tensor X3 = tcon(c, X2, {-1, 20, -2, 21, -3, -4, 22}, Ad, ld);  // (i, E2, Ap1, Aopen, Adp1, Adopen)
+  Tensor P1r = op::reshape(c, P, {E.dims[1], D, D, chin});
+  return tcon(c, X3, {-1, 30, 31, -3, 32, -4}, P1r, {30, 31, 32, -2});
+}
+
+void finish(Ctx& c, Env& e, int ci, const Tensor& C_, int cj, const Tensor& Cb, int ek, const Tensor& E_) {
+  e.C[ci] = op::renormalize(c, C_);
+  e.E[ek] = op::renormalize(c, E_);
+  e.C[cj] = op::renormalize(c, Cb);
+}
+
+std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi) {
+  const Tensor &C1 = e.C[0], &C4 = e.C[3], &E1 = e.E[0], &E3 = e.E[2], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
+  // get_projector_left (src/ctm/ctmrg.jl:137-160)
+  Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});
+  Tensor CEE4 = tcon(c, C1E1, {1, -2, -5, -3}, E4, {1, -1, -6, -4});
+  Tensor CEEA = tcon(c, CEE4, {-1, -2, -3, -4, 1, 2}, A, {1, 2, -5, -6, -7});
+  Tensor LU = tcon(c, CEEA, {-1, -4, 1, 2, -2, -5, 3}, Ad, {1, 2, -3, -6, 3});
+  Tensor R1 = tr2(c, mat(c, LU, 3));
+  Tensor C4E3 = tcon(c, C4, {-1, 1}, E3, {1, -2, -3, -4});
+  Tensor CEE4b = tcon(c, C4E3, {1, -2, -6, -4}, E4, {-1, 1, -5, -3});
+  Tensor CEEAb = tcon(c, CEE4b, {-1, -2, -3, -4, 1, 2}, A, {-5, 1, 2, -6, -7});
+  Tensor LB = tcon(c, CEEAb, {-1, -4, 1, 2, -2, -5, 3}, Ad, {-3, 1, 2, -6, 3});
+  Tensor R2 = mat(c, LB, 3);
+  Proj p = get_projector(c, e, R1, R2, chi);
+  // proj_left (src/ctm/ctmrg.jl:267-291)
+  Tensor C1E1p = tcon(c, C1, {-2, 1}, E1, {1, -1, -3, -4});
+  Tensor nC1 = tr2(c, op::matmul(c, op::reshape(c, C1E1p, {C1E1p.dims[0], C1E1p.numel() / C1E1p.dims[0]}), p.P1));
+  Tensor C4E3p = tcon(c, C4, {-1, 1}, E3, {1, -4, -2, -3});
+  int64_t lastd = C4E3p.dims[3];
+  Tensor nC4 = op::matmul(c, p.P2, op::reshape(c, C4E3p, {C4E3p.numel() / lastd, lastd}));
+  Tensor nE4 = edge_update(c, e, E4, {-2, 1, -4, -5, -7}, {-3, 1, -6, -8, 2}, p.P1, p.P2);
+  finish(c, e, 0, nC1, 3, nC4, 3, nE4);
+  return p.s;
+}
+
+void right_rg(Ctx& c, Env& e, int64_t chi) {
+  const Tensor &C2 = e.C[1], &C3 = e.C[2], &E1 = e.E[0], &E2 = e.E[1], &E3 = e.E[2], &A = e.A, &Ad = e.Ad;
+  // get_projector_right (src/ctm/ctmrg.jl:176-198)
+  Tensor C2E1 = tcon(c, C2, {1, -2}, E1, {-1, 1, -3, -4});
+  Tensor CEE2 = tcon(c, C2E1, {-1, 1, -5, -3}, E2, {1, -2, -6, -4});
+  Tensor CEEA = tcon(c, CEE2, {-1, -2, -3, -4, 1, 2}, A, {1, -5, -6, 2, -7});
+  Tensor RU = tcon(c, CEEA, {-1, -4, 1, 2, -2, -5, 3}, Ad, {1, -3, -6, 2, 3});
+  Tensor R1 = mat(c, RU, 3);
+  Tensor C3E3 = tcon(c, C3, {-1, 1}, E3, {-2, 1, -3, -4});
+  Tensor CEE2b = tcon(c, C3E3, {1, -2, -5, -3}, E2, {-1, 1, -6, -4});
+  Tensor CEEAb = tcon(c, CEE2b, {-1, -2, -5, -6, 1, 2}, A, {-3, -4, 1, 2, -7});
+  Tensor RB = tcon(c, CEEAb, {-1, -4, -2, -5, 1, 2, 3}, Ad, {-3, -6, 1, 2, 3});
+  Tensor R2 = mat(c, RB, 3);
+  Proj p = get_projector(c, e, R1, R2, chi);
+  // proj_right (src/ctm/ctmrg.jl:293-318)
+  Tensor C2E1p = tcon(c, C2, {1, -2}, E1, {-1, 1, -3, -4});
+  Tensor nC2 = op::matmul(c, op::reshape(c, C2E1p, {C2E1p.dims[0], C2E1p.numel() / C2E1p.dims[0]}), p.P1);
+  Tensor C3E3p = tcon(c, C3, {-1, 1}, E3, {-4, 1, -2, -3});
+  int64_t lastd = C3E3p.dims[3];
+  Tensor nC3 = op::matmul(c, p.P2, op::reshape(c, C3E3p, {C3E3p.numel() / lastd, lastd}));
+  Tensor nE2 = edge_update(c, e, E2, {-2, -5, -4, 1, -7}, {-3, -8, -6, 1, 2}, p.P1, p.P2);
+  finish(c, e, 1, nC2, 2, nC3, 1, nE2);
+}
+
+void top_rg(Ctx& c, Env& e, int64_t chi) {
+  const Tensor &C1 = e.C[0], &C2 = e.C[1], &E1 = e.E[0], &E2 = e.E[1], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
+  // get_projector_top (src/ctm/ctmrg.jl:214-236)
+  Tensor C1E4 = tcon(c, C1, {1, -2}, E4, {1, -1, -3, -4});
+  Tensor CEE1 = tcon(c, C1E4, {-1, 1, -6, -4}, E1, {1, -2, -5, -3});
+  Tensor CEEA = tcon(c, CEE1, {-1, -2, -5, -6, 1, 2}, A, {1, 2, -3, -4, -7});
+  Tensor UL = tcon(c, CEEA, {-1, -4, -2, -5, 1, 2, 3}, Ad, {1, 2, -3, -6, 3});
+  Tensor R1 = mat(c, UL, 3);
+  Tensor C2E2 = tcon(c, C2, {-1, 1}, E2, {1, -2, -3, -4});
+  Tensor CEE1b = tcon(c, C2E2, {1, -2, -6, -4}, E1, {-1, 1, -5, -3});
+  Tensor CEEAb = tcon(c, CEE1b, {-1, -2, -5, -6, 1, 2}, A, {1, -3, -4, 2, -7});
+  Tensor UR = tcon(c, CEEAb, {-1, -4, -2, -5, 1, 2, 3}, Ad, {1, -3, -6, 2, 3});
+  Tensor R2 = mat(c, UR, 3);
+  Proj p = get_projector(c, e, R1, R2, chi);
+  // proj_top (src/ctm/ctmrg.jl:320-345)
+  Tensor C1E4p = tcon(c, C1, {1, -2}, E4, {1, -1, -3, -4});
+  Tensor nC1 = op::matmul(c, op::reshape(c, C1E4p, {C1E4p.dims[0], C1E4p.numel() / C1E4p.dims[0]}), p.P1);
+  Tensor C2E2p = tcon(c, C2, {-1, 1}, E2, {1, -4, -2, -3});
+  int64_t lastd = C2E2p.dims[3];
+  Tensor nC2 = op::matmul(c, p.P2, op::reshape(c, C2E2p, {C2E2p.numel() / lastd, lastd}));
+  Tensor nE1 = edge_update(c, e, E1, {1, -2, -5, -4, -7}, {1, -3, -8, -6, 2}, p.P1, p.P2);
+  finish(c, e, 0, nC1, 1, nC2, 0, nE1);
+}
+
+void bottom_rg(Ctx& c, Env& e, int64_t chi) {
+  const Tensor &C3 = e.C[2], &C4 = e.C[3], &E2 = e.E[1], &E3 = e.E[2], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
+  // get_projector_bottom (src/ctm/ctmrg.jl:238-252... via 226-250)
+  Tensor C4E4 = tcon(c, C4, {1, -2}, E4, {-1, 1, -3, -4});
+  Tensor CEE3 = tcon(c, C4E4, {-1, 1, -5, -3}, E3, {1, -2, -6, -4});
+  Tensor CEEA = tcon(c, CEE3, {-1, -2, -5, -6, 1, 2}, A, {-3, 1, 2, -4, -7});
+  Tensor BL = tcon(c, CEEA, {-1, -4, -2, -5, 1, 2, 3}, Ad, {-3, 1, 2, -6, 3});
+  Tensor R1 = mat(c, BL, 3);
+  Tensor C3E2 = tcon(c, C3, {1, -2}, E2, {-1, 1, -3, -4});
+  Tensor CEE3b = tcon(c, C3E2, {-1, 1, -6, -4}, E3, {-2, 1, -5, -3});
+  Tensor CEEAb = tcon(c, CEE3b, {-1, -2, -5, -6, 1, 2}, A, {-3, -4, 1, 2, -7});
+  Tensor BR = tcon(c, CEEAb, {-1, -4, -2, -5, 1, 2, 3}, Ad, {-3, -6, 1, 2, 3});
+  Tensor R2 = tr2(c, mat(c, BR, 3));
+  Proj p = get_projector(c, e, R1, R2, chi);
+  // proj_bottom (src/ctm/ctmrg.jl:347-375)
+  Tensor C4E4p = tcon(c, C4, {1, -2}, E4, {-1, 1, -3, -4});
+  Tensor nC4 = op::matmul(c, op::reshape(c, C4E4p, {C4E4p.dims[0], C4E4p.numel() / C4E4p.dims[0]}), p.P1);
+  Tensor C3E2p = tcon(c, C3, {1, -1}, E2, {-4, 1, -2, -3});
+  int64_t lastd = C3E2p.dims[3];
+  Tensor nC3 = tr2(c, op::matmul(c, p.P2, op::reshape(c, C3E2p, {C3E2p.numel() / lastd, lastd})));
+  Tensor nE3 = edge_update(c, e, E3, {-5, -2, 1, -4, -7}, {-8, -3, 1, -6, 2}, p.P1, p.P2);
+  finish(c, e, 3, nC4, 2, nC3, 2, nE3);
+}
+
+// ---- density matrices (src/graph/evaluation.jl:169-248) ---------------------
+Tensor dm_h(Ctx& c, const Tensor& C1, const Tensor& C2, const Tensor& C3, const Tensor& C4, const Tensor& E1l,
+            const Tensor& E1r, const Tensor& E2, const Tensor& E3l, const Tensor& E3r, const Tensor& E4,
+            const Tensor& Al, const Tensor& Ar, const Tensor& Adl, const Tensor& Adr) {
+  Tensor C1E1 = tcon(c, C1, {-1, 1}, E1l, {1, -2, -3, -4});
+  Tensor LU = tcon(c, C1E1, {-1, -2, -3, 1}, Adl, {1, -4, -5, -6, -7});
+  Tensor C4E3 = tcon(c, C4, {-1, 1}, E3l, {1, -2, -3, -4});
+  Tensor CEE4 = tcon(c, E4, {-1, 1, -3, -5}, C4E3, {1, -2, -4, -6});
+  Tensor LB = tcon(c, CEE4, {-1, -2, 1, 2, -4, -5}, Al, {-3, 1, 2, -6, -7});
+  Tensor Lt = tcon(c, LB, {1, -2, 2, 3, 4, -3, -5}, LU, {1, -1, 2, 3, 4, -4, -6});
+  Tensor C2E1 = tcon(c, C2, {1, -2}, E1r, {-1, 1, -3, -4});
+  Tensor RU = tcon(c, C2E1, {-1, -2, 1, -3}, Ar, {1, -4, -5, -6, -7});
+  Tensor C3E3 = tcon(c, C3, {-1, 1}, E3r, {-2, 1, -3, -4});
+  Tensor CEE2 = tcon(c, C3E3, {1, -2, -3, -5}, E2, {-1, 1, -4, -6});
+  Tensor RB = tcon(c, CEE2, {-1, -2, -3, -4, 1, 2}, Adr, {-5, -6, 1, 2, -7});
+  Tensor Rt = tcon(c, RU, {-1, 1, 2, -3, 3, 4, -5}, RB, {1, -2, 3, 4, 2, -4, -6});
+  Tensor roh = tcon(c, Lt, {1, 2, 3, 4, -1, -3}, Rt, {1, 2, 3, 4, -2, -4});
+  int64_t d = Al.dims[4];
+  return op::reshape(c, roh, {d * d, d * d});
+}
+
+Tensor dm_v(Ctx& c, const Tensor& C1, const Tensor& C2, const Tensor& C3, const Tensor& C4, const Tensor& E1,
+            const Tensor& E2u, const Tensor& E2d, const Tensor& E3, const Tensor& E4u, const Tensor& E4d,
+            const Tensor& Au, const Tensor& Ad_, const Tensor& Adu, const Tensor& Add) {
+  Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});
+  Tensor CEE4 = tcon(c, C1E1, {1, -2, -3, -5}, E4u, {1, -1, -4, -6});
+  Tensor UL = tcon(c, CEE4, {-1, -2, 1, 2, -3, -4}, Au, {1, 2, -5, -6, -7});
+  Tensor C2E2 = tcon(c, C2, {-1, 1}, E2u, {1, -2, -3, -4});
+  Tensor UR = tcon(c, C2E2, {-1, -2, -3, 1}, Adu, {-4, -5, -6, 1, -7});
+  Tensor U = tcon(c, UL, {-1, 1, 3, 4, -3, 2, -5}, UR, {1, -2, 2, 3, 4, -4, -6});
+  Tensor C4E3 = tcon(c, C4, {-1, 1}, E3, {1, -2, -3, -4});
+  Tensor CEE4b = tcon(c, C4E3, {1, -2, -4, -6}, E4d, {-1, 1, -3, -5});
+  Tensor BL = tcon(c, CEE4b, {-1, -2, -3, -4, 1, 2}, Add, {-5, 1, 2, -6, -7});
+  Tensor C3E2 = tcon(c, C3, {1, -2}, E2d, {-1, 1, -3, -4});
+  Tensor BR = tcon(c, C3E2, {-1, -2, 1, -3}, Ad_, {-4, -5, -6, 1, -7});
+  Tensor B = tcon(c, BL, {-1, 1, 2, 3, -4, 4, -6}, BR, {-2, 1, 4, -3, 2, 3, -5});
+  Tensor rov = tcon(c, U, {1, 2, 3, 4, -1, -3}, B, {1, 2, 3, 4, -2, -4});
+  int64_t d = Au.dims[4];
+  return op::reshape(c, rov, {d * d, d * d});
+}
+
+}  // namespace
+
+void init_ctm1(Ctx& c, Env& e) {
+  const Tensor &A = e.A, &Ad = e.Ad;
+  const int64_t D = A.dims[0];
+  auto C = [&](const L& la, const L& lb) {
+    return op::renormalize(c, op::reshape(c, tcon(c, A, la, Ad, lb), {D * D, D * D}));
+  };
+  auto E = [&](const L& la, const L& lb) {
+    return op::renormalize(c, op::reshape(c, tcon(c, A, la, Ad, lb), {D * D, D * D, D, D}));
+  };
+  // src/ctm/ctm_tensor.jl:97-130
+  e.C[0] = C({1, 2, -1, -3, 3}, {1, 2, -2, -4, 3});
+  e.C[1] = C({1, -1, -3, 2, 3}, {1, -2, -4, 2, 3});
+  e.C[2] = C({-1, -3, 1, 2, 3}, {-2, -4, 1, 2, 3});
+  e.C[3] = C({-1, 1, 2, -3, 3}, {-2, 1, 2, -4, 3});
+  e.E[0] = E({1, -1, -5, -3, 2}, {1, -2, -6, -4, 2});
+  e.E[1] = E({-1, -5, -3, 1, 2}, {-2, -6, -4, 1, 2});
+  e.E[2] = E({-5, -1, 1, -3, 2}, {-6, -2, 1, -4, 2});
+  e.E[3] = E({-1, 1, -3, -5, 2}, {-2, 1, -4, -6, 2});
+}
+
+std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi) {
+  op::prec().mixed = e.opt.mixed;
+  auto s = left_rg(c, e, chi);
+  right_rg(c, e, chi);
+  top_rg(c, e, chi);
+  bottom_rg(c, e, chi);
+  return s;
+}
+
+void get_dms(Ctx& c, const Env& e, Tensor& roh, Tensor& rov) {
+  const auto& C = e.C;
+  const auto& E = e.E;
+  roh = dm_h(c, C[0], C[1], C[2], C[3], E[0], E[0], E[1], E[2], E[2], E[3], e.A, e.A, e.Ad, e.Ad);
+  rov = dm_v(c, C[0], C[1], C[2], C[3], E[0], E[1], E[1], E[2], E[3], E[3], e.A, e.A, e.Ad, e.Ad);
+}
+
+static Env as_dtype(Ctx& c, const Env& e, DType dt) {
+  if (e.dt == dt) return e;
+  Env r = e;
+  r.dt = dt;
+  r.A = op::cast(c, e.A, dt);
+  r.Ad = op::cast(c, e.Ad, dt);
+  for (int i = 0; i < 4; ++i) {
+    r.C[i] = op::cast(c, e.C[i], dt);
+    r.E[i] = op::cast(c, e.E[i], dt);
+  }
+  return r;
+}
+
+Tensor energy(Ctx& c, const Env& e0, const Tensor& H1, const Tensor& H2) {
+  Env e = (e0.dt == DType::F32 && e0.opt.energy_fp64) ? as_dtype(c, e0, DType::F64) : e0;
+  op::prec().mixed = e.opt.mixed;
+  Tensor roh, rov;
+  get_dms(c, e, roh, rov);
+  if (roh.dt == DType::F32) {
+    roh = op::cast(c, roh, DType::F64);
+    rov = op::cast(c, rov, DType::F64);
+  }
+  const int64_t m = roh.dims[0];
+  IPEPS_CHECK(H1.dims.size() == 2 && H1.dims[0] == m && H1.dims[1] == m, "energy: H dimension must be d^2");
+  Tensor out = c.empty(DType::F64, {8});
+  k::energy2(c, roh.dt, m, roh.data(), rov.data(), H1.ptr<double>(), H2.ptr<double>(), out.ptr<double>());
+  if (op::recording() && (roh.node || rov.node)) {
+    auto nh = roh.node, nv = rov.node;
+    Tensor rh = roh, rv = rov;
+    rh.node.reset();
+    rv.node.reset();
+    Tensor h1 = H1, h2 = H2;
+    out.node = op::make_node([=](Ctx& c, Node&) {
+      Tensor gh = c.empty(rh.dt, rh.dims), gv = c.empty(rv.dt, rv.dims);
+      k::energy2_back(c, rh.dt, m, rh.data(), rv.data(), h1.ptr<double>(), h2.ptr<double>(), gh.data(), gv.data());
+      if (nh) op::acc(c, nh, gh);
+      if (nv) op::acc(c, nv, gv);
+    });
+  }
+  return out;
+}
+
+void gs_norm(Ctx& c, const Env& e, double out[2]) {
+  const auto& C = e.C;
+  const auto& E = e.E;
+  Tensor C1E1 = tcon(c, C[0], {-1, 1}, E[0], {1, -2, -3, -4});
+  Tensor CEC2 = tcon(c, C1E1, {-1, 1, -3, -4}, C[1], {1, -2});
+  Tensor CECE4 = tcon(c, CEC2, {1, -2, -3, -5}, E[3], {1, -1, -4, -6});
+  Tensor C4E3 = tcon(c, C[3], {-1, 1}, E[2], {1, -2, -3, -4});
+  Tensor CEC3 = tcon(c, C4E3, {-1, 1, -3, -4}, C[2], {-2, 1});
+  Tensor CECE2 = tcon(c, CEC3, {-1, 1, -3, -5}, E[1], {-2, 1, -4, -6});
+  Tensor ndm = tcon(c, CECE4, {1, 2, -1, -2, -5, -6}, CECE2, {1, 2, -3, -4, -7, -8});
+  Tensor ndmAd = tcon(c, ndm, {-1, -2, -3, -4, 1, 2, 3, 4}, e.Ad, {1, 2, 3, 4, -5});
+  Tensor nrm = tcon(c, ndmAd, {1, 2, 3, 4, 5}, e.A, {1, 2, 3, 4, 5});
+  Tensor n64 = op::to_dtype(c, nrm, nrm.dt == DType::F32 ? DType::F64 : nrm.dt);
+  if (n64.dt == DType::C128) {
+    to_host(c, n64, out);
+  } else {
+    to_host(c, n64, out);
+    out[1] = 0.0;
+  }
+}
+
+static void read_energy(Ctx& c, const Tensor& t, double E[2]) {
+  double h[8];
+  to_host(c, t, h);
+  E[0] = h[0];
+  E[1] = h[1];
+}
+
+RunResult run_ctm(Ctx& c, Env& e, int64_t chi, int min_iter, int max_iter, double tol, int conv_mode,
+                  const Tensor& H1, const Tensor& H2, bool) {
+  // src/ctm/ctmrg.jl:1-55: diffs starts as [1.0], conv as the scalar 1.0
+  RunResult r;
+  r.diffs.push_back(1.0);
+  std::vector<double> conv = {1.0, 0.0};  // complex scalar or spectrum
+  bool conv_is_scalar = true;
+  for (int i = 1; i <= max_iter; ++i) {
+    r.iters = i;
+    std::vector<double> s = rg_step(c, e, chi);
+    r.s_last = s;
+    std::vector<double> old = conv;
+    bool old_scalar = conv_is_scalar;
+    // conv only matters when it can decide the stop (i >= min_iter) or feeds
+    // the next diff (i == min_iter - 1); skipping the rest changes nothing.
+    const bool need = (conv_mode != 0) || (i >= min_iter - 1 && i < max_iter) || i == max_iter;
+    if (conv_mode == 0) {
+      if (need) {
+        TapeScope ts(nullptr);
+        Tensor Et = energy(c, e, H1, H2);
+        double E[2];
+        read_energy(c, Et, E);
+        conv = {E[0], E[1]};
+        conv_is_scalar = true;
+        r.E[0] = E[0];
+        r.E[1] = E[1];
+      } else {
+        conv = {NAN, NAN};
+        conv_is_scalar = true;
+      }
+    } else {
+      conv.clear();
+      for (double v : s) { conv.push_back(v); conv.push_back(0.0); }
+      conv_is_scalar = s.size() == 1;
+    }
+    // lengths equal -> diff = norm(conv - old)
+    size_t len_new = conv.size() / 2, len_old = old.size() / 2;
+    (void)old_scalar;
+    if (len_new == len_old) {
+      double acc = 0.0;
+      for (size_t q = 0; q < conv.size(); ++q) acc += (conv[q] - old[q]) * (conv[q] - old[q]);
+      r.diffs.push_back(std::sqrt(acc));
+    }
+    if (i >= min_iter && r.diffs.back() < tol) break;
+  }
+  return r;
+}
+
+namespace {
+
+struct Checkpoint {
+  std::array<Tensor, 4> C, E;
+};
+
+Checkpoint save(const Env& e) {
+  Checkpoint k;
+  for (int i = 0; i < 4; ++i) {
+    k.C[i] = e.C[i];
+    k.E[i] = e.E[i];
+    k.C[i].node.reset();
+    k.E[i].node.reset();
+  }
+  return k;
+}
+
+void load(Env& e, const Checkpoint& k) {
+  for (int i = 0; i < 4; ++i) {
+    e.C[i] = k.C[i];
+    e.E[i] = k.E[i];
+  }
+}
+
+}  // namespace
+
+// Reverse sweep over checkpointed steps: cotC/cotE hold the cotangents of the
+// environment after the last step; A's cotangent accumulates into gA.
+static void backprop_steps(Ctx& c, Env& e, const std::vector<Checkpoint>& ck, int64_t chi, std::array<Tensor, 4> cotC,
+                           std::array<Tensor, 4> cotE, Tensor& gA) {
+  for (int t = (int)ck.size() - 1; t >= 0; --t) {
+    Tape tape;
+    tape.active = true;
+    TapeScope ts(&tape);
+    load(e, ck[t]);
+    Tensor A = op::leaf(e.A), Ad;
+    e.A = A;
+    e.Ad = op::conj(c, A);
+    for (int i = 0; i < 4; ++i) {
+      e.C[i] = op::leaf(e.C[i]);
+      e.E[i] = op::leaf(e.E[i]);
+    }
+    std::array<std::shared_ptr<Node>, 4> inC, inE;
+    for (int i = 0; i < 4; ++i) {
+      inC[i] = e.C[i].node;
+      inE[i] = e.E[i].node;
+    }
+    rg_step(c, e, chi);
+    for (int i = 0; i < 4; ++i) {
+      if (cotC[i].defined()) op::acc(c, e.C[i].node, cotC[i]);
+      if (cotE[i].defined()) op::acc(c, e.E[i].node, cotE[i]);
+    }
+    tape.active = false;
+    op::backward(c, tape);
+    if (A.node->grad.defined()) {
+      if (!gA.defined()) gA = op::clone(c, A.node->grad);
+      else k::axpy(c, gA.dt, gA.numel(), 1.0, A.node->grad.data(), gA.data());
+    }
+    for (int i = 0; i < 4; ++i) {
+      cotC[i] = inC[i]->grad;
+      cotE[i] = inE[i]->grad;
+    }
+    e.A.node.reset();
+    e.Ad.node.reset();
+    for (int i = 0; i < 4; ++i) {
+      e.C[i].node.reset();
+      e.E[i].node.reset();
+    }
+  }
+}
+
+GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options& opt, const Tensor& H1,
+                          const Tensor& H2, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol) {
+  GradResult g;
+  TapeScope none(nullptr);
+  // x = renormalize(A0) outside the AD (src/optimize/optim_gs.jl:55)
+  Tensor x = op::renormalize(c, A0);
+  Env e;
+  e.opt = opt;
+  e.dt = env_dt;
+  e.A = op::cast(c, x, env_dt);
+  e.Ad = op::conj(c, e.A);
+  init_ctm1(c, e);
+  auto r = run_ctm(c, e, chi, min_iter, max_iter, tol, 0, H1, H2);
+  g.conv_iters = r.iters;
+  // differentiated run (run_gs): same stopping rule with max_iter = ad_max_iter
+  std::vector<Checkpoint> ck;
+  std::vector<double> conv = {1.0, 0.0};
+  std::vector<double> diffs = {1.0};
+  int it = 0;
+  for (int i = 1; i <= ad_max_iter; ++i) {
+    it = i;
+    ck.push_back(save(e));
+    rg_step(c, e, chi);
+    const bool need = (i >= min_iter - 1 && i < ad_max_iter);
+    if (need) {
+      double E[2];
+      read_energy(c, energy(c, e, H1, H2), E);
+      std::vector<double> nc = {E[0], E[1]};
+      diffs.push_back(std::hypot(nc[0] - conv[0], nc[1] - conv[1]));
+      conv = nc;
+    } else {
+      conv = {NAN, NAN};
+      diffs.push_back(NAN);
+    }
+    if (i >= min_iter && diffs.back() < tol) break;
+  }
+  g.ad_iters = it;
+  // energy on the tape -> cotangents of the final environment (and of A)
+  std::array<Tensor, 4> cotC, cotE;
+  Tensor gA;
+  {
+    Tape tape;
+    tape.active = true;
+    TapeScope ts(&tape);
+    Tensor A = op::leaf(e.A);
+    e.A = A;
+    e.Ad = op::conj(c, A);
+    for (int i = 0; i < 4; ++i) {
+      e.C[i] = op::leaf(e.C[i]);
+      e.E[i] = op::leaf(e.E[i]);
+    }
+    Tensor Et = energy(c, e, H1, H2);
+    Tensor seed = c.empty(DType::F64, {8});
+    op::acc(c, Et.node, seed);  // the energy node ignores the seed value (dL/dRe E = 1)
+    tape.active = false;
+    read_energy(c, Et, g.E);
+    op::backward(c, tape);
+    for (int i = 0; i < 4; ++i) {
+      cotC[i] = e.C[i].node->grad;
+      cotE[i] = e.E[i].node->grad;
+    }
+    if (A.node->grad.defined()) gA = op::clone(c, A.node->grad);
+    e.A.node.reset();
+    e.Ad.node.reset();
+    for (int i = 0; i < 4; ++i) {
+      e.C[i].node.reset();
+      e.E[i].node.reset();
+    }
+  }
+  backprop_steps(c, e, ck, chi, cotC, cotE, gA);
+  DType gdt = env_dt == DType::F32 ? DType::F64 : env_dt;
+  g.grad = op::to_dtype(c, gA, gdt);
+  return g;
+}
+
+Tensor step_and_adjoint(Ctx& c, Env& e, int64_t chi, const Tensor& H1, const Tensor& H2, double E_out[2]) {
+  TapeScope none(nullptr);
+  std::vector<Checkpoint> ck = {save(e)};
+  rg_step(c, e, chi);
+  std::array<Tensor, 4> cotC, cotE;
+  Tensor gA;
+  {
+    Tape tape;
+    tape.active = true;
+    TapeScope ts(&tape);
+    Tensor A = op::leaf(e.A);
+    e.A = A;
+    e.Ad = op::conj(c, A);
+    for (int i = 0; i < 4; ++i) {
+      e.C[i] = op::leaf(e.C[i]);
+      e.E[i] = op::leaf(e.E[i]);
+    }
+    Tensor Et = energy(c, e, H1, H2);
+    op::acc(c, Et.node, c.empty(DType::F64, {8}));
+    tape.active = false;
+    op::backward(c, tape);
+    for (int i = 0; i < 4; ++i) {
+      cotC[i] = e.C[i].node->grad;
+      cotE[i] = e.E[i].node->grad;
+    }
+    if (A.node->grad.defined()) gA = op::clone(c, A.node->grad);
+    double h[8];
+    HIP_CHECK(hipMemcpyAsync(h, Et.data(), 64, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    E_out[0] = h[0];
+    E_out[1] = h[1];
+    e.A.node.reset();
+    e.Ad.node.reset();
+    for (int i = 0; i < 4; ++i) {
+      e.C[i].node.reset();
+      e.E[i].node.reset();
+    }
+  }
+  Checkpoint after = save(e);
+  backprop_steps(c, e, ck, chi, cotC, cotE, gA);
+  load(e, after);  // leave the environment advanced by one step
+  e.Ad = op::conj(c, e.A);
+  return gA;
+}
+
+}  // namespace ctm
+}  // namespace ipeps

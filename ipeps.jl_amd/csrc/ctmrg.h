@@ -1,0 +1,81 @@
+// CTMRG environment, directional moves, energy, and the step-checkpointed
+// adjoint of optim_gs's gradient.
+#pragma once
+
+#include <array>
+#include <vector>
+
+#include "ops.h"
+
+namespace ipeps {
+namespace ctm {
+
+// Engine options (settable per environment through the C ABI).
+struct Options {
+  op::SvdOpts svd;             // projector SVD (full = reference semantics, or randomized)
+  bool canonical_edge = true;  // P2-first edge update (chi^3 D^4) instead of the chi^2 D^6 intermediate
+  k::Mixed mixed = k::Mixed::BF16X3;
+  bool energy_fp64 = true;     // energy contractions in FP64 even for mixed environments
+};
+
+// Plain ground-state environment (src/ctm/ctm_tensor.jl:3-18).  Storage dtype
+// is F64, C128, or F32 (the mixed-precision engine).
+struct Env {
+  DType dt = DType::F64;
+  Tensor A, Ad;                 // (D, D, D, D, d): legs (up, left, down, right, phys)
+  std::array<Tensor, 4> C;      // C1 (down, right), C2 (left, down), C3 (up, left), C4 (up, right)
+  std::array<Tensor, 4> E;      // E1 (l, r, ket, bra), E2 (u, d, ., .), E3 (l, r, ., .), E4 (u, d, ., .)
+  Options opt;
+};
+
+// Boundary initialisation C_i, E_i from A and Ad (src/ctm/ctm_tensor.jl:97-130).
+void init_ctm1(Ctx& c, Env& e);
+
+// One CTMRG step left -> right -> top -> bottom (src/ctm/ctmrg.jl:57-64).
+// Returns the left move's truncated spectrum.
+std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi);
+
+// Two-site density matrices rho_h, rho_v (d^2 x d^2)
+// (src/graph/evaluation.jl:124-248, ground-state branch).
+void get_dms(Ctx& c, const Env& e, Tensor& roh, Tensor& rov);
+
+// E = tr(H1 roh)/tr(roh) + tr(H2 rov)/tr(rov), Na = (tr roh + tr rov)/2
+// (src/graph/evaluation.jl:15-29).  H1/H2: device C128 (m x m).  Returns a
+// device FP64[8] tensor [ReE, ImE, ReNa, ImNa, tr roh (re, im), tr rov (re, im)];
+// when recording, its node expects the seed dL/dRe(E) = 1.
+Tensor energy(Ctx& c, const Env& e, const Tensor& H1, const Tensor& H2);
+
+// <Ad|single-site environment|A> (src/graph/evaluation.jl:47-59, 110-122).
+void gs_norm(Ctx& c, const Env& e, double out[2]);
+
+struct RunResult {
+  int iters = 0;
+  std::vector<double> diffs;
+  double E[2] = {0, 0};
+  std::vector<double> s_last;
+};
+
+// run_ctm with the reference stopping rule (src/ctm/ctmrg.jl:1-55).
+// conv_mode 0: energy (conv_fun = get_gs_energy), 1: left spectrum.
+RunResult run_ctm(Ctx& c, Env& e, int64_t chi, int min_iter, int max_iter, double tol, int conv_mode,
+                  const Tensor& H1, const Tensor& H2, bool differentiable_tail = false);
+
+// Gradient contract of optim_gs's fg! (src/optimize/optim_gs.jl:53-91, 120-143):
+// x = renormalize(A0); converge from init_ctm1(x); then run ad steps from the
+// converged environment held constant and differentiate Re E w.r.t. x.
+// Step-level checkpointing keeps one step's tape alive at a time.
+struct GradResult {
+  double E[2] = {0, 0};
+  int conv_iters = 0, ad_iters = 0;
+  Tensor grad;  // same dims/dtype as A (F64 or C128; F32 env returns F64 grad)
+};
+GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options& opt, const Tensor& H1,
+                          const Tensor& H2, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol);
+
+// One differentiated step from a given environment: forward rg_step on the
+// tape, the energy and its VJP, then the step VJP; the A cotangent is
+// returned (the benchmark unit "step + adjoint").
+Tensor step_and_adjoint(Ctx& c, Env& e, int64_t chi, const Tensor& H1, const Tensor& H2, double E_out[2]);
+
+}  // namespace ctm
+}  // namespace ipeps

@@ -1,0 +1,710 @@
+// Permute, elementwise and reduction kernels (HBM-bound helpers).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.h"
+
+namespace ipeps {
+namespace k {
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline unsigned blocks_for(int64_t n, int per_thread = 1, unsigned cap = 65535u * 4) {
+  int64_t b = (n + (int64_t)TPB * per_thread - 1) / ((int64_t)TPB * per_thread);
+  if (b < 1) b = 1;
+  if (b > (int64_t)cap) b = cap;
+  return (unsigned)b;
+}
+
+// complex helpers on double2
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ inline double2 cdiv(double2 a, double2 b) {
+  double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+template <typename T>
+__device__ inline T conj_if(T v, bool c) { return v; }
+template <>
+__device__ inline double2 conj_if<double2>(double2 v, bool c) { return c ? cconj(v) : v; }
+
+__device__ inline double absval(double v) { return fabs(v); }
+__device__ inline double absval(float v) { return fabs((double)v); }
+__device__ inline double absval(double2 v) { return hypot(v.x, v.y); }
+
+struct PermP {
+  int nd;
+  int64_t odims[8];
+  int64_t istr[8];  // input stride of output axis i
+  int64_t n;
+};
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void permute_kernel(PermP p, const T* __restrict__ in, T* __restrict__ out,
+                                                      int conj) {
+  for (int64_t idx = (int64_t)blockIdx.x * TPB + threadIdx.x; idx < p.n; idx += (int64_t)gridDim.x * TPB) {
+    int64_t rem = idx, off = 0;
+#pragma unroll 8
+    for (int d = 0; d < p.nd; ++d) {
+      int64_t c = rem % p.odims[d];
+      rem /= p.odims[d];
+      off += c * p.istr[d];
+    }
+    out[idx] = conj_if(in[off], conj != 0);
+  }
+}
+
+// 2-D tiled transpose for the common case "first two output axes swap the
+// fastest input axis": out[j, i, rest] = in[i, j, rest] through LDS.
+template <typename T>
+__global__ __launch_bounds__(TPB) void transpose_kernel(int64_t R, int64_t Cc, int64_t batch, const T* __restrict__ in,
+                                                        T* __restrict__ out, int conj) {
+  __shared__ T tile[32][33];
+  int64_t b = blockIdx.z;
+  int64_t r0 = (int64_t)blockIdx.x * 32, c0 = (int64_t)blockIdx.y * 32;
+  const T* src = in + b * R * Cc;
+  T* dst = out + b * R * Cc;
+  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    int64_t r = r0 + tx, c = c0 + j;
+    if (r < R && c < Cc) tile[j][tx] = src[r + c * R];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    int64_t c = c0 + tx, r = r0 + j;
+    if (r < R && c < Cc) dst[c + r * Cc] = conj_if(tile[tx][j], conj != 0);
+  }
+}
+
+template <typename T>
+__global__ void axpy_kernel(int64_t n, double a, const T* x, T* y);
+template <>
+__global__ void axpy_kernel<double>(int64_t n, double a, const double* x, double* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] += a * x[i];
+}
+template <>
+__global__ void axpy_kernel<float>(int64_t n, double a, const float* x, float* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] += (float)a * x[i];
+}
+template <>
+__global__ void axpy_kernel<double2>(int64_t n, double a, const double2* x, double2* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+    double2 v = y[i];
+    v.x += a * x[i].x;
+    v.y += a * x[i].y;
+    y[i] = v;
+  }
+}
+
+template <typename T>
+__device__ inline T scal(T v, double f) { return (T)(v * f); }
+template <>
+__device__ inline double2 scal<double2>(double2 v, double f) { return make_double2(v.x * f, v.y * f); }
+
+template <typename T>
+__global__ void scale_dev_kernel(int64_t n, const T* x, T* y, const double* s, int mode) {
+  double f = mode == 0 ? 1.0 / s[0] : s[0];
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] = scal(x[i], f);
+}
+
+__global__ void conj_kernel(int64_t n, const double2* x, double2* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] = cconj(x[i]);
+}
+
+template <typename A, typename B>
+__global__ void convert_kernel(int64_t n, const A* x, B* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] = (B)x[i];
+}
+
+template <typename T>
+__global__ void scale_cols_kernel(int64_t rows, int64_t cols, const T* X, const double* s, T* Y, int rowwise) {
+  int64_t n = rows * cols;
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+    int64_t r = i % rows, c = i / rows;
+    Y[i] = scal(X[i], rowwise ? s[r] : s[c]);
+  }
+}
+
+// counter-based normal generator (splitmix64 + Box-Muller)
+__device__ inline uint64_t splitmix(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+__device__ inline double u01(uint64_t x) { return ((x >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+__device__ inline double normal_at(uint64_t seed, int64_t i) {
+  uint64_t a = splitmix(seed ^ (uint64_t)(2 * i)), b = splitmix(seed ^ (uint64_t)(2 * i + 1) ^ 0x5bd1e995ull);
+  double u1 = u01(a), u2 = u01(b);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+template <typename T>
+__global__ void randn_kernel(int64_t n, uint64_t seed, T* out);
+template <>
+__global__ void randn_kernel<double>(int64_t n, uint64_t seed, double* out) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) out[i] = normal_at(seed, i);
+}
+template <>
+__global__ void randn_kernel<float>(int64_t n, uint64_t seed, float* out) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) out[i] = (float)normal_at(seed, i);
+}
+template <>
+__global__ void randn_kernel<double2>(int64_t n, uint64_t seed, double2* out) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    out[i] = make_double2(normal_at(seed, 2 * i), normal_at(seed, 2 * i + 1));
+}
+
+// ---- reductions ----------------------------------------------------------
+struct MaxIdx {
+  double v;
+  int64_t i;
+};
+__device__ inline MaxIdx better(MaxIdx a, MaxIdx b) {
+  if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+  return a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void maxabs_partial(int64_t n, const T* x, double* pv, int64_t* pi) {
+  __shared__ double sv[TPB];
+  __shared__ int64_t si[TPB];
+  MaxIdx m{-1.0, (int64_t)1 << 62};
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    m = better(m, MaxIdx{absval(x[i]), i});
+  sv[threadIdx.x] = m.v;
+  si[threadIdx.x] = m.i;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      MaxIdx r = better(MaxIdx{sv[threadIdx.x], si[threadIdx.x]}, MaxIdx{sv[threadIdx.x + s], si[threadIdx.x + s]});
+      sv[threadIdx.x] = r.v;
+      si[threadIdx.x] = r.i;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = sv[0];
+    pi[blockIdx.x] = si[0];
+  }
+}
+
+__global__ __launch_bounds__(TPB) void maxabs_final(int nb, const double* pv, const int64_t* pi, double* out,
+                                                    int64_t* out_idx) {
+  __shared__ double sv[TPB];
+  __shared__ int64_t si[TPB];
+  MaxIdx m{-1.0, (int64_t)1 << 62};
+  for (int i = threadIdx.x; i < nb; i += TPB) m = better(m, MaxIdx{pv[i], pi[i]});
+  sv[threadIdx.x] = m.v;
+  si[threadIdx.x] = m.i;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      MaxIdx r = better(MaxIdx{sv[threadIdx.x], si[threadIdx.x]}, MaxIdx{sv[threadIdx.x + s], si[threadIdx.x + s]});
+      sv[threadIdx.x] = r.v;
+      si[threadIdx.x] = r.i;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sv[0];
+    if (out_idx) out_idx[0] = si[0];
+  }
+}
+
+__device__ inline double redot1(double a, double b) { return a * b; }
+__device__ inline double redot1(float a, float b) { return (double)a * (double)b; }
+__device__ inline double redot1(double2 a, double2 b) { return a.x * b.x + a.y * b.y; }
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void redot_partial(int64_t n, const T* x, const T* y, double* part) {
+  __shared__ double s[TPB];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) acc += redot1(x[i], y[i]);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = TPB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(TPB) void sum_final(int nb, const double* part, double* out) {
+  __shared__ double s[TPB];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += TPB) acc += part[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = TPB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+template <typename T>
+__global__ void renorm_back_kernel(int64_t n, const T* x, const T* gy, const double* mval, const int64_t* midx,
+                                   const double* dotv, T* gx);
+template <>
+__global__ void renorm_back_kernel<double>(int64_t n, const double* x, const double* gy, const double* mval,
+                                           const int64_t* midx, const double* dotv, double* gx) {
+  double m = mval[0];
+  int64_t im = midx[0];
+  double coef = -dotv[0] / (m * m);
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+    double v = gy[i] / m;
+    if (i == im) v += coef * (x[i] / fabs(x[i]));
+    gx[i] = v;
+  }
+}
+template <>
+__global__ void renorm_back_kernel<double2>(int64_t n, const double2* x, const double2* gy, const double* mval,
+                                            const int64_t* midx, const double* dotv, double2* gx) {
+  double m = mval[0];
+  int64_t im = midx[0];
+  double coef = -dotv[0] / (m * m);
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+    double2 v = make_double2(gy[i].x / m, gy[i].y / m);
+    if (i == im) {
+      double a = hypot(x[i].x, x[i].y);
+      v.x += coef * x[i].x / a;
+      v.y += coef * x[i].y / a;
+    }
+    gx[i] = v;
+  }
+}
+template <>
+__global__ void renorm_back_kernel<float>(int64_t n, const float* x, const float* gy, const double* mval,
+                                          const int64_t* midx, const double* dotv, float* gx) {
+  double m = mval[0];
+  int64_t im = midx[0];
+  double coef = -dotv[0] / (m * m);
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+    double v = gy[i] / m;
+    if (i == im) v += coef * (x[i] / fabs((double)x[i]));
+    gx[i] = (float)v;
+  }
+}
+
+// ---- SVD adjoint pieces -----------------------------------------------------
+__device__ inline double Fij(const double* S, int64_t i, int64_t j, double eta) {
+  double si = S[i] * S[i], sj = S[j] * S[j];
+  double f = sj - si;
+  return f / (f * f + eta);
+}
+
+// Gamma_ij = (J_ij + conj(J_ji)) s_j + s_i (K_ij + conj(K_ji)) + delta terms,
+// J_ij = F_ij UdU_ij (j < n), K_ij = F_ij VdV_ij (j < n); UdU/VdV are ld x n.
+template <typename T>
+__global__ void svd_gamma_kernel(int64_t N, int64_t n, int64_t ld, const double* S, const T* UdU, const T* VdV,
+                                 const double* dS, double eta, T* G);
+template <>
+__global__ void svd_gamma_kernel<double>(int64_t N, int64_t n, int64_t ld, const double* S, const double* UdU,
+                                         const double* VdV, const double* dS, double eta, double* G) {
+  int64_t tot = N * N;
+  for (int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x; t < tot; t += (int64_t)gridDim.x * TPB) {
+    int64_t i = t % N, j = t / N;
+    double v = 0.0;
+    double f = Fij(S, i, j, eta);
+    double Jij = (j < n && UdU) ? f * UdU[i + j * ld] : 0.0;
+    double Jji = (i < n && UdU) ? -f * UdU[j + i * ld] : 0.0;  // F_ji = -F_ij
+    double Kij = (j < n && VdV) ? f * VdV[i + j * ld] : 0.0;
+    double Kji = (i < n && VdV) ? -f * VdV[j + i * ld] : 0.0;
+    v = (Jij + Jji) * S[j] + S[i] * (Kij + Kji);
+    if (i == j && dS && i < n) v += dS[i];
+    G[t] = v;
+  }
+}
+template <>
+__global__ void svd_gamma_kernel<double2>(int64_t N, int64_t n, int64_t ld, const double* S, const double2* UdU,
+                                          const double2* VdV, const double* dS, double eta, double2* G) {
+  int64_t tot = N * N;
+  for (int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x; t < tot; t += (int64_t)gridDim.x * TPB) {
+    int64_t i = t % N, j = t / N;
+    double f = Fij(S, i, j, eta);
+    double2 z = make_double2(0.0, 0.0);
+    double2 Jij = (j < n && UdU) ? scal(UdU[i + j * ld], f) : z;
+    double2 Jji = (i < n && UdU) ? scal(UdU[j + i * ld], -f) : z;
+    double2 Kij = (j < n && VdV) ? scal(VdV[i + j * ld], f) : z;
+    double2 Kji = (i < n && VdV) ? scal(VdV[j + i * ld], -f) : z;
+    double2 a = make_double2(Jij.x + Jji.x, Jij.y - Jji.y);  // J + J^H
+    double2 b = make_double2(Kij.x + Kji.x, Kij.y - Kji.y);  // K + K^H
+    double2 v = make_double2(a.x * S[j] + S[i] * b.x, a.y * S[j] + S[i] * b.y);
+    if (i == j && i < n) {
+      if (dS) v.x += dS[i];
+      if (UdU) {
+        double sinv = S[i] / (S[i] * S[i] + eta);
+        v.y += UdU[i + i * ld].y * sinv;
+      }
+    }
+    G[t] = v;
+  }
+}
+
+__global__ void sinv_kernel(int64_t n, const double* S, double eta, double* Sinv) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    Sinv[i] = S[i] / (S[i] * S[i] + eta);
+}
+
+__global__ void inv_sqrt_kernel(int64_t n, const double* s, double* out) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) out[i] = 1.0 / sqrt(s[i]);
+}
+__global__ void inv_sqrt_back_kernel(int64_t n, const double* s, const double* g, double* gs) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    gs[i] += -0.5 * g[i] / (s[i] * sqrt(s[i]));
+}
+
+// out[c] = Re sum_r conj(X[r,c]) Y[r,c]  (or over columns for rowwise)
+template <typename T>
+__global__ __launch_bounds__(TPB) void col_redot_kernel(int64_t rows, int64_t cols, const T* X, const T* Y,
+                                                        double* out, int rowwise) {
+  __shared__ double s[TPB];
+  int64_t c = blockIdx.x;
+  double acc = 0.0;
+  if (!rowwise) {
+    for (int64_t r = threadIdx.x; r < rows; r += TPB) acc += redot1(X[r + c * rows], Y[r + c * rows]);
+  } else {
+    for (int64_t q = threadIdx.x; q < cols; q += TPB) acc += redot1(X[c + q * rows], Y[c + q * rows]);
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = TPB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = s[0];
+}
+
+__global__ void real_to_kernel_d(int64_t n, const double* x, double* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] = x[i];
+}
+__global__ void real_to_kernel_c(int64_t n, const double* x, double2* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] = make_double2(x[i], 0.0);
+}
+__global__ void real_of_kernel_c(int64_t n, const double2* x, double* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] = x[i].x;
+}
+
+// ---- energy (tiny, one thread) -------------------------------------------------
+__device__ inline double2 ld_el(const void* p, int64_t i, int cplx) {
+  if (cplx) return ((const double2*)p)[i];
+  return make_double2(((const double*)p)[i], 0.0);
+}
+
+__global__ void energy2_kernel(int64_t m, const void* rh, const void* rv, const double* H1, const double* H2,
+                               double* out, int cplx) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double2* h1 = (const double2*)H1;
+  const double2* h2 = (const double2*)H2;
+  double2 ah = make_double2(0, 0), av = make_double2(0, 0), bh = make_double2(0, 0), bv = make_double2(0, 0);
+  for (int64_t i = 0; i < m; ++i) {
+    bh.x += ld_el(rh, i + i * m, cplx).x; bh.y += ld_el(rh, i + i * m, cplx).y;
+    bv.x += ld_el(rv, i + i * m, cplx).x; bv.y += ld_el(rv, i + i * m, cplx).y;
+    for (int64_t j = 0; j < m; ++j) {
+      // tr(H rho) = sum_ij H_ij rho_ji
+      double2 t1 = cmul(h1[i + j * m], ld_el(rh, j + i * m, cplx));
+      double2 t2 = cmul(h2[i + j * m], ld_el(rv, j + i * m, cplx));
+      ah.x += t1.x; ah.y += t1.y;
+      av.x += t2.x; av.y += t2.y;
+    }
+  }
+  double2 E1 = cdiv(ah, bh), E2 = cdiv(av, bv);
+  out[0] = E1.x + E2.x;
+  out[1] = E1.y + E2.y;
+  out[2] = 0.5 * (bh.x + bv.x);
+  out[3] = 0.5 * (bh.y + bv.y);
+  out[4] = bh.x; out[5] = bh.y; out[6] = bv.x; out[7] = bv.y;
+}
+
+__global__ void energy2_back_kernel(int64_t m, const void* rh, const void* rv, const double* H1, const double* H2,
+                                    void* gh, void* gv, int cplx) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * m * m) return;
+  int which = t >= m * m;
+  int64_t e = which ? t - m * m : t;
+  const void* rho = which ? rv : rh;
+  const double2* H = (const double2*)(which ? H2 : H1);
+  // recompute a = tr(H rho), b = tr(rho) (m <= 16: cheap)
+  double2 a = make_double2(0, 0), b = make_double2(0, 0);
+  for (int64_t i = 0; i < m; ++i) {
+    double2 d = ld_el(rho, i + i * m, cplx);
+    b.x += d.x; b.y += d.y;
+    for (int64_t j = 0; j < m; ++j) {
+      double2 tt = cmul(H[i + j * m], ld_el(rho, j + i * m, cplx));
+      a.x += tt.x; a.y += tt.y;
+    }
+  }
+  // element rho_{jj', ii'} with column-major index e = r + c*m, r=row, c=col
+  int64_t r = e % m, c = e / m;
+  // dE/drho_rc = H_cr / b - delta_rc a / b^2
+  double2 g = cdiv(H[c + r * m], b);
+  if (r == c) {
+    double2 ab2 = cdiv(cdiv(a, b), b);
+    g.x -= ab2.x;
+    g.y -= ab2.y;
+  }
+  // cotangent of rho for L = Re E: conj(dE/drho) (complex) or Re (real)
+  if (cplx) ((double2*)(which ? gv : gh))[e] = cconj(g);
+  else ((double*)(which ? gv : gh))[e] = g.x;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define DISPATCH3(dt, KERNEL, ...)                                                                \
+  do {                                                                                            \
+    if ((dt) == DType::F64) { KERNEL(double, __VA_ARGS__); }                                      \
+    else if ((dt) == DType::C128) { KERNEL(double2, __VA_ARGS__); }                               \
+    else { KERNEL(float, __VA_ARGS__); }                                                          \
+  } while (0)
+
+void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::vector<int>& perm, const void* in,
+             void* out, bool conj) {
+  int nd = (int)in_dims.size();
+  IPEPS_CHECK(nd <= 8 && (int)perm.size() == nd, "permute: rank > 8 or bad perm");
+  int64_t n = 1;
+  for (auto d : in_dims) n *= d;
+  if (n == 0) return;
+  std::vector<int64_t> istride(nd);
+  int64_t s = 1;
+  for (int i = 0; i < nd; ++i) { istride[i] = s; s *= in_dims[i]; }
+  // Drop size-1 axes and merge axes that stay adjacent (in order) in both.
+  std::vector<int64_t> od, is;
+  for (int i = 0; i < nd; ++i) {
+    int64_t d = in_dims[perm[i]];
+    if (d == 1) continue;
+    if (!od.empty() && is.back() * od.back() == istride[perm[i]]) {
+      od.back() *= d;
+    } else {
+      od.push_back(d);
+      is.push_back(istride[perm[i]]);
+    }
+  }
+  const int cj = conj && dt == DType::C128;
+  if (od.size() <= 1) {  // plain copy
+    if (cj) conj_copy(c, dt, n, in, out);
+    else HIP_CHECK(hipMemcpyAsync(out, in, (size_t)n * dtype_size(dt), hipMemcpyDeviceToDevice, c.stream));
+    return;
+  }
+  // out axis0 = in axis j (stride is[0]), and in axis0 (stride 1) = out axis i:
+  // a 2-D transpose of [R x Cc] blocks when the permutation is [1,0,rest...]
+  if (od.size() >= 2 && is[1] == 1 && is[0] == od[1]) {
+    int64_t R = od[1], Cc = od[0];
+    bool rest_ok = true;
+    int64_t expect = R * Cc;
+    for (size_t a = 2; a < od.size(); ++a) {
+      if (is[a] != expect) { rest_ok = false; break; }
+      expect *= od[a];
+    }
+    if (rest_ok) {
+      int64_t batch = n / (R * Cc);
+      if (batch <= 65535) {
+        dim3 grid((unsigned)((R + 31) / 32), (unsigned)((Cc + 31) / 32), (unsigned)batch);
+#define TK(T, ...) hipLaunchKernelGGL(transpose_kernel<T>, grid, dim3(TPB), 0, c.stream, R, Cc, batch, (const T*)in, (T*)out, cj)
+        DISPATCH3(dt, TK, 0);
+#undef TK
+        HIP_CHECK(hipGetLastError());
+        return;
+      }
+    }
+  }
+  PermP p{};
+  p.nd = (int)od.size();
+  for (int i = 0; i < p.nd; ++i) { p.odims[i] = od[i]; p.istr[i] = is[i]; }
+  p.n = n;
+#define PK(T, ...) hipLaunchKernelGGL(permute_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, p, (const T*)in, (T*)out, cj)
+  DISPATCH3(dt, PK, 0);
+#undef PK
+  HIP_CHECK(hipGetLastError());
+}
+
+void fill_zero(Ctx& c, void* p, size_t bytes) {
+  if (bytes) HIP_CHECK(hipMemsetAsync(p, 0, bytes, c.stream));
+}
+
+void axpy(Ctx& c, DType dt, int64_t n, double a, const void* x, void* y) {
+  if (n <= 0) return;
+#define AK(T, ...) hipLaunchKernelGGL(axpy_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, a, (const T*)x, (T*)y)
+  DISPATCH3(dt, AK, 0);
+#undef AK
+  HIP_CHECK(hipGetLastError());
+}
+
+void scale_by_dev(Ctx& c, DType dt, int64_t n, const void* x, void* y, const double* s, int mode) {
+  if (n <= 0) return;
+#define SK(T, ...) hipLaunchKernelGGL(scale_dev_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const T*)x, (T*)y, s, mode)
+  DISPATCH3(dt, SK, 0);
+#undef SK
+  HIP_CHECK(hipGetLastError());
+}
+
+void conj_copy(Ctx& c, DType dt, int64_t n, const void* x, void* y) {
+  if (n <= 0) return;
+  if (dt != DType::C128) {
+    if (x != y) HIP_CHECK(hipMemcpyAsync(y, x, (size_t)n * dtype_size(dt), hipMemcpyDeviceToDevice, c.stream));
+    return;
+  }
+  hipLaunchKernelGGL(conj_kernel, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const double2*)x, (double2*)y);
+  HIP_CHECK(hipGetLastError());
+}
+
+void convert(Ctx& c, DType from, DType to, int64_t n, const void* x, void* y) {
+  if (n <= 0) return;
+  if (from == to) {
+    HIP_CHECK(hipMemcpyAsync(y, x, (size_t)n * dtype_size(to), hipMemcpyDeviceToDevice, c.stream));
+    return;
+  }
+  if (from == DType::F64 && to == DType::F32)
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const double*)x, (float*)y);
+  else if (from == DType::F32 && to == DType::F64)
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const float*)x, (double*)y);
+  else
+    throw Error(IPEPS_ERR_ARG, "convert: unsupported dtype pair");
+  HIP_CHECK(hipGetLastError());
+}
+
+void scale_cols(Ctx& c, DType dt, int64_t rows, int64_t cols, const void* X, const double* s, void* Y, bool rowwise) {
+  int64_t n = rows * cols;
+  if (n <= 0) return;
+#define CK(T, ...) hipLaunchKernelGGL(scale_cols_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, rows, cols, (const T*)X, s, (T*)Y, (int)rowwise)
+  DISPATCH3(dt, CK, 0);
+#undef CK
+  HIP_CHECK(hipGetLastError());
+}
+
+void randn(Ctx& c, DType dt, int64_t n, uint64_t seed, void* out) {
+  if (n <= 0) return;
+#define RK(T, ...) hipLaunchKernelGGL(randn_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, seed, (T*)out)
+  DISPATCH3(dt, RK, 0);
+#undef RK
+  HIP_CHECK(hipGetLastError());
+}
+
+void maxabs(Ctx& c, DType dt, int64_t n, const void* x, double* out, int64_t* out_idx) {
+  IPEPS_CHECK(n > 0, "maxabs of an empty tensor");
+  unsigned nb = std::min<unsigned>(blocks_for(n, 8), 1024u);
+  std::shared_ptr<Buf> scratch(new Buf{c.pool->alloc(nb * 16), nb * 16, c.pool.get()});
+  double* pv = (double*)scratch->p;
+  int64_t* pi = (int64_t*)((char*)scratch->p + nb * 8);
+#define MK(T, ...) hipLaunchKernelGGL(maxabs_partial<T>, dim3(nb), dim3(TPB), 0, c.stream, n, (const T*)x, pv, pi)
+  DISPATCH3(dt, MK, 0);
+#undef MK
+  hipLaunchKernelGGL(maxabs_final, dim3(1), dim3(TPB), 0, c.stream, (int)nb, pv, pi, out, out_idx);
+  HIP_CHECK(hipGetLastError());
+}
+
+void re_dot(Ctx& c, DType dt, int64_t n, const void* x, const void* y, double* out) {
+  unsigned nb = std::min<unsigned>(blocks_for(n, 8), 1024u);
+  std::shared_ptr<Buf> scratch(new Buf{c.pool->alloc(nb * 8), nb * 8, c.pool.get()});
+  double* part = (double*)scratch->p;
+  if (n <= 0) {
+    fill_zero(c, out, 8);
+    return;
+  }
+#define DK(T, ...) hipLaunchKernelGGL(redot_partial<T>, dim3(nb), dim3(TPB), 0, c.stream, n, (const T*)x, (const T*)y, part)
+  DISPATCH3(dt, DK, 0);
+#undef DK
+  hipLaunchKernelGGL(sum_final, dim3(1), dim3(TPB), 0, c.stream, (int)nb, part, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+void renorm_back(Ctx& c, DType dt, int64_t n, const void* x, const void* gy, const double* mval, const int64_t* midx,
+                 const double* dotv, void* gx) {
+#define BK(T, ...) hipLaunchKernelGGL(renorm_back_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const T*)x, (const T*)gy, mval, midx, dotv, (T*)gx)
+  DISPATCH3(dt, BK, 0);
+#undef BK
+  HIP_CHECK(hipGetLastError());
+}
+
+void svd_gamma(Ctx& c, DType dt, int64_t N, int64_t n, const double* S, const void* UdU, const void* VdV,
+               const double* dS, double eta, void* Gamma) {
+  int64_t tot = N * N;
+  if (dt == DType::F64)
+    hipLaunchKernelGGL(svd_gamma_kernel<double>, dim3(blocks_for(tot, 4)), dim3(TPB), 0, c.stream, N, n, N, S,
+                       (const double*)UdU, (const double*)VdV, dS, eta, (double*)Gamma);
+  else if (dt == DType::C128)
+    hipLaunchKernelGGL(svd_gamma_kernel<double2>, dim3(blocks_for(tot, 4)), dim3(TPB), 0, c.stream, N, n, N, S,
+                       (const double2*)UdU, (const double2*)VdV, dS, eta, (double2*)Gamma);
+  else
+    throw Error(IPEPS_ERR_ARG, "svd_gamma: FP64 or C128 only");
+  HIP_CHECK(hipGetLastError());
+}
+
+void svd_gamma_thin(Ctx& c, DType dt, int64_t n, const double* S, const void* UdU, const void* VdV, const double* dS,
+                    double eta, void* Gamma, double* Sinv) {
+  svd_gamma(c, dt, n, n, S, UdU, VdV, dS, eta, Gamma);
+  hipLaunchKernelGGL(sinv_kernel, dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, S, eta, Sinv);
+  HIP_CHECK(hipGetLastError());
+}
+
+void inv_sqrt(Ctx& c, int64_t n, const double* s, double* out) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(inv_sqrt_kernel, dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, s, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+void inv_sqrt_back(Ctx& c, int64_t n, const double* s, const double* g, double* gs) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(inv_sqrt_back_kernel, dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, s, g, gs);
+  HIP_CHECK(hipGetLastError());
+}
+
+void col_re_dot(Ctx& c, DType dt, int64_t rows, int64_t cols, const void* X, const void* Y, double* out,
+                bool rowwise) {
+  int64_t nb = rowwise ? rows : cols;
+  if (nb <= 0) return;
+#define RD(T, ...) hipLaunchKernelGGL(col_redot_kernel<T>, dim3((unsigned)nb), dim3(TPB), 0, c.stream, rows, cols, (const T*)X, (const T*)Y, out, (int)rowwise)
+  DISPATCH3(dt, RD, 0);
+#undef RD
+  HIP_CHECK(hipGetLastError());
+}
+
+void real_to(Ctx& c, DType dt, int64_t n, const double* x, void* y) {
+  if (n <= 0) return;
+  if (dt == DType::F64)
+    hipLaunchKernelGGL(real_to_kernel_d, dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, x, (double*)y);
+  else if (dt == DType::C128)
+    hipLaunchKernelGGL(real_to_kernel_c, dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, x, (double2*)y);
+  else
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, x, (float*)y);
+  HIP_CHECK(hipGetLastError());
+}
+
+void real_of(Ctx& c, DType dt, int64_t n, const void* x, double* y) {
+  if (n <= 0) return;
+  if (dt == DType::F64)
+    HIP_CHECK(hipMemcpyAsync(y, x, (size_t)n * 8, hipMemcpyDeviceToDevice, c.stream));
+  else if (dt == DType::C128)
+    hipLaunchKernelGGL(real_of_kernel_c, dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, (const double2*)x, y);
+  else
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, (const float*)x, y);
+  HIP_CHECK(hipGetLastError());
+}
+
+void energy2(Ctx& c, DType dt, int64_t m, const void* rho_h, const void* rho_v, const double* H1, const double* H2,
+             double* out) {
+  IPEPS_CHECK(dt != DType::F32, "energy2: FP64 reduction expects F64 or C128 density matrices");
+  hipLaunchKernelGGL(energy2_kernel, dim3(1), dim3(64), 0, c.stream, m, rho_h, rho_v, H1, H2, out,
+                     (int)(dt == DType::C128));
+  HIP_CHECK(hipGetLastError());
+}
+
+void energy2_back(Ctx& c, DType dt, int64_t m, const void* rho_h, const void* rho_v, const double* H1,
+                  const double* H2, void* grho_h, void* grho_v) {
+  IPEPS_CHECK(dt != DType::F32, "energy2_back: F64 or C128 only");
+  int64_t n = 2 * m * m;
+  hipLaunchKernelGGL(energy2_back_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, m, rho_h, rho_v,
+                     H1, H2, grho_h, grho_v, (int)(dt == DType::C128));
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace k
+}  // namespace ipeps

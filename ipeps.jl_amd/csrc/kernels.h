@@ -1,0 +1,103 @@
+// Raw kernel launchers (no autodiff).  All run on the context stream.
+#pragma once
+
+#include "common.h"
+
+namespace ipeps {
+namespace k {
+
+// Which split-bf16 arithmetic the FP32 GEMM uses on MFMA:
+//   1 = plain bf16 products, 3 = hi*hi + hi*lo + lo*hi (two-term split),
+//   6 = three-term split (hi, mid, lo) keeping every product above 2^-24.
+enum class Mixed : int { BF16 = 1, BF16X3 = 3, BF16X6 = 6 };
+
+// Column-major BLAS-style GEMM:  C = alpha * op(A) * op(B) + beta * C
+// op(X) = X, X^T (trans), conj(X) (conj) or X^H (trans && conj).
+// op(A) is M x K, op(B) is K x N, C is M x N.  Batched over `batch` with
+// element strides sA, sB, sC.  dt selects the arithmetic:
+//   F64  -> FP64 MFMA (v_mfma_f64_16x16x4)
+//   C128 -> complex FP64 as four real FP64 MFMA products
+//   F32  -> FP32 storage, split-bf16 MFMA products, FP32 accumulation
+struct GemmArgs {
+  DType dt = DType::F64;
+  bool transA = false, conjA = false, transB = false, conjB = false;
+  int64_t M = 0, N = 0, K = 0;
+  double alpha = 1.0, beta = 0.0;
+  const void* A = nullptr;
+  int64_t lda = 0, sA = 0;
+  const void* B = nullptr;
+  int64_t ldb = 0, sB = 0;
+  void* C = nullptr;
+  int64_t ldc = 0, sC = 0;
+  int64_t batch = 1;
+  Mixed mixed = Mixed::BF16X3;
+};
+void gemm(Ctx& c, const GemmArgs& g);
+
+// out = conj?(in) permuted: out dims[i] = in dims[perm[i]].  Column-major.
+void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::vector<int>& perm,
+             const void* in, void* out, bool conj);
+
+// Elementwise helpers.  n counts scalars of dt (complex = 1 element).
+void fill_zero(Ctx& c, void* p, size_t bytes);
+void axpy(Ctx& c, DType dt, int64_t n, double a, const void* x, void* y);        // y += a x
+void scale_by_dev(Ctx& c, DType dt, int64_t n, const void* x, void* y,           // y = x * f(*s)
+                  const double* s, int mode);                                    // mode 0: 1/s, 1: s
+void conj_copy(Ctx& c, DType dt, int64_t n, const void* x, void* y);
+void convert(Ctx& c, DType from, DType to, int64_t n, const void* x, void* y);   // F64<->F32
+// Column / row scaling of a column-major matrix X (rows x cols) by a real
+// vector s (FP64): Y = X diag(s) (cols) or Y = diag(s) X (rows).
+void scale_cols(Ctx& c, DType dt, int64_t rows, int64_t cols, const void* X, const double* s,
+                void* Y, bool rowwise);
+// Gaussian N x k test matrix from a counter-based generator (deterministic).
+void randn(Ctx& c, DType dt, int64_t n, uint64_t seed, void* out);
+
+// Reductions into device scalars.
+//   maxabs: out[0] = max |x|, out_idx[0] = first column-major index of it.
+void maxabs(Ctx& c, DType dt, int64_t n, const void* x, double* out, int64_t* out_idx);
+//   re_dot: out[0] = Re sum conj(x) y
+void re_dot(Ctx& c, DType dt, int64_t n, const void* x, const void* y, double* out);
+
+// Renormalisation backward:
+//   gx = gy / m  +  e_idx * (-(Re<gy, x>) / m^2) * x_idx / |x_idx|
+// with m = *mval, idx = *midx, rd = *dotv (= Re<gy, x>).
+void renorm_back(Ctx& c, DType dt, int64_t n, const void* x, const void* gy, const double* mval,
+                 const int64_t* midx, const double* dotv, void* gx);
+
+// svd_back core: builds Gamma (N x N, U's dtype) from UdU = U^H dU (N x n),
+// VdV = V^H dV (N x n), dS (n or null), S (N, FP64):
+//   Gamma = (J + J^H) S + S (K + K^H) + diag(dS) + diag(i Im(diag UdU) Sinv)
+// where J = F o UdU, K = F o VdV zero-padded beyond column n and
+// F_ij = (s_j^2 - s_i^2) / ((s_j^2 - s_i^2)^2 + eta).
+void svd_gamma(Ctx& c, DType dt, int64_t N, int64_t n, const double* S, const void* UdU,
+               const void* VdV, const double* dS, double eta, void* Gamma);
+// Same for the thin (truncated) factorisation, Gamma is n x n and the
+// diagonal also receives nothing extra (the non-square terms are GEMMs).
+// Writes also Sinv_n (n) = s/(s^2+eta).
+void svd_gamma_thin(Ctx& c, DType dt, int64_t n, const double* S, const void* UdU,
+                    const void* VdV, const double* dS, double eta, void* Gamma, double* Sinv);
+
+// s (FP64, n) -> out = s^(-1/2)
+void inv_sqrt(Ctx& c, int64_t n, const double* s, double* out);
+// gs += -0.5 * s^(-3/2) * g_isq   (backward of inv_sqrt)
+void inv_sqrt_back(Ctx& c, int64_t n, const double* s, const double* g_isq, double* gs);
+// out[j] = Re sum_i conj(X_ij) * Y_ij over columns of (rows x cols) matrices
+// (used for the gradient of a column scaling).
+void col_re_dot(Ctx& c, DType dt, int64_t rows, int64_t cols, const void* X, const void* Y,
+                double* out, bool rowwise);
+// Copy a real FP64 vector into the real parts of dt (F64/C128) storage.
+void real_to(Ctx& c, DType dt, int64_t n, const double* x, void* y);
+// Real parts (FP64) of dt storage.
+void real_of(Ctx& c, DType dt, int64_t n, const void* x, double* y);
+
+// Small energy kernel: given rho_h, rho_v (m x m, dt), H1, H2 (m x m, C128
+// on device), computes out = [Re E, Im E, Re Na, Im Na, trh re, im, trv re, im]
+// where E = tr(H1 rho_h)/tr(rho_h) + tr(H2 rho_v)/tr(rho_v), all in FP64.
+void energy2(Ctx& c, DType dt, int64_t m, const void* rho_h, const void* rho_v, const double* H1,
+             const double* H2, double* out);
+// Its backward for dL/dRe(E) = 1: grho_h, grho_v (dt, m x m).
+void energy2_back(Ctx& c, DType dt, int64_t m, const void* rho_h, const void* rho_v,
+                  const double* H1, const double* H2, void* grho_h, void* grho_v);
+
+}  // namespace k
+}  // namespace ipeps

@@ -1,0 +1,721 @@
+// Differentiable device ops, the contraction planner and the SVD/QR wrappers.
+#include "ops.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace ipeps {
+
+Tape*& current_tape() {
+  thread_local Tape* t = nullptr;
+  return t;
+}
+
+namespace op {
+
+Prec& prec() {
+  thread_local Prec p;
+  return p;
+}
+
+bool recording() {
+  Tape* t = current_tape();
+  return t && t->active;
+}
+
+static bool needs(const Tensor& t) { return recording() && (bool)t.node; }
+
+std::shared_ptr<Node> make_node(std::function<void(Ctx&, Node&)> back) {
+  auto n = std::make_shared<Node>();
+  n->back = std::move(back);
+  current_tape()->order.push_back(n);
+  return n;
+}
+
+Tensor leaf(Tensor t) {
+  if (!recording()) return t;
+  auto n = std::make_shared<Node>();
+  n->leaf = true;
+  current_tape()->order.push_back(n);
+  t.node = n;
+  return t;
+}
+
+Tensor clone(Ctx& c, const Tensor& t) {
+  Tensor r = c.empty(t.dt, t.dims);
+  if (t.bytes()) HIP_CHECK(hipMemcpyAsync(r.data(), t.data(), t.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  return r;
+}
+
+void acc(Ctx& c, const std::shared_ptr<Node>& n, const Tensor& g) {
+  if (!n) return;
+  Tensor gg = g;
+  gg.node.reset();
+  if (!n->grad.defined()) {
+    n->grad = gg;
+    return;
+  }
+  IPEPS_CHECK(n->grad.numel() == g.numel() && n->grad.dt == g.dt,
+              "acc: cotangent mismatch " + dims_str(n->grad.dims) + " vs " + dims_str(g.dims));
+  if (n->grad.buf.use_count() > 1) {
+    Tensor cl = clone(c, n->grad);
+    n->grad = cl;
+  }
+  k::axpy(c, g.dt, g.numel(), 1.0, g.data(), n->grad.data());
+}
+
+void backward(Ctx& c, Tape& t) {
+  TapeScope ts(nullptr);
+  for (auto it = t.order.rbegin(); it != t.order.rend(); ++it) {
+    Node& n = **it;
+    if (!n.back) continue;
+    bool any = n.grad.defined();
+    for (auto& s : n.slots) any = any || s.defined();
+    if (any) n.back(c, n);
+    n.grad = Tensor();
+    n.slots.clear();
+    n.back = nullptr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// permute / conj / reshape
+// ---------------------------------------------------------------------------
+Tensor permute_raw(Ctx& c, const Tensor& t, const std::vector<int>& perm, bool cj) {
+  std::vector<int64_t> od(perm.size());
+  for (size_t i = 0; i < perm.size(); ++i) od[i] = t.dims.at(perm[i]);
+  Tensor r = c.empty(t.dt, od);
+  k::permute(c, t.dt, t.dims, perm, t.data(), r.data(), cj);
+  return r;
+}
+
+static std::vector<int> inverse_perm(const std::vector<int>& p) {
+  std::vector<int> q(p.size());
+  for (size_t i = 0; i < p.size(); ++i) q[p[i]] = (int)i;
+  return q;
+}
+
+Tensor permute(Ctx& c, const Tensor& t, const std::vector<int>& perm) {
+  IPEPS_CHECK((int)perm.size() == t.nd(), "permute: rank mismatch");
+  Tensor r = permute_raw(c, t, perm, false);
+  if (needs(t)) {
+    auto in = t.node;
+    auto ip = inverse_perm(perm);
+    r.node = make_node([in, ip](Ctx& c, Node& n) { acc(c, in, permute_raw(c, n.grad, ip, false)); });
+  }
+  return r;
+}
+
+Tensor conj(Ctx& c, const Tensor& t) {
+  if (t.dt != DType::C128) {
+    Tensor r = t;  // real: identity
+    if (needs(t)) {
+      auto in = t.node;
+      r.node = make_node([in](Ctx& c, Node& n) { acc(c, in, n.grad); });
+    }
+    return r;
+  }
+  Tensor r = c.empty(t.dt, t.dims);
+  k::conj_copy(c, t.dt, t.numel(), t.data(), r.data());
+  if (needs(t)) {
+    auto in = t.node;
+    r.node = make_node([in](Ctx& c, Node& n) {
+      Tensor g = c.empty(n.grad.dt, n.grad.dims);
+      k::conj_copy(c, g.dt, g.numel(), n.grad.data(), g.data());
+      acc(c, in, g);
+    });
+  }
+  return r;
+}
+
+Tensor reshape(Ctx& c, const Tensor& t, const std::vector<int64_t>& dims) {
+  Tensor r = ipeps::reshape(t, dims);
+  if (needs(t)) {
+    auto in = t.node;
+    auto od = t.dims;
+    r.node = make_node([in, od](Ctx& c, Node& n) { acc(c, in, ipeps::reshape(n.grad, od)); });
+  }
+  return r;
+}
+
+Tensor to_dtype(Ctx& c, const Tensor& t, DType dt) {
+  if (t.dt == dt) return t;
+  Tensor r = c.empty(dt, t.dims);
+  k::convert(c, t.dt, dt, t.numel(), t.data(), r.data());
+  return r;
+}
+
+Tensor cast(Ctx& c, const Tensor& t, DType dt) {
+  if (t.dt == dt) return t;
+  Tensor r = to_dtype(c, t, dt);
+  r.node.reset();
+  if (needs(t)) {
+    auto in = t.node;
+    DType from = t.dt;
+    r.node = make_node([in, from](Ctx& c, Node& n) { acc(c, in, to_dtype(c, n.grad, from)); });
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// contraction planner
+// ---------------------------------------------------------------------------
+namespace {
+
+bool layout_is_identity(const std::vector<int>& layout, const std::vector<int64_t>& dims) {
+  int last = -1;
+  for (int ax : layout) {
+    if (dims[ax] == 1) continue;
+    if (ax < last) return false;
+    last = ax;
+  }
+  return true;
+}
+
+int64_t prod_of(const std::vector<int>& axes, const std::vector<int64_t>& dims) {
+  int64_t p = 1;
+  for (int a : axes) p *= dims[a];
+  return p;
+}
+
+std::vector<int> cat(const std::vector<int>& a, const std::vector<int>& b) {
+  std::vector<int> r(a);
+  r.insert(r.end(), b.begin(), b.end());
+  return r;
+}
+
+}  // namespace
+
+Tensor matmul_raw(Ctx& c, const Tensor& A, bool tA, bool cA, const Tensor& B, bool tB, bool cB) {
+  IPEPS_CHECK(A.nd() == 2 && B.nd() == 2, "matmul: matrices expected");
+  IPEPS_CHECK(A.dt == B.dt, "matmul: dtype mismatch");
+  int64_t M = tA ? A.dims[1] : A.dims[0];
+  int64_t K = tA ? A.dims[0] : A.dims[1];
+  int64_t K2 = tB ? B.dims[1] : B.dims[0];
+  int64_t N = tB ? B.dims[0] : B.dims[1];
+  IPEPS_CHECK(K == K2, "matmul: inner dimension mismatch " + dims_str(A.dims) + " x " + dims_str(B.dims));
+  Tensor C = c.empty(A.dt, {M, N});
+  k::GemmArgs g;
+  g.dt = A.dt;
+  g.transA = tA; g.conjA = cA; g.transB = tB; g.conjB = cB;
+  g.M = M; g.N = N; g.K = K;
+  g.A = A.data(); g.lda = std::max<int64_t>(1, A.dims[0]);
+  g.B = B.data(); g.ldb = std::max<int64_t>(1, B.dims[0]);
+  g.C = C.data(); g.ldc = std::max<int64_t>(1, M);
+  g.mixed = prec().mixed;
+  k::gemm(c, g);
+  return C;
+}
+
+Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA, const Tensor& B,
+                const std::vector<int>& lb, bool conjB) {
+  const int na = A.nd(), nb = B.nd();
+  IPEPS_CHECK((int)la.size() == na && (int)lb.size() == nb, "tcon: label count does not match tensor rank");
+  IPEPS_CHECK(A.dt == B.dt, "tcon: dtype mismatch");
+  std::vector<int> ca, cb;
+  for (int p = 0; p < na; ++p) {
+    if (la[p] > 0) {
+      auto it = std::find(lb.begin(), lb.end(), la[p]);
+      IPEPS_CHECK(it != lb.end(), "tcon: positive label appears in one tensor only");
+      int q = (int)(it - lb.begin());
+      IPEPS_CHECK(A.dims[p] == B.dims[q], "tcon: contracted dimensions differ");
+      ca.push_back(p);
+      cb.push_back(q);
+    }
+  }
+  for (int q = 0; q < nb; ++q)
+    if (lb[q] > 0) IPEPS_CHECK(std::find(la.begin(), la.end(), lb[q]) != la.end(), "tcon: unmatched label");
+  int nout = 0;
+  for (int l : la) nout += l < 0;
+  for (int l : lb) nout += l < 0;
+  std::vector<int64_t> odims(nout, -1);
+  std::vector<int> osrc(nout, -1), opos(nout, -1);
+  for (int p = 0; p < na; ++p)
+    if (la[p] < 0) {
+      int kk = -la[p] - 1;
+      IPEPS_CHECK(kk < nout && osrc[kk] < 0, "tcon: bad output label");
+      odims[kk] = A.dims[p]; osrc[kk] = 0; opos[kk] = p;
+    }
+  for (int q = 0; q < nb; ++q)
+    if (lb[q] < 0) {
+      int kk = -lb[q] - 1;
+      IPEPS_CHECK(kk < nout && osrc[kk] < 0, "tcon: bad output label");
+      odims[kk] = B.dims[q]; osrc[kk] = 1; opos[kk] = q;
+    }
+  std::vector<int> fa_nat, fb_nat, fa_out, fb_out;
+  for (int p = 0; p < na; ++p) if (la[p] < 0) fa_nat.push_back(p);
+  for (int q = 0; q < nb; ++q) if (lb[q] < 0) fb_nat.push_back(q);
+  for (int kk = 0; kk < nout; ++kk) (osrc[kk] == 0 ? fa_out : fb_out).push_back(opos[kk]);
+
+  // contracted orders: A's natural order, or B's natural order
+  std::vector<int> ca_b(ca), cb_b(cb);
+  {
+    std::vector<int> idx(cb.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int i, int j) { return cb[i] < cb[j]; });
+    for (size_t i = 0; i < idx.size(); ++i) { ca_b[i] = ca[idx[i]]; cb_b[i] = cb[idx[i]]; }
+  }
+  const int64_t M = prod_of(fa_nat, A.dims), N = prod_of(fb_nat, B.dims), K = prod_of(ca, A.dims);
+  const int64_t nA = A.numel(), nB = B.numel(), nC = M * N;
+
+  struct Choice {
+    int64_t cost;
+    std::vector<int> CA, CB, FA, FB;
+    bool ct;
+  } best{INT64_MAX, {}, {}, {}, {}, false};
+  for (int co = 0; co < 2; ++co) {
+    const auto& CA = co ? ca_b : ca;
+    const auto& CB = co ? cb_b : cb;
+    for (int fo = 0; fo < 4; ++fo) {
+      const auto& FA = (fo & 1) ? fa_out : fa_nat;
+      const auto& FB = (fo & 2) ? fb_out : fb_nat;
+      int64_t costA = (layout_is_identity(cat(FA, CA), A.dims) || layout_is_identity(cat(CA, FA), A.dims)) ? 0 : nA;
+      int64_t costB = (layout_is_identity(cat(CB, FB), B.dims) || layout_is_identity(cat(FB, CB), B.dims)) ? 0 : nB;
+      for (int ct = 0; ct < 2; ++ct) {
+        // result legs in output numbering
+        std::vector<int> res;
+        auto push = [&](const std::vector<int>& F, int src) {
+          for (int pos : F)
+            for (int kk = 0; kk < nout; ++kk)
+              if (osrc[kk] == src && opos[kk] == pos) res.push_back(kk);
+        };
+        if (!ct) { push(FA, 0); push(FB, 1); }
+        else { push(FB, 1); push(FA, 0); }
+        int64_t costC = layout_is_identity(res, odims) ? 0 : nC;
+        int64_t cost = costA + costB + costC;
+        if (cost < best.cost) best = Choice{cost, CA, CB, FA, FB, (bool)ct};
+      }
+    }
+  }
+
+  // materialise operands
+  Tensor Am = A, Bm = B;
+  bool a_kxm;  // A stored as K x M (else M x K)
+  bool cjA = conjA, cjB = conjB;
+  if (layout_is_identity(cat(best.FA, best.CA), A.dims)) a_kxm = false;
+  else if (layout_is_identity(cat(best.CA, best.FA), A.dims)) a_kxm = true;
+  else {
+    Am = permute_raw(c, A, cat(best.FA, best.CA), conjA && A.dt == DType::C128);
+    cjA = false;
+    a_kxm = false;
+  }
+  bool b_kxn;  // B stored as K x N (else N x K)
+  if (layout_is_identity(cat(best.CB, best.FB), B.dims)) b_kxn = true;
+  else if (layout_is_identity(cat(best.FB, best.CB), B.dims)) b_kxn = false;
+  else {
+    Bm = permute_raw(c, B, cat(best.CB, best.FB), conjB && B.dt == DType::C128);
+    cjB = false;
+    b_kxn = true;
+  }
+  std::vector<int64_t> rdims;
+  std::vector<int> res;
+  auto push = [&](const std::vector<int>& F, int src) {
+    for (int pos : F)
+      for (int kk = 0; kk < nout; ++kk)
+        if (osrc[kk] == src && opos[kk] == pos) { res.push_back(kk); rdims.push_back(odims[kk]); }
+  };
+  if (!best.ct) { push(best.FA, 0); push(best.FB, 1); }
+  else { push(best.FB, 1); push(best.FA, 0); }
+  Tensor R = c.empty(A.dt, rdims);
+  if (nC > 0) {
+    k::GemmArgs g;
+    g.dt = A.dt;
+    g.K = K;
+    g.mixed = prec().mixed;
+    if (!best.ct) {
+      g.M = M; g.N = N;
+      g.A = Am.data(); g.transA = a_kxm; g.lda = std::max<int64_t>(1, a_kxm ? K : M); g.conjA = cjA;
+      g.B = Bm.data(); g.transB = !b_kxn; g.ldb = std::max<int64_t>(1, b_kxn ? K : N); g.conjB = cjB;
+      g.C = R.data(); g.ldc = std::max<int64_t>(1, M);
+    } else {
+      g.M = N; g.N = M;
+      g.A = Bm.data(); g.transA = b_kxn; g.lda = std::max<int64_t>(1, b_kxn ? K : N); g.conjA = cjB;
+      g.B = Am.data(); g.transB = !a_kxm; g.ldb = std::max<int64_t>(1, a_kxm ? K : M); g.conjB = cjA;
+      g.C = R.data(); g.ldc = std::max<int64_t>(1, N);
+    }
+    if (K == 0) k::fill_zero(c, R.data(), R.bytes());
+    else k::gemm(c, g);
+  }
+  if (layout_is_identity(res, odims)) return ipeps::reshape(R, odims);
+  // R has legs res[0..]; output wants leg kk at position kk
+  std::vector<int> perm(nout);
+  for (int kk = 0; kk < nout; ++kk) {
+    auto it = std::find(res.begin(), res.end(), kk);
+    perm[kk] = (int)(it - res.begin());
+  }
+  return permute_raw(c, R, perm, false);
+}
+
+Tensor tcon(Ctx& c, const Tensor& A, const std::vector<int>& la, const Tensor& B, const std::vector<int>& lb) {
+  Tensor C = tcon_raw(c, A, la, false, B, lb, false);
+  C.node.reset();
+  if (needs(A) || needs(B)) {
+    auto nA = A.node, nB = B.node;
+    Tensor Ak = A, Bk = B;  // values kept for the VJP
+    Ak.node.reset();
+    Bk.node.reset();
+    const int na = A.nd(), nb = B.nd();
+    int nout = C.nd();
+    // labels for the VJPs
+    std::vector<int> lc_a(nout), lb_a(nb), la_b(na), lc_b(nout);
+    const int BIG = 100000;
+    for (int kk = 0; kk < nout; ++kk) {
+      int lab = -(kk + 1);
+      auto ia = std::find(la.begin(), la.end(), lab);
+      if (ia != la.end()) {
+        int p = (int)(ia - la.begin());
+        lc_a[kk] = -(p + 1);
+        lc_b[kk] = BIG + kk;
+        la_b[p] = BIG + kk;
+      } else {
+        int q = (int)(std::find(lb.begin(), lb.end(), lab) - lb.begin());
+        lc_a[kk] = BIG + kk;
+        lb_a[q] = BIG + kk;
+        lc_b[kk] = -(q + 1);
+      }
+    }
+    for (int p = 0; p < na; ++p)
+      if (la[p] > 0) {
+        int q = (int)(std::find(lb.begin(), lb.end(), la[p]) - lb.begin());
+        lb_a[q] = -(p + 1);
+        la_b[p] = -(q + 1);
+      }
+    bool gA = needs(A), gB = needs(B);
+    C.node = make_node([=](Ctx& c, Node& n) {
+      if (gA) acc(c, nA, tcon_raw(c, n.grad, lc_a, false, Bk, lb_a, true));
+      if (gB) acc(c, nB, tcon_raw(c, Ak, la_b, true, n.grad, lc_b, false));
+    });
+  }
+  return C;
+}
+
+Tensor matmul(Ctx& c, const Tensor& A, const Tensor& B) { return tcon(c, A, {-1, 1}, B, {1, -2}); }
+
+Tensor adjoint_matmul(Ctx& c, const Tensor& A, const Tensor& B) {
+  return tcon(c, conj(c, A), {1, -1}, B, {1, -2});
+}
+
+// ---------------------------------------------------------------------------
+// renormalize / scalings
+// ---------------------------------------------------------------------------
+Tensor renormalize(Ctx& c, const Tensor& t) {
+  auto sc = std::make_shared<Buf>();
+  sc->bytes = 32;
+  sc->p = c.pool->alloc(32);
+  sc->pool = c.pool.get();
+  double* m = (double*)sc->p;
+  int64_t* idx = (int64_t*)((char*)sc->p + 8);
+  k::maxabs(c, t.dt, t.numel(), t.data(), m, idx);
+  Tensor r = c.empty(t.dt, t.dims);
+  k::scale_by_dev(c, t.dt, t.numel(), t.data(), r.data(), m, 0);
+  if (needs(t)) {
+    auto in = t.node;
+    Tensor x = t;
+    x.node.reset();
+    r.node = make_node([in, x, sc](Ctx& c, Node& n) {
+      double* mv = (double*)sc->p;
+      int64_t* iv = (int64_t*)((char*)sc->p + 8);
+      double* dotv = (double*)((char*)sc->p + 16);
+      k::re_dot(c, x.dt, x.numel(), n.grad.data(), x.data(), dotv);
+      Tensor g = c.empty(x.dt, x.dims);
+      k::renorm_back(c, x.dt, x.numel(), x.data(), n.grad.data(), mv, iv, dotv, g.data());
+      acc(c, in, g);
+    });
+  }
+  return r;
+}
+
+Tensor scale_vec(Ctx& c, const Tensor& X, const Tensor& s, bool rowwise) {
+  IPEPS_CHECK(X.nd() == 2 && s.dt == DType::F64, "scale_vec: matrix and FP64 vector expected");
+  IPEPS_CHECK(s.numel() == (rowwise ? X.dims[0] : X.dims[1]), "scale_vec: length mismatch");
+  Tensor Y = c.empty(X.dt, X.dims);
+  k::scale_cols(c, X.dt, X.dims[0], X.dims[1], X.data(), s.ptr<double>(), Y.data(), rowwise);
+  if (needs(X) || needs(s)) {
+    auto nX = X.node, nS = s.node;
+    Tensor Xk = X, sk = s;
+    Xk.node.reset();
+    sk.node.reset();
+    Y.node = make_node([=](Ctx& c, Node& n) {
+      if (nX) {
+        Tensor g = c.empty(Xk.dt, Xk.dims);
+        k::scale_cols(c, Xk.dt, Xk.dims[0], Xk.dims[1], n.grad.data(), sk.ptr<double>(), g.data(), rowwise);
+        acc(c, nX, g);
+      }
+      if (nS) {
+        Tensor g = c.empty(DType::F64, sk.dims);
+        k::col_re_dot(c, Xk.dt, Xk.dims[0], Xk.dims[1], Xk.data(), n.grad.data(), g.ptr<double>(), rowwise);
+        acc(c, nS, g);
+      }
+    });
+  }
+  return Y;
+}
+
+Tensor inv_sqrt(Ctx& c, const Tensor& s) {
+  IPEPS_CHECK(s.dt == DType::F64, "inv_sqrt: FP64 vector expected");
+  Tensor r = c.empty(DType::F64, s.dims);
+  k::inv_sqrt(c, s.numel(), s.ptr<double>(), r.ptr<double>());
+  if (needs(s)) {
+    auto in = s.node;
+    Tensor sk = s;
+    sk.node.reset();
+    r.node = make_node([in, sk](Ctx& c, Node& n) {
+      Tensor g = c.zeros(DType::F64, sk.dims);
+      k::inv_sqrt_back(c, sk.numel(), sk.ptr<double>(), n.grad.ptr<double>(), g.ptr<double>());
+      acc(c, in, g);
+    });
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// rocSOLVER wrappers
+// ---------------------------------------------------------------------------
+static void solver_check(rocblas_status s, const char* what) {
+  if (s != rocblas_status_success)
+    throw Error(IPEPS_ERR_SOLVER, std::string(what) + " failed with rocblas_status " + std::to_string((int)s));
+}
+
+// Full SVD of a square/rectangular F64/C128 matrix: A = U diag(S) Vh, returns V = Vh^H.
+// left: rocblas_svect_all or _singular.
+static void gesvd(Ctx& c, const Tensor& Ain, Tensor& U, Tensor& S, Tensor& V, bool thin) {
+  IPEPS_CHECK(Ain.nd() == 2, "gesvd: matrix expected");
+  int64_t m = Ain.dims[0], n = Ain.dims[1];
+  int64_t mn = std::min(m, n);
+  Tensor A = clone(c, Ain);
+  rocblas_handle h = (rocblas_handle)c.solver;
+  int64_t ucols = thin ? mn : m;
+  U = c.empty(Ain.dt, {m, ucols});
+  S = c.empty(DType::F64, {mn});
+  Tensor Vh = c.empty(Ain.dt, {n, n});
+  Tensor E = c.empty(DType::F64, {std::max<int64_t>(1, mn)});
+  Tensor info = c.empty(DType::F64, {1});
+  rocblas_svect ls = thin ? rocblas_svect_singular : rocblas_svect_all;
+  if (Ain.dt == DType::F64) {
+    solver_check(rocsolver_dgesvd(h, ls, rocblas_svect_all, (rocblas_int)m, (rocblas_int)n, A.ptr<double>(),
+                                  (rocblas_int)m, S.ptr<double>(), U.ptr<double>(), (rocblas_int)m,
+                                  Vh.ptr<double>(), (rocblas_int)n, E.ptr<double>(), rocblas_outofplace,
+                                  (rocblas_int*)info.data()),
+                 "rocsolver_dgesvd");
+  } else if (Ain.dt == DType::C128) {
+    solver_check(rocsolver_zgesvd(h, ls, rocblas_svect_all, (rocblas_int)m, (rocblas_int)n,
+                                  (rocblas_double_complex*)A.data(), (rocblas_int)m, S.ptr<double>(),
+                                  (rocblas_double_complex*)U.data(), (rocblas_int)m,
+                                  (rocblas_double_complex*)Vh.data(), (rocblas_int)n, E.ptr<double>(),
+                                  rocblas_outofplace, (rocblas_int*)info.data()),
+                 "rocsolver_zgesvd");
+  } else {
+    throw Error(IPEPS_ERR_ARG, "gesvd: F64 or C128 only");
+  }
+  int hinfo = 0;
+  HIP_CHECK(hipMemcpyAsync(&hinfo, info.data(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  if (hinfo != 0) throw Error(IPEPS_ERR_SOLVER, "gesvd did not converge, info=" + std::to_string(hinfo));
+  // V = Vh^H, keep the first mn columns when thin
+  Tensor Vfull = permute_raw(c, Vh, {1, 0}, true);
+  if (thin && mn < n) {
+    V = c.empty(Ain.dt, {n, mn});
+    HIP_CHECK(hipMemcpyAsync(V.data(), Vfull.data(), V.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  } else {
+    V = Vfull;
+  }
+}
+
+void svd_thin_raw(Ctx& c, const Tensor& X, Tensor& W, Tensor& s, Tensor& Z) { gesvd(c, X, W, s, Z, true); }
+void svd_full_raw(Ctx& c, const Tensor& X, Tensor& U, Tensor& s, Tensor& V) { gesvd(c, X, U, s, V, false); }
+
+Tensor orth(Ctx& c, const Tensor& Yin) {
+  IPEPS_CHECK(Yin.nd() == 2 && Yin.dims[0] >= Yin.dims[1], "orth: tall matrix expected");
+  int64_t m = Yin.dims[0], n = Yin.dims[1];
+  Tensor Y = clone(c, Yin);
+  Tensor tau = c.empty(Yin.dt, {n});
+  rocblas_handle h = (rocblas_handle)c.solver;
+  if (Y.dt == DType::F64) {
+    solver_check(rocsolver_dgeqrf(h, (rocblas_int)m, (rocblas_int)n, Y.ptr<double>(), (rocblas_int)m, tau.ptr<double>()),
+                 "rocsolver_dgeqrf");
+    solver_check(rocsolver_dorgqr(h, (rocblas_int)m, (rocblas_int)n, (rocblas_int)n, Y.ptr<double>(), (rocblas_int)m,
+                                  tau.ptr<double>()),
+                 "rocsolver_dorgqr");
+  } else if (Y.dt == DType::C128) {
+    solver_check(rocsolver_zgeqrf(h, (rocblas_int)m, (rocblas_int)n, (rocblas_double_complex*)Y.data(), (rocblas_int)m,
+                                  (rocblas_double_complex*)tau.data()),
+                 "rocsolver_zgeqrf");
+    solver_check(rocsolver_zungqr(h, (rocblas_int)m, (rocblas_int)n, (rocblas_int)n, (rocblas_double_complex*)Y.data(),
+                                  (rocblas_int)m, (rocblas_double_complex*)tau.data()),
+                 "rocsolver_zungqr");
+  } else {
+    throw Error(IPEPS_ERR_ARG, "orth: F64 or C128 only");
+  }
+  return Y;
+}
+
+// ---------------------------------------------------------------------------
+// truncated SVD of M = R1 R2 with the reference cutoff rule
+// ---------------------------------------------------------------------------
+static int64_t cutoff_rule(const std::vector<double>& s, int64_t chi) {
+  // src/utility/wrapper.jl:37-55
+  double smax = 0.0;
+  for (double v : s) smax = std::max(smax, v);
+  int64_t above = 0;
+  for (double v : s) above += (v / smax > 1e-12);
+  int64_t n = std::min<int64_t>(chi, above);
+  if ((int64_t)s.size() > n && n >= 1 && std::fabs(s[n] - s[n - 1]) < 1e-10) n += 1;
+  return n;
+}
+
+static Tensor first_cols(Ctx& c, const Tensor& X, int64_t n) {
+  if (X.dims[1] == n) return X;
+  Tensor r = c.empty(X.dt, {X.dims[0], n});
+  HIP_CHECK(hipMemcpyAsync(r.data(), X.data(), r.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  return r;
+}
+
+static Tensor first_n(Ctx& c, const Tensor& s, int64_t n) {
+  Tensor r = c.empty(s.dt, {n});
+  HIP_CHECK(hipMemcpyAsync(r.data(), s.data(), r.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  return r;
+}
+
+static DType solve_dt(DType dt) { return dt == DType::F32 ? DType::F64 : dt; }
+
+SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, const SvdOpts& o) {
+  IPEPS_CHECK(R1.nd() == 2 && R2.nd() == 2 && R1.dims[1] == R2.dims[0], "svd_trunc: R1 R2 shape mismatch");
+  IPEPS_CHECK(R1.dims[0] == R2.dims[1], "svd_trunc: half-system matrix must be square");
+  const int64_t N = R1.dims[0];
+  const DType dt = R1.dt, sdt = solve_dt(dt);
+  SVDResult out;
+  const bool full = (o.mode == 0) || (N <= o.full_below) || (N <= (std::min<int64_t>(chi, N) + o.oversample));
+  Tensor Uf, Sf, Vf;  // full (mode 0) or the k computed triplets (mode 1)
+  if (full) {
+    Tensor M = matmul_raw(c, R1, false, false, R2, false, false);
+    Tensor Ms = to_dtype(c, M, sdt);
+    gesvd(c, Ms, Uf, Sf, Vf, false);
+  } else {
+    const int64_t ntar = std::min<int64_t>(chi, N);
+    const int64_t kk = std::min<int64_t>(N, ntar + o.oversample);
+    Tensor Om = c.empty(dt, {N, kk});
+    k::randn(c, dt, N * kk, o.seed ^ (c.rng_counter += 0x9e3779b97f4a7c15ull), Om.data());
+    auto apply_M = [&](const Tensor& X) {  // M X = R1 (R2 X)
+      return matmul_raw(c, R1, false, false, matmul_raw(c, R2, false, false, X, false, false), false, false);
+    };
+    auto apply_MH = [&](const Tensor& X) {  // M^H X = R2^H (R1^H X)
+      return matmul_raw(c, R2, true, true, matmul_raw(c, R1, true, true, X, false, false), false, false);
+    };
+    Tensor Q = orth(c, to_dtype(c, apply_M(Om), sdt));
+    for (int q = 0; q < o.power_iters; ++q) {
+      Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt));
+      Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt));
+    }
+    Tensor Bt = to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt);  // N x k = M^H Q
+    Tensor W, s, X;
+    gesvd(c, Bt, W, s, X, true);  // Bt = W diag(s) X^H  ->  M ~ (Q X) diag(s) W^H
+    Uf = matmul_raw(c, Q, false, false, X, false, false);
+    Sf = s;
+    Vf = W;
+  }
+  const int64_t ns = Sf.numel();
+  out.s_host.resize(ns);
+  HIP_CHECK(hipMemcpyAsync(out.s_host.data(), Sf.data(), ns * 8, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  const int64_t n = cutoff_rule(out.s_host, std::min<int64_t>(chi, N));
+  IPEPS_CHECK(n >= 1 && n <= ns, "svd_trunc: empty truncation");
+  out.n = n;
+  Tensor Un = first_cols(c, Uf, n), Vn = first_cols(c, Vf, n), Sn = first_n(c, Sf, n);
+  out.U = to_dtype(c, Un, dt);
+  out.V = to_dtype(c, Vn, dt);
+  out.S = Sn;
+  if (needs(R1) || needs(R2)) {
+    auto base = make_node(nullptr);
+    base->slots.resize(3);
+    std::weak_ptr<Node> wb = base;
+    auto mk_slot = [&](int i, Tensor& t) {
+      t.node = make_node([wb, i](Ctx& c, Node& n) {
+        auto b = wb.lock();
+        if (b) b->slots[i] = n.grad;
+      });
+    };
+    auto n1 = R1.node, n2 = R2.node;
+    Tensor R1k = R1, R2k = R2;
+    R1k.node.reset();
+    R2k.node.reset();
+    const double eta = o.eta;
+    if (full) {
+      base->back = [=](Ctx& c, Node& b) {
+        // reference svd_back on the square M (src/utility/autodiff.jl:41-75)
+        const int64_t NN = Uf.dims[0];
+        Tensor dU = b.slots[0].defined() ? to_dtype(c, b.slots[0], sdt) : Tensor();
+        Tensor dV = b.slots[2].defined() ? to_dtype(c, b.slots[2], sdt) : Tensor();
+        Tensor UdU, VdV;
+        if (dU.defined()) UdU = matmul_raw(c, Uf, true, true, dU, false, false);
+        if (dV.defined()) VdV = matmul_raw(c, Vf, true, true, dV, false, false);
+        Tensor G = c.empty(sdt, {NN, NN});
+        k::svd_gamma(c, sdt, NN, n, Sf.ptr<double>(), UdU.defined() ? UdU.data() : nullptr,
+                     VdV.defined() ? VdV.data() : nullptr, b.slots[1].defined() ? b.slots[1].ptr<double>() : nullptr,
+                     eta, G.data());
+        Tensor Mbar = matmul_raw(c, matmul_raw(c, Uf, false, false, G, false, false), false, false, Vf, true, true);
+        Tensor Mb = to_dtype(c, Mbar, dt);
+        if (n1) acc(c, n1, matmul_raw(c, Mb, false, false, R2k, true, true));
+        if (n2) acc(c, n2, matmul_raw(c, R1k, true, true, Mb, false, false));
+      };
+    } else {
+      Tensor Ut = Un, Vt = Vn, St = Sn;
+      base->back = [=](Ctx& c, Node& b) {
+        // thin svd_back (src/utility/autodiff.jl:41-75 with the non-square terms),
+        // M-bar = X1 V^H + U X2^H propagated to R1, R2 without forming M.
+        Tensor dU = b.slots[0].defined() ? to_dtype(c, b.slots[0], sdt) : c.zeros(sdt, Ut.dims);
+        Tensor dV = b.slots[2].defined() ? to_dtype(c, b.slots[2], sdt) : c.zeros(sdt, Vt.dims);
+        Tensor UdU = matmul_raw(c, Ut, true, true, dU, false, false);  // n x n
+        Tensor VdV = matmul_raw(c, Vt, true, true, dV, false, false);
+        Tensor G = c.empty(sdt, {n, n});
+        Tensor Sinv = c.empty(DType::F64, {n});
+        k::svd_gamma_thin(c, sdt, n, St.ptr<double>(), UdU.data(), VdV.data(),
+                          b.slots[1].defined() ? b.slots[1].ptr<double>() : nullptr, eta, G.data(), Sinv.ptr<double>());
+        // X1 = U G + (dU - U UdU) diag(Sinv)
+        Tensor X1 = matmul_raw(c, Ut, false, false, G, false, false);
+        Tensor T1 = matmul_raw(c, Ut, false, false, UdU, false, false);
+        k::axpy(c, sdt, T1.numel(), -1.0, T1.data(), dU.data());  // dU - U UdU (dU is a private copy)
+        Tensor T1s = c.empty(sdt, dU.dims);
+        k::scale_cols(c, sdt, dU.dims[0], dU.dims[1], dU.data(), Sinv.ptr<double>(), T1s.data(), false);
+        k::axpy(c, sdt, X1.numel(), 1.0, T1s.data(), X1.data());
+        // X2 = (dV - V VdV) diag(Sinv)
+        Tensor T2 = matmul_raw(c, Vt, false, false, VdV, false, false);
+        k::axpy(c, sdt, T2.numel(), -1.0, T2.data(), dV.data());
+        Tensor X2 = c.empty(sdt, dV.dims);
+        k::scale_cols(c, sdt, dV.dims[0], dV.dims[1], dV.data(), Sinv.ptr<double>(), X2.data(), false);
+        Tensor X1d = to_dtype(c, X1, dt), X2d = to_dtype(c, X2, dt);
+        Tensor Ud = to_dtype(c, Ut, dt), Vd = to_dtype(c, Vt, dt);
+        if (n1) {
+          // R1bar = Mbar R2^H = X1 (R2 V)^H + U (R2 X2)^H
+          Tensor W1 = matmul_raw(c, R2k, false, false, Vd, false, false);
+          Tensor W2 = matmul_raw(c, R2k, false, false, X2d, false, false);
+          Tensor g = matmul_raw(c, X1d, false, false, W1, true, true);
+          Tensor g2 = matmul_raw(c, Ud, false, false, W2, true, true);
+          k::axpy(c, dt, g.numel(), 1.0, g2.data(), g.data());
+          acc(c, n1, g);
+        }
+        if (n2) {
+          // R2bar = R1^H Mbar = (R1^H X1) V^H + (R1^H U) X2^H
+          Tensor Y1 = matmul_raw(c, R1k, true, true, X1d, false, false);
+          Tensor Y2 = matmul_raw(c, R1k, true, true, Ud, false, false);
+          Tensor g = matmul_raw(c, Y1, false, false, Vd, true, true);
+          Tensor g2 = matmul_raw(c, Y2, false, false, X2d, true, true);
+          k::axpy(c, dt, g.numel(), 1.0, g2.data(), g.data());
+          acc(c, n2, g);
+        }
+      };
+    }
+    mk_slot(0, out.U);
+    mk_slot(1, out.S);
+    mk_slot(2, out.V);
+    // keep the base node alive as long as the slot nodes are on the tape
+    (void)base;
+  }
+  return out;
+}
+
+}  // namespace op
+}  // namespace ipeps

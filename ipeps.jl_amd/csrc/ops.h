@@ -1,0 +1,118 @@
+// Differentiable tensor operations on the device and the reverse-mode tape.
+//
+// Every op runs its forward kernels on the context stream.  When the input
+// carries a tape node and a tape is recording, the op records a closure that
+// maps the output cotangent to input cotangents (hand-written VJPs).  The
+// cotangent convention is the one of Zygote/ChainRules: for a real loss L of
+// complex z the gradient is dL/dRe z + i dL/dIm z, so the VJP of a linear map
+// is its adjoint.
+#pragma once
+
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace ipeps {
+
+struct Node {
+  Tensor grad;                                  // accumulated cotangent of the output
+  std::vector<Tensor> slots;                    // per-output cotangents of multi-output ops
+  std::function<void(Ctx&, Node&)> back;        // propagate to inputs (null for leaves)
+  bool leaf = false;
+};
+
+struct Tape {
+  bool active = false;
+  std::vector<std::shared_ptr<Node>> order;
+  void clear() { order.clear(); }
+};
+
+// Current tape of this thread (null when not recording).
+Tape*& current_tape();
+
+struct TapeScope {
+  Tape* prev;
+  explicit TapeScope(Tape* t) : prev(current_tape()) { current_tape() = t; }
+  ~TapeScope() { current_tape() = prev; }
+};
+
+namespace op {
+
+// Precision knobs of the contraction engine.
+struct Prec {
+  k::Mixed mixed = k::Mixed::BF16X3;
+};
+Prec& prec();
+
+bool recording();
+// Mark t as a differentiable leaf on the current tape.
+Tensor leaf(Tensor t);
+// Reverse sweep over the tape (the loss node's grad must be seeded).
+void backward(Ctx& c, Tape& t);
+// Add g to n's cotangent.
+void acc(Ctx& c, const std::shared_ptr<Node>& n, const Tensor& g);
+// Fresh node on the current tape.
+std::shared_ptr<Node> make_node(std::function<void(Ctx&, Node&)> back);
+
+// ---- raw (non-recording) helpers -----------------------------------------
+Tensor clone(Ctx& c, const Tensor& t);
+Tensor permute_raw(Ctx& c, const Tensor& t, const std::vector<int>& perm, bool conj = false);
+Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA, const Tensor& B,
+                const std::vector<int>& lb, bool conjB);
+Tensor to_dtype(Ctx& c, const Tensor& t, DType dt);
+// C = op(A) op(B) for matrices (column-major), raw.
+Tensor matmul_raw(Ctx& c, const Tensor& A, bool tA, bool cA, const Tensor& B, bool tB, bool cB);
+
+// ---- differentiable ops -------------------------------------------------------
+// ncon-style contraction: positive labels are summed, negative labels are the
+// output legs in the order -1, -2, ...  (src/utility/wrapper.jl:15-23)
+Tensor tcon(Ctx& c, const Tensor& A, const std::vector<int>& la, const Tensor& B, const std::vector<int>& lb);
+Tensor reshape(Ctx& c, const Tensor& t, const std::vector<int64_t>& dims);
+Tensor permute(Ctx& c, const Tensor& t, const std::vector<int>& perm);
+Tensor conj(Ctx& c, const Tensor& t);
+// X ./ maximum(abs.(X)) with the gradient of maximum at the first maximum
+// in column-major order (src/utility/wrapper.jl:57).
+Tensor renormalize(Ctx& c, const Tensor& t);
+// X diag(s) (rowwise=false) or diag(s) X (rowwise=true), s a real FP64 vector.
+Tensor scale_vec(Ctx& c, const Tensor& X, const Tensor& s, bool rowwise);
+// s .^ (-1/2) for a real FP64 vector.
+Tensor inv_sqrt(Ctx& c, const Tensor& s);
+Tensor matmul(Ctx& c, const Tensor& A, const Tensor& B);       // A * B
+Tensor adjoint_matmul(Ctx& c, const Tensor& A, const Tensor& B);  // A^H * B
+Tensor cast(Ctx& c, const Tensor& t, DType dt);                   // differentiable F64 <-> F32
+
+struct SVDResult {
+  Tensor U, S, V;   // U: N x n, S: n (FP64), V: N x n
+  std::vector<double> s_host;  // all computed singular values (descending)
+  int64_t n = 0;
+};
+
+// Options of the truncated SVD of the half-system matrix.
+struct SvdOpts {
+  int mode = 0;          // 0 = full SVD of M = R1 R2 (reference semantics), 1 = randomized on (R1, R2)
+  int oversample = 16;   // k = n_target + oversample
+  int power_iters = 1;   // q
+  int full_below = 0;    // use the full SVD when N <= full_below (randomized mode)
+  uint64_t seed = 1234;
+  double eta = 1e-12;    // Lorentzian broadening of the adjoint (src/utility/autodiff.jl:41)
+};
+
+// Truncated SVD with the reference cutoff rule (src/utility/wrapper.jl:25-55):
+// n = min(chi, #{s/smax > 1e-12}); if |s[n+1]-s[n]| < 1e-10 then n += 1.
+// mode 0: M is formed, full SVD, adjoint = full svd_back (autodiff.jl:41-75).
+// mode 1: randomized range finder on M = R1 R2, adjoint = thin svd_back
+//         propagated to R1 and R2 without forming M.
+SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, const SvdOpts& o);
+
+// Orthonormal basis of the columns of Y (N x k, F64 or C128): Householder QR.
+Tensor orth(Ctx& c, const Tensor& Y);
+// Full SVD X = U diag(s) V^H (U m x m, V n x n, V returned, not V^H).
+void svd_full_raw(Ctx& c, const Tensor& X, Tensor& U, Tensor& s, Tensor& V);
+// Thin SVD of a tall matrix X (m x n, m >= n): X = W diag(s) Z^H.
+void svd_thin_raw(Ctx& c, const Tensor& X, Tensor& W, Tensor& s, Tensor& Z);
+
+}  // namespace op
+}  // namespace ipeps

@@ -1,0 +1,128 @@
+// Context, caching allocator and host/device copies.
+#include <rocblas/rocblas.h>
+
+#include <sstream>
+
+#include "common.h"
+
+namespace ipeps {
+
+Pool::~Pool() { trim(); }
+
+void* Pool::alloc(size_t bytes) {
+  size_t b = round(bytes);
+  auto it = free_.find(b);
+  void* p = nullptr;
+  if (it != free_.end()) {
+    p = it->second;
+    free_.erase(it);
+    cached_ -= b;
+  } else {
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      // release the cache and retry once
+      (void)hipGetLastError();
+      trim();
+      HIP_CHECK(hipMalloc(&p, b));
+    }
+  }
+  live_ += b;
+  if (live_ > peak_) peak_ = live_;
+  return p;
+}
+
+void Pool::release(void* p, size_t bytes) {
+  size_t b = round(bytes);
+  free_.emplace(b, p);
+  live_ -= b;
+  cached_ += b;
+}
+
+void Pool::trim() {
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+  cached_ = 0;
+}
+
+Tensor Ctx::empty(DType dt, const std::vector<int64_t>& dims) {
+  Tensor t;
+  t.dt = dt;
+  t.dims = dims;
+  size_t bytes = t.bytes();
+  auto b = std::make_shared<Buf>();
+  b->bytes = bytes ? bytes : 1;
+  b->p = pool->alloc(b->bytes);
+  b->pool = pool.get();
+  t.buf = b;
+  return t;
+}
+
+Tensor Ctx::zeros(DType dt, const std::vector<int64_t>& dims) {
+  Tensor t = empty(dt, dims);
+  if (t.bytes()) HIP_CHECK(hipMemsetAsync(t.data(), 0, t.bytes(), stream));
+  return t;
+}
+
+Ctx* ctx_create(int device) {
+  int n = 0;
+  HIP_CHECK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) throw Error(IPEPS_ERR_ARG, "ctx_create: no such HIP device");
+  HIP_CHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  std::string arch = prop.gcnArchName;
+  if (arch.rfind("gfx950", 0) != 0)
+    throw Error(IPEPS_ERR_STATE, "ctx_create: this library is built for gfx950 only, device is " + arch);
+  auto* c = new Ctx();
+  c->device = device;
+  HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->pool.reset(new Pool(device));
+  rocblas_handle h;
+  if (rocblas_create_handle(&h) != rocblas_status_success) {
+    delete c;
+    throw Error(IPEPS_ERR_SOLVER, "rocblas_create_handle failed");
+  }
+  rocblas_set_stream(h, c->stream);
+  c->solver = h;
+  return c;
+}
+
+void ctx_destroy(Ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->solver) rocblas_destroy_handle((rocblas_handle)c->solver);
+  c->pool.reset();
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+void to_device(Ctx& c, Tensor& t, const void* host) {
+  if (t.bytes()) HIP_CHECK(hipMemcpyAsync(t.data(), host, t.bytes(), hipMemcpyHostToDevice, c.stream));
+  c.sync();
+}
+
+void to_host(Ctx& c, const Tensor& t, void* host) {
+  if (t.bytes()) HIP_CHECK(hipMemcpyAsync(host, t.data(), t.bytes(), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+}
+
+Tensor reshape(const Tensor& t, const std::vector<int64_t>& dims) {
+  Tensor r = t;
+  r.node.reset();
+  int64_t n = 1;
+  for (auto d : dims) n *= d;
+  IPEPS_CHECK(n == t.numel(), "reshape: element count mismatch " + dims_str(t.dims) + " -> " + dims_str(dims));
+  r.dims = dims;
+  return r;
+}
+
+std::string dims_str(const std::vector<int64_t>& d) {
+  std::ostringstream s;
+  s << "(";
+  for (size_t i = 0; i < d.size(); ++i) s << (i ? "," : "") << d[i];
+  s << ")";
+  return s.str();
+}
+
+}  // namespace ipeps

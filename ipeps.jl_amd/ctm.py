@@ -1,0 +1,341 @@
+"""Host-side mirror of the reference CTMRG API over the HIP engine.
+
+Names, arguments and return values follow yitan1/iPEPS.jl so that code (and
+tests) written against the reference read the same:
+
+* ``CTMTensors(A, cfg)``             src/ctm/ctm_tensor.jl:63-69 (init_ctm1)
+* ``rg_step(ts, chi)``               src/ctm/ctmrg.jl:57-64
+* ``run_ctm(ts, conv_fun=...)``      src/ctm/ctmrg.jl:1-55
+* ``get_gs_energy(ts, H)``           src/graph/evaluation.jl:15-29
+* ``get_gs_norm(ts)``                src/graph/evaluation.jl:47-59
+* ``compute_gs_energy(A, H, cfg)``   src/graph/evaluation.jl:1-13
+* ``gs_energy_grad(A0, H, cfg)``     fg! of src/optimize/optim_gs.jl:53-91
+* ``tcon(xs, ind_xs)``               src/utility/wrapper.jl:15-23
+* ``svd1(A)`` / ``svd_back(...)``    src/utility/autodiff.jl:22-75
+
+Arrays are numpy arrays in Julia index order (X[i1, ...] is Julia
+X[i1+1, ...]); they are handed to the C ABI in column-major memory order.
+Everything runs on the GPU through libipeps_hip.so; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import C128, F64, MIXED, IpepsError, check, lib
+
+_DT = {"f64": F64, "c128": C128, "mixed": MIXED}
+
+
+def _dtype_code(dtype, A):
+    if dtype is None:
+        return C128 if np.iscomplexobj(A) else F64
+    if isinstance(dtype, str):
+        return _DT[dtype]
+    return int(dtype)
+
+
+def _host_np(code):
+    return np.complex128 if code == C128 else np.float64
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i64(seq):
+    return (C.c_int64 * len(seq))(*[int(x) for x in seq])
+
+
+class Context:
+    """A device context (stream, allocator, solver handle)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().ipeps_ctx_create(int(device), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().ipeps_ctx_destroy(self.h)
+            self.h = None
+
+    def sync(self):
+        check(lib().ipeps_ctx_sync(self.h))
+
+    def mem(self):
+        live, peak = C.c_int64(), C.c_int64()
+        check(lib().ipeps_ctx_mem(self.h, C.byref(live), C.byref(peak)))
+        return live.value, peak.value
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_ctx = None
+
+
+def default_context():
+    global _ctx
+    if _ctx is None:
+        _ctx = Context(0)
+    return _ctx
+
+
+def _hpair(H):
+    hs = [np.asfortranarray(np.asarray(h, dtype=np.complex128)) for h in H]
+    if len(hs) == 1:
+        hs = hs * 2
+    m = hs[0].shape[0]
+    for h in hs:
+        if h.shape != (m, m):
+            raise ValueError("H terms must be square and equal-sized")
+    return hs, m
+
+
+ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "canonical_edge",
+               "mixed", "energy_fp64")
+
+
+class CTMTensors:
+    """Device-resident plain environment (src/ctm/ctm_tensor.jl:3-18).
+
+    ``CTMTensors(A, cfg)`` runs init_ctm1 on the GPU.  ``dtype`` is "f64",
+    "c128" or "mixed" (FP32 storage, split-bf16 MFMA contractions, FP64
+    projector SVD and energy).  ``options`` are engine options (see
+    ``ENV_OPTIONS``).
+    """
+
+    def __init__(self, A, cfg=None, dtype=None, options=None, ctx=None):
+        self.ctx = ctx or default_context()
+        self.Params = dict(cfg or {})
+        A = np.asarray(A)
+        if A.ndim != 5:
+            raise ValueError("A must be (D, D, D, D, d)")
+        self.code = _dtype_code(dtype, A)
+        self._np = _host_np(self.code)
+        Af = np.asfortranarray(A, dtype=self._np)
+        h = C.c_void_p()
+        check(lib().ipeps_env_create(self.ctx.h, self.code, _i64(A.shape), _ptr(Af), C.byref(h)))
+        self.h = h
+        for k, v in (options or {}).items():
+            self.set_option(k, v)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().ipeps_env_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def set_option(self, key, value):
+        check(lib().ipeps_env_set_option(self.h, key.encode(), float(value)))
+
+    def tensor(self, which):
+        dims = (C.c_int64 * 5)()
+        nd = C.c_int()
+        check(lib().ipeps_env_tensor_dims(self.h, int(which), dims, C.byref(nd)))
+        shape = tuple(dims[i] for i in range(nd.value))
+        out = np.empty(shape, dtype=self._np, order="F")
+        check(lib().ipeps_env_get_tensor(self.h, int(which), _ptr(out)))
+        return out
+
+    def set_tensor(self, which, arr):
+        a = np.asfortranarray(arr, dtype=self._np)
+        check(lib().ipeps_env_set_tensor(self.h, int(which), a.ndim, _i64(a.shape), _ptr(a)))
+
+    @property
+    def Cs(self):
+        return [self.tensor(i) for i in range(4)]
+
+    @property
+    def Es(self):
+        return [self.tensor(4 + i) for i in range(4)]
+
+    @property
+    def A(self):
+        return self.tensor(8)
+
+    def set_A(self, A):
+        a = np.asfortranarray(A, dtype=self._np)
+        check(lib().ipeps_env_set_A(self.h, _ptr(a)))
+
+    def set_env(self, Cs, Es):
+        for i in range(4):
+            self.set_tensor(i, Cs[i])
+            self.set_tensor(4 + i, Es[i])
+
+
+def rg_step(ts, chi):
+    """One left/right/top/bottom step; returns (ts, left spectrum)."""
+    s = np.empty(int(chi) + 2, dtype=np.float64)
+    n = C.c_int64()
+    check(lib().ipeps_rg_step(ts.h, int(chi), _dp(s), s.size, C.byref(n)))
+    return ts, s[: n.value].copy()
+
+
+def run_ctm(ts, conv_fun="energy", H=None):
+    """run_ctm with the reference stopping rule.
+
+    conv_fun: "energy" (get_gs_energy with ``H``) or None (left spectrum).
+    Params keys chi, min_iter, max_iter, rg_tol as in the reference
+    (fallbacks 30, 4, 20, 1e-6).  Returns (ts, conv) where conv is the last
+    energy (complex) or spectrum; ts.last_iters / ts.last_diffs are set.
+    """
+    p = ts.Params
+    chi = int(p.get("chi", 30))
+    min_iter = int(p.get("min_iter", 4))
+    max_iter = int(p.get("max_iter", 20))
+    tol = float(p.get("rg_tol", 1e-6))
+    mode = 0 if conv_fun == "energy" else 1
+    if mode == 0:
+        (h1, h2), m = _hpair(H)
+        p1, p2 = _ptr(h1), _ptr(h2)
+    else:
+        p1 = p2 = None
+        m = 1
+    iters = C.c_int()
+    E = np.zeros(2)
+    diffs = np.zeros(max_iter + 2)
+    nd = C.c_int64()
+    check(lib().ipeps_run_ctm(ts.h, chi, min_iter, max_iter, tol, mode, p1, p2, m, C.byref(iters), _dp(E),
+                              _dp(diffs), diffs.size, C.byref(nd)))
+    ts.last_iters = iters.value
+    ts.last_diffs = diffs[: nd.value].copy()
+    if mode == 0:
+        return ts, complex(E[0], E[1])
+    return ts, None
+
+
+def get_gs_energy(ts, H):
+    """(E, Na) as complex numbers."""
+    (h1, h2), m = _hpair(H)
+    E = np.zeros(2)
+    N = np.zeros(2)
+    check(lib().ipeps_gs_energy(ts.h, _ptr(h1), _ptr(h2), m, _dp(E), _dp(N)))
+    return complex(E[0], E[1]), complex(N[0], N[1])
+
+
+def get_gs_norm(ts):
+    out = np.zeros(2)
+    check(lib().ipeps_gs_norm(ts.h, _dp(out)))
+    return complex(out[0], out[1])
+
+
+def compute_gs_energy(A, H, cfg, dtype=None, options=None, ctx=None):
+    """src/graph/evaluation.jl:1-13: converge from init_ctm1 and evaluate E."""
+    ts = CTMTensors(A, cfg, dtype=dtype, options=options, ctx=ctx)
+    ts, _ = run_ctm(ts, "energy", H)
+    E, N = get_gs_energy(ts, H)
+    return E, N, ts
+
+
+def step_adjoint(ts, chi, H):
+    """One rg_step + energy + the hand-written adjoint; returns (E, dRe(E)/dA)."""
+    (h1, h2), m = _hpair(H)
+    E = np.zeros(2)
+    A = ts.A
+    g = np.empty(A.shape, dtype=np.complex128 if ts.code == C128 else np.float64, order="F")
+    check(lib().ipeps_step_adjoint(ts.h, int(chi), _ptr(h1), _ptr(h2), m, _dp(E), _ptr(g)))
+    return complex(E[0], E[1]), g
+
+
+def gs_energy_grad(A0, H, cfg, dtype=None, options=None, ctx=None):
+    """Energy and gradient as optim_gs's fg! (src/optimize/optim_gs.jl:53-91).
+
+    Returns (E complex, g, (converging iterations, differentiated steps)).
+    """
+    ctx = ctx or default_context()
+    A0 = np.asarray(A0)
+    code = _dtype_code(dtype, A0)
+    Af = np.asfortranarray(A0, dtype=_host_np(code))
+    (h1, h2), m = _hpair(H)
+    chi = int(cfg.get("chi", 30))
+    min_iter = int(cfg.get("min_iter", 4))
+    max_iter = int(cfg.get("max_iter", 20))
+    ad_max_iter = int(cfg.get("ad_max_iter", 4))
+    tol = float(cfg.get("rg_tol", 1e-6))
+    opts = ";".join(f"{k}={float(v)}" for k, v in (options or {}).items()).encode()
+    E = np.zeros(2)
+    g = np.empty(A0.shape, dtype=_host_np(code), order="F")
+    it = (C.c_int * 2)()
+    check(lib().ipeps_gs_energy_grad(ctx.h, code, _i64(A0.shape), _ptr(Af), _ptr(h1), _ptr(h2), m, chi,
+                                     min_iter, max_iter, ad_max_iter, tol, opts, _dp(E), _ptr(g), it))
+    return complex(E[0], E[1]), g, (it[0], it[1])
+
+
+def tcon(xs, ind_xs, ctx=None):
+    """Pairwise ncon contraction on the device (src/utility/wrapper.jl:15-23)."""
+    if len(xs) != 2:
+        raise ValueError("tcon contracts two tensors")
+    ctx = ctx or default_context()
+    A, B = (np.asarray(x) for x in xs)
+    cplx = np.iscomplexobj(A) or np.iscomplexobj(B)
+    code = C128 if cplx else F64
+    dt = _host_np(code)
+    Af, Bf = np.asfortranarray(A, dtype=dt), np.asfortranarray(B, dtype=dt)
+    la = (C.c_int * max(1, A.ndim))(*ind_xs[0])
+    lb = (C.c_int * max(1, B.ndim))(*ind_xs[1])
+    nout = sum(1 for l in list(ind_xs[0]) + list(ind_xs[1]) if l < 0)
+    dimsC = (C.c_int64 * 16)()
+    ndC = C.c_int()
+    sizes = {}
+    for ind, X in ((ind_xs[0], A), (ind_xs[1], B)):
+        for l, d in zip(ind, X.shape):
+            if l < 0:
+                sizes[-l - 1] = d
+    shape = tuple(sizes[k] for k in range(nout))
+    out = np.empty(shape, dtype=dt, order="F")
+    check(lib().ipeps_tcon(ctx.h, code, A.ndim, _i64(A.shape), _ptr(Af), la, B.ndim, _i64(B.shape), _ptr(Bf), lb,
+                           dimsC, C.byref(ndC), _ptr(out), max(1, out.size)))
+    return out
+
+
+def svd1(A, ctx=None):
+    """Full SVD of a square matrix on the device: (U, S, V) with A = U S V^H."""
+    ctx = ctx or default_context()
+    A = np.asarray(A)
+    code = C128 if np.iscomplexobj(A) else F64
+    dt = _host_np(code)
+    m, n = A.shape
+    Af = np.asfortranarray(A, dtype=dt)
+    U = np.empty((m, m), dtype=dt, order="F")
+    V = np.empty((n, n), dtype=dt, order="F")
+    S = np.empty(min(m, n))
+    check(lib().ipeps_svd(ctx.h, code, m, n, _ptr(Af), _ptr(U), _dp(S), _ptr(V)))
+    return U, S, V
+
+
+def svd_back(U, S, V, dU=None, dS=None, dV=None, eta=1e-12, ctx=None):
+    """Reference SVD adjoint for a square matrix (src/utility/autodiff.jl:41-75)."""
+    ctx = ctx or default_context()
+    code = C128 if np.iscomplexobj(U) else F64
+    dt = _host_np(code)
+    N = U.shape[0]
+    n = next((x.shape[-1] if x.ndim == 2 else x.shape[0]) for x in (dU, dS, dV) if x is not None)
+    Uf, Vf = np.asfortranarray(U, dtype=dt), np.asfortranarray(V, dtype=dt)
+    Sf = np.ascontiguousarray(S, dtype=np.float64)
+    args = []
+    for x, isS in ((dU, False), (dS, True), (dV, False)):
+        if x is None:
+            args.append(None)
+        else:
+            args.append(np.ascontiguousarray(x, dtype=np.float64) if isS else np.asfortranarray(x, dtype=dt))
+    out = np.empty((N, N), dtype=dt, order="F")
+    check(lib().ipeps_svd_back(ctx.h, code, N, n, _ptr(Uf), _dp(Sf), _ptr(Vf),
+                               _ptr(args[0]) if args[0] is not None else None,
+                               _dp(args[1]) if args[1] is not None else None,
+                               _ptr(args[2]) if args[2] is not None else None, float(eta), _ptr(out)))
+    return out
+
+
+__all__ = ["Context", "CTMTensors", "rg_step", "run_ctm", "get_gs_energy", "get_gs_norm", "compute_gs_energy",
+           "step_adjoint", "gs_energy_grad", "tcon", "svd1", "svd_back", "default_context", "IpepsError",
+           "ENV_OPTIONS"]

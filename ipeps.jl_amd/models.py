@@ -1,0 +1,88 @@
+"""Model Hamiltonians and the deterministic loop-gas state (host-side inputs).
+
+* ``tout``        src/model/basis.jl:16-21 (site-1 index fastest)
+* ``ising``       src/model/basis.jl:23-29
+* ``heisenberg``  src/model/basis.jl:31-35
+* ``honeycomb``   src/model/honeycomb.jl:1-14
+* ``init_hb_gs``  src/model/lee_gs.jl:2-28 (D = 2 loop-gas state)
+"""
+import math
+
+import numpy as np
+
+sigmax = np.array([[0, 1], [1, 0]], dtype=np.float64)
+sigmay = np.array([[0, -1j], [1j, 0]], dtype=np.complex128)
+sigmaz = np.array([[1, 0], [0, -1]], dtype=np.float64)
+sI = np.eye(2)
+Sx, Sy, Sz = sigmax / 2, sigmay / 2, sigmaz / 2
+
+
+def tout(a, b):
+    # c[i1, i2, j1, j2] = a[i1, j1] b[i2, j2], reshaped column-major
+    c = np.einsum("ac,bd->abcd", a, b)
+    s = c.shape
+    return c.reshape(s[0] * s[1], s[2] * s[3], order="F")
+
+
+def ising(h=2.0):
+    H = -tout(sigmax, sigmax) - h * tout(sigmaz, sI) / 4 - h * tout(sI, sigmaz) / 4
+    return [H, H]
+
+
+def heisenberg(Jz=1.0):
+    H = Jz * tout(Sz, Sz) - tout(Sx, Sx) - tout(Sy, Sy)
+    return [H.astype(np.complex128), H.astype(np.complex128)]
+
+
+def honeycomb(Jx=1.0, Jy=1.0, Jz=1.0, dir="ZZ"):
+    def t4(a, b, c, d):
+        return tout(tout(a, b), tout(c, d))
+
+    if dir == "ZZ":
+        hv = Jx * t4(sI, sigmax, sigmax, sI) + (t4(sigmaz, sigmaz, sI, sI) + t4(sI, sI, sigmaz, sigmaz)) / 4
+        hh = Jy * t4(sI, sigmay, sigmay, sI) + (t4(sigmaz, sigmaz, sI, sI) + t4(sI, sI, sigmaz, sigmaz)) / 4
+    elif dir == "XX":
+        hv = Jy * t4(sI, sigmay, sigmay, sI) + (t4(sigmax, sigmax, sI, sI) + t4(sI, sI, sigmax, sigmax)) / 4
+        hh = Jz * t4(sI, sigmaz, sigmaz, sI) + (t4(sigmax, sigmax, sI, sI) + t4(sI, sI, sigmax, sigmax)) / 4
+    elif dir == "YY":
+        hv = Jz * t4(sI, sigmaz, sigmaz, sI) + (t4(sigmay, sigmay, sI, sI) + t4(sI, sI, sigmay, sigmay)) / 4
+        hh = Jx * t4(sI, sigmax, sigmax, sI) + (t4(sigmay, sigmay, sI, sI) + t4(sI, sI, sigmay, sigmay)) / 4
+    else:
+        raise ValueError(dir)
+    return [-np.real(hh), -np.real(hv)]
+
+
+def get_Q_op():
+    Q = np.zeros((2, 2, 2, 2, 2), dtype=np.complex128)
+    Q[0, 0, 0] = sI
+    Q[0, 1, 1] = sigmax
+    Q[1, 0, 1] = sigmay
+    Q[1, 1, 0] = sigmaz
+    return Q
+
+
+def init_hb_gs(D=2, dir="XX"):
+    if D != 2:
+        raise NotImplementedError("only the D=2 loop-gas state")
+    Q = get_Q_op()
+    u = 1 / math.sqrt(3)
+    s111 = np.array([1 + u, u + 1j * u]) / math.sqrt(2 + 2 * u)
+    T = np.einsum("abcdp,p->abcd", Q, s111)
+    if dir == "ZZ":
+        A = np.einsum("abxe,cdxf->abcdef", T, T)
+    elif dir == "XX":
+        A = np.einsum("xabe,xcdf->abcdef", T, T)
+    elif dir == "YY":
+        A = np.einsum("bxae,dxcf->abcdef", T, T)
+    else:
+        raise ValueError(dir)
+    return A.reshape(2, 2, 2, 2, 4, order="F")
+
+
+def seeded_tensor(shape, seed, unit_max=False, dtype=np.float64):
+    """numpy default_rng(seed).standard_normal in Fortran order."""
+    v = np.random.default_rng(seed).standard_normal(int(np.prod(shape)))
+    a = v.reshape(shape, order="F")
+    if unit_max:
+        a = a / np.abs(a).max()
+    return np.asfortranarray(a.astype(dtype))

@@ -1,0 +1,222 @@
+"""GPU parity tests: the HIP engine (through the C ABI) against the oracle and
+the reference's committed fixtures.  Tolerances are stated per test."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import ctmrg_oracle as O
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _npz(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def _t(x):
+    return torch.from_numpy(np.ascontiguousarray(x))
+
+
+# ---------------------------------------------------------------- primitives
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_tcon_matches_einsum(pkg, cplx):
+    rng = np.random.default_rng(7)
+
+    def r(*s):
+        a = rng.standard_normal(s)
+        if cplx:
+            a = a + 1j * rng.standard_normal(s)
+        return a
+
+    cases = [
+        ((5, 7), [-1, 1], (7, 3), [1, -2]),                     # plain matmul
+        ((7, 5), [1, -1], (7, 3), [1, -2]),                     # A^T
+        ((5, 7), [-2, 1], (3, 7), [-1, 1]),                     # output transposed
+        ((3, 4, 5, 2), [-1, 1, -3, 2], (2, 4, 6), [2, 1, -2]),  # needs permutes
+        ((2, 3, 4, 5, 2), [1, 2, -1, -3, 3], (2, 3, 4, 5, 2), [1, 2, -2, -4, 3]),  # init_ctm1 C1
+        ((65, 33), [-1, 1], (33, 129), [1, -2]),                # ragged tiles
+        ((4, 1, 3), [-1, -2, 1], (3, 1), [1, -3]),              # unit dims
+        ((6,), [1], (6, 5), [1, -1]),                           # vector
+    ]
+    for sa, la, sb, lb in cases:
+        A, B = r(*sa), r(*sb)
+        got = pkg.tcon([A, B], [la, lb])
+        ref = O.tcon([_t(A), _t(B)], [la, lb]).numpy()
+        assert got.shape == ref.shape, (sa, la, sb, lb)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(ref).max()))
+
+
+def test_large_gemm(pkg):
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((300, 257))
+    B = rng.standard_normal((257, 190))
+    got = pkg.tcon([A, B], [[-1, 1], [1, -2]])
+    np.testing.assert_allclose(got, A @ B, rtol=1e-12, atol=1e-11)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_svd_and_adjoint(pkg, cplx):
+    rng = np.random.default_rng(2)
+    N, n = 24, 9
+    M = rng.standard_normal((N, N)) + (1j * rng.standard_normal((N, N)) if cplx else 0)
+    U, S, V = pkg.svd1(M)
+    np.testing.assert_allclose((U * S) @ V.conj().T, M, atol=1e-12)
+    assert np.all(np.diff(S) <= 0)
+    dU = rng.standard_normal((N, n)) + (1j * rng.standard_normal((N, n)) if cplx else 0)
+    dV = rng.standard_normal((N, n)) + (1j * rng.standard_normal((N, n)) if cplx else 0)
+    dS = rng.standard_normal(n)
+    got = pkg.svd_back(U, S, V, dU, dS, dV)
+    Zp = np.zeros((N, N - n), dtype=dU.dtype)
+    ref = O.svd_back(_t(U), _t(S), _t(V), _t(np.hstack([dU, Zp])),
+                     _t(np.concatenate([dS, np.zeros(N - n)])), _t(np.hstack([dV, Zp]))).numpy()
+    np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+
+
+# ---------------------------------------------------------------- CTMRG forward
+
+@pytest.mark.parametrize("which", ["ising_real", "loopgas_complex"])
+def test_init_ctm1_parity(pkg, which):
+    if which == "ising_real":
+        A = O.seeded_tensor((2, 2, 2, 2, 2), 0).numpy()
+    else:
+        A = O.init_hb_gs(2, dir="XX").numpy()
+    ts = pkg.CTMTensors(A, {})
+    Cs, Es = O.init_ctm1(_t(A), _t(A).conj())
+    for got, ref in zip(ts.Cs + ts.Es, Cs + Es):
+        np.testing.assert_allclose(got, ref.numpy(), rtol=0, atol=1e-14)
+
+
+def test_rg_step_spectra_and_energy(pkg):
+    """Gauge-invariant step-by-step parity: left spectrum and energy, 1e-10."""
+    A = O.seeded_tensor((2, 2, 2, 2, 2), 11).numpy()
+    H = O.ising(2.0)
+    ts = pkg.CTMTensors(A, {})
+    ots = O.make_ctm(_t(A), {})
+    for step in range(4):
+        ts, s = pkg.rg_step(ts, 8)
+        ots, os_ = O.rg_step(ots, 8)
+        np.testing.assert_allclose(s, os_.numpy(), rtol=1e-10)
+        E, _ = pkg.get_gs_energy(ts, [h.numpy() for h in H])
+        Eo, _ = O.get_gs_energy(ots, H)
+        assert abs(E - complex(Eo)) < 1e-10, (step, E, Eo)
+
+
+def test_g4_fixed_environment(pkg):
+    """hb_test_D2_X32/basis.jld2: energy of the converged env with the shifted H is 0."""
+    e = _npz("env_loopgas_D2_X32.npz")
+    ts = pkg.CTMTensors(e["A"], {})
+    ts.set_env([e[f"C{i}"] for i in range(1, 5)], [e[f"E{i}"] for i in range(1, 5)])
+    E, _ = pkg.get_gs_energy(ts, [e["H1"], e["H2"]])
+    assert abs(E) < 1e-12
+    assert abs(pkg.get_gs_norm(ts) - 1.0) < 1e-10
+    H = O.honeycomb(1, 1, dir="XX")
+    Egs, _ = pkg.get_gs_energy(ts, [h.numpy() for h in H])
+    ots = O.CTMTensors(_t(e["A"]), _t(e["Ad"]), [_t(e[f"C{i}"]) for i in range(1, 5)],
+                       [_t(e[f"E{i}"]) for i in range(1, 5)])
+    Eo, _ = O.get_gs_energy(ots, H)
+    assert abs(Egs - complex(Eo)) < 1e-12
+
+
+def test_config1_ising_energy(pkg):
+    """Ising h=2, A (2,2,2,2,2) seed 0, chi=30, rg_tol 1e-6, min 4, max 30."""
+    A = O.seeded_tensor((2, 2, 2, 2, 2), 0)
+    H = O.ising(2.0)
+    cfg = dict(chi=30, min_iter=4, max_iter=30, rg_tol=1e-6)
+    E, _, ts = pkg.compute_gs_energy(A.numpy(), [h.numpy() for h in H], cfg)
+    Eo, _, _, it = O.compute_gs_energy(A, H, cfg)
+    assert ts.last_iters == it
+    assert abs(E - complex(Eo)) < 1e-10
+
+
+# ---------------------------------------------------------------- gradient
+
+def test_g1_g2_loopgas_gradient(pkg):
+    """energies[1] / gradnorms[1] of simulation/test_test_D2_X32/gs.jld2."""
+    ref = _npz("gs_loopgas_D2_X32.npz")
+    A0 = O.init_hb_gs(2, dir="XX").numpy()
+    H = [h.numpy() for h in O.honeycomb(1, 1, dir="XX")]
+    E, g, it = pkg.gs_energy_grad(A0, H, dict(O.DEFAULT_CFG))
+    assert abs(E.real - ref["energies"][0]) < 1e-10
+    assert abs(np.linalg.norm(g) - ref["gradnorms"][0]) / ref["gradnorms"][0] < 1e-8
+
+
+def test_g3_gradient_vector(pkg):
+    """Optimised A: xk.value and the Optim gradient vector g(x)."""
+    ref = _npz("gs_loopgas_D2_X32.npz")
+    H = [h.numpy() for h in O.honeycomb(1, 1, dir="XX")]
+    E, g, _ = pkg.gs_energy_grad(ref["A"], H, dict(O.DEFAULT_CFG))
+    assert abs(E.real - float(ref["value"])) < 1e-10
+    assert np.linalg.norm(g - ref["gx"]) < 1e-10
+
+
+def test_gradient_matches_oracle_real(pkg):
+    """Real Ising D=2 chi=12: gradient vector vs the oracle, 1e-9 relative."""
+    A0 = O.seeded_tensor((2, 2, 2, 2, 2), 5, unit_max=True)
+    H = O.ising(2.0)
+    cfg = dict(chi=12, min_iter=4, max_iter=12, ad_max_iter=3, rg_tol=1e-8)
+    E, g, _ = pkg.gs_energy_grad(A0.numpy(), [h.numpy() for h in H], cfg)
+    Eo, go, _ = O.gs_energy_grad(A0, H, cfg)
+    assert abs(E - complex(Eo)) < 1e-10
+    go = go.numpy()
+    assert np.linalg.norm(g - go) / np.linalg.norm(go) < 1e-9
+
+
+def test_gradient_central_fd(pkg):
+    """Central differences (eps=1e-6) of the engine's own energy on 8 entries."""
+    A0 = O.seeded_tensor((2, 2, 2, 2, 2), 9, unit_max=True)
+    H = O.ising(2.0)
+    Hn = [h.numpy() for h in H]
+    cfg = dict(chi=8, min_iter=4, max_iter=10, ad_max_iter=2, rg_tol=1e-9)
+    E, g, _ = pkg.gs_energy_grad(A0.numpy(), Hn, cfg)
+    # the differentiated function: env converged at x, then 2 steps with A = x + d
+    x = A0.numpy() / np.abs(A0.numpy()).max()
+    base = pkg.CTMTensors(x, cfg)
+    pkg.run_ctm(base, "energy", Hn)
+    Cs, Es = base.Cs, base.Es
+
+    def f(xx):
+        ts = pkg.CTMTensors(xx, cfg)
+        ts.set_env(Cs, Es)
+        for _ in range(2):
+            pkg.rg_step(ts, cfg["chi"])
+        return pkg.get_gs_energy(ts, Hn)[0].real
+
+    rng = np.random.default_rng(0)
+    idx = rng.choice(x.size, 8, replace=False)
+    eps = 1e-6
+    gf = g.reshape(-1, order="F")
+    for k in idx:
+        d = np.zeros(x.size)
+        d[k] = eps
+        d = d.reshape(x.shape, order="F")
+        fd = (f(x + d) - f(x - d)) / (2 * eps)
+        assert abs(fd - gf[k]) < 1e-6 * max(1.0, abs(fd)), (k, fd, gf[k])
+
+
+# ---------------------------------------------------------------- fast modes
+
+def test_randomized_projector_energy(pkg):
+    """Randomized range finder (svd_mode=1) vs the full SVD on a D=3 state."""
+    A = O.seeded_tensor((3, 3, 3, 3, 2), 4).numpy()
+    H = [h.numpy() for h in O.heisenberg(1.0)]
+    cfg = dict(chi=24, min_iter=10, max_iter=10, rg_tol=0.0)
+    E_full, _, _ = pkg.compute_gs_energy(A, H, cfg)
+    E_rand, _, _ = pkg.compute_gs_energy(A, H, cfg, options=dict(svd_mode=1, oversample=16, power_iters=2))
+    print("full", E_full, "rand", E_rand, "diff", abs(E_full - E_rand))
+    assert abs(E_full - E_rand) < 1e-8
+
+
+@pytest.mark.parametrize("mixed", [3, 6])
+def test_mixed_precision_energy(pkg, mixed):
+    A = O.seeded_tensor((3, 3, 3, 3, 2), 4).numpy()
+    H = [h.numpy() for h in O.heisenberg(1.0)]
+    cfg = dict(chi=24, min_iter=10, max_iter=10, rg_tol=0.0)
+    E64, _, _ = pkg.compute_gs_energy(A, H, cfg)
+    Em, _, _ = pkg.compute_gs_energy(A, H, cfg, dtype="mixed", options=dict(mixed=mixed))
+    print("mixed", mixed, "E64", E64, "Emixed", Em, "diff", abs(E64 - Em))
+    assert abs(E64 - Em) < (1e-4 if mixed == 3 else 1e-6)
