@@ -47,6 +47,14 @@ int ipeps_ctx_destroy(ipeps_ctx* ctx);
 int ipeps_ctx_sync(ipeps_ctx* ctx);
 /* Live / peak device bytes of the context allocator. */
 int ipeps_ctx_mem(ipeps_ctx* ctx, int64_t* live, int64_t* peak);
+/* Kernel timing: when on, every GEMM / permute / solver launch is bracketed
+ * by hipEvents on the context stream.  Engine instrumentation, no reference
+ * counterpart. */
+int ipeps_ctx_set_timing(ipeps_ctx* ctx, int on);
+/* Per-kernel totals since the last reset as JSON
+ * {"name": [launches, total_ms, algorithmic_flops, algorithmic_bytes], ...};
+ * reset != 0 clears them. */
+int ipeps_ctx_kernel_stats(ipeps_ctx* ctx, char* buf, int64_t cap, int reset);
 
 /* CTMTensors(A, cfg) -> init_ctm1 (src/ctm/ctm_tensor.jl:63-69, 97-130).
  * dims = (D, D, D, D, d); A is F64 or C128 host data (MIXED takes F64). */
@@ -108,6 +116,15 @@ int ipeps_gs_energy_grad(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const
 int ipeps_tcon(ipeps_ctx* ctx, int dtype, int ndA, const int64_t* dimsA, const void* A, const int* la, int ndB,
                const int64_t* dimsB, const void* B, const int* lb, int64_t* dimsC, int* ndC, void* C_host,
                int64_t C_cap);
+
+/* Matrix product on the device, the GEMM that OMEinsum hands to BLAS for a
+ * two-index contraction (wrapper.jl:23): C = alpha op(A) op(B) + beta C,
+ * column-major, op = X, X^T, conj(X) or X^H.  dtype IPEPS_F64 / IPEPS_C128
+ * (FP64 MFMA) or IPEPS_MIXED (float host data, split-bf16 MFMA with
+ * `mixed` = 1, 3 or 6 products, FP32 accumulation). */
+int ipeps_gemm(ipeps_ctx* ctx, int dtype, int mixed, int transA, int conjA, int transB, int conjB, int64_t M,
+               int64_t N, int64_t K, double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+               double beta, void* C, int64_t ldc);
 
 /* svd1 (src/utility/autodiff.jl:22-25): full SVD A = U diag(S) V^H of an
  * m x n matrix; U is m x m, V is n x n (V, not V^H), S has min(m,n). */

@@ -6,9 +6,9 @@ ABI, include/ipeps_hip.h); this package is the ctypes host side.
 """
 from ._lib import C128, F64, MIXED, LIB_PATH, IpepsError, lib
 from .ctm import (ENV_OPTIONS, Context, CTMTensors, compute_gs_energy, default_context, get_gs_energy,
-                  get_gs_norm, gs_energy_grad, rg_step, run_ctm, step_adjoint, svd1, svd_back, tcon)
+                  get_gs_norm, gemm, gs_energy_grad, rg_step, run_ctm, step_adjoint, svd1, svd_back, tcon)
 from . import models
 
 __all__ = ["C128", "F64", "MIXED", "LIB_PATH", "IpepsError", "lib", "ENV_OPTIONS", "Context", "CTMTensors",
            "compute_gs_energy", "default_context", "get_gs_energy", "get_gs_norm", "gs_energy_grad", "rg_step",
-           "run_ctm", "step_adjoint", "svd1", "svd_back", "tcon", "models"]
+           "run_ctm", "step_adjoint", "svd1", "svd_back", "tcon", "gemm", "models"]

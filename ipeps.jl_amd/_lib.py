@@ -20,6 +20,8 @@ SIGNATURES = {
     "ipeps_ctx_destroy": (C.c_int, [_P]),
     "ipeps_ctx_sync": (C.c_int, [_P]),
     "ipeps_ctx_mem": (C.c_int, [_P, _I64P, _I64P]),
+    "ipeps_ctx_set_timing": (C.c_int, [_P, C.c_int]),
+    "ipeps_ctx_kernel_stats": (C.c_int, [_P, C.c_char_p, C.c_int64, C.c_int]),
     "ipeps_env_create": (C.c_int, [_P, C.c_int, _I64P, _P, C.POINTER(_P)]),
     "ipeps_env_destroy": (C.c_int, [_P]),
     "ipeps_env_set_option": (C.c_int, [_P, C.c_char_p, C.c_double]),
@@ -37,6 +39,8 @@ SIGNATURES = {
                                        C.c_int, C.c_double, C.c_char_p, _DP, _P, _IP]),
     "ipeps_tcon": (C.c_int, [_P, C.c_int, C.c_int, _I64P, _P, _IP, C.c_int, _I64P, _P, _IP, _I64P, _IP, _P,
                              C.c_int64]),
+    "ipeps_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64,
+                             C.c_int64, C.c_double, _P, C.c_int64, _P, C.c_int64, C.c_double, _P, C.c_int64]),
     "ipeps_svd": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, _P, _P, _DP, _P]),
     "ipeps_svd_back": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, _P, _DP, _P, _P, _DP, _P, C.c_double, _P]),
 }

@@ -153,6 +153,36 @@ int ipeps_ctx_mem(ipeps_ctx* ctx, int64_t* live, int64_t* peak) {
   });
 }
 
+int ipeps_ctx_set_timing(ipeps_ctx* ctx, int on) {
+  return guard([&] {
+    IPEPS_CHECK(ctx, "null ctx");
+    ctx->c->resolve_timing();
+    ctx->c->timing = on != 0;
+  });
+}
+
+int ipeps_ctx_kernel_stats(ipeps_ctx* ctx, char* buf, int64_t cap, int reset) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && buf && cap > 0, "bad arguments");
+    Ctx& c = *ctx->c;
+    c.resolve_timing();
+    std::ostringstream s;
+    s.precision(17);
+    s << "{";
+    bool first = true;
+    for (auto& kv : c.kstats) {
+      s << (first ? "" : ", ") << "\"" << kv.first << "\": [" << kv.second.launches << ", " << kv.second.ms << ", "
+        << kv.second.flops << ", " << kv.second.bytes << "]";
+      first = false;
+    }
+    s << "}";
+    std::string str = s.str();
+    IPEPS_CHECK((int64_t)str.size() < cap, "kernel_stats: buffer too small");
+    std::memcpy(buf, str.c_str(), str.size() + 1);
+    if (reset) c.kstats.clear();
+  });
+}
+
 int ipeps_env_create(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const void* A, ipeps_env** out) {
   return guard([&] {
     IPEPS_CHECK(ctx && out && dims, "null argument");
@@ -330,6 +360,37 @@ int ipeps_tcon(ipeps_ctx* ctx, int dtype, int ndA, const int64_t* dimsA, const v
       IPEPS_CHECK(C_cap >= C.numel(), "tcon: output buffer too small");
       to_host(c, C, C_host);
     }
+  });
+}
+
+int ipeps_gemm(ipeps_ctx* ctx, int dtype, int mixed, int transA, int conjA, int transB, int conjB, int64_t M,
+               int64_t N, int64_t K, double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+               double beta, void* C, int64_t ldc) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && A && B && C && M >= 0 && N >= 0 && K >= 0, "bad arguments");
+    Ctx& c = *ctx->c;
+    DType dt = dev_dt(dtype);
+    const int64_t ar = transA ? K : M, ac = transA ? M : K, br = transB ? N : K, bc = transB ? K : N;
+    IPEPS_CHECK(lda >= std::max<int64_t>(1, ar) && ldb >= std::max<int64_t>(1, br) && ldc >= std::max<int64_t>(1, M),
+                "gemm: leading dimension too small");
+    Tensor tA = c.empty(dt, {lda, std::max<int64_t>(1, ac)}), tB = c.empty(dt, {ldb, std::max<int64_t>(1, bc)});
+    Tensor tC = c.empty(dt, {ldc, std::max<int64_t>(1, N)});
+    to_device(c, tA, A);
+    to_device(c, tB, B);
+    to_device(c, tC, C);
+    k::GemmArgs g;
+    g.dt = dt;
+    g.transA = transA; g.conjA = conjA; g.transB = transB; g.conjB = conjB;
+    g.M = M; g.N = N; g.K = K; g.alpha = alpha; g.beta = beta;
+    g.A = tA.data(); g.lda = lda;
+    g.B = tB.data(); g.ldb = ldb;
+    g.C = tC.data(); g.ldc = ldc;
+    if (dtype == IPEPS_MIXED) {
+      IPEPS_CHECK(mixed == 1 || mixed == 3 || mixed == 6, "gemm: mixed must be 1, 3 or 6");
+      g.mixed = (k::Mixed)mixed;
+    }
+    k::gemm(c, g);
+    to_host(c, tC, C);
   });
 }
 

@@ -129,10 +129,21 @@ struct Ctx {
   std::unique_ptr<Pool> pool;
   void* solver = nullptr;  // rocblas_handle
   uint64_t rng_counter = 0x9e3779b97f4a7c15ull;
-  // kernel-time instrumentation (hipEvents around the named phases)
+  // Kernel-time instrumentation: when `timing` is set, launches are bracketed
+  // by hipEvents on this stream and resolved by kernel_stats().
+  struct KEvent {
+    std::string name;
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  struct KStat {
+    long launches = 0;
+    double ms = 0, flops = 0, bytes = 0;
+  };
   bool timing = false;
-  std::map<std::string, double> phase_ms;
-  std::map<std::string, long> phase_calls;
+  std::vector<KEvent> kev;
+  std::map<std::string, KStat> kstats;
+  void resolve_timing();
 
   Tensor empty(DType dt, const std::vector<int64_t>& dims);
   Tensor zeros(DType dt, const std::vector<int64_t>& dims);
@@ -141,6 +152,27 @@ struct Ctx {
 
 Ctx* ctx_create(int device);
 void ctx_destroy(Ctx* c);
+
+// RAII bracket of one launch for the timing instrumentation.
+struct TimedLaunch {
+  Ctx& c;
+  bool on;
+  Ctx::KEvent ev;
+  TimedLaunch(Ctx& cc, const std::string& name, double flops, double bytes) : c(cc), on(cc.timing) {
+    if (!on) return;
+    ev.name = name;
+    ev.flops = flops;
+    ev.bytes = bytes;
+    HIP_CHECK(hipEventCreate(&ev.a));
+    HIP_CHECK(hipEventCreate(&ev.b));
+    HIP_CHECK(hipEventRecord(ev.a, c.stream));
+  }
+  ~TimedLaunch() {
+    if (!on) return;
+    (void)hipEventRecord(ev.b, c.stream);
+    c.kev.push_back(ev);
+  }
+};
 
 // Copy helpers (host <-> device, on the context stream, synchronous).
 void to_device(Ctx& c, Tensor& t, const void* host);

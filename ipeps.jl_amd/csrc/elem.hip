@@ -459,6 +459,10 @@ __global__ void energy2_back_kernel(int64_t m, const void* rh, const void* rv, c
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static const char* dtype_name(DType dt) {
+  return dt == DType::F64 ? "double" : dt == DType::C128 ? "HIP_vector_type<double, 2u>" : "float";
+}
+
 #define DISPATCH3(dt, KERNEL, ...)                                                                \
   do {                                                                                            \
     if ((dt) == DType::F64) { KERNEL(double, __VA_ARGS__); }                                      \
@@ -506,8 +510,9 @@ void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::v
     }
     if (rest_ok) {
       int64_t batch = n / (R * Cc);
-      if (batch <= 65535) {
+      if (batch <= 65535 && (Cc + 31) / 32 <= 65535) {
         dim3 grid((unsigned)((R + 31) / 32), (unsigned)((Cc + 31) / 32), (unsigned)batch);
+        TimedLaunch tl(c, std::string("transpose_kernel<") + dtype_name(dt) + ">", 0.0, 2.0 * n * dtype_size(dt));
 #define TK(T, ...) hipLaunchKernelGGL(transpose_kernel<T>, grid, dim3(TPB), 0, c.stream, R, Cc, batch, (const T*)in, (T*)out, cj)
         DISPATCH3(dt, TK, 0);
 #undef TK
@@ -520,6 +525,7 @@ void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::v
   p.nd = (int)od.size();
   for (int i = 0; i < p.nd; ++i) { p.odims[i] = od[i]; p.istr[i] = is[i]; }
   p.n = n;
+  TimedLaunch tl(c, std::string("permute_kernel<") + dtype_name(dt) + ">", 0.0, 2.0 * n * dtype_size(dt));
 #define PK(T, ...) hipLaunchKernelGGL(permute_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, p, (const T*)in, (T*)out, cj)
   DISPATCH3(dt, PK, 0);
 #undef PK
@@ -641,6 +647,12 @@ void svd_gamma(Ctx& c, DType dt, int64_t N, int64_t n, const double* S, const vo
 void svd_gamma_thin(Ctx& c, DType dt, int64_t n, const double* S, const void* UdU, const void* VdV, const double* dS,
                     double eta, void* Gamma, double* Sinv) {
   svd_gamma(c, dt, n, n, S, UdU, VdV, dS, eta, Gamma);
+  hipLaunchKernelGGL(sinv_kernel, dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, S, eta, Sinv);
+  HIP_CHECK(hipGetLastError());
+}
+
+void svd_sinv(Ctx& c, int64_t n, const double* S, double eta, double* Sinv) {
+  if (n <= 0) return;
   hipLaunchKernelGGL(sinv_kernel, dim3(blocks_for(n)), dim3(TPB), 0, c.stream, n, S, eta, Sinv);
   HIP_CHECK(hipGetLastError());
 }

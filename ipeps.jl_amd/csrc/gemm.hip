@@ -15,7 +15,8 @@
 //   A operand: lane l holds A[l % 16][kgroup = l / 16] (f64: 1 value,
 //              bf16 16x16x32: 8 consecutive k = 8*(l/16) + e)
 //   B operand: lane l holds B[kgroup = l / 16][l % 16]
-//   C/D      : lane l, element r holds C[4*(l/16) + r][l % 16]
+//   C/D      : f64 16x16x4: lane l, element r holds C[l/16 + 4*r][l % 16]
+//              bf16 16x16x32 (f32 out): C[4*(l/16) + r][l % 16]
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int64_t row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        int64_t row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
         int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
         if (row < p.M && col < p.N) {
           double v = p.alpha * acc[i][j][r];
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int64_t row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        int64_t row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
         int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
         if (row < p.M && col < p.N) {
           double vr = p.alpha * cr[i][j][r], vi = p.alpha * ci[i][j][r];
@@ -351,6 +352,15 @@ void gemm(Ctx& c, const GemmArgs& g) {
   IPEPS_CHECK(grid.y <= 65535, "gemm: N too large for the grid");
   dim3 block(256);
   const bool at = g.transA, bt = g.transB;
+  const char* tf[2] = {"false", "true"};
+  std::string name;
+  double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
+  if (g.dt == DType::F64) name = std::string("gemm_f64_kernel<") + tf[at] + ", " + tf[bt] + ">";
+  else if (g.dt == DType::C128) { name = std::string("gemm_c128_kernel<") + tf[at] + ", " + tf[bt] + ">"; flops *= 4; }
+  else name = std::string("gemm_f32_kernel<") + tf[at] + ", " + tf[bt] + ", " + std::to_string(g.mixed == Mixed::BF16 ? 1 : g.mixed == Mixed::BF16X3 ? 2 : 3) + ">";
+  const double bytes = ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N * (g.beta != 0.0 ? 2 : 1)) *
+                       (double)dtype_size(g.dt) * (double)g.batch;
+  TimedLaunch tl(c, name, flops, bytes);
   if (g.dt == DType::F64) {
     if (!at && !bt) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, c.stream, p);
     else if (at && !bt) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, c.stream, p);

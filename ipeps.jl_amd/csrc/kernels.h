@@ -77,6 +77,8 @@ void svd_gamma(Ctx& c, DType dt, int64_t N, int64_t n, const double* S, const vo
 void svd_gamma_thin(Ctx& c, DType dt, int64_t n, const double* S, const void* UdU,
                     const void* VdV, const double* dS, double eta, void* Gamma, double* Sinv);
 
+// Sinv[i] = s_i / (s_i^2 + eta), i < n
+void svd_sinv(Ctx& c, int64_t n, const double* S, double eta, double* Sinv);
 // s (FP64, n) -> out = s^(-1/2)
 void inv_sqrt(Ctx& c, int64_t n, const double* s, double* out);
 // gs += -0.5 * s^(-3/2) * g_isq   (backward of inv_sqrt)

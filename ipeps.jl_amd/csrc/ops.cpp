@@ -495,6 +495,7 @@ static void gesvd(Ctx& c, const Tensor& Ain, Tensor& U, Tensor& S, Tensor& V, bo
   Tensor E = c.empty(DType::F64, {std::max<int64_t>(1, mn)});
   Tensor info = c.empty(DType::F64, {1});
   rocblas_svect ls = thin ? rocblas_svect_singular : rocblas_svect_all;
+  std::unique_ptr<TimedLaunch> tl(new TimedLaunch(c, thin ? "rocsolver_gesvd_thin" : "rocsolver_gesvd", 0.0, 0.0));
   if (Ain.dt == DType::F64) {
     solver_check(rocsolver_dgesvd(h, ls, rocblas_svect_all, (rocblas_int)m, (rocblas_int)n, A.ptr<double>(),
                                   (rocblas_int)m, S.ptr<double>(), U.ptr<double>(), (rocblas_int)m,
@@ -511,6 +512,7 @@ static void gesvd(Ctx& c, const Tensor& Ain, Tensor& U, Tensor& S, Tensor& V, bo
   } else {
     throw Error(IPEPS_ERR_ARG, "gesvd: F64 or C128 only");
   }
+  tl.reset();
   int hinfo = 0;
   HIP_CHECK(hipMemcpyAsync(&hinfo, info.data(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
   c.sync();
@@ -534,6 +536,7 @@ Tensor orth(Ctx& c, const Tensor& Yin) {
   Tensor Y = clone(c, Yin);
   Tensor tau = c.empty(Yin.dt, {n});
   rocblas_handle h = (rocblas_handle)c.solver;
+  TimedLaunch tl(c, "rocsolver_geqrf_orgqr", 0.0, 0.0);
   if (Y.dt == DType::F64) {
     solver_check(rocsolver_dgeqrf(h, (rocblas_int)m, (rocblas_int)n, Y.ptr<double>(), (rocblas_int)m, tau.ptr<double>()),
                  "rocsolver_dgeqrf");
@@ -584,21 +587,23 @@ static DType solve_dt(DType dt) { return dt == DType::F32 ? DType::F64 : dt; }
 
 SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, const SvdOpts& o) {
   IPEPS_CHECK(R1.nd() == 2 && R2.nd() == 2 && R1.dims[1] == R2.dims[0], "svd_trunc: R1 R2 shape mismatch");
-  IPEPS_CHECK(R1.dims[0] == R2.dims[1], "svd_trunc: half-system matrix must be square");
-  const int64_t N = R1.dims[0];
+  const int64_t m = R1.dims[0], inner = R1.dims[1], nc = R2.dims[1];
+  const int64_t kmin = std::min(m, nc);
+  // new_chi = min(chi, size(R1, 2)) (src/ctm/ctmrg.jl:255)
+  const int64_t chi_eff = std::min<int64_t>(chi, inner);
   const DType dt = R1.dt, sdt = solve_dt(dt);
   SVDResult out;
-  const bool full = (o.mode == 0) || (N <= o.full_below) || (N <= (std::min<int64_t>(chi, N) + o.oversample));
-  Tensor Uf, Sf, Vf;  // full (mode 0) or the k computed triplets (mode 1)
+  const int64_t ntar = std::min(chi_eff, kmin);
+  const bool full = (o.mode == 0) || (kmin <= o.full_below) || (kmin <= ntar + o.oversample);
+  Tensor Uf, Sf, Vf;  // thin factors of M = R1 R2: U m x k, S k, V nc x k
   if (full) {
+    // LinearAlgebra.svd(M) (thin) as svd1 does (src/utility/autodiff.jl:22-25)
     Tensor M = matmul_raw(c, R1, false, false, R2, false, false);
-    Tensor Ms = to_dtype(c, M, sdt);
-    gesvd(c, Ms, Uf, Sf, Vf, false);
+    gesvd(c, to_dtype(c, M, sdt), Uf, Sf, Vf, true);
   } else {
-    const int64_t ntar = std::min<int64_t>(chi, N);
-    const int64_t kk = std::min<int64_t>(N, ntar + o.oversample);
-    Tensor Om = c.empty(dt, {N, kk});
-    k::randn(c, dt, N * kk, o.seed ^ (c.rng_counter += 0x9e3779b97f4a7c15ull), Om.data());
+    const int64_t kk = std::min<int64_t>(kmin, ntar + o.oversample);
+    Tensor Om = c.empty(dt, {nc, kk});
+    k::randn(c, dt, nc * kk, o.seed ^ (c.rng_counter += 0x9e3779b97f4a7c15ull), Om.data());
     auto apply_M = [&](const Tensor& X) {  // M X = R1 (R2 X)
       return matmul_raw(c, R1, false, false, matmul_raw(c, R2, false, false, X, false, false), false, false);
     };
@@ -610,7 +615,7 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
       Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt));
       Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt));
     }
-    Tensor Bt = to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt);  // N x k = M^H Q
+    Tensor Bt = to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt);  // nc x kk = M^H Q
     Tensor W, s, X;
     gesvd(c, Bt, W, s, X, true);  // Bt = W diag(s) X^H  ->  M ~ (Q X) diag(s) W^H
     Uf = matmul_raw(c, Q, false, false, X, false, false);
@@ -621,7 +626,7 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
   out.s_host.resize(ns);
   HIP_CHECK(hipMemcpyAsync(out.s_host.data(), Sf.data(), ns * 8, hipMemcpyDeviceToHost, c.stream));
   c.sync();
-  const int64_t n = cutoff_rule(out.s_host, std::min<int64_t>(chi, N));
+  const int64_t n = cutoff_rule(out.s_host, chi_eff);
   IPEPS_CHECK(n >= 1 && n <= ns, "svd_trunc: empty truncation");
   out.n = n;
   Tensor Un = first_cols(c, Uf, n), Vn = first_cols(c, Vf, n), Sn = first_n(c, Sf, n);
@@ -633,9 +638,9 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     base->slots.resize(3);
     std::weak_ptr<Node> wb = base;
     auto mk_slot = [&](int i, Tensor& t) {
-      t.node = make_node([wb, i](Ctx& c, Node& n) {
+      t.node = make_node([wb, i](Ctx& c, Node& nd) {
         auto b = wb.lock();
-        if (b) b->slots[i] = n.grad;
+        if (b) b->slots[i] = nd.grad;
       });
     };
     auto n1 = R1.node, n2 = R2.node;
@@ -643,76 +648,78 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     R1k.node.reset();
     R2k.node.reset();
     const double eta = o.eta;
-    if (full) {
-      base->back = [=](Ctx& c, Node& b) {
-        // reference svd_back on the square M (src/utility/autodiff.jl:41-75)
-        const int64_t NN = Uf.dims[0];
-        Tensor dU = b.slots[0].defined() ? to_dtype(c, b.slots[0], sdt) : Tensor();
-        Tensor dV = b.slots[2].defined() ? to_dtype(c, b.slots[2], sdt) : Tensor();
-        Tensor UdU, VdV;
-        if (dU.defined()) UdU = matmul_raw(c, Uf, true, true, dU, false, false);
-        if (dV.defined()) VdV = matmul_raw(c, Vf, true, true, dV, false, false);
-        Tensor G = c.empty(sdt, {NN, NN});
-        k::svd_gamma(c, sdt, NN, n, Sf.ptr<double>(), UdU.defined() ? UdU.data() : nullptr,
-                     VdV.defined() ? VdV.data() : nullptr, b.slots[1].defined() ? b.slots[1].ptr<double>() : nullptr,
-                     eta, G.data());
-        Tensor Mbar = matmul_raw(c, matmul_raw(c, Uf, false, false, G, false, false), false, false, Vf, true, true);
-        Tensor Mb = to_dtype(c, Mbar, dt);
-        if (n1) acc(c, n1, matmul_raw(c, Mb, false, false, R2k, true, true));
-        if (n2) acc(c, n2, matmul_raw(c, R1k, true, true, Mb, false, false));
+    // Factors the adjoint works with: all k thin triplets (full mode, the
+    // reference svd_back on svd1's output) or the kept n (randomized mode,
+    // discarded triplets treated as zero).
+    Tensor Ub = full ? Uf : Un, Vb = full ? Vf : Vn, Sb = full ? Sf : Sn;
+    base->back = [=](Ctx& c, Node& b) {
+      // svd_back (src/utility/autodiff.jl:41-75) for U (m x kb), V (nc x kb):
+      //   Mbar = U G V^H + (dU - U U^H dU) Sinv V^H + U Sinv (dV - V V^H dV)^H
+      // with the cotangents dU, dV (m x n, nc x n) zero beyond column n.
+      const int64_t kb = Sb.numel();
+      auto priv = [&](int slot, const std::vector<int64_t>& dims) {
+        if (!b.slots[slot].defined()) return c.zeros(sdt, dims);
+        Tensor t = to_dtype(c, b.slots[slot], sdt);
+        return t.buf == b.slots[slot].buf ? clone(c, t) : t;
       };
-    } else {
-      Tensor Ut = Un, Vt = Vn, St = Sn;
-      base->back = [=](Ctx& c, Node& b) {
-        // thin svd_back (src/utility/autodiff.jl:41-75 with the non-square terms),
-        // M-bar = X1 V^H + U X2^H propagated to R1, R2 without forming M.
-        Tensor dU = b.slots[0].defined() ? to_dtype(c, b.slots[0], sdt) : c.zeros(sdt, Ut.dims);
-        Tensor dV = b.slots[2].defined() ? to_dtype(c, b.slots[2], sdt) : c.zeros(sdt, Vt.dims);
-        Tensor UdU = matmul_raw(c, Ut, true, true, dU, false, false);  // n x n
-        Tensor VdV = matmul_raw(c, Vt, true, true, dV, false, false);
-        Tensor G = c.empty(sdt, {n, n});
-        Tensor Sinv = c.empty(DType::F64, {n});
-        k::svd_gamma_thin(c, sdt, n, St.ptr<double>(), UdU.data(), VdV.data(),
-                          b.slots[1].defined() ? b.slots[1].ptr<double>() : nullptr, eta, G.data(), Sinv.ptr<double>());
-        // X1 = U G + (dU - U UdU) diag(Sinv)
-        Tensor X1 = matmul_raw(c, Ut, false, false, G, false, false);
-        Tensor T1 = matmul_raw(c, Ut, false, false, UdU, false, false);
-        k::axpy(c, sdt, T1.numel(), -1.0, T1.data(), dU.data());  // dU - U UdU (dU is a private copy)
-        Tensor T1s = c.empty(sdt, dU.dims);
-        k::scale_cols(c, sdt, dU.dims[0], dU.dims[1], dU.data(), Sinv.ptr<double>(), T1s.data(), false);
-        k::axpy(c, sdt, X1.numel(), 1.0, T1s.data(), X1.data());
-        // X2 = (dV - V VdV) diag(Sinv)
-        Tensor T2 = matmul_raw(c, Vt, false, false, VdV, false, false);
+      Tensor dU = priv(0, {m, n}), dV = priv(2, {nc, n});
+      Tensor UdU = matmul_raw(c, Ub, true, true, dU, false, false);  // kb x n
+      Tensor VdV = matmul_raw(c, Vb, true, true, dV, false, false);
+      Tensor G = c.empty(sdt, {kb, kb});
+      k::svd_gamma(c, sdt, kb, n, Sb.ptr<double>(), UdU.data(), VdV.data(),
+                   b.slots[1].defined() ? b.slots[1].ptr<double>() : nullptr, eta, G.data());
+      Tensor Sinv = c.empty(DType::F64, {n});
+      k::svd_sinv(c, n, Sb.ptr<double>(), eta, Sinv.ptr<double>());
+      Tensor Un_ = first_cols(c, Ub, n), Vn_ = first_cols(c, Vb, n);
+      // X1 = U G V^H-part left factor and the non-square corrections
+      Tensor X1 = matmul_raw(c, Ub, false, false, G, false, false);  // m x kb
+      Tensor X2;                                                      // nc x n
+      const bool sq_u = (m == kb), sq_v = (nc == kb);
+      if (!sq_u) {
+        Tensor T1 = matmul_raw(c, Ub, false, false, UdU, false, false);  // m x n
+        k::axpy(c, sdt, T1.numel(), -1.0, T1.data(), dU.data());
+        k::scale_cols(c, sdt, m, n, dU.data(), Sinv.ptr<double>(), dU.data(), false);
+      }
+      if (!sq_v) {
+        Tensor T2 = matmul_raw(c, Vb, false, false, VdV, false, false);
         k::axpy(c, sdt, T2.numel(), -1.0, T2.data(), dV.data());
-        Tensor X2 = c.empty(sdt, dV.dims);
-        k::scale_cols(c, sdt, dV.dims[0], dV.dims[1], dV.data(), Sinv.ptr<double>(), X2.data(), false);
-        Tensor X1d = to_dtype(c, X1, dt), X2d = to_dtype(c, X2, dt);
-        Tensor Ud = to_dtype(c, Ut, dt), Vd = to_dtype(c, Vt, dt);
-        if (n1) {
-          // R1bar = Mbar R2^H = X1 (R2 V)^H + U (R2 X2)^H
-          Tensor W1 = matmul_raw(c, R2k, false, false, Vd, false, false);
-          Tensor W2 = matmul_raw(c, R2k, false, false, X2d, false, false);
-          Tensor g = matmul_raw(c, X1d, false, false, W1, true, true);
-          Tensor g2 = matmul_raw(c, Ud, false, false, W2, true, true);
+        X2 = c.empty(sdt, {nc, n});
+        k::scale_cols(c, sdt, nc, n, dV.data(), Sinv.ptr<double>(), X2.data(), false);
+      }
+      // Mbar = X1 Vb^H + [dU' Vn^H] + [Un X2^H], applied to R1/R2 without forming Mbar
+      Tensor X1d = to_dtype(c, X1, dt), Vbd = to_dtype(c, Vb, dt);
+      Tensor dUd = sq_u ? Tensor() : to_dtype(c, dU, dt), Vnd = sq_u ? Tensor() : to_dtype(c, Vn_, dt);
+      Tensor X2d = sq_v ? Tensor() : to_dtype(c, X2, dt), Und = sq_v ? Tensor() : to_dtype(c, Un_, dt);
+      if (n1) {
+        // R1bar = Mbar R2^H
+        Tensor g = matmul_raw(c, X1d, false, false, matmul_raw(c, R2k, false, false, Vbd, false, false), true, true);
+        if (!sq_u) {
+          Tensor g2 = matmul_raw(c, dUd, false, false, matmul_raw(c, R2k, false, false, Vnd, false, false), true, true);
           k::axpy(c, dt, g.numel(), 1.0, g2.data(), g.data());
-          acc(c, n1, g);
         }
-        if (n2) {
-          // R2bar = R1^H Mbar = (R1^H X1) V^H + (R1^H U) X2^H
-          Tensor Y1 = matmul_raw(c, R1k, true, true, X1d, false, false);
-          Tensor Y2 = matmul_raw(c, R1k, true, true, Ud, false, false);
-          Tensor g = matmul_raw(c, Y1, false, false, Vd, true, true);
-          Tensor g2 = matmul_raw(c, Y2, false, false, X2d, true, true);
+        if (!sq_v) {
+          Tensor g3 = matmul_raw(c, Und, false, false, matmul_raw(c, R2k, false, false, X2d, false, false), true, true);
+          k::axpy(c, dt, g.numel(), 1.0, g3.data(), g.data());
+        }
+        acc(c, n1, g);
+      }
+      if (n2) {
+        // R2bar = R1^H Mbar
+        Tensor g = matmul_raw(c, matmul_raw(c, R1k, true, true, X1d, false, false), false, false, Vbd, true, true);
+        if (!sq_u) {
+          Tensor g2 = matmul_raw(c, matmul_raw(c, R1k, true, true, dUd, false, false), false, false, Vnd, true, true);
           k::axpy(c, dt, g.numel(), 1.0, g2.data(), g.data());
-          acc(c, n2, g);
         }
-      };
-    }
+        if (!sq_v) {
+          Tensor g3 = matmul_raw(c, matmul_raw(c, R1k, true, true, Und, false, false), false, false, X2d, true, true);
+          k::axpy(c, dt, g.numel(), 1.0, g3.data(), g.data());
+        }
+        acc(c, n2, g);
+      }
+    };
     mk_slot(0, out.U);
     mk_slot(1, out.S);
     mk_slot(2, out.V);
-    // keep the base node alive as long as the slot nodes are on the tape
-    (void)base;
   }
   return out;
 }

@@ -63,6 +63,22 @@ Tensor Ctx::zeros(DType dt, const std::vector<int64_t>& dims) {
   return t;
 }
 
+void Ctx::resolve_timing() {
+  HIP_CHECK(hipStreamSynchronize(stream));
+  for (auto& e : kev) {
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+    auto& s = kstats[e.name];
+    s.launches += 1;
+    s.ms += ms;
+    s.flops += e.flops;
+    s.bytes += e.bytes;
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  kev.clear();
+}
+
 Ctx* ctx_create(int device) {
   int n = 0;
   HIP_CHECK(hipGetDeviceCount(&n));

@@ -71,6 +71,16 @@ class Context:
         check(lib().ipeps_ctx_mem(self.h, C.byref(live), C.byref(peak)))
         return live.value, peak.value
 
+    def set_timing(self, on=True):
+        check(lib().ipeps_ctx_set_timing(self.h, int(bool(on))))
+
+    def kernel_stats(self, reset=True):
+        """{kernel: (launches, total_ms, algorithmic_flops, algorithmic_bytes)}."""
+        import json
+        buf = C.create_string_buffer(1 << 20)
+        check(lib().ipeps_ctx_kernel_stats(self.h, buf, len(buf), int(bool(reset))))
+        return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}
+
     def __del__(self):
         try:
             self.close()
@@ -298,6 +308,29 @@ def tcon(xs, ind_xs, ctx=None):
     return out
 
 
+def gemm(A, B, transA=False, conjA=False, transB=False, conjB=False, mixed=None, alpha=1.0, beta=0.0, C=None,
+         ctx=None):
+    """alpha op(A) op(B) + beta C on the device (FP64 / complex FP64 MFMA, or
+    split-bf16 MFMA on float32 data when ``mixed`` is 1, 3 or 6)."""
+    ctx = ctx or default_context()
+    A, B = np.asarray(A), np.asarray(B)
+    if mixed:
+        code, dt = MIXED, np.float32
+    else:
+        cplx = np.iscomplexobj(A) or np.iscomplexobj(B) or (C is not None and np.iscomplexobj(C))
+        code = C128 if cplx else F64
+        dt = _host_np(code)
+    Af, Bf = np.asfortranarray(A, dtype=dt), np.asfortranarray(B, dtype=dt)
+    M = A.shape[1] if transA else A.shape[0]
+    K = A.shape[0] if transA else A.shape[1]
+    N = B.shape[0] if transB else B.shape[1]
+    Cf = np.zeros((M, N), dtype=dt, order="F") if C is None else np.asfortranarray(C, dtype=dt).copy(order="F")
+    check(lib().ipeps_gemm(ctx.h, code, int(mixed or 0), int(transA), int(conjA), int(transB), int(conjB), M, N, K,
+                           float(alpha), _ptr(Af), max(1, Af.shape[0]), _ptr(Bf), max(1, Bf.shape[0]), float(beta),
+                           _ptr(Cf), max(1, M)))
+    return Cf
+
+
 def svd1(A, ctx=None):
     """Full SVD of a square matrix on the device: (U, S, V) with A = U S V^H."""
     ctx = ctx or default_context()
@@ -337,5 +370,5 @@ def svd_back(U, S, V, dU=None, dS=None, dV=None, eta=1e-12, ctx=None):
 
 
 __all__ = ["Context", "CTMTensors", "rg_step", "run_ctm", "get_gs_energy", "get_gs_norm", "compute_gs_energy",
-           "step_adjoint", "gs_energy_grad", "tcon", "svd1", "svd_back", "default_context", "IpepsError",
+           "step_adjoint", "gs_energy_grad", "tcon", "gemm", "svd1", "svd_back", "default_context", "IpepsError",
            "ENV_OPTIONS"]

@@ -207,8 +207,10 @@ def test_randomized_projector_energy(pkg):
     cfg = dict(chi=24, min_iter=10, max_iter=10, rg_tol=0.0)
     E_full, _, _ = pkg.compute_gs_energy(A, H, cfg)
     E_rand, _, _ = pkg.compute_gs_energy(A, H, cfg, options=dict(svd_mode=1, oversample=16, power_iters=2))
-    print("full", E_full, "rand", E_rand, "diff", abs(E_full - E_rand))
-    assert abs(E_full - E_rand) < 1e-8
+    E_rand2, _, _ = pkg.compute_gs_energy(A, H, cfg, options=dict(svd_mode=1, oversample=40, power_iters=4))
+    print("full", E_full, "rand k+16 q2", abs(E_full - E_rand), "rand k+40 q4", abs(E_full - E_rand2))
+    assert abs(E_full - E_rand) < 1e-6
+    assert abs(E_full - E_rand2) < abs(E_full - E_rand)
 
 
 @pytest.mark.parametrize("mixed", [3, 6])
@@ -220,3 +222,30 @@ def test_mixed_precision_energy(pkg, mixed):
     Em, _, _ = pkg.compute_gs_energy(A, H, cfg, dtype="mixed", options=dict(mixed=mixed))
     print("mixed", mixed, "E64", E64, "Emixed", Em, "diff", abs(E64 - Em))
     assert abs(E64 - Em) < (1e-4 if mixed == 3 else 1e-6)
+
+
+@pytest.mark.parametrize("kind", ["f64", "c128", "mixed3", "mixed6", "mixed1"])
+def test_gemm_all_ops(pkg, kind):
+    """Every transpose / conjugate combination against numpy."""
+    rng = np.random.default_rng(11)
+    M, N, K = 70, 45, 37
+    cplx = kind == "c128"
+    mixed = {"mixed1": 1, "mixed3": 3, "mixed6": 6}.get(kind)
+    tol = {None: 1e-12, 1: 2e-2, 3: 1e-4, 6: 1e-5}[mixed]
+    for tA in (False, True):
+        for tB in (False, True):
+            for cA in ((False, True) if cplx else (False,)):
+                for cB in ((False, True) if cplx else (False,)):
+                    sa = (K, M) if tA else (M, K)
+                    sb = (N, K) if tB else (K, N)
+                    A = rng.standard_normal(sa) + (1j * rng.standard_normal(sa) if cplx else 0)
+                    B = rng.standard_normal(sb) + (1j * rng.standard_normal(sb) if cplx else 0)
+                    C0 = rng.standard_normal((M, N)) + (1j * rng.standard_normal((M, N)) if cplx else 0)
+                    oA = A.T if tA else A
+                    oA = oA.conj() if cA else oA
+                    oB = B.T if tB else B
+                    oB = oB.conj() if cB else oB
+                    ref = 0.7 * oA @ oB + 0.3 * C0
+                    got = pkg.gemm(A, B, tA, cA, tB, cB, mixed=mixed, alpha=0.7, beta=0.3, C=C0)
+                    err = np.abs(got - ref).max() / np.abs(ref).max()
+                    assert err < tol, (kind, tA, cA, tB, cB, err)
