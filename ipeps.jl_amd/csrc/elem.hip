@@ -293,10 +293,21 @@ __global__ void renorm_back_kernel<float>(int64_t n, const float* x, const float
 }
 
 // ---- SVD adjoint pieces -----------------------------------------------------
+// F_ij = (s_j^2 - s_i^2) / ((s_j^2 - s_i^2)^2 + eta) with the squares rounded
+// before the difference, as S2' .- S2 does (src/utility/autodiff.jl:46-47):
+// a fused multiply-add here would leave a rounding residue instead of an exact
+// 0 for equal singular values, which eta = 1e-12 then amplifies.
 __device__ inline double Fij(const double* S, int64_t i, int64_t j, double eta) {
+#pragma clang fp contract(off)
   double si = S[i] * S[i], sj = S[j] * S[j];
   double f = sj - si;
   return f / (f * f + eta);
+}
+
+__device__ inline double sinv_of(double s, double eta) {
+#pragma clang fp contract(off)
+  double s2 = s * s;
+  return s / (s2 + eta);
 }
 
 // Gamma_ij = (J_ij + conj(J_ji)) s_j + s_i (K_ij + conj(K_ji)) + delta terms,
@@ -339,7 +350,7 @@ __global__ void svd_gamma_kernel<double2>(int64_t N, int64_t n, int64_t ld, cons
     if (i == j && i < n) {
       if (dS) v.x += dS[i];
       if (UdU) {
-        double sinv = S[i] / (S[i] * S[i] + eta);
+        double sinv = sinv_of(S[i], eta);
         v.y += UdU[i + i * ld].y * sinv;
       }
     }
@@ -349,7 +360,7 @@ __global__ void svd_gamma_kernel<double2>(int64_t N, int64_t n, int64_t ld, cons
 
 __global__ void sinv_kernel(int64_t n, const double* S, double eta, double* Sinv) {
   for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
-    Sinv[i] = S[i] / (S[i] * S[i] + eta);
+    Sinv[i] = sinv_of(S[i], eta);
 }
 
 __global__ void inv_sqrt_kernel(int64_t n, const double* s, double* out) {

@@ -77,6 +77,50 @@ def test_svd_and_adjoint(pkg, cplx):
     np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
 
 
+def _gamma_ref(S, UdU, VdV, dS, n, eta=1e-12):
+    N = S.size
+    G = np.zeros((N, N), dtype=np.complex128)
+    for i in range(N):
+        for j in range(N):
+            d = S[j] ** 2 - S[i] ** 2
+            f = d / (d * d + eta)
+            Jij = f * UdU[i, j] if (UdU is not None and j < n) else 0
+            Jji = -f * UdU[j, i] if (UdU is not None and i < n) else 0
+            Kij = f * VdV[i, j] if (VdV is not None and j < n) else 0
+            Kji = -f * VdV[j, i] if (VdV is not None and i < n) else 0
+            v = (Jij + np.conj(Jji)) * S[j] + S[i] * (Kij + np.conj(Kji))
+            if i == j and i < n:
+                if dS is not None:
+                    v += dS[i]
+                if UdU is not None:
+                    v += 1j * UdU[i, i].imag * S[i] / (S[i] ** 2 + eta)
+            G[i, j] = v
+    return G
+
+
+def test_svd_back_gamma_terms_complex(pkg):
+    """U = V = I isolates Gamma; each cotangent separately."""
+    rng = np.random.default_rng(5)
+    N, n = 6, 4
+    S = np.sort(rng.uniform(0.5, 2.0, N))[::-1].copy()
+    I = np.eye(N, dtype=np.complex128)
+    dU = rng.standard_normal((N, n)) + 1j * rng.standard_normal((N, n))
+    dV = rng.standard_normal((N, n)) + 1j * rng.standard_normal((N, n))
+    dS = rng.standard_normal(n)
+    errs = {}
+    for name, args in [("dU", (dU, None, None)), ("dV", (None, None, dV)), ("dS", (None, dS, None)),
+                       ("all", (dU, dS, dV))]:
+        got = pkg.svd_back(I, S, I, *args)
+        ref = _gamma_ref(S, args[0], args[2], args[1], n)
+        errs[name] = float(np.abs(got - ref).max())
+        if errs[name] > 1e-12:
+            D = got - ref
+            print(name, "max|diff|", errs[name], "diag", np.abs(np.diag(D)).max(), "re", np.abs(D.real).max(),
+                  "im", np.abs(D.imag).max())
+            print(np.round(D, 3))
+    assert max(errs.values()) < 1e-12, errs
+
+
 # ---------------------------------------------------------------- CTMRG forward
 
 @pytest.mark.parametrize("which", ["ising_real", "loopgas_complex"])
