@@ -81,6 +81,7 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
   else if (key == "eta") o.svd.eta = v;
   else if (key == "canonical_edge") o.canonical_edge = v != 0.0;
   else if (key == "energy_fp64") o.energy_fp64 = v != 0.0;
+  else if (key == "checkpoint") o.checkpoint = v != 0.0;
   else if (key == "mixed") {
     int m = (int)v;
     IPEPS_CHECK(m == 1 || m == 3 || m == 6, "mixed must be 1, 3 or 6");
@@ -158,6 +159,7 @@ int ipeps_ctx_set_timing(ipeps_ctx* ctx, int on) {
     IPEPS_CHECK(ctx, "null ctx");
     ctx->c->resolve_timing();
     ctx->c->timing = on != 0;
+    ctx->c->timing_detail = on >= 2;
   });
 }
 

@@ -141,6 +141,7 @@ struct Ctx {
     double ms = 0, flops = 0, bytes = 0;
   };
   bool timing = false;
+  bool timing_detail = false;  // key GEMM statistics by shape as well
   std::vector<KEvent> kev;
   std::map<std::string, KStat> kstats;
   void resolve_timing();

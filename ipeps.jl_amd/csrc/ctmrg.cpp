@@ -447,6 +447,29 @@ static void backprop_steps(Ctx& c, Env& e, const std::vector<Checkpoint>& ck, in
   }
 }
 
+// Leaves on the tape for A (Ad = conj(A) is derived) and, optionally, the
+// environment; returns the A leaf.
+static Tensor mark_leaves(Ctx& c, Env& e, bool env_leaves) {
+  Tensor A = op::leaf(e.A);
+  e.A = A;
+  e.Ad = op::conj(c, A);
+  if (env_leaves)
+    for (int i = 0; i < 4; ++i) {
+      e.C[i] = op::leaf(e.C[i]);
+      e.E[i] = op::leaf(e.E[i]);
+    }
+  return A;
+}
+
+static void strip_nodes(Env& e) {
+  e.A.node.reset();
+  e.Ad.node.reset();
+  for (int i = 0; i < 4; ++i) {
+    e.C[i].node.reset();
+    e.E[i].node.reset();
+  }
+}
+
 GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options& opt, const Tensor& H1,
                           const Tensor& H2, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol) {
   GradResult g;
@@ -461,17 +484,27 @@ GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options&
   init_ctm1(c, e);
   auto r = run_ctm(c, e, chi, min_iter, max_iter, tol, 0, H1, H2);
   g.conv_iters = r.iters;
-  // differentiated run (run_gs): same stopping rule with max_iter = ad_max_iter
+  // Differentiated run (run_gs): the same stopping rule with max_iter =
+  // ad_max_iter from the converged environment, which is held constant.
+  // Without checkpointing one tape spans every step (memory ~ steps x the
+  // per-step intermediates); with checkpointing only (C, E) per step is
+  // kept and each step is recomputed on its own tape in the reverse sweep.
+  Tape tape;
+  tape.active = !opt.checkpoint;
+  TapeScope ts(&tape);
+  Tensor Aleaf;
+  if (!opt.checkpoint) Aleaf = mark_leaves(c, e, false);
   std::vector<Checkpoint> ck;
   std::vector<double> conv = {1.0, 0.0};
   std::vector<double> diffs = {1.0};
   int it = 0;
   for (int i = 1; i <= ad_max_iter; ++i) {
     it = i;
-    ck.push_back(save(e));
+    if (opt.checkpoint) ck.push_back(save(e));
     rg_step(c, e, chi);
     const bool need = (i >= min_iter - 1 && i < ad_max_iter);
     if (need) {
+      TapeScope off(nullptr);  // conv_fun runs under Zygote.@ignore (src/ctm/ctmrg.jl:22)
       double E[2];
       read_energy(c, energy(c, e, H1, H2), E);
       std::vector<double> nc = {E[0], E[1]};
@@ -484,39 +517,35 @@ GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options&
     if (i >= min_iter && diffs.back() < tol) break;
   }
   g.ad_iters = it;
-  // energy on the tape -> cotangents of the final environment (and of A)
-  std::array<Tensor, 4> cotC, cotE;
   Tensor gA;
-  {
-    Tape tape;
-    tape.active = true;
-    TapeScope ts(&tape);
-    Tensor A = op::leaf(e.A);
-    e.A = A;
-    e.Ad = op::conj(c, A);
-    for (int i = 0; i < 4; ++i) {
-      e.C[i] = op::leaf(e.C[i]);
-      e.E[i] = op::leaf(e.E[i]);
-    }
+  if (!opt.checkpoint) {
     Tensor Et = energy(c, e, H1, H2);
-    Tensor seed = c.empty(DType::F64, {8});
-    op::acc(c, Et.node, seed);  // the energy node ignores the seed value (dL/dRe E = 1)
+    op::acc(c, Et.node, c.empty(DType::F64, {8}));  // the energy node takes dL/dRe(E) = 1
     tape.active = false;
     read_energy(c, Et, g.E);
     op::backward(c, tape);
+    if (Aleaf.node->grad.defined()) gA = Aleaf.node->grad;
+    else gA = c.zeros(Aleaf.dt, Aleaf.dims);
+    strip_nodes(e);
+  } else {
+    std::array<Tensor, 4> cotC, cotE;
+    Tape t2;
+    t2.active = true;
+    TapeScope ts2(&t2);
+    Tensor A = mark_leaves(c, e, true);
+    Tensor Et = energy(c, e, H1, H2);
+    op::acc(c, Et.node, c.empty(DType::F64, {8}));
+    t2.active = false;
+    read_energy(c, Et, g.E);
+    op::backward(c, t2);
     for (int i = 0; i < 4; ++i) {
       cotC[i] = e.C[i].node->grad;
       cotE[i] = e.E[i].node->grad;
     }
     if (A.node->grad.defined()) gA = op::clone(c, A.node->grad);
-    e.A.node.reset();
-    e.Ad.node.reset();
-    for (int i = 0; i < 4; ++i) {
-      e.C[i].node.reset();
-      e.E[i].node.reset();
-    }
+    strip_nodes(e);
+    backprop_steps(c, e, ck, chi, cotC, cotE, gA);
   }
-  backprop_steps(c, e, ck, chi, cotC, cotE, gA);
   DType gdt = env_dt == DType::F32 ? DType::F64 : env_dt;
   g.grad = op::to_dtype(c, gA, gdt);
   return g;
@@ -524,46 +553,47 @@ GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options&
 
 Tensor step_and_adjoint(Ctx& c, Env& e, int64_t chi, const Tensor& H1, const Tensor& H2, double E_out[2]) {
   TapeScope none(nullptr);
-  std::vector<Checkpoint> ck = {save(e)};
-  rg_step(c, e, chi);
-  std::array<Tensor, 4> cotC, cotE;
-  Tensor gA;
-  {
-    Tape tape;
-    tape.active = true;
-    TapeScope ts(&tape);
-    Tensor A = op::leaf(e.A);
-    e.A = A;
-    e.Ad = op::conj(c, A);
-    for (int i = 0; i < 4; ++i) {
-      e.C[i] = op::leaf(e.C[i]);
-      e.E[i] = op::leaf(e.E[i]);
+  if (e.opt.checkpoint) {
+    std::vector<Checkpoint> ck = {save(e)};
+    rg_step(c, e, chi);
+    std::array<Tensor, 4> cotC, cotE;
+    Tensor gA;
+    {
+      Tape tape;
+      tape.active = true;
+      TapeScope ts(&tape);
+      Tensor A = mark_leaves(c, e, true);
+      Tensor Et = energy(c, e, H1, H2);
+      op::acc(c, Et.node, c.empty(DType::F64, {8}));
+      tape.active = false;
+      op::backward(c, tape);
+      for (int i = 0; i < 4; ++i) {
+        cotC[i] = e.C[i].node->grad;
+        cotE[i] = e.E[i].node->grad;
+      }
+      if (A.node->grad.defined()) gA = op::clone(c, A.node->grad);
+      read_energy(c, Et, E_out);
+      strip_nodes(e);
     }
-    Tensor Et = energy(c, e, H1, H2);
-    op::acc(c, Et.node, c.empty(DType::F64, {8}));
-    tape.active = false;
-    op::backward(c, tape);
-    for (int i = 0; i < 4; ++i) {
-      cotC[i] = e.C[i].node->grad;
-      cotE[i] = e.E[i].node->grad;
-    }
-    if (A.node->grad.defined()) gA = op::clone(c, A.node->grad);
-    double h[8];
-    HIP_CHECK(hipMemcpyAsync(h, Et.data(), 64, hipMemcpyDeviceToHost, c.stream));
-    c.sync();
-    E_out[0] = h[0];
-    E_out[1] = h[1];
-    e.A.node.reset();
-    e.Ad.node.reset();
-    for (int i = 0; i < 4; ++i) {
-      e.C[i].node.reset();
-      e.E[i].node.reset();
-    }
+    Checkpoint after = save(e);
+    backprop_steps(c, e, ck, chi, cotC, cotE, gA);
+    load(e, after);  // leave the environment advanced by one step
+    e.Ad = op::conj(c, e.A);
+    return gA;
   }
-  Checkpoint after = save(e);
-  backprop_steps(c, e, ck, chi, cotC, cotE, gA);
-  load(e, after);  // leave the environment advanced by one step
-  e.Ad = op::conj(c, e.A);
+  // one tape over the step and the energy
+  Tape tape;
+  tape.active = true;
+  TapeScope ts(&tape);
+  Tensor A = mark_leaves(c, e, false);
+  rg_step(c, e, chi);
+  Tensor Et = energy(c, e, H1, H2);
+  op::acc(c, Et.node, c.empty(DType::F64, {8}));
+  tape.active = false;
+  op::backward(c, tape);
+  Tensor gA = A.node->grad.defined() ? A.node->grad : c.zeros(A.dt, A.dims);
+  read_energy(c, Et, E_out);
+  strip_nodes(e);
   return gA;
 }
 

@@ -16,6 +16,7 @@ struct Options {
   bool canonical_edge = true;  // P2-first edge update (chi^3 D^4) instead of the chi^2 D^6 intermediate
   k::Mixed mixed = k::Mixed::BF16X3;
   bool energy_fp64 = true;     // energy contractions in FP64 even for mixed environments
+  bool checkpoint = false;     // recompute each step in the reverse sweep instead of keeping its tape
 };
 
 // Plain ground-state environment (src/ctm/ctm_tensor.jl:3-18).  Storage dtype

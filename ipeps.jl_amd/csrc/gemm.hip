@@ -39,6 +39,7 @@ struct GemmP {
   void* C;
   int64_t ldc, sC;
   int conjA, conjB;
+  int64_t kchunk;  // > 0: split-K, blockIdx.z indexes the K chunk (batch must be 1)
 };
 
 constexpr int BM = 64, BN = 64;
@@ -65,7 +66,9 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = (v4f64){0.0, 0.0, 0.0, 0.0};
 
-  for (int64_t k0 = 0; k0 < p.K; k0 += BK) {
+  // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
+  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - (int64_t)blockIdx.z * p.kchunk) : p.K;
+  for (int64_t k0 = 0; k0 < Keff; k0 += BK) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       int m, kk;
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
       else { kk = tid & 15; m = (tid >> 4) + 16 * r; }
       int64_t gm = m0 + m, gk = k0 + kk;
       double v = 0.0;
-      if (gm < p.M && gk < p.K) v = AT ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
+      if (gm < p.M && gk < Keff) v = AT ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
       As[kk][m] = v;
     }
 #pragma unroll
@@ -83,7 +86,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
       else { n = tid & 63; kk = (tid >> 6) + 4 * r; }
       int64_t gn = n0 + n, gk = k0 + kk;
       double v = 0.0;
-      if (gn < p.N && gk < p.K) v = BT ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
+      if (gn < p.N && gk < Keff) v = BT ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
       Bs[kk][n] = v;
     }
     __syncthreads();
@@ -145,7 +148,9 @@ __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
       ci[i][j] = (v4f64){0.0, 0.0, 0.0, 0.0};
     }
 
-  for (int64_t k0 = 0; k0 < p.K; k0 += BK) {
+  // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
+  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - (int64_t)blockIdx.z * p.kchunk) : p.K;
+  for (int64_t k0 = 0; k0 < Keff; k0 += BK) {
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       int m, kk;
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
       else { kk = tid & 7; m = (tid >> 3) + 32 * r; }
       int64_t gm = m0 + m, gk = k0 + kk;
       double2 v = make_double2(0.0, 0.0);
-      if (gm < p.M && gk < p.K) v = AT ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
+      if (gm < p.M && gk < Keff) v = AT ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
       Ar[kk][m] = v.x;
       Ai[kk][m] = sa * v.y;
     }
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
       else { n = tid & 63; kk = (tid >> 6) + 4 * r; }
       int64_t gn = n0 + n, gk = k0 + kk;
       double2 v = make_double2(0.0, 0.0);
-      if (gn < p.N && gk < p.K) v = BT ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
+      if (gn < p.N && gk < Keff) v = BT ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
       Br[kk][n] = v.x;
       Bi[kk][n] = sb * v.y;
     }
@@ -248,7 +253,9 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = (v4f32){0.f, 0.f, 0.f, 0.f};
 
-  for (int64_t k0 = 0; k0 < p.K; k0 += BK) {
+  // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
+  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - (int64_t)blockIdx.z * p.kchunk) : p.K;
+  for (int64_t k0 = 0; k0 < Keff; k0 += BK) {
     // 64 x 32 tile = 2048 values, 8 per thread.
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -257,7 +264,7 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
       else { kk = tid & 31; m = (tid >> 5) + 8 * r; }
       int64_t gm = m0 + m, gk = k0 + kk;
       float v = 0.f;
-      if (gm < p.M && gk < p.K) v = AT ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
+      if (gm < p.M && gk < Keff) v = AT ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
 #pragma unroll
       for (int s = 0; s < NSPLIT; ++s) {
         unsigned short h = bf16_rne(v);
@@ -272,7 +279,7 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
       else { n = tid & 63; kk = (tid >> 6) + 4 * r; }
       int64_t gn = n0 + n, gk = k0 + kk;
       float v = 0.f;
-      if (gn < p.N && gk < p.K) v = BT ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
+      if (gn < p.N && gk < Keff) v = BT ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
 #pragma unroll
       for (int s = 0; s < NSPLIT; ++s) {
         unsigned short h = bf16_rne(v);
@@ -327,40 +334,40 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
 }
 
 // ---------------------------------------------------------------------------
+// split-K reduction: C = alpha * sum_s W_s + beta * C
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T zero_of() { return T(0); }
+template <>
+__device__ inline double2 zero_of<double2>() { return make_double2(0.0, 0.0); }
+__device__ inline double2 add_(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double add_(double a, double b) { return a + b; }
+__device__ inline float add_(float a, float b) { return a + b; }
+__device__ inline double2 axpby_(double al, double2 s, double be, double2 c) {
+  return make_double2(al * s.x + be * c.x, al * s.y + be * c.y);
+}
+__device__ inline double axpby_(double al, double s, double be, double c) { return al * s + be * c; }
+__device__ inline float axpby_(double al, float s, double be, float c) { return (float)(al * s + be * c); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_reduce(int64_t M, int64_t N, int64_t S, const T* __restrict__ W,
+                                                     double alpha, double beta, T* __restrict__ C, int64_t ldc) {
+  const int64_t MN = M * N;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < MN; t += (int64_t)gridDim.x * 256) {
+    T acc = zero_of<T>();
+    for (int64_t s = 0; s < S; ++s) acc = add_(acc, W[s * MN + t]);
+    int64_t m = t % M, n = t / M;
+    T* dst = C + m + n * ldc;
+    *dst = beta != 0.0 ? axpby_(alpha, acc, beta, *dst) : axpby_(alpha, acc, 0.0, zero_of<T>());
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
-void gemm(Ctx& c, const GemmArgs& g) {
-  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
-  if (g.K <= 0) {
-    IPEPS_CHECK(g.beta == 0.0 || g.beta == 1.0, "gemm: K=0 with beta not in {0,1}");
-    if (g.beta == 0.0) {
-      size_t es = dtype_size(g.dt);
-      for (int64_t b = 0; b < g.batch; ++b)
-        for (int64_t j = 0; j < g.N; ++j)
-          fill_zero(c, (char*)g.C + (b * g.sC + j * g.ldc) * es, g.M * es);
-    }
-    return;
-  }
-  IPEPS_CHECK(g.batch <= 65535, "gemm: batch too large");
-  GemmP p;
-  p.M = g.M; p.N = g.N; p.K = g.K; p.alpha = g.alpha; p.beta = g.beta;
-  p.A = g.A; p.lda = g.lda; p.sA = g.sA;
-  p.B = g.B; p.ldb = g.ldb; p.sB = g.sB;
-  p.C = g.C; p.ldc = g.ldc; p.sC = g.sC;
-  p.conjA = g.conjA; p.conjB = g.conjB;
-  dim3 grid((unsigned)((g.M + BM - 1) / BM), (unsigned)((g.N + BN - 1) / BN), (unsigned)g.batch);
-  IPEPS_CHECK(grid.y <= 65535, "gemm: N too large for the grid");
+static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid) {
   dim3 block(256);
   const bool at = g.transA, bt = g.transB;
-  const char* tf[2] = {"false", "true"};
-  std::string name;
-  double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
-  if (g.dt == DType::F64) name = std::string("gemm_f64_kernel<") + tf[at] + ", " + tf[bt] + ">";
-  else if (g.dt == DType::C128) { name = std::string("gemm_c128_kernel<") + tf[at] + ", " + tf[bt] + ">"; flops *= 4; }
-  else name = std::string("gemm_f32_kernel<") + tf[at] + ", " + tf[bt] + ", " + std::to_string(g.mixed == Mixed::BF16 ? 1 : g.mixed == Mixed::BF16X3 ? 2 : 3) + ">";
-  const double bytes = ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N * (g.beta != 0.0 ? 2 : 1)) *
-                       (double)dtype_size(g.dt) * (double)g.batch;
-  TimedLaunch tl(c, name, flops, bytes);
   if (g.dt == DType::F64) {
     if (!at && !bt) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, c.stream, p);
     else if (at && !bt) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, c.stream, p);
@@ -387,6 +394,87 @@ void gemm(Ctx& c, const GemmArgs& g) {
 #undef IPEPS_F32_LAUNCH
   }
   HIP_CHECK(hipGetLastError());
+}
+
+void gemm(Ctx& c, const GemmArgs& g) {
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  if (g.K <= 0) {
+    IPEPS_CHECK(g.beta == 0.0 || g.beta == 1.0, "gemm: K=0 with beta not in {0,1}");
+    if (g.beta == 0.0) {
+      size_t es = dtype_size(g.dt);
+      for (int64_t b = 0; b < g.batch; ++b)
+        for (int64_t j = 0; j < g.N; ++j)
+          fill_zero(c, (char*)g.C + (b * g.sC + j * g.ldc) * es, g.M * es);
+    }
+    return;
+  }
+  IPEPS_CHECK(g.batch <= 65535, "gemm: batch too large");
+  GemmP p;
+  p.M = g.M; p.N = g.N; p.K = g.K; p.alpha = g.alpha; p.beta = g.beta;
+  p.A = g.A; p.lda = g.lda; p.sA = g.sA;
+  p.B = g.B; p.ldb = g.ldb; p.sB = g.sB;
+  p.C = g.C; p.ldc = g.ldc; p.sC = g.sC;
+  p.conjA = g.conjA; p.conjB = g.conjB;
+  p.kchunk = 0;
+  dim3 grid((unsigned)((g.M + BM - 1) / BM), (unsigned)((g.N + BN - 1) / BN), (unsigned)g.batch);
+  IPEPS_CHECK(grid.y <= 65535, "gemm: N too large for the grid");
+  const bool at = g.transA, bt = g.transB;
+  const char* tf[2] = {"false", "true"};
+  std::string name;
+  double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
+  if (g.dt == DType::F64) name = std::string("gemm_f64_kernel<") + tf[at] + ", " + tf[bt] + ">";
+  else if (g.dt == DType::C128) { name = std::string("gemm_c128_kernel<") + tf[at] + ", " + tf[bt] + ">"; flops *= 4; }
+  else name = std::string("gemm_f32_kernel<") + tf[at] + ", " + tf[bt] + ", " + std::to_string(g.mixed == Mixed::BF16 ? 1 : g.mixed == Mixed::BF16X3 ? 2 : 3) + ">";
+  const double bytes = ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N * (g.beta != 0.0 ? 2 : 1)) *
+                       (double)dtype_size(g.dt) * (double)g.batch;
+  if (c.timing_detail)
+    name += " M=" + std::to_string(g.M) + " N=" + std::to_string(g.N) + " K=" + std::to_string(g.K) +
+            " b=" + std::to_string(g.batch);
+  // Split-K when the output tiles cannot fill the 256 CUs and K is long:
+  // each z-slice of the grid reduces one K chunk into a workspace, then a
+  // reduction kernel applies alpha / beta.
+  const int64_t tiles = (int64_t)grid.x * grid.y * g.batch;
+  int64_t S = 1, kc = 0;
+  if (g.batch == 1 && tiles < 600 && g.K >= 1024) {
+    S = std::min<int64_t>((1024 + tiles - 1) / tiles, g.K / 256);
+    if (S >= 2) {
+      kc = ((g.K + S - 1) / S + 63) / 64 * 64;
+      S = (g.K + kc - 1) / kc;
+    }
+    if (S > 65535) S = 65535;
+  }
+  if (S >= 2) {
+    const size_t es = dtype_size(g.dt);
+    const size_t wbytes = (size_t)S * g.M * g.N * es;
+    std::shared_ptr<Buf> ws(new Buf{c.pool->alloc(wbytes), wbytes, c.pool.get()});
+    GemmP q = p;
+    q.alpha = 1.0;
+    q.beta = 0.0;
+    q.C = ws->p;
+    q.ldc = g.M;
+    q.sC = g.M * g.N;
+    q.kchunk = kc;
+    q.sA = at ? kc : kc * g.lda;
+    q.sB = bt ? kc * g.ldb : kc;
+    grid.z = (unsigned)S;
+    TimedLaunch tl(c, name, flops, bytes);
+    launch(c, g, q, grid);
+    const int64_t MN = g.M * g.N;
+    unsigned rb = (unsigned)std::min<int64_t>((MN + 255) / 256, 4096);
+    if (g.dt == DType::F64)
+      hipLaunchKernelGGL(splitk_reduce<double>, dim3(rb), dim3(256), 0, c.stream, g.M, g.N, S, (const double*)ws->p,
+                         g.alpha, g.beta, (double*)g.C, g.ldc);
+    else if (g.dt == DType::C128)
+      hipLaunchKernelGGL(splitk_reduce<double2>, dim3(rb), dim3(256), 0, c.stream, g.M, g.N, S,
+                         (const double2*)ws->p, g.alpha, g.beta, (double2*)g.C, g.ldc);
+    else
+      hipLaunchKernelGGL(splitk_reduce<float>, dim3(rb), dim3(256), 0, c.stream, g.M, g.N, S, (const float*)ws->p,
+                         g.alpha, g.beta, (float*)g.C, g.ldc);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  TimedLaunch tl(c, name, flops, bytes);
+  launch(c, g, p, grid);
 }
 
 }  // namespace k

@@ -528,6 +528,43 @@ static void gesvd(Ctx& c, const Tensor& Ain, Tensor& U, Tensor& S, Tensor& V, bo
 }
 
 void svd_thin_raw(Ctx& c, const Tensor& X, Tensor& W, Tensor& s, Tensor& Z) { gesvd(c, X, W, s, Z, true); }
+
+// Thin one-sided-Jacobi SVD of a tall matrix (rocSOLVER gesvdj): X = U diag(S) V^H.
+static void gesvdj_thin(Ctx& c, const Tensor& Ain, Tensor& U, Tensor& S, Tensor& V) {
+  IPEPS_CHECK(Ain.nd() == 2 && Ain.dims[0] >= Ain.dims[1], "gesvdj: tall matrix expected");
+  const int64_t m = Ain.dims[0], n = Ain.dims[1];
+  Tensor A = clone(c, Ain);
+  U = c.empty(Ain.dt, {m, n});
+  S = c.empty(DType::F64, {n});
+  Tensor Vh = c.empty(Ain.dt, {n, n});
+  Tensor scratch = c.empty(DType::F64, {4});  // residual, n_sweeps, info
+  double* res = scratch.ptr<double>();
+  rocblas_int* nsw = (rocblas_int*)(res + 1);
+  rocblas_int* info = (rocblas_int*)(res + 2);
+  rocblas_handle h = (rocblas_handle)c.solver;
+  {
+    TimedLaunch tl(c, "rocsolver_gesvdj", 0.0, 0.0);
+    const double tol = 1e-15;
+    if (Ain.dt == DType::F64)
+      solver_check(rocsolver_dgesvdj(h, rocblas_svect_singular, rocblas_svect_singular, (rocblas_int)m, (rocblas_int)n,
+                                     A.ptr<double>(), (rocblas_int)m, tol, res, 100, nsw, S.ptr<double>(),
+                                     U.ptr<double>(), (rocblas_int)m, Vh.ptr<double>(), (rocblas_int)n, info),
+                   "rocsolver_dgesvdj");
+    else if (Ain.dt == DType::C128)
+      solver_check(rocsolver_zgesvdj(h, rocblas_svect_singular, rocblas_svect_singular, (rocblas_int)m, (rocblas_int)n,
+                                     (rocblas_double_complex*)A.data(), (rocblas_int)m, tol, res, 100, nsw,
+                                     S.ptr<double>(), (rocblas_double_complex*)U.data(), (rocblas_int)m,
+                                     (rocblas_double_complex*)Vh.data(), (rocblas_int)n, info),
+                   "rocsolver_zgesvdj");
+    else
+      throw Error(IPEPS_ERR_ARG, "gesvdj: F64 or C128 only");
+  }
+  int hinfo = 0;
+  HIP_CHECK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  if (hinfo != 0) throw Error(IPEPS_ERR_SOLVER, "gesvdj did not converge, info=" + std::to_string(hinfo));
+  V = permute_raw(c, Vh, {1, 0}, true);
+}
 void svd_full_raw(Ctx& c, const Tensor& X, Tensor& U, Tensor& s, Tensor& V) { gesvd(c, X, U, s, V, false); }
 
 Tensor orth(Ctx& c, const Tensor& Yin) {
@@ -617,7 +654,7 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     }
     Tensor Bt = to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt);  // nc x kk = M^H Q
     Tensor W, s, X;
-    gesvd(c, Bt, W, s, X, true);  // Bt = W diag(s) X^H  ->  M ~ (Q X) diag(s) W^H
+    gesvdj_thin(c, Bt, W, s, X);  // Bt = W diag(s) X^H  ->  M ~ (Q X) diag(s) W^H
     Uf = matmul_raw(c, Q, false, false, X, false, false);
     Sf = s;
     Vf = W;
@@ -648,10 +685,12 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     R1k.node.reset();
     R2k.node.reset();
     const double eta = o.eta;
-    // Factors the adjoint works with: all k thin triplets (full mode, the
-    // reference svd_back on svd1's output) or the kept n (randomized mode,
-    // discarded triplets treated as zero).
-    Tensor Ub = full ? Uf : Un, Vb = full ? Vf : Vn, Sb = full ? Sf : Sn;
+    // The adjoint works with every computed triplet: all min(m, nc) thin
+    // triplets in full mode (the reference svd_back on svd1's output), the k
+    // oversampled ones in randomized mode, where the couplings F_ij between
+    // kept and computed-but-discarded triplets are exact and only triplets
+    // beyond k are treated as zero.
+    Tensor Ub = Uf, Vb = Vf, Sb = Sf;
     base->back = [=](Ctx& c, Node& b) {
       // svd_back (src/utility/autodiff.jl:41-75) for U (m x kb), V (nc x kb):
       //   Mbar = U G V^H + (dU - U U^H dU) Sinv V^H + U Sinv (dV - V V^H dV)^H

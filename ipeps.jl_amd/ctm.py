@@ -72,7 +72,8 @@ class Context:
         return live.value, peak.value
 
     def set_timing(self, on=True):
-        check(lib().ipeps_ctx_set_timing(self.h, int(bool(on))))
+        """on: False/0 off, True/1 per kernel, 2 per kernel and GEMM shape."""
+        check(lib().ipeps_ctx_set_timing(self.h, int(on)))
 
     def kernel_stats(self, reset=True):
         """{kernel: (launches, total_ms, algorithmic_flops, algorithmic_bytes)}."""
@@ -110,7 +111,7 @@ def _hpair(H):
 
 
 ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "canonical_edge",
-               "mixed", "energy_fp64")
+               "mixed", "energy_fp64", "checkpoint")
 
 
 class CTMTensors:
