@@ -109,6 +109,18 @@ int ipeps_gs_energy_grad(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const
                          int ad_max_iter, double tol, const char* options, double* E_out, void* grad_out,
                          int* iters_out);
 
+/* get_es_grad (src/optimize/optim_es.jl:167-194) for one tangent tensor B:
+ * env must be a converged, normalised ground-state environment in IPEPS_C128;
+ * the momentum is (px, py) pi as optim_es's arguments (optim_es.jl:124-125).
+ * Converges the nested environment [X, X_B, X_Bd, X_BB] from env with
+ * conv = get_es_energy / get_all_norm[1] (min/max_iter, tol), differentiates
+ * ad_max_iter nested steps + get_es_energy w.r.t. B and writes gH (dRe(E4)/dB),
+ * envB (get_all_norm's B environment), E4 and Nb.  B, gH, envB: (D,D,D,D,d)
+ * ComplexF64. */
+int ipeps_es_grad(ipeps_env* env, const double* H1, const double* H2, int64_t hdim, double px, double py,
+                  const void* B, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol, void* gH_out,
+                  void* envB_out, double* E4_out, double* Nb_out);
+
 /* tcon (src/utility/wrapper.jl:15-23): ncon-style pairwise contraction on the
  * device.  Positive labels are summed, negative ones are output legs -1, -2,
  * ...  dimsC receives the output dims (room for 16), C_host the result

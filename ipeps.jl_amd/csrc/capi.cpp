@@ -344,6 +344,24 @@ int ipeps_gs_energy_grad(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const
   });
 }
 
+int ipeps_es_grad(ipeps_env* env, const double* H1, const double* H2, int64_t hdim, double px, double py,
+                  const void* B, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol, void* gH_out,
+                  void* envB_out, double* E4_out, double* Nb_out) {
+  return guard([&] {
+    IPEPS_CHECK(env && B && chi > 0, "bad arguments");
+    IPEPS_CHECK(env->dtype == IPEPS_C128, "es_grad: the environment must be complex (IPEPS_C128)");
+    Ctx& c = *env->ctx->c;
+    Tensor tB = upload(c, DType::C128, DType::C128, env->env.A.dims, B);
+    const double pi = 3.14159265358979323846;
+    auto r = ctm::es_grad(c, env->env, tB, upload_H(c, H1, hdim), upload_H(c, H2, hdim), px * pi, py * pi, chi,
+                          min_iter, max_iter, ad_max_iter, tol);
+    if (gH_out) download(c, r.gH, DType::C128, gH_out);
+    if (envB_out) download(c, r.envB, DType::C128, envB_out);
+    if (E4_out) *E4_out = r.E4;
+    if (Nb_out) { Nb_out[0] = r.Nb[0]; Nb_out[1] = r.Nb[1]; }
+  });
+}
+
 int ipeps_tcon(ipeps_ctx* ctx, int dtype, int ndA, const int64_t* dimsA, const void* A, const int* la, int ndB,
                const int64_t* dimsB, const void* B, const int* lb, int64_t* dimsC, int* ndC, void* C_host,
                int64_t C_cap) {

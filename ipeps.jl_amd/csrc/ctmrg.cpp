@@ -82,7 +82,7 @@ void finish(Ctx& c, Env& e, int ci, const Tensor& C_, int cj, const Tensor& Cb, 
   e.C[cj] = op::renormalize(c, Cb);
 }
 
-std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi) {
+std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   const Tensor &C1 = e.C[0], &C4 = e.C[3], &E1 = e.E[0], &E3 = e.E[2], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
   // get_projector_left (src/ctm/ctmrg.jl:137-160)
   Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});
@@ -96,6 +96,10 @@ std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi) {
   Tensor LB = tcon(c, CEEAb, {-1, -4, 1, 2, -2, -5, 3}, Ad, {-3, 1, 2, -6, 3});
   Tensor R2 = mat(c, LB, 3);
   Proj p = get_projector(c, e, R1, R2, chi);
+  if (only) {
+    *only = p;
+    return p.s;
+  }
   // proj_left (src/ctm/ctmrg.jl:267-291)
   Tensor C1E1p = tcon(c, C1, {-2, 1}, E1, {1, -1, -3, -4});
   Tensor nC1 = tr2(c, op::matmul(c, op::reshape(c, C1E1p, {C1E1p.dims[0], C1E1p.numel() / C1E1p.dims[0]}), p.P1));
@@ -107,7 +111,7 @@ std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi) {
   return p.s;
 }
 
-void right_rg(Ctx& c, Env& e, int64_t chi) {
+void right_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   const Tensor &C2 = e.C[1], &C3 = e.C[2], &E1 = e.E[0], &E2 = e.E[1], &E3 = e.E[2], &A = e.A, &Ad = e.Ad;
   // get_projector_right (src/ctm/ctmrg.jl:176-198)
   Tensor C2E1 = tcon(c, C2, {1, -2}, E1, {-1, 1, -3, -4});
@@ -121,6 +125,10 @@ void right_rg(Ctx& c, Env& e, int64_t chi) {
   Tensor RB = tcon(c, CEEAb, {-1, -4, -2, -5, 1, 2, 3}, Ad, {-3, -6, 1, 2, 3});
   Tensor R2 = mat(c, RB, 3);
   Proj p = get_projector(c, e, R1, R2, chi);
+  if (only) {
+    *only = p;
+    return;
+  }
   // proj_right (src/ctm/ctmrg.jl:293-318)
   Tensor C2E1p = tcon(c, C2, {1, -2}, E1, {-1, 1, -3, -4});
   Tensor nC2 = op::matmul(c, op::reshape(c, C2E1p, {C2E1p.dims[0], C2E1p.numel() / C2E1p.dims[0]}), p.P1);
@@ -131,7 +139,7 @@ void right_rg(Ctx& c, Env& e, int64_t chi) {
   finish(c, e, 1, nC2, 2, nC3, 1, nE2);
 }
 
-void top_rg(Ctx& c, Env& e, int64_t chi) {
+void top_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   const Tensor &C1 = e.C[0], &C2 = e.C[1], &E1 = e.E[0], &E2 = e.E[1], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
   // get_projector_top (src/ctm/ctmrg.jl:214-236)
   Tensor C1E4 = tcon(c, C1, {1, -2}, E4, {1, -1, -3, -4});
@@ -145,6 +153,10 @@ void top_rg(Ctx& c, Env& e, int64_t chi) {
   Tensor UR = tcon(c, CEEAb, {-1, -4, -2, -5, 1, 2, 3}, Ad, {1, -3, -6, 2, 3});
   Tensor R2 = mat(c, UR, 3);
   Proj p = get_projector(c, e, R1, R2, chi);
+  if (only) {
+    *only = p;
+    return;
+  }
   // proj_top (src/ctm/ctmrg.jl:320-345)
   Tensor C1E4p = tcon(c, C1, {1, -2}, E4, {1, -1, -3, -4});
   Tensor nC1 = op::matmul(c, op::reshape(c, C1E4p, {C1E4p.dims[0], C1E4p.numel() / C1E4p.dims[0]}), p.P1);
@@ -155,7 +167,7 @@ void top_rg(Ctx& c, Env& e, int64_t chi) {
   finish(c, e, 0, nC1, 1, nC2, 0, nE1);
 }
 
-void bottom_rg(Ctx& c, Env& e, int64_t chi) {
+void bottom_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   const Tensor &C3 = e.C[2], &C4 = e.C[3], &E2 = e.E[1], &E3 = e.E[2], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
   // get_projector_bottom (src/ctm/ctmrg.jl:238-252... via 226-250)
   Tensor C4E4 = tcon(c, C4, {1, -2}, E4, {-1, 1, -3, -4});
@@ -169,6 +181,10 @@ void bottom_rg(Ctx& c, Env& e, int64_t chi) {
   Tensor BR = tcon(c, CEEAb, {-1, -4, -2, -5, 1, 2, 3}, Ad, {-3, -6, 1, 2, 3});
   Tensor R2 = tr2(c, mat(c, BR, 3));
   Proj p = get_projector(c, e, R1, R2, chi);
+  if (only) {
+    *only = p;
+    return;
+  }
   // proj_bottom (src/ctm/ctmrg.jl:347-375)
   Tensor C4E4p = tcon(c, C4, {1, -2}, E4, {-1, 1, -3, -4});
   Tensor nC4 = op::matmul(c, op::reshape(c, C4E4p, {C4E4p.dims[0], C4E4p.numel() / C4E4p.dims[0]}), p.P1);
@@ -221,6 +237,20 @@ Tensor dm_v(Ctx& c, const Tensor& C1, const Tensor& C2, const Tensor& C3, const 
 }
 
 }  // namespace
+
+void projector(Ctx& c, const Env& e0, int dir, int64_t chi, Tensor& P1, Tensor& P2, std::vector<double>& s) {
+  Env e = e0;  // shares the tensors; the *_rg functions stop after the projector
+  Proj p;
+  switch (dir) {
+    case 0: left_rg(c, e, chi, &p); break;
+    case 1: right_rg(c, e, chi, &p); break;
+    case 2: top_rg(c, e, chi, &p); break;
+    default: bottom_rg(c, e, chi, &p); break;
+  }
+  P1 = p.P1;
+  P2 = p.P2;
+  s = p.s;
+}
 
 void init_ctm1(Ctx& c, Env& e) {
   const Tensor &A = e.A, &Ad = e.Ad;

@@ -32,6 +32,10 @@ struct Env {
 // Boundary initialisation C_i, E_i from A and Ad (src/ctm/ctm_tensor.jl:97-130).
 void init_ctm1(Ctx& c, Env& e);
 
+// Projectors (P1, P2) and spectrum of one move from the plain environment:
+// dir 0 left, 1 right, 2 top, 3 bottom (src/ctm/ctmrg.jl:122-265).
+void projector(Ctx& c, const Env& e, int dir, int64_t chi, Tensor& P1, Tensor& P2, std::vector<double>& s);
+
 // One CTMRG step left -> right -> top -> bottom (src/ctm/ctmrg.jl:57-64).
 // Returns the left move's truncated spectrum.
 std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi);
@@ -72,6 +76,41 @@ struct GradResult {
 };
 GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options& opt, const Tensor& H1,
                           const Tensor& H2, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol);
+
+// ---------------------------------------------------------------------------
+// Excited states: nested tensors [X, X_B, X_Bd, X_BB]
+// (src/nested_tensor/nested_tensor.jl, src/ctm/ctm_tensor.jl:20-55)
+// ---------------------------------------------------------------------------
+struct NT {
+  std::array<Tensor, 4> c;  // undefined Tensor = EmptyT
+  bool nested = true;
+};
+
+struct NestedEnv {
+  Env g;                        // plain (ground-state) parts: A, Ad, C, E
+  Tensor B, Bd;                 // tangent tensor and its conjugate
+  std::array<Tensor, 4> BC, BdC, BBC, BE, BdE, BBE;
+  double px = 0.0, py = 0.0;    // momentum phases (already multiplied by pi)
+};
+
+// Nested rg_step: projectors from the plain parts, nested renormalisation and
+// the momentum shift of up_* (src/ctm/ctmrg.jl:66-120, 267-491).
+std::vector<double> nested_rg_step(Ctx& c, NestedEnv& e, int64_t chi);
+// get_es_energy (src/graph/evaluation.jl:31-45): Re E[4]; when recording the
+// returned FP64[4] tensor's node expects the seed dL/dRe(E4) = 1.
+Tensor es_energy(Ctx& c, const NestedEnv& e, const Tensor& H1, const Tensor& H2);
+// get_all_norm (src/graph/evaluation.jl:61-77): Nb (re, im) and envB.
+void all_norm(Ctx& c, const NestedEnv& e, double Nb[2], Tensor& envB);
+
+struct EsGrad {
+  Tensor gH, envB;   // (D,D,D,D,d) C128: d Re E4 / dB and the B environment
+  double E4 = 0.0, Nb[2] = {0, 0};
+  int conv_iters = 0;
+};
+// get_es_grad (src/optimize/optim_es.jl:167-194) from a converged, normalised
+// ground-state environment g and one tangent tensor B.
+EsGrad es_grad(Ctx& c, const Env& g, const Tensor& B, const Tensor& H1, const Tensor& H2, double px, double py,
+               int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol);
 
 // One differentiated step from a given environment: forward rg_step on the
 // tape, the energy and its VJP, then the step VJP; the A cotangent is

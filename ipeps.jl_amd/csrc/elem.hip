@@ -465,11 +465,88 @@ __global__ void energy2_back_kernel(int64_t m, const void* rh, const void* rv, c
   else ((double*)(which ? gv : gh))[e] = g.x;
 }
 
+template <typename T>
+__global__ void scale_cplx_kernel(int64_t n, double zr, double zi, const T* x, T* y);
+template <>
+__global__ void scale_cplx_kernel<double2>(int64_t n, double zr, double zi, const double2* x, double2* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    y[i] = cmul(make_double2(zr, zi), x[i]);
+}
+template <>
+__global__ void scale_cplx_kernel<double>(int64_t n, double zr, double zi, const double* x, double* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] = zr * x[i];
+}
+template <>
+__global__ void scale_cplx_kernel<float>(int64_t n, double zr, double zi, const float* x, float* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    y[i] = (float)(zr * x[i]);
+}
+
+__global__ void es_energy_kernel(int64_t m, const double2* rh1, const double2* rv1, const double2* rh4,
+                                 const double2* rv4, const double2* H1, const double2* H2, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double Nh = 0, Nv = 0;
+  double2 ah = make_double2(0, 0), av = make_double2(0, 0);
+  for (int64_t i = 0; i < m; ++i) {
+    Nh += rh1[i + i * m].x;
+    Nv += rv1[i + i * m].x;
+    for (int64_t j = 0; j < m; ++j) {
+      double2 t1 = cmul(H1[i + j * m], rh4[j + i * m]);
+      double2 t2 = cmul(H2[i + j * m], rv4[j + i * m]);
+      ah.x += t1.x; ah.y += t1.y;
+      av.x += t2.x; av.y += t2.y;
+    }
+  }
+  out[0] = ah.x / Nh + av.x / Nv;
+  out[1] = ah.y / Nh + av.y / Nv;
+  out[2] = Nh;
+  out[3] = Nv;
+}
+
+__global__ void es_energy_back_kernel(int64_t m, const double2* H1, const double2* H2, const double* out, double2* gh,
+                                      double2* gv) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m * m) return;
+  int64_t r = t % m, c = t / m;
+  // d E4 / d rho_rc = H_cr / N ; cotangent for L = Re E4 is its conjugate
+  gh[t] = cconj(make_double2(H1[c + r * m].x / out[2], H1[c + r * m].y / out[2]));
+  gv[t] = cconj(make_double2(H2[c + r * m].x / out[3], H2[c + r * m].y / out[3]));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+void scale_cplx(Ctx& c, DType dt, int64_t n, double zr, double zi, const void* x, void* y) {
+  if (n <= 0) return;
+  IPEPS_CHECK(dt == DType::C128 || zi == 0.0, "scale_cplx: complex factor on real data");
+  if (dt == DType::C128)
+    hipLaunchKernelGGL(scale_cplx_kernel<double2>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, zr, zi,
+                       (const double2*)x, (double2*)y);
+  else if (dt == DType::F64)
+    hipLaunchKernelGGL(scale_cplx_kernel<double>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, zr, zi,
+                       (const double*)x, (double*)y);
+  else
+    hipLaunchKernelGGL(scale_cplx_kernel<float>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, zr, zi,
+                       (const float*)x, (float*)y);
+  HIP_CHECK(hipGetLastError());
+}
+
+void es_energy(Ctx& c, int64_t m, const void* rh1, const void* rv1, const void* rh4, const void* rv4, const double* H1,
+               const double* H2, double* out) {
+  hipLaunchKernelGGL(es_energy_kernel, dim3(1), dim3(64), 0, c.stream, m, (const double2*)rh1, (const double2*)rv1,
+                     (const double2*)rh4, (const double2*)rv4, (const double2*)H1, (const double2*)H2, out);
+  HIP_CHECK(hipGetLastError());
+}
+
+void es_energy_back(Ctx& c, int64_t m, const double* H1, const double* H2, const double* out, void* g_rh4,
+                    void* g_rv4) {
+  int64_t n = m * m;
+  hipLaunchKernelGGL(es_energy_back_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, m,
+                     (const double2*)H1, (const double2*)H2, out, (double2*)g_rh4, (double2*)g_rv4);
+  HIP_CHECK(hipGetLastError());
+}
 static const char* dtype_name(DType dt) {
   return dt == DType::F64 ? "double" : dt == DType::C128 ? "HIP_vector_type<double, 2u>" : "float";
 }

@@ -101,5 +101,17 @@ void energy2(Ctx& c, DType dt, int64_t m, const void* rho_h, const void* rho_v, 
 void energy2_back(Ctx& c, DType dt, int64_t m, const void* rho_h, const void* rho_v,
                   const double* H1, const double* H2, void* grho_h, void* grho_v);
 
+// y = z x for a complex scalar z (C128 data; F64/F32 data need zi == 0).
+void scale_cplx(Ctx& c, DType dt, int64_t n, double zr, double zi, const void* x, void* y);
+// Excited-state energy (src/graph/evaluation.jl:31-45) from the component-1
+// and component-4 density matrices (C128, m x m):
+//   out = [Re E4, Im E4, Nh, Nv] with Nh = Re tr(rh1), Nv = Re tr(rv1),
+//   E4 = tr(H1 rh4)/Nh + tr(H2 rv4)/Nv.
+void es_energy(Ctx& c, int64_t m, const void* rh1, const void* rv1, const void* rh4, const void* rv4, const double* H1,
+               const double* H2, double* out);
+// Cotangents of rh4, rv4 for L = Re E4 (Nh, Nv from out): H^H / N.
+void es_energy_back(Ctx& c, int64_t m, const double* H1, const double* H2, const double* out, void* g_rh4,
+                    void* g_rv4);
+
 }  // namespace k
 }  // namespace ipeps

@@ -161,6 +161,36 @@ Tensor cast(Ctx& c, const Tensor& t, DType dt) {
   return r;
 }
 
+Tensor add(Ctx& c, const Tensor& a, const Tensor& b) {
+  IPEPS_CHECK(a.dt == b.dt && a.numel() == b.numel(), "add: operand mismatch");
+  Tensor r = clone(c, a);
+  r.dims = a.dims;
+  k::axpy(c, a.dt, a.numel(), 1.0, b.data(), r.data());
+  if (needs(a) || needs(b)) {
+    auto na = a.node, nb = b.node;
+    auto da = a.dims, db = b.dims;
+    r.node = make_node([na, nb, da, db](Ctx& c, Node& n) {
+      if (na) acc(c, na, ipeps::reshape(n.grad, da));
+      if (nb) acc(c, nb, ipeps::reshape(n.grad, db));
+    });
+  }
+  return r;
+}
+
+Tensor scale_cplx(Ctx& c, const Tensor& t, double zr, double zi) {
+  Tensor r = c.empty(t.dt, t.dims);
+  k::scale_cplx(c, t.dt, t.numel(), zr, zi, t.data(), r.data());
+  if (needs(t)) {
+    auto in = t.node;
+    r.node = make_node([in, zr, zi](Ctx& c, Node& n) {
+      Tensor g = c.empty(n.grad.dt, n.grad.dims);
+      k::scale_cplx(c, g.dt, g.numel(), zr, -zi, n.grad.data(), g.data());  // conj(z) * ybar
+      acc(c, in, g);
+    });
+  }
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // contraction planner
 // ---------------------------------------------------------------------------
@@ -424,6 +454,42 @@ Tensor renormalize(Ctx& c, const Tensor& t) {
       Tensor g = c.empty(x.dt, x.dims);
       k::renorm_back(c, x.dt, x.numel(), x.data(), n.grad.data(), mv, iv, dotv, g.data());
       acc(c, in, g);
+    });
+  }
+  return r;
+}
+
+Tensor renormalize_by(Ctx& c, const Tensor& x, const Tensor& ref) {
+  auto sc = std::make_shared<Buf>();
+  sc->bytes = 32;
+  sc->p = c.pool->alloc(32);
+  sc->pool = c.pool.get();
+  double* m = (double*)sc->p;
+  int64_t* idx = (int64_t*)((char*)sc->p + 8);
+  k::maxabs(c, ref.dt, ref.numel(), ref.data(), m, idx);
+  Tensor r = c.empty(x.dt, x.dims);
+  k::scale_by_dev(c, x.dt, x.numel(), x.data(), r.data(), m, 0);
+  if (needs(x) || needs(ref)) {
+    auto nx = x.node, nr = ref.node;
+    Tensor xk = x, rk = ref;
+    xk.node.reset();
+    rk.node.reset();
+    r.node = make_node([nx, nr, xk, rk, sc](Ctx& c, Node& n) {
+      double* mv = (double*)sc->p;
+      int64_t* iv = (int64_t*)((char*)sc->p + 8);
+      double* dotv = (double*)((char*)sc->p + 16);
+      if (nx) {
+        Tensor g = c.empty(xk.dt, xk.dims);
+        k::scale_by_dev(c, xk.dt, xk.numel(), n.grad.data(), g.data(), mv, 0);
+        acc(c, nx, g);
+      }
+      if (nr) {
+        k::re_dot(c, xk.dt, xk.numel(), n.grad.data(), xk.data(), dotv);
+        Tensor z = c.zeros(rk.dt, rk.dims);
+        Tensor g = c.empty(rk.dt, rk.dims);
+        k::renorm_back(c, rk.dt, rk.numel(), rk.data(), z.data(), mv, iv, dotv, g.data());
+        acc(c, nr, g);
+      }
     });
   }
   return r;

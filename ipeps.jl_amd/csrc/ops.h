@@ -76,6 +76,9 @@ Tensor conj(Ctx& c, const Tensor& t);
 // X ./ maximum(abs.(X)) with the gradient of maximum at the first maximum
 // in column-major order (src/utility/wrapper.jl:57).
 Tensor renormalize(Ctx& c, const Tensor& t);
+// x ./ maximum(abs.(ref)): the nested renormalisation of the B components by
+// the plain component's factor (src/nested_tensor/nested_tensor.jl:41-46).
+Tensor renormalize_by(Ctx& c, const Tensor& x, const Tensor& ref);
 // X diag(s) (rowwise=false) or diag(s) X (rowwise=true), s a real FP64 vector.
 Tensor scale_vec(Ctx& c, const Tensor& X, const Tensor& s, bool rowwise);
 // s .^ (-1/2) for a real FP64 vector.
@@ -83,6 +86,8 @@ Tensor inv_sqrt(Ctx& c, const Tensor& s);
 Tensor matmul(Ctx& c, const Tensor& A, const Tensor& B);       // A * B
 Tensor adjoint_matmul(Ctx& c, const Tensor& A, const Tensor& B);  // A^H * B
 Tensor cast(Ctx& c, const Tensor& t, DType dt);                   // differentiable F64 <-> F32
+Tensor add(Ctx& c, const Tensor& a, const Tensor& b);             // a + b
+Tensor scale_cplx(Ctx& c, const Tensor& t, double zr, double zi); // z t, z a host constant
 
 struct SVDResult {
   Tensor U, S, V;   // U: N x n, S: n (FP64), V: N x n

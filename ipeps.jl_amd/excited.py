@@ -1,0 +1,126 @@
+"""Excited states (nested-tensor CTMRG) over the HIP engine.
+
+Mirrors src/optimize/optim_es.jl and src/graph/expectation.jl:
+
+* ``get_es_grad(ts, H, Bi, px, py, cfg)``   optim_es.jl:167-194 (GPU)
+* ``optim_es(ts, H, basis, px, py, cfg)``   optim_es.jl:91-165: effH, effN,
+  envB columns for every tangent basis vector (GPU per column)
+* ``normalize_gs``, ``substract_gs_energy``, ``get_tangent_basis``
+                                            optim_es.jl:212-258
+* ``compute_es(effH, effN, nrmB_cut)``      expectation.jl:13-56: the small
+  generalised eigenproblem, host FP64 (LAPACK via scipy), as in the reference
+* ``make_es_path(step)``                    optim_es.jl:1-20
+"""
+import ctypes as C
+
+import numpy as np
+import scipy.linalg as sla
+
+from ._lib import C128, check, lib
+from .ctm import CTMTensors, _dp, _hpair, _ptr, get_gs_energy, get_gs_norm
+
+
+def get_es_grad(ts, H, Bi, px, py, cfg):
+    """(gradH, gradN) flattened in Julia (column-major) order, plus (E4, Nb).
+
+    ``ts``: converged, normalised complex ground-state ``CTMTensors``;
+    ``H``: the E_gs-shifted two-site terms; (px, py) in units of pi.
+    """
+    if ts.code != C128:
+        raise ValueError("get_es_grad needs a complex (c128) environment")
+    shape = ts.A.shape
+    B = np.asfortranarray(np.asarray(Bi, dtype=np.complex128).reshape(shape, order="F"))
+    (h1, h2), m = _hpair(H)
+    gH = np.empty(shape, dtype=np.complex128, order="F")
+    envB = np.empty(shape, dtype=np.complex128, order="F")
+    E4 = np.zeros(1)
+    Nb = np.zeros(2)
+    check(lib().ipeps_es_grad(ts.h, _ptr(h1), _ptr(h2), m, float(px), float(py), _ptr(B), int(cfg.get("chi", 30)),
+                              int(cfg.get("min_iter", 4)), int(cfg.get("max_iter", 20)), int(cfg.get("ad_max_iter", 4)),
+                              float(cfg.get("rg_tol", 1e-6)), _ptr(gH), _ptr(envB), _dp(E4), _dp(Nb)))
+    return gH.reshape(-1, order="F"), envB.reshape(-1, order="F"), (float(E4[0]), complex(Nb[0], Nb[1]))
+
+
+def optim_es(ts, H, basis, px, py, cfg, columns=None):
+    """effH, effN, envB for the tangent basis at momentum (px, py) pi (optim_es.jl:127-153)."""
+    basis = np.asarray(basis, dtype=np.complex128)
+    n = basis.shape[1]
+    effH = np.zeros((n, n), dtype=np.complex128)
+    effN = np.zeros((n, n), dtype=np.complex128)
+    envB = np.zeros(basis.shape, dtype=np.complex128)
+    for i in (range(n) if columns is None else columns):
+        gH, gN, _ = get_es_grad(ts, H, basis[:, i], px, py, cfg)
+        envB[:, i] = gN
+        effH[:, i] = basis.conj().T @ gH / 2
+        effN[:, i] = basis.conj().T @ gN
+    return effH, effN, envB
+
+
+def normalize_gs(ts):
+    """optim_es.jl:212-222: A -> A / sqrt|<A|env|A>|."""
+    nrm = get_gs_norm(ts)
+    A = ts.A / np.sqrt(abs(nrm))
+    ts.set_A(A)
+    return ts
+
+
+def substract_gs_energy(ts, H):
+    """optim_es.jl:224-234: H_i - (E_gs/2) 1."""
+    E, _ = get_gs_energy(ts, H)
+    e = E.real / 2
+    return [np.asarray(h, dtype=np.complex128) - e * np.eye(np.asarray(h).shape[0]) for h in H]
+
+
+def get_tangent_basis(ts):
+    """optim_es.jl:236-258 ("default" basis): nullspace of the single-site
+    environment contracted with Ad, shape (D^4 d, D^4 d - 1)."""
+    from .ctm import tcon
+    Cs, Es = ts.Cs, ts.Es
+    C1E1 = tcon([Cs[0], Es[0]], [[-1, 1], [1, -2, -3, -4]])
+    CEC2 = tcon([C1E1, Cs[1]], [[-1, 1, -3, -4], [1, -2]])
+    CECE4 = tcon([CEC2, Es[3]], [[1, -2, -3, -5], [1, -1, -4, -6]])
+    C4E3 = tcon([Cs[3], Es[2]], [[-1, 1], [1, -2, -3, -4]])
+    CEC3 = tcon([C4E3, Cs[2]], [[-1, 1, -3, -4], [-2, 1]])
+    CECE2 = tcon([CEC3, Es[1]], [[-1, 1, -3, -5], [-2, 1, -4, -6]])
+    ndm = tcon([CECE4, CECE2], [[1, 2, -1, -2, -5, -6], [1, 2, -3, -4, -7, -8]])
+    Ad = np.conj(ts.A)
+    ndm_Ad = tcon([ndm, Ad], [[-1, -2, -3, -4, 1, 2, 3, 4], [1, 2, 3, 4, -5]])
+    row = ndm_Ad.reshape(1, -1, order="F")
+    return sla.null_space(row)
+
+
+def compute_es(effH, effN, nrmB_cut=1e-3):
+    """expectation.jl:13-56: Hermitise, project on the leading norm
+    eigenvectors, solve H2 v = e N2 v.  Returns (es ascending, vecs, P)."""
+    H = (effH + effH.conj().T) / 2
+    N = (effN + effN.conj().T) / 2
+    ev_N, P = np.linalg.eigh(N)
+    idx = np.argsort(ev_N.real)[::-1]
+    ev_N = ev_N[idx]
+    if isinstance(nrmB_cut, (int, np.integer)):
+        selected = ev_N > ev_N[nrmB_cut]
+    else:
+        selected = (ev_N / ev_N.max()) > nrmB_cut
+    P = P[:, idx][:, selected]
+    N2 = P.conj().T @ N @ P
+    H2 = P.conj().T @ H @ P
+    H2 = (H2 + H2.conj().T) / 2
+    N2 = (N2 + N2.conj().T) / 2
+    es, vecs = sla.eigh(H2, N2)
+    ixs = np.argsort(es.real)
+    return es[ixs], vecs[:, ixs], P
+
+
+def make_es_path(step=0.1):
+    """optim_es.jl:1-20: the (kx, ky) path in units of pi."""
+    n = int(round(1 / step))
+    kx1 = np.ones(n - 1)
+    ky1 = np.arange(1, n) * step
+    kx2 = 1 - np.arange(0, n + 1) * step
+    ky2 = 1 - np.arange(0, n + 1) * step
+    kx3 = np.arange(1, n) * step
+    ky3 = np.zeros(n - 1)
+    m = int(round(0.5 / step))
+    kx4 = 1 - np.arange(0, m) * step
+    ky4 = np.arange(0, m) * step
+    return np.concatenate([kx1, kx2, kx3, kx4]), np.concatenate([ky1, ky2, ky3, ky4])

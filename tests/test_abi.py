@@ -46,3 +46,37 @@ def test_models_match_oracle(pkg):
         for x, y in zip(a, b):
             np.testing.assert_array_equal(x, y.numpy())
     np.testing.assert_array_equal(M.init_hb_gs(2, "XX"), O.init_hb_gs(2, "XX").numpy())
+
+
+def test_compute_es_generalised_eigenproblem(pkg):
+    """compute_es on the reference's effH/effN: H2 v = e N2 v, v^H N2 v = 1."""
+    import numpy as np
+    from conftest import GOLDEN
+    es = np.load(os.path.join(GOLDEN, "es_ising_D2_X32.npz"))
+    H, Nm = es["effH_0.5_0"], es["effN_0.5_0"]
+    e, v, P = pkg.compute_es(H, Nm, 1e-3)
+    Hh = (H + H.conj().T) / 2
+    Nh = (Nm + Nm.conj().T) / 2
+    H2 = P.conj().T @ Hh @ P
+    N2 = P.conj().T @ Nh @ P
+    assert np.all(np.diff(e) >= -1e-12)
+    np.testing.assert_allclose(H2 @ v, (N2 @ v) * e, atol=1e-9)
+    np.testing.assert_allclose(v.conj().T @ N2 @ v, np.eye(v.shape[1]), atol=1e-9)
+    ev = np.linalg.eigvalsh(Nh)
+    assert P.shape[1] == int(np.sum(ev / ev.max() > 1e-3))
+
+
+def test_make_es_path_matches_files(pkg):
+    """The 34 momenta of make_es_path(0.1) are the es_*.jld2 files the reference wrote."""
+    import numpy as np
+    from conftest import GOLDEN
+    es = np.load(os.path.join(GOLDEN, "es_ising_D2_X32.npz"))
+    kx, ky = pkg.make_es_path(0.1)
+    tags = {k[5:] for k in es.files if k.startswith("effH_")}
+
+    def fmt(x):
+        x = round(float(x), 10)
+        return str(int(x)) if x == int(x) else repr(x)
+
+    got = {f"{fmt(a)}_{fmt(b)}" for a, b in zip(kx, ky)}
+    assert got == tags
