@@ -42,8 +42,53 @@ def test_es_grad_matches_g5_and_oracle(pkg):
         assert np.abs(gN - es["envB_0.5_0"][:, i]).max() < 1e-6
         ogH, ogN = N.get_es_grad(ots, Ho, _t(basis[:, i]))
         oeffH = (_t(basis).mH @ ogH / 2).numpy()
-        assert np.abs(effH - oeffH).max() < 1e-9 * max(1.0, np.abs(oeffH).max())
-        assert np.abs(gN - ogN.numpy()).max() < 1e-9
+        # The kept spectrum of this chi=32 Ising environment reaches 2.5e-12 of s_max, so the last
+        # kept singular vectors are fixed only to ~eps/s and are scaled by S^-1/2 in the projectors;
+        # the nested iteration does not converge (conv stalls near 1e-4) and carries that through 20
+        # steps.  Swapping the oracle's own LAPACK driver (gesdd -> gesvd) already moves effH by
+        # 2e-9 after one step; GPU (rocSOLVER gesvd) vs oracle ends at 3e-7 relative after 24 steps.
+        assert np.abs(effH - oeffH).max() < 2e-6 * max(1.0, np.abs(oeffH).max())
+        assert np.abs(gN - ogN.numpy()).max() < 1e-7
+
+
+def _fixed_steps_case(which):
+    if which == "random":
+        A = O.seeded_tensor((2, 2, 2, 2, 2), 7).to(torch.complex128)
+        cfg0 = dict(chi=8, min_iter=4, max_iter=8, rg_tol=1e-12)
+        with torch.no_grad():
+            _, _, plain, _ = O.compute_gs_energy(A, O.ising(2.0), cfg0)
+        gen = np.random.default_rng(3)
+        Bv = gen.standard_normal(32) + 1j * gen.standard_normal(32)
+        return plain, Bv, 8, (0.3, 0.7)
+    e = np.load(os.path.join(GOLDEN, "env_ising_D2_X32.npz"))
+    plain = O.CTMTensors(_t(e["A"]), _t(e["Ad"]), [_t(e[f"C{i}"]) for i in range(1, 5)],
+                         [_t(e[f"E{i}"]) for i in range(1, 5)], dict(O.DEFAULT_CFG))
+    return plain, e["basis"][:, 0], 32, (0.5, 0.0)
+
+
+# tolerance (relative) per case: the random chi=8 state is well conditioned; the committed chi=32
+# Ising environment keeps singular values down to 2.5e-12 s_max, where two backward-stable SVDs
+# legitimately differ (oracle gesdd vs gesvd: 5.6e-10 after one step, 1.3e-9 after three).
+FIXED_TOL = {("random", 1): 1e-10, ("random", 2): 1e-10, ("random", 3): 1e-10, ("g5", 1): 5e-9, ("g5", 3): 1e-7}
+
+
+@pytest.mark.parametrize("which,steps", sorted(FIXED_TOL))
+def test_es_grad_fixed_steps_vs_oracle(pkg, which, steps):
+    """Fixed nested step counts (no stopping rule): gH / envB vs the oracle, on a random complex
+    D=2 chi=8 state and on the committed Ising D=2 chi=32 environment."""
+    plain, Bv, chi, (px, py) = _fixed_steps_case(which)
+    Hs = N.substract_gs_energy(plain, O.ising(2.0))
+    cfg = dict(chi=chi, min_iter=steps, max_iter=steps, ad_max_iter=1, rg_tol=0.0)
+    ts = pkg.CTMTensors(plain.A.numpy(), cfg)
+    ts.set_env([c.numpy() for c in plain.Cs], [x.numpy() for x in plain.Es])
+    gH, gN, (E4, Nb) = pkg.get_es_grad(ts, [h.numpy() for h in Hs], Bv, px, py, cfg)
+    ots = N.NestedCTM(plain.A, plain.Ad, plain.Cs, plain.Es, params=dict(cfg, px=px * math.pi, py=py * math.pi))
+    ogH, ogN = N.get_es_grad(ots, Hs, _t(Bv))
+    print(which, "steps", steps, "E4", E4, "|dgH|", np.abs(gH - ogH.numpy()).max(), "|dgN|",
+          np.abs(gN - ogN.numpy()).max())
+    tol = FIXED_TOL[(which, steps)]
+    assert np.abs(gH - ogH.numpy()).max() < tol * max(1.0, np.abs(ogH.numpy()).max())
+    assert np.abs(gN - ogN.numpy()).max() < tol * max(1.0, np.abs(ogN.numpy()).max())
 
 
 def test_optim_es_columns_and_bands(pkg):
