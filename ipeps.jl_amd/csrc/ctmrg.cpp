@@ -195,48 +195,67 @@ void bottom_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   finish(c, e, 3, nC4, 2, nC3, 2, nE3);
 }
 
-// ---- density matrices (src/graph/evaluation.jl:169-248) ---------------------
-Tensor dm_h(Ctx& c, const Tensor& C1, const Tensor& C2, const Tensor& C3, const Tensor& C4, const Tensor& E1l,
-            const Tensor& E1r, const Tensor& E2, const Tensor& E3l, const Tensor& E3r, const Tensor& E4,
-            const Tensor& Al, const Tensor& Ar, const Tensor& Adl, const Tensor& Adr) {
-  Tensor C1E1 = tcon(c, C1, {-1, 1}, E1l, {1, -2, -3, -4});
-  Tensor LU = tcon(c, C1E1, {-1, -2, -3, 1}, Adl, {1, -4, -5, -6, -7});
-  Tensor C4E3 = tcon(c, C4, {-1, 1}, E3l, {1, -2, -3, -4});
-  Tensor CEE4 = tcon(c, E4, {-1, 1, -3, -5}, C4E3, {1, -2, -4, -6});
-  Tensor LB = tcon(c, CEE4, {-1, -2, 1, 2, -4, -5}, Al, {-3, 1, 2, -6, -7});
-  Tensor Lt = tcon(c, LB, {1, -2, 2, 3, 4, -3, -5}, LU, {1, -1, 2, 3, 4, -4, -6});
-  Tensor C2E1 = tcon(c, C2, {1, -2}, E1r, {-1, 1, -3, -4});
-  Tensor RU = tcon(c, C2E1, {-1, -2, 1, -3}, Ar, {1, -4, -5, -6, -7});
-  Tensor C3E3 = tcon(c, C3, {-1, 1}, E3r, {-2, 1, -3, -4});
-  Tensor CEE2 = tcon(c, C3E3, {1, -2, -3, -5}, E2, {-1, 1, -4, -6});
-  Tensor RB = tcon(c, CEE2, {-1, -2, -3, -4, 1, 2}, Adr, {-5, -6, 1, 2, -7});
-  Tensor Rt = tcon(c, RU, {-1, 1, 2, -3, 3, 4, -5}, RB, {1, -2, 3, 4, 2, -4, -6});
-  Tensor roh = tcon(c, Lt, {1, 2, 3, 4, -1, -3}, Rt, {1, 2, 3, 4, -2, -4});
-  int64_t d = Al.dims[4];
-  return op::reshape(c, roh, {d * d, d * d});
+// ---- density matrices (src/graph/evaluation.jl:124-248) ---------------------
+// The reference builds rho_h from a left half (C1 E1 E4 C4 E3 A Ad) and a
+// right half, and rho_v from an upper and a lower half, with the physical legs
+// opened early (its dominant term is chi^3 D^5 d^2 per half).  Here the
+// upper-left corner C1 E1 E4 A Ad and the lower-right corner C3 E3 E2 A Ad are
+// built once, with both physical legs open (chi^2 D^4 d^2 elements each), and
+// each corner is closed twice: UL with C4 E3 (left half of rho_h) and with
+// C2 E2 (upper half of rho_v); LR with C2 E1 (right half of rho_h) and with
+// C4 E4 (lower half of rho_v).  The halves come out in the reference's leg
+// order, so the final rho contractions are the reference's
+// (evaluation.jl:183,207,231,247); the dominant term is chi^3 D^4 d^2.
+//
+// Legs: C1 (down, right), C2 (left, down), C3 (up, left), C4 (up, right);
+// E1/E3 (left, right, ket, bra), E2/E4 (up, down, ket, bra);
+// A (up, left, down, right, phys).
+
+// (e1r, e4d, A_d, A_r, Ad_d, Ad_r, p, p')
+Tensor corner_ul(Ctx& c, const Tensor& C1, const Tensor& E1, const Tensor& E4, const Tensor& A, const Tensor& Ad) {
+  Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});
+  Tensor X1 = tcon(c, C1E1, {1, -1, -2, -3}, E4, {1, -4, -5, -6});
+  Tensor X2 = tcon(c, X1, {-1, 1, -2, -3, 2, -4}, A, {1, 2, -5, -6, -7});
+  return tcon(c, X2, {-1, 1, -2, 2, -3, -4, -7}, Ad, {1, 2, -5, -6, -8});
 }
 
-Tensor dm_v(Ctx& c, const Tensor& C1, const Tensor& C2, const Tensor& C3, const Tensor& C4, const Tensor& E1,
-            const Tensor& E2u, const Tensor& E2d, const Tensor& E3, const Tensor& E4u, const Tensor& E4d,
-            const Tensor& Au, const Tensor& Ad_, const Tensor& Adu, const Tensor& Add) {
-  Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});
-  Tensor CEE4 = tcon(c, C1E1, {1, -2, -3, -5}, E4u, {1, -1, -4, -6});
-  Tensor UL = tcon(c, CEE4, {-1, -2, 1, 2, -3, -4}, Au, {1, 2, -5, -6, -7});
-  Tensor C2E2 = tcon(c, C2, {-1, 1}, E2u, {1, -2, -3, -4});
-  Tensor UR = tcon(c, C2E2, {-1, -2, -3, 1}, Adu, {-4, -5, -6, 1, -7});
-  Tensor U = tcon(c, UL, {-1, 1, 3, 4, -3, 2, -5}, UR, {1, -2, 2, 3, 4, -4, -6});
-  Tensor C4E3 = tcon(c, C4, {-1, 1}, E3, {1, -2, -3, -4});
-  Tensor CEE4b = tcon(c, C4E3, {1, -2, -4, -6}, E4d, {-1, 1, -3, -5});
-  Tensor BL = tcon(c, CEE4b, {-1, -2, -3, -4, 1, 2}, Add, {-5, 1, 2, -6, -7});
-  Tensor C3E2 = tcon(c, C3, {1, -2}, E2d, {-1, 1, -3, -4});
-  Tensor BR = tcon(c, C3E2, {-1, -2, 1, -3}, Ad_, {-4, -5, -6, 1, -7});
-  Tensor B = tcon(c, BL, {-1, 1, 2, 3, -4, 4, -6}, BR, {-2, 1, 4, -3, 2, 3, -5});
-  Tensor rov = tcon(c, U, {1, 2, 3, 4, -1, -3}, B, {1, 2, 3, 4, -2, -4});
-  int64_t d = Au.dims[4];
-  return op::reshape(c, rov, {d * d, d * d});
+// (e3l, e2u, A_u, A_l, Ad_u, Ad_l, p, p')
+Tensor corner_lr(Ctx& c, const Tensor& C3, const Tensor& E3, const Tensor& E2, const Tensor& A, const Tensor& Ad) {
+  Tensor C3E3 = tcon(c, C3, {-1, 1}, E3, {-2, 1, -3, -4});
+  Tensor Y1 = tcon(c, C3E3, {1, -1, -2, -3}, E2, {-4, 1, -5, -6});
+  Tensor Y2 = tcon(c, Y1, {-1, 1, -2, -3, 2, -4}, A, {-5, -6, 1, 2, -7});
+  return tcon(c, Y2, {-1, 1, -2, 2, -3, -4, -7}, Ad, {-5, -6, 1, 2, -8});
+}
+
+// rho from two halves (x1, x2, a, ad, p, p') sharing their first four legs
+Tensor close_dm(Ctx& c, const Tensor& h1, const Tensor& h2) {
+  Tensor r = tcon(c, h1, {1, 2, 3, 4, -1, -3}, h2, {1, 2, 3, 4, -2, -4});
+  int64_t d = h1.dims[4];
+  return op::reshape(c, r, {d * d, d * d});
 }
 
 }  // namespace
+
+void get_dms(Ctx& c, const Env& e, Tensor& roh, Tensor& rov) {
+  const auto& C = e.C;
+  const auto& E = e.E;
+  Tensor UL = corner_ul(c, C[0], E[0], E[3], e.A, e.Ad);
+  Tensor LR = corner_lr(c, C[2], E[2], E[1], e.A, e.Ad);
+  // left half of rho_h: UL closed with C4 E3 -> (e1r, e3r, A_r, Ad_r, p, p')
+  Tensor C4E3 = tcon(c, C[3], {-1, 1}, E[2], {1, -2, -3, -4});
+  Tensor Lh = tcon(c, UL, {-1, 1, 2, -3, 3, -4, -5, -6}, C4E3, {1, -2, 2, 3});
+  // right half of rho_h: LR closed with C2 E1 -> (e1l, e3l, A_l, Ad_l, p, p')
+  Tensor C2E1 = tcon(c, C[1], {1, -2}, E[0], {-1, 1, -3, -4});
+  Tensor Rh = tcon(c, LR, {-2, 1, 2, -3, 3, -4, -5, -6}, C2E1, {-1, 1, 2, 3});
+  roh = close_dm(c, Lh, Rh);
+  // upper half of rho_v: UL closed with C2 E2 -> (e4d, e2d, A_d, Ad_d, p, p')
+  Tensor C2E2 = tcon(c, C[1], {-1, 1}, E[1], {1, -2, -3, -4});
+  Tensor Uv = tcon(c, UL, {1, -1, -3, 2, -4, 3, -5, -6}, C2E2, {1, -2, 2, 3});
+  // lower half of rho_v: LR closed with C4 E4 -> (e4u, e2u, A_u, Ad_u, p, p')
+  Tensor C4E4 = tcon(c, C[3], {1, -2}, E[3], {-1, 1, -3, -4});
+  Tensor Bv = tcon(c, LR, {1, -2, -3, 2, -4, 3, -5, -6}, C4E4, {-1, 1, 2, 3});
+  rov = close_dm(c, Uv, Bv);
+}
 
 void projector(Ctx& c, const Env& e0, int dir, int64_t chi, Tensor& P1, Tensor& P2, std::vector<double>& s) {
   Env e = e0;  // shares the tensors; the *_rg functions stop after the projector
@@ -279,13 +298,6 @@ std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi) {
   top_rg(c, e, chi);
   bottom_rg(c, e, chi);
   return s;
-}
-
-void get_dms(Ctx& c, const Env& e, Tensor& roh, Tensor& rov) {
-  const auto& C = e.C;
-  const auto& E = e.E;
-  roh = dm_h(c, C[0], C[1], C[2], C[3], E[0], E[0], E[1], E[2], E[2], E[3], e.A, e.A, e.Ad, e.Ad);
-  rov = dm_v(c, C[0], C[1], C[2], C[3], E[0], E[1], E[1], E[2], E[3], E[3], e.A, e.A, e.Ad, e.Ad);
 }
 
 static Env as_dtype(Ctx& c, const Env& e, DType dt) {
