@@ -41,7 +41,10 @@ int ipeps_version(void);
 /* Message of the last failed call on this thread ("" if none). */
 const char* ipeps_last_error(void);
 
-/* Context: device, stream, caching allocator, solver handle. */
+/* Context: device, stream, caching allocator, solver handle.  Environments
+ * share ownership of the context they were created in: ipeps_ctx_destroy
+ * releases the handle, and the device resources are freed once the last
+ * environment made from it is destroyed too (handles may go in any order). */
 int ipeps_ctx_create(int device, ipeps_ctx** out);
 int ipeps_ctx_destroy(ipeps_ctx* ctx);
 int ipeps_ctx_sync(ipeps_ctx* ctx);
@@ -62,8 +65,12 @@ int ipeps_env_create(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const voi
 int ipeps_env_destroy(ipeps_env* env);
 /* Engine options: "svd_mode" (0 full SVD of M = R1 R2 as the reference,
  * 1 randomized range finder), "oversample", "power_iters", "full_below",
- * "seed", "canonical_edge" (0/1), "mixed" (1, 3 or 6 bf16 products),
- * "energy_fp64" (0/1). */
+ * "seed", "eta", "canonical_edge" (0/1), "mixed" (1, 3 or 6 bf16 products),
+ * "energy_fp64" (0/1), "checkpoint" (0/1), and for the randomized mode
+ * "fast_orth" (0/1: CholeskyQR2 range bases with Householder fallback),
+ * "fast_svd" (0/1: small SVD from Gram eigenpairs with one-sided Jacobi
+ * fallback) and "gram_guard" (s_min/s_max below which the Gram route is not
+ * taken, default 1e-4). */
 int ipeps_env_set_option(ipeps_env* env, const char* key, double value);
 /* which: 0..3 = C1..C4, 4..7 = E1..E4, 8 = A.  dims has room for 5. */
 int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd);
@@ -142,6 +149,14 @@ int ipeps_gemm(ipeps_ctx* ctx, int dtype, int mixed, int transA, int conjA, int 
  * m x n matrix; U is m x m, V is n x n (V, not V^H), S has min(m,n). */
 int ipeps_svd(ipeps_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A, void* U, double* S, void* V);
 
+/* Thin SVD of a tall m x n matrix (m >= n): A = U diag(S) V^H, S descending,
+ * U m x n, V n x n.  The small-SVD step of the randomized projector
+ * (B = M^H Q; src/utility/autodiff.jl:22-25 is svd1, which this replaces for
+ * tall operands).  method 0: one-sided Jacobi (rocSOLVER gesvdj); 1: Gram
+ * eigenpairs B^H B with one Newton-Schulz orthonormalisation of U, taken
+ * only when S_min/S_max >= guard, else method 0.  *method_used reports it. */
+int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double guard, int64_t m, int64_t n, const void* A,
+                   void* U, double* S, void* V, int* method_used);
 /* svd_back (src/utility/autodiff.jl:41-75) for a square N x N matrix with
  * full U, S, V and cotangents of the first n columns dU (N x n), dS (n),
  * dV (N x n) (any may be NULL); eta is the Lorentzian broadening (1e-12 in

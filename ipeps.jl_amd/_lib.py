@@ -44,6 +44,7 @@ SIGNATURES = {
     "ipeps_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64,
                              C.c_int64, C.c_double, _P, C.c_int64, _P, C.c_int64, C.c_double, _P, C.c_int64]),
     "ipeps_svd": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, _P, _P, _DP, _P]),
+    "ipeps_svd_thin": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_int64, C.c_int64, _P, _P, _DP, _P, _IP]),
     "ipeps_svd_back": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, _P, _DP, _P, _P, _DP, _P, C.c_double, _P]),
 }
 

@@ -1,5 +1,6 @@
 // extern "C" entry points (declared in include/ipeps_hip.h).
 #include <cstring>
+#include <memory>
 #include <sstream>
 #include <string>
 
@@ -8,12 +9,15 @@
 
 using namespace ipeps;
 
+// A context is shared by its handle and by every environment made from it:
+// the device resources go when the last of them is destroyed, so handles may
+// be destroyed in any order.
 struct ipeps_ctx {
-  Ctx* c = nullptr;
+  std::shared_ptr<Ctx> c;
 };
 
 struct ipeps_env {
-  ipeps_ctx* ctx = nullptr;
+  std::shared_ptr<Ctx> hold;  // declared first: outlives the tensors in env
   int dtype = IPEPS_F64;
   ctm::Env env;
 };
@@ -79,6 +83,9 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
   else if (key == "full_below") o.svd.full_below = (int)v;
   else if (key == "seed") o.svd.seed = (uint64_t)v;
   else if (key == "eta") o.svd.eta = v;
+  else if (key == "fast_orth") o.svd.fast_orth = (int)v;
+  else if (key == "fast_svd") o.svd.fast_svd = (int)v;
+  else if (key == "gram_guard") o.svd.gram_guard = v;
   else if (key == "canonical_edge") o.canonical_edge = v != 0.0;
   else if (key == "energy_fp64") o.energy_fp64 = v != 0.0;
   else if (key == "checkpoint") o.checkpoint = v != 0.0;
@@ -120,21 +127,15 @@ const char* ipeps_last_error(void) { return g_err.c_str(); }
 int ipeps_ctx_create(int device, ipeps_ctx** out) {
   return guard([&] {
     IPEPS_CHECK(out, "null out");
+    std::shared_ptr<Ctx> c(ctx_create(device), [](Ctx* p) { ctx_destroy(p); });
     auto* h = new ipeps_ctx();
-    try {
-      h->c = ctx_create(device);
-    } catch (...) {
-      delete h;
-      throw;
-    }
+    h->c = std::move(c);
     *out = h;
   });
 }
 
 int ipeps_ctx_destroy(ipeps_ctx* ctx) {
   return guard([&] {
-    if (!ctx) return;
-    ctx_destroy(ctx->c);
     delete ctx;
   });
 }
@@ -192,7 +193,7 @@ int ipeps_env_create(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const voi
     IPEPS_CHECK(dims[0] == dims[1] && dims[1] == dims[2] && dims[2] == dims[3], "A must be (D,D,D,D,d)");
     Ctx& c = *ctx->c;
     auto* e = new ipeps_env();
-    e->ctx = ctx;
+    e->hold = ctx->c;
     e->dtype = dtype;
     e->env.dt = dev_dt(dtype);
     e->env.opt.mixed = k::Mixed::BF16X3;
@@ -232,7 +233,7 @@ int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd) {
 int ipeps_env_get_tensor(ipeps_env* env, int which, void* host) {
   return guard([&] {
     IPEPS_CHECK(env, "null env");
-    download(*env->ctx->c, env_slot(env->env, which), host_dt(env->dtype), host);
+    download(*env->hold, env_slot(env->env, which), host_dt(env->dtype), host);
   });
 }
 
@@ -240,7 +241,7 @@ int ipeps_env_set_tensor(ipeps_env* env, int which, int nd, const int64_t* dims,
   return guard([&] {
     IPEPS_CHECK(env && dims && nd > 0 && nd <= 5, "bad arguments");
     std::vector<int64_t> d(dims, dims + nd);
-    Ctx& c = *env->ctx->c;
+    Ctx& c = *env->hold;
     Tensor t = upload(c, host_dt(env->dtype), dev_dt(env->dtype), d, host);
     env_slot(env->env, which) = t;
     if (which == 8) env->env.Ad = op::conj(c, t);
@@ -250,7 +251,7 @@ int ipeps_env_set_tensor(ipeps_env* env, int which, int nd, const int64_t* dims,
 int ipeps_env_set_A(ipeps_env* env, const void* A) {
   return guard([&] {
     IPEPS_CHECK(env, "null env");
-    Ctx& c = *env->ctx->c;
+    Ctx& c = *env->hold;
     Tensor t = upload(c, host_dt(env->dtype), dev_dt(env->dtype), env->env.A.dims, A);
     env->env.A = t;
     env->env.Ad = op::conj(c, t);
@@ -261,7 +262,7 @@ int ipeps_rg_step(ipeps_env* env, int64_t chi, double* s_out, int64_t s_cap, int
   return guard([&] {
     IPEPS_CHECK(env && chi > 0, "bad arguments");
     TapeScope ts(nullptr);
-    auto s = ctm::rg_step(*env->ctx->c, env->env, chi);
+    auto s = ctm::rg_step(*env->hold, env->env, chi);
     if (s_len) *s_len = (int64_t)s.size();
     if (s_out) {
       IPEPS_CHECK(s_cap >= (int64_t)s.size(), "rg_step: s_out too small");
@@ -275,7 +276,7 @@ int ipeps_run_ctm(ipeps_env* env, int64_t chi, int min_iter, int max_iter, doubl
                   int64_t diffs_cap, int64_t* n_diffs) {
   return guard([&] {
     IPEPS_CHECK(env && chi > 0 && max_iter >= 1, "bad arguments");
-    Ctx& c = *env->ctx->c;
+    Ctx& c = *env->hold;
     TapeScope ts(nullptr);
     Tensor h1, h2;
     if (conv_mode == 0) {
@@ -296,7 +297,7 @@ int ipeps_run_ctm(ipeps_env* env, int64_t chi, int min_iter, int max_iter, doubl
 int ipeps_gs_energy(ipeps_env* env, const double* H1, const double* H2, int64_t hdim, double* E_out, double* N_out) {
   return guard([&] {
     IPEPS_CHECK(env, "null env");
-    Ctx& c = *env->ctx->c;
+    Ctx& c = *env->hold;
     TapeScope ts(nullptr);
     Tensor out = ctm::energy(c, env->env, upload_H(c, H1, hdim), upload_H(c, H2, hdim));
     double h[8];
@@ -310,7 +311,7 @@ int ipeps_gs_norm(ipeps_env* env, double* out) {
   return guard([&] {
     IPEPS_CHECK(env && out, "null argument");
     TapeScope ts(nullptr);
-    ctm::gs_norm(*env->ctx->c, env->env, out);
+    ctm::gs_norm(*env->hold, env->env, out);
   });
 }
 
@@ -318,7 +319,7 @@ int ipeps_step_adjoint(ipeps_env* env, int64_t chi, const double* H1, const doub
                        void* gradA_host) {
   return guard([&] {
     IPEPS_CHECK(env && chi > 0, "bad arguments");
-    Ctx& c = *env->ctx->c;
+    Ctx& c = *env->hold;
     double E[2];
     Tensor g = ctm::step_and_adjoint(c, env->env, chi, upload_H(c, H1, hdim), upload_H(c, H2, hdim), E);
     if (E_out) { E_out[0] = E[0]; E_out[1] = E[1]; }
@@ -350,7 +351,7 @@ int ipeps_es_grad(ipeps_env* env, const double* H1, const double* H2, int64_t hd
   return guard([&] {
     IPEPS_CHECK(env && B && chi > 0, "bad arguments");
     IPEPS_CHECK(env->dtype == IPEPS_C128, "es_grad: the environment must be complex (IPEPS_C128)");
-    Ctx& c = *env->ctx->c;
+    Ctx& c = *env->hold;
     Tensor tB = upload(c, DType::C128, DType::C128, env->env.A.dims, B);
     const double pi = 3.14159265358979323846;
     auto r = ctm::es_grad(c, env->env, tB, upload_H(c, H1, hdim), upload_H(c, H2, hdim), px * pi, py * pi, chi,
@@ -426,6 +427,24 @@ int ipeps_svd(ipeps_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A, vo
     download(c, tU, dt, U);
     to_host(c, tS, S);
     download(c, tV, dt, V);
+  });
+}
+
+int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double gram_guard, int64_t m, int64_t n, const void* A, void* U,
+                   double* S, void* V, int* method_used) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && m >= n && n > 0 && A && U && S && V, "bad arguments (m >= n > 0 required)");
+    IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_C128, "svd_thin: F64 or C128");
+    IPEPS_CHECK(method == 0 || method == 1, "svd_thin: method must be 0 or 1");
+    Ctx& c = *ctx->c;
+    DType dt = host_dt(dtype);
+    Tensor tA = upload(c, dt, dt, {m, n}, A);
+    Tensor tU, tS, tV;
+    int used = op::svd_tall(c, tA, method, gram_guard, tU, tS, tV);
+    download(c, tU, dt, U);
+    to_host(c, tS, S);
+    download(c, tV, dt, V);
+    if (method_used) *method_used = used;
   });
 }
 

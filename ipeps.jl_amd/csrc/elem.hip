@@ -513,6 +513,56 @@ __global__ void es_energy_back_kernel(int64_t m, const double2* H1, const double
   gv[t] = cconj(make_double2(H2[c + r * m].x / out[3], H2[c + r * m].y / out[3]));
 }
 
+// ---- projector-solver helpers ------------------------------------------------
+__device__ inline double dev_from_eye(double g, bool d) { return fabs(d ? g - 1.0 : g); }
+__device__ inline double dev_from_eye(float g, bool d) { return fabs(d ? (double)g - 1.0 : (double)g); }
+__device__ inline double dev_from_eye(double2 g, bool d) {
+  double re = d ? g.x - 1.0 : g.x;
+  return sqrt(re * re + g.y * g.y);
+}
+
+// one workgroup; a NaN anywhere makes the result +inf
+template <typename T>
+__global__ __launch_bounds__(TPB) void eye_dev_kernel(int64_t n, const T* G, double* out) {
+  __shared__ double red[TPB];
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < n * n; i += TPB) {
+    double v = dev_from_eye(G[i], (i % n) == (i / n));
+    m = isnan(v) ? INFINITY : fmax(m, v);
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+template <typename T>
+__global__ void eig_to_svd_kernel(int64_t n, const double* lam, const T* W, double* s, T* Wr, double* sinv) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n * n; i += (int64_t)gridDim.x * TPB) {
+    int64_t r = i % n, col = i / n;
+    Wr[i] = W[r + (n - 1 - col) * n];
+    if (r == 0) {
+      double l = lam[n - 1 - col];
+      double v = l > 0.0 ? sqrt(l) : 0.0;
+      s[col] = v;
+      sinv[col] = v > 0.0 ? 1.0 / v : 0.0;
+    }
+  }
+}
+
+__device__ inline void ns_el(double& g, bool d) { g = (d ? 1.5 : 0.0) - 0.5 * g; }
+__device__ inline void ns_el(float& g, bool d) { g = (d ? 1.5f : 0.0f) - 0.5f * g; }
+__device__ inline void ns_el(double2& g, bool d) { g = make_double2((d ? 1.5 : 0.0) - 0.5 * g.x, -0.5 * g.y); }
+
+template <typename T>
+__global__ void ns_coeff_kernel(int64_t n, T* G) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n * n; i += (int64_t)gridDim.x * TPB)
+    ns_el(G[i], (i % n) == (i / n));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -803,6 +853,30 @@ void energy2_back(Ctx& c, DType dt, int64_t m, const void* rho_h, const void* rh
   int64_t n = 2 * m * m;
   hipLaunchKernelGGL(energy2_back_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, m, rho_h, rho_v,
                      H1, H2, grho_h, grho_v, (int)(dt == DType::C128));
+  HIP_CHECK(hipGetLastError());
+}
+
+void eye_dev(Ctx& c, DType dt, int64_t n, const void* G, double* out) {
+  IPEPS_CHECK(n >= 1, "eye_dev: empty matrix");
+#define EK(T, ...) hipLaunchKernelGGL(eye_dev_kernel<T>, dim3(1), dim3(TPB), 0, c.stream, n, (const T*)G, out)
+  DISPATCH3(dt, EK, 0);
+#undef EK
+  HIP_CHECK(hipGetLastError());
+}
+
+void eig_to_svd(Ctx& c, DType dt, int64_t n, const double* lam, const void* W, double* s, void* Wr, double* sinv) {
+  if (n <= 0) return;
+#define VK(T, ...) hipLaunchKernelGGL(eig_to_svd_kernel<T>, dim3(blocks_for(n * n)), dim3(TPB), 0, c.stream, n, lam, (const T*)W, s, (T*)Wr, sinv)
+  DISPATCH3(dt, VK, 0);
+#undef VK
+  HIP_CHECK(hipGetLastError());
+}
+
+void ns_coeff(Ctx& c, DType dt, int64_t n, void* G) {
+  if (n <= 0) return;
+#define NK(T, ...) hipLaunchKernelGGL(ns_coeff_kernel<T>, dim3(blocks_for(n * n)), dim3(TPB), 0, c.stream, n, (T*)G)
+  DISPATCH3(dt, NK, 0);
+#undef NK
   HIP_CHECK(hipGetLastError());
 }
 

@@ -113,5 +113,15 @@ void es_energy(Ctx& c, int64_t m, const void* rh1, const void* rv1, const void* 
 void es_energy_back(Ctx& c, int64_t m, const double* H1, const double* H2, const double* out, void* g_rh4,
                     void* g_rv4);
 
+// ---- small dense helpers of the projector solvers (n x n, F64 or C128) ----
+// out[0] = max_ij |G_ij - delta_ij|  (one workgroup; n up to a few thousand)
+void eye_dev(Ctx& c, DType dt, int64_t n, const void* G, double* out);
+// Eigen pairs (ascending lam, columns of W) of a Gram matrix B^H B to the
+// singular values of B in descending order: s_i = sqrt(max(lam_{n-1-i}, 0)),
+// Wr[:, i] = W[:, n-1-i], sinv_i = 1/s_i (0 where s_i = 0).
+void eig_to_svd(Ctx& c, DType dt, int64_t n, const double* lam, const void* W, double* s, void* Wr, double* sinv);
+// G <- (3 I - G) / 2  (one Newton-Schulz step X <- X (3I - X^H X)/2)
+void ns_coeff(Ctx& c, DType dt, int64_t n, void* G);
+
 }  // namespace k
 }  // namespace ipeps

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 namespace ipeps {
@@ -633,7 +634,7 @@ static void gesvdj_thin(Ctx& c, const Tensor& Ain, Tensor& U, Tensor& S, Tensor&
 }
 void svd_full_raw(Ctx& c, const Tensor& X, Tensor& U, Tensor& s, Tensor& V) { gesvd(c, X, U, s, V, false); }
 
-Tensor orth(Ctx& c, const Tensor& Yin) {
+static Tensor orth_householder(Ctx& c, const Tensor& Yin) {
   IPEPS_CHECK(Yin.nd() == 2 && Yin.dims[0] >= Yin.dims[1], "orth: tall matrix expected");
   int64_t m = Yin.dims[0], n = Yin.dims[1];
   Tensor Y = clone(c, Yin);
@@ -657,6 +658,129 @@ Tensor orth(Ctx& c, const Tensor& Yin) {
     throw Error(IPEPS_ERR_ARG, "orth: F64 or C128 only");
   }
   return Y;
+}
+
+// Cholesky factor (upper) in place: G = R^H R; LAPACK-style info to a device int.
+static void potrf_upper(Ctx& c, Tensor& G, int* info) {
+  const rocblas_int n = (rocblas_int)G.dims[0];
+  rocblas_handle h = (rocblas_handle)c.solver;
+  TimedLaunch tl(c, "rocsolver_potrf", 0.0, 0.0);
+  if (G.dt == DType::F64)
+    solver_check(rocsolver_dpotrf(h, rocblas_fill_upper, n, G.ptr<double>(), n, (rocblas_int*)info), "rocsolver_dpotrf");
+  else if (G.dt == DType::C128)
+    solver_check(rocsolver_zpotrf(h, rocblas_fill_upper, n, (rocblas_double_complex*)G.data(), n, (rocblas_int*)info),
+                 "rocsolver_zpotrf");
+  else
+    throw Error(IPEPS_ERR_ARG, "potrf: F64 or C128 only");
+}
+
+// X <- X R^-1 for an upper-triangular R (n x n) and X (m x n).
+static void trsm_right_upper(Ctx& c, const Tensor& R, Tensor& X) {
+  const rocblas_int m = (rocblas_int)X.dims[0], n = (rocblas_int)X.dims[1];
+  rocblas_handle h = (rocblas_handle)c.solver;
+  solver_check(rocblas_set_pointer_mode(h, rocblas_pointer_mode_host), "rocblas_set_pointer_mode");
+  TimedLaunch tl(c, "rocblas_trsm", 0.0, 0.0);
+  if (X.dt == DType::F64) {
+    const double one = 1.0;
+    solver_check(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                               rocblas_diagonal_non_unit, m, n, &one, R.ptr<double>(), n, X.ptr<double>(), m),
+                 "rocblas_dtrsm");
+  } else if (X.dt == DType::C128) {
+    const rocblas_double_complex one(1.0, 0.0);
+    solver_check(rocblas_ztrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                               rocblas_diagonal_non_unit, m, n, &one, (const rocblas_double_complex*)R.data(), n,
+                               (rocblas_double_complex*)X.data(), m),
+                 "rocblas_ztrsm");
+  } else {
+    throw Error(IPEPS_ERR_ARG, "trsm: F64 or C128 only");
+  }
+}
+
+// CholeskyQR2: Q1 = Y R1^-1 with R1^H R1 = Y^H Y, then the same on Q1.  The
+// second pass is trusted when its Gram is within 1/2 of I (then cond(Q1)^2 <= 3
+// and Q is orthonormal to working precision).  Returns false when a Cholesky
+// failed or that check did not hold; Q is then undefined.
+static bool orth_cholqr2(Ctx& c, const Tensor& Y, Tensor& Q) {
+  const int64_t n = Y.dims[1];
+  Tensor st = c.zeros(DType::F64, {3});  // [info1, info2, |G2 - I|_max]
+  int* info1 = (int*)st.ptr<double>();
+  int* info2 = (int*)(st.ptr<double>() + 1);
+  Tensor G = matmul_raw(c, Y, true, true, Y, false, false);
+  potrf_upper(c, G, info1);
+  Q = clone(c, Y);
+  trsm_right_upper(c, G, Q);
+  Tensor G2 = matmul_raw(c, Q, true, true, Q, false, false);
+  k::eye_dev(c, Q.dt, n, G2.data(), st.ptr<double>() + 2);
+  potrf_upper(c, G2, info2);
+  trsm_right_upper(c, G2, Q);
+  double hs[3];
+  HIP_CHECK(hipMemcpyAsync(hs, st.data(), sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  int i1, i2;
+  std::memcpy(&i1, &hs[0], sizeof(int));
+  std::memcpy(&i2, &hs[1], sizeof(int));
+  return i1 == 0 && i2 == 0 && hs[2] <= 0.5;
+}
+
+Tensor orth(Ctx& c, const Tensor& Y, bool fast) {
+  IPEPS_CHECK(Y.nd() == 2 && Y.dims[0] >= Y.dims[1], "orth: tall matrix expected");
+  Tensor Q;
+  if (fast && orth_cholqr2(c, Y, Q)) return Q;
+  return orth_householder(c, Y);
+}
+
+// Thin SVD of a tall B (m x n) = W diag(s) X^H from the eigenpairs of B^H B:
+// s = sqrt(lambda) descending, X the eigenvectors, W = B X diag(1/s) followed
+// by one Newton-Schulz step W <- W (3I - W^H W)/2.  Eigenvalues carry an
+// absolute error ~ u lambda_max, so the route is taken only when
+// s_min / s_max >= guard; returns false otherwise (outputs undefined).
+static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s, Tensor& X) {
+  const int64_t m = B.dims[0], n = B.dims[1];
+  Tensor G = matmul_raw(c, B, true, true, B, false, false);
+  Tensor lam = c.empty(DType::F64, {n});
+  Tensor E = c.empty(DType::F64, {n});
+  Tensor st = c.zeros(DType::F64, {1});
+  rocblas_handle h = (rocblas_handle)c.solver;
+  {
+    TimedLaunch tl(c, "rocsolver_syevd", 0.0, 0.0);
+    if (B.dt == DType::F64)
+      solver_check(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, G.ptr<double>(),
+                                    (rocblas_int)n, lam.ptr<double>(), E.ptr<double>(), (rocblas_int*)st.data()),
+                   "rocsolver_dsyevd");
+    else if (B.dt == DType::C128)
+      solver_check(rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n,
+                                    (rocblas_double_complex*)G.data(), (rocblas_int)n, lam.ptr<double>(),
+                                    E.ptr<double>(), (rocblas_int*)st.data()),
+                   "rocsolver_zheevd");
+    else
+      throw Error(IPEPS_ERR_ARG, "gram_svd: F64 or C128 only");
+  }
+  X = c.empty(B.dt, {n, n});
+  s = c.empty(DType::F64, {n});
+  Tensor sinv = c.empty(DType::F64, {n});
+  k::eig_to_svd(c, B.dt, n, lam.ptr<double>(), G.data(), s.ptr<double>(), X.data(), sinv.ptr<double>());
+  std::vector<double> sh(n);
+  double hinfo = 0.0;
+  HIP_CHECK(hipMemcpyAsync(sh.data(), s.data(), n * 8, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(&hinfo, st.data(), 8, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  int info;
+  std::memcpy(&info, &hinfo, sizeof(int));
+  if (info != 0 || !(sh[0] > 0.0) || !(sh[n - 1] >= guard * sh[0])) return false;
+  Tensor BX = matmul_raw(c, B, false, false, X, false, false);
+  Tensor W0 = c.empty(B.dt, {m, n});
+  k::scale_cols(c, B.dt, m, n, BX.data(), sinv.ptr<double>(), W0.data(), false);
+  Tensor T = matmul_raw(c, W0, true, true, W0, false, false);
+  k::ns_coeff(c, B.dt, n, T.data());
+  W = matmul_raw(c, W0, false, false, T, false, false);
+  return true;
+}
+
+int svd_tall(Ctx& c, const Tensor& B, int method, double guard, Tensor& W, Tensor& s, Tensor& X) {
+  IPEPS_CHECK(B.nd() == 2 && B.dims[0] >= B.dims[1], "svd_tall: tall matrix expected");
+  if (method == 1 && gram_svd(c, B, guard, W, s, X)) return 1;
+  gesvdj_thin(c, B, W, s, X);
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -713,14 +837,16 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     auto apply_MH = [&](const Tensor& X) {  // M^H X = R2^H (R1^H X)
       return matmul_raw(c, R2, true, true, matmul_raw(c, R1, true, true, X, false, false), false, false);
     };
-    Tensor Q = orth(c, to_dtype(c, apply_M(Om), sdt));
+    const bool fo = o.fast_orth != 0;
+    Tensor Q = orth(c, to_dtype(c, apply_M(Om), sdt), fo);
     for (int q = 0; q < o.power_iters; ++q) {
-      Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt));
-      Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt));
+      Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), fo);
+      Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), fo);
     }
     Tensor Bt = to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt);  // nc x kk = M^H Q
     Tensor W, s, X;
-    gesvdj_thin(c, Bt, W, s, X);  // Bt = W diag(s) X^H  ->  M ~ (Q X) diag(s) W^H
+    // Bt = W diag(s) X^H  ->  M ~ (Q X) diag(s) W^H
+    svd_tall(c, Bt, o.fast_svd ? 1 : 0, o.gram_guard, W, s, X);
     Uf = matmul_raw(c, Q, false, false, X, false, false);
     Sf = s;
     Vf = W;

@@ -103,6 +103,18 @@ struct SvdOpts {
   int full_below = 0;    // use the full SVD when N <= full_below (randomized mode)
   uint64_t seed = 1234;
   double eta = 1e-12;    // Lorentzian broadening of the adjoint (src/utility/autodiff.jl:41)
+  // Randomized mode only:
+  //   fast_orth = 1: range bases by CholeskyQR2 (two Gram + Cholesky + triangular
+  //                  solve passes), falling back to Householder QR when a
+  //                  Cholesky fails or the second Gram is farther than 1/2 from I;
+  //   fast_svd  = 1: the small SVD of B = M^H Q from the eigenpairs of B^H B
+  //                  plus one Newton-Schulz orthonormalisation of B's singular
+  //                  vectors, falling back to one-sided Jacobi (gesvdj) when
+  //                  s_min / s_max < gram_guard (the Gram eigenvalues then lose
+  //                  more than ~1e-8 relative accuracy).
+  int fast_orth = 1;
+  int fast_svd = 1;
+  double gram_guard = 1e-4;
 };
 
 // Truncated SVD with the reference cutoff rule (src/utility/wrapper.jl:25-55):
@@ -113,7 +125,11 @@ struct SvdOpts {
 SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, const SvdOpts& o);
 
 // Orthonormal basis of the columns of Y (N x k, F64 or C128): Householder QR.
-Tensor orth(Ctx& c, const Tensor& Y);
+Tensor orth(Ctx& c, const Tensor& Y, bool fast = false);
+// Thin SVD of a tall F64/C128 matrix B = W diag(s) X^H (s descending).
+// method 0: one-sided Jacobi (rocSOLVER gesvdj); 1: Gram eigenpairs when
+// s_min/s_max >= guard, else method 0.  Returns the method used.
+int svd_tall(Ctx& c, const Tensor& B, int method, double guard, Tensor& W, Tensor& s, Tensor& X);
 // Full SVD X = U diag(s) V^H (U m x m, V n x n, V returned, not V^H).
 void svd_full_raw(Ctx& c, const Tensor& X, Tensor& U, Tensor& s, Tensor& V);
 // Thin SVD of a tall matrix X (m x n, m >= n): X = W diag(s) Z^H.

@@ -110,8 +110,8 @@ def _hpair(H):
     return hs, m
 
 
-ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "canonical_edge",
-               "mixed", "energy_fp64", "checkpoint")
+ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "fast_orth", "fast_svd",
+               "gram_guard", "canonical_edge", "mixed", "energy_fp64", "checkpoint")
 
 
 class CTMTensors:
@@ -345,6 +345,25 @@ def svd1(A, ctx=None):
     S = np.empty(min(m, n))
     check(lib().ipeps_svd(ctx.h, code, m, n, _ptr(Af), _ptr(U), _dp(S), _ptr(V)))
     return U, S, V
+
+
+def svd_thin(A, method=1, guard=1e-4, ctx=None):
+    """Thin SVD of a tall matrix on the device: (U, S, V, method_used) with
+    A = U diag(S) V^H.  method 0: one-sided Jacobi; 1: Gram eigenpairs when
+    S_min/S_max >= guard (else 0)."""
+    ctx = ctx or default_context()
+    A = np.asarray(A)
+    code = C128 if np.iscomplexobj(A) else F64
+    dt = _host_np(code)
+    m, n = A.shape
+    Af = np.asfortranarray(A, dtype=dt)
+    U = np.empty((m, n), dtype=dt, order="F")
+    V = np.empty((n, n), dtype=dt, order="F")
+    S = np.empty(n)
+    used = C.c_int(-1)
+    check(lib().ipeps_svd_thin(ctx.h, code, int(method), float(guard), m, n, _ptr(Af), _ptr(U), _dp(S), _ptr(V),
+                               C.byref(used)))
+    return U, S, V, used.value
 
 
 def svd_back(U, S, V, dU=None, dS=None, dV=None, eta=1e-12, ctx=None):

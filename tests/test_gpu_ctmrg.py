@@ -77,6 +77,58 @@ def test_svd_and_adjoint(pkg, cplx):
     np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("decay", [1e-1, 1e-3, 1e-6])
+def test_svd_thin_methods(pkg, cplx, decay):
+    """Tall thin SVD, both methods, against numpy on matrices with a known
+    spectrum from 1 down to `decay`: values to 1e-12 relative of S_max (Jacobi)
+    or u / (S_i/S_max)^2 (Gram); A = U S V^H and orthonormal U, V."""
+    rng = np.random.default_rng(3)
+    m, n = 300, 40
+
+    def orthonormal(r, c):
+        X = rng.standard_normal((r, c)) + (1j * rng.standard_normal((r, c)) if cplx else 0)
+        return np.linalg.qr(X)[0]
+
+    s0 = np.logspace(0, np.log10(decay), n)
+    A = (orthonormal(m, n) * s0) @ orthonormal(n, n).conj().T
+    for method in (0, 1):
+        U, S, V, used = pkg.svd_thin(A, method=method)
+        assert used == (method if decay >= 1e-4 else 0)
+        tol = 1e-12 if used == 0 else 1e-15 / decay ** 2
+        assert np.abs(S - s0).max() < tol, (method, np.abs(S - s0).max())
+        assert np.abs((U * S) @ V.conj().T - A).max() < 1e-11
+        assert np.abs(U.conj().T @ U - np.eye(n)).max() < 1e-12
+        assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("spread", [0.3, 3e-3])
+def test_svd_thin_clustered_tail(pkg, spread):
+    """A projector-like spectrum (10 decades-spaced values, then 30 values within
+    `spread` of 1.5e-3): the truncated V S^-1 U^H at a cut inside the cluster and
+    the kept subspace, both methods, against LAPACK.  Bounds: u / relative gap."""
+    rng = np.random.default_rng(5)
+    m, n, cut = 300, 40, 24
+
+    def orth(r, c):
+        return np.linalg.qr(rng.standard_normal((r, c)))[0]
+
+    s0 = np.concatenate([np.logspace(0, -2, 10), np.sort(1.5e-3 * (1 + spread * rng.random(30)))[::-1]])
+    A = (orth(m, n) * s0) @ orth(n, n).T
+    Un, Sn, Vnh = np.linalg.svd(A, full_matrices=False)
+    Pn = Vnh[:cut].T @ np.diag(1 / Sn[:cut]) @ Un[:, :cut].T
+    bound = 1e-8 if spread > 0.1 else 1e-5
+    for method in (0, 1):
+        U, S, V, used = pkg.svd_thin(A, method=method)
+        assert used == method
+        P = V[:, :cut] @ np.diag(1 / S[:cut]) @ U[:, :cut].T
+        cos_min = np.linalg.svd(U[:, :cut].T @ Un[:, :cut], compute_uv=False).min()
+        print(spread, method, np.abs(P - Pn).max() / np.abs(Pn).max(), 1 - cos_min)
+        assert np.abs(S - Sn).max() < 1e-13
+        assert 1 - cos_min < 1e-12
+        assert np.abs(P - Pn).max() < bound * np.abs(Pn).max()
+
+
 def _gamma_ref(S, UdU, VdV, dS, n, eta=1e-12):
     N = S.size
     G = np.zeros((N, N), dtype=np.complex128)
@@ -148,6 +200,19 @@ def test_rg_step_spectra_and_energy(pkg):
         E, _ = pkg.get_gs_energy(ts, [h.numpy() for h in H])
         Eo, _ = O.get_gs_energy(ots, H)
         assert abs(E - complex(Eo)) < 1e-10, (step, E, Eo)
+
+
+def test_env_outlives_context_handle(pkg):
+    """Destroying a context handle before its environments is allowed: the
+    environment keeps the device context alive and stays usable."""
+    A = O.seeded_tensor((2, 2, 2, 2, 2), 0).numpy()
+    ctx = pkg.Context()
+    ts = pkg.CTMTensors(A, {}, ctx=ctx)
+    ctx.close()
+    ts, s = pkg.rg_step(ts, 8)
+    E, _ = pkg.get_gs_energy(ts, [h.numpy() for h in O.ising(2.0)])
+    assert np.isfinite(E.real) and len(s) > 0
+    del ts
 
 
 def test_g4_fixed_environment(pkg):
@@ -255,6 +320,41 @@ def test_randomized_projector_energy(pkg):
     print("full", E_full, "rand k+16 q2", abs(E_full - E_rand), "rand k+40 q4", abs(E_full - E_rand2))
     assert abs(E_full - E_rand) < 1e-6
     assert abs(E_full - E_rand2) < abs(E_full - E_rand)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "c128"])
+def test_fast_solver_routes_match_exact(pkg, dtype):
+    """Randomized mode, CholeskyQR2 + Gram-eigen small SVD against Householder QR +
+    one-sided Jacobi on the same test matrices.  With k = chi+40 and q = 3 the range
+    is resolved and the two routes agree (energy 1e-10, gradient 1e-5);
+    the fast routes must run on this D=3 state.  (At the
+    default k = chi+16, q = 1 both routes carry the range-finder error, ~1e-7 in E
+    here, and differ from each other at that level.)"""
+    A = O.seeded_tensor((3, 3, 3, 3, 2), 4).numpy()
+    if dtype == "c128":
+        A = A + 0.3j * O.seeded_tensor((3, 3, 3, 3, 2), 5).numpy()
+    H = [h.numpy() for h in O.heisenberg(1.0)]
+    cfg = dict(chi=24, min_iter=6, max_iter=6, ad_max_iter=2, rg_tol=0.0)
+    base = dict(svd_mode=1, oversample=40, power_iters=3)
+    runs = []
+    for extra in ({}, dict(fast_orth=0, fast_svd=0)):
+        ctx = pkg.Context()  # fresh context: both routes draw the same test matrices
+        ctx.set_timing(1)
+        E, g, _ = pkg.gs_energy_grad(A, H, cfg, dtype=dtype, options=dict(base, **extra), ctx=ctx)
+        runs.append((E, g, ctx.kernel_stats(reset=True)))
+        ctx.close()
+    (Ef, gf, st), (Ee, ge, st_exact) = runs
+    print(dtype, "fast routes:", sorted(k for k in st if k.startswith("roc")), "exact:",
+          sorted(k for k in st_exact if k.startswith("roc")))
+    # the Gram route runs (calls whose tail falls below the guard go to gesvdj)
+    assert "rocsolver_syevd" in st and "rocsolver_potrf" in st
+    assert "rocsolver_geqrf_orgqr" not in st
+    assert "rocsolver_gesvdj" in st_exact and "rocsolver_geqrf_orgqr" in st_exact
+    print("dE", abs(Ef - Ee), "grad rel", np.abs(gf - ge).max() / np.abs(ge).max())
+    assert abs(Ef - Ee) < 1e-10
+    # the adjoint couples every computed triplet through F_ij = 1/(s_j^2 - s_i^2),
+    # which scales the Gram route's tail-vector error u s_1^2 / (s_i ds)
+    assert np.abs(gf - ge).max() < 1e-5 * np.abs(ge).max()
 
 
 @pytest.mark.parametrize("mixed", [3, 6])
