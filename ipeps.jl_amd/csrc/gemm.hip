@@ -40,6 +40,9 @@ struct GemmP {
   int64_t ldc, sC;
   int conjA, conjB;
   int64_t kchunk;  // > 0: split-K, blockIdx.z indexes the K chunk (batch must be 1)
+  // FP32 kernel: 16-B vector access allowed on the k-contiguous A / B
+  // operand and on C (pointers, leading dims and batch strides aligned)
+  int vecA, vecB, vecC;
 };
 
 constexpr int BM = 64, BN = 64;
@@ -233,104 +236,175 @@ __device__ inline unsigned short bf16_rne(float x) {
 }
 __device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
-template <bool AT, bool BT, int NSPLIT>
+// Staging of a TR x 32 slab of op(X) (rows r = m or n, 32 consecutive k).
+// Item (r, g) is the 8 values k = 8g .. 8g+7 of row r.
+//   k-contiguous operand (X[k + r ld]): item q -> g = q & 3, r = q >> 2, so
+//     four lanes read the 128 B of one row (two float4 each when aligned);
+//   row-contiguous operand (X[r + k ld]): item q -> r = q % TR, g = q / TR,
+//     each lane reads 8 scalars down k and consecutive lanes hit consecutive r.
+template <bool KV>
+__device__ inline void load8(const float* __restrict__ X, int64_t ld, int64_t r, int64_t R, int64_t kb, int64_t Keff,
+                             bool vec, float (&v)[8]) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = 0.f;
+  if (r >= R) return;
+  if (KV) {
+    const float* src = X + kb + r * ld;
+    if (vec && kb + 8 <= Keff) {
+      const float4 x = *reinterpret_cast<const float4*>(src);
+      const float4 y = *reinterpret_cast<const float4*>(src + 4);
+      v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+      v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (kb + e < Keff) v[e] = src[e];
+    }
+  } else {
+    const float* src = X + r + kb * ld;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (kb + e < Keff) v[e] = src[e * ld];
+  }
+}
+
+// hi / mid / lo bf16 planes of 8 values, 16 B per plane (plane stride in elements)
+template <int NSPLIT>
+__device__ inline void split_store(unsigned short* dst, int plane, const float (&v)[8]) {
+  float r[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) r[e] = v[e];
+#pragma unroll
+  for (int s = 0; s < NSPLIT; ++s) {
+    v8u16 h;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const unsigned short b = bf16_rne(r[e]);
+      h[e] = b;
+      r[e] -= bf16_to_f(b);
+    }
+    *reinterpret_cast<v8u16*>(dst + s * plane) = h;
+  }
+}
+
+// FP32 storage, split-bf16 MFMA.  Block tile TBM x TBN (128 x 128 or
+// 128 x 64), K tile 32, 256 threads = 4 waves in a 2 x 2 grid, each wave
+// owning a (TBM/2) x (TBN/2) block of 16 x 16 MFMA outputs.  The next K tile
+// is fetched into registers while the MFMAs of the current one run, then
+// split into NSPLIT bf16 planes and written to the other of two LDS buffers:
+// one barrier per K step.  LDS rows are k-contiguous and 80 B apart, so the
+// 16-B fragment reads of 16 consecutive rows hit distinct banks.
+template <bool AT, bool BT, int NSPLIT, int TBM, int TBN>
 __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
-  constexpr int BK = 32;
-  constexpr int LDK = BK + 8;  // bf16 elements per LDS row (k contiguous), padded
-  __shared__ unsigned short As[NSPLIT][BM][LDK];
-  __shared__ unsigned short Bs[NSPLIT][BN][LDK];
+  constexpr int BK = 32, LDK = BK + 8;
+  constexpr int WTM = TBM / 2, WTN = TBN / 2;
+  constexpr int TI = WTM / 16, TJ = WTN / 16;
+  constexpr int IA = TBM / 64, IB = TBN / 64;  // staging items per thread
+  constexpr int PA = TBM * LDK, PB = TBN * LDK;
+  __shared__ __attribute__((aligned(16))) unsigned short As[2][NSPLIT][TBM][LDK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[2][NSPLIT][TBN][LDK];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
+  const int64_t m0 = (int64_t)blockIdx.x * TBM, n0 = (int64_t)blockIdx.y * TBN;
   const float* A = (const float*)p.A + (int64_t)blockIdx.z * p.sA;
   const float* B = (const float*)p.B + (int64_t)blockIdx.z * p.sB;
   float* C = (float*)p.C + (int64_t)blockIdx.z * p.sC;
-
-  v4f32 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = (v4f32){0.f, 0.f, 0.f, 0.f};
-
   // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
   const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - (int64_t)blockIdx.z * p.kchunk) : p.K;
-  for (int64_t k0 = 0; k0 < Keff; k0 += BK) {
-    // 64 x 32 tile = 2048 values, 8 per thread.
+  const bool vecA = p.vecA != 0, vecB = p.vecB != 0;
+
+  int ar[IA], ag[IA], br[IB], bg[IB];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      int m, kk;
-      if (!AT) { m = tid & 63; kk = (tid >> 6) + 4 * r; }
-      else { kk = tid & 31; m = (tid >> 5) + 8 * r; }
-      int64_t gm = m0 + m, gk = k0 + kk;
-      float v = 0.f;
-      if (gm < p.M && gk < Keff) v = AT ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
+  for (int i = 0; i < IA; ++i) {
+    const int q = tid + 256 * i;
+    if (AT) { ag[i] = q & 3; ar[i] = q >> 2; }
+    else { ar[i] = q % TBM; ag[i] = q / TBM; }
+  }
 #pragma unroll
-      for (int s = 0; s < NSPLIT; ++s) {
-        unsigned short h = bf16_rne(v);
-        As[s][m][kk] = h;
-        v -= bf16_to_f(h);
-      }
-    }
+  for (int i = 0; i < IB; ++i) {
+    const int q = tid + 256 * i;
+    if (!BT) { bg[i] = q & 3; br[i] = q >> 2; }
+    else { br[i] = q % TBN; bg[i] = q / TBN; }
+  }
+  float ra[IA][8], rb[IB][8];
+  auto fetch = [&](int64_t k0) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      int n, kk;
-      if (!BT) { kk = tid & 31; n = (tid >> 5) + 8 * r; }
-      else { n = tid & 63; kk = (tid >> 6) + 4 * r; }
-      int64_t gn = n0 + n, gk = k0 + kk;
-      float v = 0.f;
-      if (gn < p.N && gk < Keff) v = BT ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
+    for (int i = 0; i < IA; ++i) load8<AT>(A, p.lda, m0 + ar[i], p.M, k0 + 8 * ag[i], Keff, vecA, ra[i]);
 #pragma unroll
-      for (int s = 0; s < NSPLIT; ++s) {
-        unsigned short h = bf16_rne(v);
-        Bs[s][n][kk] = h;
-        v -= bf16_to_f(h);
-      }
-    }
-    __syncthreads();
-    v8bf16 a[NSPLIT][2], b[NSPLIT][2];
-    const int kq = 8 * (lane >> 4);
+    for (int i = 0; i < IB; ++i) load8<!BT>(B, p.ldb, n0 + br[i], p.N, k0 + 8 * bg[i], Keff, vecB, rb[i]);
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < IA; ++i) split_store<NSPLIT>(&As[buf][0][ar[i]][8 * ag[i]], PA, ra[i]);
+#pragma unroll
+    for (int i = 0; i < IB; ++i) split_store<NSPLIT>(&Bs[buf][0][br[i]][8 * bg[i]], PB, rb[i]);
+  };
+
+  v4f32 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = (v4f32){0.f, 0.f, 0.f, 0.f};
+
+  const int64_t nk = (Keff + BK - 1) / BK;
+  fetch(0);
+  stage(0);
+  __syncthreads();
+  const int kq = 8 * (lane >> 4), lr = lane & 15;
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int cur = (int)(kt & 1);
+    if (kt + 1 < nk) fetch((kt + 1) * BK);
+    v8bf16 a[NSPLIT][TI], b[NSPLIT][TJ];
 #pragma unroll
     for (int s = 0; s < NSPLIT; ++s) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        v8u16 t = *reinterpret_cast<const v8u16*>(&As[s][wm * 32 + i * 16 + (lane & 15)][kq]);
-        a[s][i] = __builtin_bit_cast(v8bf16, t);
-      }
+      for (int i = 0; i < TI; ++i)
+        a[s][i] = __builtin_bit_cast(v8bf16, *reinterpret_cast<const v8u16*>(&As[cur][s][wm * WTM + 16 * i + lr][kq]));
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        v8u16 t = *reinterpret_cast<const v8u16*>(&Bs[s][wn * 32 + j * 16 + (lane & 15)][kq]);
-        b[s][j] = __builtin_bit_cast(v8bf16, t);
-      }
+      for (int j = 0; j < TJ; ++j)
+        b[s][j] = __builtin_bit_cast(v8bf16, *reinterpret_cast<const v8u16*>(&Bs[cur][s][wn * WTN + 16 * j + lr][kq]));
     }
+    // smallest products first
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int t = NSPLIT - 1; t >= 0; --t)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        // smallest products first
+      for (int sa = 0; sa <= t; ++sa)
 #pragma unroll
-        for (int t = NSPLIT - 1; t >= 0; --t)
+        for (int i = 0; i < TI; ++i)
 #pragma unroll
-          for (int sa = 0; sa <= t; ++sa)
+          for (int j = 0; j < TJ; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sa][i], b[t - sa][j], acc[i][j], 0, 0, 0);
-      }
+    if (kt + 1 < nk) stage(cur ^ 1);
     __syncthreads();
   }
+
+  const float al = (float)p.alpha, be = (float)p.beta;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int64_t row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
-        int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (row < p.M && col < p.N) {
-          float v = (float)p.alpha * acc[i][j][r];
-          float* dst = C + row + col * p.ldc;
-          if (p.beta != 0.0) v += (float)p.beta * *dst;
-          *dst = v;
+    for (int j = 0; j < TJ; ++j) {
+      const int64_t col = n0 + wn * WTN + 16 * j + lr;
+      const int64_t row0 = m0 + wm * WTM + 16 * i + 4 * (lane >> 4);
+      if (col >= p.N || row0 >= p.M) continue;
+      float* dst = C + row0 + col * p.ldc;
+      if (p.vecC && row0 + 3 < p.M) {
+        float4 v = make_float4(al * acc[i][j][0], al * acc[i][j][1], al * acc[i][j][2], al * acc[i][j][3]);
+        if (p.beta != 0.0) {
+          const float4 o = *reinterpret_cast<const float4*>(dst);
+          v.x += be * o.x; v.y += be * o.y; v.z += be * o.z; v.w += be * o.w;
         }
+        *reinterpret_cast<float4*>(dst) = v;
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (row0 + r < p.M) {
+            float v = al * acc[i][j][r];
+            if (p.beta != 0.0) v += be * dst[r];
+            dst[r] = v;
+          }
       }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -365,6 +439,19 @@ __global__ __launch_bounds__(256) void splitk_reduce(int64_t M, int64_t N, int64
 // ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
+// FP32 block tile: 128 x 64 for narrow outputs, else 128 x 128
+static int f32_tile_n(int64_t N) { return N <= 192 ? 64 : 128; }
+
+static bool al16(const void* ptr) { return ((uintptr_t)ptr & 15u) == 0; }
+
+// 16-B access flags of the FP32 kernel for operands as given in p
+static void set_vec_flags(const GemmArgs& g, GemmP& p) {
+  const bool one = g.batch == 1;
+  p.vecA = g.transA && al16(p.A) && p.lda % 4 == 0 && (one || p.sA % 4 == 0);
+  p.vecB = !g.transB && al16(p.B) && p.ldb % 4 == 0 && (one || p.sB % 4 == 0);
+  p.vecC = al16(p.C) && p.ldc % 4 == 0 && (p.sC % 4 == 0 || (one && p.kchunk == 0));
+}
+
 static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid) {
   dim3 block(256);
   const bool at = g.transA, bt = g.transB;
@@ -379,12 +466,17 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid) {
     else if (!at && bt) hipLaunchKernelGGL((gemm_c128_kernel<false, true>), grid, block, 0, c.stream, p);
     else hipLaunchKernelGGL((gemm_c128_kernel<true, true>), grid, block, 0, c.stream, p);
   } else {
-#define IPEPS_F32_LAUNCH(NS)                                                                         \
-  do {                                                                                               \
-    if (!at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<false, false, NS>), grid, block, 0, c.stream, p); \
-    else if (at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<true, false, NS>), grid, block, 0, c.stream, p); \
-    else if (!at && bt) hipLaunchKernelGGL((gemm_f32_kernel<false, true, NS>), grid, block, 0, c.stream, p); \
-    else hipLaunchKernelGGL((gemm_f32_kernel<true, true, NS>), grid, block, 0, c.stream, p);          \
+#define IPEPS_F32_LAUNCH_T(NS, TN)                                                                            \
+  do {                                                                                                        \
+    if (!at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<false, false, NS, 128, TN>), grid, block, 0, c.stream, p); \
+    else if (at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<true, false, NS, 128, TN>), grid, block, 0, c.stream, p); \
+    else if (!at && bt) hipLaunchKernelGGL((gemm_f32_kernel<false, true, NS, 128, TN>), grid, block, 0, c.stream, p); \
+    else hipLaunchKernelGGL((gemm_f32_kernel<true, true, NS, 128, TN>), grid, block, 0, c.stream, p);          \
+  } while (0)
+#define IPEPS_F32_LAUNCH(NS)                                  \
+  do {                                                        \
+    if (f32_tile_n(g.N) == 64) IPEPS_F32_LAUNCH_T(NS, 64);    \
+    else IPEPS_F32_LAUNCH_T(NS, 128);                         \
   } while (0)
     switch (g.mixed) {
       case Mixed::BF16: IPEPS_F32_LAUNCH(1); break;
@@ -392,6 +484,7 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid) {
       case Mixed::BF16X6: IPEPS_F32_LAUNCH(3); break;
     }
 #undef IPEPS_F32_LAUNCH
+#undef IPEPS_F32_LAUNCH_T
   }
   HIP_CHECK(hipGetLastError());
 }
@@ -416,7 +509,9 @@ void gemm(Ctx& c, const GemmArgs& g) {
   p.C = g.C; p.ldc = g.ldc; p.sC = g.sC;
   p.conjA = g.conjA; p.conjB = g.conjB;
   p.kchunk = 0;
-  dim3 grid((unsigned)((g.M + BM - 1) / BM), (unsigned)((g.N + BN - 1) / BN), (unsigned)g.batch);
+  const int64_t tbm = g.dt == DType::F32 ? 128 : BM, tbn = g.dt == DType::F32 ? f32_tile_n(g.N) : BN;
+  set_vec_flags(g, p);
+  dim3 grid((unsigned)((g.M + tbm - 1) / tbm), (unsigned)((g.N + tbn - 1) / tbn), (unsigned)g.batch);
   IPEPS_CHECK(grid.y <= 65535, "gemm: N too large for the grid");
   const bool at = g.transA, bt = g.transB;
   const char* tf[2] = {"false", "true"};
@@ -456,6 +551,7 @@ void gemm(Ctx& c, const GemmArgs& g) {
     q.kchunk = kc;
     q.sA = at ? kc : kc * g.lda;
     q.sB = bt ? kc * g.ldb : kc;
+    set_vec_flags(g, q);
     grid.z = (unsigned)S;
     TimedLaunch tl(c, name, flops, bytes);
     launch(c, g, q, grid);
