@@ -152,11 +152,21 @@ def main():
     log(f"[bench] mixed: {a.steps} steps in {t_m:.3f} s, E={E_m.real:.12f}")
     out_extra = {}
     if not a.no_fp64:
-        t_64, E_64, g_64, st_64, _ = run_path("f64", a.warmup, a.steps, {})
         import numpy as np
-        dE = abs(E_m.real - E_64.real)
-        gerr = float(np.linalg.norm(g_m - g_64) / np.linalg.norm(g_64))
-        log(f"[bench] fp64 : {a.steps} steps in {t_64:.3f} s, E={E_64.real:.12f}, |dE|={dE:.2e}, grad rel {gerr:.2e}")
+        t_64, E_64, g_64, st_64, ts_64 = run_path("f64", a.warmup, a.steps, {})
+        log(f"[bench] fp64 : {a.steps} steps in {t_64:.3f} s, E={E_64.real:.12f}")
+        # Precision error of one step + adjoint: both engines start from the
+        # same FP64 environment (the FP64 run's last state) with the same
+        # random test matrices, so the difference is the mixed arithmetic.
+        Cs, Es = ts_64.Cs, ts_64.Es
+        same = {}
+        for dt, extra in (("f64", {}), ("mixed", dict(mixed=a.mixed))):
+            ts = pkg.CTMTensors(A, {}, dtype=dt, options=dict(base_opts, **extra))
+            ts.set_env(Cs, Es)
+            same[dt] = pkg.step_adjoint(ts, chi, H)
+        dE = abs(same["mixed"][0].real - same["f64"][0].real)
+        gerr = float(np.linalg.norm(same["mixed"][1] - same["f64"][1]) / np.linalg.norm(same["f64"][1]))
+        log(f"[bench] same-input step+adjoint: |dE/site| mixed vs fp64 = {dE:.2e}, grad rel = {gerr:.2e}")
         out_extra["fp64_gpu"] = dict(value=a.steps / t_64, ms_per_step=1e3 * t_64 / a.steps,
                                      roofline=kernel_roofline(st_64))
         out_extra["speedup_vs_fp64_gpu"] = t_64 / t_m

@@ -66,11 +66,17 @@ int ipeps_env_destroy(ipeps_env* env);
 /* Engine options: "svd_mode" (0 full SVD of M = R1 R2 as the reference,
  * 1 randomized range finder), "oversample", "power_iters", "full_below",
  * "seed", "eta", "canonical_edge" (0/1), "mixed" (1, 3 or 6 bf16 products),
- * "energy_fp64" (0/1), "checkpoint" (0/1), and for the randomized mode
+ * "energy_fp64" (0/1: energy contractions of a MIXED environment in FP64),
+ * "energy_mixed" (1, 3 or 6: bf16 products of those contractions when not
+ * FP64, default 6; the density-matrix traces and ratios are FP64 always),
+ * "checkpoint" (0/1), and for the randomized mode
  * "fast_orth" (0/1: CholeskyQR2 range bases with Householder fallback),
  * "fast_svd" (0/1: small SVD from Gram eigenpairs with one-sided Jacobi
  * fallback) and "gram_guard" (s_min/s_max below which the Gram route is not
- * taken, default 1e-4). */
+ * taken, default 1e-4).  "draws" sets the position in the environment's
+ * stream of random test matrices (keyed by "seed"; it advances by one per
+ * randomized projector): environments with equal seed, draws and tensors
+ * build identical projectors whatever their dtype. */
 int ipeps_env_set_option(ipeps_env* env, const char* key, double value);
 /* which: 0..3 = C1..C4, 4..7 = E1..E4, 8 = A.  dims has room for 5. */
 int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd);

@@ -89,7 +89,11 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
   else if (key == "canonical_edge") o.canonical_edge = v != 0.0;
   else if (key == "energy_fp64") o.energy_fp64 = v != 0.0;
   else if (key == "checkpoint") o.checkpoint = v != 0.0;
-  else if (key == "mixed") {
+  else if (key == "energy_mixed") {
+    int m = (int)v;
+    IPEPS_CHECK(m == 1 || m == 3 || m == 6, "energy_mixed must be 1, 3 or 6");
+    o.energy_mixed = (k::Mixed)m;
+  } else if (key == "mixed") {
     int m = (int)v;
     IPEPS_CHECK(m == 1 || m == 3 || m == 6, "mixed must be 1, 3 or 6");
     o.mixed = (k::Mixed)m;
@@ -217,7 +221,12 @@ int ipeps_env_destroy(ipeps_env* env) {
 int ipeps_env_set_option(ipeps_env* env, const char* key, double value) {
   return guard([&] {
     IPEPS_CHECK(env && key, "null argument");
-    set_opt(env->env.opt, key, value);
+    if (std::string(key) == "draws") {
+      IPEPS_CHECK(value >= 0.0, "draws must be >= 0");
+      env->env.draws = (uint64_t)value;
+    } else {
+      set_opt(env->env.opt, key, value);
+    }
   });
 }
 

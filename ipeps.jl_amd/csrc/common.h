@@ -128,7 +128,6 @@ struct Ctx {
   hipStream_t stream = nullptr;
   std::unique_ptr<Pool> pool;
   void* solver = nullptr;  // rocblas_handle
-  uint64_t rng_counter = 0x9e3779b97f4a7c15ull;
   // Kernel-time instrumentation: when `timing` is set, launches are bracketed
   // by hipEvents on this stream and resolved by kernel_stats().
   struct KEvent {

@@ -31,6 +31,7 @@ struct Proj {
 // P1 = R2 V S^-1/2, P2 = S^-1/2 U^H R1.
 Proj get_projector(Ctx& c, const Env& e, const Tensor& R1, const Tensor& R2, int64_t chi) {
   op::SvdOpts so = e.opt.svd;
+  so.seed = op::draw_seed(e.opt.svd.seed, e.draws++);
   auto r = op::svd_trunc(c, R1, R2, chi, so);
   Tensor isq = op::inv_sqrt(c, r.S);
   Proj p;
@@ -315,7 +316,7 @@ static Env as_dtype(Ctx& c, const Env& e, DType dt) {
 
 Tensor energy(Ctx& c, const Env& e0, const Tensor& H1, const Tensor& H2) {
   Env e = (e0.dt == DType::F32 && e0.opt.energy_fp64) ? as_dtype(c, e0, DType::F64) : e0;
-  op::prec().mixed = e.opt.mixed;
+  op::PrecScope ps(e.opt.energy_mixed);
   Tensor roh, rov;
   get_dms(c, e, roh, rov);
   if (roh.dt == DType::F32) {
@@ -422,10 +423,12 @@ namespace {
 
 struct Checkpoint {
   std::array<Tensor, 4> C, E;
+  uint64_t draws = 0;
 };
 
 Checkpoint save(const Env& e) {
   Checkpoint k;
+  k.draws = e.draws;
   for (int i = 0; i < 4; ++i) {
     k.C[i] = e.C[i];
     k.E[i] = e.E[i];
@@ -436,6 +439,7 @@ Checkpoint save(const Env& e) {
 }
 
 void load(Env& e, const Checkpoint& k) {
+  e.draws = k.draws;
   for (int i = 0; i < 4; ++i) {
     e.C[i] = k.C[i];
     e.E[i] = k.E[i];

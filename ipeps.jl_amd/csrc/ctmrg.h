@@ -15,7 +15,10 @@ struct Options {
   op::SvdOpts svd;             // projector SVD (full = reference semantics, or randomized)
   bool canonical_edge = true;  // P2-first edge update (chi^3 D^4) instead of the chi^2 D^6 intermediate
   k::Mixed mixed = k::Mixed::BF16X3;
-  bool energy_fp64 = true;     // energy contractions in FP64 even for mixed environments
+  // Energy contractions of an F32 environment: FP64 (energy_fp64) or split-bf16
+  // with `energy_mixed` products; the trace and the ratios are FP64 either way.
+  bool energy_fp64 = false;
+  k::Mixed energy_mixed = k::Mixed::BF16X6;
   bool checkpoint = false;     // recompute each step in the reverse sweep instead of keeping its tape
 };
 
@@ -27,6 +30,11 @@ struct Env {
   std::array<Tensor, 4> C;      // C1 (down, right), C2 (left, down), C3 (up, left), C4 (up, right)
   std::array<Tensor, 4> E;      // E1 (l, r, ket, bra), E2 (u, d, ., .), E3 (l, r, ., .), E4 (u, d, ., .)
   Options opt;
+  // Randomized projectors draw their test matrices from a counter-based
+  // stream keyed by (opt.svd.seed, draws): two environments in the same
+  // state with the same seed draw the same matrices whatever their dtype,
+  // and a recomputed step replays its forward draws.
+  mutable uint64_t draws = 0;
 };
 
 // Boundary initialisation C_i, E_i from A and Ad (src/ctm/ctm_tensor.jl:97-130).

@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include "kernels.h"
+#include "permute_impl.h"
 
 namespace ipeps {
 namespace k {
@@ -60,25 +61,44 @@ __global__ __launch_bounds__(TPB) void permute_kernel(PermP p, const T* __restri
   }
 }
 
-// 2-D tiled transpose for the common case "first two output axes swap the
-// fastest input axis": out[j, i, rest] = in[i, j, rest] through LDS.
+// Rows kernel (fastest output axis = fastest input axis); see permute_impl.h.
 template <typename T>
-__global__ __launch_bounds__(TPB) void transpose_kernel(int64_t R, int64_t Cc, int64_t batch, const T* __restrict__ in,
-                                                        T* __restrict__ out, int conj) {
-  __shared__ T tile[32][33];
-  int64_t b = blockIdx.z;
-  int64_t r0 = (int64_t)blockIdx.x * 32, c0 = (int64_t)blockIdx.y * 32;
-  const T* src = in + b * R * Cc;
-  T* dst = out + b * R * Cc;
-  int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  for (int j = ty; j < 32; j += 8) {
-    int64_t r = r0 + tx, c = c0 + j;
-    if (r < R && c < Cc) tile[j][tx] = src[r + c * R];
-  }
+__global__ __launch_bounds__(TPB) void permute_rows_kernel(PermRowsP p, const T* __restrict__ in,
+                                                           T* __restrict__ out, int conj) {
+  __shared__ uint32_t row_off[256];
+  const uint32_t blk = blockIdx.x;
+  perm_rows_phase1(p, blk, threadIdx.x, row_off);
   __syncthreads();
-  for (int j = ty; j < 32; j += 8) {
-    int64_t c = c0 + tx, r = r0 + j;
-    if (r < R && c < Cc) dst[c + r * Cc] = conj_if(tile[tx][j], conj != 0);
+  const uint32_t total = p.rb * p.chunk;
+  for (uint32_t e = threadIdx.x; e < total; e += TPB) {
+    uint32_t io, oo;
+    if (perm_rows_elem(p, blk, e, row_off, io, oo)) out[oo] = conj_if(in[io], conj != 0);
+  }
+}
+
+// Tiled kernel (the input-contiguous axis j is not the first output axis);
+// TS x TS tiles through LDS, see permute_impl.h.
+template <typename T, int TS>
+__global__ __launch_bounds__(TPB) void permute_tiled_kernel(PermTileP p, const T* __restrict__ in,
+                                                            T* __restrict__ out, int conj) {
+  __shared__ T tile[TS][TS + 1];
+  __shared__ uint32_t lead_off[TS];
+  for (uint32_t rest = blockIdx.y; rest < p.nrest_total; rest += gridDim.y) {
+    uint32_t x0, y0, in_rest, out_rest;
+    perm_tile_block(p, blockIdx.x, rest, x0, y0, in_rest, out_rest);
+    if (threadIdx.x < TS && y0 + threadIdx.x < p.wl) lead_off[threadIdx.x] = perm_tile_lead(p, y0 + threadIdx.x);
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < TS * TS; e += TPB) {
+      const uint32_t x = e % TS, y = e / TS;
+      if (x0 + x < p.wj && y0 + y < p.wl) tile[y][x] = in[in_rest + lead_off[y] + x0 + x];
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < TS * TS; e += TPB) {
+      const uint32_t y = e % TS, x = e / TS;
+      if (x0 + x < p.wj && y0 + y < p.wl)
+        out[out_rest + y0 + y + (x0 + x) * p.ostr_j] = conj_if(tile[y][x], conj != 0);
+    }
+    __syncthreads();
   }
 }
 
@@ -636,34 +656,53 @@ void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::v
     else HIP_CHECK(hipMemcpyAsync(out, in, (size_t)n * dtype_size(dt), hipMemcpyDeviceToDevice, c.stream));
     return;
   }
-  // out axis0 = in axis j (stride is[0]), and in axis0 (stride 1) = out axis i:
-  // a 2-D transpose of [R x Cc] blocks when the permutation is [1,0,rest...]
-  if (od.size() >= 2 && is[1] == 1 && is[0] == od[1]) {
-    int64_t R = od[1], Cc = od[0];
-    bool rest_ok = true;
-    int64_t expect = R * Cc;
-    for (size_t a = 2; a < od.size(); ++a) {
-      if (is[a] != expect) { rest_ok = false; break; }
-      expect *= od[a];
-    }
-    if (rest_ok) {
-      int64_t batch = n / (R * Cc);
-      if (batch <= 65535 && (Cc + 31) / 32 <= 65535) {
-        dim3 grid((unsigned)((R + 31) / 32), (unsigned)((Cc + 31) / 32), (unsigned)batch);
-        TimedLaunch tl(c, std::string("transpose_kernel<") + dtype_name(dt) + ">", 0.0, 2.0 * n * dtype_size(dt));
-#define TK(T, ...) hipLaunchKernelGGL(transpose_kernel<T>, grid, dim3(TPB), 0, c.stream, R, Cc, batch, (const T*)in, (T*)out, cj)
-        DISPATCH3(dt, TK, 0);
-#undef TK
-        HIP_CHECK(hipGetLastError());
-        return;
+  if (n < ((int64_t)1 << 31)) {
+    PermRowsP rp;
+    PermTileP tp;
+    const int esize = (int)dtype_size(dt);
+    const int kind = plan_permute((int)od.size(), od.data(), is.data(), esize, rp, tp);
+    if (kind >= 0) {
+      std::string kname = std::string(kind == 0 ? "permute_rows_kernel<" : "permute_tiled_kernel<") + dtype_name(dt) + ">";
+      if (c.timing_detail) {
+        kname += " od=";
+        for (size_t i = 0; i < od.size(); ++i) kname += std::to_string(od[i]) + (i + 1 < od.size() ? "," : "");
+        kname += " is=";
+        for (size_t i = 0; i < is.size(); ++i) kname += std::to_string(is[i]) + (i + 1 < is.size() ? "," : "");
       }
+      TimedLaunch tl(c, kname, 0.0, 2.0 * n * dtype_size(dt));
+      if (kind == 0) {
+        const unsigned nb = (unsigned)(((int64_t)rp.nrows + rp.rb - 1) / rp.rb * rp.nchunk);
+#define RK(T, ...) hipLaunchKernelGGL(permute_rows_kernel<T>, dim3(nb), dim3(TPB), 0, c.stream, rp, (const T*)in, (T*)out, cj)
+        DISPATCH3(dt, RK, 0);
+#undef RK
+      } else {
+        dim3 grid(tp.ntx * tp.nty, std::min<uint32_t>(tp.nrest_total, 65535u));
+        if (esize >= 16)
+          hipLaunchKernelGGL((permute_tiled_kernel<double2, 32>), grid, dim3(TPB), 0, c.stream, tp, (const double2*)in,
+                             (double2*)out, cj);
+        else if (esize == 8)
+          hipLaunchKernelGGL((permute_tiled_kernel<double, 64>), grid, dim3(TPB), 0, c.stream, tp, (const double*)in,
+                             (double*)out, cj);
+        else
+          hipLaunchKernelGGL((permute_tiled_kernel<float, 64>), grid, dim3(TPB), 0, c.stream, tp, (const float*)in,
+                             (float*)out, cj);
+      }
+      HIP_CHECK(hipGetLastError());
+      return;
     }
   }
   PermP p{};
   p.nd = (int)od.size();
   for (int i = 0; i < p.nd; ++i) { p.odims[i] = od[i]; p.istr[i] = is[i]; }
   p.n = n;
-  TimedLaunch tl(c, std::string("permute_kernel<") + dtype_name(dt) + ">", 0.0, 2.0 * n * dtype_size(dt));
+  std::string pname = std::string("permute_kernel<") + dtype_name(dt) + ">";
+  if (c.timing_detail) {
+    pname += " od=";
+    for (int i = 0; i < p.nd; ++i) pname += std::to_string(od[i]) + (i + 1 < p.nd ? "," : "");
+    pname += " is=";
+    for (int i = 0; i < p.nd; ++i) pname += std::to_string(is[i]) + (i + 1 < p.nd ? "," : "");
+  }
+  TimedLaunch tl(c, pname, 0.0, 2.0 * n * dtype_size(dt));
 #define PK(T, ...) hipLaunchKernelGGL(permute_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, p, (const T*)in, (T*)out, cj)
   DISPATCH3(dt, PK, 0);
 #undef PK

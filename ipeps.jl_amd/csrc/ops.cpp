@@ -416,7 +416,9 @@ Tensor tcon(Ctx& c, const Tensor& A, const std::vector<int>& la, const Tensor& B
         la_b[p] = -(q + 1);
       }
     bool gA = needs(A), gB = needs(B);
+    const k::Mixed mx = prec().mixed;
     C.node = make_node([=](Ctx& c, Node& n) {
+      PrecScope ps(mx);
       if (gA) acc(c, nA, tcon_raw(c, n.grad, lc_a, false, Bk, lb_a, true));
       if (gB) acc(c, nB, tcon_raw(c, Ak, la_b, true, n.grad, lc_b, false));
     });
@@ -810,6 +812,13 @@ static Tensor first_n(Ctx& c, const Tensor& s, int64_t n) {
   return r;
 }
 
+uint64_t draw_seed(uint64_t base, uint64_t i) {
+  uint64_t z = base + (i + 1) * 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
 static DType solve_dt(DType dt) { return dt == DType::F32 ? DType::F64 : dt; }
 
 SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, const SvdOpts& o) {
@@ -830,7 +839,7 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
   } else {
     const int64_t kk = std::min<int64_t>(kmin, ntar + o.oversample);
     Tensor Om = c.empty(dt, {nc, kk});
-    k::randn(c, dt, nc * kk, o.seed ^ (c.rng_counter += 0x9e3779b97f4a7c15ull), Om.data());
+    k::randn(c, dt, nc * kk, o.seed, Om.data());
     auto apply_M = [&](const Tensor& X) {  // M X = R1 (R2 X)
       return matmul_raw(c, R1, false, false, matmul_raw(c, R2, false, false, X, false, false), false, false);
     };
@@ -883,7 +892,9 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     // kept and computed-but-discarded triplets are exact and only triplets
     // beyond k are treated as zero.
     Tensor Ub = Uf, Vb = Vf, Sb = Sf;
+    const k::Mixed mx = prec().mixed;
     base->back = [=](Ctx& c, Node& b) {
+      PrecScope ps(mx);
       // svd_back (src/utility/autodiff.jl:41-75) for U (m x kb), V (nc x kb):
       //   Mbar = U G V^H + (dU - U U^H dU) Sinv V^H + U Sinv (dV - V V^H dV)^H
       // with the cotangents dU, dV (m x n, nc x n) zero beyond column n.

@@ -46,6 +46,13 @@ struct Prec {
   k::Mixed mixed = k::Mixed::BF16X3;
 };
 Prec& prec();
+// Sets prec().mixed for a scope (the VJPs restore the setting their forward
+// op ran with).
+struct PrecScope {
+  k::Mixed prev;
+  explicit PrecScope(k::Mixed m) : prev(prec().mixed) { prec().mixed = m; }
+  ~PrecScope() { prec().mixed = prev; }
+};
 
 bool recording();
 // Mark t as a differentiable leaf on the current tape.
@@ -122,7 +129,10 @@ struct SvdOpts {
 // mode 0: M is formed, full SVD, adjoint = full svd_back (autodiff.jl:41-75).
 // mode 1: randomized range finder on M = R1 R2, adjoint = thin svd_back
 //         propagated to R1 and R2 without forming M.
+// o.seed is used as is for the test matrix (see draw_seed).
 SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, const SvdOpts& o);
+// Seed of the i-th draw of a stream keyed by `base` (splitmix64 finaliser).
+uint64_t draw_seed(uint64_t base, uint64_t i);
 
 // Orthonormal basis of the columns of Y (N x k, F64 or C128): Householder QR.
 Tensor orth(Ctx& c, const Tensor& Y, bool fast = false);

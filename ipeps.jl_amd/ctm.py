@@ -111,7 +111,8 @@ def _hpair(H):
 
 
 ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "fast_orth", "fast_svd",
-               "gram_guard", "canonical_edge", "mixed", "energy_fp64", "checkpoint")
+               "gram_guard", "canonical_edge", "mixed", "energy_fp64", "energy_mixed",
+               "checkpoint", "draws")
 
 
 class CTMTensors:

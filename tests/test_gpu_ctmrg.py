@@ -51,6 +51,28 @@ def test_tcon_matches_einsum(pkg, cplx):
         np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(ref).max()))
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_permutes_bit_exact(pkg, cplx):
+    """Leg permutations (rows and LDS-tiled permute kernels) through tcon with a
+    one-element partner: X (x) [1.0] with the output legs reordered must equal
+    numpy's transpose bit for bit, including shapes whose merged row length
+    exceeds one block (row chunking) and tiles with ragged edges."""
+    rng = np.random.default_rng(11)
+    shapes = [(144, 6, 6, 6), (6, 6, 2, 144, 3), (70, 3, 129), (2, 5000, 3), (5000, 2, 3), (3, 1, 7, 1, 5),
+              (9, 8, 7, 6, 5, 4), (65, 65, 3)]
+    for shp in shapes:
+        for _ in range(4):
+            perm = rng.permutation(len(shp))
+            X = rng.standard_normal(shp)
+            if cplx:
+                X = X + 1j * rng.standard_normal(shp)
+            la = [-(int(np.where(perm == a)[0][0]) + 1) for a in range(len(shp))]
+            got = pkg.tcon([X, np.ones(1, dtype=X.dtype)], [la, [-(len(shp) + 1)]])
+            ref = np.transpose(X, perm)[..., None]
+            assert got.shape == ref.shape, (shp, perm)
+            assert np.array_equal(got, ref), (shp, perm)
+
+
 def test_large_gemm(pkg):
     rng = np.random.default_rng(1)
     A = rng.standard_normal((300, 257))
@@ -396,3 +418,24 @@ def test_gemm_all_ops(pkg, kind, shape):
                     got = pkg.gemm(A, B, tA, cA, tB, cB, mixed=mixed, alpha=0.7, beta=0.3, C=C0)
                     err = np.abs(got - ref).max() / np.abs(ref).max()
                     assert err < tol, (kind, tA, cA, tB, cB, err)
+
+
+@pytest.mark.parametrize("emode", ["fp64", "bf16x6", "bf16x3"])
+def test_mixed_energy_contractions_fixed_env(pkg, emode):
+    """Energy of one fixed FP64 environment evaluated by a mixed engine (FP32
+    storage): contractions in FP64, or split-bf16 with 6 / 3 products; the
+    density-matrix traces and ratios are FP64 in every mode.  Bound on
+    |E_mixed - E_fp64| per site: 1e-7 for FP64 and bf16x6 contractions (the
+    stated mixed-path tolerance), 1e-5 for bf16x3."""
+    A = O.seeded_tensor((3, 3, 3, 3, 2), 4).numpy()
+    H = [h.numpy() for h in O.heisenberg(1.0)]
+    ts = pkg.CTMTensors(A, dict(chi=24, min_iter=8, max_iter=8, rg_tol=0.0))
+    ts, _ = pkg.run_ctm(ts, "energy", H)
+    E64, _ = pkg.get_gs_energy(ts, H)
+    opts = {"fp64": dict(energy_fp64=1), "bf16x6": dict(energy_fp64=0, energy_mixed=6),
+            "bf16x3": dict(energy_fp64=0, energy_mixed=3)}[emode]
+    tm = pkg.CTMTensors(A, dtype="mixed", options=opts)
+    tm.set_env(ts.Cs, ts.Es)
+    Em, _ = pkg.get_gs_energy(tm, H)
+    print(emode, "E64", E64.real, "dE", abs(Em.real - E64.real))
+    assert abs(Em.real - E64.real) < (1e-5 if emode == "bf16x3" else 1e-7)
