@@ -35,6 +35,7 @@ extern "C" {
 
 typedef struct ipeps_ctx ipeps_ctx;
 typedef struct ipeps_env ipeps_env;
+typedef struct ipeps_batch ipeps_batch;
 
 /* Library version (major*10000 + minor*100 + patch). */
 int ipeps_version(void);
@@ -65,7 +66,9 @@ int ipeps_env_create(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const voi
 int ipeps_env_destroy(ipeps_env* env);
 /* Engine options: "svd_mode" (0 full SVD of M = R1 R2 as the reference,
  * 1 randomized range finder), "oversample", "power_iters", "full_below",
- * "seed", "eta", "canonical_edge" (0/1), "mixed" (1, 3 or 6 bf16 products),
+ * "seed", "eta", "mixed_cutoff" (relative singular-value cutoff of MIXED
+ * environments, default 1e-6; F64 / C128 keep the reference's 1e-12),
+ * "canonical_edge" (0/1), "mixed" (1, 3 or 6 bf16 products),
  * "energy_fp64" (0/1: energy contractions of a MIXED environment in FP64),
  * "energy_mixed" (1, 3 or 6: bf16 products of those contractions when not
  * FP64, default 6; the density-matrix traces and ratios are FP64 always),
@@ -169,6 +172,28 @@ int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double guard, int64_t 
  * the reference).  dA_out is N x N. */
 int ipeps_svd_back(ipeps_ctx* ctx, int dtype, int64_t N, int64_t n, const void* U, const double* S, const void* V,
                    const void* dU, const double* dS, const void* dV, double eta, void* dA_out);
+
+/* Batched independent environments (many instances of CTMTensors +
+ * rg_step, src/ctm/ctm_tensor.jl:63-69 and src/ctm/ctmrg.jl:57-64; the
+ * reference runs such instances one after another, e.g. the per-basis-vector
+ * loop of optim_es.jl:127-153 and its es_resume / part_save job split).
+ * n instances with their own A (dims: n x 5, A[i] host data in the dtype's
+ * host format), chi[i], and Hamiltonian terms H1[i], H2[i] (hdim[i] =
+ * d^2); init_ctm1 runs for each.  `workers` contexts (one HIP stream and host
+ * thread each, 1..64) share the instances by a longest-processing-time
+ * assignment on the per-step cost.  options: "key=value;..." as
+ * ipeps_env_set_option, applied to every instance. */
+int ipeps_batch_create(int device, int n, int dtype, const int64_t* dims, const void* const* A, const int64_t* chi,
+                       const double* const* H1, const double* const* H2, const int64_t* hdim, int workers,
+                       const char* options, ipeps_batch** out);
+int ipeps_batch_destroy(ipeps_batch* b);
+/* Advance every instance by `steps` rg_steps, then evaluate get_gs_energy
+ * (src/graph/evaluation.jl:15-29) of each: E_out (n x 2: re, im); wall time
+ * of the call in seconds_out. */
+int ipeps_batch_run(ipeps_batch* b, int steps, double* E_out, double* seconds_out);
+/* Copy out tensor `which` (as ipeps_env_get_tensor) of instance `inst`;
+ * dims (room for 5) / nd receive its shape; host may be NULL. */
+int ipeps_batch_get_tensor(ipeps_batch* b, int inst, int which, int64_t* dims, int* nd, void* host);
 
 #ifdef __cplusplus
 }

@@ -1,8 +1,12 @@
 // extern "C" entry points (declared in include/ipeps_hip.h).
+#include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <sstream>
 #include <string>
+#include <thread>
 
 #include "../../include/ipeps_hip.h"
 #include "ctmrg.h"
@@ -20,6 +24,27 @@ struct ipeps_env {
   std::shared_ptr<Ctx> hold;  // declared first: outlives the tensors in env
   int dtype = IPEPS_F64;
   ctm::Env env;
+};
+
+// Independent environments advanced together by a pool of workers.  Each
+// worker owns a context (HIP stream, allocator, solver handle) and the
+// environments assigned to it; the workers run on their own host threads, so
+// their streams keep the device busy concurrently while any one of them waits
+// on a host-side truncation decision.
+struct ipeps_batch {
+  struct Inst {
+    int worker = 0;
+    int64_t chi = 0;
+    double cost = 0.0;
+    ctm::Env env;
+    Tensor H1, H2;
+    double E[2] = {0, 0};
+    int64_t steps = 0;
+  };
+  int device = 0;
+  int dtype = IPEPS_F64;
+  std::vector<std::shared_ptr<Ctx>> workers;  // declared before insts: outlives their tensors
+  std::vector<Inst> insts;
 };
 
 namespace {
@@ -83,6 +108,7 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
   else if (key == "full_below") o.svd.full_below = (int)v;
   else if (key == "seed") o.svd.seed = (uint64_t)v;
   else if (key == "eta") o.svd.eta = v;
+  else if (key == "mixed_cutoff") o.svd.mixed_cutoff = v;
   else if (key == "fast_orth") o.svd.fast_orth = (int)v;
   else if (key == "fast_svd") o.svd.fast_svd = (int)v;
   else if (key == "gram_guard") o.svd.gram_guard = v;
@@ -475,6 +501,127 @@ int ipeps_svd_back(ipeps_ctx* ctx, int dtype, int64_t N, int64_t n, const void* 
                  VdV.defined() ? VdV.data() : nullptr, tdS.defined() ? tdS.ptr<double>() : nullptr, eta, G.data());
     Tensor R = op::matmul_raw(c, op::matmul_raw(c, tU, false, false, G, false, false), false, false, tV, true, true);
     download(c, R, dt, dA_out);
+  });
+}
+
+// ---- batched independent environments -------------------------------------
+
+int ipeps_batch_create(int device, int n, int dtype, const int64_t* dims, const void* const* A, const int64_t* chi,
+                       const double* const* H1, const double* const* H2, const int64_t* hdim, int workers,
+                       const char* options, ipeps_batch** out) {
+  return guard([&] {
+    IPEPS_CHECK(out && n > 0 && dims && A && chi && H1 && H2 && hdim, "bad arguments");
+    IPEPS_CHECK(workers >= 1 && workers <= 64, "workers must be 1..64");
+    ctm::Options opt;
+    opt.mixed = k::Mixed::BF16X3;
+    parse_opts(opt, options);
+    std::unique_ptr<ipeps_batch> b(new ipeps_batch());
+    b->device = device;
+    b->dtype = dtype;
+    b->insts.resize(n);
+    // longest-processing-time assignment by the per-step cost estimate
+    // chi^3 D^4 + (chi D^2)^2 (chi + 16) of the corner and projector terms
+    for (int i = 0; i < n; ++i) {
+      const int64_t* d = dims + 5 * i;
+      for (int a = 0; a < 5; ++a) IPEPS_CHECK(d[a] > 0, "A dims must be positive");
+      IPEPS_CHECK(d[0] == d[1] && d[1] == d[2] && d[2] == d[3], "A must be (D,D,D,D,d)");
+      IPEPS_CHECK(chi[i] > 0 && A[i] && H1[i] && H2[i] && hdim[i] == d[4] * d[4], "bad instance arguments");
+      const double D2 = (double)(d[0] * d[0]), x = (double)chi[i];
+      b->insts[i].chi = chi[i];
+      b->insts[i].cost = x * x * x * D2 * D2 + (x * D2) * (x * D2) * (x + 16.0);
+    }
+    const int nw = std::min(workers, n);
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->insts[x].cost > b->insts[y].cost; });
+    std::vector<double> load(nw, 0.0);
+    for (int i : order) {
+      int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      b->insts[i].worker = w;
+      load[w] += b->insts[i].cost;
+    }
+    for (int w = 0; w < nw; ++w)
+      b->workers.emplace_back(std::shared_ptr<Ctx>(ctx_create(device), [](Ctx* p) { ctx_destroy(p); }));
+    for (int i = 0; i < n; ++i) {
+      auto& in = b->insts[i];
+      Ctx& c = *b->workers[in.worker];
+      const int64_t* d = dims + 5 * i;
+      TapeScope ts(nullptr);
+      in.env.dt = dev_dt(dtype);
+      in.env.opt = opt;
+      in.env.A = upload(c, host_dt(dtype), dev_dt(dtype), {d[0], d[1], d[2], d[3], d[4]}, A[i]);
+      in.env.Ad = op::conj(c, in.env.A);
+      ctm::init_ctm1(c, in.env);
+      in.H1 = upload_H(c, H1[i], hdim[i]);
+      in.H2 = upload_H(c, H2[i], hdim[i]);
+    }
+    *out = b.release();
+  });
+}
+
+int ipeps_batch_destroy(ipeps_batch* b) {
+  return guard([&] { delete b; });
+}
+
+int ipeps_batch_run(ipeps_batch* b, int steps, double* E_out, double* seconds_out) {
+  return guard([&] {
+    IPEPS_CHECK(b && steps >= 0, "bad arguments");
+    const int nw = (int)b->workers.size();
+    std::vector<std::string> errs(nw);
+    auto t0 = std::chrono::steady_clock::now();
+    auto work = [&](int w) {
+      try {
+        HIP_CHECK(hipSetDevice(b->device));
+        Ctx& c = *b->workers[w];
+        TapeScope ts(nullptr);
+        for (int s = 0; s < steps; ++s)
+          for (auto& in : b->insts)
+            if (in.worker == w) {
+              ctm::rg_step(c, in.env, in.chi);
+              ++in.steps;
+            }
+        for (auto& in : b->insts)
+          if (in.worker == w) {
+            Tensor e = ctm::energy(c, in.env, in.H1, in.H2);
+            double h[8];
+            to_host(c, e, h);
+            in.E[0] = h[0];
+            in.E[1] = h[1];
+          }
+        c.sync();
+      } catch (const std::exception& e) {
+        errs[w] = e.what();
+      }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nw; ++w) th.emplace_back(work, w);
+    work(0);
+    for (auto& t : th) t.join();
+    auto t1 = std::chrono::steady_clock::now();
+    for (int w = 0; w < nw; ++w)
+      if (!errs[w].empty()) throw Error(IPEPS_ERR_INTERNAL, "batch worker " + std::to_string(w) + ": " + errs[w]);
+    if (seconds_out) *seconds_out = std::chrono::duration<double>(t1 - t0).count();
+    if (E_out)
+      for (size_t i = 0; i < b->insts.size(); ++i) {
+        E_out[2 * i] = b->insts[i].E[0];
+        E_out[2 * i + 1] = b->insts[i].E[1];
+      }
+  });
+}
+
+// which as ipeps_env_get_tensor; dims (room for 5) and nd describe the tensor
+int ipeps_batch_get_tensor(ipeps_batch* b, int inst, int which, int64_t* dims, int* nd, void* host) {
+  return guard([&] {
+    IPEPS_CHECK(b && inst >= 0 && inst < (int)b->insts.size(), "bad instance");
+    auto& in = b->insts[inst];
+    Ctx& c = *b->workers[in.worker];
+    HIP_CHECK(hipSetDevice(b->device));
+    const Tensor& t = env_slot(in.env, which);
+    if (dims && nd) {
+      *nd = t.nd();
+      for (int i = 0; i < t.nd() && i < 5; ++i) dims[i] = t.dims[i];
+    }
+    if (host) download(c, t, host_dt(b->dtype), host);
   });
 }
 

@@ -439,8 +439,11 @@ __global__ __launch_bounds__(256) void splitk_reduce(int64_t M, int64_t N, int64
 // ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
-// FP32 block tile: 128 x 64 for narrow outputs, else 128 x 128
-static int f32_tile_n(int64_t N) { return N <= 192 ? 64 : 128; }
+// FP32 block tile: 128 x 64 for outputs up to 64 columns; 64 x 192 for
+// 129..192 columns (one tile row spans the output, so the M x K operand is
+// streamed from HBM once); else 128 x 128.
+static int f32_tile_n(int64_t N) { return N <= 64 ? 64 : (N > 128 && N <= 192) ? 192 : 128; }
+static int f32_tile_m(int64_t N) { return f32_tile_n(N) == 192 ? 64 : 128; }
 
 static bool al16(const void* ptr) { return ((uintptr_t)ptr & 15u) == 0; }
 
@@ -466,17 +469,19 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid) {
     else if (!at && bt) hipLaunchKernelGGL((gemm_c128_kernel<false, true>), grid, block, 0, c.stream, p);
     else hipLaunchKernelGGL((gemm_c128_kernel<true, true>), grid, block, 0, c.stream, p);
   } else {
-#define IPEPS_F32_LAUNCH_T(NS, TN)                                                                            \
-  do {                                                                                                        \
-    if (!at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<false, false, NS, 128, TN>), grid, block, 0, c.stream, p); \
-    else if (at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<true, false, NS, 128, TN>), grid, block, 0, c.stream, p); \
-    else if (!at && bt) hipLaunchKernelGGL((gemm_f32_kernel<false, true, NS, 128, TN>), grid, block, 0, c.stream, p); \
-    else hipLaunchKernelGGL((gemm_f32_kernel<true, true, NS, 128, TN>), grid, block, 0, c.stream, p);          \
+#define IPEPS_F32_LAUNCH_T(NS, TM, TN)                                                                       \
+  do {                                                                                                       \
+    if (!at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<false, false, NS, TM, TN>), grid, block, 0, c.stream, p); \
+    else if (at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<true, false, NS, TM, TN>), grid, block, 0, c.stream, p); \
+    else if (!at && bt) hipLaunchKernelGGL((gemm_f32_kernel<false, true, NS, TM, TN>), grid, block, 0, c.stream, p); \
+    else hipLaunchKernelGGL((gemm_f32_kernel<true, true, NS, TM, TN>), grid, block, 0, c.stream, p);          \
   } while (0)
-#define IPEPS_F32_LAUNCH(NS)                                  \
-  do {                                                        \
-    if (f32_tile_n(g.N) == 64) IPEPS_F32_LAUNCH_T(NS, 64);    \
-    else IPEPS_F32_LAUNCH_T(NS, 128);                         \
+#define IPEPS_F32_LAUNCH(NS)                                          \
+  do {                                                                \
+    const int tn = f32_tile_n(g.N);                                   \
+    if (tn == 64) IPEPS_F32_LAUNCH_T(NS, 128, 64);                    \
+    else if (tn == 192) IPEPS_F32_LAUNCH_T(NS, 64, 192);              \
+    else IPEPS_F32_LAUNCH_T(NS, 128, 128);                            \
   } while (0)
     switch (g.mixed) {
       case Mixed::BF16: IPEPS_F32_LAUNCH(1); break;
@@ -509,7 +514,7 @@ void gemm(Ctx& c, const GemmArgs& g) {
   p.C = g.C; p.ldc = g.ldc; p.sC = g.sC;
   p.conjA = g.conjA; p.conjB = g.conjB;
   p.kchunk = 0;
-  const int64_t tbm = g.dt == DType::F32 ? 128 : BM, tbn = g.dt == DType::F32 ? f32_tile_n(g.N) : BN;
+  const int64_t tbm = g.dt == DType::F32 ? f32_tile_m(g.N) : BM, tbn = g.dt == DType::F32 ? f32_tile_n(g.N) : BN;
   set_vec_flags(g, p);
   dim3 grid((unsigned)((g.M + tbm - 1) / tbm), (unsigned)((g.N + tbn - 1) / tbn), (unsigned)g.batch);
   IPEPS_CHECK(grid.y <= 65535, "gemm: N too large for the grid");
@@ -530,8 +535,12 @@ void gemm(Ctx& c, const GemmArgs& g) {
   // reduction kernel applies alpha / beta.
   const int64_t tiles = (int64_t)grid.x * grid.y * g.batch;
   int64_t S = 1, kc = 0;
+  // FP32: about 1024 blocks with K chunks of at least 8 K tiles; FP64 / C128
+  // (short K tiles, latency-bound per step): about 2048 blocks, chunks >= 64.
+  const bool f32 = g.dt == DType::F32;
+  const int64_t kmin = f32 ? 256 : 64, target = f32 ? 1024 : 2048;
   if (g.batch == 1 && tiles < 600 && g.K >= 1024) {
-    S = std::min<int64_t>((1024 + tiles - 1) / tiles, g.K / 256);
+    S = std::min<int64_t>((target + tiles - 1) / tiles, g.K / kmin);
     if (S >= 2) {
       kc = ((g.K + S - 1) / S + 63) / 64 * 64;
       S = (g.K + kc - 1) / kc;

@@ -788,12 +788,12 @@ int svd_tall(Ctx& c, const Tensor& B, int method, double guard, Tensor& W, Tenso
 // ---------------------------------------------------------------------------
 // truncated SVD of M = R1 R2 with the reference cutoff rule
 // ---------------------------------------------------------------------------
-static int64_t cutoff_rule(const std::vector<double>& s, int64_t chi) {
-  // src/utility/wrapper.jl:37-55
+static int64_t cutoff_rule(const std::vector<double>& s, int64_t chi, double rel) {
+  // src/utility/wrapper.jl:37-55 (rel = 1e-12 there)
   double smax = 0.0;
   for (double v : s) smax = std::max(smax, v);
   int64_t above = 0;
-  for (double v : s) above += (v / smax > 1e-12);
+  for (double v : s) above += (v / smax > rel);
   int64_t n = std::min<int64_t>(chi, above);
   if ((int64_t)s.size() > n && n >= 1 && std::fabs(s[n] - s[n - 1]) < 1e-10) n += 1;
   return n;
@@ -864,7 +864,10 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
   out.s_host.resize(ns);
   HIP_CHECK(hipMemcpyAsync(out.s_host.data(), Sf.data(), ns * 8, hipMemcpyDeviceToHost, c.stream));
   c.sync();
-  const int64_t n = cutoff_rule(out.s_host, chi_eff);
+  // FP32 engines keep only triplets above their noise floor (s / s_max >
+  // mixed_cutoff): P = R2 V S^-1/2 amplifies the rounding error of smaller ones.
+  const double rel = dt == DType::F32 ? std::max(1e-12, o.mixed_cutoff) : 1e-12;
+  const int64_t n = cutoff_rule(out.s_host, chi_eff, rel);
   IPEPS_CHECK(n >= 1 && n <= ns, "svd_trunc: empty truncation");
   out.n = n;
   Tensor Un = first_cols(c, Uf, n), Vn = first_cols(c, Vf, n), Sn = first_n(c, Sf, n);

@@ -110,6 +110,9 @@ struct SvdOpts {
   int full_below = 0;    // use the full SVD when N <= full_below (randomized mode)
   uint64_t seed = 1234;
   double eta = 1e-12;    // Lorentzian broadening of the adjoint (src/utility/autodiff.jl:41)
+  // Relative singular-value cutoff of FP32 (mixed) engines; F64 / C128 use the
+  // reference's 1e-12 (src/utility/wrapper.jl:38-39).
+  double mixed_cutoff = 1e-6;
   // Randomized mode only:
   //   fast_orth = 1: range bases by CholeskyQR2 (two Gram + Cholesky + triangular
   //                  solve passes), falling back to Householder QR when a

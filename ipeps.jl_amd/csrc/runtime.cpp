@@ -99,6 +99,8 @@ Ctx* ctx_create(int device) {
     throw Error(IPEPS_ERR_SOLVER, "rocblas_create_handle failed");
   }
   rocblas_set_stream(h, c->stream);
+  // deterministic results: no atomics-based reductions inside rocBLAS/rocSOLVER
+  rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed);
   c->solver = h;
   return c;
 }

@@ -110,7 +110,8 @@ def _hpair(H):
     return hs, m
 
 
-ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "fast_orth", "fast_svd",
+ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "mixed_cutoff", "fast_orth",
+               "fast_svd",
                "gram_guard", "canonical_edge", "mixed", "energy_fp64", "energy_mixed",
                "checkpoint", "draws")
 

@@ -390,11 +390,13 @@ def test_mixed_precision_energy(pkg, mixed):
     assert abs(E64 - Em) < (1e-4 if mixed == 3 else 1e-6)
 
 
-@pytest.mark.parametrize("shape", [(70, 45, 37), (300, 200, 150), (36, 72, 5000), (257, 130, 2051)])
+@pytest.mark.parametrize("shape", [(70, 45, 37), (300, 200, 150), (36, 72, 5000), (257, 130, 2051),
+                                   (1030, 160, 3001), (100, 192, 40)])
 @pytest.mark.parametrize("kind", ["f64", "c128", "mixed3", "mixed6", "mixed1"])
 def test_gemm_all_ops(pkg, kind, shape):
-    """Every transpose / conjugate combination against numpy.  Shapes reach both
-    FP32 block tiles (N <= 192 and N > 192), aligned (multiple-of-4) and
+    """Every transpose / conjugate combination against numpy.  Shapes reach the
+    FP32 block tiles (N <= 64, 65..128 and N > 192: 128-row tiles; 129..192:
+    64 x 192 tiles), aligned (multiple-of-4) and
     unaligned leading dimensions, ragged edges and split-K (long K, few tiles)."""
     rng = np.random.default_rng(11)
     M, N, K = shape
