@@ -173,6 +173,23 @@ int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double guard, int64_t 
 int ipeps_svd_back(ipeps_ctx* ctx, int dtype, int64_t N, int64_t n, const void* U, const double* S, const void* V,
                    const void* dU, const double* dS, const void* dV, double eta, void* dA_out);
 
+/* correlation_spin (src/graph/expectation.jl:246-271): direction 0 = "h"
+ * (get_envs_TM_h: L0 = C1 E4 C4, R0 = C2 E2 C3, columns E1 T E3), 1 = "v"
+ * (get_envs_TM_v: U0 = C1 E1 C2, D0 = C4 E3 C3, rows E4 T E2).  op1 / op2:
+ * d x d ComplexF64 (column-major), inserted as A[..,p1] op[p1,p2] Ad[..,p2].
+ * s2s_out receives max_iter complex values (re, im), m1 / m2 = con_LR(L, R0)
+ * and con_LR(L0, R).  Complex FP64 on the device for every env dtype. */
+int ipeps_correlation_spin(ipeps_env* env, const double* op1, const double* op2, int64_t d, int max_iter,
+                           int direction, double* s2s_out, double* m1_out, double* m2_out);
+/* Transfer matrix of correlation_TM (src/graph/expectation.jl:193-220):
+ * TM = tcon([E4, E2], [[-1,-3,1,2], [-2,-4,1,2]]) for direction 0 ("h") or
+ * [E1, E3] for 1 ("v"), reshaped column-major to rows x cols.  tm_apply
+ * writes TM X for X (cols x nvec) in the env's host dtype (the operator the
+ * reference hands to eigsolve); tm_dense writes TM itself (rows x cols). */
+int ipeps_tm_dims(ipeps_env* env, int direction, int64_t* rows, int64_t* cols);
+int ipeps_tm_apply(ipeps_env* env, int direction, int64_t nvec, const void* X, void* Y);
+int ipeps_tm_dense(ipeps_env* env, int direction, void* TM_out);
+
 /* Batched independent environments (many instances of CTMTensors +
  * rg_step, src/ctm/ctm_tensor.jl:63-69 and src/ctm/ctmrg.jl:57-64; the
  * reference runs such instances one after another, e.g. the per-basis-vector

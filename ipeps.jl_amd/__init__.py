@@ -10,10 +10,12 @@ from .ctm import (ENV_OPTIONS, Context, CTMTensors, compute_gs_energy, default_c
 from . import models
 from . import excited
 from . import batch
+from . import expect
+from .expect import correlation_spin, correlation_TM
 from .batch import BatchedCTM, c4_instances
 from .excited import compute_es, get_es_grad, make_es_path, optim_es
 
 __all__ = ["C128", "F64", "MIXED", "LIB_PATH", "IpepsError", "lib", "ENV_OPTIONS", "Context", "CTMTensors",
            "compute_gs_energy", "default_context", "get_gs_energy", "get_gs_norm", "gs_energy_grad", "rg_step",
            "run_ctm", "step_adjoint", "svd1", "svd_back", "svd_thin", "tcon", "gemm", "models", "BatchedCTM",
-           "c4_instances"]
+           "c4_instances", "correlation_spin", "correlation_TM"]

@@ -46,6 +46,10 @@ SIGNATURES = {
     "ipeps_svd": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, _P, _P, _DP, _P]),
     "ipeps_svd_thin": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_int64, C.c_int64, _P, _P, _DP, _P, _IP]),
     "ipeps_svd_back": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, _P, _DP, _P, _P, _DP, _P, C.c_double, _P]),
+    "ipeps_correlation_spin": (C.c_int, [_P, _DP, _DP, C.c_int64, C.c_int, C.c_int, _DP, _DP, _DP]),
+    "ipeps_tm_dims": (C.c_int, [_P, C.c_int, _I64P, _I64P]),
+    "ipeps_tm_apply": (C.c_int, [_P, C.c_int, C.c_int64, _P, _P]),
+    "ipeps_tm_dense": (C.c_int, [_P, C.c_int, _P]),
     "ipeps_batch_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _I64P, C.POINTER(_P), _I64P, C.POINTER(_DP),
                                      C.POINTER(_DP), _I64P, C.c_int, C.c_char_p, C.POINTER(_P)]),
     "ipeps_batch_destroy": (C.c_int, [_P]),

@@ -9,6 +9,7 @@
 #include <thread>
 
 #include "../../include/ipeps_hip.h"
+#include "corr.h"
 #include "ctmrg.h"
 
 using namespace ipeps;
@@ -501,6 +502,64 @@ int ipeps_svd_back(ipeps_ctx* ctx, int dtype, int64_t N, int64_t n, const void* 
                  VdV.defined() ? VdV.data() : nullptr, tdS.defined() ? tdS.ptr<double>() : nullptr, eta, G.data());
     Tensor R = op::matmul_raw(c, op::matmul_raw(c, tU, false, false, G, false, false), false, false, tV, true, true);
     download(c, R, dt, dA_out);
+  });
+}
+
+// ---- transfer matrix and correlations -----------------------------------------
+
+// E tensors of a mixed environment widened to FP64 for the transfer matrix
+static ctm::Env tm_env(Ctx& c, const ctm::Env& e) {
+  if (e.dt != DType::F32) return e;
+  ctm::Env r = e;
+  r.dt = DType::F64;
+  for (int i = 0; i < 4; ++i) r.E[i] = op::to_dtype(c, e.E[i], DType::F64);
+  return r;
+}
+
+int ipeps_correlation_spin(ipeps_env* env, const double* op1, const double* op2, int64_t d, int max_iter,
+                           int direction, double* s2s_out, double* m1_out, double* m2_out) {
+  return guard([&] {
+    IPEPS_CHECK(env && op1 && op2 && s2s_out && max_iter >= 1, "bad arguments");
+    Ctx& c = *env->hold;
+    TapeScope ts(nullptr);
+    std::vector<double> s2s;
+    double m1[2], m2[2];
+    ctm::correlation_spin(c, env->env, upload_H(c, op1, d), upload_H(c, op2, d), max_iter, direction, s2s, m1, m2);
+    std::memcpy(s2s_out, s2s.data(), s2s.size() * sizeof(double));
+    if (m1_out) { m1_out[0] = m1[0]; m1_out[1] = m1[1]; }
+    if (m2_out) { m2_out[0] = m2[0]; m2_out[1] = m2[1]; }
+  });
+}
+
+int ipeps_tm_dims(ipeps_env* env, int direction, int64_t* rows, int64_t* cols) {
+  return guard([&] {
+    IPEPS_CHECK(env && rows && cols, "bad arguments");
+    ctm::tm_dims(env->env, direction, *rows, *cols);
+  });
+}
+
+int ipeps_tm_dense(ipeps_env* env, int direction, void* TM_out) {
+  return guard([&] {
+    IPEPS_CHECK(env && TM_out, "bad arguments");
+    Ctx& c = *env->hold;
+    TapeScope ts(nullptr);
+    Tensor TM = ctm::tm_dense(c, tm_env(c, env->env), direction);
+    download(c, TM, host_dt(env->dtype), TM_out);
+  });
+}
+
+int ipeps_tm_apply(ipeps_env* env, int direction, int64_t nvec, const void* X, void* Y) {
+  return guard([&] {
+    IPEPS_CHECK(env && X && Y && nvec >= 1, "bad arguments");
+    Ctx& c = *env->hold;
+    TapeScope ts(nullptr);
+    int64_t rows, cols;
+    ctm::tm_dims(env->env, direction, rows, cols);
+    ctm::Env e = tm_env(c, env->env);
+    const DType hdt = host_dt(env->dtype);
+    Tensor tX = upload(c, hdt, e.E[0].dt, {cols, nvec}, X);
+    Tensor tY = ctm::tm_apply(c, e, direction, tX);
+    download(c, tY, hdt, Y);
   });
 }
 

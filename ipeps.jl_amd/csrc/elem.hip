@@ -142,6 +142,13 @@ __global__ void convert_kernel(int64_t n, const A* x, B* y) {
   for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) y[i] = (B)x[i];
 }
 
+// real -> complex (zero imaginary part)
+template <typename A>
+__global__ void to_complex_kernel(int64_t n, const A* x, double2* y) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    y[i] = make_double2((double)x[i], 0.0);
+}
+
 template <typename T>
 __global__ void scale_cols_kernel(int64_t rows, int64_t cols, const T* X, const double* s, T* Y, int rowwise) {
   int64_t n = rows * cols;
@@ -749,6 +756,10 @@ void convert(Ctx& c, DType from, DType to, int64_t n, const void* x, void* y) {
     hipLaunchKernelGGL((convert_kernel<double, float>), dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const double*)x, (float*)y);
   else if (from == DType::F32 && to == DType::F64)
     hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const float*)x, (double*)y);
+  else if (from == DType::F64 && to == DType::C128)
+    hipLaunchKernelGGL((to_complex_kernel<double>), dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const double*)x, (double2*)y);
+  else if (from == DType::F32 && to == DType::C128)
+    hipLaunchKernelGGL((to_complex_kernel<float>), dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const float*)x, (double2*)y);
   else
     throw Error(IPEPS_ERR_ARG, "convert: unsupported dtype pair");
   HIP_CHECK(hipGetLastError());
