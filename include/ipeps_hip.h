@@ -73,7 +73,8 @@ int ipeps_env_destroy(ipeps_env* env);
  * "energy_mixed" (1, 3 or 6: bf16 products of those contractions when not
  * FP64, default 6; the density-matrix traces and ratios are FP64 always),
  * "checkpoint" (0/1), and for the randomized mode
- * "fast_orth" (0/1: CholeskyQR2 range bases with Householder fallback),
+ * "fast_orth" (0: Householder QR range bases; 1: CholeskyQR2 with Householder
+ * fallback; 2: CholeskyQR2 on the one-workgroup FP64 Cholesky + inverse),
  * "fast_svd" (0/1: small SVD from Gram eigenpairs with one-sided Jacobi
  * fallback) and "gram_guard" (s_min/s_max below which the Gram route is not
  * taken, default 1e-4).  "draws" sets the position in the environment's

@@ -5,6 +5,7 @@
 
 #include "kernels.h"
 #include "permute_impl.h"
+#include "smallchol_impl.h"
 
 namespace ipeps {
 namespace k {
@@ -927,6 +928,63 @@ void ns_coeff(Ctx& c, DType dt, int64_t n, void* G) {
 #define NK(T, ...) hipLaunchKernelGGL(ns_coeff_kernel<T>, dim3(blocks_for(n * n)), dim3(TPB), 0, c.stream, n, (T*)G)
   DISPATCH3(dt, NK, 0);
 #undef NK
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace k
+}  // namespace ipeps
+
+// ---------------------------------------------------------------------------
+// Small SPD Cholesky + triangular inverse in one workgroup (smallchol_impl.h)
+// ---------------------------------------------------------------------------
+namespace ipeps {
+namespace k {
+
+static_assert(SCHOL_MAXN == SCHOL_MAXN_HOST, "small_chol_inv size limit");
+
+__global__ __launch_bounds__(SCHOL_THREADS) void small_chol_inv_kernel(int n, const double* __restrict__ G, int64_t ldg,
+                                                                       double* __restrict__ Linv, int64_t ldl,
+                                                                       int* __restrict__ info) {
+  __shared__ double P[SCHOL_MAXN * (SCHOL_MAXN + 1) / 2 + SCHOL_MAXN];
+  __shared__ int fail;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double* x = P + n * (n + 1) / 2;
+  if (tid == 0) fail = 0;
+  schol_load(n, G, ldg, P, tid, nt);
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    schol_pivot(n, j, P, &fail, tid);
+    __syncthreads();
+    if (fail) break;
+    schol_scale(n, j, P, tid, nt);
+    __syncthreads();
+    schol_update(n, j, P, tid, nt);
+    __syncthreads();
+  }
+  if (fail) {
+    if (tid == 0) *info = fail;
+    return;
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    sinv_prep(n, j, P, x, tid, nt);
+    __syncthreads();
+    for (int base = 0; base < n - j - 1; base += nt / 4) {
+      const int i = j + 1 + base + tid / 4, r = tid % 4;
+      double s = i < n ? sinv_partial(n, j, P, x, i, r) : 0.0;
+      s += __shfl_xor(s, 1, 4);
+      s += __shfl_xor(s, 2, 4);
+      if (i < n && r == 0) sinv_store(n, j, P, i, s);
+    }
+    __syncthreads();
+  }
+  schol_store(n, P, Linv, ldl, tid, nt);
+  if (tid == 0) *info = 0;
+}
+
+void small_chol_inv(Ctx& c, int n, const double* G, int64_t ldg, double* Linv, int64_t ldl, int* info) {
+  IPEPS_CHECK(n >= 1 && n <= SCHOL_MAXN, "small_chol_inv: n out of range");
+  TimedLaunch tl(c, "small_chol_inv_kernel", (double)n * n * n, 0.0);
+  hipLaunchKernelGGL(small_chol_inv_kernel, dim3(1), dim3(SCHOL_THREADS), 0, c.stream, n, G, ldg, Linv, ldl, info);
   HIP_CHECK(hipGetLastError());
 }
 

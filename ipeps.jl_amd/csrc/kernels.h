@@ -122,6 +122,10 @@ void eye_dev(Ctx& c, DType dt, int64_t n, const void* G, double* out);
 void eig_to_svd(Ctx& c, DType dt, int64_t n, const double* lam, const void* W, double* s, void* Wr, double* sinv);
 // G <- (3 I - G) / 2  (one Newton-Schulz step X <- X (3I - X^H X)/2)
 void ns_coeff(Ctx& c, DType dt, int64_t n, void* G);
+constexpr int SCHOL_MAXN_HOST = 192;  // largest n of small_chol_inv
+// L^-1 of the Cholesky factor G = L L^T of an SPD n x n FP64 matrix (n <= 192)
+// in one workgroup; info (device int) = 0, or j + 1 if pivot j is not positive.
+void small_chol_inv(Ctx& c, int n, const double* G, int64_t ldg, double* Linv, int64_t ldl, int* info);
 
 }  // namespace k
 }  // namespace ipeps

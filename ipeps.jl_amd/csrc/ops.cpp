@@ -702,8 +702,34 @@ static void trsm_right_upper(Ctx& c, const Tensor& R, Tensor& X) {
 // second pass is trusted when its Gram is within 1/2 of I (then cond(Q1)^2 <= 3
 // and Q is orthonormal to working precision).  Returns false when a Cholesky
 // failed or that check did not hold; Q is then undefined.
-static bool orth_cholqr2(Ctx& c, const Tensor& Y, Tensor& Q) {
+// CholeskyQR2 for FP64 with n <= 192 on the one-workgroup Cholesky +
+// triangular inverse: Q1 = Y L1^-T, Q = Q1 L2^-T (two GEMMs instead of the
+// triangular solves).  Same acceptance test as orth_cholqr2.
+static bool orth_cholqr2_small(Ctx& c, const Tensor& Y, Tensor& Q) {
   const int64_t n = Y.dims[1];
+  Tensor st = c.zeros(DType::F64, {3});  // [info1, info2, |G2 - I|_max]
+  int* info1 = (int*)st.ptr<double>();
+  int* info2 = (int*)(st.ptr<double>() + 1);
+  Tensor G = matmul_raw(c, Y, true, false, Y, false, false);
+  Tensor Li = c.empty(DType::F64, {n, n});
+  k::small_chol_inv(c, (int)n, G.ptr<double>(), n, Li.ptr<double>(), n, info1);
+  Tensor Q1 = matmul_raw(c, Y, false, false, Li, true, false);
+  Tensor G2 = matmul_raw(c, Q1, true, false, Q1, false, false);
+  k::eye_dev(c, DType::F64, n, G2.data(), st.ptr<double>() + 2);
+  k::small_chol_inv(c, (int)n, G2.ptr<double>(), n, Li.ptr<double>(), n, info2);
+  Q = matmul_raw(c, Q1, false, false, Li, true, false);
+  double hs[3];
+  HIP_CHECK(hipMemcpyAsync(hs, st.data(), sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  int i1, i2;
+  std::memcpy(&i1, &hs[0], sizeof(int));
+  std::memcpy(&i2, &hs[1], sizeof(int));
+  return i1 == 0 && i2 == 0 && hs[2] <= 0.5;
+}
+
+static bool orth_cholqr2(Ctx& c, const Tensor& Y, Tensor& Q, bool small_chol) {
+  const int64_t n = Y.dims[1];
+  if (small_chol && Y.dt == DType::F64 && n <= k::SCHOL_MAXN_HOST) return orth_cholqr2_small(c, Y, Q);
   Tensor st = c.zeros(DType::F64, {3});  // [info1, info2, |G2 - I|_max]
   int* info1 = (int*)st.ptr<double>();
   int* info2 = (int*)(st.ptr<double>() + 1);
@@ -724,10 +750,10 @@ static bool orth_cholqr2(Ctx& c, const Tensor& Y, Tensor& Q) {
   return i1 == 0 && i2 == 0 && hs[2] <= 0.5;
 }
 
-Tensor orth(Ctx& c, const Tensor& Y, bool fast) {
+Tensor orth(Ctx& c, const Tensor& Y, int fast) {
   IPEPS_CHECK(Y.nd() == 2 && Y.dims[0] >= Y.dims[1], "orth: tall matrix expected");
   Tensor Q;
-  if (fast && orth_cholqr2(c, Y, Q)) return Q;
+  if (fast && orth_cholqr2(c, Y, Q, fast == 2)) return Q;
   return orth_householder(c, Y);
 }
 
@@ -846,7 +872,7 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     auto apply_MH = [&](const Tensor& X) {  // M^H X = R2^H (R1^H X)
       return matmul_raw(c, R2, true, true, matmul_raw(c, R1, true, true, X, false, false), false, false);
     };
-    const bool fo = o.fast_orth != 0;
+    const int fo = o.fast_orth;
     Tensor Q = orth(c, to_dtype(c, apply_M(Om), sdt), fo);
     for (int q = 0; q < o.power_iters; ++q) {
       Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), fo);

@@ -117,6 +117,7 @@ struct SvdOpts {
   //   fast_orth = 1: range bases by CholeskyQR2 (two Gram + Cholesky + triangular
   //                  solve passes), falling back to Householder QR when a
   //                  Cholesky fails or the second Gram is farther than 1/2 from I;
+  //                  2: the same with the one-workgroup FP64 Cholesky + inverse;
   //   fast_svd  = 1: the small SVD of B = M^H Q from the eigenpairs of B^H B
   //                  plus one Newton-Schulz orthonormalisation of B's singular
   //                  vectors, falling back to one-sided Jacobi (gesvdj) when
@@ -137,8 +138,11 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
 // Seed of the i-th draw of a stream keyed by `base` (splitmix64 finaliser).
 uint64_t draw_seed(uint64_t base, uint64_t i);
 
-// Orthonormal basis of the columns of Y (N x k, F64 or C128): Householder QR.
-Tensor orth(Ctx& c, const Tensor& Y, bool fast = false);
+// Orthonormal basis of the columns of Y (N x k, F64 or C128): Householder QR
+// (fast = 0), CholeskyQR2 with rocSOLVER Cholesky factors and triangular
+// solves (1) or with the one-workgroup Cholesky + inverse for FP64, k <= 192
+// (2); 1 and 2 fall back to Householder as described at orth_cholqr2.
+Tensor orth(Ctx& c, const Tensor& Y, int fast = 0);
 // Thin SVD of a tall F64/C128 matrix B = W diag(s) X^H (s descending).
 // method 0: one-sided Jacobi (rocSOLVER gesvdj); 1: Gram eigenpairs when
 // s_min/s_max >= guard, else method 0.  Returns the method used.

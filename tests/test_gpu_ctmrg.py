@@ -441,3 +441,24 @@ def test_mixed_energy_contractions_fixed_env(pkg, emode):
     Em, _ = pkg.get_gs_energy(tm, H)
     print(emode, "E64", E64.real, "dE", abs(Em.real - E64.real))
     assert abs(Em.real - E64.real) < (1e-5 if emode == "bf16x3" else 1e-7)
+
+
+def test_small_cholesky_route_matches_rocsolver(pkg):
+    """fast_orth=2 (CholeskyQR2 on the one-workgroup FP64 Cholesky + triangular
+    inverse) against fast_orth=1 (rocSOLVER potrf + trsm) on the same test
+    matrices: energy 1e-10, gradient 1e-6 relative."""
+    A = O.seeded_tensor((3, 3, 3, 3, 2), 4).numpy()
+    H = [h.numpy() for h in O.heisenberg(1.0)]
+    cfg = dict(chi=24, min_iter=6, max_iter=6, ad_max_iter=2, rg_tol=0.0)
+    runs = []
+    for fo in (1, 2):
+        ctx = pkg.Context()
+        ctx.set_timing(1)
+        E, g, _ = pkg.gs_energy_grad(A, H, cfg, dtype="f64", options=dict(svd_mode=1, fast_orth=fo), ctx=ctx)
+        runs.append((E, g, ctx.kernel_stats(reset=True)))
+        ctx.close()
+    (E1, g1, st1), (E2, g2, st2) = runs
+    assert "small_chol_inv_kernel" in st2 and "small_chol_inv_kernel" not in st1
+    print("dE", abs(E1 - E2), "grad rel", np.abs(g1 - g2).max() / np.abs(g1).max())
+    assert abs(E1 - E2) < 1e-10
+    assert np.abs(g1 - g2).max() < 1e-6 * np.abs(g1).max()
