@@ -72,7 +72,10 @@ int ipeps_env_destroy(ipeps_env* env);
  * "energy_fp64" (0/1: energy contractions of a MIXED environment in FP64),
  * "energy_mixed" (1, 3 or 6: bf16 products of those contractions when not
  * FP64, default 6; the density-matrix traces and ratios are FP64 always),
- * "checkpoint" (0/1), and for the randomized mode
+ * "checkpoint" (0/1), "fused_absorb" (0/1, default 1: MIXED environments
+ * with 4 <= D <= 6 and d <= 2 absorb A and Ad into the enlarged corners in
+ * one FP32-MFMA kernel instead of two split-bf16 GEMMs), and for the
+ * randomized mode
  * "fast_orth" (0: Householder QR range bases; 1: CholeskyQR2 with Householder
  * fallback; 2: CholeskyQR2 on the one-workgroup FP64 Cholesky + inverse),
  * "fast_svd" (0/1: small SVD from Gram eigenpairs with one-sided Jacobi
@@ -145,6 +148,19 @@ int ipeps_es_grad(ipeps_env* env, const double* H1, const double* H2, int64_t hd
 int ipeps_tcon(ipeps_ctx* ctx, int dtype, int ndA, const int64_t* dimsA, const void* A, const int* la, int ndB,
                const int64_t* dimsB, const void* B, const int* lb, int64_t* dimsC, int* ndC, void* C_host,
                int64_t C_cap);
+
+/* The site absorption of an enlarged corner (two tcon calls of
+ * get_projector_left etc., src/ctm/ctmrg.jl:139-140) as one contraction:
+ * Y = X A Ad with ncon labels lx (6), la (5), lad (5): X has two open legs,
+ * two legs shared with A and two with Ad; A and Ad share their physical leg.
+ * dtype IPEPS_MIXED (float host data; fused != 0 runs the fused FP32-MFMA
+ * kernel when 4 <= D <= 6 and d <= 2, else two split-bf16 contractions) or
+ * IPEPS_F64 (two contractions).  Y_out receives Y (output legs -1..-6).  With
+ * a cotangent G (Y's shape) the VJP is written to gX_out / gA_out / gAd_out
+ * (each may be NULL). */
+int ipeps_absorb(ipeps_ctx* ctx, int dtype, int fused, const int64_t* dimsX, const void* X, const int* lx,
+                 const int64_t* dimsA, const void* A, const int* la, const void* Ad, const int* lad, const void* G,
+                 void* Y_out, void* gX_out, void* gA_out, void* gAd_out);
 
 /* Matrix product on the device, the GEMM that OMEinsum hands to BLAS for a
  * two-index contraction (wrapper.jl:23): C = alpha op(A) op(B) + beta C,

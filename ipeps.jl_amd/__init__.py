@@ -5,7 +5,7 @@ The numerics run in libipeps_hip.so (hand-written HIP kernels behind a C
 ABI, include/ipeps_hip.h); this package is the ctypes host side.
 """
 from ._lib import C128, F64, MIXED, LIB_PATH, IpepsError, lib
-from .ctm import (ENV_OPTIONS, Context, CTMTensors, compute_gs_energy, default_context, get_gs_energy,
+from .ctm import (ENV_OPTIONS, absorb, Context, CTMTensors, compute_gs_energy, default_context, get_gs_energy,
                   get_gs_norm, gemm, gs_energy_grad, rg_step, run_ctm, step_adjoint, svd1, svd_back, svd_thin, tcon)
 from . import models
 from . import excited

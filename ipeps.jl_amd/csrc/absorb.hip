@@ -1,0 +1,340 @@
+// Fused absorption of a site into an enlarged corner (FP32 data, FP32 MFMA):
+//
+//   Y[r, a, b] = sum_{u, v, p} X[r, u, v] A[u, a, p] Ad[v, b, p]
+//
+// r runs over the two row legs of X (the environment legs), u / v over the
+// two ket / bra legs that meet A / Ad (u, v, a, b < U = D^2), p over the
+// physical leg.  Per row this is Y_r = sum_p A_p^T X_r Ad_p: T_p = X_r Ad_p
+// is accumulated by v_mfma_f32_16x16x4f32 and fed, from the accumulator
+// registers, as the A operand of the second product Y_r^T += T_p^T A_p.  The
+// accumulator of a 16 x 16 tile holds D[4 (l/16) + i][l % 16] in lane l,
+// element i; taking the k index of the second product's sub-step s as
+// u = 16 tu + 4 (l/16) + s makes element s of lane l exactly the operand lane
+// l must supply, so the intermediate never leaves registers.
+//
+// The same kernel gives the cotangent of X (A and Ad enter transposed), and a
+// second kernel the cotangents of A and Ad:
+//   Abar[u, a, p]  = sum_r (X_r Ad_p)     Ybar_r^T   [u, a]
+//   Adbar[v, b, p] = sum_r (X_r^T A_p)    Ybar_r     [v, b]
+// accumulated per wave over the rows a persistent block visits, then summed
+// over waves (LDS) and blocks (a reduction kernel): no atomics.
+//
+// X and Y are addressed through strides: X[r1, r2, u1, u2, v1, v2] with
+// u = u1 + D u2 and v = v1 + D v2, each leg with its own stride; rows of a
+// block are consecutive r1 (the stride-1 leg when there is one) for one r2.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace ipeps {
+namespace k {
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int ABS_UMAX = 36;  // U = D^2 <= 36 (D <= 6)
+constexpr int ABS_PMAX = 2;   // physical extent
+constexpr int ABS_LD = ABS_UMAX + 1;
+constexpr int ABS_RB = 16;    // rows per block (forward)
+constexpr int ABS_RBW = 8;    // rows per block (weight gradient)
+constexpr int ABS_NW = 8;     // waves per block (two per SIMD)
+constexpr int ABS_THREADS = 64 * ABS_NW;
+
+__device__ inline int64_t off2(int i, int D, int64_t s1, int64_t s2) { return (int64_t)(i % D) * s1 + (int64_t)(i / D) * s2; }
+
+__device__ inline v4f mfma4(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// stage RB rows (r1 = r10 .. r10+RB-1, fixed r2) of a strided (r, x, y) tensor
+// into LDS as dst[rr][x][y] with row stride U * ABS_LD; rows past n1 are zero
+template <int RB>
+__device__ inline void stage_rows(const float* __restrict__ src, const AbsorbLegs& g, int U, int D, int r10, int r2,
+                                  float* dst) {
+  const int total = U * U * RB;
+  for (int e = threadIdx.x; e < total; e += ABS_THREADS) {
+    const int rr = e % RB, xy = e / RB;
+    const int x = xy % U, y = xy / U;
+    const int r1 = r10 + rr;
+    float v = 0.f;
+    if (r1 < g.n1)
+      v = src[(int64_t)r1 * g.s1 + (int64_t)r2 * g.s2 + off2(x, D, g.sx1, g.sx2) + off2(y, D, g.sy1, g.sy2)];
+    dst[rr * U * ABS_LD + x * ABS_LD + y] = v;
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(ABS_THREADS) void absorb_fwd_kernel(AbsorbP p) {
+  __shared__ float xs[ABS_RB * ABS_UMAX * ABS_LD];
+  __shared__ float am[ABS_UMAX * ABS_UMAX * ABS_PMAX], adm[ABS_UMAX * ABS_UMAX * ABS_PMAX];
+  const int U = p.U, P = p.P, D = p.D;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int nb1 = (p.x.n1 + ABS_RB - 1) / ABS_RB;
+  const int r2 = blockIdx.x / nb1, r10 = (blockIdx.x % nb1) * ABS_RB;
+  for (int e = threadIdx.x; e < U * U * P; e += ABS_THREADS) {
+    am[e] = p.Am[e];
+    adm[e] = p.Adm[e];
+  }
+  stage_rows<ABS_RB>(p.X, p.x, U, D, r10, r2, xs);
+  __syncthreads();
+  for (int rr = wave; rr < ABS_RB; rr += ABS_NW) {
+    const int r1 = r10 + rr;
+    if (r1 >= p.x.n1) break;
+    const float* xr = xs + rr * U * ABS_LD;
+    v4f yacc[NT][NT];  // [tb][ta]: Y^T[b][a]
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) yacc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+    for (int pp = 0; pp < P; ++pp) {
+      v4f tacc[NT][NT];  // [tu][tb]: T[u][b] = (X_r Ad_p)[u][b]
+#pragma unroll
+      for (int i = 0; i < NT; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) tacc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < (U + 3) / 4; ++ks) {
+        const int v = 4 * ks + lg;
+        float ao[NT], bo[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const int u = 16 * t + lr, b = 16 * t + lr;
+          ao[t] = (u < U && v < U) ? xr[u * ABS_LD + v] : 0.f;
+          bo[t] = (v < U && b < U) ? adm[v + U * (b + U * pp)] : 0.f;
+        }
+#pragma unroll
+        for (int tu = 0; tu < NT; ++tu)
+#pragma unroll
+          for (int tb = 0; tb < NT; ++tb) tacc[tu][tb] = mfma4(ao[tu], bo[tb], tacc[tu][tb]);
+      }
+#pragma unroll
+      for (int tu = 0; tu < NT; ++tu)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int u = 16 * tu + 4 * lg + s;
+          float bo[NT];
+#pragma unroll
+          for (int ta = 0; ta < NT; ++ta) {
+            const int a = 16 * ta + lr;
+            bo[ta] = (u < U && a < U) ? am[u + U * (a + U * pp)] : 0.f;
+          }
+#pragma unroll
+          for (int tb = 0; tb < NT; ++tb)
+#pragma unroll
+            for (int ta = 0; ta < NT; ++ta) yacc[tb][ta] = mfma4(tacc[tu][tb][s], bo[ta], yacc[tb][ta]);
+        }
+    }
+    const int64_t rowoff = (int64_t)r1 * p.y.s1 + (int64_t)r2 * p.y.s2;
+#pragma unroll
+    for (int tb = 0; tb < NT; ++tb)
+#pragma unroll
+      for (int ta = 0; ta < NT; ++ta)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int b = 16 * tb + 4 * lg + i, a = 16 * ta + lr;
+          if (a < U && b < U) p.Y[rowoff + off2(a, D, p.y.sx1, p.y.sx2) + off2(b, D, p.y.sy1, p.y.sy2)] = yacc[tb][ta][i];
+        }
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
+  __shared__ float xs[ABS_RBW * ABS_UMAX * ABS_LD];
+  __shared__ float gs[ABS_RBW * ABS_UMAX * ABS_LD];
+  __shared__ float am[ABS_UMAX * ABS_UMAX * ABS_PMAX], adm[ABS_UMAX * ABS_UMAX * ABS_PMAX];
+  const int U = p.U, P = p.P, D = p.D;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  for (int e = threadIdx.x; e < U * U * P; e += ABS_THREADS) {
+    am[e] = p.Am[e];
+    adm[e] = p.Adm[e];
+  }
+  v4f abar[ABS_PMAX][NT][NT];   // [p][ta][tu]: Abar^T[a][u]
+  v4f adbar[ABS_PMAX][NT][NT];  // [p][tb][tv]: Adbar^T[b][v]
+#pragma unroll
+  for (int q = 0; q < ABS_PMAX; ++q)
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        abar[q][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+        adbar[q][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+      }
+  const int nb1 = (p.x.n1 + ABS_RBW - 1) / ABS_RBW;
+  const int nblk = nb1 * p.x.n2;
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int r2 = blk / nb1, r10 = (blk % nb1) * ABS_RBW;
+    __syncthreads();  // previous rows fully consumed
+    stage_rows<ABS_RBW>(p.X, p.x, U, D, r10, r2, xs);
+    stage_rows<ABS_RBW>(p.G, p.g, U, D, r10, r2, gs);
+    __syncthreads();
+    for (int rr = wave; rr < ABS_RBW; rr += ABS_NW) {
+      if (r10 + rr >= p.x.n1) break;
+      const float* xr = xs + rr * U * ABS_LD;  // X_r[u][v]
+      const float* gr = gs + rr * U * ABS_LD;  // G_r[a][b]
+#pragma unroll
+      for (int pp = 0; pp < ABS_PMAX; ++pp) {
+        if (pp >= P) break;
+        // Tt[b][u] = sum_v Ad_p[v][b] X_r[u][v]
+        v4f tt[NT][NT];
+#pragma unroll
+        for (int i = 0; i < NT; ++i)
+#pragma unroll
+          for (int j = 0; j < NT; ++j) tt[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < (U + 3) / 4; ++ks) {
+          const int v = 4 * ks + lg;
+          float ao[NT], bo[NT];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const int b = 16 * t + lr, u = 16 * t + lr;
+            ao[t] = (v < U && b < U) ? adm[v + U * (b + U * pp)] : 0.f;
+            bo[t] = (v < U && u < U) ? xr[u * ABS_LD + v] : 0.f;
+          }
+#pragma unroll
+          for (int tb = 0; tb < NT; ++tb)
+#pragma unroll
+            for (int tu = 0; tu < NT; ++tu) tt[tb][tu] = mfma4(ao[tb], bo[tu], tt[tb][tu]);
+        }
+        // Abar^T[a][u] += sum_b G_r[a][b] Tt[b][u]
+#pragma unroll
+        for (int tb = 0; tb < NT; ++tb)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int b = 16 * tb + 4 * lg + s;
+            float ao[NT];
+#pragma unroll
+            for (int ta = 0; ta < NT; ++ta) {
+              const int a = 16 * ta + lr;
+              ao[ta] = (a < U && b < U) ? gr[a * ABS_LD + b] : 0.f;
+            }
+#pragma unroll
+            for (int ta = 0; ta < NT; ++ta)
+#pragma unroll
+              for (int tu = 0; tu < NT; ++tu) abar[pp][ta][tu] = mfma4(ao[ta], tt[tb][tu][s], abar[pp][ta][tu]);
+          }
+        // S[a][v] = sum_u A_p[u][a] X_r[u][v]
+        v4f sa[NT][NT];
+#pragma unroll
+        for (int i = 0; i < NT; ++i)
+#pragma unroll
+          for (int j = 0; j < NT; ++j) sa[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < (U + 3) / 4; ++ks) {
+          const int u = 4 * ks + lg;
+          float ao[NT], bo[NT];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const int a = 16 * t + lr, v = 16 * t + lr;
+            ao[t] = (u < U && a < U) ? am[u + U * (a + U * pp)] : 0.f;
+            bo[t] = (u < U && v < U) ? xr[u * ABS_LD + v] : 0.f;
+          }
+#pragma unroll
+          for (int ta = 0; ta < NT; ++ta)
+#pragma unroll
+            for (int tv = 0; tv < NT; ++tv) sa[ta][tv] = mfma4(ao[ta], bo[tv], sa[ta][tv]);
+        }
+        // Adbar^T[b][v] += sum_a G_r[a][b] S[a][v]
+#pragma unroll
+        for (int ta = 0; ta < NT; ++ta)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int a = 16 * ta + 4 * lg + s;
+            float ao[NT];
+#pragma unroll
+            for (int tb = 0; tb < NT; ++tb) {
+              const int b = 16 * tb + lr;
+              ao[tb] = (a < U && b < U) ? gr[a * ABS_LD + b] : 0.f;
+            }
+#pragma unroll
+            for (int tb = 0; tb < NT; ++tb)
+#pragma unroll
+              for (int tv = 0; tv < NT; ++tv) adbar[pp][tb][tv] = mfma4(ao[tb], sa[ta][tv][s], adbar[pp][tb][tv]);
+          }
+      }
+    }
+  }
+  // sum the waves in LDS, one physical index at a time (xs and gs each hold
+  // 8 * 36 * 37 >= ABS_NW * U * U floats)
+  float* red = xs;
+#pragma unroll
+  for (int pp = 0; pp < ABS_PMAX; ++pp) {
+    if (pp >= P) break;
+    __syncthreads();
+#pragma unroll
+    for (int t1 = 0; t1 < NT; ++t1)
+#pragma unroll
+      for (int t2 = 0; t2 < NT; ++t2)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int m = 16 * t1 + 4 * lg + i, n = 16 * t2 + lr;  // (a, u) resp. (b, v)
+          if (m < U && n < U) {
+            red[wave * U * U + n + U * m] = abar[pp][t1][t2][i];    // Abar[u = n][a = m]
+            gs[wave * U * U + n + U * m] = adbar[pp][t1][t2][i];    // Adbar[v = n][b = m]
+          }
+        }
+    __syncthreads();
+    float* outA = p.partial + (int64_t)blockIdx.x * 2 * U * U * P + (int64_t)pp * U * U;
+    float* outD = outA + (int64_t)U * U * P;
+    for (int e = threadIdx.x; e < U * U; e += ABS_THREADS) {
+      float sa = 0.f, sd = 0.f;
+#pragma unroll
+      for (int w = 0; w < ABS_NW; ++w) {
+        sa += red[w * U * U + e];
+        sd += gs[w * U * U + e];
+      }
+      outA[e] = sa;
+      outD[e] = sd;
+    }
+  }
+}
+
+// out[e] = sum_blk partial[blk][e], e < n (n = 2 U^2 P)
+__global__ __launch_bounds__(256) void absorb_reduce_kernel(int nblk, int n, const float* __restrict__ partial,
+                                                            float* __restrict__ out) {
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * n + e];
+    out[e] = s;
+  }
+}
+
+}  // namespace
+
+bool absorb_supported(int D, int P) { return D >= 4 && D * D <= ABS_UMAX && P >= 1 && P <= ABS_PMAX; }
+
+void absorb_fwd(Ctx& c, const AbsorbP& p) {
+  IPEPS_CHECK(absorb_supported(p.D, p.P) && p.U == p.D * p.D, "absorb: unsupported D / d");
+  const int nb1 = (p.x.n1 + ABS_RB - 1) / ABS_RB;
+  const int64_t nb = (int64_t)nb1 * p.x.n2;
+  IPEPS_CHECK(nb >= 1 && nb < (1ll << 31), "absorb: grid");
+  const double rows = (double)p.x.n1 * p.x.n2;
+  TimedLaunch tl(c, "absorb_fwd_kernel", rows * 4.0 * p.U * p.U * p.U * p.P, rows * 8.0 * p.U * p.U);
+  const dim3 blk(ABS_THREADS);
+  if (p.U <= 16) hipLaunchKernelGGL((absorb_fwd_kernel<1>), dim3((unsigned)nb), blk, 0, c.stream, p);
+  else if (p.U <= 32) hipLaunchKernelGGL((absorb_fwd_kernel<2>), dim3((unsigned)nb), blk, 0, c.stream, p);
+  else hipLaunchKernelGGL((absorb_fwd_kernel<3>), dim3((unsigned)nb), blk, 0, c.stream, p);
+  HIP_CHECK(hipGetLastError());
+}
+
+void absorb_wgrad(Ctx& c, AbsorbWP p, float* Abar_Adbar) {
+  IPEPS_CHECK(absorb_supported(p.D, p.P) && p.U == p.D * p.D, "absorb: unsupported D / d");
+  const int nb1 = (p.x.n1 + ABS_RBW - 1) / ABS_RBW;
+  const int64_t nblk = (int64_t)nb1 * p.x.n2;
+  const int grid = (int)std::min<int64_t>(nblk, 512);
+  const int n = 2 * p.U * p.U * p.P;
+  std::shared_ptr<Buf> ws(new Buf{c.pool->alloc((size_t)grid * n * 4), (size_t)grid * n * 4, c.pool.get()});
+  p.partial = (float*)ws->p;
+  const double rows = (double)p.x.n1 * p.x.n2;
+  {
+    TimedLaunch tl(c, "absorb_wgrad_kernel", rows * 8.0 * p.U * p.U * p.U * p.P, rows * 8.0 * p.U * p.U);
+    const dim3 blk(ABS_THREADS);
+    if (p.U <= 16) hipLaunchKernelGGL((absorb_wgrad_kernel<1>), dim3(grid), blk, 0, c.stream, p);
+    else if (p.U <= 32) hipLaunchKernelGGL((absorb_wgrad_kernel<2>), dim3(grid), blk, 0, c.stream, p);
+    else hipLaunchKernelGGL((absorb_wgrad_kernel<3>), dim3(grid), blk, 0, c.stream, p);
+    HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(absorb_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, grid, n, p.partial,
+                     Abar_Adbar);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace k
+}  // namespace ipeps

@@ -116,6 +116,7 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
   else if (key == "canonical_edge") o.canonical_edge = v != 0.0;
   else if (key == "energy_fp64") o.energy_fp64 = v != 0.0;
   else if (key == "checkpoint") o.checkpoint = v != 0.0;
+  else if (key == "fused_absorb") o.fused_absorb = v != 0.0;
   else if (key == "energy_mixed") {
     int m = (int)v;
     IPEPS_CHECK(m == 1 || m == 3 || m == 6, "energy_mixed must be 1, 3 or 6");
@@ -416,6 +417,56 @@ int ipeps_tcon(ipeps_ctx* ctx, int dtype, int ndA, const int64_t* dimsA, const v
     if (C_host) {
       IPEPS_CHECK(C_cap >= C.numel(), "tcon: output buffer too small");
       to_host(c, C, C_host);
+    }
+  });
+}
+
+int ipeps_absorb(ipeps_ctx* ctx, int dtype, int fused, const int64_t* dimsX, const void* X, const int* lx,
+                 const int64_t* dimsA, const void* A, const int* la, const void* Ad, const int* lad, const void* G,
+                 void* Y_out, void* gX_out, void* gA_out, void* gAd_out) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && dimsX && X && lx && dimsA && A && la && Ad && lad && Y_out, "bad arguments");
+    IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_MIXED, "absorb: F64 or MIXED (float host data)");
+    Ctx& c = *ctx->c;
+    const DType dt = dtype == IPEPS_MIXED ? DType::F32 : DType::F64;
+    const std::vector<int64_t> dx(dimsX, dimsX + 6), da(dimsA, dimsA + 5);
+    Tensor tX = upload(c, dt, dt, dx, X), tA = upload(c, dt, dt, da, A), tAd = upload(c, dt, dt, da, Ad);
+    op::Prec saved = op::prec();
+    op::prec().fused_absorb = fused != 0;
+    op::prec().mixed = k::Mixed::BF16X3;
+    Tape tape;
+    tape.active = G != nullptr;
+    TapeScope ts(G ? &tape : nullptr);
+    Tensor lX = tX, lA = tA, lAd = tAd;
+    if (G) {
+      lX = op::leaf(tX);
+      lA = op::leaf(tA);
+      lAd = op::leaf(tAd);
+    }
+    Tensor Y;
+    try {
+      Y = op::absorb(c, lX, std::vector<int>(lx, lx + 6), lA, std::vector<int>(la, la + 5), lAd,
+                     std::vector<int>(lad, lad + 5));
+    } catch (...) {
+      op::prec() = saved;
+      throw;
+    }
+    op::prec() = saved;
+    to_host(c, Y, Y_out);
+    if (G) {
+      tape.active = false;
+      Tensor tG = upload(c, dt, dt, Y.dims, G);
+      IPEPS_CHECK(Y.node != nullptr, "absorb: no tape node");
+      op::acc(c, Y.node, tG);
+      op::backward(c, tape);
+      auto out = [&](const Tensor& leafT, void* host) {
+        if (!host) return;
+        if (leafT.node->grad.defined()) to_host(c, leafT.node->grad, host);
+        else std::memset(host, 0, leafT.bytes());
+      };
+      out(lX, gX_out);
+      out(lA, gA_out);
+      out(lAd, gAd_out);
     }
   });
 }

@@ -88,13 +88,11 @@ std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   // get_projector_left (src/ctm/ctmrg.jl:137-160)
   Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});
   Tensor CEE4 = tcon(c, C1E1, {1, -2, -5, -3}, E4, {1, -1, -6, -4});
-  Tensor CEEA = tcon(c, CEE4, {-1, -2, -3, -4, 1, 2}, A, {1, 2, -5, -6, -7});
-  Tensor LU = tcon(c, CEEA, {-1, -4, 1, 2, -2, -5, 3}, Ad, {1, 2, -3, -6, 3});
+  Tensor LU = op::absorb(c, CEE4, {-1, -4, 11, 12, 21, 22}, A, {21, 22, -2, -5, 3}, Ad, {11, 12, -3, -6, 3});
   Tensor R1 = tr2(c, mat(c, LU, 3));
   Tensor C4E3 = tcon(c, C4, {-1, 1}, E3, {1, -2, -3, -4});
   Tensor CEE4b = tcon(c, C4E3, {1, -2, -6, -4}, E4, {-1, 1, -5, -3});
-  Tensor CEEAb = tcon(c, CEE4b, {-1, -2, -3, -4, 1, 2}, A, {-5, 1, 2, -6, -7});
-  Tensor LB = tcon(c, CEEAb, {-1, -4, 1, 2, -2, -5, 3}, Ad, {-3, 1, 2, -6, 3});
+  Tensor LB = op::absorb(c, CEE4b, {-1, -4, 11, 12, 21, 22}, A, {-2, 21, 22, -5, 3}, Ad, {-3, 11, 12, -6, 3});
   Tensor R2 = mat(c, LB, 3);
   Proj p = get_projector(c, e, R1, R2, chi);
   if (only) {
@@ -117,13 +115,11 @@ void right_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   // get_projector_right (src/ctm/ctmrg.jl:176-198)
   Tensor C2E1 = tcon(c, C2, {1, -2}, E1, {-1, 1, -3, -4});
   Tensor CEE2 = tcon(c, C2E1, {-1, 1, -5, -3}, E2, {1, -2, -6, -4});
-  Tensor CEEA = tcon(c, CEE2, {-1, -2, -3, -4, 1, 2}, A, {1, -5, -6, 2, -7});
-  Tensor RU = tcon(c, CEEA, {-1, -4, 1, 2, -2, -5, 3}, Ad, {1, -3, -6, 2, 3});
+  Tensor RU = op::absorb(c, CEE2, {-1, -4, 11, 12, 21, 22}, A, {21, -2, -5, 22, 3}, Ad, {11, -3, -6, 12, 3});
   Tensor R1 = mat(c, RU, 3);
   Tensor C3E3 = tcon(c, C3, {-1, 1}, E3, {-2, 1, -3, -4});
   Tensor CEE2b = tcon(c, C3E3, {1, -2, -5, -3}, E2, {-1, 1, -6, -4});
-  Tensor CEEAb = tcon(c, CEE2b, {-1, -2, -5, -6, 1, 2}, A, {-3, -4, 1, 2, -7});
-  Tensor RB = tcon(c, CEEAb, {-1, -4, -2, -5, 1, 2, 3}, Ad, {-3, -6, 1, 2, 3});
+  Tensor RB = op::absorb(c, CEE2b, {-1, -4, 11, 12, 21, 22}, A, {-2, -5, 21, 22, 3}, Ad, {-3, -6, 11, 12, 3});
   Tensor R2 = mat(c, RB, 3);
   Proj p = get_projector(c, e, R1, R2, chi);
   if (only) {
@@ -145,13 +141,11 @@ void top_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   // get_projector_top (src/ctm/ctmrg.jl:214-236)
   Tensor C1E4 = tcon(c, C1, {1, -2}, E4, {1, -1, -3, -4});
   Tensor CEE1 = tcon(c, C1E4, {-1, 1, -6, -4}, E1, {1, -2, -5, -3});
-  Tensor CEEA = tcon(c, CEE1, {-1, -2, -5, -6, 1, 2}, A, {1, 2, -3, -4, -7});
-  Tensor UL = tcon(c, CEEA, {-1, -4, -2, -5, 1, 2, 3}, Ad, {1, 2, -3, -6, 3});
+  Tensor UL = op::absorb(c, CEE1, {-1, -4, 11, 12, 21, 22}, A, {21, 22, -2, -5, 3}, Ad, {11, 12, -3, -6, 3});
   Tensor R1 = mat(c, UL, 3);
   Tensor C2E2 = tcon(c, C2, {-1, 1}, E2, {1, -2, -3, -4});
   Tensor CEE1b = tcon(c, C2E2, {1, -2, -6, -4}, E1, {-1, 1, -5, -3});
-  Tensor CEEAb = tcon(c, CEE1b, {-1, -2, -5, -6, 1, 2}, A, {1, -3, -4, 2, -7});
-  Tensor UR = tcon(c, CEEAb, {-1, -4, -2, -5, 1, 2, 3}, Ad, {1, -3, -6, 2, 3});
+  Tensor UR = op::absorb(c, CEE1b, {-1, -4, 11, 12, 21, 22}, A, {21, -2, -5, 22, 3}, Ad, {11, -3, -6, 12, 3});
   Tensor R2 = mat(c, UR, 3);
   Proj p = get_projector(c, e, R1, R2, chi);
   if (only) {
@@ -173,13 +167,11 @@ void bottom_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   // get_projector_bottom (src/ctm/ctmrg.jl:238-252... via 226-250)
   Tensor C4E4 = tcon(c, C4, {1, -2}, E4, {-1, 1, -3, -4});
   Tensor CEE3 = tcon(c, C4E4, {-1, 1, -5, -3}, E3, {1, -2, -6, -4});
-  Tensor CEEA = tcon(c, CEE3, {-1, -2, -5, -6, 1, 2}, A, {-3, 1, 2, -4, -7});
-  Tensor BL = tcon(c, CEEA, {-1, -4, -2, -5, 1, 2, 3}, Ad, {-3, 1, 2, -6, 3});
+  Tensor BL = op::absorb(c, CEE3, {-1, -4, 11, 12, 21, 22}, A, {-2, 21, 22, -5, 3}, Ad, {-3, 11, 12, -6, 3});
   Tensor R1 = mat(c, BL, 3);
   Tensor C3E2 = tcon(c, C3, {1, -2}, E2, {-1, 1, -3, -4});
   Tensor CEE3b = tcon(c, C3E2, {-1, 1, -6, -4}, E3, {-2, 1, -5, -3});
-  Tensor CEEAb = tcon(c, CEE3b, {-1, -2, -5, -6, 1, 2}, A, {-3, -4, 1, 2, -7});
-  Tensor BR = tcon(c, CEEAb, {-1, -4, -2, -5, 1, 2, 3}, Ad, {-3, -6, 1, 2, 3});
+  Tensor BR = op::absorb(c, CEE3b, {-1, -4, 11, 12, 21, 22}, A, {-2, -5, 21, 22, 3}, Ad, {-3, -6, 11, 12, 3});
   Tensor R2 = tr2(c, mat(c, BR, 3));
   Proj p = get_projector(c, e, R1, R2, chi);
   if (only) {
@@ -294,6 +286,7 @@ void init_ctm1(Ctx& c, Env& e) {
 
 std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi) {
   op::prec().mixed = e.opt.mixed;
+  op::prec().fused_absorb = e.opt.fused_absorb;
   auto s = left_rg(c, e, chi);
   right_rg(c, e, chi);
   top_rg(c, e, chi);

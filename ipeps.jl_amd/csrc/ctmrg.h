@@ -20,6 +20,7 @@ struct Options {
   bool energy_fp64 = false;
   k::Mixed energy_mixed = k::Mixed::BF16X6;
   bool checkpoint = false;     // recompute each step in the reverse sweep instead of keeping its tape
+  bool fused_absorb = true;    // fused FP32-MFMA site absorption in the corners (mixed engine, 4 <= D <= 6)
 };
 
 // Plain ground-state environment (src/ctm/ctm_tensor.jl:3-18).  Storage dtype

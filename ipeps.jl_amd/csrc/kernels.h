@@ -122,6 +122,36 @@ void eye_dev(Ctx& c, DType dt, int64_t n, const void* G, double* out);
 void eig_to_svd(Ctx& c, DType dt, int64_t n, const double* lam, const void* W, double* s, void* Wr, double* sinv);
 // G <- (3 I - G) / 2  (one Newton-Schulz step X <- X (3I - X^H X)/2)
 void ns_coeff(Ctx& c, DType dt, int64_t n, void* G);
+// Fused site absorption (absorb.hip): Y[r,a,b] = sum X[r,u,v] A[u,a,p] Ad[v,b,p]
+// on FP32 data.  A tensor with two row legs (extents n1, n2, strides s1, s2)
+// and two groups of two legs of extent D (group x: index i = i1 + D i2 at
+// strides sx1, sx2; group y likewise).
+struct AbsorbLegs {
+  int n1 = 0, n2 = 0;
+  int64_t s1 = 0, s2 = 0, sx1 = 0, sx2 = 0, sy1 = 0, sy2 = 0;
+};
+struct AbsorbP {
+  int U = 0, P = 0, D = 0;    // U = D^2, P = physical extent
+  AbsorbLegs x, y;            // X: rows, (u, v); Y: rows, (a, b)
+  const float* X = nullptr;
+  float* Y = nullptr;
+  const float* Am = nullptr;  // A as [u + U (a + U p)]
+  const float* Adm = nullptr; // Ad as [v + U (b + U p)]
+};
+struct AbsorbWP {
+  int U = 0, P = 0, D = 0;
+  AbsorbLegs x, g;            // X: rows, (u, v); G = dL/dY: rows, (a, b)
+  const float* X = nullptr;
+  const float* G = nullptr;
+  const float* Am = nullptr;
+  const float* Adm = nullptr;
+  float* partial = nullptr;   // set by absorb_wgrad
+};
+bool absorb_supported(int D, int P);
+void absorb_fwd(Ctx& c, const AbsorbP& p);
+// out (device, 2 U^2 P floats): Abar as [u + U (a + U p)], then Adbar as [v + U (b + U p)]
+void absorb_wgrad(Ctx& c, AbsorbWP p, float* out);
+
 constexpr int SCHOL_MAXN_HOST = 192;  // largest n of small_chol_inv
 // L^-1 of the Cholesky factor G = L L^T of an SPD n x n FP64 matrix (n <= 192)
 // in one workgroup; info (device int) = 0, or j + 1 if pivot j is not positive.

@@ -428,6 +428,211 @@ Tensor tcon(Ctx& c, const Tensor& A, const std::vector<int>& la, const Tensor& B
 
 Tensor matmul(Ctx& c, const Tensor& A, const Tensor& B) { return tcon(c, A, {-1, 1}, B, {1, -2}); }
 
+// ---------------------------------------------------------------------------
+// fused site absorption
+// ---------------------------------------------------------------------------
+namespace {
+
+struct AbsorbPlan {
+  bool ok = false;
+  int D = 0, P = 0;
+  std::vector<int64_t> odims;       // output dims (labels -1, -2, ...)
+  std::vector<int> permA, permAd;   // A -> [u1, u2, a1, a2, p], Ad -> [v1, v2, b1, b2, p]
+  k::AbsorbLegs xl, yl;             // X: rows, (u, v); Y: rows, (a, b)
+};
+
+std::vector<int64_t> col_strides(const std::vector<int64_t>& d) {
+  std::vector<int64_t> s(d.size());
+  int64_t a = 1;
+  for (size_t i = 0; i < d.size(); ++i) { s[i] = a; a *= d[i]; }
+  return s;
+}
+
+// X (6 legs: 2 rows, 2 shared with A, 2 shared with Ad), A / Ad (D,D,D,D,P
+// with 2 legs shared with X, 2 free, 1 shared with each other).
+AbsorbPlan plan_absorb(const Tensor& X, const std::vector<int>& lx, const Tensor& A, const std::vector<int>& la,
+                       const Tensor& Ad, const std::vector<int>& lad) {
+  AbsorbPlan pl;
+  if (X.nd() != 6 || A.nd() != 5 || Ad.nd() != 5 || lx.size() != 6 || la.size() != 5 || lad.size() != 5) return pl;
+  if (X.dt != DType::F32 || A.dt != DType::F32 || Ad.dt != DType::F32) return pl;
+  auto find = [](const std::vector<int>& l, int lab) {
+    for (size_t i = 0; i < l.size(); ++i)
+      if (l[i] == lab) return (int)i;
+    return -1;
+  };
+  std::vector<int> rowsX, uX, vX, uA, vAd;  // X positions of row / u / v legs; partner positions
+  for (int i = 0; i < 6; ++i) {
+    if (lx[i] < 0) { rowsX.push_back(i); continue; }
+    const int ja = find(la, lx[i]), jd = find(lad, lx[i]);
+    if (ja >= 0 && jd < 0) { uX.push_back(i); uA.push_back(ja); }
+    else if (jd >= 0 && ja < 0) { vX.push_back(i); vAd.push_back(jd); }
+    else return pl;
+  }
+  if (rowsX.size() != 2 || uX.size() != 2 || vX.size() != 2) return pl;
+  int pA = -1, pAd = -1;
+  std::vector<int> aA, bAd;
+  for (int j = 0; j < 5; ++j) {
+    if (la[j] < 0) aA.push_back(j);
+    else if (find(lx, la[j]) < 0) { if (pA >= 0) return pl; pA = j; pAd = find(lad, la[j]); if (pAd < 0) return pl; }
+  }
+  for (int j = 0; j < 5; ++j)
+    if (lad[j] < 0) bAd.push_back(j);
+  if (aA.size() != 2 || bAd.size() != 2 || pA < 0) return pl;
+  const int64_t D = A.dims[0], P = A.dims[pA];
+  for (int j = 0; j < 5; ++j) {
+    if (j != pA && A.dims[j] != D) return pl;
+    if (j != pAd && Ad.dims[j] != D) return pl;
+  }
+  if (Ad.dims[pAd] != P || !k::absorb_supported((int)D, (int)P)) return pl;
+  // u = i(first shared A leg) + D i(second); same for v, a, b
+  auto order2 = [](std::vector<int>& partner, std::vector<int>* other) {
+    if (partner[0] > partner[1]) {
+      std::swap(partner[0], partner[1]);
+      if (other) std::swap((*other)[0], (*other)[1]);
+    }
+  };
+  order2(uA, &uX);
+  order2(vAd, &vX);
+  const int nout = 6;
+  pl.odims.assign(nout, 0);
+  std::vector<int> opos_rows(2), opos_a(2), opos_b(2);
+  for (int r = 0; r < 2; ++r) {
+    const int o = -lx[rowsX[r]] - 1;
+    if (o < 0 || o >= nout) return pl;
+    pl.odims[o] = X.dims[rowsX[r]];
+    opos_rows[r] = o;
+  }
+  for (int r = 0; r < 2; ++r) {
+    const int oa = -la[aA[r]] - 1, ob = -lad[bAd[r]] - 1;
+    if (oa < 0 || oa >= nout || ob < 0 || ob >= nout) return pl;
+    pl.odims[oa] = D;
+    pl.odims[ob] = D;
+    opos_a[r] = oa;
+    opos_b[r] = ob;
+  }
+  for (auto d : pl.odims)
+    if (d <= 0) return pl;
+  const auto xs = col_strides(X.dims), ys = col_strides(pl.odims);
+  int64_t ny = 1;
+  for (auto d : pl.odims) ny *= d;
+  if (X.numel() >= ((int64_t)1 << 31) || ny >= ((int64_t)1 << 31)) return pl;
+  // rows: the leg with the smaller X stride is r1 (consecutive rows of a block)
+  int r1 = 0, r2 = 1;
+  if (xs[rowsX[1]] < xs[rowsX[0]]) std::swap(r1, r2);
+  pl.xl.n1 = (int)X.dims[rowsX[r1]];
+  pl.xl.n2 = (int)X.dims[rowsX[r2]];
+  pl.xl.s1 = xs[rowsX[r1]];
+  pl.xl.s2 = xs[rowsX[r2]];
+  pl.xl.sx1 = xs[uX[0]];
+  pl.xl.sx2 = xs[uX[1]];
+  pl.xl.sy1 = xs[vX[0]];
+  pl.xl.sy2 = xs[vX[1]];
+  pl.yl.n1 = pl.xl.n1;
+  pl.yl.n2 = pl.xl.n2;
+  pl.yl.s1 = ys[opos_rows[r1]];
+  pl.yl.s2 = ys[opos_rows[r2]];
+  pl.yl.sx1 = ys[opos_a[0]];
+  pl.yl.sx2 = ys[opos_a[1]];
+  pl.yl.sy1 = ys[opos_b[0]];
+  pl.yl.sy2 = ys[opos_b[1]];
+  pl.permA = {uA[0], uA[1], aA[0], aA[1], pA};
+  pl.permAd = {vAd[0], vAd[1], bAd[0], bAd[1], pAd};
+  pl.D = (int)D;
+  pl.P = (int)P;
+  pl.ok = true;
+  return pl;
+}
+
+// raw fused launch: Y (dims ydims) from X through legs xl -> yl
+Tensor absorb_raw(Ctx& c, const Tensor& X, const k::AbsorbLegs& xl, const Tensor& Am, const Tensor& Adm,
+                  const std::vector<int64_t>& ydims, const k::AbsorbLegs& yl, int D, int P) {
+  Tensor Y = c.empty(DType::F32, ydims);
+  k::AbsorbP p;
+  p.U = D * D;
+  p.P = P;
+  p.D = D;
+  p.x = xl;
+  p.y = yl;
+  p.X = X.ptr<float>();
+  p.Y = Y.ptr<float>();
+  p.Am = Am.ptr<float>();
+  p.Adm = Adm.ptr<float>();
+  k::absorb_fwd(c, p);
+  return Y;
+}
+
+}  // namespace
+
+Tensor absorb(Ctx& c, const Tensor& X, const std::vector<int>& lx, const Tensor& A, const std::vector<int>& la,
+              const Tensor& Ad, const std::vector<int>& lad) {
+  AbsorbPlan pl;
+  if (prec().fused_absorb) pl = plan_absorb(X, lx, A, la, Ad, lad);
+  if (!pl.ok) {
+    // two pairwise contractions: X with A over the u legs, then with Ad
+    std::vector<int> l1x(lx.size()), l1a(la.size()), l2xa, l2d = lad;
+    int next = 1;
+    for (size_t i = 0; i < lx.size(); ++i) {
+      bool shared_a = std::find(la.begin(), la.end(), lx[i]) != la.end() && lx[i] > 0;
+      if (shared_a) l1x[i] = lx[i];
+      else { l1x[i] = -(next++); l2xa.push_back(lx[i]); }
+    }
+    for (size_t j = 0; j < la.size(); ++j) {
+      bool shared_x = la[j] > 0 && std::find(lx.begin(), lx.end(), la[j]) != lx.end();
+      if (shared_x) l1a[j] = la[j];
+      else { l1a[j] = -(next++); l2xa.push_back(la[j]); }
+    }
+    Tensor XA = tcon(c, X, l1x, A, l1a);
+    return tcon(c, XA, l2xa, Ad, l2d);
+  }
+  const int U = pl.D * pl.D;
+  Tensor Am = reshape(c, permute(c, A, pl.permA), {U, U, pl.P});
+  Tensor Adm = reshape(c, permute(c, Ad, pl.permAd), {U, U, pl.P});
+  Tensor Y = absorb_raw(c, X, pl.xl, Am, Adm, pl.odims, pl.yl, pl.D, pl.P);
+  if (needs(X) || needs(Am) || needs(Adm)) {
+    auto nX = X.node, nA = Am.node, nD = Adm.node;
+    Tensor Xk = X, Ak = Am, Dk = Adm;
+    Xk.node.reset();
+    Ak.node.reset();
+    Dk.node.reset();
+    const AbsorbPlan plk = pl;
+    Y.node = make_node([=](Ctx& c, Node& n) {
+      const Tensor& G = n.grad;
+      IPEPS_CHECK(G.dt == DType::F32, "absorb: FP32 cotangent expected");
+      if (nX) {
+        // Xbar_r = sum_p A_p G_r Ad_p^T: the forward kernel with A, Ad transposed
+        Tensor AmT = permute_raw(c, Ak, {1, 0, 2}), AdmT = permute_raw(c, Dk, {1, 0, 2});
+        acc(c, nX, absorb_raw(c, G, plk.yl, AmT, AdmT, Xk.dims, plk.xl, plk.D, plk.P));
+      }
+      if (nA || nD) {
+        Tensor out = c.empty(DType::F32, {2 * U * U * plk.P});
+        k::AbsorbWP w;
+        w.U = U;
+        w.P = plk.P;
+        w.D = plk.D;
+        w.x = plk.xl;
+        w.g = plk.yl;
+        w.X = Xk.ptr<float>();
+        w.G = G.ptr<float>();
+        w.Am = Ak.ptr<float>();
+        w.Adm = Dk.ptr<float>();
+        k::absorb_wgrad(c, w, out.ptr<float>());
+        if (nA) {
+          Tensor gA = out;  // first half of out
+          gA.dims = {U, U, plk.P};
+          acc(c, nA, clone(c, gA));
+        }
+        if (nD) {
+          Tensor gD = out;
+          gD.offset += (size_t)U * U * plk.P * sizeof(float);
+          gD.dims = {U, U, plk.P};
+          acc(c, nD, clone(c, gD));
+        }
+      }
+    });
+  }
+  return Y;
+}
+
 Tensor adjoint_matmul(Ctx& c, const Tensor& A, const Tensor& B) {
   return tcon(c, conj(c, A), {1, -1}, B, {1, -2});
 }

@@ -44,6 +44,7 @@ namespace op {
 // Precision knobs of the contraction engine.
 struct Prec {
   k::Mixed mixed = k::Mixed::BF16X3;
+  bool fused_absorb = true;  // op::absorb may use the fused FP32-MFMA kernel
 };
 Prec& prec();
 // Sets prec().mixed for a scope (the VJPs restore the setting their forward
@@ -90,6 +91,13 @@ Tensor renormalize_by(Ctx& c, const Tensor& x, const Tensor& ref);
 Tensor scale_vec(Ctx& c, const Tensor& X, const Tensor& s, bool rowwise);
 // s .^ (-1/2) for a real FP64 vector.
 Tensor inv_sqrt(Ctx& c, const Tensor& s);
+// Y = X A Ad contracted as the labels say (ncon convention), where X has two
+// open legs, two legs shared with A and two with Ad, and A / Ad share their
+// physical leg: the enlarged-corner absorption of a site.  FP32 data with
+// D^2 <= 36, D >= 4 and a physical extent <= 2 run the fused FP32-MFMA kernel
+// (absorb.hip) with its own VJPs; anything else two tcon contractions.
+Tensor absorb(Ctx& c, const Tensor& X, const std::vector<int>& lx, const Tensor& A, const std::vector<int>& la,
+              const Tensor& Ad, const std::vector<int>& lad);
 Tensor matmul(Ctx& c, const Tensor& A, const Tensor& B);       // A * B
 Tensor adjoint_matmul(Ctx& c, const Tensor& A, const Tensor& B);  // A^H * B
 Tensor cast(Ctx& c, const Tensor& t, DType dt);                   // differentiable F64 <-> F32

@@ -113,7 +113,7 @@ def _hpair(H):
 ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "mixed_cutoff", "fast_orth",
                "fast_svd",
                "gram_guard", "canonical_edge", "mixed", "energy_fp64", "energy_mixed",
-               "checkpoint", "draws")
+               "checkpoint", "fused_absorb", "draws")
 
 
 class CTMTensors:
@@ -334,6 +334,31 @@ def gemm(A, B, transA=False, conjA=False, transB=False, conjB=False, mixed=None,
     return Cf
 
 
+def absorb(X, lx, A, la, Ad, lad, G=None, mixed=True, fused=True, ctx=None):
+    """Y = X A Ad (ncon labels; one enlarged-corner site absorption) on the
+    device; with a cotangent G also (gX, gA, gAd).  mixed: float32 data
+    (fused FP32-MFMA kernel when fused and 4 <= D <= 6, d <= 2); else FP64."""
+    ctx = ctx or default_context()
+    dt = np.float32 if mixed else np.float64
+    Xf, Af, Adf = (np.asfortranarray(np.asarray(x), dtype=dt) for x in (X, A, Ad))
+    sizes = {}
+    for ind, T in ((lx, Xf), (la, Af), (lad, Adf)):
+        for l, d in zip(ind, T.shape):
+            if l < 0:
+                sizes[-l - 1] = d
+    Y = np.empty(tuple(sizes[k] for k in range(len(sizes))), dtype=dt, order="F")
+    ints = lambda l: (C.c_int * len(l))(*[int(v) for v in l])
+    Gf = None if G is None else np.asfortranarray(np.asarray(G), dtype=dt)
+    gX = np.empty_like(Xf) if G is not None else None
+    gA = np.empty_like(Af) if G is not None else None
+    gAd = np.empty_like(Adf) if G is not None else None
+    p = lambda a: None if a is None else _ptr(a)
+    check(lib().ipeps_absorb(ctx.h, MIXED if mixed else F64, int(bool(fused)), _i64(Xf.shape), _ptr(Xf), ints(lx),
+                             _i64(Af.shape), _ptr(Af), ints(la), _ptr(Adf), ints(lad), p(Gf), _ptr(Y), p(gX), p(gA),
+                             p(gAd)))
+    return Y if G is None else (Y, gX, gA, gAd)
+
+
 def svd1(A, ctx=None):
     """Full SVD of a square matrix on the device: (U, S, V) with A = U S V^H."""
     ctx = ctx or default_context()
@@ -391,6 +416,6 @@ def svd_back(U, S, V, dU=None, dS=None, dV=None, eta=1e-12, ctx=None):
     return out
 
 
-__all__ = ["Context", "CTMTensors", "rg_step", "run_ctm", "get_gs_energy", "get_gs_norm", "compute_gs_energy",
+__all__ = ["absorb", "Context", "CTMTensors", "rg_step", "run_ctm", "get_gs_energy", "get_gs_norm", "compute_gs_energy",
            "step_adjoint", "gs_energy_grad", "tcon", "gemm", "svd1", "svd_back", "default_context", "IpepsError",
            "ENV_OPTIONS"]
