@@ -57,13 +57,30 @@ def flops_per_unit(D, chi, d=2, k_extra=16, q=1):
     return 3 * f_fwd
 
 
-def kernel_roofline(stats):
-    """Dominant kernel (largest total time) and its achieved rate."""
-    name, (launches, ms, flops, bytes_) = max(stats.items(), key=lambda kv: kv[1][1])
+LIBRARY_PREFIXES = ("rocsolver_", "rocblas_")
+
+
+def dominant_kernel(stats):
+    """Hand-written kernel with the largest total time (library calls, which
+    carry no algorithmic counts, are reported separately)."""
+    own = {k: v for k, v in stats.items() if not k.startswith(LIBRARY_PREFIXES)}
+    return max(own.items(), key=lambda kv: kv[1][1])[0] if own else None
+
+
+def kernel_roofline(stats, name=None):
+    """Achieved rate of one kernel (default: the dominant hand-written one)
+    from its algorithmic flops / bytes per launch and its average launch time."""
+    name = name or dominant_kernel(stats)
+    launches, ms, flops, bytes_ = stats[name]
     avg_ms = ms / max(1, launches)
-    if name.startswith("gemm_"):
+    if name.startswith("gemm_") or name.startswith("absorb_"):
         unit = "TFLOP/s"
-        peak = PEAK_TFLOPS["bf16"] if name.startswith("gemm_f32") else PEAK_TFLOPS["f64"]
+        if name.startswith("gemm_f32"):
+            peak = PEAK_TFLOPS["bf16"]   # split-bf16 products on the bf16 MFMA
+        elif name.startswith("absorb_"):
+            peak = PEAK_TFLOPS["f32"]    # FP32 MFMA
+        else:
+            peak = PEAK_TFLOPS["f64"]
         achieved = (flops / launches) / (avg_ms * 1e-3) / 1e12
         bound = "mfma"
     else:
@@ -71,8 +88,24 @@ def kernel_roofline(stats):
         peak = PEAK_HBM_GBS
         achieved = (bytes_ / launches) / (avg_ms * 1e-3) / 1e9 if bytes_ else 0.0
         bound = "hbm"
+    traffic, source = measured_traffic(name)
     return dict(kernel=name, bound=bound, achieved=achieved, peak=peak, unit=unit, frac=achieved / peak,
-                traffic=None, launches=launches, avg_ms=avg_ms, total_ms=ms)
+                traffic=traffic, traffic_source=source, algorithmic_bytes=bytes_ / max(1, launches),
+                launches=launches, avg_ms=avg_ms, total_ms=ms)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summaries (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), as a
+    line 'traffic_bytes_per_launch <kernel>: <bytes>' in profiles/*.md."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch ") and ":" in line:
+                k, v = line[len("traffic_bytes_per_launch "):].rsplit(":", 1)
+                if k.strip() == kernel:
+                    return float(v), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline_sample(D, chi, seed):
@@ -102,6 +135,29 @@ def cpu_baseline_sample(D, chi, seed):
                 move_seconds=t)
 
 
+def batch_c4(pkg, steps=10, workers=8):
+    """Uneven 64-instance workload (D in {2,3,4}, chi in {16..96}): mixed and
+    FP64 engines advance all instances `steps` rg_steps after one warm-up
+    step; aggregate instance-steps/s and |dE/site| mixed vs FP64."""
+    import numpy as np
+    ins = pkg.c4_instances()
+    As, chis, Hs = [x[0] for x in ins], [x[1] for x in ins], [x[2] for x in ins]
+    out = {}
+    Es = {}
+    for dt in ("mixed", "f64"):
+        b = pkg.BatchedCTM(As, chis, Hs, dtype=dt, workers=workers, options=dict(svd_mode=1))
+        b.run(1)
+        E, sec = b.run(steps)
+        Es[dt] = E
+        out[dt] = dict(instance_steps_per_s=steps * len(ins) / sec, seconds=sec)
+        b.close()
+        log(f"[bench] c4 batch {dt}: {steps} steps x {len(ins)} instances in {sec:.2f} s")
+    d = np.abs(Es["mixed"].real - Es["f64"].real)
+    out.update(instances=len(ins), steps=1 + steps, workers=workers,
+               dE_per_site_median=float(np.median(d)), dE_per_site_max=float(d.max()))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +169,7 @@ def main():
     ap.add_argument("--power-iters", type=int, default=1)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -129,15 +186,23 @@ def main():
     sat_steps = 2  # chi saturates after the first step (N = D^4 >= chi)
 
     def run_path(dtype, n_warm, n_time, extra):
+        """Warm-up steps (the last one with every launch timed, to find the
+        kernel mix and the dominant hand-written kernel), then the timed steps
+        with HIP events around the dominant kernel's launches only."""
         ts = pkg.CTMTensors(A, {}, dtype=dtype, options=dict(base_opts, **extra))
         for _ in range(sat_steps):
             pkg.rg_step(ts, chi)
         E = g = None
-        for _ in range(n_warm):
+        for i in range(max(1, n_warm)):
+            if i == max(1, n_warm) - 1:
+                ctx.sync()
+                ctx.set_timing(True)
+                ctx.kernel_stats(reset=True)
             E, g = pkg.step_adjoint(ts, chi, H)
         ctx.sync()
-        ctx.set_timing(True)
-        ctx.kernel_stats(reset=True)
+        mix = ctx.kernel_stats(reset=True)
+        dom = dominant_kernel(mix)
+        ctx.set_timing(True, prefix=dom)
         t0 = time.perf_counter()
         for _ in range(n_time):
             E, g = pkg.step_adjoint(ts, chi, H)
@@ -145,15 +210,16 @@ def main():
         t = time.perf_counter() - t0
         stats = ctx.kernel_stats(reset=True)
         ctx.set_timing(False)
-        return t, E, g, stats, ts
+        stats = {k: v for k, v in stats.items() if k == dom}
+        return t, E, g, stats, ts, mix
 
     log(f"[bench] workload {a.workload}: D={D} chi={chi} N={chi*D*D}; mixed bf16x{a.mixed}")
-    t_m, E_m, g_m, st_m, ts_m = run_path("mixed", a.warmup, a.steps, dict(mixed=a.mixed))
+    t_m, E_m, g_m, st_m, ts_m, mix_m = run_path("mixed", a.warmup, a.steps, dict(mixed=a.mixed))
     log(f"[bench] mixed: {a.steps} steps in {t_m:.3f} s, E={E_m.real:.12f}")
     out_extra = {}
     if not a.no_fp64:
         import numpy as np
-        t_64, E_64, g_64, st_64, ts_64 = run_path("f64", a.warmup, a.steps, {})
+        t_64, E_64, g_64, st_64, ts_64, _ = run_path("f64", a.warmup, a.steps, {})
         log(f"[bench] fp64 : {a.steps} steps in {t_64:.3f} s, E={E_64.real:.12f}")
         # Precision error of one step + adjoint: both engines start from the
         # same FP64 environment (the FP64 run's last state) with the same
@@ -172,8 +238,11 @@ def main():
         out_extra["speedup_vs_fp64_gpu"] = t_64 / t_m
         out_extra["dE_per_site_vs_fp64"] = dE
         out_extra["grad_rel_err_vs_fp64"] = gerr
-    for k, v in sorted(st_m.items(), key=lambda kv: -kv[1][1])[:12]:
+    tot_mix = sum(v[1] for v in mix_m.values())
+    log(f"[bench] kernel mix of one mixed step + adjoint (all launches timed): {tot_mix:.1f} ms")
+    for k, v in sorted(mix_m.items(), key=lambda kv: -kv[1][1])[:14]:
         log(f"[bench]   {k:48s} n={v[0]:5d} {v[1]:9.3f} ms")
+    lib_ms = sum(v[1] for k, v in mix_m.items() if k.startswith(LIBRARY_PREFIXES))
     roof = kernel_roofline(st_m)
     value = a.steps / t_m
     F = flops_per_unit(D, chi, d, a.oversample, a.power_iters)
@@ -203,7 +272,11 @@ def main():
         "model_flops_rate_tflops": F * value / 1e12,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "step_mix": {"timed_kernels_ms": tot_mix, "rocsolver_rocblas_ms": lib_ms,
+                     "top": {k: round(v[1], 3) for k, v in sorted(mix_m.items(), key=lambda kv: -kv[1][1])[:8]}},
     }
+    if not a.no_batch:
+        res["batched_c4"] = batch_c4(pkg)
     res.update(out_extra)
     print(json.dumps(res), flush=True)
 

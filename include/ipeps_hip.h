@@ -55,6 +55,10 @@ int ipeps_ctx_mem(ipeps_ctx* ctx, int64_t* live, int64_t* peak);
  * by hipEvents on the context stream.  Engine instrumentation, no reference
  * counterpart. */
 int ipeps_ctx_set_timing(ipeps_ctx* ctx, int on);
+/* Restrict the timing brackets to launches whose kernel name starts with
+ * prefix (NULL or "" = every launch): a measured region then carries events
+ * around the kernel of interest only. */
+int ipeps_ctx_set_timing_filter(ipeps_ctx* ctx, const char* prefix);
 /* Per-kernel totals since the last reset as JSON
  * {"name": [launches, total_ms, algorithmic_flops, algorithmic_bytes], ...};
  * reset != 0 clears them. */

@@ -21,6 +21,7 @@ SIGNATURES = {
     "ipeps_ctx_sync": (C.c_int, [_P]),
     "ipeps_ctx_mem": (C.c_int, [_P, _I64P, _I64P]),
     "ipeps_ctx_set_timing": (C.c_int, [_P, C.c_int]),
+    "ipeps_ctx_set_timing_filter": (C.c_int, [_P, C.c_char_p]),
     "ipeps_ctx_kernel_stats": (C.c_int, [_P, C.c_char_p, C.c_int64, C.c_int]),
     "ipeps_env_create": (C.c_int, [_P, C.c_int, _I64P, _P, C.POINTER(_P)]),
     "ipeps_env_destroy": (C.c_int, [_P]),

@@ -196,6 +196,14 @@ int ipeps_ctx_set_timing(ipeps_ctx* ctx, int on) {
   });
 }
 
+int ipeps_ctx_set_timing_filter(ipeps_ctx* ctx, const char* prefix) {
+  return guard([&] {
+    IPEPS_CHECK(ctx, "null ctx");
+    ctx->c->resolve_timing();
+    ctx->c->timing_filter = prefix ? prefix : "";
+  });
+}
+
 int ipeps_ctx_kernel_stats(ipeps_ctx* ctx, char* buf, int64_t cap, int reset) {
   return guard([&] {
     IPEPS_CHECK(ctx && buf && cap > 0, "bad arguments");

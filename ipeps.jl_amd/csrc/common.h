@@ -141,6 +141,7 @@ struct Ctx {
   };
   bool timing = false;
   bool timing_detail = false;  // key GEMM statistics by shape as well
+  std::string timing_filter;   // bracket only launches whose name starts with it (empty: all)
   std::vector<KEvent> kev;
   std::map<std::string, KStat> kstats;
   void resolve_timing();
@@ -158,7 +159,8 @@ struct TimedLaunch {
   Ctx& c;
   bool on;
   Ctx::KEvent ev;
-  TimedLaunch(Ctx& cc, const std::string& name, double flops, double bytes) : c(cc), on(cc.timing) {
+  TimedLaunch(Ctx& cc, const std::string& name, double flops, double bytes)
+      : c(cc), on(cc.timing && (cc.timing_filter.empty() || name.compare(0, cc.timing_filter.size(), cc.timing_filter) == 0)) {
     if (!on) return;
     ev.name = name;
     ev.flops = flops;

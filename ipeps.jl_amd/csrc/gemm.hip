@@ -562,9 +562,14 @@ void gemm(Ctx& c, const GemmArgs& g) {
     q.sB = bt ? kc * g.ldb : kc;
     set_vec_flags(g, q);
     grid.z = (unsigned)S;
-    TimedLaunch tl(c, name, flops, bytes);
-    launch(c, g, q, grid);
+    {
+      TimedLaunch tl(c, name, flops, bytes);
+      launch(c, g, q, grid);
+    }
     const int64_t MN = g.M * g.N;
+    const char* rname = g.dt == DType::F64 ? "splitk_reduce<double>" : g.dt == DType::C128 ? "splitk_reduce<double2>"
+                                                                                          : "splitk_reduce<float>";
+    TimedLaunch tr(c, rname, 0.0, (double)(S + 1) * MN * es);
     unsigned rb = (unsigned)std::min<int64_t>((MN + 255) / 256, 4096);
     if (g.dt == DType::F64)
       hipLaunchKernelGGL(splitk_reduce<double>, dim3(rb), dim3(256), 0, c.stream, g.M, g.N, S, (const double*)ws->p,

@@ -71,8 +71,10 @@ class Context:
         check(lib().ipeps_ctx_mem(self.h, C.byref(live), C.byref(peak)))
         return live.value, peak.value
 
-    def set_timing(self, on=True):
-        """on: False/0 off, True/1 per kernel, 2 per kernel and GEMM shape."""
+    def set_timing(self, on=True, prefix=None):
+        """on: False/0 off, True/1 per kernel, 2 per kernel and GEMM shape;
+        prefix: only kernels whose name starts with it are timed."""
+        check(lib().ipeps_ctx_set_timing_filter(self.h, (prefix or "").encode()))
         check(lib().ipeps_ctx_set_timing(self.h, int(on)))
 
     def kernel_stats(self, reset=True):
