@@ -76,7 +76,9 @@ int ipeps_env_destroy(ipeps_env* env);
  * "energy_fp64" (0/1: energy contractions of a MIXED environment in FP64),
  * "energy_mixed" (1, 3 or 6: bf16 products of those contractions when not
  * FP64, default 6; the density-matrix traces and ratios are FP64 always),
- * "checkpoint" (0/1), "fused_absorb" (0/1, default 1: MIXED environments
+ * "checkpoint" (0/1), "paired_moves" (0/1, default 1: the left and right
+ * moves, then the top and bottom moves, of rg_step run concurrently on two
+ * streams; results are identical either way), "fused_absorb" (0/1, default 1: MIXED environments
  * with 4 <= D <= 6 and d <= 2 absorb A and Ad into the enlarged corners in
  * one FP32-MFMA kernel instead of two split-bf16 GEMMs), and for the
  * randomized mode

@@ -117,6 +117,7 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
   else if (key == "energy_fp64") o.energy_fp64 = v != 0.0;
   else if (key == "checkpoint") o.checkpoint = v != 0.0;
   else if (key == "fused_absorb") o.fused_absorb = v != 0.0;
+  else if (key == "paired_moves") o.paired_moves = v != 0.0;
   else if (key == "energy_mixed") {
     int m = (int)v;
     IPEPS_CHECK(m == 1 || m == 3 || m == 6, "energy_mixed must be 1, 3 or 6");

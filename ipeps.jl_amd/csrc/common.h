@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -83,6 +84,7 @@ class Pool {
     return (b + step - 1) / step * step;
   }
   int device_;
+  std::mutex mu_;  // blocks of a pool may be released by another thread
   std::multimap<size_t, void*> free_;
   size_t live_ = 0, peak_ = 0, cached_ = 0;
 };
@@ -146,6 +148,10 @@ struct Ctx {
   std::map<std::string, KStat> kstats;
   void resolve_timing();
 
+  // Second lane for the independent moves of a CTMRG step (own stream,
+  // allocator and solver handle), created on first use.
+  std::unique_ptr<Ctx> lane;
+  Ctx& second_lane();
   Tensor empty(DType dt, const std::vector<int64_t>& dims);
   Tensor zeros(DType dt, const std::vector<int64_t>& dims);
   void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }

@@ -5,6 +5,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <thread>
 
 namespace ipeps {
 namespace ctm {
@@ -29,9 +31,11 @@ struct Proj {
 
 // get_projector (src/ctm/ctmrg.jl:254-265): M = R1 R2, truncated SVD,
 // P1 = R2 V S^-1/2, P2 = S^-1/2 U^H R1.
-Proj get_projector(Ctx& c, const Env& e, const Tensor& R1, const Tensor& R2, int64_t chi) {
+// `draw` is the position of this projector's test matrix in the environment's
+// stream: e.draws + direction (rg_step advances e.draws by 4 per step).
+Proj get_projector(Ctx& c, const Env& e, const Tensor& R1, const Tensor& R2, int64_t chi, uint64_t draw) {
   op::SvdOpts so = e.opt.svd;
-  so.seed = op::draw_seed(e.opt.svd.seed, e.draws++);
+  so.seed = op::draw_seed(e.opt.svd.seed, draw);
   auto r = op::svd_trunc(c, R1, R2, chi, so);
   Tensor isq = op::inv_sqrt(c, r.S);
   Proj p;
@@ -83,7 +87,7 @@ void finish(Ctx& c, Env& e, int ci, const Tensor& C_, int cj, const Tensor& Cb, 
   e.C[cj] = op::renormalize(c, Cb);
 }
 
-std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
+std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) {
   const Tensor &C1 = e.C[0], &C4 = e.C[3], &E1 = e.E[0], &E3 = e.E[2], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
   // get_projector_left (src/ctm/ctmrg.jl:137-160)
   Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});
@@ -94,7 +98,7 @@ std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   Tensor CEE4b = tcon(c, C4E3, {1, -2, -6, -4}, E4, {-1, 1, -5, -3});
   Tensor LB = op::absorb(c, CEE4b, {-1, -4, 11, 12, 21, 22}, A, {-2, 21, 22, -5, 3}, Ad, {-3, 11, 12, -6, 3});
   Tensor R2 = mat(c, LB, 3);
-  Proj p = get_projector(c, e, R1, R2, chi);
+  Proj p = get_projector(c, e, R1, R2, chi, draw);
   if (only) {
     *only = p;
     return p.s;
@@ -110,7 +114,7 @@ std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   return p.s;
 }
 
-void right_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
+void right_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) {
   const Tensor &C2 = e.C[1], &C3 = e.C[2], &E1 = e.E[0], &E2 = e.E[1], &E3 = e.E[2], &A = e.A, &Ad = e.Ad;
   // get_projector_right (src/ctm/ctmrg.jl:176-198)
   Tensor C2E1 = tcon(c, C2, {1, -2}, E1, {-1, 1, -3, -4});
@@ -121,7 +125,7 @@ void right_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   Tensor CEE2b = tcon(c, C3E3, {1, -2, -5, -3}, E2, {-1, 1, -6, -4});
   Tensor RB = op::absorb(c, CEE2b, {-1, -4, 11, 12, 21, 22}, A, {-2, -5, 21, 22, 3}, Ad, {-3, -6, 11, 12, 3});
   Tensor R2 = mat(c, RB, 3);
-  Proj p = get_projector(c, e, R1, R2, chi);
+  Proj p = get_projector(c, e, R1, R2, chi, draw);
   if (only) {
     *only = p;
     return;
@@ -136,7 +140,7 @@ void right_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   finish(c, e, 1, nC2, 2, nC3, 1, nE2);
 }
 
-void top_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
+void top_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) {
   const Tensor &C1 = e.C[0], &C2 = e.C[1], &E1 = e.E[0], &E2 = e.E[1], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
   // get_projector_top (src/ctm/ctmrg.jl:214-236)
   Tensor C1E4 = tcon(c, C1, {1, -2}, E4, {1, -1, -3, -4});
@@ -147,7 +151,7 @@ void top_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   Tensor CEE1b = tcon(c, C2E2, {1, -2, -6, -4}, E1, {-1, 1, -5, -3});
   Tensor UR = op::absorb(c, CEE1b, {-1, -4, 11, 12, 21, 22}, A, {21, -2, -5, 22, 3}, Ad, {11, -3, -6, 12, 3});
   Tensor R2 = mat(c, UR, 3);
-  Proj p = get_projector(c, e, R1, R2, chi);
+  Proj p = get_projector(c, e, R1, R2, chi, draw);
   if (only) {
     *only = p;
     return;
@@ -162,7 +166,7 @@ void top_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   finish(c, e, 0, nC1, 1, nC2, 0, nE1);
 }
 
-void bottom_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
+void bottom_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) {
   const Tensor &C3 = e.C[2], &C4 = e.C[3], &E2 = e.E[1], &E3 = e.E[2], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
   // get_projector_bottom (src/ctm/ctmrg.jl:238-252... via 226-250)
   Tensor C4E4 = tcon(c, C4, {1, -2}, E4, {-1, 1, -3, -4});
@@ -173,7 +177,7 @@ void bottom_rg(Ctx& c, Env& e, int64_t chi, Proj* only = nullptr) {
   Tensor CEE3b = tcon(c, C3E2, {-1, 1, -6, -4}, E3, {-2, 1, -5, -3});
   Tensor BR = op::absorb(c, CEE3b, {-1, -4, 11, 12, 21, 22}, A, {-2, -5, 21, 22, 3}, Ad, {-3, -6, 11, 12, 3});
   Tensor R2 = tr2(c, mat(c, BR, 3));
-  Proj p = get_projector(c, e, R1, R2, chi);
+  Proj p = get_projector(c, e, R1, R2, chi, draw);
   if (only) {
     *only = p;
     return;
@@ -253,11 +257,12 @@ void get_dms(Ctx& c, const Env& e, Tensor& roh, Tensor& rov) {
 void projector(Ctx& c, const Env& e0, int dir, int64_t chi, Tensor& P1, Tensor& P2, std::vector<double>& s) {
   Env e = e0;  // shares the tensors; the *_rg functions stop after the projector
   Proj p;
+  const uint64_t draw = e0.draws + (uint64_t)dir;
   switch (dir) {
-    case 0: left_rg(c, e, chi, &p); break;
-    case 1: right_rg(c, e, chi, &p); break;
-    case 2: top_rg(c, e, chi, &p); break;
-    default: bottom_rg(c, e, chi, &p); break;
+    case 0: left_rg(c, e, chi, draw, &p); break;
+    case 1: right_rg(c, e, chi, draw, &p); break;
+    case 2: top_rg(c, e, chi, draw, &p); break;
+    default: bottom_rg(c, e, chi, draw, &p); break;
   }
   P1 = p.P1;
   P2 = p.P2;
@@ -284,13 +289,82 @@ void init_ctm1(Ctx& c, Env& e) {
   e.E[3] = E({-1, 1, -3, -5, 2}, {-2, 1, -4, -6, 2});
 }
 
+namespace {
+
+// f on the context's stream and g on its second lane, concurrently (their
+// inputs and outputs are disjoint parts of the environment).  The lane
+// waits for everything the main stream enqueued before the fork; the main
+// stream waits for the lane's work after the join.  Tensors the lane drops
+// are returned to their pools only after that wait, and the lane's kernel
+// timings are merged into the context's.
+template <typename F, typename G>
+void run_pair(Ctx& c, Env& e, F&& f, G&& g) {
+  Ctx& l = c.second_lane();
+  std::array<Tensor, 4> holdC = e.C, holdE = e.E;  // released after the join
+  hipEvent_t fork, join;
+  HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+  HIP_CHECK(hipEventRecord(fork, c.stream));
+  HIP_CHECK(hipStreamWaitEvent(l.stream, fork, 0));
+  l.timing = c.timing;
+  l.timing_detail = c.timing_detail;
+  l.timing_filter = c.timing_filter;
+  // The lane records into its own tape, appended after the main lane's nodes
+  // at the join: the tape order is then the sequential schedule's, so the
+  // reverse sweep accumulates shared cotangents in the same order.
+  Tape* tape = current_tape();
+  Tape lane_tape;
+  lane_tape.active = tape && tape->active;
+  const op::Prec prec = op::prec();
+  std::exception_ptr lane_err;
+  std::thread th([&] {
+    try {
+      HIP_CHECK(hipSetDevice(l.device));
+      TapeScope ts(tape ? &lane_tape : nullptr);
+      op::prec() = prec;
+      g(l);
+      HIP_CHECK(hipEventRecord(join, l.stream));
+    } catch (...) {
+      lane_err = std::current_exception();
+      (void)hipEventRecord(join, l.stream);
+    }
+  });
+  std::exception_ptr main_err;
+  try {
+    f(c);
+  } catch (...) {
+    main_err = std::current_exception();
+  }
+  th.join();
+  if (tape) tape->order.insert(tape->order.end(), lane_tape.order.begin(), lane_tape.order.end());
+  (void)hipStreamWaitEvent(c.stream, join, 0);
+  (void)hipEventDestroy(fork);
+  (void)hipEventDestroy(join);
+  c.kev.insert(c.kev.end(), l.kev.begin(), l.kev.end());
+  l.kev.clear();
+  if (main_err) std::rethrow_exception(main_err);
+  if (lane_err) std::rethrow_exception(lane_err);
+}
+
+}  // namespace
+
 std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi) {
   op::prec().mixed = e.opt.mixed;
   op::prec().fused_absorb = e.opt.fused_absorb;
-  auto s = left_rg(c, e, chi);
-  right_rg(c, e, chi);
-  top_rg(c, e, chi);
-  bottom_rg(c, e, chi);
+  const uint64_t d0 = e.draws;
+  std::vector<double> s;
+  if (e.opt.paired_moves) {
+    // left || right, then top || bottom (src/ctm/ctmrg.jl:57-64: right reads
+    // none of left's outputs, bottom none of top's)
+    run_pair(c, e, [&](Ctx& cc) { s = left_rg(cc, e, chi, d0 + 0); }, [&](Ctx& cc) { right_rg(cc, e, chi, d0 + 1); });
+    run_pair(c, e, [&](Ctx& cc) { top_rg(cc, e, chi, d0 + 2); }, [&](Ctx& cc) { bottom_rg(cc, e, chi, d0 + 3); });
+  } else {
+    s = left_rg(c, e, chi, d0 + 0);
+    right_rg(c, e, chi, d0 + 1);
+    top_rg(c, e, chi, d0 + 2);
+    bottom_rg(c, e, chi, d0 + 3);
+  }
+  e.draws = d0 + 4;
   return s;
 }
 

@@ -21,6 +21,7 @@ struct Options {
   k::Mixed energy_mixed = k::Mixed::BF16X6;
   bool checkpoint = false;     // recompute each step in the reverse sweep instead of keeping its tape
   bool fused_absorb = true;    // fused FP32-MFMA site absorption in the corners (mixed engine, 4 <= D <= 6)
+  bool paired_moves = true;    // left || right and top || bottom on two streams / host threads
 };
 
 // Plain ground-state environment (src/ctm/ctm_tensor.jl:3-18).  Storage dtype
@@ -32,9 +33,10 @@ struct Env {
   std::array<Tensor, 4> E;      // E1 (l, r, ket, bra), E2 (u, d, ., .), E3 (l, r, ., .), E4 (u, d, ., .)
   Options opt;
   // Randomized projectors draw their test matrices from a counter-based
-  // stream keyed by (opt.svd.seed, draws): two environments in the same
-  // state with the same seed draw the same matrices whatever their dtype,
-  // and a recomputed step replays its forward draws.
+  // stream keyed by (opt.svd.seed, draws + direction); rg_step advances
+  // draws by 4.  Two environments in the same state with the same seed draw
+  // the same matrices whatever their dtype or move schedule, and a
+  // recomputed step replays its forward draws.
   mutable uint64_t draws = 0;
 };
 

@@ -237,6 +237,7 @@ std::vector<double> nested_rg_step(Ctx& c, NestedEnv& e, int64_t chi) {
   move(c, e, 1, chi);
   move(c, e, 2, chi);
   move(c, e, 3, chi);
+  e.g.draws += 4;  // the projectors of the four moves used draws + 0..3
   return s;
 }
 

@@ -33,7 +33,7 @@ static bool needs(const Tensor& t) { return recording() && (bool)t.node; }
 std::shared_ptr<Node> make_node(std::function<void(Ctx&, Node&)> back) {
   auto n = std::make_shared<Node>();
   n->back = std::move(back);
-  current_tape()->order.push_back(n);
+  current_tape()->push(n);
   return n;
 }
 
@@ -41,7 +41,7 @@ Tensor leaf(Tensor t) {
   if (!recording()) return t;
   auto n = std::make_shared<Node>();
   n->leaf = true;
-  current_tape()->order.push_back(n);
+  current_tape()->push(n);
   t.node = n;
   return t;
 }

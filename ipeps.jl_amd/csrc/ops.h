@@ -10,6 +10,7 @@
 
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -27,6 +28,11 @@ struct Node {
 struct Tape {
   bool active = false;
   std::vector<std::shared_ptr<Node>> order;
+  std::mutex mu;  // two lanes of a CTMRG step record concurrently
+  void push(std::shared_ptr<Node> n) {
+    std::lock_guard<std::mutex> g(mu);
+    order.push_back(std::move(n));
+  }
   void clear() { order.clear(); }
 };
 

@@ -10,6 +10,7 @@ namespace ipeps {
 Pool::~Pool() { trim(); }
 
 void* Pool::alloc(size_t bytes) {
+  std::lock_guard<std::mutex> g(mu_);
   size_t b = round(bytes);
   auto it = free_.find(b);
   void* p = nullptr;
@@ -32,6 +33,7 @@ void* Pool::alloc(size_t bytes) {
 }
 
 void Pool::release(void* p, size_t bytes) {
+  std::lock_guard<std::mutex> g(mu_);
   size_t b = round(bytes);
   free_.emplace(b, p);
   live_ -= b;
@@ -79,6 +81,11 @@ void Ctx::resolve_timing() {
   kev.clear();
 }
 
+Ctx& Ctx::second_lane() {
+  if (!lane) lane.reset(ctx_create(device));
+  return *lane;
+}
+
 Ctx* ctx_create(int device) {
   int n = 0;
   HIP_CHECK(hipGetDeviceCount(&n));
@@ -108,6 +115,9 @@ Ctx* ctx_create(int device) {
 void ctx_destroy(Ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  if (c->lane) {
+    ctx_destroy(c->lane.release());
+  }
   (void)hipStreamSynchronize(c->stream);
   if (c->solver) rocblas_destroy_handle((rocblas_handle)c->solver);
   c->pool.reset();

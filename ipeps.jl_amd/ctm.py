@@ -115,7 +115,7 @@ def _hpair(H):
 ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "mixed_cutoff", "fast_orth",
                "fast_svd",
                "gram_guard", "canonical_edge", "mixed", "energy_fp64", "energy_mixed",
-               "checkpoint", "fused_absorb", "draws")
+               "checkpoint", "fused_absorb", "paired_moves", "draws")
 
 
 class CTMTensors:

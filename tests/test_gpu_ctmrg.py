@@ -462,3 +462,25 @@ def test_small_cholesky_route_matches_rocsolver(pkg):
     print("dE", abs(E1 - E2), "grad rel", np.abs(g1 - g2).max() / np.abs(g1).max())
     assert abs(E1 - E2) < 1e-10
     assert np.abs(g1 - g2).max() < 1e-6 * np.abs(g1).max()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "mixed"])
+def test_paired_moves_match_sequential(pkg, dtype):
+    """rg_step with left || right and top || bottom on two streams / threads
+    (paired_moves=1, the default) gives bit-identical environments, energy
+    and step gradient to the sequential schedule, randomized projector
+    included (draw index = position of the move in the step)."""
+    A = O.seeded_tensor((4, 4, 4, 4, 2), 3).numpy()
+    H = [h.numpy() for h in O.heisenberg(1.0)]
+    out = []
+    for pm in (0, 1):
+        ts = pkg.CTMTensors(A, dtype=dtype, options=dict(svd_mode=1, paired_moves=pm))
+        for _ in range(3):
+            pkg.rg_step(ts, 24)
+        E, g = pkg.step_adjoint(ts, 24, H)
+        out.append((E, g, ts.Cs, ts.Es))
+    (E0, g0, C0, E0s), (E1, g1, C1, E1s) = out
+    assert E0 == E1
+    assert np.array_equal(g0, g1)
+    for a, b in zip(C0 + E0s, C1 + E1s):
+        assert np.array_equal(a, b)
