@@ -286,14 +286,17 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
   }
 }
 
-// out[e] = sum_blk partial[blk][e], e < n (n = 2 U^2 P)
-__global__ __launch_bounds__(256) void absorb_reduce_kernel(int nblk, int n, const float* __restrict__ partial,
+// out[y][e] = sum of in[b][e] over b in [y F, min(nblk, (y+1) F)), e < n;
+// run twice (F = 16, then F = all) so that each thread adds at most 16 terms
+// in the first pass and the blocks of the first pass cover (n / 256) x (nblk / 16)
+__global__ __launch_bounds__(256) void absorb_reduce_kernel(int nblk, int n, int F, const float* __restrict__ in,
                                                             float* __restrict__ out) {
-  for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
-    float s = 0.f;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * n + e];
-    out[e] = s;
-  }
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n) return;
+  const int b0 = blockIdx.y * F, b1 = min(nblk, b0 + F);
+  float s = 0.f;
+  for (int b = b0; b < b1; ++b) s += in[(int64_t)b * n + e];
+  out[(int64_t)blockIdx.y * n + e] = s;
 }
 
 }  // namespace
@@ -331,8 +334,12 @@ void absorb_wgrad(Ctx& c, AbsorbWP p, float* Abar_Adbar) {
     else hipLaunchKernelGGL((absorb_wgrad_kernel<3>), dim3(grid), blk, 0, c.stream, p);
     HIP_CHECK(hipGetLastError());
   }
-  hipLaunchKernelGGL(absorb_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, grid, n, p.partial,
-                     Abar_Adbar);
+  const int F = 16, G = (grid + F - 1) / F;
+  std::shared_ptr<Buf> ws2(new Buf{c.pool->alloc((size_t)G * n * 4), (size_t)G * n * 4, c.pool.get()});
+  hipLaunchKernelGGL(absorb_reduce_kernel, dim3((n + 255) / 256, G), dim3(256), 0, c.stream, grid, n, F, p.partial,
+                     (float*)ws2->p);
+  hipLaunchKernelGGL(absorb_reduce_kernel, dim3((n + 255) / 256, 1), dim3(256), 0, c.stream, G, n, G,
+                     (const float*)ws2->p, Abar_Adbar);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -436,6 +436,20 @@ __global__ __launch_bounds__(256) void splitk_reduce(int64_t M, int64_t N, int64
   }
 }
 
+// W2[y][e] = sum of W[s][e] over s in [y F, min(S, (y+1) F)): folds many
+// split-K slices so that the final reduction has few terms per element and
+// the fold itself runs on S/F x MN/256 blocks
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_fold(int64_t MN, int64_t S, int64_t F, const T* __restrict__ W,
+                                                   T* __restrict__ W2) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= MN) return;
+  const int64_t s0 = (int64_t)blockIdx.y * F, s1 = min(S, s0 + F);
+  T acc = zero_of<T>();
+  for (int64_t s = s0; s < s1; ++s) acc = add_(acc, W[s * MN + e]);
+  W2[(int64_t)blockIdx.y * MN + e] = acc;
+}
+
 // ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
@@ -571,6 +585,25 @@ void gemm(Ctx& c, const GemmArgs& g) {
                                                                                           : "splitk_reduce<float>";
     TimedLaunch tr(c, rname, 0.0, (double)(S + 1) * MN * es);
     unsigned rb = (unsigned)std::min<int64_t>((MN + 255) / 256, 4096);
+    std::shared_ptr<Buf> ws2;
+    if (S > 32) {
+      const int64_t F = 16, G = (S + F - 1) / F;
+      const size_t w2bytes = (size_t)G * MN * es;
+      ws2.reset(new Buf{c.pool->alloc(w2bytes), w2bytes, c.pool.get()});
+      dim3 fgrid((unsigned)((MN + 255) / 256), (unsigned)G);
+      if (g.dt == DType::F64)
+        hipLaunchKernelGGL(splitk_fold<double>, fgrid, dim3(256), 0, c.stream, MN, S, F, (const double*)ws->p,
+                           (double*)ws2->p);
+      else if (g.dt == DType::C128)
+        hipLaunchKernelGGL(splitk_fold<double2>, fgrid, dim3(256), 0, c.stream, MN, S, F, (const double2*)ws->p,
+                           (double2*)ws2->p);
+      else
+        hipLaunchKernelGGL(splitk_fold<float>, fgrid, dim3(256), 0, c.stream, MN, S, F, (const float*)ws->p,
+                           (float*)ws2->p);
+      HIP_CHECK(hipGetLastError());
+      ws = ws2;
+      S = G;
+    }
     if (g.dt == DType::F64)
       hipLaunchKernelGGL(splitk_reduce<double>, dim3(rb), dim3(256), 0, c.stream, g.M, g.N, S, (const double*)ws->p,
                          g.alpha, g.beta, (double*)g.C, g.ldc);
