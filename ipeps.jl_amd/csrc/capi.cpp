@@ -633,6 +633,7 @@ int ipeps_batch_create(int device, int n, int dtype, const int64_t* dims, const 
     IPEPS_CHECK(workers >= 1 && workers <= 64, "workers must be 1..64");
     ctm::Options opt;
     opt.mixed = k::Mixed::BF16X3;
+    opt.paired_moves = false;  // the worker pool already runs instances concurrently
     parse_opts(opt, options);
     std::unique_ptr<ipeps_batch> b(new ipeps_batch());
     b->device = device;

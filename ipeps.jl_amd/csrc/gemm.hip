@@ -47,6 +47,27 @@ struct GemmP {
 
 constexpr int BM = 64, BN = 64;
 
+// XCD-aware block order.  Workgroups are dealt to the 8 XCDs round-robin in
+// dispatch order, each XCD with its own L2.  The hardware block number lin is
+// remapped so that XCD j works through the contiguous logical range
+// [j T/8, (j+1) T/8) with the M tile fastest: the blocks an XCD runs together
+// then share their N tile and K chunk, i.e. the same B panel stays in that
+// XCD's L2 while the A panels stream.  Used when T is a multiple of 8.
+struct BlockXYZ {
+  int64_t x, y, z;
+};
+__device__ inline BlockXYZ xcd_block() {
+  const int64_t gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  const int64_t T = gx * gy * gz;
+  int64_t lin = blockIdx.x + gx * (blockIdx.y + gy * (int64_t)blockIdx.z);
+  if (T % 8 == 0) lin = (lin % 8) * (T / 8) + lin / 8;
+  BlockXYZ b;
+  b.x = lin % gx;
+  b.y = (lin / gx) % gy;
+  b.z = lin / (gx * gy);
+  return b;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 real
 // ---------------------------------------------------------------------------
@@ -58,10 +79,11 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
-  const double* A = (const double*)p.A + (int64_t)blockIdx.z * p.sA;
-  const double* B = (const double*)p.B + (int64_t)blockIdx.z * p.sB;
-  double* C = (double*)p.C + (int64_t)blockIdx.z * p.sC;
+  const BlockXYZ bk = xcd_block();
+  const int64_t m0 = bk.x * BM, n0 = bk.y * BN;
+  const double* A = (const double*)p.A + bk.z * p.sA;
+  const double* B = (const double*)p.B + bk.z * p.sB;
+  double* C = (double*)p.C + bk.z * p.sC;
 
   v4f64 acc[2][2];
 #pragma unroll
@@ -70,7 +92,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
     for (int j = 0; j < 2; ++j) acc[i][j] = (v4f64){0.0, 0.0, 0.0, 0.0};
 
   // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
-  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - (int64_t)blockIdx.z * p.kchunk) : p.K;
+  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
   for (int64_t k0 = 0; k0 < Keff; k0 += BK) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -136,10 +158,11 @@ __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
-  const double2* A = (const double2*)p.A + (int64_t)blockIdx.z * p.sA;
-  const double2* B = (const double2*)p.B + (int64_t)blockIdx.z * p.sB;
-  double2* C = (double2*)p.C + (int64_t)blockIdx.z * p.sC;
+  const BlockXYZ bk = xcd_block();
+  const int64_t m0 = bk.x * BM, n0 = bk.y * BN;
+  const double2* A = (const double2*)p.A + bk.z * p.sA;
+  const double2* B = (const double2*)p.B + bk.z * p.sB;
+  double2* C = (double2*)p.C + bk.z * p.sC;
   const double sa = p.conjA ? -1.0 : 1.0, sb = p.conjB ? -1.0 : 1.0;
 
   v4f64 cr[2][2], ci[2][2];
@@ -152,7 +175,7 @@ __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
     }
 
   // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
-  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - (int64_t)blockIdx.z * p.kchunk) : p.K;
+  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
   for (int64_t k0 = 0; k0 < Keff; k0 += BK) {
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
@@ -306,12 +329,13 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * TBM, n0 = (int64_t)blockIdx.y * TBN;
-  const float* A = (const float*)p.A + (int64_t)blockIdx.z * p.sA;
-  const float* B = (const float*)p.B + (int64_t)blockIdx.z * p.sB;
-  float* C = (float*)p.C + (int64_t)blockIdx.z * p.sC;
+  const BlockXYZ bk = xcd_block();
+  const int64_t m0 = bk.x * TBM, n0 = bk.y * TBN;
+  const float* A = (const float*)p.A + bk.z * p.sA;
+  const float* B = (const float*)p.B + bk.z * p.sB;
+  float* C = (float*)p.C + bk.z * p.sC;
   // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
-  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - (int64_t)blockIdx.z * p.kchunk) : p.K;
+  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
   const bool vecA = p.vecA != 0, vecB = p.vecB != 0;
 
   int ar[IA], ag[IA], br[IB], bg[IB];
@@ -455,9 +479,10 @@ __global__ __launch_bounds__(256) void splitk_fold(int64_t MN, int64_t S, int64_
 // ---------------------------------------------------------------------------
 // FP32 block tile: 128 x 64 for outputs up to 64 columns; 64 x 192 for
 // 129..192 columns (one tile row spans the output, so the M x K operand is
-// streamed from HBM once); else 128 x 128.
+// streamed from HBM once; a 256-row tile measured slower on the skinny
+// chi-sized products: fewer, single-wave-per-SIMD blocks); else 128 x 128.
 static int f32_tile_n(int64_t N) { return N <= 64 ? 64 : (N > 128 && N <= 192) ? 192 : 128; }
-static int f32_tile_m(int64_t N) { return f32_tile_n(N) == 192 ? 64 : 128; }
+static int f32_tile_m(int64_t N, Mixed) { return f32_tile_n(N) == 192 ? 64 : 128; }
 
 static bool al16(const void* ptr) { return ((uintptr_t)ptr & 15u) == 0; }
 
@@ -490,17 +515,17 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid) {
     else if (!at && bt) hipLaunchKernelGGL((gemm_f32_kernel<false, true, NS, TM, TN>), grid, block, 0, c.stream, p); \
     else hipLaunchKernelGGL((gemm_f32_kernel<true, true, NS, TM, TN>), grid, block, 0, c.stream, p);          \
   } while (0)
-#define IPEPS_F32_LAUNCH(NS)                                          \
+#define IPEPS_F32_LAUNCH(NS, TM192)                                   \
   do {                                                                \
     const int tn = f32_tile_n(g.N);                                   \
     if (tn == 64) IPEPS_F32_LAUNCH_T(NS, 128, 64);                    \
-    else if (tn == 192) IPEPS_F32_LAUNCH_T(NS, 64, 192);              \
+    else if (tn == 192) IPEPS_F32_LAUNCH_T(NS, TM192, 192);           \
     else IPEPS_F32_LAUNCH_T(NS, 128, 128);                            \
   } while (0)
     switch (g.mixed) {
-      case Mixed::BF16: IPEPS_F32_LAUNCH(1); break;
-      case Mixed::BF16X3: IPEPS_F32_LAUNCH(2); break;
-      case Mixed::BF16X6: IPEPS_F32_LAUNCH(3); break;
+      case Mixed::BF16: IPEPS_F32_LAUNCH(1, 64); break;
+      case Mixed::BF16X3: IPEPS_F32_LAUNCH(2, 64); break;
+      case Mixed::BF16X6: IPEPS_F32_LAUNCH(3, 64); break;
     }
 #undef IPEPS_F32_LAUNCH
 #undef IPEPS_F32_LAUNCH_T
@@ -528,7 +553,7 @@ void gemm(Ctx& c, const GemmArgs& g) {
   p.C = g.C; p.ldc = g.ldc; p.sC = g.sC;
   p.conjA = g.conjA; p.conjB = g.conjB;
   p.kchunk = 0;
-  const int64_t tbm = g.dt == DType::F32 ? f32_tile_m(g.N) : BM, tbn = g.dt == DType::F32 ? f32_tile_n(g.N) : BN;
+  const int64_t tbm = g.dt == DType::F32 ? f32_tile_m(g.N, g.mixed) : BM, tbn = g.dt == DType::F32 ? f32_tile_n(g.N) : BN;
   set_vec_flags(g, p);
   dim3 grid((unsigned)((g.M + tbm - 1) / tbm), (unsigned)((g.N + tbn - 1) / tbn), (unsigned)g.batch);
   IPEPS_CHECK(grid.y <= 65535, "gemm: N too large for the grid");
