@@ -265,30 +265,44 @@ __device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((u
 //     four lanes read the 128 B of one row (two float4 each when aligned);
 //   row-contiguous operand (X[r + k ld]): item q -> r = q % TR, g = q / TR,
 //     each lane reads 8 scalars down k and consecutive lanes hit consecutive r.
+// The loads are unconditional, so a K step's loads stay in flight together:
+// the row index is clamped to the last row by the caller (rows past the end
+// only feed output rows / columns that are never written), and in the last,
+// partial K tile of a block k is clamped too and the values past the end are
+// zeroed at staging (zero8).
 template <bool KV>
-__device__ inline void load8(const float* __restrict__ X, int64_t ld, int64_t r, int64_t R, int64_t kb, int64_t Keff,
-                             bool vec, float (&v)[8]) {
-#pragma unroll
-  for (int e = 0; e < 8; ++e) v[e] = 0.f;
-  if (r >= R) return;
+__device__ inline void load8(const float* __restrict__ X, int64_t ld, int64_t rc, int64_t kb, bool vec,
+                             float (&v)[8]) {
   if (KV) {
-    const float* src = X + kb + r * ld;
-    if (vec && kb + 8 <= Keff) {
+    const float* src = X + kb + rc * ld;
+    if (vec) {
       const float4 x = *reinterpret_cast<const float4*>(src);
       const float4 y = *reinterpret_cast<const float4*>(src + 4);
       v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
       v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
     } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e)
-        if (kb + e < Keff) v[e] = src[e];
+      for (int e = 0; e < 8; ++e) v[e] = src[e];
     }
   } else {
-    const float* src = X + r + kb * ld;
+    const float* src = X + rc + kb * ld;
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
-      if (kb + e < Keff) v[e] = src[e * ld];
+    for (int e = 0; e < 8; ++e) v[e] = src[e * ld];
   }
+}
+template <bool KV>
+__device__ inline void load8_tail(const float* __restrict__ X, int64_t ld, int64_t rc, int64_t kb, int64_t Keff,
+                                  float (&v)[8]) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int64_t k = kb + e < Keff ? kb + e : Keff - 1;
+    v[e] = KV ? X[k + rc * ld] : X[rc + k * ld];
+  }
+}
+__device__ inline void zero8(float (&v)[8], int64_t kb, int64_t Keff) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e)
+    if (kb + e >= Keff) v[e] = 0.f;
 }
 
 // hi / mid / lo bf16 planes of 8 values, 16 B per plane (plane stride in elements)
@@ -339,30 +353,46 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
   const bool vecA = p.vecA != 0, vecB = p.vecB != 0;
 
   int ar[IA], ag[IA], br[IB], bg[IB];
+  int64_t arc[IA], brc[IB];  // clamped rows
 #pragma unroll
   for (int i = 0; i < IA; ++i) {
     const int q = tid + 256 * i;
     if (AT) { ag[i] = q & 3; ar[i] = q >> 2; }
     else { ar[i] = q % TBM; ag[i] = q / TBM; }
+    arc[i] = min(m0 + ar[i], p.M - 1);
   }
 #pragma unroll
   for (int i = 0; i < IB; ++i) {
     const int q = tid + 256 * i;
     if (!BT) { bg[i] = q & 3; br[i] = q >> 2; }
     else { br[i] = q % TBN; bg[i] = q / TBN; }
+    brc[i] = min(n0 + br[i], p.N - 1);
   }
   float ra[IA][8], rb[IB][8];
   auto fetch = [&](int64_t k0) {
 #pragma unroll
-    for (int i = 0; i < IA; ++i) load8<AT>(A, p.lda, m0 + ar[i], p.M, k0 + 8 * ag[i], Keff, vecA, ra[i]);
+    for (int i = 0; i < IA; ++i) load8<AT>(A, p.lda, arc[i], k0 + 8 * ag[i], vecA, ra[i]);
 #pragma unroll
-    for (int i = 0; i < IB; ++i) load8<!BT>(B, p.ldb, n0 + br[i], p.N, k0 + 8 * bg[i], Keff, vecB, rb[i]);
+    for (int i = 0; i < IB; ++i) load8<!BT>(B, p.ldb, brc[i], k0 + 8 * bg[i], vecB, rb[i]);
+  };
+  auto fetch_tail = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < IA; ++i) load8_tail<AT>(A, p.lda, arc[i], k0 + 8 * ag[i], Keff, ra[i]);
+#pragma unroll
+    for (int i = 0; i < IB; ++i) load8_tail<!BT>(B, p.ldb, brc[i], k0 + 8 * bg[i], Keff, rb[i]);
   };
   auto stage = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < IA; ++i) split_store<NSPLIT>(&As[buf][0][ar[i]][8 * ag[i]], PA, ra[i]);
 #pragma unroll
     for (int i = 0; i < IB; ++i) split_store<NSPLIT>(&Bs[buf][0][br[i]][8 * bg[i]], PB, rb[i]);
+  };
+  auto stage_tail = [&](int buf, int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < IA; ++i) zero8(ra[i], k0 + 8 * ag[i], Keff);
+#pragma unroll
+    for (int i = 0; i < IB; ++i) zero8(rb[i], k0 + 8 * bg[i], Keff);
+    stage(buf);
   };
 
   v4f32 acc[TI][TJ];
@@ -371,14 +401,24 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
 #pragma unroll
     for (int j = 0; j < TJ; ++j) acc[i][j] = (v4f32){0.f, 0.f, 0.f, 0.f};
 
-  const int64_t nk = (Keff + BK - 1) / BK;
-  fetch(0);
-  stage(0);
+  // K tiles [0, nfull) lie inside the block's K range; the last tile is
+  // partial when Keff is not a multiple of BK.
+  const int64_t nk = (Keff + BK - 1) / BK, nfull = Keff / BK;
+  if (nfull > 0) {
+    fetch(0);
+    stage(0);
+  } else {
+    fetch_tail(0);
+    stage_tail(0, 0);
+  }
   __syncthreads();
   const int kq = 8 * (lane >> 4), lr = lane & 15;
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int cur = (int)(kt & 1);
-    if (kt + 1 < nk) fetch((kt + 1) * BK);
+    const int64_t kn = (kt + 1) * BK;
+    const bool more = kt + 1 < nk, full = kt + 1 < nfull;
+    if (full) fetch(kn);
+    else if (more) fetch_tail(kn);
     v8bf16 a[NSPLIT][TI], b[NSPLIT][TJ];
 #pragma unroll
     for (int s = 0; s < NSPLIT; ++s) {
@@ -399,7 +439,8 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
 #pragma unroll
           for (int j = 0; j < TJ; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sa][i], b[t - sa][j], acc[i][j], 0, 0, 0);
-    if (kt + 1 < nk) stage(cur ^ 1);
+    if (full) stage(cur ^ 1);
+    else if (more) stage_tail(cur ^ 1, kn);
     __syncthreads();
   }
 
