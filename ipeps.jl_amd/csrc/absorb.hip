@@ -35,113 +35,155 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int ABS_UMAX = 36;  // U = D^2 <= 36 (D <= 6)
 constexpr int ABS_PMAX = 2;   // physical extent
-constexpr int ABS_LD = ABS_UMAX + 1;
-constexpr int ABS_RB = 16;    // rows per block (forward)
-constexpr int ABS_RBW = 8;    // rows per block (weight gradient)
-constexpr int ABS_NW = 8;     // waves per block (two per SIMD)
+constexpr int ABS_RB = 16;    // rows per chunk (forward)
+constexpr int ABS_RBW = 8;    // rows per chunk (weight gradient)
+constexpr int ABS_NW = 4;     // waves per block (one per SIMD: 512 registers for accumulators and prefetch)
 constexpr int ABS_THREADS = 64 * ABS_NW;
-
-__device__ inline int64_t off2(int i, int D, int64_t s1, int64_t s2) { return (int64_t)(i % D) * s1 + (int64_t)(i / D) * s2; }
 
 __device__ inline v4f mfma4(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
-// stage RB rows (r1 = r10 .. r10+RB-1, fixed r2) of a strided (r, x, y) tensor
-// into LDS as dst[rr][x][y] with row stride U * ABS_LD; rows past n1 are zero
-template <int RB>
-__device__ inline void stage_rows(const float* __restrict__ src, const AbsorbLegs& g, int U, int D, int r10, int r2,
-                                  float* dst) {
-  const int total = U * U * RB;
-  for (int e = threadIdx.x; e < total; e += ABS_THREADS) {
-    const int rr = e % RB, xy = e / RB;
-    const int x = xy % U, y = xy / U;
-    const int r1 = r10 + rr;
-    float v = 0.f;
-    if (r1 < g.n1)
-      v = src[(int64_t)r1 * g.s1 + (int64_t)r2 * g.s2 + off2(x, D, g.sx1, g.sx2) + off2(y, D, g.sy1, g.sy2)];
-    dst[rr * U * ABS_LD + x * ABS_LD + y] = v;
-  }
+// Element offsets fit in 32 bits (the planner admits tensors of < 2^31
+// elements), and D is a compile-time constant, so the index math is a few
+// multiply-shifts.
+template <int D>
+__device__ inline int off2(int i, int s1, int s2) { return (i % D) * s1 + (i / D) * s2; }
+
+struct Legs32 {
+  int n1, n2, s1, s2, sx1, sx2, sy1, sy2;
+};
+__device__ inline Legs32 legs32(const AbsorbLegs& g) {
+  return Legs32{g.n1, g.n2, (int)g.s1, (int)g.s2, (int)g.sx1, (int)g.sx2, (int)g.sy1, (int)g.sy2};
 }
 
-template <int NT>
+// RB rows (r1 = r10 .. r10+RB-1 of one r2) of a strided (r, x, y) tensor,
+// held in registers between the fetch and the LDS store dst[rr][x][y] (row
+// stride U * (U + 1)).  Element e of the chunk is (rr = e % RB, x, y); lanes
+// with consecutive e read consecutive rows (the stride-1 leg), and every load
+// is unconditional: rows past n1 read row n1 - 1 (their results are never
+// written) and the elements past the chunk end re-read its last element.
+template <int D, int RB>
+struct RowChunk {
+  static constexpr int U = D * D, LD = U + 1, TOT = U * U * RB;
+  static constexpr int NE = (TOT + ABS_THREADS - 1) / ABS_THREADS;
+  float v[NE];
+  __device__ inline void fetch(const float* __restrict__ src, const Legs32& g, int r10, int r2) {
+    const float* base = src + r2 * g.s2;
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = min((int)threadIdx.x + j * ABS_THREADS, TOT - 1);
+      const int rr = e % RB, xy = e / RB;
+      const int r1 = min(r10 + rr, g.n1 - 1);
+      v[j] = base[r1 * g.s1 + off2<D>(xy % U, g.sx1, g.sx2) + off2<D>(xy / U, g.sy1, g.sy2)];
+    }
+  }
+  __device__ inline void store(float* __restrict__ dst) const {
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int e = (int)threadIdx.x + j * ABS_THREADS;
+      if (e < TOT) {
+        const int rr = e % RB, xy = e / RB;
+        dst[rr * U * LD + (xy % U) * LD + xy / U] = v[j];
+      }
+    }
+  }
+};
+
+// Persistent blocks walk the row chunks (RB rows of one r2) with stride
+// gridDim.x; the next chunk's loads are in flight while the MFMAs of the
+// current one run.
+template <int D>
 __global__ __launch_bounds__(ABS_THREADS) void absorb_fwd_kernel(AbsorbP p) {
-  __shared__ float xs[ABS_RB * ABS_UMAX * ABS_LD];
-  __shared__ float am[ABS_UMAX * ABS_UMAX * ABS_PMAX], adm[ABS_UMAX * ABS_UMAX * ABS_PMAX];
-  const int U = p.U, P = p.P, D = p.D;
+  constexpr int U = D * D, LD = U + 1, NT = (U + 15) / 16;
+  __shared__ float xs[ABS_RB * U * LD];
+  __shared__ float am[U * U * ABS_PMAX], adm[U * U * ABS_PMAX];
+  const int P = p.P;
+  const Legs32 gx = legs32(p.x), gy = legs32(p.y);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  const int nb1 = (p.x.n1 + ABS_RB - 1) / ABS_RB;
-  const int r2 = blockIdx.x / nb1, r10 = (blockIdx.x % nb1) * ABS_RB;
+  const int nb1 = (gx.n1 + ABS_RB - 1) / ABS_RB, nchunk = nb1 * gx.n2;
   for (int e = threadIdx.x; e < U * U * P; e += ABS_THREADS) {
     am[e] = p.Am[e];
     adm[e] = p.Adm[e];
   }
-  stage_rows<ABS_RB>(p.X, p.x, U, D, r10, r2, xs);
-  __syncthreads();
-  for (int rr = wave; rr < ABS_RB; rr += ABS_NW) {
-    const int r1 = r10 + rr;
-    if (r1 >= p.x.n1) break;
-    const float* xr = xs + rr * U * ABS_LD;
-    v4f yacc[NT][NT];  // [tb][ta]: Y^T[b][a]
-#pragma unroll
-    for (int i = 0; i < NT; ++i)
-#pragma unroll
-      for (int j = 0; j < NT; ++j) yacc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
-    for (int pp = 0; pp < P; ++pp) {
-      v4f tacc[NT][NT];  // [tu][tb]: T[u][b] = (X_r Ad_p)[u][b]
+  RowChunk<D, ABS_RB> rc;
+  int ch = blockIdx.x;
+  if (ch < nchunk) rc.fetch(p.X, gx, (ch % nb1) * ABS_RB, ch / nb1);
+  for (; ch < nchunk; ch += gridDim.x) {
+    const int r2 = ch / nb1, r10 = (ch % nb1) * ABS_RB;
+    __syncthreads();  // the previous chunk's rows are consumed
+    rc.store(xs);
+    __syncthreads();
+    const int nx = ch + gridDim.x;
+    if (nx < nchunk) rc.fetch(p.X, gx, (nx % nb1) * ABS_RB, nx / nb1);
+    for (int rr = wave; rr < ABS_RB; rr += ABS_NW) {
+      const int r1 = r10 + rr;
+      if (r1 >= gx.n1) break;
+      const float* xr = xs + rr * U * LD;
+      v4f yacc[NT][NT];  // [tb][ta]: Y^T[b][a]
 #pragma unroll
       for (int i = 0; i < NT; ++i)
 #pragma unroll
-        for (int j = 0; j < NT; ++j) tacc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
-      for (int ks = 0; ks < (U + 3) / 4; ++ks) {
-        const int v = 4 * ks + lg;
-        float ao[NT], bo[NT];
+        for (int j = 0; j < NT; ++j) yacc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+      for (int pp = 0; pp < P; ++pp) {
+        v4f tacc[NT][NT];  // [tu][tb]: T[u][b] = (X_r Ad_p)[u][b]
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const int u = 16 * t + lr, b = 16 * t + lr;
-          ao[t] = (u < U && v < U) ? xr[u * ABS_LD + v] : 0.f;
-          bo[t] = (v < U && b < U) ? adm[v + U * (b + U * pp)] : 0.f;
+        for (int i = 0; i < NT; ++i)
+#pragma unroll
+          for (int j = 0; j < NT; ++j) tacc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < (U + 3) / 4; ++ks) {
+          const int v = 4 * ks + lg;
+          float ao[NT], bo[NT];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const int u = 16 * t + lr, b = 16 * t + lr;
+            ao[t] = (u < U && v < U) ? xr[u * LD + v] : 0.f;
+            bo[t] = (v < U && b < U) ? adm[v + U * (b + U * pp)] : 0.f;
+          }
+#pragma unroll
+          for (int tu = 0; tu < NT; ++tu)
+#pragma unroll
+            for (int tb = 0; tb < NT; ++tb) tacc[tu][tb] = mfma4(ao[tu], bo[tb], tacc[tu][tb]);
         }
 #pragma unroll
         for (int tu = 0; tu < NT; ++tu)
 #pragma unroll
-          for (int tb = 0; tb < NT; ++tb) tacc[tu][tb] = mfma4(ao[tu], bo[tb], tacc[tu][tb]);
-      }
+          for (int s = 0; s < 4; ++s) {
+            const int u = 16 * tu + 4 * lg + s;
+            float bo[NT];
 #pragma unroll
-      for (int tu = 0; tu < NT; ++tu)
+            for (int ta = 0; ta < NT; ++ta) {
+              const int a = 16 * ta + lr;
+              bo[ta] = (u < U && a < U) ? am[u + U * (a + U * pp)] : 0.f;
+            }
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int u = 16 * tu + 4 * lg + s;
-          float bo[NT];
+            for (int tb = 0; tb < NT; ++tb)
 #pragma unroll
-          for (int ta = 0; ta < NT; ++ta) {
-            const int a = 16 * ta + lr;
-            bo[ta] = (u < U && a < U) ? am[u + U * (a + U * pp)] : 0.f;
+              for (int ta = 0; ta < NT; ++ta) yacc[tb][ta] = mfma4(tacc[tu][tb][s], bo[ta], yacc[tb][ta]);
           }
+      }
+      float* yrow = p.Y + r1 * gy.s1 + r2 * gy.s2;
 #pragma unroll
-          for (int tb = 0; tb < NT; ++tb)
+      for (int tb = 0; tb < NT; ++tb)
 #pragma unroll
-            for (int ta = 0; ta < NT; ++ta) yacc[tb][ta] = mfma4(tacc[tu][tb][s], bo[ta], yacc[tb][ta]);
-        }
+        for (int ta = 0; ta < NT; ++ta)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int b = 16 * tb + 4 * lg + i, a = 16 * ta + lr;
+            if (a < U && b < U) yrow[off2<D>(a, gy.sx1, gy.sx2) + off2<D>(b, gy.sy1, gy.sy2)] = yacc[tb][ta][i];
+          }
     }
-    const int64_t rowoff = (int64_t)r1 * p.y.s1 + (int64_t)r2 * p.y.s2;
-#pragma unroll
-    for (int tb = 0; tb < NT; ++tb)
-#pragma unroll
-      for (int ta = 0; ta < NT; ++ta)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int b = 16 * tb + 4 * lg + i, a = 16 * ta + lr;
-          if (a < U && b < U) p.Y[rowoff + off2(a, D, p.y.sx1, p.y.sx2) + off2(b, D, p.y.sy1, p.y.sy2)] = yacc[tb][ta][i];
-        }
   }
 }
 
-template <int NT>
+template <int D>
 __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
-  __shared__ float xs[ABS_RBW * ABS_UMAX * ABS_LD];
-  __shared__ float gs[ABS_RBW * ABS_UMAX * ABS_LD];
-  __shared__ float am[ABS_UMAX * ABS_UMAX * ABS_PMAX], adm[ABS_UMAX * ABS_UMAX * ABS_PMAX];
-  const int U = p.U, P = p.P, D = p.D;
+  constexpr int U = D * D, LD = U + 1, NT = (U + 15) / 16;
+  __shared__ float xs[ABS_RBW * U * LD];
+  __shared__ float gs[ABS_RBW * U * LD];
+  __shared__ float am[U * U * ABS_PMAX], adm[U * U * ABS_PMAX];
+  const int P = p.P;
+  const Legs32 gx = legs32(p.x), gg = legs32(p.g);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lg = lane >> 4;
   for (int e = threadIdx.x; e < U * U * P; e += ABS_THREADS) {
@@ -159,18 +201,29 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
         abar[q][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
         adbar[q][i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
       }
-  const int nb1 = (p.x.n1 + ABS_RBW - 1) / ABS_RBW;
-  const int nblk = nb1 * p.x.n2;
-  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-    const int r2 = blk / nb1, r10 = (blk % nb1) * ABS_RBW;
+  const int nb1 = (gx.n1 + ABS_RBW - 1) / ABS_RBW;
+  const int nblk = nb1 * gx.n2;
+  RowChunk<D, ABS_RBW> cx, cg;
+  int blk = blockIdx.x;
+  if (blk < nblk) {
+    cx.fetch(p.X, gx, (blk % nb1) * ABS_RBW, blk / nb1);
+    cg.fetch(p.G, gg, (blk % nb1) * ABS_RBW, blk / nb1);
+  }
+  for (; blk < nblk; blk += gridDim.x) {
+    const int r10 = (blk % nb1) * ABS_RBW;
     __syncthreads();  // previous rows fully consumed
-    stage_rows<ABS_RBW>(p.X, p.x, U, D, r10, r2, xs);
-    stage_rows<ABS_RBW>(p.G, p.g, U, D, r10, r2, gs);
+    cx.store(xs);
+    cg.store(gs);
     __syncthreads();
+    const int nx = blk + gridDim.x;
+    if (nx < nblk) {
+      cx.fetch(p.X, gx, (nx % nb1) * ABS_RBW, nx / nb1);
+      cg.fetch(p.G, gg, (nx % nb1) * ABS_RBW, nx / nb1);
+    }
     for (int rr = wave; rr < ABS_RBW; rr += ABS_NW) {
-      if (r10 + rr >= p.x.n1) break;
-      const float* xr = xs + rr * U * ABS_LD;  // X_r[u][v]
-      const float* gr = gs + rr * U * ABS_LD;  // G_r[a][b]
+      if (r10 + rr >= gx.n1) break;
+      const float* xr = xs + rr * U * LD;  // X_r[u][v]
+      const float* gr = gs + rr * U * LD;  // G_r[a][b]
 #pragma unroll
       for (int pp = 0; pp < ABS_PMAX; ++pp) {
         if (pp >= P) break;
@@ -180,6 +233,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
         for (int i = 0; i < NT; ++i)
 #pragma unroll
           for (int j = 0; j < NT; ++j) tt[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
         for (int ks = 0; ks < (U + 3) / 4; ++ks) {
           const int v = 4 * ks + lg;
           float ao[NT], bo[NT];
@@ -187,7 +241,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
           for (int t = 0; t < NT; ++t) {
             const int b = 16 * t + lr, u = 16 * t + lr;
             ao[t] = (v < U && b < U) ? adm[v + U * (b + U * pp)] : 0.f;
-            bo[t] = (v < U && u < U) ? xr[u * ABS_LD + v] : 0.f;
+            bo[t] = (v < U && u < U) ? xr[u * LD + v] : 0.f;
           }
 #pragma unroll
           for (int tb = 0; tb < NT; ++tb)
@@ -204,7 +258,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
 #pragma unroll
             for (int ta = 0; ta < NT; ++ta) {
               const int a = 16 * ta + lr;
-              ao[ta] = (a < U && b < U) ? gr[a * ABS_LD + b] : 0.f;
+              ao[ta] = (a < U && b < U) ? gr[a * LD + b] : 0.f;
             }
 #pragma unroll
             for (int ta = 0; ta < NT; ++ta)
@@ -217,6 +271,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
         for (int i = 0; i < NT; ++i)
 #pragma unroll
           for (int j = 0; j < NT; ++j) sa[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
         for (int ks = 0; ks < (U + 3) / 4; ++ks) {
           const int u = 4 * ks + lg;
           float ao[NT], bo[NT];
@@ -224,7 +279,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
           for (int t = 0; t < NT; ++t) {
             const int a = 16 * t + lr, v = 16 * t + lr;
             ao[t] = (u < U && a < U) ? am[u + U * (a + U * pp)] : 0.f;
-            bo[t] = (u < U && v < U) ? xr[u * ABS_LD + v] : 0.f;
+            bo[t] = (u < U && v < U) ? xr[u * LD + v] : 0.f;
           }
 #pragma unroll
           for (int ta = 0; ta < NT; ++ta)
@@ -241,7 +296,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
 #pragma unroll
             for (int tb = 0; tb < NT; ++tb) {
               const int b = 16 * tb + lr;
-              ao[tb] = (a < U && b < U) ? gr[a * ABS_LD + b] : 0.f;
+              ao[tb] = (a < U && b < U) ? gr[a * LD + b] : 0.f;
             }
 #pragma unroll
             for (int tb = 0; tb < NT; ++tb)
@@ -252,7 +307,8 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
     }
   }
   // sum the waves in LDS, one physical index at a time (xs and gs each hold
-  // 8 * 36 * 37 >= ABS_NW * U * U floats)
+  // RBW * U * (U + 1) >= ABS_NW * U * U floats)
+  static_assert(ABS_RBW * LD >= ABS_NW * U, "wave reduction buffer");
   float* red = xs;
 #pragma unroll
   for (int pp = 0; pp < ABS_PMAX; ++pp) {
@@ -308,12 +364,15 @@ void absorb_fwd(Ctx& c, const AbsorbP& p) {
   const int nb1 = (p.x.n1 + ABS_RB - 1) / ABS_RB;
   const int64_t nb = (int64_t)nb1 * p.x.n2;
   IPEPS_CHECK(nb >= 1 && nb < (1ll << 31), "absorb: grid");
+  // one block per CU (LDS-bound), each walking an equal share of the chunks
+  const int64_t per = (nb + 255) / 256;
+  const unsigned grid = (unsigned)((nb + per - 1) / per);
   const double rows = (double)p.x.n1 * p.x.n2;
   TimedLaunch tl(c, "absorb_fwd_kernel", rows * 4.0 * p.U * p.U * p.U * p.P, rows * 8.0 * p.U * p.U);
   const dim3 blk(ABS_THREADS);
-  if (p.U <= 16) hipLaunchKernelGGL((absorb_fwd_kernel<1>), dim3((unsigned)nb), blk, 0, c.stream, p);
-  else if (p.U <= 32) hipLaunchKernelGGL((absorb_fwd_kernel<2>), dim3((unsigned)nb), blk, 0, c.stream, p);
-  else hipLaunchKernelGGL((absorb_fwd_kernel<3>), dim3((unsigned)nb), blk, 0, c.stream, p);
+  if (p.D == 4) hipLaunchKernelGGL((absorb_fwd_kernel<4>), dim3(grid), blk, 0, c.stream, p);
+  else if (p.D == 5) hipLaunchKernelGGL((absorb_fwd_kernel<5>), dim3(grid), blk, 0, c.stream, p);
+  else hipLaunchKernelGGL((absorb_fwd_kernel<6>), dim3(grid), blk, 0, c.stream, p);
   HIP_CHECK(hipGetLastError());
 }
 
@@ -321,7 +380,7 @@ void absorb_wgrad(Ctx& c, AbsorbWP p, float* Abar_Adbar) {
   IPEPS_CHECK(absorb_supported(p.D, p.P) && p.U == p.D * p.D, "absorb: unsupported D / d");
   const int nb1 = (p.x.n1 + ABS_RBW - 1) / ABS_RBW;
   const int64_t nblk = (int64_t)nb1 * p.x.n2;
-  const int grid = (int)std::min<int64_t>(nblk, 512);
+  const int grid = (int)std::min<int64_t>(nblk, 256);  // one persistent block per CU
   const int n = 2 * p.U * p.U * p.P;
   std::shared_ptr<Buf> ws(new Buf{c.pool->alloc((size_t)grid * n * 4), (size_t)grid * n * 4, c.pool.get()});
   p.partial = (float*)ws->p;
@@ -329,9 +388,9 @@ void absorb_wgrad(Ctx& c, AbsorbWP p, float* Abar_Adbar) {
   {
     TimedLaunch tl(c, "absorb_wgrad_kernel", rows * 8.0 * p.U * p.U * p.U * p.P, rows * 8.0 * p.U * p.U);
     const dim3 blk(ABS_THREADS);
-    if (p.U <= 16) hipLaunchKernelGGL((absorb_wgrad_kernel<1>), dim3(grid), blk, 0, c.stream, p);
-    else if (p.U <= 32) hipLaunchKernelGGL((absorb_wgrad_kernel<2>), dim3(grid), blk, 0, c.stream, p);
-    else hipLaunchKernelGGL((absorb_wgrad_kernel<3>), dim3(grid), blk, 0, c.stream, p);
+    if (p.D == 4) hipLaunchKernelGGL((absorb_wgrad_kernel<4>), dim3(grid), blk, 0, c.stream, p);
+    else if (p.D == 5) hipLaunchKernelGGL((absorb_wgrad_kernel<5>), dim3(grid), blk, 0, c.stream, p);
+    else hipLaunchKernelGGL((absorb_wgrad_kernel<6>), dim3(grid), blk, 0, c.stream, p);
     HIP_CHECK(hipGetLastError());
   }
   const int F = 16, G = (grid + F - 1) / F;
