@@ -76,6 +76,8 @@ int ipeps_env_destroy(ipeps_env* env);
  * "energy_fp64" (0/1: energy contractions of a MIXED environment in FP64),
  * "energy_mixed" (1, 3 or 6: bf16 products of those contractions when not
  * FP64, default 6; the density-matrix traces and ratios are FP64 always),
+ * "energy_mixed_back" (0, 1, 3 or 6: bf16 products of the VJP of those
+ * contractions; default 0 = the "mixed" setting of the step),
  * "checkpoint" (0/1), "paired_moves" (0/1, default 1: the left and right
  * moves, then the top and bottom moves, of rg_step run concurrently on two
  * streams; results are identical either way), "fused_absorb" (0/1, default 1: MIXED environments

@@ -122,6 +122,10 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
     int m = (int)v;
     IPEPS_CHECK(m == 1 || m == 3 || m == 6, "energy_mixed must be 1, 3 or 6");
     o.energy_mixed = (k::Mixed)m;
+  } else if (key == "energy_mixed_back") {
+    int m = (int)v;
+    IPEPS_CHECK(m == 0 || m == 1 || m == 3 || m == 6, "energy_mixed_back must be 0, 1, 3 or 6");
+    o.energy_mixed_back = m;
   } else if (key == "mixed") {
     int m = (int)v;
     IPEPS_CHECK(m == 1 || m == 3 || m == 6, "mixed must be 1, 3 or 6");
@@ -443,6 +447,7 @@ int ipeps_absorb(ipeps_ctx* ctx, int dtype, int fused, const int64_t* dimsX, con
     op::Prec saved = op::prec();
     op::prec().fused_absorb = fused != 0;
     op::prec().mixed = k::Mixed::BF16X3;
+    op::prec().back = k::Mixed::BF16X3;
     Tape tape;
     tape.active = G != nullptr;
     TapeScope ts(G ? &tape : nullptr);

@@ -350,6 +350,7 @@ void run_pair(Ctx& c, Env& e, F&& f, G&& g) {
 
 std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi) {
   op::prec().mixed = e.opt.mixed;
+  op::prec().back = e.opt.mixed;
   op::prec().fused_absorb = e.opt.fused_absorb;
   const uint64_t d0 = e.draws;
   std::vector<double> s;
@@ -383,7 +384,8 @@ static Env as_dtype(Ctx& c, const Env& e, DType dt) {
 
 Tensor energy(Ctx& c, const Env& e0, const Tensor& H1, const Tensor& H2) {
   Env e = (e0.dt == DType::F32 && e0.opt.energy_fp64) ? as_dtype(c, e0, DType::F64) : e0;
-  op::PrecScope ps(e.opt.energy_mixed);
+  // the VJP of the contractions at energy_mixed_back (0: the step's setting)
+  op::PrecScope ps(e.opt.energy_mixed, e.opt.energy_mixed_back == 0 ? e.opt.mixed : (k::Mixed)e.opt.energy_mixed_back);
   Tensor roh, rov;
   get_dms(c, e, roh, rov);
   if (roh.dt == DType::F32) {

@@ -19,6 +19,7 @@ struct Options {
   // with `energy_mixed` products; the trace and the ratios are FP64 either way.
   bool energy_fp64 = false;
   k::Mixed energy_mixed = k::Mixed::BF16X6;
+  int energy_mixed_back = 0;   // products of the energy's VJP: 1, 3, 6, or 0 = `mixed`
   bool checkpoint = false;     // recompute each step in the reverse sweep instead of keeping its tape
   bool fused_absorb = true;    // fused FP32-MFMA site absorption in the corners (mixed engine, 4 <= D <= 6)
   bool paired_moves = true;    // left || right and top || bottom on two streams / host threads

@@ -232,6 +232,7 @@ NT single_dm(Ctx& c, const NestedEnv& e) {
 
 std::vector<double> nested_rg_step(Ctx& c, NestedEnv& e, int64_t chi) {
   op::prec().mixed = e.g.opt.mixed;
+  op::prec().back = e.g.opt.mixed;
   IPEPS_CHECK(e.g.dt == DType::C128 && e.B.dt == DType::C128, "nested CTMRG runs in complex FP64");
   auto s = move(c, e, 0, chi);
   move(c, e, 1, chi);

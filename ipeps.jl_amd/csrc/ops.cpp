@@ -416,7 +416,7 @@ Tensor tcon(Ctx& c, const Tensor& A, const std::vector<int>& la, const Tensor& B
         la_b[p] = -(q + 1);
       }
     bool gA = needs(A), gB = needs(B);
-    const k::Mixed mx = prec().mixed;
+    const k::Mixed mx = prec().back;
     C.node = make_node([=](Ctx& c, Node& n) {
       PrecScope ps(mx);
       if (gA) acc(c, nA, tcon_raw(c, n.grad, lc_a, false, Bk, lb_a, true));
@@ -1132,7 +1132,7 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     // kept and computed-but-discarded triplets are exact and only triplets
     // beyond k are treated as zero.
     Tensor Ub = Uf, Vb = Vf, Sb = Sf;
-    const k::Mixed mx = prec().mixed;
+    const k::Mixed mx = prec().back;
     base->back = [=](Ctx& c, Node& b) {
       PrecScope ps(mx);
       // svd_back (src/utility/autodiff.jl:41-75) for U (m x kb), V (nc x kb):

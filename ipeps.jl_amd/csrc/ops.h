@@ -49,16 +49,24 @@ namespace op {
 
 // Precision knobs of the contraction engine.
 struct Prec {
-  k::Mixed mixed = k::Mixed::BF16X3;
+  k::Mixed mixed = k::Mixed::BF16X3;  // products of the forward ops
+  k::Mixed back = k::Mixed::BF16X3;   // products of the VJPs recorded meanwhile
   bool fused_absorb = true;  // op::absorb may use the fused FP32-MFMA kernel
 };
 Prec& prec();
-// Sets prec().mixed for a scope (the VJPs restore the setting their forward
-// op ran with).
+// Sets prec().mixed and prec().back for a scope; each VJP runs with the
+// `back` setting in force when its forward op was recorded.
 struct PrecScope {
-  k::Mixed prev;
-  explicit PrecScope(k::Mixed m) : prev(prec().mixed) { prec().mixed = m; }
-  ~PrecScope() { prec().mixed = prev; }
+  k::Mixed prev, prev_back;
+  explicit PrecScope(k::Mixed m) : PrecScope(m, m) {}
+  PrecScope(k::Mixed m, k::Mixed b) : prev(prec().mixed), prev_back(prec().back) {
+    prec().mixed = m;
+    prec().back = b;
+  }
+  ~PrecScope() {
+    prec().mixed = prev;
+    prec().back = prev_back;
+  }
 };
 
 bool recording();

@@ -114,7 +114,7 @@ def _hpair(H):
 
 ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "mixed_cutoff", "fast_orth",
                "fast_svd",
-               "gram_guard", "canonical_edge", "mixed", "energy_fp64", "energy_mixed",
+               "gram_guard", "canonical_edge", "mixed", "energy_fp64", "energy_mixed", "energy_mixed_back",
                "checkpoint", "fused_absorb", "paired_moves", "draws")
 
 
