@@ -69,13 +69,61 @@ __device__ inline BlockXYZ xcd_block() {
 }
 
 // ---------------------------------------------------------------------------
-// FP64 real
+// FP64 real and complex FP64: 64 x 64 block tile, 4 waves in a 2 x 2 grid,
+// each wave a 32 x 32 block of 16 x 16 f64 MFMA outputs.  The operand tiles
+// are fetched into registers one K step ahead with unconditional loads (row
+// indices clamped: rows past the end only feed outputs that are never
+// written; in the last, partial K tile k is clamped too and the values past
+// Keff are zeroed at staging), then written to the other of two LDS buffers
+// while the MFMAs of the current one run: one barrier per K step.
 // ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T zero_t();
+template <>
+__device__ inline double zero_t<double>() { return 0.0; }
+template <>
+__device__ inline double2 zero_t<double2>() { return make_double2(0.0, 0.0); }
+
+// Staging coordinates of one operand tile of TR rows (m or n) x BK k's:
+// item r of a thread is (row[r], kk[r]); `kfast` when k is the contiguous
+// index of the operand in memory.
+template <typename T, int TR, int BK>
+struct TileLoader {
+  static constexpr int NI = TR * BK / 256;
+  int row[NI], kk[NI];
+  const T* base[NI];  // element (clamped row, k = 0)
+  int64_t ks;         // element stride along k
+  T v[NI];
+  __device__ inline void init(const T* X, int64_t ld, bool kfast, int64_t r0, int64_t R, int tid) {
+    ks = kfast ? 1 : ld;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      if (kfast) { kk[i] = tid % BK; row[i] = tid / BK + (256 / BK) * i; }
+      else { row[i] = tid % TR; kk[i] = tid / TR + (256 / TR) * i; }
+      const int64_t r = min(r0 + row[i], R - 1);
+      base[i] = X + (kfast ? r * ld : r);
+    }
+  }
+  __device__ inline void fetch(int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) v[i] = base[i][(k0 + kk[i]) * ks];
+  }
+  __device__ inline void fetch_tail(int64_t k0, int64_t Keff) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) v[i] = base[i][min(k0 + kk[i], Keff - 1) * ks];
+  }
+  __device__ inline void zero_tail(int64_t k0, int64_t Keff) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+      if (k0 + kk[i] >= Keff) v[i] = zero_t<T>();
+  }
+};
+
 template <bool AT, bool BT>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
   constexpr int BK = 16;
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN + 1];
+  __shared__ double As[2][BK][BM + 1];
+  __shared__ double Bs[2][BK][BN + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -84,6 +132,19 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
   const double* A = (const double*)p.A + bk.z * p.sA;
   const double* B = (const double*)p.B + bk.z * p.sB;
   double* C = (double*)p.C + bk.z * p.sC;
+  // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
+  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
+
+  TileLoader<double, BM, BK> la;
+  TileLoader<double, BN, BK> lb;
+  la.init(A, p.lda, AT, m0, p.M, tid);
+  lb.init(B, p.ldb, !BT, n0, p.N, tid);
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < la.NI; ++i) As[buf][la.kk[i]][la.row[i]] = la.v[i];
+#pragma unroll
+    for (int i = 0; i < lb.NI; ++i) Bs[buf][lb.kk[i]][lb.row[i]] = lb.v[i];
+  };
 
   v4f64 acc[2][2];
 #pragma unroll
@@ -91,42 +152,47 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = (v4f64){0.0, 0.0, 0.0, 0.0};
 
-  // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
-  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
-  for (int64_t k0 = 0; k0 < Keff; k0 += BK) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int m, kk;
-      if (!AT) { m = tid & 63; kk = (tid >> 6) + 4 * r; }
-      else { kk = tid & 15; m = (tid >> 4) + 16 * r; }
-      int64_t gm = m0 + m, gk = k0 + kk;
-      double v = 0.0;
-      if (gm < p.M && gk < Keff) v = AT ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
-      As[kk][m] = v;
+  const int64_t nk = (Keff + BK - 1) / BK, nfull = Keff / BK;
+  if (nfull > 0) {
+    la.fetch(0);
+    lb.fetch(0);
+  } else {
+    la.fetch_tail(0, Keff);
+    lb.fetch_tail(0, Keff);
+    la.zero_tail(0, Keff);
+    lb.zero_tail(0, Keff);
+  }
+  stage(0);
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int cur = (int)(kt & 1);
+    const int64_t kn = (kt + 1) * BK;
+    const bool more = kt + 1 < nk, full = kt + 1 < nfull;
+    if (full) {
+      la.fetch(kn);
+      lb.fetch(kn);
+    } else if (more) {
+      la.fetch_tail(kn, Keff);
+      lb.fetch_tail(kn, Keff);
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int n, kk;
-      if (!BT) { kk = tid & 15; n = (tid >> 4) + 16 * r; }
-      else { n = tid & 63; kk = (tid >> 6) + 4 * r; }
-      int64_t gn = n0 + n, gk = k0 + kk;
-      double v = 0.0;
-      if (gn < p.N && gk < Keff) v = BT ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
-      Bs[kk][n] = v;
-    }
-    __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       double a[2], b[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+      for (int i = 0; i < 2; ++i) a[i] = As[cur][kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < 2; ++j) b[j] = Bs[cur][kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) {
+      if (!full) {
+        la.zero_tail(kn, Keff);
+        lb.zero_tail(kn, Keff);
+      }
+      stage(cur ^ 1);
     }
     __syncthreads();
   }
@@ -147,14 +213,11 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
       }
 }
 
-// ---------------------------------------------------------------------------
-// Complex FP64
-// ---------------------------------------------------------------------------
 template <bool AT, bool BT>
 __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
   constexpr int BK = 8;
-  __shared__ double Ar[BK][BM + 1], Ai[BK][BM + 1];
-  __shared__ double Br[BK][BN + 1], Bi[BK][BN + 1];
+  __shared__ double Ar[2][BK][BM + 1], Ai[2][BK][BM + 1];
+  __shared__ double Br[2][BK][BN + 1], Bi[2][BK][BN + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -164,6 +227,25 @@ __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
   const double2* B = (const double2*)p.B + bk.z * p.sB;
   double2* C = (double2*)p.C + bk.z * p.sC;
   const double sa = p.conjA ? -1.0 : 1.0, sb = p.conjB ? -1.0 : 1.0;
+  // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
+  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
+
+  TileLoader<double2, BM, BK> la;
+  TileLoader<double2, BN, BK> lb;
+  la.init(A, p.lda, AT, m0, p.M, tid);
+  lb.init(B, p.ldb, !BT, n0, p.N, tid);
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < la.NI; ++i) {
+      Ar[buf][la.kk[i]][la.row[i]] = la.v[i].x;
+      Ai[buf][la.kk[i]][la.row[i]] = sa * la.v[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < lb.NI; ++i) {
+      Br[buf][lb.kk[i]][lb.row[i]] = lb.v[i].x;
+      Bi[buf][lb.kk[i]][lb.row[i]] = sb * lb.v[i].y;
+    }
+  };
 
   v4f64 cr[2][2], ci[2][2];
 #pragma unroll
@@ -174,47 +256,44 @@ __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
       ci[i][j] = (v4f64){0.0, 0.0, 0.0, 0.0};
     }
 
-  // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
-  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
-  for (int64_t k0 = 0; k0 < Keff; k0 += BK) {
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      int m, kk;
-      if (!AT) { m = tid & 63; kk = (tid >> 6) + 4 * r; }
-      else { kk = tid & 7; m = (tid >> 3) + 32 * r; }
-      int64_t gm = m0 + m, gk = k0 + kk;
-      double2 v = make_double2(0.0, 0.0);
-      if (gm < p.M && gk < Keff) v = AT ? A[gk + gm * p.lda] : A[gm + gk * p.lda];
-      Ar[kk][m] = v.x;
-      Ai[kk][m] = sa * v.y;
+  const int64_t nk = (Keff + BK - 1) / BK, nfull = Keff / BK;
+  if (nfull > 0) {
+    la.fetch(0);
+    lb.fetch(0);
+  } else {
+    la.fetch_tail(0, Keff);
+    lb.fetch_tail(0, Keff);
+    la.zero_tail(0, Keff);
+    lb.zero_tail(0, Keff);
+  }
+  stage(0);
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int cur = (int)(kt & 1);
+    const int64_t kn = (kt + 1) * BK;
+    const bool more = kt + 1 < nk, full = kt + 1 < nfull;
+    if (full) {
+      la.fetch(kn);
+      lb.fetch(kn);
+    } else if (more) {
+      la.fetch_tail(kn, Keff);
+      lb.fetch_tail(kn, Keff);
     }
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      int n, kk;
-      if (!BT) { kk = tid & 7; n = (tid >> 3) + 32 * r; }
-      else { n = tid & 63; kk = (tid >> 6) + 4 * r; }
-      int64_t gn = n0 + n, gk = k0 + kk;
-      double2 v = make_double2(0.0, 0.0);
-      if (gn < p.N && gk < Keff) v = BT ? B[gn + gk * p.ldb] : B[gk + gn * p.ldb];
-      Br[kk][n] = v.x;
-      Bi[kk][n] = sb * v.y;
-    }
-    __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       double ar[2], ai[2], nai[2], br[2], bi[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         int mi = wm * 32 + i * 16 + (lane & 15);
-        ar[i] = Ar[kk + (lane >> 4)][mi];
-        ai[i] = Ai[kk + (lane >> 4)][mi];
+        ar[i] = Ar[cur][kk + (lane >> 4)][mi];
+        ai[i] = Ai[cur][kk + (lane >> 4)][mi];
         nai[i] = -ai[i];
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         int nj = wn * 32 + j * 16 + (lane & 15);
-        br[j] = Br[kk + (lane >> 4)][nj];
-        bi[j] = Bi[kk + (lane >> 4)][nj];
+        br[j] = Br[cur][kk + (lane >> 4)][nj];
+        bi[j] = Bi[cur][kk + (lane >> 4)][nj];
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -225,6 +304,13 @@ __global__ __launch_bounds__(256) void gemm_c128_kernel(GemmP p) {
           ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], ci[i][j], 0, 0, 0);
           ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], ci[i][j], 0, 0, 0);
         }
+    }
+    if (more) {
+      if (!full) {
+        la.zero_tail(kn, Keff);
+        lb.zero_tail(kn, Keff);
+      }
+      stage(cur ^ 1);
     }
     __syncthreads();
   }
