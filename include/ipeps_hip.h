@@ -85,7 +85,8 @@ int ipeps_env_destroy(ipeps_env* env);
  * one FP32-MFMA kernel instead of two split-bf16 GEMMs), and for the
  * randomized mode
  * "fast_orth" (0: Householder QR range bases; 1: CholeskyQR2 with Householder
- * fallback; 2: CholeskyQR2 on the one-workgroup FP64 Cholesky + inverse),
+ * fallback; 2, the default: CholeskyQR2 on the one-workgroup blocked FP64
+ * Cholesky + inverse when k <= 160, else as 1),
  * "fast_svd" (0/1: small SVD from Gram eigenpairs with one-sided Jacobi
  * fallback) and "gram_guard" (s_min/s_max below which the Gram route is not
  * taken, default 1e-4).  "draws" sets the position in the environment's
@@ -191,6 +192,14 @@ int ipeps_svd(ipeps_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A, vo
  * only when S_min/S_max >= guard, else method 0.  *method_used reports it. */
 int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double guard, int64_t m, int64_t n, const void* A,
                    void* U, double* S, void* V, int* method_used);
+/* Orthonormal basis Q (m x n) of the columns of a tall Y (m >= n): the range
+ * basis step of the randomized projector that replaces svd1's full SVD
+ * (src/utility/autodiff.jl:22-25; the reference has no separate routine).
+ * method 0: Householder QR; 1: CholeskyQR2 on rocSOLVER potrf / rocBLAS trsm;
+ * 2: CholeskyQR2 on the one-workgroup blocked Cholesky + triangular inverse
+ * (F64, n <= 160).  Methods 1 and 2 fall back to 0 when a Cholesky fails or
+ * the second Gram matrix is farther than 1/2 from I. */
+int ipeps_orth(ipeps_ctx* ctx, int dtype, int method, int64_t m, int64_t n, const void* Y, void* Q);
 /* svd_back (src/utility/autodiff.jl:41-75) for a square N x N matrix with
  * full U, S, V and cotangents of the first n columns dU (N x n), dS (n),
  * dV (N x n) (any may be NULL); eta is the Lorentzian broadening (1e-12 in

@@ -47,6 +47,7 @@ SIGNATURES = {
     "ipeps_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64,
                              C.c_int64, C.c_double, _P, C.c_int64, _P, C.c_int64, C.c_double, _P, C.c_int64]),
     "ipeps_svd": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, _P, _P, _DP, _P]),
+    "ipeps_orth": (C.c_int, [_P, C.c_int, C.c_int, C.c_int64, C.c_int64, _P, _P]),
     "ipeps_svd_thin": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_int64, C.c_int64, _P, _P, _DP, _P, _IP]),
     "ipeps_svd_back": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, _P, _DP, _P, _P, _DP, _P, C.c_double, _P]),
     "ipeps_correlation_spin": (C.c_int, [_P, _DP, _DP, C.c_int64, C.c_int, C.c_int, _DP, _DP, _DP]),

@@ -549,6 +549,18 @@ int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double gram_guard, int
   });
 }
 
+int ipeps_orth(ipeps_ctx* ctx, int dtype, int method, int64_t m, int64_t n, const void* Y, void* Q) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && m >= n && n > 0 && Y && Q, "bad arguments (m >= n > 0 required)");
+    IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_C128, "orth: F64 or C128");
+    IPEPS_CHECK(method >= 0 && method <= 2, "orth: method must be 0, 1 or 2");
+    Ctx& c = *ctx->c;
+    DType dt = host_dt(dtype);
+    Tensor tY = upload(c, dt, dt, {m, n}, Y);
+    download(c, op::orth(c, tY, method), dt, Q);
+  });
+}
+
 int ipeps_svd_back(ipeps_ctx* ctx, int dtype, int64_t N, int64_t n, const void* U, const double* S, const void* V,
                    const void* dU, const double* dS, const void* dV, double eta, void* dA_out) {
   return guard([&] {

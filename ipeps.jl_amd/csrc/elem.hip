@@ -5,7 +5,7 @@
 
 #include "kernels.h"
 #include "permute_impl.h"
-#include "smallchol_impl.h"
+#include "blockchol_impl.h"
 
 namespace ipeps {
 namespace k {
@@ -935,56 +935,126 @@ void ns_coeff(Ctx& c, DType dt, int64_t n, void* G) {
 }  // namespace ipeps
 
 // ---------------------------------------------------------------------------
-// Small SPD Cholesky + triangular inverse in one workgroup (smallchol_impl.h)
+// Small SPD Cholesky + triangular inverse in one workgroup (blockchol_impl.h)
 // ---------------------------------------------------------------------------
 namespace ipeps {
 namespace k {
 
-static_assert(SCHOL_MAXN == SCHOL_MAXN_HOST, "small_chol_inv size limit");
+static_assert(BCH_MAXN == SCHOL_MAXN_HOST, "small_chol_inv size limit");
 
-__global__ __launch_bounds__(SCHOL_THREADS) void small_chol_inv_kernel(int n, const double* __restrict__ G, int64_t ldg,
-                                                                       double* __restrict__ Linv, int64_t ldl,
-                                                                       int* __restrict__ info) {
-  __shared__ double P[SCHOL_MAXN * (SCHOL_MAXN + 1) / 2 + SCHOL_MAXN];
+__device__ inline double readlane_d(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), l);
+  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// (a) diagonal tile K by wave 0: lane i < 32 holds row i; the tile is replaced
+// by the inverse of its Cholesky factor (zeros above the diagonal).  Returns
+// the 1-based failing pivot within the tile, 0 on success (uniform).
+__device__ inline int bch_diag(double* tile, int lane) {
+  const int i = lane & (BCH_T - 1);
+  double c[BCH_T];
+#pragma unroll
+  for (int j = 0; j < BCH_T; ++j) c[j] = tile[j * BCH_LD + i];
+  int fail = 0;
+#pragma unroll
+  for (int k = 0; k < BCH_T; ++k) {
+    const double pk = readlane_d(c[k], k);
+    if (!(pk > 0.0) || !(pk < 1.7e308)) {
+      if (!fail) fail = k + 1;
+    }
+    const double sk = sqrt(fabs(pk)), rk = 1.0 / sk;
+    c[k] = i == k ? sk : c[k] * rk;  // L[i][k] (rows below k; rows above hold garbage)
+#pragma unroll
+    for (int j = k + 1; j < BCH_T; ++j) c[j] -= c[k] * readlane_d(c[k], j);
+  }
+  // row i of L^-1: x[j] = -(sum_{m=j+1..i} x[m] L[m][j]) / L[j][j], x[i] = 1 / L[i][i]
+  double x[BCH_T];
+#pragma unroll
+  for (int j = 0; j < BCH_T; ++j) x[j] = 0.0;
+#pragma unroll
+  for (int j = BCH_T - 1; j >= 0; --j) {
+    double sum = 0.0;
+#pragma unroll
+    for (int m = j + 1; m < BCH_T; ++m) sum += x[m] * readlane_d(c[j], m);
+    const double ljj = readlane_d(c[j], j);
+    x[j] = i == j ? 1.0 / ljj : (i > j ? -sum / ljj : 0.0);
+  }
+  if (lane < BCH_T) {
+#pragma unroll
+    for (int j = 0; j < BCH_T; ++j) tile[j * BCH_LD + i] = x[j];
+  }
+  return fail;
+}
+
+__global__ __launch_bounds__(BCH_THREADS) void small_chol_inv_kernel(int n, const double* __restrict__ G, int64_t ldg,
+                                                                     double* __restrict__ Linv, int64_t ldl,
+                                                                     int* __restrict__ info) {
+  __shared__ double T[BCH_NTILES * BCH_TE];
+  __shared__ double W[BCH_TE];
   __shared__ int fail;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  double* x = P + n * (n + 1) / 2;
+  const int tid = threadIdx.x, nt = BCH_THREADS;
+  const int NB = (n + BCH_T - 1) / BCH_T;
+  constexpr int TT = BCH_T * BCH_T;
   if (tid == 0) fail = 0;
-  schol_load(n, G, ldg, P, tid, nt);
+  bch_load(n, NB, G, ldg, T, tid, nt);
   __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    schol_pivot(n, j, P, &fail, tid);
+  for (int K = 0; K < NB; ++K) {
+    if (tid < 64) {
+      const int f = bch_diag(T + bch_tix(K, K) * BCH_TE, tid);
+      if (tid == 0 && f) fail = K * BCH_T + f;
+    }
     __syncthreads();
     if (fail) break;
-    schol_scale(n, j, P, tid, nt);
+    // (b) panel: results held in registers until every thread has read A_IK
+    const int npan = (NB - K - 1) * TT;
+    constexpr int PMAX = (BCH_NBMAX - 1) * TT / BCH_THREADS;
+    double pv[PMAX];
+    int pidx[PMAX];
+#pragma unroll
+    for (int q = 0; q < PMAX; ++q) {
+      const int e = tid + q * nt;
+      pidx[q] = -1;
+      if (e < npan) {
+        int I, r, j;
+        pv[q] = bch_panel(T, NB, K, e, &I, &r, &j);
+        pidx[q] = bch_at(I, K, r, j);
+      }
+    }
     __syncthreads();
-    schol_update(n, j, P, tid, nt);
+#pragma unroll
+    for (int q = 0; q < PMAX; ++q)
+      if (pidx[q] >= 0) T[pidx[q]] = pv[q];
+    __syncthreads();
+    // (c) trailing tiles (disjoint from the panel it reads)
+    const int m = NB - K - 1, ntr = m * (m + 1) / 2 * TT;
+    for (int e = tid; e < ntr; e += nt) bch_update(T, NB, K, e);
     __syncthreads();
   }
   if (fail) {
-    if (tid == 0) *info = fail;
+    if (tid == 0) *info = fail <= n ? fail : n;
     return;
   }
-  for (int j = n - 1; j >= 0; --j) {
-    sinv_prep(n, j, P, x, tid, nt);
-    __syncthreads();
-    for (int base = 0; base < n - j - 1; base += nt / 4) {
-      const int i = j + 1 + base + tid / 4, r = tid % 4;
-      double s = i < n ? sinv_partial(n, j, P, x, i, r) : 0.0;
-      s += __shfl_xor(s, 1, 4);
-      s += __shfl_xor(s, 2, 4);
-      if (i < n && r == 0) sinv_store(n, j, P, i, s);
+  for (int J = NB - 2; J >= 0; --J) {
+    for (int I = NB - 1; I > J; --I) {
+      for (int e = tid; e < TT; e += nt) W[(e / BCH_T) * BCH_LD + e % BCH_T] = bch_isum(T, I, J, e % BCH_T, e / BCH_T);
+      __syncthreads();
+      for (int e = tid; e < TT; e += nt) T[bch_at(I, J, e % BCH_T, e / BCH_T)] = bch_iprod(T, W, J, e % BCH_T, e / BCH_T);
+      __syncthreads();
     }
-    __syncthreads();
   }
-  schol_store(n, P, Linv, ldl, tid, nt);
+  for (int e = tid; e < n * n; e += nt) {
+    const int i = e % n, j = e / n;
+    Linv[i + j * ldl] = i >= j ? T[bch_at(i / BCH_T, j / BCH_T, i % BCH_T, j % BCH_T)] : 0.0;
+  }
   if (tid == 0) *info = 0;
 }
 
 void small_chol_inv(Ctx& c, int n, const double* G, int64_t ldg, double* Linv, int64_t ldl, int* info) {
-  IPEPS_CHECK(n >= 1 && n <= SCHOL_MAXN, "small_chol_inv: n out of range");
+  IPEPS_CHECK(n >= 1 && n <= BCH_MAXN, "small_chol_inv: n out of range");
   TimedLaunch tl(c, "small_chol_inv_kernel", (double)n * n * n, 0.0);
-  hipLaunchKernelGGL(small_chol_inv_kernel, dim3(1), dim3(SCHOL_THREADS), 0, c.stream, n, G, ldg, Linv, ldl, info);
+  hipLaunchKernelGGL(small_chol_inv_kernel, dim3(1), dim3(BCH_THREADS), 0, c.stream, n, G, ldg, Linv, ldl, info);
   HIP_CHECK(hipGetLastError());
 }
 

@@ -152,9 +152,10 @@ void absorb_fwd(Ctx& c, const AbsorbP& p);
 // out (device, 2 U^2 P floats): Abar as [u + U (a + U p)], then Adbar as [v + U (b + U p)]
 void absorb_wgrad(Ctx& c, AbsorbWP p, float* out);
 
-constexpr int SCHOL_MAXN_HOST = 192;  // largest n of small_chol_inv
-// L^-1 of the Cholesky factor G = L L^T of an SPD n x n FP64 matrix (n <= 192)
-// in one workgroup; info (device int) = 0, or j + 1 if pivot j is not positive.
+constexpr int SCHOL_MAXN_HOST = 160;  // largest n of small_chol_inv
+// L^-1 of the Cholesky factor G = L L^T of an SPD n x n FP64 matrix (n <= 160)
+// in one workgroup (blocked, 32 x 32 tiles in LDS); info (device int) = 0, or
+// j + 1 if pivot j is not positive.
 void small_chol_inv(Ctx& c, int n, const double* G, int64_t ldg, double* Linv, int64_t ldl, int* info);
 
 }  // namespace k

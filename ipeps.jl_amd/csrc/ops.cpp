@@ -907,7 +907,7 @@ static void trsm_right_upper(Ctx& c, const Tensor& R, Tensor& X) {
 // second pass is trusted when its Gram is within 1/2 of I (then cond(Q1)^2 <= 3
 // and Q is orthonormal to working precision).  Returns false when a Cholesky
 // failed or that check did not hold; Q is then undefined.
-// CholeskyQR2 for FP64 with n <= 192 on the one-workgroup Cholesky +
+// CholeskyQR2 for FP64 with n <= 160 on the one-workgroup blocked Cholesky +
 // triangular inverse: Q1 = Y L1^-T, Q = Q1 L2^-T (two GEMMs instead of the
 // triangular solves).  Same acceptance test as orth_cholqr2.
 static bool orth_cholqr2_small(Ctx& c, const Tensor& Y, Tensor& Q) {

@@ -139,13 +139,14 @@ struct SvdOpts {
   //   fast_orth = 1: range bases by CholeskyQR2 (two Gram + Cholesky + triangular
   //                  solve passes), falling back to Householder QR when a
   //                  Cholesky fails or the second Gram is farther than 1/2 from I;
-  //                  2: the same with the one-workgroup FP64 Cholesky + inverse;
+  //                  2: the same with the one-workgroup blocked FP64 Cholesky +
+  //                  inverse (F64, k <= 160; otherwise as 1);
   //   fast_svd  = 1: the small SVD of B = M^H Q from the eigenpairs of B^H B
   //                  plus one Newton-Schulz orthonormalisation of B's singular
   //                  vectors, falling back to one-sided Jacobi (gesvdj) when
   //                  s_min / s_max < gram_guard (the Gram eigenvalues then lose
   //                  more than ~1e-8 relative accuracy).
-  int fast_orth = 1;
+  int fast_orth = 2;
   int fast_svd = 1;
   double gram_guard = 1e-4;
 };

@@ -376,6 +376,22 @@ def svd1(A, ctx=None):
     return U, S, V
 
 
+def orth(Y, method=1, ctx=None):
+    """Orthonormal basis of the columns of a tall Y on the device (range-basis
+    step of the randomized projector).  method 0: Householder QR; 1:
+    CholeskyQR2 (rocSOLVER); 2: CholeskyQR2 on the one-workgroup blocked
+    Cholesky (F64, n <= 160)."""
+    ctx = ctx or default_context()
+    Y = np.asarray(Y)
+    code = C128 if np.iscomplexobj(Y) else F64
+    dt = _host_np(code)
+    m, n = Y.shape
+    Yf = np.asfortranarray(Y, dtype=dt)
+    Q = np.empty((m, n), dtype=dt, order="F")
+    check(lib().ipeps_orth(ctx.h, code, int(method), m, n, _ptr(Yf), _ptr(Q)))
+    return Q
+
+
 def svd_thin(A, method=1, guard=1e-4, ctx=None):
     """Thin SVD of a tall matrix on the device: (U, S, V, method_used) with
     A = U diag(S) V^H.  method 0: one-sided Jacobi; 1: Gram eigenpairs when

@@ -368,8 +368,10 @@ def test_fast_solver_routes_match_exact(pkg, dtype):
     (Ef, gf, st), (Ee, ge, st_exact) = runs
     print(dtype, "fast routes:", sorted(k for k in st if k.startswith("roc")), "exact:",
           sorted(k for k in st_exact if k.startswith("roc")))
-    # the Gram route runs (calls whose tail falls below the guard go to gesvdj)
-    assert "rocsolver_syevd" in st and "rocsolver_potrf" in st
+    # the Gram route runs (calls whose tail falls below the guard go to gesvdj);
+    # CholeskyQR2 on the blocked one-workgroup Cholesky for FP64, rocSOLVER for C128
+    chol = "small_chol_inv_kernel" if dtype == "f64" else "rocsolver_potrf"
+    assert "rocsolver_syevd" in st and chol in st
     assert "rocsolver_geqrf_orgqr" not in st
     assert "rocsolver_gesvdj" in st_exact and "rocsolver_geqrf_orgqr" in st_exact
     print("dE", abs(Ef - Ee), "grad rel", np.abs(gf - ge).max() / np.abs(ge).max())
@@ -441,6 +443,35 @@ def test_mixed_energy_contractions_fixed_env(pkg, emode):
     Em, _ = pkg.get_gs_energy(tm, H)
     print(emode, "E64", E64.real, "dE", abs(Em.real - E64.real))
     assert abs(Em.real - E64.real) < (1e-5 if emode == "bf16x3" else 1e-7)
+
+
+@pytest.mark.parametrize("n", [1, 31, 33, 96, 160])
+def test_orth_methods(pkg, n):
+    """Range bases of a tall Y (5184 x n, cond(Y) ~ 1e4): Householder (0),
+    CholeskyQR2 on rocSOLVER (1) and on the blocked one-workgroup Cholesky
+    (2, tile edges and the n = 160 limit covered).  Q^T Q = I to 1e-13, Q
+    spans Y, and 1 and 2 agree to 1e-10 (both are the Q of the QR with a
+    positive-diagonal R; 0 agrees up to column signs)."""
+    rng = np.random.default_rng(n)
+    m = 5184
+    Y = rng.standard_normal((m, n)) @ np.diag(np.logspace(0, -4, n)) @ np.linalg.qr(rng.standard_normal((n, n)))[0]
+    Qs = [pkg.orth(Y, method=k) for k in (0, 1, 2)]
+    for Q in Qs:
+        assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-13
+        assert np.abs(Q @ (Q.T @ Y) - Y).max() < 1e-12 * np.abs(Y).max()
+    assert np.abs(Qs[1] - Qs[2]).max() < 1e-10
+    sg = np.sign(np.sum(Qs[0] * Qs[1], axis=0))
+    assert np.abs(Qs[0] * sg - Qs[1]).max() < 1e-10
+
+
+def test_orth_blocked_cholesky_kernel_runs(pkg):
+    """method 2 goes through small_chol_inv_kernel (not rocSOLVER potrf)."""
+    ctx = pkg.Context()
+    ctx.set_timing(1)
+    pkg.orth(np.random.default_rng(0).standard_normal((2000, 160)), method=2, ctx=ctx)
+    st = ctx.kernel_stats(reset=True)
+    ctx.close()
+    assert "small_chol_inv_kernel" in st and "rocsolver_potrf" not in st
 
 
 def test_small_cholesky_route_matches_rocsolver(pkg):
