@@ -43,6 +43,8 @@ struct GemmP {
   // FP32 kernel: 16-B vector access allowed on the k-contiguous A / B
   // operand and on C (pointers, leading dims and batch strides aligned)
   int vecA, vecB, vecC;
+  // FP32 kernel, pre-split operands: elements per bf16 plane
+  int64_t planeA, planeB;
 };
 
 constexpr int BM = 64, BN = 64;
@@ -410,19 +412,84 @@ __device__ inline void split_store(unsigned short* dst, int plane, const float (
   }
 }
 
-// FP32 storage, split-bf16 MFMA.  Block tile TBM x TBN (128 x 128 or
-// 128 x 64), K tile 32, 256 threads = 4 waves in a 2 x 2 grid, each wave
-// owning a (TBM/2) x (TBN/2) block of 16 x 16 MFMA outputs.  The next K tile
-// is fetched into registers while the MFMAs of the current one run, then
-// split into NSPLIT bf16 planes and written to the other of two LDS buffers:
-// one barrier per K step.  LDS rows are k-contiguous and 80 B apart, so the
-// 16-B fragment reads of 16 consecutive rows hit distinct banks.
-template <bool AT, bool BT, int NSPLIT, int TBM, int TBN>
+// One operand of the FP32 kernel: TR rows (m or n) x BK k's per K step,
+// staged as NSPLIT bf16 planes.  MODE 0: FP32, row-contiguous (X[r + k ld]);
+// 1: FP32, k-contiguous (X[k + r ld]); 2: pre-split, NSPLIT bf16 planes
+// P_s[k + r ld] (k-contiguous, zero-padded in k to a multiple of 32, planes
+// `plane` elements apart; split_operand below).  Item (r, g) of a thread is
+// the 8 values k = 8g .. 8g+7 of row r.
+template <int MODE, int NSPLIT, int TR>
+struct F32Operand {
+  static constexpr int NI = TR / 64;
+  int r[NI], g[NI];
+  int64_t rc[NI];  // clamped rows
+  float v[MODE == 2 ? 1 : NI][8];
+  v8u16 h[MODE == 2 ? NI : 1][NSPLIT];
+  __device__ inline void init(int tid, int64_t r0, int64_t R) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int q = tid + 256 * i;
+      if (MODE == 0) { r[i] = q % TR; g[i] = q / TR; }
+      else { g[i] = q & 3; r[i] = q >> 2; }
+      rc[i] = min(r0 + r[i], R - 1);
+    }
+  }
+  __device__ inline void fetch(const void* X, int64_t ld, int64_t plane, int64_t k0, bool vec) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      if constexpr (MODE == 2) {
+        const unsigned short* src = (const unsigned short*)X + rc[i] * ld + k0 + 8 * g[i];
+#pragma unroll
+        for (int s = 0; s < NSPLIT; ++s) h[i][s] = *reinterpret_cast<const v8u16*>(src + s * plane);
+      } else {
+        load8<MODE == 1>((const float*)X, ld, rc[i], k0 + 8 * g[i], vec, v[i]);
+      }
+    }
+  }
+  // last, partial K tile of a block (pre-split planes are zero-padded)
+  __device__ inline void fetch_tail(const void* X, int64_t ld, int64_t plane, int64_t k0, int64_t Keff) {
+    if constexpr (MODE == 2) {
+      fetch(X, ld, plane, k0, true);
+    } else {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) load8_tail<MODE == 1>((const float*)X, ld, rc[i], k0 + 8 * g[i], Keff, v[i]);
+    }
+  }
+  __device__ inline void zero_tail(int64_t k0, int64_t Keff) {
+    if constexpr (MODE != 2) {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) zero8(v[i], k0 + 8 * g[i], Keff);
+    }
+  }
+  // dst: plane 0 of one LDS buffer, rows LDK apart, planes `lplane` apart
+  template <int LDK>
+  __device__ inline void stage(unsigned short* dst, int lplane) const {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      unsigned short* d = dst + r[i] * LDK + 8 * g[i];
+      if constexpr (MODE == 2) {
+#pragma unroll
+        for (int s = 0; s < NSPLIT; ++s) *reinterpret_cast<v8u16*>(d + s * lplane) = h[i][s];
+      } else {
+        split_store<NSPLIT>(d, lplane, v[i]);
+      }
+    }
+  }
+};
+
+// FP32 storage, split-bf16 MFMA.  Block tile TBM x TBN (128 x 128,
+// 128 x 64 or 64 x 192), K tile 32, 256 threads = 4 waves in a 2 x 2 grid,
+// each wave owning a (TBM/2) x (TBN/2) block of 16 x 16 MFMA outputs.  The
+// next K tile is fetched into registers while the MFMAs of the current one
+// run, then split into NSPLIT bf16 planes (or copied, for a pre-split
+// operand) and written to the other of two LDS buffers: one barrier per K
+// step.  LDS rows are k-contiguous and 80 B apart, so the 16-B fragment reads
+// of 16 consecutive rows hit distinct banks.
+template <int MA, int MB, int NSPLIT, int TBM, int TBN>
 __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
   constexpr int BK = 32, LDK = BK + 8;
   constexpr int WTM = TBM / 2, WTN = TBN / 2;
   constexpr int TI = WTM / 16, TJ = WTN / 16;
-  constexpr int IA = TBM / 64, IB = TBN / 64;  // staging items per thread
   constexpr int PA = TBM * LDK, PB = TBN * LDK;
   __shared__ __attribute__((aligned(16))) unsigned short As[2][NSPLIT][TBM][LDK];
   __shared__ __attribute__((aligned(16))) unsigned short Bs[2][NSPLIT][TBN][LDK];
@@ -431,53 +498,32 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
   const int wm = wave & 1, wn = wave >> 1;
   const BlockXYZ bk = xcd_block();
   const int64_t m0 = bk.x * TBM, n0 = bk.y * TBN;
-  const float* A = (const float*)p.A + bk.z * p.sA;
-  const float* B = (const float*)p.B + bk.z * p.sB;
+  const void* A = MA == 2 ? (const void*)((const unsigned short*)p.A + bk.z * p.sA) : (const void*)((const float*)p.A + bk.z * p.sA);
+  const void* B = MB == 2 ? (const void*)((const unsigned short*)p.B + bk.z * p.sB) : (const void*)((const float*)p.B + bk.z * p.sB);
   float* C = (float*)p.C + bk.z * p.sC;
   // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
   const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
   const bool vecA = p.vecA != 0, vecB = p.vecB != 0;
 
-  int ar[IA], ag[IA], br[IB], bg[IB];
-  int64_t arc[IA], brc[IB];  // clamped rows
-#pragma unroll
-  for (int i = 0; i < IA; ++i) {
-    const int q = tid + 256 * i;
-    if (AT) { ag[i] = q & 3; ar[i] = q >> 2; }
-    else { ar[i] = q % TBM; ag[i] = q / TBM; }
-    arc[i] = min(m0 + ar[i], p.M - 1);
-  }
-#pragma unroll
-  for (int i = 0; i < IB; ++i) {
-    const int q = tid + 256 * i;
-    if (!BT) { bg[i] = q & 3; br[i] = q >> 2; }
-    else { br[i] = q % TBN; bg[i] = q / TBN; }
-    brc[i] = min(n0 + br[i], p.N - 1);
-  }
-  float ra[IA][8], rb[IB][8];
+  F32Operand<MA, NSPLIT, TBM> oa;
+  F32Operand<MB, NSPLIT, TBN> ob;
+  oa.init(tid, m0, p.M);
+  ob.init(tid, n0, p.N);
   auto fetch = [&](int64_t k0) {
-#pragma unroll
-    for (int i = 0; i < IA; ++i) load8<AT>(A, p.lda, arc[i], k0 + 8 * ag[i], vecA, ra[i]);
-#pragma unroll
-    for (int i = 0; i < IB; ++i) load8<!BT>(B, p.ldb, brc[i], k0 + 8 * bg[i], vecB, rb[i]);
+    oa.fetch(A, p.lda, p.planeA, k0, vecA);
+    ob.fetch(B, p.ldb, p.planeB, k0, vecB);
   };
   auto fetch_tail = [&](int64_t k0) {
-#pragma unroll
-    for (int i = 0; i < IA; ++i) load8_tail<AT>(A, p.lda, arc[i], k0 + 8 * ag[i], Keff, ra[i]);
-#pragma unroll
-    for (int i = 0; i < IB; ++i) load8_tail<!BT>(B, p.ldb, brc[i], k0 + 8 * bg[i], Keff, rb[i]);
+    oa.fetch_tail(A, p.lda, p.planeA, k0, Keff);
+    ob.fetch_tail(B, p.ldb, p.planeB, k0, Keff);
   };
   auto stage = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < IA; ++i) split_store<NSPLIT>(&As[buf][0][ar[i]][8 * ag[i]], PA, ra[i]);
-#pragma unroll
-    for (int i = 0; i < IB; ++i) split_store<NSPLIT>(&Bs[buf][0][br[i]][8 * bg[i]], PB, rb[i]);
+    oa.template stage<LDK>(&As[buf][0][0][0], PA);
+    ob.template stage<LDK>(&Bs[buf][0][0][0], PB);
   };
   auto stage_tail = [&](int buf, int64_t k0) {
-#pragma unroll
-    for (int i = 0; i < IA; ++i) zero8(ra[i], k0 + 8 * ag[i], Keff);
-#pragma unroll
-    for (int i = 0; i < IB; ++i) zero8(rb[i], k0 + 8 * bg[i], Keff);
+    oa.zero_tail(k0, Keff);
+    ob.zero_tail(k0, Keff);
     stage(buf);
   };
 
@@ -602,6 +648,59 @@ __global__ __launch_bounds__(256) void splitk_fold(int64_t MN, int64_t S, int64_
 }
 
 // ---------------------------------------------------------------------------
+// Operand pre-splitting: op(X) (R rows x K) -> NSPLIT bf16 planes
+// P_s[k + r Kp], k-contiguous, zero for K <= k < Kp (Kp a multiple of 32).
+// Item (r, g) = the 8 values k = 8g .. 8g+7 of row r; consecutive threads take
+// consecutive g for a k-contiguous X, consecutive r otherwise (coalesced reads).
+// ---------------------------------------------------------------------------
+// k-contiguous X: item (r, g) per thread, consecutive threads along k.
+template <int NSPLIT>
+__global__ __launch_bounds__(256) void split_operand_kv_kernel(const float* __restrict__ X, int64_t ld, int64_t R,
+                                                               int64_t K, int64_t Kp, int vec,
+                                                               unsigned short* __restrict__ P) {
+  const int64_t ng = Kp / 8, items = R * ng;
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= items) return;
+  const int64_t r = q / ng, kb = 8 * (q % ng);
+  float v[8];
+  if (kb + 8 <= K) {
+    load8<true>(X, ld, r, kb, vec != 0, v);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = kb + e < K ? X[kb + e + r * ld] : 0.f;
+  }
+  split_store<NSPLIT>(P + r * Kp + kb, (int)(R * Kp), v);
+}
+
+// row-contiguous X: 64 rows x 64 k per block, read along r into LDS, written
+// along k (8 threads cover the 64 k of one row).
+template <int NSPLIT>
+__global__ __launch_bounds__(256) void split_operand_rows_kernel(const float* __restrict__ X, int64_t ld, int64_t R,
+                                                                 int64_t K, int64_t Kp,
+                                                                 unsigned short* __restrict__ P) {
+  __shared__ float tile[64][65];  // [k][r]
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int idx = tid + 256 * i, rl = idx & 63, kl = idx >> 6;
+    const int64_t r = min(r0 + rl, R - 1), k = k0 + kl;
+    tile[kl][rl] = k < K ? X[r + k * ld] : 0.f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int item = tid + 256 * j, g = item & 7, rl = item >> 3;
+    const int64_t r = r0 + rl, kb = k0 + 8 * g;
+    if (r >= R || kb >= Kp) continue;
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = tile[8 * g + e][rl];
+    split_store<NSPLIT>(P + r * Kp + kb, (int)(R * Kp), v);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
 // FP32 block tile: 128 x 64 for outputs up to 64 columns; 64 x 192 for
@@ -621,7 +720,8 @@ static void set_vec_flags(const GemmArgs& g, GemmP& p) {
   p.vecC = al16(p.C) && p.ldc % 4 == 0 && (p.sC % 4 == 0 || (one && p.kchunk == 0));
 }
 
-static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid) {
+// ma / mb: operand modes of the FP32 kernel (0 row-contiguous, 1 k-contiguous, 2 pre-split)
+static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid, int ma, int mb) {
   dim3 block(256);
   const bool at = g.transA, bt = g.transB;
   if (g.dt == DType::F64) {
@@ -635,12 +735,17 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid) {
     else if (!at && bt) hipLaunchKernelGGL((gemm_c128_kernel<false, true>), grid, block, 0, c.stream, p);
     else hipLaunchKernelGGL((gemm_c128_kernel<true, true>), grid, block, 0, c.stream, p);
   } else {
-#define IPEPS_F32_LAUNCH_T(NS, TM, TN)                                                                       \
-  do {                                                                                                       \
-    if (!at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<false, false, NS, TM, TN>), grid, block, 0, c.stream, p); \
-    else if (at && !bt) hipLaunchKernelGGL((gemm_f32_kernel<true, false, NS, TM, TN>), grid, block, 0, c.stream, p); \
-    else if (!at && bt) hipLaunchKernelGGL((gemm_f32_kernel<false, true, NS, TM, TN>), grid, block, 0, c.stream, p); \
-    else hipLaunchKernelGGL((gemm_f32_kernel<true, true, NS, TM, TN>), grid, block, 0, c.stream, p);          \
+#define IPEPS_F32_LAUNCH_MB(MA, NS, TM, TN)                                                                 \
+  do {                                                                                                      \
+    if (mb == 0) hipLaunchKernelGGL((gemm_f32_kernel<MA, 0, NS, TM, TN>), grid, block, 0, c.stream, p);      \
+    else if (mb == 1) hipLaunchKernelGGL((gemm_f32_kernel<MA, 1, NS, TM, TN>), grid, block, 0, c.stream, p); \
+    else hipLaunchKernelGGL((gemm_f32_kernel<MA, 2, NS, TM, TN>), grid, block, 0, c.stream, p);              \
+  } while (0)
+#define IPEPS_F32_LAUNCH_T(NS, TM, TN)            \
+  do {                                            \
+    if (ma == 0) IPEPS_F32_LAUNCH_MB(0, NS, TM, TN); \
+    else if (ma == 1) IPEPS_F32_LAUNCH_MB(1, NS, TM, TN); \
+    else IPEPS_F32_LAUNCH_MB(2, NS, TM, TN);      \
   } while (0)
 #define IPEPS_F32_LAUNCH(NS, TM192)                                   \
   do {                                                                \
@@ -656,8 +761,38 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid) {
     }
 #undef IPEPS_F32_LAUNCH
 #undef IPEPS_F32_LAUNCH_T
+#undef IPEPS_F32_LAUNCH_MB
   }
   HIP_CHECK(hipGetLastError());
+}
+
+static int nsplit_of(Mixed m) { return m == Mixed::BF16 ? 1 : m == Mixed::BF16X3 ? 2 : 3; }
+
+// Pre-split op(X) (R rows x K; kv: X is k-contiguous, X[k + r ld]) into
+// NSPLIT bf16 planes of R x Kp, Kp = K rounded up to 32; returns the buffer.
+static std::shared_ptr<Buf> split_operand(Ctx& c, const float* X, int64_t ld, bool kv, int64_t R, int64_t K,
+                                          int ns, int64_t* Kp_out) {
+  const int64_t Kp = (K + 31) / 32 * 32;
+  const size_t bytes = (size_t)ns * R * Kp * 2;
+  std::shared_ptr<Buf> buf(new Buf{c.pool->alloc(bytes), bytes, c.pool.get()});
+  const int64_t items = R * (Kp / 8);
+  TimedLaunch tl(c, "split_operand", 0.0, (double)R * K * 4 + (double)bytes);
+  const dim3 grid((unsigned)((items + 255) / 256));
+  unsigned short* P = (unsigned short*)buf->p;
+  const int vec = al16(X) && ld % 4 == 0;
+  const dim3 tgrid((unsigned)((R + 63) / 64), (unsigned)((Kp + 63) / 64));
+#define IPEPS_SPLIT(NS)                                                                                              \
+  do {                                                                                                               \
+    if (kv) hipLaunchKernelGGL((split_operand_kv_kernel<NS>), grid, dim3(256), 0, c.stream, X, ld, R, K, Kp, vec, P); \
+    else hipLaunchKernelGGL((split_operand_rows_kernel<NS>), tgrid, dim3(256), 0, c.stream, X, ld, R, K, Kp, P);     \
+  } while (0)
+  if (ns == 1) IPEPS_SPLIT(1);
+  else if (ns == 2) IPEPS_SPLIT(2);
+  else IPEPS_SPLIT(3);
+#undef IPEPS_SPLIT
+  HIP_CHECK(hipGetLastError());
+  *Kp_out = Kp;
+  return buf;
 }
 
 void gemm(Ctx& c, const GemmArgs& g) {
@@ -682,8 +817,35 @@ void gemm(Ctx& c, const GemmArgs& g) {
   p.kchunk = 0;
   const int64_t tbm = g.dt == DType::F32 ? f32_tile_m(g.N, g.mixed) : BM, tbn = g.dt == DType::F32 ? f32_tile_n(g.N) : BN;
   set_vec_flags(g, p);
+  p.planeA = p.planeB = 0;
   dim3 grid((unsigned)((g.M + tbm - 1) / tbm), (unsigned)((g.N + tbn - 1) / tbn), (unsigned)g.batch);
   IPEPS_CHECK(grid.y <= 65535, "gemm: N too large for the grid");
+  // FP32 operand modes: an operand that at least 3 blocks along the other
+  // output dimension re-read is split into bf16 planes once (split_operand)
+  // instead of in every block's staging.
+  int ma = g.transA ? 1 : 0, mb = g.transB ? 0 : 1;
+  std::shared_ptr<Buf> splitA, splitB;
+  if (g.dt == DType::F32 && g.batch == 1) {
+    const int ns = nsplit_of(g.mixed);
+    const int64_t Kp = (g.K + 31) / 32 * 32;
+    const bool fits = Kp <= (int64_t)64 * 65535;
+    if (grid.y >= 3 && fits && g.M * Kp < ((int64_t)1 << 31)) {
+      int64_t kp;
+      splitA = split_operand(c, (const float*)g.A, g.lda, g.transA, g.M, g.K, ns, &kp);
+      p.A = splitA->p;
+      p.lda = kp;
+      p.planeA = g.M * kp;
+      ma = 2;
+    }
+    if (grid.x >= 3 && fits && g.N * Kp < ((int64_t)1 << 31)) {
+      int64_t kp;
+      splitB = split_operand(c, (const float*)g.B, g.ldb, !g.transB, g.N, g.K, ns, &kp);
+      p.B = splitB->p;
+      p.ldb = kp;
+      p.planeB = g.N * kp;
+      mb = 2;
+    }
+  }
   const bool at = g.transA, bt = g.transB;
   const char* tf[2] = {"false", "true"};
   std::string name;
@@ -724,13 +886,14 @@ void gemm(Ctx& c, const GemmArgs& g) {
     q.ldc = g.M;
     q.sC = g.M * g.N;
     q.kchunk = kc;
-    q.sA = at ? kc : kc * g.lda;
-    q.sB = bt ? kc * g.ldb : kc;
+    // element offset of K chunk 1 (k-contiguous and pre-split operands: kc)
+    q.sA = (g.dt == DType::F32 ? ma != 0 : at) ? kc : kc * g.lda;
+    q.sB = (g.dt == DType::F32 ? mb != 0 : !bt) ? kc : kc * g.ldb;
     set_vec_flags(g, q);
     grid.z = (unsigned)S;
     {
       TimedLaunch tl(c, name, flops, bytes);
-      launch(c, g, q, grid);
+      launch(c, g, q, grid, ma, mb);
     }
     const int64_t MN = g.M * g.N;
     const char* rname = g.dt == DType::F64 ? "splitk_reduce<double>" : g.dt == DType::C128 ? "splitk_reduce<double2>"
@@ -769,7 +932,7 @@ void gemm(Ctx& c, const GemmArgs& g) {
     return;
   }
   TimedLaunch tl(c, name, flops, bytes);
-  launch(c, g, p, grid);
+  launch(c, g, p, grid, ma, mb);
 }
 
 }  // namespace k

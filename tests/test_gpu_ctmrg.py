@@ -393,13 +393,15 @@ def test_mixed_precision_energy(pkg, mixed):
 
 
 @pytest.mark.parametrize("shape", [(70, 45, 37), (300, 200, 150), (36, 72, 5000), (257, 130, 2051),
-                                   (1030, 160, 3001), (100, 192, 40)])
+                                   (1030, 160, 3001), (100, 192, 40), (400, 500, 77), (200, 600, 2100)])
 @pytest.mark.parametrize("kind", ["f64", "c128", "mixed3", "mixed6", "mixed1"])
 def test_gemm_all_ops(pkg, kind, shape):
     """Every transpose / conjugate combination against numpy.  Shapes reach the
     FP32 block tiles (N <= 64, 65..128 and N > 192: 128-row tiles; 129..192:
     64 x 192 tiles), aligned (multiple-of-4) and
-    unaligned leading dimensions, ragged edges and split-K (long K, few tiles)."""
+    unaligned leading dimensions, ragged edges and split-K (long K, few tiles),
+    and the FP32 operands pre-split into bf16 planes (B alone, A alone with
+    split-K, both; K not a multiple of 32)."""
     rng = np.random.default_rng(11)
     M, N, K = shape
     cplx = kind == "c128"
