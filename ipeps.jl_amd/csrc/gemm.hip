@@ -852,7 +852,8 @@ void gemm(Ctx& c, const GemmArgs& g) {
   double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
   if (g.dt == DType::F64) name = std::string("gemm_f64_kernel<") + tf[at] + ", " + tf[bt] + ">";
   else if (g.dt == DType::C128) { name = std::string("gemm_c128_kernel<") + tf[at] + ", " + tf[bt] + ">"; flops *= 4; }
-  else name = std::string("gemm_f32_kernel<") + tf[at] + ", " + tf[bt] + ", " + std::to_string(g.mixed == Mixed::BF16 ? 1 : g.mixed == Mixed::BF16X3 ? 2 : 3) + ">";
+  // FP32: the kernel's template head <operand mode A, operand mode B, bf16 planes>
+  else name = "gemm_f32_kernel<" + std::to_string(ma) + ", " + std::to_string(mb) + ", " + std::to_string(nsplit_of(g.mixed)) + ">";
   const double bytes = ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N * (g.beta != 0.0 ? 2 : 1)) *
                        (double)dtype_size(g.dt) * (double)g.batch;
   if (c.timing_detail)
