@@ -71,12 +71,12 @@ Tensor edge_update(Ctx& c, const Env& e, const Tensor& E, const L& ia, const L& 
   const int64_t chip = Pd.dims[0], chin = P.dims[1];
   Tensor P2r = op::reshape(c, Pd, {chip, chi_e, D, D});
   Tensor X1 = tcon(c, P2r, {-1, 1, -2, -3}, E, {1, -4, -5, -6});  // (i, ap2, adp2, E2, Eket, Ebra)
-  L la(5);
-  la[a_p2] = 10; la[a_E] = 11; la[a_p1] = -5; la[a_open] = -6; la[a_ph] = -7;
-  Tensor X2 = tcon(c, X1, {-1, 10, -2, -3, 11, -4}, A, la);       // (i, adp2, E2, Ebra, Ap1, Aopen, ph)
-  L ld(5);
-  ld[d_p2] = 20; ld[d_E] = 21; ld[d_ph] = 22; ld[d_p1] = -5; ld[d_open] = -6;
-  Tensor X3 = tcon(c, X2, {-1, 20, -2, 21, -3, -4, 22}, Ad, ld);  // (i, E2, Ap1, Aopen, Adp1, Adopen)
+  // A and Ad absorbed over (ap2, Eket) / (adp2, Ebra) and the physical leg:
+  // one site absorption (fused kernel on the mixed engine)
+  L la(5), ld(5);
+  la[a_p2] = 10; la[a_E] = 11; la[a_p1] = -3; la[a_open] = -4; la[a_ph] = 3;
+  ld[d_p2] = 20; ld[d_E] = 21; ld[d_ph] = 3; ld[d_p1] = -5; ld[d_open] = -6;
+  Tensor X3 = op::absorb(c, X1, {-1, 10, 20, -2, 11, 21}, A, la, Ad, ld);  // (i, E2, Ap1, Aopen, Adp1, Adopen)
   Tensor P1r = op::reshape(c, P, {E.dims[1], D, D, chin});
   return tcon(c, X3, {-1, 30, 31, -3, 32, -4}, P1r, {30, 31, 32, -2});
 }
