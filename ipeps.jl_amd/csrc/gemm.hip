@@ -821,11 +821,12 @@ void gemm(Ctx& c, const GemmArgs& g) {
   dim3 grid((unsigned)((g.M + tbm - 1) / tbm), (unsigned)((g.N + tbn - 1) / tbn), (unsigned)g.batch);
   IPEPS_CHECK(grid.y <= 65535, "gemm: N too large for the grid");
   // FP32 operand modes: an operand that at least 3 blocks along the other
-  // output dimension re-read is split into bf16 planes once (split_operand)
-  // instead of in every block's staging.
+  // output dimension re-read over a long K (>= 512: the staging conversion
+  // then outweighs the extra pass) is split into bf16 planes once
+  // (split_operand) instead of in every block's staging.
   int ma = g.transA ? 1 : 0, mb = g.transB ? 0 : 1;
   std::shared_ptr<Buf> splitA, splitB;
-  if (g.dt == DType::F32 && g.batch == 1) {
+  if (g.dt == DType::F32 && g.batch == 1 && g.K >= 512) {
     const int ns = nsplit_of(g.mixed);
     const int64_t Kp = (g.K + 31) / 32 * 32;
     const bool fits = Kp <= (int64_t)64 * 65535;
