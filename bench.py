@@ -73,24 +73,30 @@ def kernel_roofline(stats, name=None):
     name = name or dominant_kernel(stats)
     launches, ms, flops, bytes_ = stats[name]
     avg_ms = ms / max(1, launches)
-    if name.startswith("gemm_") or name.startswith("absorb_"):
-        unit = "TFLOP/s"
-        if name.startswith("gemm_f32"):
-            peak = PEAK_TFLOPS["bf16"]   # split-bf16 products on the bf16 MFMA
-        elif name.startswith("absorb_"):
-            peak = PEAK_TFLOPS["f32"]    # FP32 MFMA
-        else:
-            peak = PEAK_TFLOPS["f64"]
+    mfma_peak = None
+    if name.startswith("gemm_f32"):
+        # split-bf16: NSPLIT planes give NSPLIT (NSPLIT + 1) / 2 bf16 products per flop
+        ns = int(name.split("<")[1].split(",")[2].strip(" >")) if name.count(",") >= 2 else 2
+        mfma_peak = PEAK_TFLOPS["bf16"] / (ns * (ns + 1) // 2)
+    elif name.startswith("absorb_"):
+        mfma_peak = PEAK_TFLOPS["f32"]
+    elif name.startswith("gemm_"):
+        mfma_peak = PEAK_TFLOPS["f64"]
+    # The bound follows the launches' algorithmic arithmetic intensity: below
+    # the ridge point (peak flops / peak HBM bandwidth) the kernel is HBM-bound.
+    intensity = flops / bytes_ if bytes_ else float("inf")
+    if mfma_peak is not None and intensity >= mfma_peak * 1e12 / (PEAK_HBM_GBS * 1e9):
+        unit, peak, bound = "TFLOP/s", mfma_peak, "mfma"
         achieved = (flops / launches) / (avg_ms * 1e-3) / 1e12
-        bound = "mfma"
     else:
-        unit = "GB/s"
-        peak = PEAK_HBM_GBS
+        unit, peak, bound = "GB/s", PEAK_HBM_GBS, "hbm"
         achieved = (bytes_ / launches) / (avg_ms * 1e-3) / 1e9 if bytes_ else 0.0
-        bound = "hbm"
     traffic, source = measured_traffic(name)
     return dict(kernel=name, bound=bound, achieved=achieved, peak=peak, unit=unit, frac=achieved / peak,
                 traffic=traffic, traffic_source=source, algorithmic_bytes=bytes_ / max(1, launches),
+                algorithmic_flops=flops / max(1, launches), intensity_flop_per_byte=intensity,
+                mfma_peak_tflops=mfma_peak,
+                mfma_tflops=(flops / launches) / (avg_ms * 1e-3) / 1e12 if mfma_peak else None,
                 launches=launches, avg_ms=avg_ms, total_ms=ms)
 
 
