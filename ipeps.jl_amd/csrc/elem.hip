@@ -6,6 +6,7 @@
 #include "kernels.h"
 #include "permute_impl.h"
 #include "blockchol_impl.h"
+#include "tridiag_impl.h"
 
 namespace ipeps {
 namespace k {
@@ -1055,6 +1056,209 @@ void small_chol_inv(Ctx& c, int n, const double* G, int64_t ldg, double* Linv, i
   IPEPS_CHECK(n >= 1 && n <= BCH_MAXN, "small_chol_inv: n out of range");
   TimedLaunch tl(c, "small_chol_inv_kernel", (double)n * n * n, 0.0);
   hipLaunchKernelGGL(small_chol_inv_kernel, dim3(1), dim3(BCH_THREADS), 0, c.stream, n, G, ldg, Linv, ldl, info);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace k
+}  // namespace ipeps
+
+// ---------------------------------------------------------------------------
+// Symmetric tridiagonal reduction (one workgroup) and formation of Q
+// (one wave per column), tridiag_impl.h
+// ---------------------------------------------------------------------------
+namespace ipeps {
+namespace k {
+
+__device__ inline double wave_sum_d(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+__global__ __launch_bounds__(TRI_THREADS) void sytrd_kernel(int n, const double* __restrict__ G, int64_t ldg,
+                                                            double* __restrict__ d, double* __restrict__ e,
+                                                            double* __restrict__ tau, double* __restrict__ V) {
+  constexpr int NWAVE = TRI_THREADS / 64;
+  constexpr int RW = (TRI_MAXN + 63) / 64;   // rows per lane
+  constexpr int NBLK = (TRI_MAXN + 31) / 32;  // 32-row blocks of a column
+  __shared__ double P[TRI_MAXN * (TRI_MAXN + 1) / 2];
+  __shared__ double v[TRI_MAXN], w[TRI_MAXN];
+  __shared__ double part[NWAVE][TRI_MAXN];  // per-wave partials of sum_{c <= r} A'(r, c) v_c
+  __shared__ double tpart[NBLK][TRI_MAXN];  // per-block partials of sum_{r > c} A'(r, c) v_r
+  __shared__ double red[NWAVE];
+  __shared__ double dl[TRI_MAXN], el[TRI_MAXN], tl[TRI_MAXN];
+  __shared__ double sc[3];  // tau, scale
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int c = 0; c < n; ++c)
+    for (int r = c + tid; r < n; r += TRI_THREADS) P[tri_off(n, c) + r - c] = G[r + c * ldg];
+  __syncthreads();
+  for (int i = 0; i + 2 < n; ++i) {
+    const int m = n - 1 - i;
+    const int ci = tri_off(n, i);  // A(i + 1 + r, i) = P[ci + 1 + r]
+    // (a) reflector of x = A(i+1 .., i)
+    if (wave == 0) {
+      double x2 = 0.0;
+      for (int r = 1 + lane; r < m; r += 64) {
+        const double x = P[ci + 1 + r];
+        x2 += x * x;
+      }
+      x2 = wave_sum_d(x2);
+      if (lane == 0) {
+        const double alpha = P[ci + 1];
+        double t = 0.0, beta = alpha, scale = 0.0;
+        if (x2 > 0.0) {
+          beta = -copysign(sqrt(alpha * alpha + x2), alpha);
+          t = (beta - alpha) / beta;
+          scale = 1.0 / (alpha - beta);
+        }
+        sc[0] = t;
+        sc[1] = scale;
+        dl[i] = P[ci];
+        el[i] = beta;
+        tl[i] = t;
+      }
+    }
+    __syncthreads();
+    const double t = sc[0], scale = sc[1];
+    // v (v_0 = 1), kept in column i of P for the reflector output
+    for (int r = tid; r < m; r += TRI_THREADS) {
+      const double vr = r == 0 ? 1.0 : P[ci + 1 + r] * scale;
+      v[r] = vr;
+      P[ci + 1 + r] = vr;
+    }
+    __syncthreads();
+    if (t == 0.0) continue;  // H(i) = I (uniform)
+    // (b) p = tau A' v from the lower triangle of A' = A(i+1 .., i+1 ..):
+    //     lower part: waves over columns c, lanes down the rows r >= c;
+    //     transposed part: (column c, 32-row block) items, a thread each.
+    double pp[RW];
+#pragma unroll
+    for (int q = 0; q < RW; ++q) pp[q] = 0.0;
+    for (int c = wave; c < m; c += NWAVE) {
+      const double vc = v[c];
+      const int base = tri_off(n, i + 1 + c) - c;  // A'(r, c) = P[base + r], r >= c
+#pragma unroll
+      for (int q = 0; q < RW; ++q) {
+        const int r = lane + 64 * q;
+        if (r >= c && r < m) pp[q] += P[base + r] * vc;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RW; ++q) {
+      const int r = lane + 64 * q;
+      if (r < m) part[wave][r] = pp[q];
+    }
+    for (int it = tid; it < NBLK * m; it += TRI_THREADS) {
+      const int c = it % m, blk = it / m;
+      const int r0 = max(c + 1, 32 * blk), r1 = min(m, 32 * blk + 32);
+      const int base = tri_off(n, i + 1 + c) - c;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four chains of loads in flight
+      int r = r0;
+      for (; r + 3 < r1; r += 4) {
+        s0 += P[base + r] * v[r];
+        s1 += P[base + r + 1] * v[r + 1];
+        s2 += P[base + r + 2] * v[r + 2];
+        s3 += P[base + r + 3] * v[r + 3];
+      }
+      for (; r < r1; ++r) s0 += P[base + r] * v[r];
+      tpart[blk][c] = (s0 + s1) + (s2 + s3);
+    }
+    __syncthreads();
+    // (c) p and p . v; every thread then sums the wave totals itself
+    double prod = 0.0;
+    if (tid < m) {
+      double pr = 0.0;
+#pragma unroll
+      for (int q = 0; q < NWAVE; ++q) pr += part[q][tid];
+#pragma unroll
+      for (int q = 0; q < NBLK; ++q) pr += tpart[q][tid];
+      pr *= t;
+      w[tid] = pr;
+      prod = pr * v[tid];
+    }
+    prod = wave_sum_d(prod);
+    if (lane == 0) red[wave] = prod;
+    __syncthreads();
+    double dot = 0.0;
+#pragma unroll
+    for (int q = 0; q < NWAVE; ++q) dot += red[q];
+    // (d) w = p - (tau / 2)(p . v) v
+    if (tid < m) w[tid] += -0.5 * t * dot * v[tid];
+    __syncthreads();
+    // (e) A' -= v w^T + w v^T (lower), one wave per column
+    for (int c = wave; c < m; c += NWAVE) {
+      const double vc = v[c], wc = w[c];
+      const int base = tri_off(n, i + 1 + c) - c;
+      for (int r = c + lane; r < m; r += 64) P[base + r] -= v[r] * wc + w[r] * vc;
+    }
+    __syncthreads();
+  }
+  // outputs: d, e, tau and the reflectors V (column-major n x n)
+  for (int q = tid; q < n; q += TRI_THREADS) {
+    if (q + 2 < n) {
+      d[q] = dl[q];
+      e[q] = el[q];
+      tau[q] = tl[q];
+    } else if (q + 2 == n) {
+      d[q] = P[tri_off(n, q)];
+      e[q] = P[tri_off(n, q) + 1];
+      tau[q] = 0.0;
+    } else {
+      d[q] = P[tri_off(n, q)];
+    }
+  }
+  for (int idx = tid; idx < n * n; idx += TRI_THREADS) {
+    const int r = idx % n, c = idx / n;
+    double val = 0.0;
+    if (c + 2 < n && r > c) val = P[tri_off(n, c) + r - c];  // v with v_0 = 1 at r = c + 1
+    V[idx] = val;
+  }
+}
+
+// column c of Q = H(0) ... H(c-1) e_c (one wave; rows lane + 64 t)
+__global__ __launch_bounds__(256) void orgtr_kernel(int n, const double* __restrict__ V, const double* __restrict__ tau,
+                                                    double* __restrict__ Q, int64_t ldq) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (c >= n) return;
+  constexpr int R = (TRI_MAXN + 63) / 64;
+  double q[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) q[t] = (lane + 64 * t == c) ? 1.0 : 0.0;
+  for (int i = min(c - 1, n - 3); i >= 0; --i) {
+    const double* vi = V + (int64_t)i * n;
+    double vr[R];
+    double dot = 0.0;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const int r = lane + 64 * t;
+      vr[t] = r < n ? vi[r] : 0.0;
+      dot += vr[t] * q[t];
+    }
+    dot = wave_sum_d(dot) * tau[i];
+#pragma unroll
+    for (int t = 0; t < R; ++t) q[t] -= vr[t] * dot;
+  }
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int r = lane + 64 * t;
+    if (r < n) Q[r + (int64_t)c * ldq] = q[t];
+  }
+}
+
+void sytrd_q(Ctx& c, int n, const double* G, int64_t ldg, double* d, double* e, double* Q, int64_t ldq) {
+  IPEPS_CHECK(n >= 1 && n <= TRI_MAXN, "sytrd_q: n out of range");
+  std::shared_ptr<Buf> ws(new Buf{c.pool->alloc((size_t)(n * n + n) * 8), (size_t)(n * n + n) * 8, c.pool.get()});
+  double* V = (double*)ws->p;
+  double* tau = V + (size_t)n * n;
+  {
+    TimedLaunch tl(c, "sytrd_kernel", 4.0 / 3.0 * n * n * n, 0.0);
+    hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(TRI_THREADS), 0, c.stream, n, G, ldg, d, e, tau, V);
+  }
+  HIP_CHECK(hipGetLastError());
+  {
+    TimedLaunch tl(c, "orgtr_kernel", 4.0 / 3.0 * n * n * n, 0.0);
+    hipLaunchKernelGGL(orgtr_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream, n, V, tau, Q, ldq);
+  }
   HIP_CHECK(hipGetLastError());
 }
 

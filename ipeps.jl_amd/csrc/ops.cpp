@@ -980,7 +980,17 @@ static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s
   Tensor E = c.empty(DType::F64, {n});
   Tensor st = c.zeros(DType::F64, {1});
   rocblas_handle h = (rocblas_handle)c.solver;
-  {
+  if (B.dt == DType::F64 && n <= k::TRI_MAXN_HOST) {
+    // Householder tridiagonalisation and Q on the device (one workgroup + one
+    // wave per column), then rocSOLVER's divide and conquer on (d, e, Q)
+    Tensor Q = c.empty(DType::F64, {n, n});
+    k::sytrd_q(c, (int)n, G.ptr<double>(), n, lam.ptr<double>(), E.ptr<double>(), Q.ptr<double>(), n);
+    TimedLaunch tl(c, "rocsolver_stedc", 0.0, 0.0);
+    solver_check(rocsolver_dstedc(h, rocblas_evect_original, (rocblas_int)n, lam.ptr<double>(), E.ptr<double>(),
+                                  Q.ptr<double>(), (rocblas_int)n, (rocblas_int*)st.data()),
+                 "rocsolver_dstedc");
+    G = Q;
+  } else {
     TimedLaunch tl(c, "rocsolver_syevd", 0.0, 0.0);
     if (B.dt == DType::F64)
       solver_check(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, G.ptr<double>(),

@@ -124,6 +124,29 @@ def test_svd_thin_methods(pkg, cplx, decay):
         assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-12
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 33, 64, 160])
+def test_svd_thin_gram_sizes(pkg, n):
+    """Gram route (device Householder tridiagonalisation + Q, rocSOLVER stedc)
+    at the sizes it serves (k <= 160, tile and wave edges): singular values to
+    1e-13 of S_max against numpy, A = U S V^T, orthonormal U and V, and the
+    kernels that ran."""
+    rng = np.random.default_rng(n)
+    m = 2000
+    s0 = np.logspace(0, -2, n)
+    A = (np.linalg.qr(rng.standard_normal((m, n)))[0] * s0) @ np.linalg.qr(rng.standard_normal((n, n)))[0].T
+    ctx = pkg.Context()
+    ctx.set_timing(1)
+    U, S, V, used = pkg.svd_thin(A, method=1, ctx=ctx)
+    st = ctx.kernel_stats(reset=True)
+    ctx.close()
+    assert used == 1
+    assert "sytrd_kernel" in st and "rocsolver_stedc" in st and "rocsolver_syevd" not in st
+    assert np.abs(S - s0).max() < 1e-13
+    assert np.abs((U * S) @ V.T - A).max() < 1e-12
+    assert np.abs(U.T @ U - np.eye(n)).max() < 1e-12
+    assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12
+
+
 @pytest.mark.parametrize("spread", [0.3, 3e-3])
 def test_svd_thin_clustered_tail(pkg, spread):
     """A projector-like spectrum (10 decades-spaced values, then 30 values within
@@ -371,7 +394,8 @@ def test_fast_solver_routes_match_exact(pkg, dtype):
     # the Gram route runs (calls whose tail falls below the guard go to gesvdj);
     # CholeskyQR2 on the blocked one-workgroup Cholesky for FP64, rocSOLVER for C128
     chol = "small_chol_inv_kernel" if dtype == "f64" else "rocsolver_potrf"
-    assert "rocsolver_syevd" in st and chol in st
+    eig = "rocsolver_stedc" if dtype == "f64" else "rocsolver_syevd"
+    assert eig in st and chol in st
     assert "rocsolver_geqrf_orgqr" not in st
     assert "rocsolver_gesvdj" in st_exact and "rocsolver_geqrf_orgqr" in st_exact
     print("dE", abs(Ef - Ee), "grad rel", np.abs(gf - ge).max() / np.abs(ge).max())
