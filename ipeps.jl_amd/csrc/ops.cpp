@@ -908,8 +908,9 @@ static void trsm_right_upper(Ctx& c, const Tensor& R, Tensor& X) {
 // and Q is orthonormal to working precision).  Returns false when a Cholesky
 // failed or that check did not hold; Q is then undefined.
 // CholeskyQR2 for FP64 with n <= 160 on the one-workgroup blocked Cholesky +
-// triangular inverse: Q1 = Y L1^-T, Q = Q1 L2^-T (two GEMMs instead of the
-// triangular solves).  Same acceptance test as orth_cholqr2.
+// triangular inverse: Q1 = Y L1^-T, Q = Q1 L2^-T (GEMMs instead of the
+// triangular solves).  Same acceptance test and second-pass skip as
+// orth_cholqr2.
 static bool orth_cholqr2_small(Ctx& c, const Tensor& Y, Tensor& Q) {
   const int64_t n = Y.dims[1];
   Tensor st = c.zeros(DType::F64, {3});  // [info1, info2, |G2 - I|_max]
@@ -921,15 +922,22 @@ static bool orth_cholqr2_small(Ctx& c, const Tensor& Y, Tensor& Q) {
   Tensor Q1 = matmul_raw(c, Y, false, false, Li, true, false);
   Tensor G2 = matmul_raw(c, Q1, true, false, Q1, false, false);
   k::eye_dev(c, DType::F64, n, G2.data(), st.ptr<double>() + 2);
-  k::small_chol_inv(c, (int)n, G2.ptr<double>(), n, Li.ptr<double>(), n, info2);
-  Q = matmul_raw(c, Q1, false, false, Li, true, false);
   double hs[3];
   HIP_CHECK(hipMemcpyAsync(hs, st.data(), sizeof(hs), hipMemcpyDeviceToHost, c.stream));
   c.sync();
   int i1, i2;
   std::memcpy(&i1, &hs[0], sizeof(int));
+  if (i1 != 0 || !(hs[2] <= 0.5)) return false;
+  if (hs[2] <= 1e-12) {
+    Q = Q1;
+    return true;
+  }
+  k::small_chol_inv(c, (int)n, G2.ptr<double>(), n, Li.ptr<double>(), n, info2);
+  Q = matmul_raw(c, Q1, false, false, Li, true, false);
+  HIP_CHECK(hipMemcpyAsync(hs, st.data(), sizeof(hs), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
   std::memcpy(&i2, &hs[1], sizeof(int));
-  return i1 == 0 && i2 == 0 && hs[2] <= 0.5;
+  return i2 == 0;
 }
 
 // The second pass is skipped when the first already left Q orthonormal to
