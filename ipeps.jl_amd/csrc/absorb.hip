@@ -35,7 +35,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int ABS_UMAX = 36;  // U = D^2 <= 36 (D <= 6)
 constexpr int ABS_PMAX = 2;   // physical extent
-constexpr int ABS_RB = 16;    // rows per chunk (forward)
+constexpr int ABS_RB = 8;     // rows per chunk (forward)
 constexpr int ABS_RBW = 8;    // rows per chunk (weight gradient)
 constexpr int ABS_NW = 4;     // waves per block (one per SIMD: 512 registers for accumulators and prefetch)
 constexpr int ABS_THREADS = 64 * ABS_NW;
@@ -88,13 +88,31 @@ struct RowChunk {
   }
 };
 
+// Chunk schedule of the persistent kernels (gridDim.x a multiple of 32).
+// Chunks 4g .. 4g+3 hold consecutive rows (32 B each), i.e. the four
+// quarters of the same 128-B lines.  They go to four blocks on the same XCD
+// (blocks are dealt to the 8 XCDs round-robin) that step through their line
+// groups in lockstep, so one line fill in that XCD's L2 serves all four.
+struct ChunkSched {
+  int j, g, gstep;  // member of the quad, first line group, group stride
+  __device__ ChunkSched() {
+    const int x = blockIdx.x % 8, y = blockIdx.x / 8;
+    j = y % 4;
+    g = x + 8 * (y / 4);
+    gstep = gridDim.x / 4;
+  }
+  __device__ int chunk(int t) const { return 4 * (g + t * gstep) + j; }
+};
+
 // Persistent blocks walk the row chunks (RB rows of one r2) with stride
 // gridDim.x; the next chunk's loads are in flight while the MFMAs of the
 // current one run.
 template <int D>
 __global__ __launch_bounds__(ABS_THREADS) void absorb_fwd_kernel(AbsorbP p) {
   constexpr int U = D * D, LD = U + 1, NT = (U + 15) / 16;
+  constexpr int YLD = ABS_RB + 1;  // staged output ys[(a + U b) YLD + row]
   __shared__ float xs[ABS_RB * U * LD];
+  __shared__ float ys[U * U * YLD];
   __shared__ float am[U * U * ABS_PMAX], adm[U * U * ABS_PMAX];
   const int P = p.P;
   const Legs32 gx = legs32(p.x), gy = legs32(p.y);
@@ -105,15 +123,16 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_fwd_kernel(AbsorbP p) {
     am[e] = p.Am[e];
     adm[e] = p.Adm[e];
   }
+  const ChunkSched sch;
   RowChunk<D, ABS_RB> rc;
-  int ch = blockIdx.x;
+  int ch = sch.chunk(0);
   if (ch < nchunk) rc.fetch(p.X, gx, (ch % nb1) * ABS_RB, ch / nb1);
-  for (; ch < nchunk; ch += gridDim.x) {
+  for (int t = 0; ch < nchunk; ch = sch.chunk(++t)) {
     const int r2 = ch / nb1, r10 = (ch % nb1) * ABS_RB;
-    __syncthreads();  // the previous chunk's rows are consumed
+    __syncthreads();  // the previous chunk's rows are consumed and its output written
     rc.store(xs);
     __syncthreads();
-    const int nx = ch + gridDim.x;
+    const int nx = sch.chunk(t + 1);
     if (nx < nchunk) rc.fetch(p.X, gx, (nx % nb1) * ABS_RB, nx / nb1);
     for (int rr = wave; rr < ABS_RB; rr += ABS_NW) {
       const int r1 = r10 + rr;
@@ -162,7 +181,6 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_fwd_kernel(AbsorbP p) {
               for (int ta = 0; ta < NT; ++ta) yacc[tb][ta] = mfma4(tacc[tu][tb][s], bo[ta], yacc[tb][ta]);
           }
       }
-      float* yrow = p.Y + r1 * gy.s1 + r2 * gy.s2;
 #pragma unroll
       for (int tb = 0; tb < NT; ++tb)
 #pragma unroll
@@ -170,8 +188,17 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_fwd_kernel(AbsorbP p) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int b = 16 * tb + 4 * lg + i, a = 16 * ta + lr;
-            if (a < U && b < U) yrow[off2<D>(a, gy.sx1, gy.sx2) + off2<D>(b, gy.sy1, gy.sy2)] = yacc[tb][ta][i];
+            if (a < U && b < U) ys[(a + U * b) * YLD + rr] = yacc[tb][ta][i];
           }
+    }
+    __syncthreads();
+    // the chunk's output, consecutive threads along the rows (the stride-1 leg)
+    const int nrow = min(ABS_RB, gx.n1 - r10);
+    float* ybase = p.Y + r10 * gy.s1 + r2 * gy.s2;
+    for (int e = threadIdx.x; e < ABS_RB * U * U; e += ABS_THREADS) {
+      const int rr = e % ABS_RB, ab = e / ABS_RB;
+      if (rr < nrow)
+        ybase[rr * gy.s1 + off2<D>(ab % U, gy.sx1, gy.sx2) + off2<D>(ab / U, gy.sy1, gy.sy2)] = ys[ab * YLD + rr];
     }
   }
 }
@@ -203,19 +230,20 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
       }
   const int nb1 = (gx.n1 + ABS_RBW - 1) / ABS_RBW;
   const int nblk = nb1 * gx.n2;
+  const ChunkSched sch;
   RowChunk<D, ABS_RBW> cx, cg;
-  int blk = blockIdx.x;
+  int blk = sch.chunk(0);
   if (blk < nblk) {
     cx.fetch(p.X, gx, (blk % nb1) * ABS_RBW, blk / nb1);
     cg.fetch(p.G, gg, (blk % nb1) * ABS_RBW, blk / nb1);
   }
-  for (; blk < nblk; blk += gridDim.x) {
+  for (int t = 0; blk < nblk; blk = sch.chunk(++t)) {
     const int r10 = (blk % nb1) * ABS_RBW;
     __syncthreads();  // previous rows fully consumed
     cx.store(xs);
     cg.store(gs);
     __syncthreads();
-    const int nx = blk + gridDim.x;
+    const int nx = sch.chunk(t + 1);
     if (nx < nblk) {
       cx.fetch(p.X, gx, (nx % nb1) * ABS_RBW, nx / nb1);
       cg.fetch(p.G, gg, (nx % nb1) * ABS_RBW, nx / nb1);
@@ -364,9 +392,8 @@ void absorb_fwd(Ctx& c, const AbsorbP& p) {
   const int nb1 = (p.x.n1 + ABS_RB - 1) / ABS_RB;
   const int64_t nb = (int64_t)nb1 * p.x.n2;
   IPEPS_CHECK(nb >= 1 && nb < (1ll << 31), "absorb: grid");
-  // one block per CU (LDS-bound), each walking an equal share of the chunks
-  const int64_t per = (nb + 255) / 256;
-  const unsigned grid = (unsigned)((nb + per - 1) / per);
+  // one block per CU (LDS-bound); the chunk schedule needs a multiple of 32
+  const unsigned grid = 256;
   const double rows = (double)p.x.n1 * p.x.n2;
   TimedLaunch tl(c, "absorb_fwd_kernel", rows * 4.0 * p.U * p.U * p.U * p.P, rows * 8.0 * p.U * p.U);
   const dim3 blk(ABS_THREADS);
@@ -380,7 +407,7 @@ void absorb_wgrad(Ctx& c, AbsorbWP p, float* Abar_Adbar) {
   IPEPS_CHECK(absorb_supported(p.D, p.P) && p.U == p.D * p.D, "absorb: unsupported D / d");
   const int nb1 = (p.x.n1 + ABS_RBW - 1) / ABS_RBW;
   const int64_t nblk = (int64_t)nb1 * p.x.n2;
-  const int grid = (int)std::min<int64_t>(nblk, 256);  // one persistent block per CU
+  const int grid = 256;  // one persistent block per CU; the chunk schedule needs a multiple of 32
   const int n = 2 * p.U * p.U * p.P;
   std::shared_ptr<Buf> ws(new Buf{c.pool->alloc((size_t)grid * n * 4), (size_t)grid * n * 4, c.pool.get()});
   p.partial = (float*)ws->p;
