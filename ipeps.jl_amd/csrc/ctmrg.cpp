@@ -315,28 +315,42 @@ void run_pair(Ctx& c, Env& e, F&& f, G&& g) {
   Tape* tape = current_tape();
   Tape lane_tape;
   lane_tape.active = tape && tape->active;
+  const bool rec = tape && tape->active;
+  // tape segments of the two lanes (the reverse sweep runs them concurrently)
+  const int smain = rec ? tape->next_seg++ : 0, slane = rec ? tape->next_seg++ : 0;
+  const size_t m0 = rec ? tape->order.size() : 0;
   const op::Prec prec = op::prec();
   std::exception_ptr lane_err;
   std::thread th([&] {
     try {
       HIP_CHECK(hipSetDevice(l.device));
       TapeScope ts(tape ? &lane_tape : nullptr);
+      current_seg() = slane;
       op::prec() = prec;
       g(l);
+      current_seg() = 0;
       HIP_CHECK(hipEventRecord(join, l.stream));
     } catch (...) {
+      current_seg() = 0;
       lane_err = std::current_exception();
       (void)hipEventRecord(join, l.stream);
     }
   });
   std::exception_ptr main_err;
+  const int seg0 = current_seg();
   try {
+    current_seg() = smain;
     f(c);
   } catch (...) {
     main_err = std::current_exception();
   }
+  current_seg() = seg0;
   th.join();
-  if (tape) tape->order.insert(tape->order.end(), lane_tape.order.begin(), lane_tape.order.end());
+  if (tape) {
+    const size_t m1 = tape->order.size();
+    tape->order.insert(tape->order.end(), lane_tape.order.begin(), lane_tape.order.end());
+    if (rec && !main_err && !lane_err) tape->pairs.push_back(Tape::Pair{m0, m1, tape->order.size(), smain, slane});
+  }
   (void)hipStreamWaitEvent(c.stream, join, 0);
   (void)hipEventDestroy(fork);
   (void)hipEventDestroy(join);

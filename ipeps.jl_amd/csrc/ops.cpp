@@ -8,12 +8,19 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <map>
+#include <thread>
 
 namespace ipeps {
 
 Tape*& current_tape() {
   thread_local Tape* t = nullptr;
   return t;
+}
+
+int& current_seg() {
+  thread_local int s = 0;
+  return s;
 }
 
 namespace op {
@@ -33,6 +40,7 @@ static bool needs(const Tensor& t) { return recording() && (bool)t.node; }
 std::shared_ptr<Node> make_node(std::function<void(Ctx&, Node&)> back) {
   auto n = std::make_shared<Node>();
   n->back = std::move(back);
+  n->seg = current_seg();
   current_tape()->push(n);
   return n;
 }
@@ -41,6 +49,7 @@ Tensor leaf(Tensor t) {
   if (!recording()) return t;
   auto n = std::make_shared<Node>();
   n->leaf = true;
+  n->seg = current_seg();
   current_tape()->push(n);
   t.node = n;
   return t;
@@ -52,10 +61,29 @@ Tensor clone(Ctx& c, const Tensor& t) {
   return r;
 }
 
+namespace {
+// While a lane sweeps its segment of a pair, cotangents for nodes outside the
+// segment (recorded before the pair) are kept here and applied after the join.
+struct Deferral {
+  int seg = 0;
+  std::vector<std::pair<std::shared_ptr<Node>, Tensor>> list;
+};
+Deferral*& deferral() {
+  thread_local Deferral* d = nullptr;
+  return d;
+}
+}  // namespace
+
 void acc(Ctx& c, const std::shared_ptr<Node>& n, const Tensor& g) {
   if (!n) return;
   Tensor gg = g;
   gg.node.reset();
+  if (Deferral* d = deferral()) {
+    if (n->seg != d->seg) {
+      d->list.emplace_back(n, gg);
+      return;
+    }
+  }
   if (!n->grad.defined()) {
     n->grad = gg;
     return;
@@ -69,17 +97,83 @@ void acc(Ctx& c, const std::shared_ptr<Node>& n, const Tensor& g) {
   k::axpy(c, g.dt, g.numel(), 1.0, g.data(), n->grad.data());
 }
 
+static void back_node(Ctx& c, Node& n) {
+  if (!n.back) return;
+  bool any = n.grad.defined();
+  for (auto& s : n.slots) any = any || s.defined();
+  if (any) n.back(c, n);
+  n.grad = Tensor();
+  n.slots.clear();
+  n.back = nullptr;
+}
+
+// Reverse sweep of a pair: the second lane's segment on the context's second
+// lane (own stream, allocator, host thread), the main segment here; the
+// cotangents each sends to earlier nodes are applied after the join, the
+// second lane's first, each in its own order -- the order of the sequential
+// sweep, which takes the second lane's nodes (recorded last) first.
+static void backward_pair(Ctx& c, Tape& t, const Tape::Pair& p) {
+  Ctx& l = c.second_lane();
+  hipEvent_t fork, join;
+  HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+  HIP_CHECK(hipEventRecord(fork, c.stream));
+  HIP_CHECK(hipStreamWaitEvent(l.stream, fork, 0));
+  l.timing = c.timing;
+  l.timing_detail = c.timing_detail;
+  l.timing_filter = c.timing_filter;
+  Deferral dl, dm;
+  dl.seg = p.slane;
+  dm.seg = p.smain;
+  const Prec prec0 = prec();
+  std::exception_ptr lane_err, main_err;
+  std::thread th([&] {
+    try {
+      HIP_CHECK(hipSetDevice(l.device));
+      prec() = prec0;
+      deferral() = &dl;
+      for (size_t i = p.l1; i-- > p.m1;) back_node(l, *t.order[i]);
+      deferral() = nullptr;
+      HIP_CHECK(hipEventRecord(join, l.stream));
+    } catch (...) {
+      deferral() = nullptr;
+      lane_err = std::current_exception();
+      (void)hipEventRecord(join, l.stream);
+    }
+  });
+  try {
+    deferral() = &dm;
+    for (size_t i = p.m1; i-- > p.m0;) back_node(c, *t.order[i]);
+    deferral() = nullptr;
+  } catch (...) {
+    deferral() = nullptr;
+    main_err = std::current_exception();
+  }
+  th.join();
+  (void)hipStreamWaitEvent(c.stream, join, 0);
+  (void)hipEventDestroy(fork);
+  (void)hipEventDestroy(join);
+  c.kev.insert(c.kev.end(), l.kev.begin(), l.kev.end());
+  l.kev.clear();
+  if (main_err) std::rethrow_exception(main_err);
+  if (lane_err) std::rethrow_exception(lane_err);
+  for (auto& e : dl.list) acc(c, e.first, e.second);
+  for (auto& e : dm.list) acc(c, e.first, e.second);
+}
+
 void backward(Ctx& c, Tape& t) {
   TapeScope ts(nullptr);
-  for (auto it = t.order.rbegin(); it != t.order.rend(); ++it) {
-    Node& n = **it;
-    if (!n.back) continue;
-    bool any = n.grad.defined();
-    for (auto& s : n.slots) any = any || s.defined();
-    if (any) n.back(c, n);
-    n.grad = Tensor();
-    n.slots.clear();
-    n.back = nullptr;
+  std::map<size_t, const Tape::Pair*> by_end;  // l1 -> pair
+  for (auto& p : t.pairs) by_end[p.l1] = &p;
+  for (size_t i = t.order.size(); i-- > 0;) {
+    auto it = by_end.find(i + 1);
+    if (it != by_end.end() && it->second->m1 < it->second->l1) {
+      const Tape::Pair& p = *it->second;
+      backward_pair(c, t, p);
+      i = p.m0;  // continue below the pair
+      continue;
+    }
+    back_node(c, *t.order[i]);
   }
 }
 

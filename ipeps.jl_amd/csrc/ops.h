@@ -23,21 +23,36 @@ struct Node {
   std::vector<Tensor> slots;                    // per-output cotangents of multi-output ops
   std::function<void(Ctx&, Node&)> back;        // propagate to inputs (null for leaves)
   bool leaf = false;
+  int seg = 0;                                  // tape segment (a lane of a concurrent pair), 0 outside
 };
 
 struct Tape {
   bool active = false;
   std::vector<std::shared_ptr<Node>> order;
   std::mutex mu;  // two lanes of a CTMRG step record concurrently
+  // Two independent computations recorded concurrently (a pair of CTMRG
+  // moves): order[m0, m1) by the main lane (segment smain), order[m1, l1) by
+  // the second lane (segment slane).  The reverse sweep runs them on two lanes.
+  struct Pair {
+    size_t m0, m1, l1;
+    int smain, slane;
+  };
+  std::vector<Pair> pairs;
+  int next_seg = 1;
   void push(std::shared_ptr<Node> n) {
     std::lock_guard<std::mutex> g(mu);
     order.push_back(std::move(n));
   }
-  void clear() { order.clear(); }
+  void clear() {
+    order.clear();
+    pairs.clear();
+  }
 };
 
 // Current tape of this thread (null when not recording).
 Tape*& current_tape();
+// Tape segment of the nodes this thread records (0: none).
+int& current_seg();
 
 struct TapeScope {
   Tape* prev;

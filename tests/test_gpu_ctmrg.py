@@ -524,9 +524,10 @@ def test_small_cholesky_route_matches_rocsolver(pkg):
 @pytest.mark.parametrize("dtype", ["f64", "mixed"])
 def test_paired_moves_match_sequential(pkg, dtype):
     """rg_step with left || right and top || bottom on two streams / threads
-    (paired_moves=1, the default) gives bit-identical environments, energy
-    and step gradient to the sequential schedule, randomized projector
-    included (draw index = position of the move in the step)."""
+    (paired_moves=1, the default), and the reverse sweep over each pair on two
+    lanes, give bit-identical environments, energy and step gradient to the
+    sequential schedule, randomized projector included (draw index = position
+    of the move in the step)."""
     A = O.seeded_tensor((4, 4, 4, 4, 2), 3).numpy()
     H = [h.numpy() for h in O.heisenberg(1.0)]
     out = []
