@@ -233,62 +233,6 @@ Tensor close_dm(Ctx& c, const Tensor& h1, const Tensor& h2) {
 
 }  // namespace
 
-void get_dms(Ctx& c, const Env& e, Tensor& roh, Tensor& rov) {
-  const auto& C = e.C;
-  const auto& E = e.E;
-  Tensor UL = corner_ul(c, C[0], E[0], E[3], e.A, e.Ad);
-  Tensor LR = corner_lr(c, C[2], E[2], E[1], e.A, e.Ad);
-  // left half of rho_h: UL closed with C4 E3 -> (e1r, e3r, A_r, Ad_r, p, p')
-  Tensor C4E3 = tcon(c, C[3], {-1, 1}, E[2], {1, -2, -3, -4});
-  Tensor Lh = tcon(c, UL, {-1, 1, 2, -3, 3, -4, -5, -6}, C4E3, {1, -2, 2, 3});
-  // right half of rho_h: LR closed with C2 E1 -> (e1l, e3l, A_l, Ad_l, p, p')
-  Tensor C2E1 = tcon(c, C[1], {1, -2}, E[0], {-1, 1, -3, -4});
-  Tensor Rh = tcon(c, LR, {-2, 1, 2, -3, 3, -4, -5, -6}, C2E1, {-1, 1, 2, 3});
-  roh = close_dm(c, Lh, Rh);
-  // upper half of rho_v: UL closed with C2 E2 -> (e4d, e2d, A_d, Ad_d, p, p')
-  Tensor C2E2 = tcon(c, C[1], {-1, 1}, E[1], {1, -2, -3, -4});
-  Tensor Uv = tcon(c, UL, {1, -1, -3, 2, -4, 3, -5, -6}, C2E2, {1, -2, 2, 3});
-  // lower half of rho_v: LR closed with C4 E4 -> (e4u, e2u, A_u, Ad_u, p, p')
-  Tensor C4E4 = tcon(c, C[3], {1, -2}, E[3], {-1, 1, -3, -4});
-  Tensor Bv = tcon(c, LR, {1, -2, -3, 2, -4, 3, -5, -6}, C4E4, {-1, 1, 2, 3});
-  rov = close_dm(c, Uv, Bv);
-}
-
-void projector(Ctx& c, const Env& e0, int dir, int64_t chi, Tensor& P1, Tensor& P2, std::vector<double>& s) {
-  Env e = e0;  // shares the tensors; the *_rg functions stop after the projector
-  Proj p;
-  const uint64_t draw = e0.draws + (uint64_t)dir;
-  switch (dir) {
-    case 0: left_rg(c, e, chi, draw, &p); break;
-    case 1: right_rg(c, e, chi, draw, &p); break;
-    case 2: top_rg(c, e, chi, draw, &p); break;
-    default: bottom_rg(c, e, chi, draw, &p); break;
-  }
-  P1 = p.P1;
-  P2 = p.P2;
-  s = p.s;
-}
-
-void init_ctm1(Ctx& c, Env& e) {
-  const Tensor &A = e.A, &Ad = e.Ad;
-  const int64_t D = A.dims[0];
-  auto C = [&](const L& la, const L& lb) {
-    return op::renormalize(c, op::reshape(c, tcon(c, A, la, Ad, lb), {D * D, D * D}));
-  };
-  auto E = [&](const L& la, const L& lb) {
-    return op::renormalize(c, op::reshape(c, tcon(c, A, la, Ad, lb), {D * D, D * D, D, D}));
-  };
-  // src/ctm/ctm_tensor.jl:97-130
-  e.C[0] = C({1, 2, -1, -3, 3}, {1, 2, -2, -4, 3});
-  e.C[1] = C({1, -1, -3, 2, 3}, {1, -2, -4, 2, 3});
-  e.C[2] = C({-1, -3, 1, 2, 3}, {-2, -4, 1, 2, 3});
-  e.C[3] = C({-1, 1, 2, -3, 3}, {-2, 1, 2, -4, 3});
-  e.E[0] = E({1, -1, -5, -3, 2}, {1, -2, -6, -4, 2});
-  e.E[1] = E({-1, -5, -3, 1, 2}, {-2, -6, -4, 1, 2});
-  e.E[2] = E({-5, -1, 1, -3, 2}, {-6, -2, 1, -4, 2});
-  e.E[3] = E({-1, 1, -3, -5, 2}, {-2, 1, -4, -6, 2});
-}
-
 namespace {
 
 // f on the context's stream and g on its second lane, concurrently (their
@@ -361,6 +305,78 @@ void run_pair(Ctx& c, Env& e, F&& f, G&& g) {
 }
 
 }  // namespace
+
+void get_dms(Ctx& c, const Env& e, Tensor& roh, Tensor& rov) {
+  const auto& C = e.C;
+  const auto& E = e.E;
+  Tensor Lh, Uv, Rh, Bv;
+  // the upper-left corner and the halves it closes
+  auto ul_side = [&](Ctx& cc) {
+    Tensor UL = corner_ul(cc, C[0], E[0], E[3], e.A, e.Ad);
+    // left half of rho_h: UL closed with C4 E3 -> (e1r, e3r, A_r, Ad_r, p, p')
+    Tensor C4E3 = tcon(cc, C[3], {-1, 1}, E[2], {1, -2, -3, -4});
+    Lh = tcon(cc, UL, {-1, 1, 2, -3, 3, -4, -5, -6}, C4E3, {1, -2, 2, 3});
+    // upper half of rho_v: UL closed with C2 E2 -> (e4d, e2d, A_d, Ad_d, p, p')
+    Tensor C2E2 = tcon(cc, C[1], {-1, 1}, E[1], {1, -2, -3, -4});
+    Uv = tcon(cc, UL, {1, -1, -3, 2, -4, 3, -5, -6}, C2E2, {1, -2, 2, 3});
+  };
+  // the lower-right corner and the halves it closes
+  auto lr_side = [&](Ctx& cc) {
+    Tensor LR = corner_lr(cc, C[2], E[2], E[1], e.A, e.Ad);
+    // right half of rho_h: LR closed with C2 E1 -> (e1l, e3l, A_l, Ad_l, p, p')
+    Tensor C2E1 = tcon(cc, C[1], {1, -2}, E[0], {-1, 1, -3, -4});
+    Rh = tcon(cc, LR, {-2, 1, 2, -3, 3, -4, -5, -6}, C2E1, {-1, 1, 2, 3});
+    // lower half of rho_v: LR closed with C4 E4 -> (e4u, e2u, A_u, Ad_u, p, p')
+    Tensor C4E4 = tcon(cc, C[3], {1, -2}, E[3], {-1, 1, -3, -4});
+    Bv = tcon(cc, LR, {1, -2, -3, 2, -4, 3, -5, -6}, C4E4, {-1, 1, 2, 3});
+  };
+  if (e.opt.paired_moves) {
+    // the two sides share only their inputs: second lane for the lower right
+    Env held = e;
+    run_pair(c, held, ul_side, lr_side);
+  } else {
+    ul_side(c);
+    lr_side(c);
+  }
+  roh = close_dm(c, Lh, Rh);
+  rov = close_dm(c, Uv, Bv);
+}
+
+void projector(Ctx& c, const Env& e0, int dir, int64_t chi, Tensor& P1, Tensor& P2, std::vector<double>& s) {
+  Env e = e0;  // shares the tensors; the *_rg functions stop after the projector
+  Proj p;
+  const uint64_t draw = e0.draws + (uint64_t)dir;
+  switch (dir) {
+    case 0: left_rg(c, e, chi, draw, &p); break;
+    case 1: right_rg(c, e, chi, draw, &p); break;
+    case 2: top_rg(c, e, chi, draw, &p); break;
+    default: bottom_rg(c, e, chi, draw, &p); break;
+  }
+  P1 = p.P1;
+  P2 = p.P2;
+  s = p.s;
+}
+
+void init_ctm1(Ctx& c, Env& e) {
+  const Tensor &A = e.A, &Ad = e.Ad;
+  const int64_t D = A.dims[0];
+  auto C = [&](const L& la, const L& lb) {
+    return op::renormalize(c, op::reshape(c, tcon(c, A, la, Ad, lb), {D * D, D * D}));
+  };
+  auto E = [&](const L& la, const L& lb) {
+    return op::renormalize(c, op::reshape(c, tcon(c, A, la, Ad, lb), {D * D, D * D, D, D}));
+  };
+  // src/ctm/ctm_tensor.jl:97-130
+  e.C[0] = C({1, 2, -1, -3, 3}, {1, 2, -2, -4, 3});
+  e.C[1] = C({1, -1, -3, 2, 3}, {1, -2, -4, 2, 3});
+  e.C[2] = C({-1, -3, 1, 2, 3}, {-2, -4, 1, 2, 3});
+  e.C[3] = C({-1, 1, 2, -3, 3}, {-2, 1, 2, -4, 3});
+  e.E[0] = E({1, -1, -5, -3, 2}, {1, -2, -6, -4, 2});
+  e.E[1] = E({-1, -5, -3, 1, 2}, {-2, -6, -4, 1, 2});
+  e.E[2] = E({-5, -1, 1, -3, 2}, {-6, -2, 1, -4, 2});
+  e.E[3] = E({-1, 1, -3, -5, 2}, {-2, 1, -4, -6, 2});
+}
+
 
 std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi) {
   op::prec().mixed = e.opt.mixed;
