@@ -64,35 +64,58 @@ __global__ __launch_bounds__(TPB) void permute_kernel(PermP p, const T* __restri
 }
 
 // Rows kernel (fastest output axis = fastest input axis); see permute_impl.h.
+// A thread's PER elements are loaded unconditionally (out-of-range ones read
+// element 0) before any is stored, so its loads are in flight together.
+constexpr int PERM_ROWS_MAX = 4096;  // rb * chunk of a block (plan_permute's target)
 template <typename T>
 __global__ __launch_bounds__(TPB) void permute_rows_kernel(PermRowsP p, const T* __restrict__ in,
                                                            T* __restrict__ out, int conj) {
+  constexpr int PER = PERM_ROWS_MAX / TPB;
   __shared__ uint32_t row_off[256];
   const uint32_t blk = blockIdx.x;
   perm_rows_phase1(p, blk, threadIdx.x, row_off);
   __syncthreads();
   const uint32_t total = p.rb * p.chunk;
-  for (uint32_t e = threadIdx.x; e < total; e += TPB) {
-    uint32_t io, oo;
-    if (perm_rows_elem(p, blk, e, row_off, io, oo)) out[oo] = conj_if(in[io], conj != 0);
+  T v[PER];
+  uint32_t oo[PER];
+  bool ok[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const uint32_t e = threadIdx.x + k * TPB;
+    uint32_t io = 0, o = 0;
+    ok[k] = e < total && perm_rows_elem(p, blk, e, row_off, io, o);
+    oo[k] = o;
+    v[k] = in[ok[k] ? io : 0u];
   }
+#pragma unroll
+  for (int k = 0; k < PER; ++k)
+    if (ok[k]) out[oo[k]] = conj_if(v[k], conj != 0);
 }
 
 // Tiled kernel (the input-contiguous axis j is not the first output axis);
-// TS x TS tiles through LDS, see permute_impl.h.
+// TS x TS tiles through LDS, see permute_impl.h.  Loads as in the rows kernel.
 template <typename T, int TS>
 __global__ __launch_bounds__(TPB) void permute_tiled_kernel(PermTileP p, const T* __restrict__ in,
                                                             T* __restrict__ out, int conj) {
+  constexpr int PER = TS * TS / TPB;
   __shared__ T tile[TS][TS + 1];
   __shared__ uint32_t lead_off[TS];
   for (uint32_t rest = blockIdx.y; rest < p.nrest_total; rest += gridDim.y) {
     uint32_t x0, y0, in_rest, out_rest;
     perm_tile_block(p, blockIdx.x, rest, x0, y0, in_rest, out_rest);
-    if (threadIdx.x < TS && y0 + threadIdx.x < p.wl) lead_off[threadIdx.x] = perm_tile_lead(p, y0 + threadIdx.x);
+    if (threadIdx.x < TS) lead_off[threadIdx.x] = y0 + threadIdx.x < p.wl ? perm_tile_lead(p, y0 + threadIdx.x) : 0u;
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < TS * TS; e += TPB) {
-      const uint32_t x = e % TS, y = e / TS;
-      if (x0 + x < p.wj && y0 + y < p.wl) tile[y][x] = in[in_rest + lead_off[y] + x0 + x];
+    T v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t e = threadIdx.x + k * TPB, x = e % TS, y = e / TS;
+      const bool ok = x0 + x < p.wj && y0 + y < p.wl;
+      v[k] = in[in_rest + (ok ? lead_off[y] + x0 + x : 0u)];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const uint32_t e = threadIdx.x + k * TPB;
+      tile[e / TS][e % TS] = v[k];
     }
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < TS * TS; e += TPB) {
@@ -681,6 +704,7 @@ void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::v
       TimedLaunch tl(c, kname, 0.0, 2.0 * n * dtype_size(dt));
       if (kind == 0) {
         const unsigned nb = (unsigned)(((int64_t)rp.nrows + rp.rb - 1) / rp.rb * rp.nchunk);
+IPEPS_CHECK(rp.rb * rp.chunk <= (uint32_t)PERM_ROWS_MAX, "permute: rows block too large");
 #define RK(T, ...) hipLaunchKernelGGL(permute_rows_kernel<T>, dim3(nb), dim3(TPB), 0, c.stream, rp, (const T*)in, (T*)out, cj)
         DISPATCH3(dt, RK, 0);
 #undef RK
