@@ -147,6 +147,18 @@ __global__ void axpy_kernel<double2>(int64_t n, double a, const double2* x, doub
   }
 }
 
+// out = x + y (out-of-place cotangent accumulation)
+template <typename T>
+__global__ void add_kernel(int64_t n, const T* __restrict__ x, const T* __restrict__ y, T* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) out[i] = x[i] + y[i];
+}
+template <>
+__global__ void add_kernel<double2>(int64_t n, const double2* __restrict__ x, const double2* __restrict__ y,
+                                    double2* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    out[i] = make_double2(x[i].x + y[i].x, x[i].y + y[i].y);
+}
+
 template <typename T>
 __device__ inline T scal(T v, double f) { return (T)(v * f); }
 template <>
@@ -749,6 +761,14 @@ void fill_zero(Ctx& c, void* p, size_t bytes) {
 void axpy(Ctx& c, DType dt, int64_t n, double a, const void* x, void* y) {
   if (n <= 0) return;
 #define AK(T, ...) hipLaunchKernelGGL(axpy_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, a, (const T*)x, (T*)y)
+  DISPATCH3(dt, AK, 0);
+#undef AK
+  HIP_CHECK(hipGetLastError());
+}
+
+void add(Ctx& c, DType dt, int64_t n, const void* x, const void* y, void* out) {
+  if (n <= 0) return;
+#define AK(T, ...) hipLaunchKernelGGL(add_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, (const T*)x, (const T*)y, (T*)out)
   DISPATCH3(dt, AK, 0);
 #undef AK
   HIP_CHECK(hipGetLastError());

@@ -41,6 +41,7 @@ void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::v
 // Elementwise helpers.  n counts scalars of dt (complex = 1 element).
 void fill_zero(Ctx& c, void* p, size_t bytes);
 void axpy(Ctx& c, DType dt, int64_t n, double a, const void* x, void* y);        // y += a x
+void add(Ctx& c, DType dt, int64_t n, const void* x, const void* y, void* out);  // out = x + y
 void scale_by_dev(Ctx& c, DType dt, int64_t n, const void* x, void* y,           // y = x * f(*s)
                   const double* s, int mode);                                    // mode 0: 1/s, 1: s
 void conj_copy(Ctx& c, DType dt, int64_t n, const void* x, void* y);

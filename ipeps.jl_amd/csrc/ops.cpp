@@ -91,8 +91,11 @@ void acc(Ctx& c, const std::shared_ptr<Node>& n, const Tensor& g) {
   IPEPS_CHECK(n->grad.numel() == g.numel() && n->grad.dt == g.dt,
               "acc: cotangent mismatch " + dims_str(n->grad.dims) + " vs " + dims_str(g.dims));
   if (n->grad.buf.use_count() > 1) {
-    Tensor cl = clone(c, n->grad);
-    n->grad = cl;
+    // the held cotangent's buffer is shared: one out-of-place sum (no copy)
+    Tensor sum = c.empty(n->grad.dt, n->grad.dims);
+    k::add(c, g.dt, g.numel(), n->grad.data(), g.data(), sum.data());
+    n->grad = sum;
+    return;
   }
   k::axpy(c, g.dt, g.numel(), 1.0, g.data(), n->grad.data());
 }
