@@ -1283,36 +1283,33 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
         X2 = c.empty(sdt, {nc, n});
         k::scale_cols(c, sdt, nc, n, dV.data(), Sinv.ptr<double>(), X2.data(), false);
       }
-      // Mbar = X1 Vb^H + [dU' Vn^H] + [Un X2^H], applied to R1/R2 without forming Mbar
-      Tensor X1d = to_dtype(c, X1, dt), Vbd = to_dtype(c, Vb, dt);
-      Tensor dUd = sq_u ? Tensor() : to_dtype(c, dU, dt), Vnd = sq_u ? Tensor() : to_dtype(c, Vn_, dt);
-      Tensor X2d = sq_v ? Tensor() : to_dtype(c, X2, dt), Und = sq_v ? Tensor() : to_dtype(c, Un_, dt);
-      if (n1) {
-        // R1bar = Mbar R2^H
-        Tensor g = matmul_raw(c, X1d, false, false, matmul_raw(c, R2k, false, false, Vbd, false, false), true, true);
-        if (!sq_u) {
-          Tensor g2 = matmul_raw(c, dUd, false, false, matmul_raw(c, R2k, false, false, Vnd, false, false), true, true);
-          k::axpy(c, dt, g.numel(), 1.0, g2.data(), g.data());
-        }
-        if (!sq_v) {
-          Tensor g3 = matmul_raw(c, Und, false, false, matmul_raw(c, R2k, false, false, X2d, false, false), true, true);
-          k::axpy(c, dt, g.numel(), 1.0, g3.data(), g.data());
-        }
-        acc(c, n1, g);
+      // Mbar = X1 Vb^H + [dU' Vn^H] + [Un X2^H] = L F^H with the factors side by
+      // side, L = [X1 | dU' | Un] (m x K3), F = [Vb | Vn | X2] (nc x K3); applied
+      // to R1 / R2 without forming Mbar: R1bar = L (R2 F)^H, R2bar = (R1^H L) F^H
+      std::vector<Tensor> lp{X1}, fp{Vb};
+      if (!sq_u) {
+        lp.push_back(dU);
+        fp.push_back(Vn_);
       }
-      if (n2) {
-        // R2bar = R1^H Mbar
-        Tensor g = matmul_raw(c, matmul_raw(c, R1k, true, true, X1d, false, false), false, false, Vbd, true, true);
-        if (!sq_u) {
-          Tensor g2 = matmul_raw(c, matmul_raw(c, R1k, true, true, dUd, false, false), false, false, Vnd, true, true);
-          k::axpy(c, dt, g.numel(), 1.0, g2.data(), g.data());
-        }
-        if (!sq_v) {
-          Tensor g3 = matmul_raw(c, matmul_raw(c, R1k, true, true, Und, false, false), false, false, X2d, true, true);
-          k::axpy(c, dt, g.numel(), 1.0, g3.data(), g.data());
-        }
-        acc(c, n2, g);
+      if (!sq_v) {
+        lp.push_back(Un_);
+        fp.push_back(X2);
       }
+      auto side_by_side = [&](const std::vector<Tensor>& parts, int64_t rows) {
+        int64_t cols = 0;
+        for (auto& t : parts) cols += t.dims[1];
+        Tensor r = c.empty(dt, {rows, cols});
+        size_t off = 0;
+        for (auto& t : parts) {
+          Tensor td = to_dtype(c, t, dt);
+          HIP_CHECK(hipMemcpyAsync((char*)r.data() + off, td.data(), td.bytes(), hipMemcpyDeviceToDevice, c.stream));
+          off += td.bytes();
+        }
+        return r;
+      };
+      Tensor L = side_by_side(lp, m), F = side_by_side(fp, nc);
+      if (n1) acc(c, n1, matmul_raw(c, L, false, false, matmul_raw(c, R2k, false, false, F, false, false), true, true));
+      if (n2) acc(c, n2, matmul_raw(c, matmul_raw(c, R1k, true, true, L, false, false), false, false, F, true, true));
     };
     mk_slot(0, out.U);
     mk_slot(1, out.S);
