@@ -171,6 +171,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--mixed", type=int, default=3, choices=[1, 3, 6])
+    ap.add_argument("--mixed-back", type=int, default=0, choices=[0, 1, 3, 6],
+                    help="bf16 products of the VJPs (0: same as --mixed)")
+    ap.add_argument("--mixed-proj", type=int, default=0, choices=[0, 1, 3, 6],
+                    help="bf16 products inside the projector (0: same as --mixed)")
+    ap.add_argument("--tune", type=int, default=0, choices=[0, 1],
+                    help="1: GEMM autotuner (unseen shapes are tuned during the chi-saturation steps)")
     ap.add_argument("--oversample", type=int, default=16)
     ap.add_argument("--power-iters", type=int, default=1)
     ap.add_argument("--no-fp64", action="store_true")
@@ -199,8 +205,10 @@ def main():
         for _ in range(sat_steps):
             pkg.rg_step(ts, chi)
         E = g = None
-        for i in range(max(1, n_warm)):
-            if i == max(1, n_warm) - 1:
+        # with the autotuner on, one more untimed step tunes the adjoint's shapes
+        n_warm = max(1, n_warm) + (1 if a.tune else 0)
+        for i in range(n_warm):
+            if i == n_warm - 1:
                 ctx.sync()
                 ctx.set_timing(True)
                 ctx.kernel_stats(reset=True)
@@ -219,8 +227,11 @@ def main():
         stats = {k: v for k, v in stats.items() if k == dom}
         return t, E, g, stats, ts, mix
 
-    log(f"[bench] workload {a.workload}: D={D} chi={chi} N={chi*D*D}; mixed bf16x{a.mixed}")
-    t_m, E_m, g_m, st_m, ts_m, mix_m = run_path("mixed", a.warmup, a.steps, dict(mixed=a.mixed))
+    mixed_opts = dict(mixed=a.mixed, mixed_back=a.mixed_back, mixed_proj=a.mixed_proj)
+    if a.tune:
+        pkg.tune(1, 1e8)
+    log(f"[bench] workload {a.workload}: D={D} chi={chi} N={chi*D*D}; mixed {mixed_opts}")
+    t_m, E_m, g_m, st_m, ts_m, mix_m = run_path("mixed", a.warmup, a.steps, mixed_opts)
     log(f"[bench] mixed: {a.steps} steps in {t_m:.3f} s, E={E_m.real:.12f}")
     out_extra = {}
     if not a.no_fp64:
@@ -232,7 +243,7 @@ def main():
         # random test matrices, so the difference is the mixed arithmetic.
         Cs, Es = ts_64.Cs, ts_64.Es
         same = {}
-        for dt, extra in (("f64", {}), ("mixed", dict(mixed=a.mixed))):
+        for dt, extra in (("f64", {}), ("mixed", mixed_opts)):
             ts = pkg.CTMTensors(A, {}, dtype=dt, options=dict(base_opts, **extra))
             ts.set_env(Cs, Es)
             same[dt] = pkg.step_adjoint(ts, chi, H)
@@ -273,7 +284,8 @@ def main():
         "config": {"workload": f"CTMRG rg_step + energy VJP + step adjoint, Heisenberg, A (D,D,D,D,d)="
                                f"({D},{D},{D},{D},{d}) randn seed {w['seed']}, chi={chi}",
                    "D": D, "chi": chi, "N": chi * D * D, "svd": f"randomized k=chi+{a.oversample} q={a.power_iters}",
-                   "mixed_products": a.mixed},
+                   "mixed_products": a.mixed, "mixed_back": a.mixed_back or a.mixed,
+                   "mixed_proj": a.mixed_proj or a.mixed, "autotune": bool(a.tune)},
         "algorithmic_flops_per_step": F,
         "model_flops_rate_tflops": F * value / 1e12,
         "roofline": roof,
@@ -281,6 +293,8 @@ def main():
         "step_mix": {"timed_kernels_ms": tot_mix, "rocsolver_rocblas_ms": lib_ms,
                      "top": {k: round(v[1], 3) for k, v in sorted(mix_m.items(), key=lambda kv: -kv[1][1])[:8]}},
     }
+    if a.tune:
+        res["autotune"] = pkg.tune_stats()
     if not a.no_batch:
         res["batched_c4"] = batch_c4(pkg)
     res.update(out_extra)

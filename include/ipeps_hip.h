@@ -64,6 +64,21 @@ int ipeps_ctx_set_timing_filter(ipeps_ctx* ctx, const char* prefix);
  * reset != 0 clears them. */
 int ipeps_ctx_kernel_stats(ipeps_ctx* ctx, char* buf, int64_t cap, int reset);
 
+/* GEMM autotuner (process-wide; engine tuning, no reference counterpart:
+ * the reference's contraction order and kernels are OMEinsum's choice,
+ * src/utility/wrapper.jl:23).  mode 0 = tile and split-K by shape rules
+ * (default), 1 = the first product of each unseen shape with at least
+ * min_flops times every candidate plan (FP32 block tiles of 64 / 128 / 192
+ * columns x split-K slice counts) on the caller's stream and keeps the
+ * fastest, 2 = use loaded / tuned entries only.  stats: entries, shapes
+ * tuned, table hits.  save / load: a text table (one "key tile_n splitk" per
+ * line) so a later run replays the same plans and hence the same arithmetic. */
+int ipeps_tune_config(int mode, double min_flops);
+int ipeps_tune_stats(int64_t out[3]);
+int ipeps_tune_clear(void);
+int ipeps_tune_save(const char* path);
+int ipeps_tune_load(const char* path);
+
 /* CTMTensors(A, cfg) -> init_ctm1 (src/ctm/ctm_tensor.jl:63-69, 97-130).
  * dims = (D, D, D, D, d); A is F64 or C128 host data (MIXED takes F64). */
 int ipeps_env_create(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const void* A, ipeps_env** out);
@@ -73,11 +88,14 @@ int ipeps_env_destroy(ipeps_env* env);
  * "seed", "eta", "mixed_cutoff" (relative singular-value cutoff of MIXED
  * environments, default 1e-6; F64 / C128 keep the reference's 1e-12),
  * "canonical_edge" (0/1), "mixed" (1, 3 or 6 bf16 products),
+ * "mixed_back" (0, 1, 3 or 6: bf16 products of the step's VJPs; default
+ * 0 = "mixed"), "mixed_proj" (0, 1, 3 or 6: bf16 products inside the
+ * projector -- range finder, P1 and P2 -- default 0 = "mixed"),
  * "energy_fp64" (0/1: energy contractions of a MIXED environment in FP64),
  * "energy_mixed" (1, 3 or 6: bf16 products of those contractions when not
  * FP64, default 6; the density-matrix traces and ratios are FP64 always),
  * "energy_mixed_back" (0, 1, 3 or 6: bf16 products of the VJP of those
- * contractions; default 0 = the "mixed" setting of the step),
+ * contractions; default 0 = the step's VJP setting),
  * "checkpoint" (0/1), "paired_moves" (0/1, default 1: the left and right
  * moves, then the top and bottom moves, of rg_step run concurrently on two
  * streams; results are identical either way), "fused_absorb" (0/1, default 1: MIXED environments

@@ -23,6 +23,11 @@ SIGNATURES = {
     "ipeps_ctx_set_timing": (C.c_int, [_P, C.c_int]),
     "ipeps_ctx_set_timing_filter": (C.c_int, [_P, C.c_char_p]),
     "ipeps_ctx_kernel_stats": (C.c_int, [_P, C.c_char_p, C.c_int64, C.c_int]),
+    "ipeps_tune_config": (C.c_int, [C.c_int, C.c_double]),
+    "ipeps_tune_stats": (C.c_int, [_I64P]),
+    "ipeps_tune_clear": (C.c_int, []),
+    "ipeps_tune_save": (C.c_int, [C.c_char_p]),
+    "ipeps_tune_load": (C.c_int, [C.c_char_p]),
     "ipeps_env_create": (C.c_int, [_P, C.c_int, _I64P, _P, C.POINTER(_P)]),
     "ipeps_env_destroy": (C.c_int, [_P]),
     "ipeps_env_set_option": (C.c_int, [_P, C.c_char_p, C.c_double]),

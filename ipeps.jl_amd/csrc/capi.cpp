@@ -126,6 +126,10 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
     int m = (int)v;
     IPEPS_CHECK(m == 0 || m == 1 || m == 3 || m == 6, "energy_mixed_back must be 0, 1, 3 or 6");
     o.energy_mixed_back = m;
+  } else if (key == "mixed_back" || key == "mixed_proj") {
+    int m = (int)v;
+    IPEPS_CHECK(m == 0 || m == 1 || m == 3 || m == 6, key + " must be 0, 1, 3 or 6");
+    (key == "mixed_back" ? o.mixed_back : o.mixed_proj) = m;
   } else if (key == "mixed") {
     int m = (int)v;
     IPEPS_CHECK(m == 1 || m == 3 || m == 6, "mixed must be 1, 3 or 6");
@@ -206,6 +210,37 @@ int ipeps_ctx_set_timing_filter(ipeps_ctx* ctx, const char* prefix) {
     IPEPS_CHECK(ctx, "null ctx");
     ctx->c->resolve_timing();
     ctx->c->timing_filter = prefix ? prefix : "";
+  });
+}
+
+int ipeps_tune_config(int mode, double min_flops) {
+  return guard([&] { k::tune_config(mode, min_flops); });
+}
+
+int ipeps_tune_stats(int64_t out[3]) {
+  return guard([&] {
+    IPEPS_CHECK(out, "null argument");
+    long v[3];
+    k::tune_stats(v);
+    for (int i = 0; i < 3; ++i) out[i] = v[i];
+  });
+}
+
+int ipeps_tune_clear(void) {
+  return guard([&] { k::tune_clear(); });
+}
+
+int ipeps_tune_save(const char* path) {
+  return guard([&] {
+    IPEPS_CHECK(path, "null path");
+    k::tune_save(path);
+  });
+}
+
+int ipeps_tune_load(const char* path) {
+  return guard([&] {
+    IPEPS_CHECK(path, "null path");
+    k::tune_load(path);
   });
 }
 

@@ -166,7 +166,7 @@ struct TimedLaunch {
   bool on;
   Ctx::KEvent ev;
   TimedLaunch(Ctx& cc, const std::string& name, double flops, double bytes)
-      : c(cc), on(cc.timing && (cc.timing_filter.empty() || name.compare(0, cc.timing_filter.size(), cc.timing_filter) == 0)) {
+      : c(cc), on(cc.timing && !name.empty() && (cc.timing_filter.empty() || name.compare(0, cc.timing_filter.size(), cc.timing_filter) == 0)) {
     if (!on) return;
     ev.name = name;
     ev.flops = flops;

@@ -36,6 +36,7 @@ struct Proj {
 Proj get_projector(Ctx& c, const Env& e, const Tensor& R1, const Tensor& R2, int64_t chi, uint64_t draw) {
   op::SvdOpts so = e.opt.svd;
   so.seed = op::draw_seed(e.opt.svd.seed, draw);
+  op::PrecScope ps(e.opt.mixed_proj ? (k::Mixed)e.opt.mixed_proj : op::prec().mixed, op::prec().back);
   auto r = op::svd_trunc(c, R1, R2, chi, so);
   Tensor isq = op::inv_sqrt(c, r.S);
   Proj p;
@@ -380,7 +381,7 @@ void init_ctm1(Ctx& c, Env& e) {
 
 std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi) {
   op::prec().mixed = e.opt.mixed;
-  op::prec().back = e.opt.mixed;
+  op::prec().back = e.opt.mixed_back ? (k::Mixed)e.opt.mixed_back : e.opt.mixed;
   op::prec().fused_absorb = e.opt.fused_absorb;
   const uint64_t d0 = e.draws;
   std::vector<double> s;
@@ -414,8 +415,9 @@ static Env as_dtype(Ctx& c, const Env& e, DType dt) {
 
 Tensor energy(Ctx& c, const Env& e0, const Tensor& H1, const Tensor& H2) {
   Env e = (e0.dt == DType::F32 && e0.opt.energy_fp64) ? as_dtype(c, e0, DType::F64) : e0;
-  // the VJP of the contractions at energy_mixed_back (0: the step's setting)
-  op::PrecScope ps(e.opt.energy_mixed, e.opt.energy_mixed_back == 0 ? e.opt.mixed : (k::Mixed)e.opt.energy_mixed_back);
+  // the VJP of the contractions at energy_mixed_back (0: the step's VJP setting)
+  const int step_back = e.opt.mixed_back ? e.opt.mixed_back : (int)e.opt.mixed;
+  op::PrecScope ps(e.opt.energy_mixed, (k::Mixed)(e.opt.energy_mixed_back == 0 ? step_back : e.opt.energy_mixed_back));
   Tensor roh, rov;
   get_dms(c, e, roh, rov);
   if (roh.dt == DType::F32) {

@@ -15,6 +15,8 @@ struct Options {
   op::SvdOpts svd;             // projector SVD (full = reference semantics, or randomized)
   bool canonical_edge = true;  // P2-first edge update (chi^3 D^4) instead of the chi^2 D^6 intermediate
   k::Mixed mixed = k::Mixed::BF16X3;
+  int mixed_back = 0;          // products of the step's VJPs: 1, 3, 6, or 0 = `mixed`
+  int mixed_proj = 0;          // products of the projector (range finder, P1 / P2): 1, 3, 6, or 0 = `mixed`
   // Energy contractions of an F32 environment: FP64 (energy_fp64) or split-bf16
   // with `energy_mixed` products; the trace and the ratios are FP64 either way.
   bool energy_fp64 = false;

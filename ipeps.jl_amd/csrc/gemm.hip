@@ -19,6 +19,8 @@
 //              bf16 16x16x32 (f32 out): C[4*(l/16) + r][l % 16]
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace ipeps {
@@ -721,7 +723,7 @@ static void set_vec_flags(const GemmArgs& g, GemmP& p) {
 }
 
 // ma / mb: operand modes of the FP32 kernel (0 row-contiguous, 1 k-contiguous, 2 pre-split)
-static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid, int ma, int mb) {
+static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid, int ma, int mb, int tn) {
   dim3 block(256);
   const bool at = g.transA, bt = g.transB;
   if (g.dt == DType::F64) {
@@ -749,7 +751,6 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid, int ma,
   } while (0)
 #define IPEPS_F32_LAUNCH(NS, TM192)                                   \
   do {                                                                \
-    const int tn = f32_tile_n(g.N);                                   \
     if (tn == 64) IPEPS_F32_LAUNCH_T(NS, 128, 64);                    \
     else if (tn == 192) IPEPS_F32_LAUNCH_T(NS, TM192, 192);           \
     else IPEPS_F32_LAUNCH_T(NS, 128, 128);                            \
@@ -795,19 +796,39 @@ static std::shared_ptr<Buf> split_operand(Ctx& c, const float* X, int64_t ld, bo
   return buf;
 }
 
-void gemm(Ctx& c, const GemmArgs& g) {
-  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
-  if (g.K <= 0) {
-    IPEPS_CHECK(g.beta == 0.0 || g.beta == 1.0, "gemm: K=0 with beta not in {0,1}");
-    if (g.beta == 0.0) {
-      size_t es = dtype_size(g.dt);
-      for (int64_t b = 0; b < g.batch; ++b)
-        for (int64_t j = 0; j < g.N; ++j)
-          fill_zero(c, (char*)g.C + (b * g.sC + j * g.ldc) * es, g.M * es);
-    }
-    return;
+// A launch plan: the FP32 block tile (by its column count; 64 x 192 tiles for
+// tn = 192, else 128 x tn) and the number of split-K slices.
+struct Plan {
+  int tn = 0;
+  int64_t S = 1;
+};
+
+static int64_t tile_m(const GemmArgs& g, const Plan& pl) { return g.dt == DType::F32 ? (pl.tn == 192 ? 64 : 128) : BM; }
+static int64_t tile_n(const GemmArgs& g, const Plan& pl) { return g.dt == DType::F32 ? pl.tn : BN; }
+static int64_t out_tiles(const GemmArgs& g, const Plan& pl) {
+  return ((g.M + tile_m(g, pl) - 1) / tile_m(g, pl)) * ((g.N + tile_n(g, pl) - 1) / tile_n(g, pl)) * g.batch;
+}
+// Split-K applies to one long-K product (batch 1, K >= 1024) whose output
+// tiles cannot fill the 256 CUs; slices are at least kmin long.
+static bool splitk_allowed(const GemmArgs& g, int64_t tiles) { return g.batch == 1 && tiles < 600 && g.K >= 1024; }
+static int64_t splitk_kmin(const GemmArgs& g) { return g.dt == DType::F32 ? 256 : 64; }
+
+// Shape rules: tile by the output column count; split-K to about 1024 blocks
+// (FP32, chunks >= 8 K tiles) or 2048 blocks (FP64 / C128, latency-bound per
+// K step, chunks >= 64).
+static Plan rule_plan(const GemmArgs& g) {
+  Plan pl;
+  pl.tn = g.dt == DType::F32 ? f32_tile_n(g.N) : BN;
+  const int64_t tiles = out_tiles(g, pl);
+  if (splitk_allowed(g, tiles)) {
+    const int64_t target = g.dt == DType::F32 ? 1024 : 2048;
+    pl.S = std::max<int64_t>(1, std::min<int64_t>((target + tiles - 1) / tiles, g.K / splitk_kmin(g)));
   }
-  IPEPS_CHECK(g.batch <= 65535, "gemm: batch too large");
+  return pl;
+}
+
+// Runs op(A) op(B) with plan `pl` (timed: bracketed for the kernel statistics).
+static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
   GemmP p;
   p.M = g.M; p.N = g.N; p.K = g.K; p.alpha = g.alpha; p.beta = g.beta;
   p.A = g.A; p.lda = g.lda; p.sA = g.sA;
@@ -815,7 +836,7 @@ void gemm(Ctx& c, const GemmArgs& g) {
   p.C = g.C; p.ldc = g.ldc; p.sC = g.sC;
   p.conjA = g.conjA; p.conjB = g.conjB;
   p.kchunk = 0;
-  const int64_t tbm = g.dt == DType::F32 ? f32_tile_m(g.N, g.mixed) : BM, tbn = g.dt == DType::F32 ? f32_tile_n(g.N) : BN;
+  const int64_t tbm = tile_m(g, pl), tbn = tile_n(g, pl);
   set_vec_flags(g, p);
   p.planeA = p.planeB = 0;
   dim3 grid((unsigned)((g.M + tbm - 1) / tbm), (unsigned)((g.N + tbn - 1) / tbn), (unsigned)g.batch);
@@ -860,22 +881,14 @@ void gemm(Ctx& c, const GemmArgs& g) {
   if (c.timing_detail)
     name += " M=" + std::to_string(g.M) + " N=" + std::to_string(g.N) + " K=" + std::to_string(g.K) +
             " b=" + std::to_string(g.batch);
-  // Split-K when the output tiles cannot fill the 256 CUs and K is long:
-  // each z-slice of the grid reduces one K chunk into a workspace, then a
-  // reduction kernel applies alpha / beta.
-  const int64_t tiles = (int64_t)grid.x * grid.y * g.batch;
+  if (!timed) name.clear();
+  // Split-K: each z-slice of the grid reduces one K chunk (a multiple of 64)
+  // into a workspace, then a reduction kernel applies alpha / beta.
   int64_t S = 1, kc = 0;
-  // FP32: about 1024 blocks with K chunks of at least 8 K tiles; FP64 / C128
-  // (short K tiles, latency-bound per step): about 2048 blocks, chunks >= 64.
-  const bool f32 = g.dt == DType::F32;
-  const int64_t kmin = f32 ? 256 : 64, target = f32 ? 1024 : 2048;
-  if (g.batch == 1 && tiles < 600 && g.K >= 1024) {
-    S = std::min<int64_t>((target + tiles - 1) / tiles, g.K / kmin);
-    if (S >= 2) {
-      kc = ((g.K + S - 1) / S + 63) / 64 * 64;
-      S = (g.K + kc - 1) / kc;
-    }
-    if (S > 65535) S = 65535;
+  if (pl.S >= 2) {
+    kc = ((g.K + pl.S - 1) / pl.S + 63) / 64 * 64;
+    S = (g.K + kc - 1) / kc;
+    IPEPS_CHECK(S <= 65535, "gemm: too many split-K slices");
   }
   if (S >= 2) {
     const size_t es = dtype_size(g.dt);
@@ -895,12 +908,12 @@ void gemm(Ctx& c, const GemmArgs& g) {
     grid.z = (unsigned)S;
     {
       TimedLaunch tl(c, name, flops, bytes);
-      launch(c, g, q, grid, ma, mb);
+      launch(c, g, q, grid, ma, mb, pl.tn);
     }
     const int64_t MN = g.M * g.N;
     const char* rname = g.dt == DType::F64 ? "splitk_reduce<double>" : g.dt == DType::C128 ? "splitk_reduce<double2>"
                                                                                           : "splitk_reduce<float>";
-    TimedLaunch tr(c, rname, 0.0, (double)(S + 1) * MN * es);
+    TimedLaunch tr(c, timed ? rname : "", 0.0, (double)(S + 1) * MN * es);
     unsigned rb = (unsigned)std::min<int64_t>((MN + 255) / 256, 4096);
     std::shared_ptr<Buf> ws2;
     if (S > 32) {
@@ -934,7 +947,217 @@ void gemm(Ctx& c, const GemmArgs& g) {
     return;
   }
   TimedLaunch tl(c, name, flops, bytes);
-  launch(c, g, p, grid, ma, mb);
+  launch(c, g, p, grid, ma, mb, pl.tn);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: with tuning on, the first product of each shape (at least
+// `min_flops`) times every candidate plan (FP32 tiles 64 / 128 / 192 columns
+// x split-K slices {1, rule / 2, rule, 2 rule}; FP64 / C128 the slices) on the
+// caller's stream into a scratch output and keeps the fastest.  The table is
+// process-wide (both lanes of a context, every context) and can be saved and
+// loaded, so a run can replay a table and reproduce its arithmetic exactly.
+// ---------------------------------------------------------------------------
+namespace {
+struct Tuner {
+  std::mutex mu;
+  int mode = 0;  // 0 off (shape rules), 1 tune unseen shapes, 2 table only
+  double min_flops = 1e8;
+  std::map<std::string, Plan> table;
+  long tuned = 0, hits = 0;
+};
+Tuner& tuner() {
+  static Tuner t;
+  return t;
+}
+
+std::string tune_key(const GemmArgs& g) {
+  GemmP p;
+  p.A = g.A; p.lda = g.lda; p.sA = g.sA; p.B = g.B; p.ldb = g.ldb; p.sB = g.sB;
+  p.C = g.C; p.ldc = g.ldc; p.sC = g.sC; p.kchunk = 0;
+  set_vec_flags(g, p);
+  char b[200];
+  snprintf(b, sizeof b, "%d.%d.%d%d.%lld.%lld.%lld.%lld.%d.%d%d%d", (int)g.dt, (int)g.mixed, (int)g.transA,
+           (int)g.transB, (long long)g.M, (long long)g.N, (long long)g.K, (long long)g.batch, (int)(g.beta != 0.0),
+           (int)p.vecA, (int)p.vecB, (int)p.vecC);
+  return b;
+}
+
+std::vector<Plan> candidates(const GemmArgs& g) {
+  std::vector<int> tns;
+  if (g.dt == DType::F32) {
+    tns = {64, 128};
+    if (g.N > 128) tns.push_back(192);
+  } else {
+    tns = {(int)BN};
+  }
+  std::vector<Plan> out;
+  for (int tn : tns) {
+    Plan pl;
+    pl.tn = tn;
+    const int64_t tiles = out_tiles(g, pl);
+    std::vector<int64_t> ss = {1};
+    if (splitk_allowed(g, tiles)) {
+      const int64_t smax = g.K / splitk_kmin(g);
+      const int64_t target = g.dt == DType::F32 ? 1024 : 2048;
+      const int64_t r = std::max<int64_t>(1, std::min<int64_t>((target + tiles - 1) / tiles, smax));
+      for (int64_t s : {r / 2, r, 2 * r})
+        if (s >= 2 && s <= smax && std::find(ss.begin(), ss.end(), s) == ss.end()) ss.push_back(s);
+    }
+    for (int64_t s : ss) {
+      pl.S = s;
+      out.push_back(pl);
+    }
+  }
+  return out;
+}
+
+Plan tune(Ctx& c, const GemmArgs& g) {
+  const std::vector<Plan> cand = candidates(g);
+  if (cand.size() == 1) return cand[0];
+  // scratch output with the caller's strides (same vector-access flags)
+  const size_t es = dtype_size(g.dt);
+  const int64_t span = (g.batch - 1) * g.sC + (g.N - 1) * g.ldc + g.M;
+  const size_t bytes = (size_t)span * es;
+  Buf scratch{c.pool->alloc(bytes), bytes, c.pool.get()};
+  GemmArgs t = g;
+  t.C = scratch.p;
+  // the candidates' launches stay out of the kernel statistics
+  struct TimingOff {
+    Ctx& c;
+    bool was;
+    explicit TimingOff(Ctx& cc) : c(cc), was(cc.timing) { c.timing = false; }
+    ~TimingOff() { c.timing = was; }
+  } off(c);
+  hipEvent_t e0, e1;
+  HIP_CHECK(hipEventCreate(&e0));
+  HIP_CHECK(hipEventCreate(&e1));
+  Plan best = cand[0];
+  float best_ms = 1e30f;
+  for (const Plan& pl : cand) {
+    execute(c, t, pl, false);  // warm-up (also the first touch of the pool blocks)
+    float ms = 1e30f;
+    for (int r = 0; r < 2; ++r) {
+      HIP_CHECK(hipEventRecord(e0, c.stream));
+      execute(c, t, pl, false);
+      HIP_CHECK(hipEventRecord(e1, c.stream));
+      HIP_CHECK(hipEventSynchronize(e1));
+      float m = 0;
+      HIP_CHECK(hipEventElapsedTime(&m, e0, e1));
+      ms = std::min(ms, m);
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = pl;
+    }
+  }
+  HIP_CHECK(hipEventDestroy(e0));
+  HIP_CHECK(hipEventDestroy(e1));
+  return best;
+}
+}  // namespace
+
+void tune_config(int mode, double min_flops) {
+  IPEPS_CHECK(mode >= 0 && mode <= 2, "tune mode must be 0, 1 or 2");
+  std::lock_guard<std::mutex> lk(tuner().mu);
+  tuner().mode = mode;
+  tuner().min_flops = min_flops;
+}
+
+void tune_stats(long out[3]) {
+  std::lock_guard<std::mutex> lk(tuner().mu);
+  out[0] = (long)tuner().table.size();
+  out[1] = tuner().tuned;
+  out[2] = tuner().hits;
+}
+
+void tune_clear() {
+  std::lock_guard<std::mutex> lk(tuner().mu);
+  tuner().table.clear();
+  tuner().tuned = tuner().hits = 0;
+}
+
+void tune_save(const std::string& path) {
+  std::lock_guard<std::mutex> lk(tuner().mu);
+  FILE* f = fopen(path.c_str(), "w");
+  IPEPS_CHECK(f, "tune_save: cannot open " + path);
+  fprintf(f, "# ipeps gemm tuning table: key tile_n splitk\n");
+  for (auto& kv : tuner().table) fprintf(f, "%s %d %lld\n", kv.first.c_str(), kv.second.tn, (long long)kv.second.S);
+  fclose(f);
+}
+
+void tune_load(const std::string& path) {
+  FILE* f = fopen(path.c_str(), "r");
+  IPEPS_CHECK(f, "tune_load: cannot open " + path);
+  std::map<std::string, Plan> t;
+  char line[512];
+  int ln = 0;
+  while (fgets(line, sizeof line, f)) {
+    ++ln;
+    if (line[0] == '#' || line[0] == '\n') continue;
+    char key[256];
+    int tn;
+    long long S;
+    if (sscanf(line, "%255s %d %lld", key, &tn, &S) != 3 || !(tn == 64 || tn == 128 || tn == 192) || S < 1 ||
+        S > 65535) {
+      fclose(f);
+      throw Error(IPEPS_ERR_ARG, "tune_load: malformed line " + std::to_string(ln) + " of " + path);
+    }
+    Plan pl;
+    pl.tn = tn;
+    pl.S = S;
+    t[key] = pl;
+  }
+  fclose(f);
+  std::lock_guard<std::mutex> lk(tuner().mu);
+  for (auto& kv : t) tuner().table[kv.first] = kv.second;
+}
+
+void gemm(Ctx& c, const GemmArgs& g) {
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  if (g.K <= 0) {
+    IPEPS_CHECK(g.beta == 0.0 || g.beta == 1.0, "gemm: K=0 with beta not in {0,1}");
+    if (g.beta == 0.0) {
+      size_t es = dtype_size(g.dt);
+      for (int64_t b = 0; b < g.batch; ++b)
+        for (int64_t j = 0; j < g.N; ++j)
+          fill_zero(c, (char*)g.C + (b * g.sC + j * g.ldc) * es, g.M * es);
+    }
+    return;
+  }
+  IPEPS_CHECK(g.batch <= 65535, "gemm: batch too large");
+  Plan pl = rule_plan(g);
+  Tuner& T = tuner();
+  int mode;
+  double min_flops;
+  {
+    std::lock_guard<std::mutex> lk(T.mu);
+    mode = T.mode;
+    min_flops = T.min_flops;
+  }
+  if (mode != 0) {
+    const std::string key = tune_key(g);
+    bool found = false;
+    {
+      std::lock_guard<std::mutex> lk(T.mu);
+      auto it = T.table.find(key);
+      if (it != T.table.end()) {
+        pl = it->second;
+        found = true;
+        ++T.hits;
+      }
+    }
+    const double fl = 2.0 * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
+    if (!found && mode == 1 && fl >= min_flops) {
+      Plan best = tune(c, g);
+      std::lock_guard<std::mutex> lk(T.mu);
+      // two lanes that tuned one shape together keep the first entry
+      auto ins = T.table.emplace(key, best);
+      pl = ins.first->second;
+      if (ins.second) ++T.tuned;
+    }
+  }
+  execute(c, g, pl, true);
 }
 
 }  // namespace k

@@ -33,6 +33,15 @@ struct GemmArgs {
   Mixed mixed = Mixed::BF16X3;
 };
 void gemm(Ctx& c, const GemmArgs& g);
+// GEMM autotuner (process-wide table of launch plans keyed by dtype, split,
+// transposes, shape, beta and operand alignment).  mode 0: shape rules,
+// 1: time the candidate plans of each unseen shape with at least min_flops
+// and keep the fastest, 2: use table entries only (no timing runs).
+void tune_config(int mode, double min_flops);
+void tune_stats(long out[3]);  // entries, shapes tuned, table hits
+void tune_clear();
+void tune_save(const std::string& path);
+void tune_load(const std::string& path);
 
 // out = conj?(in) permuted: out dims[i] = in dims[perm[i]].  Column-major.
 void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::vector<int>& perm,

@@ -18,6 +18,7 @@ X[i1+1, ...]); they are handed to the C ABI in column-major memory order.
 Everything runs on the GPU through libipeps_hip.so; there is no CPU path.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -114,7 +115,8 @@ def _hpair(H):
 
 ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "mixed_cutoff", "fast_orth",
                "fast_svd",
-               "gram_guard", "canonical_edge", "mixed", "energy_fp64", "energy_mixed", "energy_mixed_back",
+               "gram_guard", "canonical_edge", "mixed", "mixed_back", "mixed_proj", "energy_fp64", "energy_mixed",
+               "energy_mixed_back",
                "checkpoint", "fused_absorb", "paired_moves", "draws")
 
 
@@ -313,6 +315,32 @@ def tcon(xs, ind_xs, ctx=None):
     return out
 
 
+def tune(mode=1, min_flops=1e8):
+    """GEMM autotuner, process-wide.  mode 0: tiles and split-K by shape rules;
+    1: the first product of each unseen shape (>= min_flops) times the
+    candidate launch plans and keeps the fastest; 2: table entries only."""
+    check(lib().ipeps_tune_config(int(mode), float(min_flops)))
+
+
+def tune_stats():
+    """(entries, shapes tuned, table hits) of the GEMM tuning table."""
+    out = (C.c_int64 * 3)()
+    check(lib().ipeps_tune_stats(out))
+    return dict(entries=out[0], tuned=out[1], hits=out[2])
+
+
+def tune_clear():
+    check(lib().ipeps_tune_clear())
+
+
+def tune_save(path):
+    check(lib().ipeps_tune_save(os.fsencode(path)))
+
+
+def tune_load(path):
+    check(lib().ipeps_tune_load(os.fsencode(path)))
+
+
 def gemm(A, B, transA=False, conjA=False, transB=False, conjB=False, mixed=None, alpha=1.0, beta=0.0, C=None,
          ctx=None):
     """alpha op(A) op(B) + beta C on the device (FP64 / complex FP64 MFMA, or
@@ -436,4 +464,4 @@ def svd_back(U, S, V, dU=None, dS=None, dV=None, eta=1e-12, ctx=None):
 
 __all__ = ["absorb", "Context", "CTMTensors", "rg_step", "run_ctm", "get_gs_energy", "get_gs_norm", "compute_gs_energy",
            "step_adjoint", "gs_energy_grad", "tcon", "gemm", "svd1", "svd_back", "default_context", "IpepsError",
-           "ENV_OPTIONS"]
+           "ENV_OPTIONS", "tune", "tune_stats", "tune_clear", "tune_save", "tune_load"]

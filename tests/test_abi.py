@@ -80,3 +80,25 @@ def test_make_es_path_matches_files(pkg):
 
     got = {f"{fmt(a)}_{fmt(b)}" for a, b in zip(kx, ky)}
     assert got == tags
+
+
+def test_tune_table_file_handling(pkg, tmp_path):
+    """The GEMM tuning table's text format: an empty table saves and loads,
+    a malformed line and a missing file raise (no GPU call involved)."""
+    pkg.tune_clear()
+    p = tmp_path / "t.txt"
+    pkg.tune_save(str(p))
+    assert p.read_text().startswith("#")
+    pkg.tune_load(str(p))
+    assert pkg.tune_stats()["entries"] == 0
+    p.write_text("0.3.00.10.10.10.1.0.000 96 1\n")
+    with pytest.raises(pkg.IpepsError):
+        pkg.tune_load(str(p))
+    p.write_text("# header\n0.3.00.10.10.10.1.0.000 192 4\n")
+    pkg.tune_load(str(p))
+    assert pkg.tune_stats()["entries"] == 1
+    pkg.tune_clear()
+    with pytest.raises(pkg.IpepsError):
+        pkg.tune_load(str(tmp_path / "missing.txt"))
+    with pytest.raises(pkg.IpepsError):
+        pkg.tune(5)
