@@ -175,7 +175,7 @@ def main():
                     help="bf16 products of the VJPs (0: same as --mixed)")
     ap.add_argument("--mixed-proj", type=int, default=0, choices=[0, 1, 3, 6],
                     help="bf16 products inside the projector (0: same as --mixed)")
-    ap.add_argument("--tune", type=int, default=0, choices=[0, 1],
+    ap.add_argument("--tune", type=int, default=1, choices=[0, 1],
                     help="1: GEMM autotuner (unseen shapes are tuned during the chi-saturation steps)")
     ap.add_argument("--oversample", type=int, default=16)
     ap.add_argument("--power-iters", type=int, default=1)
@@ -247,14 +247,23 @@ def main():
             ts = pkg.CTMTensors(A, {}, dtype=dt, options=dict(base_opts, **extra))
             ts.set_env(Cs, Es)
             same[dt] = pkg.step_adjoint(ts, chi, H)
+        # the method's own spread: FP64 with another seed of the random test matrices
+        ts = pkg.CTMTensors(A, {}, dtype="f64", options=dict(base_opts, seed=98765))
+        ts.set_env(Cs, Es)
+        same["f64_seed"] = pkg.step_adjoint(ts, chi, H)
         dE = abs(same["mixed"][0].real - same["f64"][0].real)
-        gerr = float(np.linalg.norm(same["mixed"][1] - same["f64"][1]) / np.linalg.norm(same["f64"][1]))
-        log(f"[bench] same-input step+adjoint: |dE/site| mixed vs fp64 = {dE:.2e}, grad rel = {gerr:.2e}")
+        gnorm = np.linalg.norm(same["f64"][1])
+        gerr = float(np.linalg.norm(same["mixed"][1] - same["f64"][1]) / gnorm)
+        dE_seed = abs(same["f64_seed"][0].real - same["f64"][0].real)
+        gerr_seed = float(np.linalg.norm(same["f64_seed"][1] - same["f64"][1]) / gnorm)
+        log(f"[bench] same-input step+adjoint: |dE/site| mixed vs fp64 = {dE:.2e}, grad rel = {gerr:.2e}; "
+            f"fp64 vs fp64 with other test matrices: {dE_seed:.2e}, {gerr_seed:.2e}")
         out_extra["fp64_gpu"] = dict(value=a.steps / t_64, ms_per_step=1e3 * t_64 / a.steps,
                                      roofline=kernel_roofline(st_64))
         out_extra["speedup_vs_fp64_gpu"] = t_64 / t_m
         out_extra["dE_per_site_vs_fp64"] = dE
         out_extra["grad_rel_err_vs_fp64"] = gerr
+        out_extra["fp64_test_matrix_spread"] = dict(dE_per_site=dE_seed, grad_rel=gerr_seed)
     tot_mix = sum(v[1] for v in mix_m.values())
     log(f"[bench] kernel mix of one mixed step + adjoint (all launches timed): {tot_mix:.1f} ms")
     for k, v in sorted(mix_m.items(), key=lambda kv: -kv[1][1])[:14]:

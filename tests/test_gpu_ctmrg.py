@@ -352,6 +352,53 @@ def test_gradient_central_fd(pkg):
         assert abs(fd - gf[k]) < 1e-6 * max(1.0, abs(fd)), (k, fd, gf[k])
 
 
+@pytest.mark.parametrize("checkpoint", [0, 1])
+def test_randomized_step_adjoint_central_fd(pkg, checkpoint):
+    """The benchmark unit's gradient (step_adjoint: rg_step with the randomized
+    projector, then the energy, held-constant environment) against central
+    differences (eps=1e-6) of the same function -- the test matrices are fixed
+    by the environment's draw counter, so the forward is a smooth function of A.
+    The adjoint treats the k computed triplets as the truncated SVD of M (the
+    reference's svd_back on them), so it is the derivative of the forward when
+    the range finder resolves the kept subspace: D=3, chi=20, N=180, k=80,
+    four power iterations.  (With k=chi+16 and one power iteration on this
+    state, entries differ from the differences by up to ~1e-3 relative: the
+    range finder's own approximation.)"""
+    M = pkg.models
+    A = M.seeded_tensor((3, 3, 3, 3, 2), 3, unit_max=True)
+    H = M.heisenberg(1.0)
+    chi = 20
+    opts = dict(svd_mode=1, oversample=60, power_iters=4, checkpoint=checkpoint)
+    base = pkg.CTMTensors(A, {}, dtype="f64", options=opts)
+    for _ in range(3):
+        pkg.rg_step(base, chi)
+    Cs, Es = base.Cs, base.Es
+
+    def fresh(x):
+        ts = pkg.CTMTensors(x, {}, dtype="f64", options=opts)
+        ts.set_env(Cs, Es)
+        return ts
+
+    E, g = pkg.step_adjoint(fresh(A), chi, H)
+
+    def f(x):
+        ts = fresh(x)
+        pkg.rg_step(ts, chi)
+        return pkg.get_gs_energy(ts, H)[0].real
+
+    assert abs(f(A) - E.real) < 1e-13
+    rng = np.random.default_rng(1)
+    idx = rng.choice(A.size, 8, replace=False)
+    eps = 1e-6
+    gf = g.reshape(-1, order="F")
+    for k in idx:
+        d = np.zeros(A.size)
+        d[k] = eps
+        d = d.reshape(A.shape, order="F")
+        fd = (f(A + d) - f(A - d)) / (2 * eps)
+        assert abs(fd - gf[k]) < 1e-6 * max(1.0, abs(fd)), (k, fd, gf[k])
+
+
 # ---------------------------------------------------------------- fast modes
 
 def test_randomized_projector_energy(pkg):
