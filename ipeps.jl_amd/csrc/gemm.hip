@@ -953,7 +953,8 @@ static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
 // ---------------------------------------------------------------------------
 // Autotuner: with tuning on, the first product of each shape (at least
 // `min_flops`) times every candidate plan (FP32 tiles 64 / 128 / 192 columns
-// x split-K slices {1, rule / 2, rule, 2 rule}; FP64 / C128 the slices) on the
+// x split-K slices {rule / 2, rule, 2 rule}, and 1 where the rule asks for at
+// most 4; FP64 / C128 the slices) on the
 // caller's stream into a scratch output and keeps the fastest.  The table is
 // process-wide (both lanes of a context, every context) and can be saved and
 // loaded, so a run can replay a table and reproduce its arithmetic exactly.
@@ -996,13 +997,20 @@ std::vector<Plan> candidates(const GemmArgs& g) {
     Plan pl;
     pl.tn = tn;
     const int64_t tiles = out_tiles(g, pl);
-    std::vector<int64_t> ss = {1};
+    // slices around the rule's count; no split only where the rule asks for
+    // at most 4 slices (an unsplit long-K product on a handful of blocks can
+    // run 100x longer than the rule's plan, and timing it costs as much)
+    std::vector<int64_t> ss;
     if (splitk_allowed(g, tiles)) {
       const int64_t smax = g.K / splitk_kmin(g);
       const int64_t target = g.dt == DType::F32 ? 1024 : 2048;
       const int64_t r = std::max<int64_t>(1, std::min<int64_t>((target + tiles - 1) / tiles, smax));
+      if (r <= 4) ss.push_back(1);
       for (int64_t s : {r / 2, r, 2 * r})
         if (s >= 2 && s <= smax && std::find(ss.begin(), ss.end(), s) == ss.end()) ss.push_back(s);
+      if (ss.empty()) ss.push_back(1);
+    } else {
+      ss.push_back(1);
     }
     for (int64_t s : ss) {
       pl.S = s;

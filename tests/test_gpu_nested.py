@@ -103,3 +103,33 @@ def test_optim_es_columns_and_bands(pkg):
     effH, effN, envB = pkg.optim_es(ts, H, e["basis"], 1.0, 1.0, cfg, columns=cols)
     for i in cols:
         assert np.abs(effN[:, i] - es["effN_1_1"][:, i]).max() < 1e-9
+
+
+@pytest.mark.parametrize("steps", [1, 2])
+def test_es_grad_c5_shape_vs_oracle(pkg, steps):
+    """Config-5 shape: D=3, chi=48 (N=432), complex A from seed 4, one tangent
+    vector of D^4 d = 162 entries at momentum (pi/2, pi/2); fixed nested step
+    counts, gH / envB vs the oracle.  Prints the GPU time per column."""
+    import time
+    A = O.seeded_tensor((3, 3, 3, 3, 2), 4).to(torch.complex128)
+    A = A / A.abs().max()
+    cfg0 = dict(chi=48, min_iter=4, max_iter=6, rg_tol=1e-10)
+    with torch.no_grad():
+        _, _, plain, _ = O.compute_gs_energy(A, O.ising(2.0), cfg0)
+    gen = np.random.default_rng(3)
+    Bv = gen.standard_normal(162) + 1j * gen.standard_normal(162)
+    Hs = N.substract_gs_energy(plain, O.ising(2.0))
+    cfg = dict(chi=48, min_iter=steps, max_iter=steps, ad_max_iter=1, rg_tol=0.0)
+    ts = pkg.CTMTensors(plain.A.numpy(), cfg)
+    ts.set_env([c.numpy() for c in plain.Cs], [x.numpy() for x in plain.Es])
+    H = [h.numpy() for h in Hs]
+    pkg.get_es_grad(ts, H, Bv, 0.5, 0.5, cfg)  # warm-up
+    t0 = time.perf_counter()
+    gH, gN, (E4, Nb) = pkg.get_es_grad(ts, H, Bv, 0.5, 0.5, cfg)
+    dt = time.perf_counter() - t0
+    ots = N.NestedCTM(plain.A, plain.Ad, plain.Cs, plain.Es, params=dict(cfg, px=0.5 * math.pi, py=0.5 * math.pi))
+    ogH, ogN = N.get_es_grad(ots, Hs, _t(Bv))
+    rH = np.abs(gH - ogH.numpy()).max() / np.abs(ogH.numpy()).max()
+    rN = np.abs(gN - ogN.numpy()).max() / np.abs(ogN.numpy()).max()
+    print(f"c5 shape, {steps} step(s): GPU {dt*1e3:.1f} ms per column, rel |dgH| {rH:.2e}, rel |dgN| {rN:.2e}")
+    assert rH < 1e-8 and rN < 1e-8
