@@ -37,8 +37,14 @@ constexpr int ABS_UMAX = 36;  // U = D^2 <= 36 (D <= 6)
 constexpr int ABS_PMAX = 2;   // physical extent
 constexpr int ABS_RB = 8;     // rows per chunk (forward)
 constexpr int ABS_RBW = 8;    // rows per chunk (weight gradient)
-constexpr int ABS_NW = 4;     // waves per block (one per SIMD: 512 registers for accumulators and prefetch)
+// Waves per block (one block per CU, LDS-bound): the forward kernel runs two
+// per SIMD (one wave's LDS waits overlap the other's MFMAs; 238 VGPRs at D=6),
+// the weight gradient one per SIMD (its two accumulator sets need ~250 VGPRs
+// with the prefetch of two row chunks; at two waves per SIMD it would spill).
+constexpr int ABS_NW = 8;
 constexpr int ABS_THREADS = 64 * ABS_NW;
+constexpr int ABS_NWW = 4;
+constexpr int ABS_THREADS_W = 64 * ABS_NWW;
 
 __device__ inline v4f mfma4(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
@@ -61,16 +67,16 @@ __device__ inline Legs32 legs32(const AbsorbLegs& g) {
 // with consecutive e read consecutive rows (the stride-1 leg), and every load
 // is unconditional: rows past n1 read row n1 - 1 (their results are never
 // written) and the elements past the chunk end re-read its last element.
-template <int D, int RB>
+template <int D, int RB, int NTH>
 struct RowChunk {
   static constexpr int U = D * D, LD = U + 1, TOT = U * U * RB;
-  static constexpr int NE = (TOT + ABS_THREADS - 1) / ABS_THREADS;
+  static constexpr int NE = (TOT + NTH - 1) / NTH;
   float v[NE];
   __device__ inline void fetch(const float* __restrict__ src, const Legs32& g, int r10, int r2) {
     const float* base = src + r2 * g.s2;
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
-      const int e = min((int)threadIdx.x + j * ABS_THREADS, TOT - 1);
+      const int e = min((int)threadIdx.x + j * NTH, TOT - 1);
       const int rr = e % RB, xy = e / RB;
       const int r1 = min(r10 + rr, g.n1 - 1);
       v[j] = base[r1 * g.s1 + off2<D>(xy % U, g.sx1, g.sx2) + off2<D>(xy / U, g.sy1, g.sy2)];
@@ -79,7 +85,7 @@ struct RowChunk {
   __device__ inline void store(float* __restrict__ dst) const {
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
-      const int e = (int)threadIdx.x + j * ABS_THREADS;
+      const int e = (int)threadIdx.x + j * NTH;
       if (e < TOT) {
         const int rr = e % RB, xy = e / RB;
         dst[rr * U * LD + (xy % U) * LD + xy / U] = v[j];
@@ -124,7 +130,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_fwd_kernel(AbsorbP p) {
     adm[e] = p.Adm[e];
   }
   const ChunkSched sch;
-  RowChunk<D, ABS_RB> rc;
+  RowChunk<D, ABS_RB, ABS_THREADS> rc;
   int ch = sch.chunk(0);
   if (ch < nchunk) rc.fetch(p.X, gx, (ch % nb1) * ABS_RB, ch / nb1);
   for (int t = 0; ch < nchunk; ch = sch.chunk(++t)) {
@@ -204,7 +210,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_fwd_kernel(AbsorbP p) {
 }
 
 template <int D>
-__global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
+__global__ __launch_bounds__(ABS_THREADS_W) void absorb_wgrad_kernel(AbsorbWP p) {
   constexpr int U = D * D, LD = U + 1, NT = (U + 15) / 16;
   __shared__ float xs[ABS_RBW * U * LD];
   __shared__ float gs[ABS_RBW * U * LD];
@@ -213,7 +219,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
   const Legs32 gx = legs32(p.x), gg = legs32(p.g);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  for (int e = threadIdx.x; e < U * U * P; e += ABS_THREADS) {
+  for (int e = threadIdx.x; e < U * U * P; e += ABS_THREADS_W) {
     am[e] = p.Am[e];
     adm[e] = p.Adm[e];
   }
@@ -231,7 +237,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
   const int nb1 = (gx.n1 + ABS_RBW - 1) / ABS_RBW;
   const int nblk = nb1 * gx.n2;
   const ChunkSched sch;
-  RowChunk<D, ABS_RBW> cx, cg;
+  RowChunk<D, ABS_RBW, ABS_THREADS_W> cx, cg;
   int blk = sch.chunk(0);
   if (blk < nblk) {
     cx.fetch(p.X, gx, (blk % nb1) * ABS_RBW, blk / nb1);
@@ -248,7 +254,7 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
       cx.fetch(p.X, gx, (nx % nb1) * ABS_RBW, nx / nb1);
       cg.fetch(p.G, gg, (nx % nb1) * ABS_RBW, nx / nb1);
     }
-    for (int rr = wave; rr < ABS_RBW; rr += ABS_NW) {
+    for (int rr = wave; rr < ABS_RBW; rr += ABS_NWW) {
       if (r10 + rr >= gx.n1) break;
       const float* xr = xs + rr * U * LD;  // X_r[u][v]
       const float* gr = gs + rr * U * LD;  // G_r[a][b]
@@ -335,8 +341,8 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
     }
   }
   // sum the waves in LDS, one physical index at a time (xs and gs each hold
-  // RBW * U * (U + 1) >= ABS_NW * U * U floats)
-  static_assert(ABS_RBW * LD >= ABS_NW * U, "wave reduction buffer");
+  // RBW * U * (U + 1) >= ABS_NWW * U * U floats)
+  static_assert(ABS_RBW * LD >= ABS_NWW * U, "wave reduction buffer");
   float* red = xs;
 #pragma unroll
   for (int pp = 0; pp < ABS_PMAX; ++pp) {
@@ -357,10 +363,10 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_wgrad_kernel(AbsorbWP p) {
     __syncthreads();
     float* outA = p.partial + (int64_t)blockIdx.x * 2 * U * U * P + (int64_t)pp * U * U;
     float* outD = outA + (int64_t)U * U * P;
-    for (int e = threadIdx.x; e < U * U; e += ABS_THREADS) {
+    for (int e = threadIdx.x; e < U * U; e += ABS_THREADS_W) {
       float sa = 0.f, sd = 0.f;
 #pragma unroll
-      for (int w = 0; w < ABS_NW; ++w) {
+      for (int w = 0; w < ABS_NWW; ++w) {
         sa += red[w * U * U + e];
         sd += gs[w * U * U + e];
       }
@@ -414,7 +420,7 @@ void absorb_wgrad(Ctx& c, AbsorbWP p, float* Abar_Adbar) {
   const double rows = (double)p.x.n1 * p.x.n2;
   {
     TimedLaunch tl(c, "absorb_wgrad_kernel", rows * 8.0 * p.U * p.U * p.U * p.P, rows * 8.0 * p.U * p.U);
-    const dim3 blk(ABS_THREADS);
+    const dim3 blk(ABS_THREADS_W);
     if (p.D == 4) hipLaunchKernelGGL((absorb_wgrad_kernel<4>), dim3(grid), blk, 0, c.stream, p);
     else if (p.D == 5) hipLaunchKernelGGL((absorb_wgrad_kernel<5>), dim3(grid), blk, 0, c.stream, p);
     else hipLaunchKernelGGL((absorb_wgrad_kernel<6>), dim3(grid), blk, 0, c.stream, p);
