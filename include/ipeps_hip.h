@@ -112,6 +112,11 @@ int ipeps_env_destroy(ipeps_env* env);
  * randomized projector): environments with equal seed, draws and tensors
  * build identical projectors whatever their dtype. */
 int ipeps_env_set_option(ipeps_env* env, const char* key, double value);
+/* The environment's current position in that stream (the value "draws"
+ * sets), for a copy that must draw the same test matrices: the field the
+ * Julia-side copy of CTMTensors would carry (src/ctm/ctm_tensor.jl:3-18 has
+ * no counterpart; its projectors are deterministic). */
+int ipeps_env_get_draws(ipeps_env* env, int64_t* out);
 /* which: 0..3 = C1..C4, 4..7 = E1..E4, 8 = A.  dims has room for 5. */
 int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd);
 /* Copy out / in (host data in the env's host dtype: F64 or C128). */

@@ -306,6 +306,13 @@ int ipeps_env_set_option(ipeps_env* env, const char* key, double value) {
   });
 }
 
+int ipeps_env_get_draws(ipeps_env* env, int64_t* out) {
+  return guard([&] {
+    IPEPS_CHECK(env && out, "null argument");
+    *out = (int64_t)env->env.draws;
+  });
+}
+
 int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd) {
   return guard([&] {
     IPEPS_CHECK(env && dims && nd, "null argument");

@@ -67,6 +67,10 @@ class Pool {
  public:
   explicit Pool(int device) : device_(device) {}
   ~Pool();
+  // With IPEPS_POOL_POISON=1 in the environment every block handed out is
+  // first filled with 0xFF bytes (NaN in every floating type) on `stream`, so
+  // a kernel that reads memory nothing wrote turns its output into NaN.
+  hipStream_t stream = nullptr;
   void* alloc(size_t bytes);
   void release(void* p, size_t bytes);
   void trim();

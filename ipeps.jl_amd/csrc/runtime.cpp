@@ -1,6 +1,7 @@
 // Context, caching allocator and host/device copies.
 #include <rocblas/rocblas.h>
 
+#include <cstdlib>
 #include <sstream>
 
 #include "common.h"
@@ -29,6 +30,11 @@ void* Pool::alloc(size_t bytes) {
   }
   live_ += b;
   if (live_ > peak_) peak_ = live_;
+  static const bool poison = [] {
+    const char* v = std::getenv("IPEPS_POOL_POISON");
+    return v && v[0] == '1';
+  }();
+  if (poison) HIP_CHECK(hipMemsetAsync(p, 0xFF, b, stream));
   return p;
 }
 
@@ -100,6 +106,7 @@ Ctx* ctx_create(int device) {
   c->device = device;
   HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->pool.reset(new Pool(device));
+  c->pool->stream = c->stream;
   rocblas_handle h;
   if (rocblas_create_handle(&h) != rocblas_status_success) {
     delete c;

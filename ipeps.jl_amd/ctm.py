@@ -141,6 +141,7 @@ class CTMTensors:
         h = C.c_void_p()
         check(lib().ipeps_env_create(self.ctx.h, self.code, _i64(A.shape), _ptr(Af), C.byref(h)))
         self.h = h
+        self.options = {}
         for k, v in (options or {}).items():
             self.set_option(k, v)
 
@@ -154,6 +155,22 @@ class CTMTensors:
 
     def set_option(self, key, value):
         check(lib().ipeps_env_set_option(self.h, key.encode(), float(value)))
+        self.options[key] = value
+
+    def clone(self, ctx=None):
+        """A copy of this environment (A, C, E, options) on ``ctx`` (another
+        context: its own stream, allocator and solver handle)."""
+        t = CTMTensors(self.A, self.Params, dtype=self.code, options=self.options, ctx=ctx)
+        t.set_env(self.Cs, self.Es)
+        t.set_option("draws", self.draws)
+        return t
+
+    @property
+    def draws(self):
+        """Position in the environment's stream of random test matrices."""
+        v = C.c_int64()
+        check(lib().ipeps_env_get_draws(self.h, C.byref(v)))
+        return v.value
 
     def tensor(self, which):
         dims = (C.c_int64 * 5)()

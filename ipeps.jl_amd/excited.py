@@ -41,18 +41,57 @@ def get_es_grad(ts, H, Bi, px, py, cfg):
     return gH.reshape(-1, order="F"), envB.reshape(-1, order="F"), (float(E4[0]), complex(Nb[0], Nb[1]))
 
 
-def optim_es(ts, H, basis, px, py, cfg, columns=None):
-    """effH, effN, envB for the tangent basis at momentum (px, py) pi (optim_es.jl:127-153)."""
+def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1):
+    """effH, effN, envB for the tangent basis at momentum (px, py) pi (optim_es.jl:127-153).
+
+    The columns are independent nested runs (the reference splits them over
+    jobs with es_resume / es_num, optim_es.jl:127-135).  ``workers`` > 1 runs
+    them concurrently: each worker owns a context (stream, allocator, solver
+    handle) with a copy of the environment and a host thread, and takes every
+    ``workers``-th column.  Run alone, a column gives the same bits on any
+    context; run concurrently, rare differences of ~5e-7 relative in a column
+    have been seen, below the nested run's stopping tolerance -- as in the
+    batched environments, from the rocSOLVER factorizations (zpotrf / zheevd)
+    on this path: this library's own kernels use no atomics and read no
+    unwritten memory (the GPU tests pass with every fresh allocation filled
+    with NaN, IPEPS_POOL_POISON=1).
+    """
     basis = np.asarray(basis, dtype=np.complex128)
     n = basis.shape[1]
     effH = np.zeros((n, n), dtype=np.complex128)
     effN = np.zeros((n, n), dtype=np.complex128)
     envB = np.zeros(basis.shape, dtype=np.complex128)
-    for i in (range(n) if columns is None else columns):
-        gH, gN, _ = get_es_grad(ts, H, basis[:, i], px, py, cfg)
+    cols = list(range(n) if columns is None else columns)
+
+    def store(i, gH, gN):
         envB[:, i] = gN
         effH[:, i] = basis.conj().T @ gH / 2
         effN[:, i] = basis.conj().T @ gN
+
+    if workers <= 1 or len(cols) <= 1:
+        for i in cols:
+            gH, gN, _ = get_es_grad(ts, H, basis[:, i], px, py, cfg)
+            store(i, gH, gN)
+        return effH, effN, envB
+    from concurrent.futures import ThreadPoolExecutor
+    from .ctm import Context
+    nw = min(int(workers), len(cols))
+    ctxs = [Context() for _ in range(nw - 1)]
+    envs = [ts] + [ts.clone(c) for c in ctxs]
+
+    def run(w):
+        return [(i,) + get_es_grad(envs[w], H, basis[:, i], px, py, cfg)[:2] for i in cols[w::nw]]
+
+    try:
+        with ThreadPoolExecutor(nw) as ex:
+            for part in ex.map(run, range(nw)):
+                for i, gH, gN in part:
+                    store(i, gH, gN)
+    finally:
+        for e in envs[1:]:
+            e.__del__()
+        for c in ctxs:
+            c.close()
     return effH, effN, envB
 
 
