@@ -796,6 +796,137 @@ static std::shared_ptr<Buf> split_operand(Ctx& c, const float* X, int64_t ld, bo
   return buf;
 }
 
+// ---------------------------------------------------------------------------
+// Low-K FP32 products: C = alpha op(A) op(B) + beta C with K <= 96 and
+// N <= 96 over many rows (the density-matrix absorptions of the energy:
+// M = chi^2 D^2 d, N, K in {D^2, D^2 d}).  Such a product moves 4 (M K + M N)
+// bytes for 2 M N K flops, well below the ridge point of any MFMA type, so it
+// is HBM-bound: FP32 MFMA (v_mfma_f32_16x16x4f32, FP32 products) costs nothing
+// extra and needs no bf16 planes.  Persistent blocks hold op(B) in LDS for the
+// whole launch and walk 64-row tiles of op(A): the tile is read with
+// consecutive threads along the contiguous leg into LDS, each wave computes 16
+// rows x all N columns, and the tile of C is staged in LDS and written one
+// 64-row column segment at a time.
+// ---------------------------------------------------------------------------
+constexpr int LK_BM = 64, LK_KMAX = 96, LK_NMAX = 96;
+
+// LDS (dynamic, sized by the launch): As[k LDA + m] for max(K4, 16 NT) rows
+// (reused as Cs[n LDA + m] for the output), then Bs[k LDB + n] with
+// LDB = 16 NT + 1, zero beyond K and N.
+__host__ __device__ inline int lowk_arows(int K, int N) { return max((K + 3) / 4 * 4, (N + 15) / 16 * 16); }
+__host__ __device__ inline int lowk_lds_floats(int K, int N) {
+  return lowk_arows(K, N) * (LK_BM + 1) + (K + 3) / 4 * 4 * ((N + 15) / 16 * 16 + 1);
+}
+
+__global__ __launch_bounds__(256) void gemm_lowk_f32_kernel(GemmP p, int transA, int transB) {
+  constexpr int LDA = LK_BM + 1, NTMAX = LK_NMAX / 16;
+  extern __shared__ float lk_smem[];
+  const int M = (int)p.M, N = (int)p.N, K = (int)p.K;
+  const int K4 = (K + 3) / 4 * 4, NT = (N + 15) / 16, LDB = 16 * NT + 1;
+  float* As = lk_smem;
+  float* Bs = lk_smem + lowk_arows(K, N) * LDA;
+  const float* A = (const float*)p.A;
+  const float* B = (const float*)p.B;
+  float* C = (float*)p.C;
+  const int lda = (int)p.lda, ldb = (int)p.ldb, ldc = (int)p.ldc;
+  const float alpha = (float)p.alpha, beta = (float)p.beta;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  for (int e = tid; e < K4 * LDB; e += 256) {
+    const int k = e / LDB, n = e - k * LDB;
+    float v = 0.f;
+    if (k < K && n < N) v = transB ? B[n + (int64_t)k * ldb] : B[k + (int64_t)n * ldb];
+    Bs[e] = v;
+  }
+  const int ntile = (M + LK_BM - 1) / LK_BM;
+  // a tile of op(A) as PER values per thread: element e = tid + 256 j is
+  // (k = e / BM, m = e % BM) for A (m contiguous) and (m = e / K4, k = e % K4)
+  // for A^T (k contiguous); every load is issued before any is used, and the
+  // next tile's loads are in flight during the current tile's MFMAs
+  constexpr int PER = LK_KMAX * LK_BM / 256;
+  float v[PER];
+  auto fetch = [&](int t) {
+    const int m0 = t * LK_BM, mrows = min(LK_BM, M - m0);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int e = tid + 256 * j;
+      int k, m;
+      if (!transA) { k = e / LK_BM; m = e - k * LK_BM; }
+      else { m = e / K4; k = e - m * K4; }
+      const bool ok = e < K4 * LK_BM && k < K && m < mrows;
+      const int64_t idx = !transA ? (int64_t)(m0 + m) + (int64_t)k * lda : (int64_t)k + (int64_t)(m0 + m) * lda;
+      const float x = A[ok ? idx : 0];
+      v[j] = ok ? x : 0.f;
+    }
+  };
+  if ((int)blockIdx.x < ntile) fetch(blockIdx.x);
+  for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+    const int m0 = t * LK_BM, mrows = min(LK_BM, M - m0);
+    __syncthreads();  // Bs written (first tile); the previous tile's Cs consumed
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const int e = tid + 256 * j;
+      if (e < K4 * LK_BM) {
+        int k, m;
+        if (!transA) { k = e / LK_BM; m = e - k * LK_BM; }
+        else { m = e / K4; k = e - m * K4; }
+        As[k * LDA + m] = v[j];
+      }
+    }
+    __syncthreads();
+    if (t + (int)gridDim.x < ntile) fetch(t + gridDim.x);
+    v4f32 acc[NTMAX];
+#pragma unroll
+    for (int j = 0; j < NTMAX; ++j) acc[j] = (v4f32){0.f, 0.f, 0.f, 0.f};
+    const int mw = 16 * wave + lr;
+    for (int k0 = 0; k0 < K4; k0 += 4) {
+      const float a = As[(k0 + lg) * LDA + mw];
+#pragma unroll
+      for (int j = 0; j < NTMAX; ++j)
+        if (j < NT) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bs[(k0 + lg) * LDB + 16 * j + lr], acc[j], 0, 0, 0);
+    }
+    __syncthreads();  // every wave is done reading As
+    // accumulator element i of lane l: row 16 wave + 4 lg + i, column 16 j + lr
+#pragma unroll
+    for (int j = 0; j < NTMAX; ++j)
+      if (j < NT)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) As[(16 * j + lr) * LDA + 16 * wave + 4 * lg + i] = acc[j][i];
+    __syncthreads();
+    for (int e = tid; e < N * LK_BM; e += 256) {
+      const int n = e / LK_BM, m = e - n * LK_BM;
+      if (m < mrows) {
+        float* c = C + (int64_t)(m0 + m) + (int64_t)n * ldc;
+        const float o = alpha * As[n * LDA + m];
+        *c = beta != 0.f ? o + beta * *c : o;
+      }
+    }
+  }
+}
+
+static bool lowk_eligible(const GemmArgs& g) {
+  return g.dt == DType::F32 && g.batch == 1 && g.K <= LK_KMAX && g.N <= LK_NMAX && g.M >= 4096 &&
+         g.M < ((int64_t)1 << 31) && g.lda < ((int64_t)1 << 31) && g.ldb < ((int64_t)1 << 31) &&
+         g.ldc < ((int64_t)1 << 31);
+}
+
+static void gemm_lowk(Ctx& c, const GemmArgs& g) {
+  GemmP p{};
+  p.M = g.M; p.N = g.N; p.K = g.K; p.alpha = g.alpha; p.beta = g.beta;
+  p.A = g.A; p.lda = g.lda; p.B = g.B; p.ldb = g.ldb; p.C = g.C; p.ldc = g.ldc;
+  const int64_t ntile = (g.M + LK_BM - 1) / LK_BM;
+  const int lds = lowk_lds_floats((int)g.K, (int)g.N) * 4;
+  // persistent blocks: up to three per CU (139 VGPRs), as many as the LDS holds
+  const int per_cu = std::max(1, std::min(3, (160 * 1024) / lds));
+  const unsigned grid = (unsigned)std::min<int64_t>(ntile, 256 * per_cu);
+  std::string name = "gemm_lowk_f32_kernel";
+  if (c.timing_detail)
+    name += " M=" + std::to_string(g.M) + " N=" + std::to_string(g.N) + " K=" + std::to_string(g.K);
+  TimedLaunch tl(c, name, 2.0 * (double)g.M * g.N * g.K,
+                 ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N * (g.beta != 0.0 ? 2 : 1)) * 4.0);
+  hipLaunchKernelGGL(gemm_lowk_f32_kernel, dim3(grid), dim3(256), lds, c.stream, p, (int)g.transA, (int)g.transB);
+  HIP_CHECK(hipGetLastError());
+}
+
 // A launch plan: the FP32 block tile (by its column count; 64 x 192 tiles for
 // tn = 192, else 128 x tn) and the number of split-K slices.
 struct Plan {
@@ -1134,6 +1265,10 @@ void gemm(Ctx& c, const GemmArgs& g) {
     return;
   }
   IPEPS_CHECK(g.batch <= 65535, "gemm: batch too large");
+  if (lowk_eligible(g)) {
+    gemm_lowk(c, g);
+    return;
+  }
   Plan pl = rule_plan(g);
   Tuner& T = tuner();
   int mode;

@@ -464,7 +464,8 @@ def test_mixed_precision_energy(pkg, mixed):
 
 
 @pytest.mark.parametrize("shape", [(70, 45, 37), (300, 200, 150), (36, 72, 5000), (257, 130, 2051),
-                                   (1030, 160, 3001), (100, 192, 40), (400, 500, 77), (200, 600, 2100)])
+                                   (1030, 160, 3001), (100, 192, 40), (400, 500, 77), (200, 600, 2100),
+                                   (5000, 72, 36), (4097, 17, 5), (12000, 96, 96)])
 @pytest.mark.parametrize("kind", ["f64", "c128", "mixed3", "mixed6", "mixed1"])
 def test_gemm_all_ops(pkg, kind, shape):
     """Every transpose / conjugate combination against numpy.  Shapes reach the
@@ -566,6 +567,25 @@ def test_small_cholesky_route_matches_rocsolver(pkg):
     print("dE", abs(E1 - E2), "grad rel", np.abs(g1 - g2).max() / np.abs(g1).max())
     assert abs(E1 - E2) < 1e-10
     assert np.abs(g1 - g2).max() < 1e-6 * np.abs(g1).max()
+
+
+@pytest.mark.parametrize("shape", [(5000, 72, 36), (4096, 96, 96), (70001, 36, 72), (4100, 1, 1)])
+def test_gemm_lowk_fp32_products(pkg, shape):
+    """FP32 products with K <= 96, N <= 96 over >= 4096 rows run on the low-K
+    kernel with FP32 MFMA products whatever the bf16 split asked for: plain
+    bf16 (mixed=1, ~1e-2 error through the split kernel) must come out at FP32
+    accuracy, for every transpose and with beta."""
+    rng = np.random.default_rng(13)
+    M, N, K = shape
+    for tA in (False, True):
+        for tB in (False, True):
+            A = rng.standard_normal((K, M) if tA else (M, K)).astype(np.float32).astype(np.float64)
+            B = rng.standard_normal((N, K) if tB else (K, N)).astype(np.float32).astype(np.float64)
+            C0 = rng.standard_normal((M, N)).astype(np.float32).astype(np.float64)
+            ref = 0.7 * (A.T if tA else A) @ (B.T if tB else B) + 0.3 * C0
+            got = pkg.gemm(A, B, tA, False, tB, False, mixed=1, alpha=0.7, beta=0.3, C=C0)
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            assert err < 2e-6, (shape, tA, tB, err)
 
 
 @pytest.mark.parametrize("dtype", ["f64", "mixed"])
