@@ -74,12 +74,12 @@ def kernel_roofline(stats, name=None):
     launches, ms, flops, bytes_ = stats[name]
     avg_ms = ms / max(1, launches)
     mfma_peak = None
-    if name.startswith("gemm_f32"):
+    if name.startswith(("gemm_f32mfma", "gemm_lowk_f32", "absorb_")):
+        mfma_peak = PEAK_TFLOPS["f32"]  # FP32 MFMA products
+    elif name.startswith("gemm_f32"):
         # split-bf16: NSPLIT planes give NSPLIT (NSPLIT + 1) / 2 bf16 products per flop
         ns = int(name.split("<")[1].split(",")[2].strip(" >")) if name.count(",") >= 2 else 2
         mfma_peak = PEAK_TFLOPS["bf16"] / (ns * (ns + 1) // 2)
-    elif name.startswith("absorb_"):
-        mfma_peak = PEAK_TFLOPS["f32"]
     elif name.startswith("gemm_"):
         mfma_peak = PEAK_TFLOPS["f64"]
     # The bound follows the launches' algorithmic arithmetic intensity: below

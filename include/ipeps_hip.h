@@ -87,7 +87,8 @@ int ipeps_env_destroy(ipeps_env* env);
  * 1 randomized range finder), "oversample", "power_iters", "full_below",
  * "seed", "eta", "mixed_cutoff" (relative singular-value cutoff of MIXED
  * environments, default 1e-6; F64 / C128 keep the reference's 1e-12),
- * "canonical_edge" (0/1), "mixed" (1, 3 or 6 bf16 products),
+ * "canonical_edge" (0/1), "mixed" (1 or 3 bf16 products per FP32 product, or
+ * 6 = FP32 products on FP32 MFMA),
  * "mixed_back" (0, 1, 3 or 6: bf16 products of the step's VJPs; default
  * 0 = "mixed"), "mixed_proj" (0, 1, 3 or 6: bf16 products inside the
  * projector -- range finder, P1 and P2 -- default 0 = "mixed"),

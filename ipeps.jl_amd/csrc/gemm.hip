@@ -123,24 +123,46 @@ struct TileLoader {
   }
 };
 
-template <bool AT, bool BT>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
+// The same kernel on FP32 data with FP32 MFMA products (v_mfma_f32_16x16x4f32,
+// whose output layout differs: lane l, element r holds C[4 (l/16) + r][l % 16])
+// serves the FP32 products asked at full FP32 accuracy (the bf16x6 setting):
+// one FP32 product per term instead of six bf16 ones and no bf16 planes.
+template <typename T>
+struct RealMfma;
+template <>
+struct RealMfma<double> {
+  typedef v4f64 acc_t;
+  static __device__ inline acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct RealMfma<float> {
+  typedef v4f32 acc_t;
+  static __device__ inline acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <>
+__device__ inline float zero_t<float>() { return 0.f; }
+
+template <typename T, bool AT, bool BT>
+__global__ __launch_bounds__(256) void gemm_real_kernel(GemmP p) {
+  typedef RealMfma<T> MF;
   constexpr int BK = 16;
-  __shared__ double As[2][BK][BM + 1];
-  __shared__ double Bs[2][BK][BN + 1];
+  __shared__ T As[2][BK][BM + 1];
+  __shared__ T Bs[2][BK][BN + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const BlockXYZ bk = xcd_block();
   const int64_t m0 = bk.x * BM, n0 = bk.y * BN;
-  const double* A = (const double*)p.A + bk.z * p.sA;
-  const double* B = (const double*)p.B + bk.z * p.sB;
-  double* C = (double*)p.C + bk.z * p.sC;
+  const T* A = (const T*)p.A + bk.z * p.sA;
+  const T* B = (const T*)p.B + bk.z * p.sB;
+  T* C = (T*)p.C + bk.z * p.sC;
   // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
   const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
 
-  TileLoader<double, BM, BK> la;
-  TileLoader<double, BN, BK> lb;
+  TileLoader<T, BM, BK> la;
+  TileLoader<T, BN, BK> lb;
   la.init(A, p.lda, AT, m0, p.M, tid);
   lb.init(B, p.ldb, !BT, n0, p.N, tid);
   auto stage = [&](int buf) {
@@ -150,11 +172,11 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
     for (int i = 0; i < lb.NI; ++i) Bs[buf][lb.kk[i]][lb.row[i]] = lb.v[i];
   };
 
-  v4f64 acc[2][2];
+  typename MF::acc_t acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = (v4f64){0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 2; ++j) acc[i][j] = (typename MF::acc_t){0, 0, 0, 0};
 
   const int64_t nk = (Keff + BK - 1) / BK, nfull = Keff / BK;
   if (nfull > 0) {
@@ -181,7 +203,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
     }
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double a[2], b[2];
+      T a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) a[i] = As[cur][kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
 #pragma unroll
@@ -189,7 +211,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 2; ++j) acc[i][j] = MF::op(a[i], b[j], acc[i][j]);
     }
     if (more) {
       if (!full) {
@@ -206,13 +228,13 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmP p) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int64_t row = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;
+        int64_t row = m0 + wm * 32 + i * 16 + MF::row(lane, r);
         int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
         if (row < p.M && col < p.N) {
-          double v = p.alpha * acc[i][j][r];
-          double* dst = C + row + col * p.ldc;
-          if (p.beta != 0.0) v += p.beta * *dst;
-          *dst = v;
+          double v = p.alpha * (double)acc[i][j][r];
+          T* dst = C + row + col * p.ldc;
+          if (p.beta != 0.0) v += p.beta * (double)*dst;
+          *dst = (T)v;
         }
       }
 }
@@ -714,6 +736,12 @@ static int f32_tile_m(int64_t N, Mixed) { return f32_tile_n(N) == 192 ? 64 : 128
 
 static bool al16(const void* ptr) { return ((uintptr_t)ptr & 15u) == 0; }
 
+// FP32 data at full FP32 accuracy (the bf16x6 setting) runs on FP32 MFMA
+// (gemm_real_kernel<float>, 64 x 64 tiles); the other FP32 settings on the
+// split-bf16 kernel.
+static bool real_f32(const GemmArgs& g) { return g.dt == DType::F32 && g.mixed == Mixed::BF16X6; }
+static bool split_f32(const GemmArgs& g) { return g.dt == DType::F32 && g.mixed != Mixed::BF16X6; }
+
 // 16-B access flags of the FP32 kernel for operands as given in p
 static void set_vec_flags(const GemmArgs& g, GemmP& p) {
   const bool one = g.batch == 1;
@@ -727,10 +755,15 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid, int ma,
   dim3 block(256);
   const bool at = g.transA, bt = g.transB;
   if (g.dt == DType::F64) {
-    if (!at && !bt) hipLaunchKernelGGL((gemm_f64_kernel<false, false>), grid, block, 0, c.stream, p);
-    else if (at && !bt) hipLaunchKernelGGL((gemm_f64_kernel<true, false>), grid, block, 0, c.stream, p);
-    else if (!at && bt) hipLaunchKernelGGL((gemm_f64_kernel<false, true>), grid, block, 0, c.stream, p);
-    else hipLaunchKernelGGL((gemm_f64_kernel<true, true>), grid, block, 0, c.stream, p);
+    if (!at && !bt) hipLaunchKernelGGL((gemm_real_kernel<double, false, false>), grid, block, 0, c.stream, p);
+    else if (at && !bt) hipLaunchKernelGGL((gemm_real_kernel<double, true, false>), grid, block, 0, c.stream, p);
+    else if (!at && bt) hipLaunchKernelGGL((gemm_real_kernel<double, false, true>), grid, block, 0, c.stream, p);
+    else hipLaunchKernelGGL((gemm_real_kernel<double, true, true>), grid, block, 0, c.stream, p);
+  } else if (real_f32(g)) {
+    if (!at && !bt) hipLaunchKernelGGL((gemm_real_kernel<float, false, false>), grid, block, 0, c.stream, p);
+    else if (at && !bt) hipLaunchKernelGGL((gemm_real_kernel<float, true, false>), grid, block, 0, c.stream, p);
+    else if (!at && bt) hipLaunchKernelGGL((gemm_real_kernel<float, false, true>), grid, block, 0, c.stream, p);
+    else hipLaunchKernelGGL((gemm_real_kernel<float, true, true>), grid, block, 0, c.stream, p);
   } else if (g.dt == DType::C128) {
     if (!at && !bt) hipLaunchKernelGGL((gemm_c128_kernel<false, false>), grid, block, 0, c.stream, p);
     else if (at && !bt) hipLaunchKernelGGL((gemm_c128_kernel<true, false>), grid, block, 0, c.stream, p);
@@ -758,7 +791,7 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid, int ma,
     switch (g.mixed) {
       case Mixed::BF16: IPEPS_F32_LAUNCH(1, 64); break;
       case Mixed::BF16X3: IPEPS_F32_LAUNCH(2, 64); break;
-      case Mixed::BF16X6: IPEPS_F32_LAUNCH(3, 64); break;
+      case Mixed::BF16X6: throw Error(IPEPS_ERR_INTERNAL, "gemm: FP32 products run on gemm_real_kernel<float>");
     }
 #undef IPEPS_F32_LAUNCH
 #undef IPEPS_F32_LAUNCH_T
@@ -767,7 +800,8 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid, int ma,
   HIP_CHECK(hipGetLastError());
 }
 
-static int nsplit_of(Mixed m) { return m == Mixed::BF16 ? 1 : m == Mixed::BF16X3 ? 2 : 3; }
+// bf16 planes of the split kernel (the "6" setting runs on FP32 MFMA instead)
+static int nsplit_of(Mixed m) { return m == Mixed::BF16 ? 1 : 2; }
 
 // Pre-split op(X) (R rows x K; kv: X is k-contiguous, X[k + r ld]) into
 // NSPLIT bf16 planes of R x Kp, Kp = K rounded up to 32; returns the buffer.
@@ -788,8 +822,7 @@ static std::shared_ptr<Buf> split_operand(Ctx& c, const float* X, int64_t ld, bo
     else hipLaunchKernelGGL((split_operand_rows_kernel<NS>), tgrid, dim3(256), 0, c.stream, X, ld, R, K, Kp, P);     \
   } while (0)
   if (ns == 1) IPEPS_SPLIT(1);
-  else if (ns == 2) IPEPS_SPLIT(2);
-  else IPEPS_SPLIT(3);
+  else IPEPS_SPLIT(2);
 #undef IPEPS_SPLIT
   HIP_CHECK(hipGetLastError());
   *Kp_out = Kp;
@@ -934,25 +967,25 @@ struct Plan {
   int64_t S = 1;
 };
 
-static int64_t tile_m(const GemmArgs& g, const Plan& pl) { return g.dt == DType::F32 ? (pl.tn == 192 ? 64 : 128) : BM; }
-static int64_t tile_n(const GemmArgs& g, const Plan& pl) { return g.dt == DType::F32 ? pl.tn : BN; }
+static int64_t tile_m(const GemmArgs& g, const Plan& pl) { return split_f32(g) ? (pl.tn == 192 ? 64 : 128) : BM; }
+static int64_t tile_n(const GemmArgs& g, const Plan& pl) { return split_f32(g) ? pl.tn : BN; }
 static int64_t out_tiles(const GemmArgs& g, const Plan& pl) {
   return ((g.M + tile_m(g, pl) - 1) / tile_m(g, pl)) * ((g.N + tile_n(g, pl) - 1) / tile_n(g, pl)) * g.batch;
 }
 // Split-K applies to one long-K product (batch 1, K >= 1024) whose output
 // tiles cannot fill the 256 CUs; slices are at least kmin long.
 static bool splitk_allowed(const GemmArgs& g, int64_t tiles) { return g.batch == 1 && tiles < 600 && g.K >= 1024; }
-static int64_t splitk_kmin(const GemmArgs& g) { return g.dt == DType::F32 ? 256 : 64; }
+static int64_t splitk_kmin(const GemmArgs& g) { return split_f32(g) ? 256 : 64; }
 
 // Shape rules: tile by the output column count; split-K to about 1024 blocks
 // (FP32, chunks >= 8 K tiles) or 2048 blocks (FP64 / C128, latency-bound per
 // K step, chunks >= 64).
 static Plan rule_plan(const GemmArgs& g) {
   Plan pl;
-  pl.tn = g.dt == DType::F32 ? f32_tile_n(g.N) : BN;
+  pl.tn = split_f32(g) ? f32_tile_n(g.N) : BN;
   const int64_t tiles = out_tiles(g, pl);
   if (splitk_allowed(g, tiles)) {
-    const int64_t target = g.dt == DType::F32 ? 1024 : 2048;
+    const int64_t target = split_f32(g) ? 1024 : 2048;
     pl.S = std::max<int64_t>(1, std::min<int64_t>((target + tiles - 1) / tiles, g.K / splitk_kmin(g)));
   }
   return pl;
@@ -978,7 +1011,7 @@ static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
   // (split_operand) instead of in every block's staging.
   int ma = g.transA ? 1 : 0, mb = g.transB ? 0 : 1;
   std::shared_ptr<Buf> splitA, splitB;
-  if (g.dt == DType::F32 && g.batch == 1 && g.K >= 512) {
+  if (split_f32(g) && g.batch == 1 && g.K >= 512) {
     const int ns = nsplit_of(g.mixed);
     const int64_t Kp = (g.K + 31) / 32 * 32;
     const bool fits = Kp <= (int64_t)64 * 65535;
@@ -1005,6 +1038,7 @@ static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
   double flops = 2.0 * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
   if (g.dt == DType::F64) name = std::string("gemm_f64_kernel<") + tf[at] + ", " + tf[bt] + ">";
   else if (g.dt == DType::C128) { name = std::string("gemm_c128_kernel<") + tf[at] + ", " + tf[bt] + ">"; flops *= 4; }
+  else if (real_f32(g)) name = std::string("gemm_f32mfma_kernel<") + tf[at] + ", " + tf[bt] + ">";
   // FP32: the kernel's template head <operand mode A, operand mode B, bf16 planes>
   else name = "gemm_f32_kernel<" + std::to_string(ma) + ", " + std::to_string(mb) + ", " + std::to_string(nsplit_of(g.mixed)) + ">";
   const double bytes = ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N * (g.beta != 0.0 ? 2 : 1)) *
@@ -1033,8 +1067,8 @@ static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
     q.sC = g.M * g.N;
     q.kchunk = kc;
     // element offset of K chunk 1 (k-contiguous and pre-split operands: kc)
-    q.sA = (g.dt == DType::F32 ? ma != 0 : at) ? kc : kc * g.lda;
-    q.sB = (g.dt == DType::F32 ? mb != 0 : !bt) ? kc : kc * g.ldb;
+    q.sA = (split_f32(g) ? ma != 0 : at) ? kc : kc * g.lda;
+    q.sB = (split_f32(g) ? mb != 0 : !bt) ? kc : kc * g.ldb;
     set_vec_flags(g, q);
     grid.z = (unsigned)S;
     {
@@ -1117,7 +1151,7 @@ std::string tune_key(const GemmArgs& g) {
 
 std::vector<Plan> candidates(const GemmArgs& g) {
   std::vector<int> tns;
-  if (g.dt == DType::F32) {
+  if (split_f32(g)) {
     tns = {64, 128};
     if (g.N > 128) tns.push_back(192);
   } else {
@@ -1134,7 +1168,7 @@ std::vector<Plan> candidates(const GemmArgs& g) {
     std::vector<int64_t> ss;
     if (splitk_allowed(g, tiles)) {
       const int64_t smax = g.K / splitk_kmin(g);
-      const int64_t target = g.dt == DType::F32 ? 1024 : 2048;
+      const int64_t target = split_f32(g) ? 1024 : 2048;
       const int64_t r = std::max<int64_t>(1, std::min<int64_t>((target + tiles - 1) / tiles, smax));
       if (r <= 4) ss.push_back(1);
       for (int64_t s : {r / 2, r, 2 * r})

@@ -8,7 +8,11 @@ namespace k {
 
 // Which split-bf16 arithmetic the FP32 GEMM uses on MFMA:
 //   1 = plain bf16 products, 3 = hi*hi + hi*lo + lo*hi (two-term split),
-//   6 = three-term split (hi, mid, lo) keeping every product above 2^-24.
+//   6 = full FP32 accuracy: FP32 MFMA products (v_mfma_f32_16x16x4f32) in
+//       place of the three-term split (hi, mid, lo) it stood for, which kept
+//       every product above 2^-24.
+// Products with K, N <= 96 over >= 4096 rows use FP32 MFMA products at every
+// setting (HBM-bound, nothing to gain from bf16).
 enum class Mixed : int { BF16 = 1, BF16X3 = 3, BF16X6 = 6 };
 
 // Column-major BLAS-style GEMM:  C = alpha * op(A) * op(B) + beta * C
