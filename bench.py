@@ -26,6 +26,10 @@ WORKLOADS = {
     "small": dict(D=3, chi=32, d=2, seed=2, unit_max=False),
 }
 PEAK_TFLOPS = {"f64": 78.6, "bf16": 2500.0, "f32": 157.3}
+# GEMM plans tuned on an MI355X for this workload (bench.py --tune 1 --save-tune):
+# replaying them makes the arithmetic, and so the mixed-vs-FP64 differences,
+# the same from run to run.
+TUNE_TABLE = os.path.join(ROOT, "ipeps.jl_amd", "tune_tables", "gfx950_ctmrg_D6_chi144.txt")
 PEAK_HBM_GBS = 8000.0
 
 
@@ -175,8 +179,11 @@ def main():
                     help="bf16 products of the VJPs (0: same as --mixed)")
     ap.add_argument("--mixed-proj", type=int, default=0, choices=[0, 1, 3, 6],
                     help="bf16 products inside the projector (0: same as --mixed)")
-    ap.add_argument("--tune", type=int, default=1, choices=[0, 1],
-                    help="1: GEMM autotuner (unseen shapes are tuned during the chi-saturation steps)")
+    ap.add_argument("--tune", type=int, default=2, choices=[0, 1, 2],
+                    help="GEMM launch plans: 0 shape rules, 1 autotune now (unseen shapes are timed during the "
+                         "warm-up steps), 2 replay the tuning table --tune-file (shapes missing from it: rules)")
+    ap.add_argument("--tune-file", default=TUNE_TABLE)
+    ap.add_argument("--save-tune", default=None, help="write the tuning table after the run")
     ap.add_argument("--oversample", type=int, default=16)
     ap.add_argument("--power-iters", type=int, default=1)
     ap.add_argument("--no-fp64", action="store_true")
@@ -205,8 +212,8 @@ def main():
         for _ in range(sat_steps):
             pkg.rg_step(ts, chi)
         E = g = None
-        # with the autotuner on, one more untimed step tunes the adjoint's shapes
-        n_warm = max(1, n_warm) + (1 if a.tune else 0)
+        # while tuning, one more untimed step tunes the adjoint's shapes
+        n_warm = max(1, n_warm) + (1 if a.tune == 1 else 0)
         for i in range(n_warm):
             if i == n_warm - 1:
                 ctx.sync()
@@ -228,7 +235,13 @@ def main():
         return t, E, g, stats, ts, mix
 
     mixed_opts = dict(mixed=a.mixed, mixed_back=a.mixed_back, mixed_proj=a.mixed_proj)
-    if a.tune:
+    if a.tune == 2 and not os.path.exists(a.tune_file):
+        log(f"[bench] no tuning table at {a.tune_file}: tuning now")
+        a.tune = 1
+    if a.tune == 2:
+        pkg.tune_load(a.tune_file)
+        pkg.tune(2)
+    elif a.tune == 1:
         pkg.tune(1, 1e8)
     log(f"[bench] workload {a.workload}: D={D} chi={chi} N={chi*D*D}; mixed {mixed_opts}")
     t_m, E_m, g_m, st_m, ts_m, mix_m = run_path("mixed", a.warmup, a.steps, mixed_opts)
@@ -294,7 +307,9 @@ def main():
                                f"({D},{D},{D},{D},{d}) randn seed {w['seed']}, chi={chi}",
                    "D": D, "chi": chi, "N": chi * D * D, "svd": f"randomized k=chi+{a.oversample} q={a.power_iters}",
                    "mixed_products": a.mixed, "mixed_back": a.mixed_back or a.mixed,
-                   "mixed_proj": a.mixed_proj or a.mixed, "autotune": bool(a.tune)},
+                   "mixed_proj": a.mixed_proj or a.mixed,
+                   "gemm_plans": {0: "shape rules", 1: "autotuned in this run",
+                                  2: "tuning table " + os.path.relpath(a.tune_file, ROOT)}[a.tune]},
         "algorithmic_flops_per_step": F,
         "model_flops_rate_tflops": F * value / 1e12,
         "roofline": roof,
@@ -304,6 +319,8 @@ def main():
     }
     if a.tune:
         res["autotune"] = pkg.tune_stats()
+    if a.save_tune:
+        pkg.tune_save(a.save_tune)
     if not a.no_batch:
         res["batched_c4"] = batch_c4(pkg)
     res.update(out_extra)

@@ -102,3 +102,15 @@ def test_tune_table_file_handling(pkg, tmp_path):
         pkg.tune_load(str(tmp_path / "missing.txt"))
     with pytest.raises(pkg.IpepsError):
         pkg.tune(5)
+
+
+def test_committed_tuning_table_loads(pkg):
+    """bench.py replays the committed GEMM tuning table by default: it must parse."""
+    path = os.path.join(ROOT, "ipeps.jl_amd", "tune_tables", "gfx950_ctmrg_D6_chi144.txt")
+    pkg.tune_clear()
+    try:
+        pkg.tune_load(path)
+        n = sum(1 for l in open(path) if l.strip() and not l.startswith("#"))
+        assert pkg.tune_stats()["entries"] == n > 0
+    finally:
+        pkg.tune_clear()
