@@ -399,6 +399,51 @@ def test_randomized_step_adjoint_central_fd(pkg, checkpoint):
         assert abs(fd - gf[k]) < 1e-6 * max(1.0, abs(fd)), (k, fd, gf[k])
 
 
+def test_config2_gradient_central_fd(pkg):
+    """Config 2 (Heisenberg, A (4,4,4,4,2) from seed 1 scaled to unit max-abs,
+    chi=64, N=1024): 40 CTMRG steps, then the fg! gradient through 2
+    differentiated steps + the energy, against central differences
+    (eps=1e-6) of the same function on 32 entries chosen by seed 1.  The
+    randomized projector resolves the kept subspace (k = chi + 448 = N / 2,
+    q = 4), so its adjoint is the derivative of the forward; the perturbed runs
+    continue the environment's test-matrix stream from the same position.
+    (With k = chi + 64, q = 3 entries differ from the differences by up to
+    1e-3 at this state: the range finder's own approximation.)"""
+    M = pkg.models
+    A0 = M.seeded_tensor((4, 4, 4, 4, 2), 1, unit_max=True)
+    H = M.heisenberg(1.0)
+    cfg = dict(chi=64, min_iter=40, max_iter=40, ad_max_iter=2, rg_tol=0.0)
+    opts = dict(svd_mode=1, oversample=448, power_iters=4)
+    E, g, iters = pkg.gs_energy_grad(A0, H, cfg, dtype="f64", options=opts)
+    x = A0 / np.abs(A0).max()
+    base = pkg.CTMTensors(x, cfg, options=opts)
+    pkg.run_ctm(base, "energy", H)
+    Cs, Es, draws = base.Cs, base.Es, base.draws
+
+    def f(xx):
+        ts = pkg.CTMTensors(xx, cfg, options=opts)
+        ts.set_env(Cs, Es)
+        ts.set_option("draws", draws)
+        for _ in range(2):
+            pkg.rg_step(ts, cfg["chi"])
+        return pkg.get_gs_energy(ts, H)[0].real
+
+    assert abs(f(x) - E.real) < 1e-12, (f(x), E)
+    rng = np.random.default_rng(1)
+    idx = rng.choice(x.size, 32, replace=False)
+    eps = 1e-6
+    gf = g.reshape(-1, order="F")
+    worst = 0.0
+    for k in idx:
+        d = np.zeros(x.size)
+        d[k] = eps
+        d = d.reshape(x.shape, order="F")
+        fd = (f(x + d) - f(x - d)) / (2 * eps)
+        worst = max(worst, abs(fd - gf[k]) / max(1.0, abs(fd)))
+    print("config 2: E", E.real, "iters", iters, "worst |fd - g|", worst)
+    assert worst < 1e-6
+
+
 # ---------------------------------------------------------------- fast modes
 
 def test_randomized_projector_energy(pkg):
