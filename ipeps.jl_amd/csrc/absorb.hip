@@ -61,6 +61,16 @@ __device__ inline Legs32 legs32(const AbsorbLegs& g) {
   return Legs32{g.n1, g.n2, (int)g.s1, (int)g.s2, (int)g.sx1, (int)g.sx2, (int)g.sy1, (int)g.sy2};
 }
 
+// A / Ad gathered into LDS as [u + U (a + U p)] from their strided layout.
+template <int D>
+__device__ inline void gather_site(const float* __restrict__ src, const int (&s)[5], float* dst, int P, int nth) {
+  constexpr int U = D * D;
+  for (int e = threadIdx.x; e < U * U * P; e += nth) {
+    const int u = e % U, r = e / U, a = r % U, pp = r / U;
+    dst[e] = src[(u % D) * s[0] + (u / D) * s[1] + (a % D) * s[2] + (a / D) * s[3] + pp * s[4]];
+  }
+}
+
 // RB rows (r1 = r10 .. r10+RB-1 of one r2) of a strided (r, x, y) tensor,
 // held in registers between the fetch and the LDS store dst[rr][x][y] (row
 // stride U * (U + 1)).  Element e of the chunk is (rr = e % RB, x, y); lanes
@@ -125,10 +135,8 @@ __global__ __launch_bounds__(ABS_THREADS) void absorb_fwd_kernel(AbsorbP p) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lg = lane >> 4;
   const int nb1 = (gx.n1 + ABS_RB - 1) / ABS_RB, nchunk = nb1 * gx.n2;
-  for (int e = threadIdx.x; e < U * U * P; e += ABS_THREADS) {
-    am[e] = p.Am[e];
-    adm[e] = p.Adm[e];
-  }
+  gather_site<D>(p.Am, p.sa, am, P, ABS_THREADS);
+  gather_site<D>(p.Adm, p.sd, adm, P, ABS_THREADS);
   const ChunkSched sch;
   RowChunk<D, ABS_RB, ABS_THREADS> rc;
   int ch = sch.chunk(0);
@@ -219,10 +227,8 @@ __global__ __launch_bounds__(ABS_THREADS_W) void absorb_wgrad_kernel(AbsorbWP p)
   const Legs32 gx = legs32(p.x), gg = legs32(p.g);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  for (int e = threadIdx.x; e < U * U * P; e += ABS_THREADS_W) {
-    am[e] = p.Am[e];
-    adm[e] = p.Adm[e];
-  }
+  gather_site<D>(p.Am, p.sa, am, P, ABS_THREADS_W);
+  gather_site<D>(p.Adm, p.sd, adm, P, ABS_THREADS_W);
   v4f abar[ABS_PMAX][NT][NT];   // [p][ta][tu]: Abar^T[a][u]
   v4f adbar[ABS_PMAX][NT][NT];  // [p][tb][tv]: Adbar^T[b][v]
 #pragma unroll

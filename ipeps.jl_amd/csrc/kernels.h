@@ -149,8 +149,13 @@ struct AbsorbP {
   AbsorbLegs x, y;            // X: rows, (u, v); Y: rows, (a, b)
   const float* X = nullptr;
   float* Y = nullptr;
-  const float* Am = nullptr;  // A as [u + U (a + U p)]
-  const float* Adm = nullptr; // Ad as [v + U (b + U p)]
+  // A and Ad in any layout: element (u1 + D u2, a1 + D a2, p) of A sits at
+  // Am[u1 sa[0] + u2 sa[1] + a1 sa[2] + a2 sa[3] + p sa[4]], likewise Ad
+  // with sd for (v1, v2, b1, b2, p); the kernels gather them into LDS, so the
+  // site tensor needs no permute (its transpose is a swap of strides)
+  const float* Am = nullptr;
+  const float* Adm = nullptr;
+  int sa[5] = {0, 0, 0, 0, 0}, sd[5] = {0, 0, 0, 0, 0};
 };
 struct AbsorbWP {
   int U = 0, P = 0, D = 0;
@@ -159,6 +164,7 @@ struct AbsorbWP {
   const float* G = nullptr;
   const float* Am = nullptr;
   const float* Adm = nullptr;
+  int sa[5] = {0, 0, 0, 0, 0}, sd[5] = {0, 0, 0, 0, 0};  // as in AbsorbP
   float* partial = nullptr;   // set by absorb_wgrad
 };
 bool absorb_supported(int D, int P);

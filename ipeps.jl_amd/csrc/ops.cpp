@@ -1,4 +1,6 @@
 // Differentiable device ops, the contraction planner and the SVD/QR wrappers.
+#include <array>
+
 #include "ops.h"
 
 #include <rocblas/rocblas.h>
@@ -640,9 +642,23 @@ AbsorbPlan plan_absorb(const Tensor& X, const std::vector<int>& lx, const Tensor
   return pl;
 }
 
+// Element strides of a site tensor (dims t) for the legs (u1, u2, a1, a2, p)
+// at positions perm = {u1, u2, a1, a2, p}; transposed: the u and a legs swap.
+std::array<int, 5> site_strides(const Tensor& t, const std::vector<int>& perm, bool transposed) {
+  const auto st = col_strides(t.dims);
+  std::array<int, 5> s;
+  for (int i = 0; i < 5; ++i) s[i] = (int)st[perm[i]];
+  if (transposed) {
+    std::swap(s[0], s[2]);
+    std::swap(s[1], s[3]);
+  }
+  return s;
+}
+
 // raw fused launch: Y (dims ydims) from X through legs xl -> yl
-Tensor absorb_raw(Ctx& c, const Tensor& X, const k::AbsorbLegs& xl, const Tensor& Am, const Tensor& Adm,
-                  const std::vector<int64_t>& ydims, const k::AbsorbLegs& yl, int D, int P) {
+Tensor absorb_raw(Ctx& c, const Tensor& X, const k::AbsorbLegs& xl, const Tensor& A, const std::array<int, 5>& sa,
+                  const Tensor& Ad, const std::array<int, 5>& sd, const std::vector<int64_t>& ydims,
+                  const k::AbsorbLegs& yl, int D, int P) {
   Tensor Y = c.empty(DType::F32, ydims);
   k::AbsorbP p;
   p.U = D * D;
@@ -652,8 +668,12 @@ Tensor absorb_raw(Ctx& c, const Tensor& X, const k::AbsorbLegs& xl, const Tensor
   p.y = yl;
   p.X = X.ptr<float>();
   p.Y = Y.ptr<float>();
-  p.Am = Am.ptr<float>();
-  p.Adm = Adm.ptr<float>();
+  p.Am = A.ptr<float>();
+  p.Adm = Ad.ptr<float>();
+  for (int i = 0; i < 5; ++i) {
+    p.sa[i] = sa[i];
+    p.sd[i] = sd[i];
+  }
   k::absorb_fwd(c, p);
   return Y;
 }
@@ -682,12 +702,12 @@ Tensor absorb(Ctx& c, const Tensor& X, const std::vector<int>& lx, const Tensor&
     return tcon(c, XA, l2xa, Ad, l2d);
   }
   const int U = pl.D * pl.D;
-  Tensor Am = reshape(c, permute(c, A, pl.permA), {U, U, pl.P});
-  Tensor Adm = reshape(c, permute(c, Ad, pl.permAd), {U, U, pl.P});
-  Tensor Y = absorb_raw(c, X, pl.xl, Am, Adm, pl.odims, pl.yl, pl.D, pl.P);
-  if (needs(X) || needs(Am) || needs(Adm)) {
-    auto nX = X.node, nA = Am.node, nD = Adm.node;
-    Tensor Xk = X, Ak = Am, Dk = Adm;
+  // A and Ad are read in place (the kernels gather them through strides)
+  const auto sa = site_strides(A, pl.permA, false), sd = site_strides(Ad, pl.permAd, false);
+  Tensor Y = absorb_raw(c, X, pl.xl, A, sa, Ad, sd, pl.odims, pl.yl, pl.D, pl.P);
+  if (needs(X) || needs(A) || needs(Ad)) {
+    auto nX = X.node, nA = A.node, nD = Ad.node;
+    Tensor Xk = X, Ak = A, Dk = Ad;
     Xk.node.reset();
     Ak.node.reset();
     Dk.node.reset();
@@ -697,8 +717,8 @@ Tensor absorb(Ctx& c, const Tensor& X, const std::vector<int>& lx, const Tensor&
       IPEPS_CHECK(G.dt == DType::F32, "absorb: FP32 cotangent expected");
       if (nX) {
         // Xbar_r = sum_p A_p G_r Ad_p^T: the forward kernel with A, Ad transposed
-        Tensor AmT = permute_raw(c, Ak, {1, 0, 2}), AdmT = permute_raw(c, Dk, {1, 0, 2});
-        acc(c, nX, absorb_raw(c, G, plk.yl, AmT, AdmT, Xk.dims, plk.xl, plk.D, plk.P));
+        acc(c, nX, absorb_raw(c, G, plk.yl, Ak, site_strides(Ak, plk.permA, true), Dk,
+                              site_strides(Dk, plk.permAd, true), Xk.dims, plk.xl, plk.D, plk.P));
       }
       if (nA || nD) {
         Tensor out = c.empty(DType::F32, {2 * U * U * plk.P});
@@ -712,17 +732,29 @@ Tensor absorb(Ctx& c, const Tensor& X, const std::vector<int>& lx, const Tensor&
         w.G = G.ptr<float>();
         w.Am = Ak.ptr<float>();
         w.Adm = Dk.ptr<float>();
+        const auto ska = site_strides(Ak, plk.permA, false), skd = site_strides(Dk, plk.permAd, false);
+        for (int i = 0; i < 5; ++i) {
+          w.sa[i] = ska[i];
+          w.sd[i] = skd[i];
+        }
         k::absorb_wgrad(c, w, out.ptr<float>());
+        // the cotangents come out as (u1, u2, a1, a2, p) [resp. (v1, v2, b1, b2, p)];
+        // back to the site tensor's own leg order
+        auto back = [&](const Tensor& g, const std::vector<int>& perm, const std::vector<int64_t>& dims) {
+          std::vector<int> inv(5);
+          for (int i = 0; i < 5; ++i) inv[perm[i]] = i;
+          Tensor t = g;
+          t.dims = {dims[perm[0]], dims[perm[1]], dims[perm[2]], dims[perm[3]], dims[perm[4]]};
+          return permute_raw(c, t, inv);
+        };
         if (nA) {
           Tensor gA = out;  // first half of out
-          gA.dims = {U, U, plk.P};
-          acc(c, nA, clone(c, gA));
+          acc(c, nA, back(gA, plk.permA, Ak.dims));
         }
         if (nD) {
           Tensor gD = out;
           gD.offset += (size_t)U * U * plk.P * sizeof(float);
-          gD.dims = {U, U, plk.P};
-          acc(c, nD, clone(c, gD));
+          acc(c, nD, back(gD, plk.permAd, Dk.dims));
         }
       }
     });
