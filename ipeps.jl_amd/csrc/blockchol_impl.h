@@ -1,6 +1,7 @@
 // Blocked Cholesky factor and triangular inverse of a small SPD matrix in one
-// workgroup (FP64, n <= 160), shared by the device kernel (elem.hip) and a
-// host emulation used by the CPU tests.
+// workgroup (FP64, n <= 160): the tile layout and the per-element steps of
+// the device kernel in elem.hip, written __host__ __device__ so that they
+// also compile for the CPU.
 //
 // G (n x n, lower triangle read) = L L^T; the kernel returns L^-1 (lower,
 // zeros above).  The matrix is padded to NB 32 x 32 tiles with the identity
