@@ -1,7 +1,7 @@
 // Blocked Cholesky factor and triangular inverse of a small SPD matrix in one
-// workgroup (FP64, n <= 160): the tile layout and the per-element steps of
-// the device kernel in elem.hip, written __host__ __device__ so that they
-// also compile for the CPU.
+// workgroup (FP64, n <= 160): the tile layout and the per-element steps (b),
+// (c) and the inverse phase of the device kernel small_chol_inv_kernel in
+// elem.hip; step (a), the one-wave diagonal-tile factor, is in the kernel.
 //
 // G (n x n, lower triangle read) = L L^T; the kernel returns L^-1 (lower,
 // zeros above).  The matrix is padded to NB 32 x 32 tiles with the identity

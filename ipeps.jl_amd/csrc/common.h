@@ -74,6 +74,14 @@ class Pool {
   void* alloc(size_t bytes);
   void release(void* p, size_t bytes);
   void trim();
+  // Blocks are reused in stream order: a block released by the host becomes
+  // reusable at once, which is safe only while every kernel that touches it
+  // runs on this pool's stream.  While two streams share tensors (the two
+  // lanes of a reverse sweep), hold() parks every released block until the
+  // matching unhold(), which the caller issues only after the streams have
+  // joined; holds nest.
+  void hold();
+  void unhold();
   size_t live_bytes() const { return live_; }
   size_t peak_bytes() const { return peak_; }
 
@@ -90,6 +98,8 @@ class Pool {
   int device_;
   std::mutex mu_;  // blocks of a pool may be released by another thread
   std::multimap<size_t, void*> free_;
+  std::vector<std::pair<size_t, void*>> parked_;  // released under hold()
+  int holds_ = 0;
   size_t live_ = 0, peak_ = 0, cached_ = 0;
 };
 

@@ -976,6 +976,14 @@ static int64_t out_tiles(const GemmArgs& g, const Plan& pl) {
 // tiles cannot fill the 256 CUs; slices are at least kmin long.
 static bool splitk_allowed(const GemmArgs& g, int64_t tiles) { return g.batch == 1 && tiles < 600 && g.K >= 1024; }
 static int64_t splitk_kmin(const GemmArgs& g) { return split_f32(g) ? 256 : 64; }
+// A stored plan is applied only where it is valid for the shape: FP32 tiles of
+// 64 / 128 / 192 columns, and split-K (S >= 2) only where splitk_allowed holds
+// (execute's grid.z indexes the K chunks then, so a batched product would
+// lose every batch but the first).
+static bool plan_valid(const GemmArgs& g, const Plan& pl) {
+  if (split_f32(g) && !(pl.tn == 64 || pl.tn == 128 || pl.tn == 192)) return false;
+  return pl.S == 1 || (pl.S >= 2 && splitk_allowed(g, out_tiles(g, pl)));
+}
 
 // Shape rules: tile by the output column count; split-K to about 1024 blocks
 // (FP32, chunks >= 8 K tiles) or 2048 blocks (FP64 / C128, latency-bound per
@@ -1276,6 +1284,21 @@ void tune_load(const std::string& path) {
       fclose(f);
       throw Error(IPEPS_ERR_ARG, "tune_load: malformed line " + std::to_string(ln) + " of " + path);
     }
+    // key = dtype.mixed.tAtB.M.N.K.batch.beta.vec (tune_key): split-K plans
+    // exist for batch-1 products with K >= 1024 only
+    int kdt, kmx, ktt, kbeta;
+    long long kM, kN, kK, kbatch;
+    if (sscanf(key, "%d.%d.%d.%lld.%lld.%lld.%lld.%d.", &kdt, &kmx, &ktt, &kM, &kN, &kK, &kbatch, &kbeta) != 8 ||
+        kM < 1 || kN < 1 || kK < 1 || kbatch < 1) {
+      fclose(f);
+      throw Error(IPEPS_ERR_ARG, "tune_load: malformed key on line " + std::to_string(ln) + " of " + path);
+    }
+    if (S >= 2 && (kbatch != 1 || kK < 1024)) {
+      fclose(f);
+      throw Error(IPEPS_ERR_ARG, "tune_load: line " + std::to_string(ln) + " of " + path +
+                                     " gives split-K slices to a product with batch " + std::to_string(kbatch) +
+                                     " and K " + std::to_string(kK) + " (split-K needs batch 1, K >= 1024)");
+    }
     Plan pl;
     pl.tn = tn;
     pl.S = S;
@@ -1318,7 +1341,7 @@ void gemm(Ctx& c, const GemmArgs& g) {
     {
       std::lock_guard<std::mutex> lk(T.mu);
       auto it = T.table.find(key);
-      if (it != T.table.end()) {
+      if (it != T.table.end() && plan_valid(g, it->second)) {
         pl = it->second;
         found = true;
         ++T.hits;

@@ -117,8 +117,26 @@ static void back_node(Ctx& c, Node& n) {
 // cotangents each sends to earlier nodes are applied after the join, the
 // second lane's first, each in its own order -- the order of the sequential
 // sweep, which takes the second lane's nodes (recorded last) first.
+// Forward tensors captured by the VJP closures are read by both streams, and
+// either host thread may drop the last reference while the other stream's
+// kernels that read the block are still queued; so both pools park every
+// block released during the sweep and take them back only once the main
+// stream waits for the lane's join event.
 static void backward_pair(Ctx& c, Tape& t, const Tape::Pair& p) {
   Ctx& l = c.second_lane();
+  struct PoolHold {
+    Pool *a, *b;
+    PoolHold(Pool* x, Pool* y) : a(x), b(y) {
+      a->hold();
+      b->hold();
+    }
+    void release() {
+      if (a) a->unhold();
+      if (b) b->unhold();
+      a = b = nullptr;
+    }
+    ~PoolHold() { release(); }
+  } hold(c.pool.get(), l.pool.get());
   hipEvent_t fork, join;
   HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
   HIP_CHECK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
@@ -155,11 +173,17 @@ static void backward_pair(Ctx& c, Tape& t, const Tape::Pair& p) {
     main_err = std::current_exception();
   }
   th.join();
-  (void)hipStreamWaitEvent(c.stream, join, 0);
+  const hipError_t werr = hipStreamWaitEvent(c.stream, join, 0);
   (void)hipEventDestroy(fork);
   (void)hipEventDestroy(join);
   c.kev.insert(c.kev.end(), l.kev.begin(), l.kev.end());
   l.kev.clear();
+  if (werr != hipSuccess) {
+    // without the wait the parked blocks may still be in use: keep them parked
+    // until both streams drain
+    (void)hipDeviceSynchronize();
+  }
+  hold.release();
   if (main_err) std::rethrow_exception(main_err);
   if (lane_err) std::rethrow_exception(lane_err);
   for (auto& e : dl.list) acc(c, e.first, e.second);
@@ -583,6 +607,8 @@ AbsorbPlan plan_absorb(const Tensor& X, const std::vector<int>& lx, const Tensor
     if (j != pAd && Ad.dims[j] != D) return pl;
   }
   if (Ad.dims[pAd] != P || !k::absorb_supported((int)D, (int)P)) return pl;
+  for (int i = 0; i < 2; ++i)
+    if (X.dims[uX[i]] != D || X.dims[vX[i]] != D) return pl;
   // u = i(first shared A leg) + D i(second); same for v, a, b
   auto order2 = [](std::vector<int>& partner, std::vector<int>* other) {
     if (partner[0] > partner[1]) {

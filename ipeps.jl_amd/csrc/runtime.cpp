@@ -8,7 +8,11 @@
 
 namespace ipeps {
 
-Pool::~Pool() { trim(); }
+Pool::~Pool() {
+  trim();
+  for (auto& kv : parked_) (void)hipFree(kv.second);
+  parked_.clear();
+}
 
 void* Pool::alloc(size_t bytes) {
   std::lock_guard<std::mutex> g(mu_);
@@ -41,15 +45,34 @@ void* Pool::alloc(size_t bytes) {
 void Pool::release(void* p, size_t bytes) {
   std::lock_guard<std::mutex> g(mu_);
   size_t b = round(bytes);
-  free_.emplace(b, p);
+  if (holds_ > 0)
+    parked_.emplace_back(b, p);
+  else
+    free_.emplace(b, p);
   live_ -= b;
   cached_ += b;
 }
 
+void Pool::hold() {
+  std::lock_guard<std::mutex> g(mu_);
+  ++holds_;
+}
+
+void Pool::unhold() {
+  std::lock_guard<std::mutex> g(mu_);
+  if (holds_ > 0 && --holds_ == 0) {
+    for (auto& kv : parked_) free_.emplace(kv.first, kv.second);
+    parked_.clear();
+  }
+}
+
 void Pool::trim() {
+  // parked blocks may still be in use on another stream: they stay
+  size_t parked = 0;
+  for (auto& kv : parked_) parked += kv.first;
   for (auto& kv : free_) (void)hipFree(kv.second);
   free_.clear();
-  cached_ = 0;
+  cached_ = parked;
 }
 
 Tensor Ctx::empty(DType dt, const std::vector<int64_t>& dims) {

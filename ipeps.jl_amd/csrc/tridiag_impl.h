@@ -1,6 +1,7 @@
 // Householder reduction of a small real symmetric matrix to tridiagonal form
 // in one workgroup (FP64, n <= 160), and the formation of the orthogonal
-// factor, shared by the device kernels (elem.hip) and a host emulation.
+// factor: the storage scheme and index helpers of the device kernels in
+// elem.hip (sytrd_kernel, orgtr_kernel).
 //
 // Lower storage, LAPACK dsytd2 'L' convention: for i = 0 .. n-3 the reflector
 // H(i) = I - tau_i v v^T (v_0 = 1) maps A(i+1:n, i) to e_i e_1 and is applied

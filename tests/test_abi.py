@@ -94,7 +94,7 @@ def test_tune_table_file_handling(pkg, tmp_path):
     p.write_text("0.3.00.10.10.10.1.0.000 96 1\n")
     with pytest.raises(pkg.IpepsError):
         pkg.tune_load(str(p))
-    p.write_text("# header\n0.3.00.10.10.10.1.0.000 192 4\n")
+    p.write_text("# header\n0.3.00.10.10.2048.1.0.000 192 4\n")
     pkg.tune_load(str(p))
     assert pkg.tune_stats()["entries"] == 1
     pkg.tune_clear()
@@ -114,3 +114,23 @@ def test_committed_tuning_table_loads(pkg):
         assert pkg.tune_stats()["entries"] == n > 0
     finally:
         pkg.tune_clear()
+
+
+def test_tune_load_rejects_split_k_on_batched_products(pkg, tmp_path):
+    """A tuning table that gives split-K slices to a batched product (or to a
+    K < 1024 one) is refused at load: split-K there would compute only the
+    first batch.  Host-only: the table is parsed without touching a device."""
+    from ipeps_amd import _lib
+    lib = _lib.lib()
+    bad = tmp_path / "bad.txt"
+    # key = dtype.mixed.tAtB.M.N.K.batch.beta.vec (batch 4, K 2048), 8 slices
+    bad.write_text("# table\n2.3.00.256.256.2048.4.0.111 128 8\n")
+    assert lib.ipeps_tune_load(str(bad).encode()) != 0
+    assert b"split-K" in lib.ipeps_last_error()
+    short = tmp_path / "short.txt"
+    short.write_text("2.3.00.256.256.512.1.0.111 128 2\n")
+    assert lib.ipeps_tune_load(str(short).encode()) != 0
+    ok = tmp_path / "ok.txt"
+    ok.write_text("2.3.00.256.256.2048.1.0.111 128 8\n2.3.00.256.256.2048.4.0.111 128 1\n")
+    assert lib.ipeps_tune_load(str(ok).encode()) == 0
+    lib.ipeps_tune_clear()
