@@ -195,6 +195,22 @@ int ipeps_absorb(ipeps_ctx* ctx, int dtype, int fused, const int64_t* dimsX, con
                  const int64_t* dimsA, const void* A, const int* la, const void* Ad, const int* lad, const void* G,
                  void* Y_out, void* gX_out, void* gA_out, void* gAd_out);
 
+/* No counterpart in the reference: an allocation-free variant of
+ * ipeps_absorb for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * fused FP32 path only (4 <= D <= 6, d <= 2; other shapes are rejected), on
+ * caller-owned DEVICE pointers of FP32 data, with the labels of ipeps_absorb:
+ * Y (may be NULL when only the VJP is wanted) = X A Ad; with the cotangent G
+ * the VJP goes to gX / gA / gAd (each may be NULL; gA / gAd in A's leg order).
+ * Enqueued on the context's stream (ipeps_ctx_sync waits for it).  Calls no
+ * hipMalloc / hipFree: the weight gradient's partial sums live in a
+ * file-scope device array of at most 5.6 MiB that each GPU holds until the
+ * process ends, shared by every call on that GPU -- calls that overlap on one
+ * GPU (two contexts, two threads) race and must be serialised by the caller. */
+int ipeps_absorb_noalloc(ipeps_ctx* ctx, const int64_t* dimsX, const float* X, const int* lx, const int64_t* dimsA,
+                         const float* A, const int* la, const float* Ad, const int* lad, const float* G, float* Y,
+                         float* gX, float* gA, float* gAd);
+
 /* Matrix product on the device, the GEMM that OMEinsum hands to BLAS for a
  * two-index contraction (wrapper.jl:23): C = alpha op(A) op(B) + beta C,
  * column-major, op = X, X^T, conj(X) or X^H.  dtype IPEPS_F64 / IPEPS_C128

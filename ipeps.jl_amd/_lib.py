@@ -50,6 +50,7 @@ SIGNATURES = {
                              C.c_int64]),
     "ipeps_absorb": (C.c_int, [_P, C.c_int, C.c_int, _I64P, _P, _IP, _I64P, _P, _IP, _P, _IP, _P, _P, _P, _P,
                                _P]),
+    "ipeps_absorb_noalloc": (C.c_int, [_P, _I64P, _P, _IP, _I64P, _P, _IP, _P, _IP, _P, _P, _P, _P, _P]),
     "ipeps_gemm": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64,
                              C.c_int64, C.c_double, _P, C.c_int64, _P, C.c_int64, C.c_double, _P, C.c_int64]),
     "ipeps_svd": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, _P, _P, _DP, _P]),

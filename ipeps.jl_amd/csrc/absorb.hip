@@ -415,14 +415,31 @@ void absorb_fwd(Ctx& c, const AbsorbP& p) {
   HIP_CHECK(hipGetLastError());
 }
 
-void absorb_wgrad(Ctx& c, AbsorbWP p, float* Abar_Adbar) {
+namespace {
+constexpr int ABS_WGRID = 256;  // weight-gradient blocks: one persistent block per CU (multiple of 32)
+constexpr int ABS_RF = 16;      // partials summed per thread in the first reduction pass
+constexpr int ABS_RG = (ABS_WGRID + ABS_RF - 1) / ABS_RF;
+
+// Scratch of the allocation-free entry (absorb_wgrad_scratch): the per-block
+// partial sums, the first reduction pass and the result for the largest
+// supported site, (ABS_WGRID + ABS_RG + 1) x 2 U^2 P floats at U = 36, P = 2
+// (5.6 MiB).  Each GPU
+// the library runs on holds its own copy until the process ends; host code
+// uses only the per-device address taken by absorb_scratch_init.
+constexpr int64_t ABS_SCRATCH_FLOATS = (int64_t)(ABS_WGRID + ABS_RG + 1) * 2 * ABS_UMAX * ABS_UMAX * ABS_PMAX;
+static __device__ __align__(256) float scratch[ABS_SCRATCH_FLOATS];
+
+constexpr int ABS_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+float* scratch_ptr[ABS_MAX_DEVICES] = {};
+
+// the weight gradient with its partial sums in `partial` (ABS_WGRID x n
+// floats) and `partial2` (ABS_RG x n floats)
+void wgrad_into(Ctx& c, AbsorbWP p, float* partial, float* partial2, float* Abar_Adbar) {
   IPEPS_CHECK(absorb_supported(p.D, p.P) && p.U == p.D * p.D, "absorb: unsupported D / d");
-  const int nb1 = (p.x.n1 + ABS_RBW - 1) / ABS_RBW;
-  const int64_t nblk = (int64_t)nb1 * p.x.n2;
-  const int grid = 256;  // one persistent block per CU; the chunk schedule needs a multiple of 32
+  const int grid = ABS_WGRID;
   const int n = 2 * p.U * p.U * p.P;
-  std::shared_ptr<Buf> ws(new Buf{c.pool->alloc((size_t)grid * n * 4), (size_t)grid * n * 4, c.pool.get()});
-  p.partial = (float*)ws->p;
+  p.partial = partial;
   const double rows = (double)p.x.n1 * p.x.n2;
   {
     TimedLaunch tl(c, "absorb_wgrad_kernel", rows * 8.0 * p.U * p.U * p.U * p.P, rows * 8.0 * p.U * p.U);
@@ -432,13 +449,50 @@ void absorb_wgrad(Ctx& c, AbsorbWP p, float* Abar_Adbar) {
     else hipLaunchKernelGGL((absorb_wgrad_kernel<6>), dim3(grid), blk, 0, c.stream, p);
     HIP_CHECK(hipGetLastError());
   }
-  const int F = 16, G = (grid + F - 1) / F;
-  std::shared_ptr<Buf> ws2(new Buf{c.pool->alloc((size_t)G * n * 4), (size_t)G * n * 4, c.pool.get()});
+  const int F = ABS_RF, G = ABS_RG;
   hipLaunchKernelGGL(absorb_reduce_kernel, dim3((n + 255) / 256, G), dim3(256), 0, c.stream, grid, n, F, p.partial,
-                     (float*)ws2->p);
+                     partial2);
   hipLaunchKernelGGL(absorb_reduce_kernel, dim3((n + 255) / 256, 1), dim3(256), 0, c.stream, G, n, G,
-                     (const float*)ws2->p, Abar_Adbar);
+                     (const float*)partial2, Abar_Adbar);
   HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void absorb_wgrad(Ctx& c, AbsorbWP p, float* Abar_Adbar) {
+  const int64_t n = 2 * (int64_t)p.U * p.U * p.P;
+  std::shared_ptr<Buf> ws(new Buf{c.pool->alloc((size_t)ABS_WGRID * n * 4), (size_t)ABS_WGRID * n * 4, c.pool.get()});
+  std::shared_ptr<Buf> ws2(new Buf{c.pool->alloc((size_t)ABS_RG * n * 4), (size_t)ABS_RG * n * 4, c.pool.get()});
+  wgrad_into(c, p, (float*)ws->p, (float*)ws2->p, Abar_Adbar);
+}
+
+void absorb_scratch_init(int device) {
+  // with `device` current: each GPU holds its own copy of `scratch`
+  std::lock_guard<std::mutex> g(scratch_mu);
+  IPEPS_CHECK(device >= 0 && device < ABS_MAX_DEVICES, "absorb scratch: device index out of range");
+  if (scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  IPEPS_CHECK(ptr != nullptr, "absorb scratch: no device address");
+  scratch_ptr[device] = (float*)ptr;
+}
+
+int64_t absorb_scratch_floats() { return ABS_SCRATCH_FLOATS; }
+
+float* absorb_wgrad_scratch(Ctx& c, AbsorbWP p) {
+  const int64_t n = 2 * (int64_t)p.U * p.U * p.P;
+  IPEPS_CHECK(n > 0 && (ABS_WGRID + ABS_RG + 1) * n <= ABS_SCRATCH_FLOATS,
+              "absorb (no allocation): the site needs " + std::to_string((ABS_WGRID + ABS_RG + 1) * n * 4) +
+                  " scratch bytes, more than the " + std::to_string(ABS_SCRATCH_FLOATS * 4) + " available");
+  float* base;
+  {
+    std::lock_guard<std::mutex> g(scratch_mu);
+    IPEPS_CHECK(c.device >= 0 && c.device < ABS_MAX_DEVICES && scratch_ptr[c.device],
+                "absorb (no allocation): scratch of this device not initialised");
+    base = scratch_ptr[c.device];
+  }
+  float* out = base + (ABS_WGRID + ABS_RG) * n;
+  wgrad_into(c, p, base, base + ABS_WGRID * n, out);
+  return out;
 }
 
 }  // namespace k

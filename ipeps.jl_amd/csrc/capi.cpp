@@ -527,6 +527,26 @@ int ipeps_absorb(ipeps_ctx* ctx, int dtype, int fused, const int64_t* dimsX, con
   });
 }
 
+int ipeps_absorb_noalloc(ipeps_ctx* ctx, const int64_t* dimsX, const float* X, const int* lx, const int64_t* dimsA,
+                         const float* A, const int* la, const float* Ad, const int* lad, const float* G, float* Y,
+                         float* gX, float* gA, float* gAd) {
+  return guard([&] {
+    IPEPS_CHECK(ctx && dimsX && lx && dimsA && la && lad, "bad arguments");
+    Ctx& c = *ctx->c;
+    HIP_CHECK(hipSetDevice(c.device));
+    // no timing brackets: they create events and grow the event list
+    struct TimingOff {
+      Ctx& c;
+      bool was;
+      explicit TimingOff(Ctx& cc) : c(cc), was(cc.timing) { c.timing = false; }
+      ~TimingOff() { c.timing = was; }
+    } off(c);
+    op::absorb_noalloc(c, std::vector<int64_t>(dimsX, dimsX + 6), X, std::vector<int>(lx, lx + 6),
+                       std::vector<int64_t>(dimsA, dimsA + 5), A, std::vector<int>(la, la + 5), Ad,
+                       std::vector<int>(lad, lad + 5), G, Y, gX, gA, gAd);
+  });
+}
+
 int ipeps_gemm(ipeps_ctx* ctx, int dtype, int mixed, int transA, int conjA, int transB, int conjB, int64_t M,
                int64_t N, int64_t K, double alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
                double beta, void* C, int64_t ldc) {

@@ -171,6 +171,15 @@ bool absorb_supported(int D, int P);
 void absorb_fwd(Ctx& c, const AbsorbP& p);
 // out (device, 2 U^2 P floats): Abar as [u + U (a + U p)], then Adbar as [v + U (b + U p)]
 void absorb_wgrad(Ctx& c, AbsorbWP p, float* out);
+// The same with its partial sums and its result in the library's per-device
+// scratch array (no allocation; calls that overlap on one GPU race for it):
+// returns the device address of the result, valid until the next call on the
+// device.  The device's scratch address is taken once by absorb_scratch_init,
+// which ctx_create calls with the device current.
+float* absorb_wgrad_scratch(Ctx& c, AbsorbWP p);
+void absorb_scratch_init(int device);
+int64_t absorb_scratch_floats();
+
 
 constexpr int SCHOL_MAXN_HOST = 160;  // largest n of small_chol_inv
 // L^-1 of the Cholesky factor G = L L^T of an SPD n x n FP64 matrix (n <= 160)

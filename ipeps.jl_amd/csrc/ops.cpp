@@ -788,6 +788,73 @@ Tensor absorb(Ctx& c, const Tensor& X, const std::vector<int>& lx, const Tensor&
   return Y;
 }
 
+void absorb_noalloc(Ctx& c, const std::vector<int64_t>& dx, const float* X, const std::vector<int>& lx,
+                    const std::vector<int64_t>& da, const float* A, const std::vector<int>& la, const float* Ad,
+                    const std::vector<int>& lad, const float* G, float* Y, float* gX, float* gA, float* gAd) {
+  // shape-only tensors for the planner (no device memory behind them)
+  Tensor sX, sA, sAd;
+  sX.dt = sA.dt = sAd.dt = DType::F32;
+  sX.dims = dx;
+  sA.dims = da;
+  sAd.dims = da;
+  const AbsorbPlan pl = plan_absorb(sX, lx, sA, la, sAd, lad);
+  IPEPS_CHECK(pl.ok, "absorb (no allocation): the labels and shapes must fit the fused FP32 kernel "
+                     "(X with two open legs and two legs shared with each of A and Ad, A and Ad (D,D,D,D,d) "
+                     "sharing the physical leg, 4 <= D <= 6, d <= 2)");
+  IPEPS_CHECK(X && A && Ad, "absorb (no allocation): X, A and Ad are required");
+  IPEPS_CHECK(Y || (G && (gX || gA || gAd)), "absorb (no allocation): nothing to compute");
+  IPEPS_CHECK(G || !(gX || gA || gAd), "absorb (no allocation): the cotangents need G");
+  const int U = pl.D * pl.D;
+  auto fwd = [&](const float* in, const k::AbsorbLegs& il, bool transposed, float* out, const k::AbsorbLegs& ol) {
+    k::AbsorbP p;
+    p.U = U;
+    p.P = pl.P;
+    p.D = pl.D;
+    p.x = il;
+    p.y = ol;
+    p.X = in;
+    p.Y = out;
+    p.Am = A;
+    p.Adm = Ad;
+    const auto sa = site_strides(sA, pl.permA, transposed), sd = site_strides(sAd, pl.permAd, transposed);
+    for (int i = 0; i < 5; ++i) {
+      p.sa[i] = sa[i];
+      p.sd[i] = sd[i];
+    }
+    k::absorb_fwd(c, p);
+  };
+  if (Y) fwd(X, pl.xl, false, Y, pl.yl);
+  if (!G) return;
+  if (gX) fwd(G, pl.yl, true, gX, pl.xl);  // Xbar_r = sum_p A_p G_r Ad_p^T
+  if (gA || gAd) {
+    k::AbsorbWP w;
+    w.U = U;
+    w.P = pl.P;
+    w.D = pl.D;
+    w.x = pl.xl;
+    w.g = pl.yl;
+    w.X = X;
+    w.G = G;
+    w.Am = A;
+    w.Adm = Ad;
+    const auto ska = site_strides(sA, pl.permA, false), skd = site_strides(sAd, pl.permAd, false);
+    for (int i = 0; i < 5; ++i) {
+      w.sa[i] = ska[i];
+      w.sd[i] = skd[i];
+    }
+    const float* out = k::absorb_wgrad_scratch(c, w);
+    // (u1, u2, a1, a2, p) resp. (v1, v2, b1, b2, p) -> the site tensor's leg order
+    auto back = [&](const float* g, const std::vector<int>& perm, float* dst) {
+      std::vector<int> inv(5);
+      for (int i = 0; i < 5; ++i) inv[perm[i]] = i;
+      std::vector<int64_t> gd = {da[perm[0]], da[perm[1]], da[perm[2]], da[perm[3]], da[perm[4]]};
+      k::permute(c, DType::F32, gd, inv, g, dst, false);
+    };
+    if (gA) back(out, pl.permA, gA);
+    if (gAd) back(out + (int64_t)U * U * pl.P, pl.permAd, gAd);
+  }
+}
+
 Tensor adjoint_matmul(Ctx& c, const Tensor& A, const Tensor& B) {
   return tcon(c, conj(c, A), {1, -1}, B, {1, -2});
 }

@@ -127,6 +127,14 @@ Tensor inv_sqrt(Ctx& c, const Tensor& s);
 // (absorb.hip) with its own VJPs; anything else two tcon contractions.
 Tensor absorb(Ctx& c, const Tensor& X, const std::vector<int>& lx, const Tensor& A, const std::vector<int>& la,
               const Tensor& Ad, const std::vector<int>& lad);
+// absorb's fused FP32 kernels on caller-owned device memory, allocating
+// nothing: Y (if non-null) = X A Ad, and with the cotangent G the VJP into
+// gX / gA / gAd (each may be null).  The weight gradient's partial sums live
+// in the library's per-device scratch array, so calls that overlap on one GPU
+// race.  Shapes outside the fused kernel's are rejected.
+void absorb_noalloc(Ctx& c, const std::vector<int64_t>& dx, const float* X, const std::vector<int>& lx,
+                    const std::vector<int64_t>& da, const float* A, const std::vector<int>& la, const float* Ad,
+                    const std::vector<int>& lad, const float* G, float* Y, float* gX, float* gA, float* gAd);
 Tensor matmul(Ctx& c, const Tensor& A, const Tensor& B);       // A * B
 Tensor adjoint_matmul(Ctx& c, const Tensor& A, const Tensor& B);  // A^H * B
 Tensor cast(Ctx& c, const Tensor& t, DType dt);                   // differentiable F64 <-> F32

@@ -5,6 +5,7 @@
 #include <sstream>
 
 #include "common.h"
+#include "kernels.h"
 
 namespace ipeps {
 
@@ -125,6 +126,8 @@ Ctx* ctx_create(int device) {
   std::string arch = prop.gcnArchName;
   if (arch.rfind("gfx950", 0) != 0)
     throw Error(IPEPS_ERR_STATE, "ctx_create: this library is built for gfx950 only, device is " + arch);
+  // the device's address of the allocation-free entries' scratch array
+  k::absorb_scratch_init(device);
   auto* c = new Ctx();
   c->device = device;
   HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
