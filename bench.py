@@ -119,30 +119,67 @@ def measured_traffic(kernel):
 
 
 def cpu_baseline_sample(D, chi, seed):
-    """Oracle (reference op order) on one left move at the workload's shape."""
+    """The oracle (FP64, CPU) on one left move of the workload's step, forward
+    and adjoint (torch autograd through the reference's svd_back), in the
+    reference op order (M = R1 R2, full LAPACK SVD of the N x N M, the
+    chi^2 D^6 edge intermediate) and in the canonical order (the edge update
+    P2-first), plus the energy with its VJP; step + adjoint = 4 moves + 1
+    energy.  Threads: OMP_NUM_THREADS if set (the CPU share of a GPU box),
+    else every core of the process's affinity mask; both counts reported."""
     import torch
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ctmrg_oracle as O
 
-    threads = min(16, len(os.sched_getaffinity(0)))
+    host_cores = len(os.sched_getaffinity(0))
+    threads = min(host_cores, int(os.environ.get("OMP_NUM_THREADS") or host_cores))
     torch.set_num_threads(threads)
     A = O.seeded_tensor((D, D, D, D, 2), seed)
+    H = O.heisenberg(1.0)
     g = torch.Generator().manual_seed(0)
     Cs = [torch.randn(chi, chi, generator=g, dtype=torch.float64) for _ in range(4)]
     Es = [torch.randn(chi, chi, D, D, generator=g, dtype=torch.float64) for _ in range(4)]
-    ts = O.CTMTensors(A, A.conj(), Cs, Es, dict(chi=chi))
-    with torch.no_grad():
-        t0 = time.perf_counter()
-        P, Pd, _ = O.get_projector_left(ts, chi)
-        O.proj_left(ts, P, Pd)
-        t = time.perf_counter() - t0
-    per_unit = 12.0 * t  # 4 moves x (forward + adjoint ~ 2x forward)
-    return dict(value=1.0 / per_unit, unit="step+grad/s", cores=threads, kind="port",
-                sample=(f"one left move of the D={D} chi={chi} step in the reference op order (R1, R2, "
-                        f"M = R1 R2, full LAPACK gesdd of the {chi*D*D}^2 M, P1/P2, proj_left with the "
-                        f"chi^2 D^6 edge intermediate), {t:.1f} s on {threads} threads; step+adjoint "
-                        f"extrapolated as 4 moves x 3"),
-                move_seconds=t)
+    w = [torch.randn(chi, chi, generator=g, dtype=torch.float64), torch.randn(chi, chi, D, D, generator=g,
+                                                                              dtype=torch.float64),
+         torch.randn(chi, chi, generator=g, dtype=torch.float64)]
+
+    def left_move(order):
+        O.EDGE_ORDER = order
+        try:
+            x = A.clone().requires_grad_(True)
+            ts = O.CTMTensors(x, x.conj(), Cs, Es, dict(chi=chi))
+            t0 = time.perf_counter()
+            ts2, _ = O.left_rg(ts, chi)
+            loss = (w[0] * ts2.Cs[0]).sum() + (w[1] * ts2.Es[3]).sum() + (w[2] * ts2.Cs[3]).sum()
+            t1 = time.perf_counter()
+            loss.backward()
+            t2 = time.perf_counter()
+        finally:
+            O.EDGE_ORDER = "reference"
+        return t1 - t0, t2 - t1
+
+    ref_f, ref_b = left_move("reference")
+    can_f, can_b = left_move("canonical")
+    x = A.clone().requires_grad_(True)
+    ts = O.CTMTensors(x, x.conj(), Cs, Es, dict(chi=chi))
+    t0 = time.perf_counter()
+    E = O.get_gs_energy(ts, H)[0].real
+    t1 = time.perf_counter()
+    E.backward()
+    t2 = time.perf_counter()
+    en_f, en_b = t1 - t0, t2 - t1
+    unit_ref = 4 * (ref_f + ref_b) + en_f + en_b
+    unit_can = 4 * (can_f + can_b) + en_f + en_b
+    return dict(value=1.0 / unit_ref, unit="step+grad/s", cores=threads, host_cores=host_cores, kind="port",
+                sample=(f"oracle, FP64, {threads} threads ({host_cores} cores in the affinity mask): one left move "
+                        f"of the D={D} chi={chi} step forward + adjoint in the reference op order (full LAPACK SVD "
+                        f"of the {chi*D*D}^2 M, chi^2 D^6 edge intermediate) {ref_f:.1f} + {ref_b:.1f} s, canonical "
+                        f"order {can_f:.1f} + {can_b:.1f} s; energy + VJP {en_f:.1f} + {en_b:.1f} s; step + adjoint "
+                        f"= 4 moves + 1 energy"),
+                reference_order=dict(move_forward_s=ref_f, move_adjoint_s=ref_b, unit_s=unit_ref,
+                                     value=1.0 / unit_ref),
+                canonical_order=dict(move_forward_s=can_f, move_adjoint_s=can_b, unit_s=unit_can,
+                                     value=1.0 / unit_can),
+                energy_forward_s=en_f, energy_adjoint_s=en_b)
 
 
 def batch_c4(pkg, steps=10, workers=8):
@@ -177,15 +214,20 @@ def main():
     ap.add_argument("--mixed", type=int, default=3, choices=[1, 3, 6])
     ap.add_argument("--mixed-back", type=int, default=0, choices=[0, 1, 3, 6],
                     help="bf16 products of the VJPs (0: same as --mixed)")
-    ap.add_argument("--mixed-proj", type=int, default=0, choices=[0, 1, 3, 6],
-                    help="bf16 products inside the projector (0: same as --mixed)")
+    ap.add_argument("--mixed-proj", type=int, default=6, choices=[0, 1, 3, 6],
+                    help="bf16 products inside the projector (0: same as --mixed; 6: FP32 products)")
     ap.add_argument("--tune", type=int, default=2, choices=[0, 1, 2],
                     help="GEMM launch plans: 0 shape rules, 1 autotune now (unseen shapes are timed during the "
                          "warm-up steps), 2 replay the tuning table --tune-file (shapes missing from it: rules)")
     ap.add_argument("--tune-file", default=TUNE_TABLE)
     ap.add_argument("--save-tune", default=None, help="write the tuning table after the run")
-    ap.add_argument("--oversample", type=int, default=16)
-    ap.add_argument("--power-iters", type=int, default=1)
+    # k = chi + 144 = 2 chi and 4 power iterations resolve the kept subspace of
+    # this workload's flat spectrum at the cut (FP64 energy within ~1e-10 of
+    # the full SVD's, independent of the test matrices)
+    ap.add_argument("--oversample", type=int, default=144)
+    ap.add_argument("--power-iters", type=int, default=4)
+    ap.add_argument("--complement-iters", type=int, default=0,
+                    help="CG iterations of the adjoint's discarded-subspace term (0: its thin-SVD limit)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -201,7 +243,8 @@ def main():
     A = M.seeded_tensor((D, D, D, D, d), w["seed"], unit_max=w["unit_max"])
     H = M.heisenberg(1.0)
     ctx = pkg.default_context()
-    base_opts = dict(svd_mode=1, oversample=a.oversample, power_iters=a.power_iters)
+    base_opts = dict(svd_mode=1, oversample=a.oversample, power_iters=a.power_iters,
+                     complement_iters=a.complement_iters)
     sat_steps = 2  # chi saturates after the first step (N = D^4 >= chi)
 
     def run_path(dtype, n_warm, n_time, extra):
@@ -277,6 +320,20 @@ def main():
         out_extra["dE_per_site_vs_fp64"] = dE
         out_extra["grad_rel_err_vs_fp64"] = gerr
         out_extra["fp64_test_matrix_spread"] = dict(dE_per_site=dE_seed, grad_rel=gerr_seed)
+        # against a resolved FP64 projector (k = chi + 256, 8 power iterations,
+        # the adjoint's discarded-subspace term by CG): at this state within
+        # ~1e-14 in energy and ~3e-6 in gradient of the full-SVD reference
+        # semantics, which take about a minute per step + adjoint here
+        ts = pkg.CTMTensors(A, {}, dtype="f64", options=dict(svd_mode=1, oversample=256, power_iters=8,
+                                                           complement_iters=60, complement_tol=1e-10))
+        ts.set_env(Cs, Es)
+        E_r, g_r = pkg.step_adjoint(ts, chi, H)
+        gn_r = np.linalg.norm(g_r)
+        out_extra["vs_fp64_resolved"] = {
+            k: dict(dE_per_site=abs(same[k][0].real - E_r.real),
+                    grad_rel=float(np.linalg.norm(same[k][1] - g_r) / gn_r)) for k in ("mixed", "f64")}
+        log(f"[bench] against the resolved FP64 projector: mixed {out_extra['vs_fp64_resolved']['mixed']}, "
+            f"fp64 {out_extra['vs_fp64_resolved']['f64']}")
     tot_mix = sum(v[1] for v in mix_m.values())
     log(f"[bench] kernel mix of one mixed step + adjoint (all launches timed): {tot_mix:.1f} ms")
     for k, v in sorted(mix_m.items(), key=lambda kv: -kv[1][1])[:14]:
@@ -305,7 +362,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"CTMRG rg_step + energy VJP + step adjoint, Heisenberg, A (D,D,D,D,d)="
                                f"({D},{D},{D},{D},{d}) randn seed {w['seed']}, chi={chi}",
-                   "D": D, "chi": chi, "N": chi * D * D, "svd": f"randomized k=chi+{a.oversample} q={a.power_iters}",
+                   "D": D, "chi": chi, "N": chi * D * D,
+                   "svd": f"randomized k=chi+{a.oversample} q={a.power_iters}"
+                          + (f" complement CG {a.complement_iters}" if a.complement_iters else ""),
                    "mixed_products": a.mixed, "mixed_back": a.mixed_back or a.mixed,
                    "mixed_proj": a.mixed_proj or a.mixed,
                    "gemm_plans": {0: "shape rules", 1: "autotuned in this run",

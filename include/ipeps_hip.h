@@ -107,8 +107,16 @@ int ipeps_env_destroy(ipeps_env* env);
  * fallback; 2, the default: CholeskyQR2 on the one-workgroup blocked FP64
  * Cholesky + inverse when k <= 160, else as 1),
  * "fast_svd" (0/1: small SVD from Gram eigenpairs with one-sided Jacobi
- * fallback) and "gram_guard" (s_min/s_max below which the Gram route is not
- * taken, default 1e-4).  "draws" sets the position in the environment's
+ * fallback), "gram_guard" (s_min/s_max below which the Gram route is not
+ * taken, default 1e-4), "power_iters_max" and "resid_tol" (adaptive power
+ * iterations: after "power_iters", while the a-posteriori residual
+ * max_i ||M v_i - s_i u_i|| / s_1 of the kept triplets exceeds resid_tol and
+ * fewer than power_iters_max have run, one more; resid_tol 0 = off), and
+ * "complement_iters" / "complement_tol" (the adjoint's couplings between the
+ * kept triplets and those beyond the computed k: 0 = their s_j -> 0 limit,
+ * the thin-SVD terms; > 0 = solved exactly by conjugate gradients, at most
+ * that many iterations, to the relative residual complement_tol, default
+ * 1e-8).  "draws" sets the position in the environment's
  * stream of random test matrices (keyed by "seed"; it advances by one per
  * randomized projector): environments with equal seed, draws and tensors
  * build identical projectors whatever their dtype. */
@@ -118,6 +126,15 @@ int ipeps_env_set_option(ipeps_env* env, const char* key, double value);
  * Julia-side copy of CTMTensors would carry (src/ctm/ctm_tensor.jl:3-18 has
  * no counterpart; its projectors are deterministic). */
 int ipeps_env_get_draws(ipeps_env* env, int64_t* out);
+/* Counters of the environment's randomized projectors since the last reset
+ * (engine instrumentation; the reference's projectors are exact SVDs,
+ * src/ctm/ctmrg.jl:254-265): out = [projectors built, power iterations
+ * (sum), power iterations (largest in one projector), largest final
+ * a-posteriori residual / s_1, checks made, discarded-subspace solves in the
+ * adjoint, their CG iterations (sum), largest final relative CG residual,
+ * solves abandoned for a non-positive-definite operator (thin terms used)].
+ * reset != 0 clears them after the read. */
+int ipeps_env_projector_stats(ipeps_env* env, double out[9], int reset);
 /* which: 0..3 = C1..C4, 4..7 = E1..E4, 8 = A.  dims has room for 5. */
 int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd);
 /* Copy out / in (host data in the env's host dtype: F64 or C128). */

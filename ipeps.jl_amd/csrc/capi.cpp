@@ -113,6 +113,12 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
   else if (key == "fast_orth") o.svd.fast_orth = (int)v;
   else if (key == "fast_svd") o.svd.fast_svd = (int)v;
   else if (key == "gram_guard") o.svd.gram_guard = v;
+  else if (key == "power_iters_max") o.svd.power_iters_max = (int)v;
+  else if (key == "resid_tol") o.svd.resid_tol = v;
+  else if (key == "complement_iters") {
+    IPEPS_CHECK(v >= 0, "complement_iters must be >= 0");
+    o.svd.complement_iters = (int)v;
+  } else if (key == "complement_tol") o.svd.complement_tol = v;
   else if (key == "canonical_edge") o.canonical_edge = v != 0.0;
   else if (key == "energy_fp64") o.energy_fp64 = v != 0.0;
   else if (key == "checkpoint") o.checkpoint = v != 0.0;
@@ -277,6 +283,7 @@ int ipeps_env_create(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const voi
     e->dtype = dtype;
     e->env.dt = dev_dt(dtype);
     e->env.opt.mixed = k::Mixed::BF16X3;
+    e->env.opt.svd.stats = std::make_shared<op::SvdStats>();
     try {
       TapeScope ts(nullptr);
       e->env.A = upload(c, host_dt(dtype), dev_dt(dtype), {dims[0], dims[1], dims[2], dims[3], dims[4]}, A);
@@ -310,6 +317,27 @@ int ipeps_env_get_draws(ipeps_env* env, int64_t* out) {
   return guard([&] {
     IPEPS_CHECK(env && out, "null argument");
     *out = (int64_t)env->env.draws;
+  });
+}
+
+int ipeps_env_projector_stats(ipeps_env* env, double out[9], int reset) {
+  return guard([&] {
+    IPEPS_CHECK(env && out, "null argument");
+    auto& st = env->env.opt.svd.stats;
+    if (!st) st = std::make_shared<op::SvdStats>();
+    {
+      std::lock_guard<std::mutex> g(st->mu);
+      out[0] = (double)st->calls;
+      out[1] = (double)st->power_iters;
+      out[2] = (double)st->power_max;
+      out[3] = st->resid_max;
+      out[4] = (double)st->checks;
+      out[5] = (double)st->cg_calls;
+      out[6] = (double)st->cg_iters;
+      out[7] = st->cg_resid_max;
+      out[8] = (double)st->cg_fallbacks;
+    }
+    if (reset) st->reset();
   });
 }
 

@@ -195,6 +195,32 @@ __global__ void scale_cols_kernel(int64_t rows, int64_t cols, const T* X, const 
   }
 }
 
+__device__ inline float addv(float a, float b) { return a + b; }
+__device__ inline double addv(double a, double b) { return a + b; }
+__device__ inline double2 addv(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+// per-column ratio a_c = num[c] / den[c] (0 where den[c] == 0)
+__device__ inline double col_ratio(const double* num, const double* den, int64_t c) {
+  const double d = den[c];
+  return d != 0.0 ? num[c] / d : 0.0;
+}
+// X[:, c] += sign a_c P[:, c]
+template <typename T>
+__global__ void col_axpy_ratio_kernel(int64_t rows, int64_t cols, const double* num, const double* den, double sign,
+                                      const T* P, T* X) {
+  const int64_t n = rows * cols;
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    X[i] = addv(X[i], scal(P[i], sign * col_ratio(num, den, i / rows)));
+}
+// P[:, c] = R[:, c] + a_c P[:, c]
+template <typename T>
+__global__ void col_xpby_ratio_kernel(int64_t rows, int64_t cols, const double* num, const double* den, const T* R,
+                                      T* P) {
+  const int64_t n = rows * cols;
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
+    P[i] = addv(R[i], scal(P[i], col_ratio(num, den, i / rows)));
+}
+
 // counter-based normal generator (splitmix64 + Box-Muller)
 __device__ inline uint64_t splitmix(uint64_t x) {
   x += 0x9e3779b97f4a7c15ull;
@@ -815,6 +841,26 @@ void scale_cols(Ctx& c, DType dt, int64_t rows, int64_t cols, const void* X, con
   int64_t n = rows * cols;
   if (n <= 0) return;
 #define CK(T, ...) hipLaunchKernelGGL(scale_cols_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, rows, cols, (const T*)X, s, (T*)Y, (int)rowwise)
+  DISPATCH3(dt, CK, 0);
+#undef CK
+  HIP_CHECK(hipGetLastError());
+}
+
+void col_axpy_ratio(Ctx& c, DType dt, int64_t rows, int64_t cols, const double* num, const double* den, double sign,
+                    const void* P, void* X) {
+  const int64_t n = rows * cols;
+  if (n <= 0) return;
+#define CK(T, ...) hipLaunchKernelGGL(col_axpy_ratio_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, rows, cols, num, den, sign, (const T*)P, (T*)X)
+  DISPATCH3(dt, CK, 0);
+#undef CK
+  HIP_CHECK(hipGetLastError());
+}
+
+void col_xpby_ratio(Ctx& c, DType dt, int64_t rows, int64_t cols, const double* num, const double* den, const void* R,
+                    void* P) {
+  const int64_t n = rows * cols;
+  if (n <= 0) return;
+#define CK(T, ...) hipLaunchKernelGGL(col_xpby_ratio_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, rows, cols, num, den, (const T*)R, (T*)P)
   DISPATCH3(dt, CK, 0);
 #undef CK
   HIP_CHECK(hipGetLastError());

@@ -63,6 +63,15 @@ void convert(Ctx& c, DType from, DType to, int64_t n, const void* x, void* y);  
 // vector s (FP64): Y = X diag(s) (cols) or Y = diag(s) X (rows).
 void scale_cols(Ctx& c, DType dt, int64_t rows, int64_t cols, const void* X, const double* s,
                 void* Y, bool rowwise);
+// Column-wise updates of a block of vectors with per-column ratios
+// a_c = num[c] / den[c] (FP64 device vectors; a_c = 0 where den[c] = 0):
+//   col_axpy_ratio: X[:, c] += sign a_c P[:, c]
+//   col_xpby_ratio: P[:, c] = R[:, c] + a_c P[:, c]
+// (the steps of a conjugate-gradient solve with one system per column).
+void col_axpy_ratio(Ctx& c, DType dt, int64_t rows, int64_t cols, const double* num, const double* den, double sign,
+                    const void* P, void* X);
+void col_xpby_ratio(Ctx& c, DType dt, int64_t rows, int64_t cols, const double* num, const double* den, const void* R,
+                    void* P);
 // Gaussian N x k test matrix from a counter-based generator (deterministic).
 void randn(Ctx& c, DType dt, int64_t n, uint64_t seed, void* out);
 
@@ -179,7 +188,6 @@ void absorb_wgrad(Ctx& c, AbsorbWP p, float* out);
 float* absorb_wgrad_scratch(Ctx& c, AbsorbWP p);
 void absorb_scratch_init(int device);
 int64_t absorb_scratch_floats();
-
 
 constexpr int SCHOL_MAXN_HOST = 160;  // largest n of small_chol_inv
 // L^-1 of the Cholesky factor G = L L^T of an SPD n x n FP64 matrix (n <= 160)

@@ -1289,6 +1289,108 @@ static Tensor first_n(Ctx& c, const Tensor& s, int64_t n) {
   return r;
 }
 
+// Discarded-subspace term of svd_back (src/utility/autodiff.jl:41-75) beyond
+// the k computed triplets.  With the full factorisation and dU, dV zero past
+// the n kept columns, the pairs (kept i, discarded j) contribute
+//   sum_j u_j [s_i u_j^H du_i + s_j v_j^H dv_i] / (s_i^2 - s_j^2) v_i^H
+//       + u_i ( sum_j v_j [s_j u_j^H du_i + s_i v_j^H dv_i] / (s_i^2 - s_j^2) )^H
+// = x_i v_i^H + u_i y_i^H, and (x_i, y_i) solves
+//   s_i x - Mp y = a_i,   s_i y - Mp^H x = b_i,
+// Mp = (I - Ub Ub^H) M (I - Vb Vb^H), a = (I - Ub Ub^H) dU, b = (I - Vb Vb^H) dV
+// (expand x, y over the discarded u_j, v_j: the coefficients above are the
+// solution, term by term).  The block operator [s_i, -Mp; -Mp^H, s_i] is
+// Hermitian with eigenvalues s_i -+ s_j, positive definite while s_i exceeds
+// every singular value of Mp, so one conjugate-gradient solve per column runs
+// on the whole block at once (M applied as R1 (R2 .), never formed).  X, Y
+// hold the starting point (a / s_i, b / s_i: the thin-SVD terms) and receive
+// the solution; on a non-positive curvature they are left at that start.
+static void complement_cg(Ctx& c, DType dt, DType sdt, const Tensor& R1, const Tensor& R2, const Tensor& Ub,
+                          const Tensor& Vb, const Tensor& Sn, const Tensor& a, const Tensor& b, Tensor& X,
+                          Tensor& Y, int iters, double tol, const std::shared_ptr<SvdStats>& stats) {
+  const int64_t m = a.dims[0], nc = b.dims[0], n = a.dims[1];
+  auto proj_out = [&](const Tensor& B, const Tensor& Z) {  // (I - B B^H) Z
+    Tensor t = matmul_raw(c, B, true, true, Z, false, false);
+    Tensor r = clone(c, Z);
+    Tensor BT = matmul_raw(c, B, false, false, t, false, false);
+    k::axpy(c, sdt, r.numel(), -1.0, BT.data(), r.data());
+    return r;
+  };
+  auto apply = [&](const Tensor& Xi, const Tensor& Yi, Tensor& AX, Tensor& AY) {
+    Tensor Yp = to_dtype(c, proj_out(Vb, Yi), dt);
+    Tensor MY = proj_out(Ub, to_dtype(c, matmul_raw(c, R1, false, false, matmul_raw(c, R2, false, false, Yp, false,
+                                                                                      false), false, false), sdt));
+    Tensor Xp = to_dtype(c, proj_out(Ub, Xi), dt);
+    Tensor MX = proj_out(Vb, to_dtype(c, matmul_raw(c, R2, true, true, matmul_raw(c, R1, true, true, Xp, false,
+                                                                                     false), false, false), sdt));
+    AX = c.empty(sdt, {m, n});
+    AY = c.empty(sdt, {nc, n});
+    k::scale_cols(c, sdt, m, n, Xi.data(), Sn.ptr<double>(), AX.data(), false);
+    k::scale_cols(c, sdt, nc, n, Yi.data(), Sn.ptr<double>(), AY.data(), false);
+    k::axpy(c, sdt, AX.numel(), -1.0, MY.data(), AX.data());
+    k::axpy(c, sdt, AY.numel(), -1.0, MX.data(), AY.data());
+  };
+  auto coldot = [&](const Tensor& P1, const Tensor& Q1, const Tensor& P2, const Tensor& Q2, Tensor& out) {
+    Tensor t = c.empty(DType::F64, {n});
+    k::col_re_dot(c, sdt, m, n, P1.data(), Q1.data(), out.ptr<double>(), false);
+    k::col_re_dot(c, sdt, nc, n, P2.data(), Q2.data(), t.ptr<double>(), false);
+    k::axpy(c, DType::F64, n, 1.0, t.data(), out.data());
+  };
+  Tensor X0 = clone(c, X), Y0 = clone(c, Y);
+  Tensor AX, AY;
+  apply(X, Y, AX, AY);
+  Tensor Rx = clone(c, a), Ry = clone(c, b);
+  k::axpy(c, sdt, Rx.numel(), -1.0, AX.data(), Rx.data());
+  k::axpy(c, sdt, Ry.numel(), -1.0, AY.data(), Ry.data());
+  Tensor Px = clone(c, Rx), Py = clone(c, Ry);
+  Tensor rr = c.empty(DType::F64, {n}), rr_new = c.empty(DType::F64, {n}), pap = c.empty(DType::F64, {n});
+  Tensor bb = c.empty(DType::F64, {n});
+  coldot(Rx, Rx, Ry, Ry, rr);
+  coldot(a, a, b, b, bb);
+  std::vector<double> hb(n), hr(n), hp(n);
+  HIP_CHECK(hipMemcpyAsync(hb.data(), bb.data(), n * 8, hipMemcpyDeviceToHost, c.stream));
+  int it = 0;
+  double rel = 0.0;
+  bool bad = false;
+  const int check_every = 4;
+  while (it < iters) {
+    Tensor APx, APy;
+    apply(Px, Py, APx, APy);
+    coldot(Px, APx, Py, APy, pap);
+    k::col_axpy_ratio(c, sdt, m, n, rr.ptr<double>(), pap.ptr<double>(), 1.0, Px.data(), X.data());
+    k::col_axpy_ratio(c, sdt, nc, n, rr.ptr<double>(), pap.ptr<double>(), 1.0, Py.data(), Y.data());
+    k::col_axpy_ratio(c, sdt, m, n, rr.ptr<double>(), pap.ptr<double>(), -1.0, APx.data(), Rx.data());
+    k::col_axpy_ratio(c, sdt, nc, n, rr.ptr<double>(), pap.ptr<double>(), -1.0, APy.data(), Ry.data());
+    coldot(Rx, Rx, Ry, Ry, rr_new);
+    k::col_xpby_ratio(c, sdt, m, n, rr_new.ptr<double>(), rr.ptr<double>(), Rx.data(), Px.data());
+    k::col_xpby_ratio(c, sdt, nc, n, rr_new.ptr<double>(), rr.ptr<double>(), Ry.data(), Py.data());
+    std::swap(rr, rr_new);
+    ++it;
+    if (it % check_every == 0 || it == iters) {
+      HIP_CHECK(hipMemcpyAsync(hr.data(), rr.data(), n * 8, hipMemcpyDeviceToHost, c.stream));
+      HIP_CHECK(hipMemcpyAsync(hp.data(), pap.data(), n * 8, hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+      rel = 0.0;
+      for (int64_t j = 0; j < n; ++j) {
+        if (!(hp[j] > 0.0) && hr[j] > 0.0) bad = true;  // non-positive curvature
+        if (!std::isfinite(hr[j])) bad = true;
+        if (hb[j] > 0.0) rel = std::max(rel, std::sqrt(std::max(hr[j], 0.0) / hb[j]));
+      }
+      if (bad || rel < tol) break;
+    }
+  }
+  if (bad) {
+    HIP_CHECK(hipMemcpyAsync(X.data(), X0.data(), X.bytes(), hipMemcpyDeviceToDevice, c.stream));
+    HIP_CHECK(hipMemcpyAsync(Y.data(), Y0.data(), Y.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  }
+  if (stats) {
+    std::lock_guard<std::mutex> g(stats->mu);
+    ++stats->cg_calls;
+    stats->cg_iters += it;
+    if (bad) ++stats->cg_fallbacks;
+    else stats->cg_resid_max = std::max(stats->cg_resid_max, rel);
+  }
+}
+
 uint64_t draw_seed(uint64_t base, uint64_t i) {
   uint64_t z = base + (i + 1) * 0x9e3779b97f4a7c15ull;
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -1325,17 +1427,58 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     };
     const int fo = o.fast_orth;
     Tensor Q = orth(c, to_dtype(c, apply_M(Om), sdt), fo);
-    for (int q = 0; q < o.power_iters; ++q) {
+    int q = 0;
+    for (; q < o.power_iters; ++q) {
       Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), fo);
       Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), fo);
     }
-    Tensor Bt = to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt);  // nc x kk = M^H Q
-    Tensor W, s, X;
-    // Bt = W diag(s) X^H  ->  M ~ (Q X) diag(s) W^H
-    svd_tall(c, Bt, o.fast_svd ? 1 : 0, o.gram_guard, W, s, X);
-    Uf = matmul_raw(c, Q, false, false, X, false, false);
-    Sf = s;
-    Vf = W;
+    const int qmax = std::max(o.power_iters, o.power_iters_max);
+    double resid = -1.0;
+    for (;;) {
+      Tensor Bt = to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt);  // nc x kk = M^H Q
+      Tensor W, s, X;
+      // Bt = W diag(s) X^H  ->  M ~ (Q X) diag(s) W^H
+      svd_tall(c, Bt, o.fast_svd ? 1 : 0, o.gram_guard, W, s, X);
+      Uf = matmul_raw(c, Q, false, false, X, false, false);
+      Sf = s;
+      Vf = W;
+      if (o.resid_tol <= 0.0 || q >= qmax) break;
+      // a-posteriori check of the triplets the cutoff keeps: M^H u_i = s_i v_i
+      // by construction, so the residual is r_i = M v_i - s_i u_i
+      std::vector<double> sh(kk);
+      HIP_CHECK(hipMemcpyAsync(sh.data(), s.data(), kk * 8, hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+      const double rel0 = dt == DType::F32 ? std::max(1e-12, o.mixed_cutoff) : 1e-12;
+      const int64_t nk = std::max<int64_t>(1, std::min<int64_t>(cutoff_rule(sh, chi_eff, rel0), kk));
+      Tensor Rm = to_dtype(c, apply_M(to_dtype(c, first_cols(c, Vf, nk), dt)), sdt);  // m x nk
+      Tensor US = c.empty(sdt, {m, nk});
+      k::scale_cols(c, sdt, m, nk, Uf.data(), s.ptr<double>(), US.data(), false);
+      k::axpy(c, sdt, m * nk, -1.0, US.data(), Rm.data());
+      Tensor rr = c.empty(DType::F64, {nk});
+      k::col_re_dot(c, sdt, m, nk, Rm.data(), Rm.data(), rr.ptr<double>(), false);
+      std::vector<double> rh(nk);
+      HIP_CHECK(hipMemcpyAsync(rh.data(), rr.data(), nk * 8, hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+      double rmax = 0.0;
+      for (double v : rh) rmax = std::max(rmax, std::sqrt(std::max(v, 0.0)));
+      resid = sh[0] > 0.0 ? rmax / sh[0] : 0.0;
+      if (o.stats) {
+        std::lock_guard<std::mutex> g(o.stats->mu);
+        ++o.stats->checks;
+      }
+      if (!(resid > o.resid_tol)) break;
+      // one more power iteration from Bt = M^H Q
+      Tensor Z = orth(c, Bt, fo);
+      Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), fo);
+      ++q;
+    }
+    if (o.stats) {
+      std::lock_guard<std::mutex> g(o.stats->mu);
+      ++o.stats->calls;
+      o.stats->power_iters += q;
+      o.stats->power_max = std::max<long>(o.stats->power_max, q);
+      if (resid >= 0.0) o.stats->resid_max = std::max(o.stats->resid_max, resid);
+    }
   }
   const int64_t ns = Sf.numel();
   out.s_host.resize(ns);
@@ -1373,6 +1516,9 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     // beyond k are treated as zero.
     Tensor Ub = Uf, Vb = Vf, Sb = Sf;
     const k::Mixed mx = prec().back;
+    const int cg_iters = full ? 0 : o.complement_iters;
+    const double cg_tol = o.complement_tol;
+    const std::shared_ptr<SvdStats> stats = o.stats;
     base->back = [=](Ctx& c, Node& b) {
       PrecScope ps(mx);
       // svd_back (src/utility/autodiff.jl:41-75) for U (m x kb), V (nc x kb):
@@ -1407,6 +1553,16 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
         k::axpy(c, sdt, T2.numel(), -1.0, T2.data(), dV.data());
         X2 = c.empty(sdt, {nc, n});
         k::scale_cols(c, sdt, nc, n, dV.data(), Sinv.ptr<double>(), X2.data(), false);
+      }
+      if (!sq_u && !sq_v && cg_iters > 0) {
+        // the couplings to the triplets beyond the computed k, exactly:
+        // dU' and X2 (their s_j -> 0 limit) start the solve and are replaced
+        // by its solution; dU still holds (I - Ub Ub^H) dU_orig Sinv
+        Tensor a = c.empty(sdt, {m, n}), bb = c.empty(sdt, {nc, n});
+        Tensor Sn = first_n(c, Sb, n);
+        k::scale_cols(c, sdt, m, n, dU.data(), Sn.ptr<double>(), a.data(), false);
+        k::scale_cols(c, sdt, nc, n, X2.data(), Sn.ptr<double>(), bb.data(), false);
+        complement_cg(c, dt, sdt, R1k, R2k, Ub, Vb, Sn, a, bb, dU, X2, cg_iters, cg_tol, stats);
       }
       // Mbar = X1 Vb^H + [dU' Vn^H] + [Un X2^H] = L F^H with the factors side by
       // side, L = [X1 | dU' | Un] (m x K3), F = [Vb | Vn | X2] (nc x K3); applied

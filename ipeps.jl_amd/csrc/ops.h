@@ -147,6 +147,26 @@ struct SVDResult {
   int64_t n = 0;
 };
 
+// Counters of the randomized projectors of one environment (shared by the
+// two lanes of a paired step, hence the lock).
+struct SvdStats {
+  std::mutex mu;
+  long calls = 0;          // randomized projectors built
+  long power_iters = 0;    // power iterations, summed over calls
+  long power_max = 0;      // largest count in one call
+  double resid_max = 0.0;  // largest final a-posteriori residual (relative to s_1)
+  long checks = 0;         // a-posteriori checks made
+  long cg_calls = 0;       // discarded-subspace solves in the adjoint
+  long cg_iters = 0;       // their CG iterations, summed
+  double cg_resid_max = 0.0;  // largest final relative CG residual
+  long cg_fallbacks = 0;   // solves abandoned (operator not positive definite): thin term used
+  void reset() {
+    std::lock_guard<std::mutex> g(mu);
+    calls = power_iters = power_max = checks = cg_calls = cg_iters = cg_fallbacks = 0;
+    resid_max = cg_resid_max = 0.0;
+  }
+};
+
 // Options of the truncated SVD of the half-system matrix.
 struct SvdOpts {
   int mode = 0;          // 0 = full SVD of M = R1 R2 (reference semantics), 1 = randomized on (R1, R2)
@@ -172,6 +192,20 @@ struct SvdOpts {
   int fast_orth = 2;
   int fast_svd = 1;
   double gram_guard = 1e-4;
+  // Adaptive power iterations: after the first power_iters, while
+  // resid_tol > 0 and fewer than power_iters_max have run, the kept triplets
+  // are checked a posteriori -- max_i ||M v_i - s_i u_i|| / s_1 over the n
+  // triplets the cutoff keeps (M^H u_i = s_i v_i holds by construction) --
+  // and one more power iteration runs while the check exceeds resid_tol.
+  int power_iters_max = 0;  // <= power_iters: no adaptive iterations
+  double resid_tol = 0.0;
+  // Adjoint: the discarded-subspace term of svd_back beyond the k computed
+  // triplets.  0: its s_j -> 0 limit (the thin-SVD terms); > 0: solved by
+  // conjugate gradients, at most complement_iters iterations, to a relative
+  // residual complement_tol.
+  int complement_iters = 0;
+  double complement_tol = 1e-8;
+  std::shared_ptr<SvdStats> stats;  // optional counters
 };
 
 // Truncated SVD with the reference cutoff rule (src/utility/wrapper.jl:25-55):

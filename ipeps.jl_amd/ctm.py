@@ -115,7 +115,7 @@ def _hpair(H):
 
 ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "mixed_cutoff", "fast_orth",
                "fast_svd",
-               "gram_guard", "canonical_edge", "mixed", "mixed_back", "mixed_proj", "energy_fp64", "energy_mixed",
+               "gram_guard", "power_iters_max", "resid_tol", "complement_iters", "complement_tol", "canonical_edge", "mixed", "mixed_back", "mixed_proj", "energy_fp64", "energy_mixed",
                "energy_mixed_back",
                "checkpoint", "fused_absorb", "paired_moves", "draws")
 
@@ -200,6 +200,16 @@ class CTMTensors:
     def set_A(self, A):
         a = np.asfortranarray(A, dtype=self._np)
         check(lib().ipeps_env_set_A(self.h, _ptr(a)))
+
+    def projector_stats(self, reset=False):
+        """Counters of the randomized projectors since the last reset: dict with
+        calls, power_iters, power_max, resid_max, checks, cg_calls, cg_iters,
+        cg_resid_max, cg_fallbacks."""
+        out = np.zeros(9)
+        check(lib().ipeps_env_projector_stats(self.h, _dp(out), int(bool(reset))))
+        keys = ("calls", "power_iters", "power_max", "resid_max", "checks", "cg_calls", "cg_iters", "cg_resid_max",
+                "cg_fallbacks")
+        return {k: (float(v) if k.endswith("max") and "power" not in k else int(v)) for k, v in zip(keys, out)}
 
     def set_env(self, Cs, Es):
         for i in range(4):

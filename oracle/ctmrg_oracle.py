@@ -310,7 +310,34 @@ def get_projector_bottom(ts, chi):
 # --------------------------------------------------------------------------
 
 
+EDGE_ORDER = "reference"   # or "canonical": _edge_canonical (CPU-baseline timing variant)
+
+
+def _edge_canonical(E, A, Ad, ia, iad, P, Pd):
+    """The edge update of _edge in the canonical pairwise order: Pd with E
+    (chi^3 D^4), then A and Ad (2 chi^2 D^6 d), then P (chi^3 D^4), instead of
+    the reference's chi^2 D^6 intermediate E A Ad.  Same result to rounding
+    (tests/test_oracle.py pins it against _edge)."""
+    # legs of the reference's EAAd (src/ctm/ctmrg.jl:285-289): o1 = E leg 1,
+    # o2 / o5 / o7 = A's labels -2 / -4 / -5, o3 / o6 / o8 = Ad's -3 / -6 / -8,
+    # o4 = E leg 2; E leg 3 meets A's label 1, E leg 4 meets Ad's label 1, and
+    # A's -7 meets Ad's label 2 (the physical leg)
+    la = {-2: "b", -4: "d", -5: "g", -7: "p", 1: "x"}
+    lad = {-3: "e", -6: "h", -8: "k", 1: "y", 2: "p"}
+    sa = "".join(la[l] for l in ia)
+    sd = "".join(lad[l] for l in iad)
+    chi1, D = Pd.shape[0], A.shape[0]
+    Pd4 = reshape_f(Pd, chi1, E.shape[0], D, D)            # (i, a, b, e)
+    P4 = reshape_f(P, E.shape[1], D, D, P.shape[1])         # (c, d, h, j)
+    T1 = torch.einsum("iabe,acxy->ibecxy", Pd4, E)
+    T2 = torch.einsum(f"ibecxy,{sa}->iecydgp", T1, A)
+    T3 = torch.einsum(f"iecydgp,{sd}->icdghk", T2, Ad)
+    return torch.einsum("icdghk,cdhj->ijgk", T3, P4)
+
+
 def _edge(E, A, Ad, ia, iad, P, Pd):
+    if EDGE_ORDER == "canonical":
+        return _edge_canonical(E, A, Ad, ia, iad, P, Pd)
     EA = tcon([E, A], [[-1, -3, 1, -6], ia])
     EAAd = tcon([EA, Ad], [[-1, -2, -4, -5, -7, 1, 2], iad])
     s = EAAd.shape

@@ -83,3 +83,20 @@ def test_mixed_env_correlations_close_to_fp64(pkg):
     s32, _, _ = pkg.correlation_spin(tm, op, op, max_iter=4)
     for a, b in zip(s64, s32):
         assert abs(a - b) < 1e-5 * abs(a)
+
+
+@pytest.mark.parametrize("direction", ["h", "v"])
+def test_transfer_matrix_product_edges_closed_form(pkg, direction):
+    """The engine's transfer matrix (tm_dense and the matrix-free apply) on
+    product-structured edges against the closed form (v . z) kron(w, u)
+    (tests/test_oracle_corr.py), complex data."""
+    from test_oracle_corr import product_edge_env, product_edge_tm
+    A, Cs, Es, u, v, w, z = product_edge_env(direction)
+    ts = pkg.CTMTensors(A, dtype="c128")
+    ts.set_env(Cs, Es)
+    ref = product_edge_tm(u, v, w, z)
+    TM = pkg.expect.transfer_matrix(ts, direction)
+    np.testing.assert_allclose(TM, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    X = np.random.default_rng(3).standard_normal((TM.shape[1], 4)) + 0j
+    Y = pkg.expect.transfer_operator(ts, direction).matmat(X)
+    np.testing.assert_allclose(Y, ref @ X, rtol=0, atol=1e-11 * np.abs(ref @ X).max())

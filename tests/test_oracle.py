@@ -174,3 +174,29 @@ def test_svd_back_matches_fd():
         with torch.no_grad():
             fd = (loss(M + d) - loss(M - d)) / (2 * eps)
         assert abs(fd.item() - Mg.grad[i, j].item()) < 1e-6
+
+
+@pytest.mark.parametrize("ia,iad", [
+    ([-2, 1, -4, -5, -7], [-3, 1, -6, -8, 2]),   # proj_left
+    ([-2, -5, -4, 1, -7], [-3, -8, -6, 1, 2]),   # proj_right
+    ([1, -2, -5, -4, -7], [1, -3, -8, -6, 2]),   # proj_top
+    ([-5, -2, 1, -4, -7], [-8, -3, 1, -6, 2]),   # proj_bottom
+])
+def test_edge_canonical_order_matches_reference_order(ia, iad):
+    """The canonical-order edge update (CPU-baseline timing variant) equals the
+    reference-order one, complex data, every direction's label lists."""
+    g = torch.Generator().manual_seed(3)
+    D, chi, chi1, d = 3, 5, 4, 2
+    cr = lambda *s: torch.complex(torch.randn(*s, generator=g, dtype=torch.float64),
+                                  torch.randn(*s, generator=g, dtype=torch.float64))
+    E, A, Ad = cr(chi, chi, D, D), cr(D, D, D, D, d), cr(D, D, D, D, d)
+    P, Pd = cr(chi * D * D, chi1), cr(chi1, chi * D * D)
+    O.EDGE_ORDER = "reference"
+    r = O._edge(E, A, Ad, ia, iad, P, Pd)
+    try:
+        O.EDGE_ORDER = "canonical"
+        cno = O._edge(E, A, Ad, ia, iad, P, Pd)
+    finally:
+        O.EDGE_ORDER = "reference"
+    assert cno.shape == r.shape
+    assert torch.allclose(cno, r, rtol=1e-12, atol=1e-12 * r.abs().max())

@@ -111,3 +111,53 @@ def test_transfer_matrix_spectrum():
     X = O.reshape_f(x, E4.shape[1], E2.shape[1])
     Y = torch.einsum("acij,bdij,cd->ab", E4, E2, X)
     np.testing.assert_allclose(O.flat_f(Y).numpy(), (TM @ x).numpy(), rtol=1e-12, atol=1e-12)
+
+
+def product_edge_env(direction, chi=5, D=3, seed=11):
+    """An environment whose two transfer-matrix edges are products,
+    X[a,c,i,j] = u[a,c] v[i,j] and Y[b,d,i,j] = w[b,d] z[i,j], with u, v, w, z
+    complex and non-symmetric; returns (A, Cs, Es, u, v, w, z) as numpy."""
+    g = np.random.default_rng(seed)
+    cr = lambda *s: g.standard_normal(s) + 1j * g.standard_normal(s)
+    u, w, v, z = cr(chi, chi), cr(chi, chi), cr(D, D), cr(D, D)
+    X = np.einsum("ac,ij->acij", u, v)
+    Y = np.einsum("bd,ij->bdij", w, z)
+    Es = [cr(chi, chi, D, D) for _ in range(4)]
+    if direction == "h":
+        Es[3], Es[1] = X, Y   # TM = tcon([E4, E2], ...)
+    else:
+        Es[0], Es[2] = X, Y   # TM = tcon([E1, E3], ...)
+    Cs = [cr(chi, chi) for _ in range(4)]
+    A = cr(D, D, D, D, 2)
+    return A, Cs, Es, u, v, w, z
+
+
+def product_edge_tm(u, v, w, z):
+    """The closed form: TM[(a,b),(c,d)] = (sum_ij v_ij z_ij) u[a,c] w[b,d]
+    (column-major pair indices a + chi b), i.e. (v . z) kron(w, u), whose
+    eigenvalues are (v . z) lambda_i(u) lambda_j(w)."""
+    return (v * z).sum() * np.kron(w, u)
+
+
+@pytest.mark.parametrize("direction", ["h", "v"])
+def test_transfer_matrix_product_edges_closed_form(direction):
+    """correlation_TM's transfer matrix (expectation.jl:193-220) on product
+    edges against the closed form: a ket/bra pairing swapped between the two
+    edges gives sum v_ij z_ji instead (v, z are not symmetric), a wrong
+    reshape or leg order a different matrix and spectrum."""
+    A, Cs, Es, u, v, w, z = product_edge_env(direction)
+    t = torch.from_numpy
+    ts = O.CTMTensors(t(A), t(A.conj()), [t(c) for c in Cs], [t(e) for e in Es])
+    es1, es, TM = Q.correlation_TM(ts, direction, full=True)
+    ref = product_edge_tm(u, v, w, z)
+    np.testing.assert_allclose(TM.numpy(), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    lam = (v * z).sum() * np.outer(np.linalg.eigvals(u), np.linalg.eigvals(w)).ravel()
+    for e in lam:  # every closed-form eigenvalue is in the computed spectrum
+        assert np.min(np.abs(es - e)) < 1e-10 * np.abs(lam).max()
+    np.testing.assert_allclose(es1, np.sort(np.abs(lam))[::-1], rtol=1e-10)
+    # the check catches a contraction that pairs the ket and bra legs
+    # crosswise: that TM is (sum v_ij z_ji) kron(w, u), another matrix here
+    X, Y = (Es[3], Es[1]) if direction == "h" else (Es[0], Es[2])
+    chi = u.shape[0]
+    bad = np.einsum("acij,bdji->abcd", X, Y).reshape(chi * chi, chi * chi, order="F")
+    assert np.abs(bad - ref).max() > 1e-3 * np.abs(ref).max()
