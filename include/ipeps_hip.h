@@ -105,7 +105,8 @@ int ipeps_env_destroy(ipeps_env* env);
  * randomized mode
  * "fast_orth" (0: Householder QR range bases; 1: CholeskyQR2 with Householder
  * fallback; 2, the default: CholeskyQR2 on the one-workgroup blocked FP64
- * Cholesky + inverse when k <= 160, else as 1),
+ * Cholesky + inverse when k <= 160, on its 2 x 2 block split when k <= 320,
+ * else as 1),
  * "fast_svd" (0/1: small SVD from Gram eigenpairs with one-sided Jacobi
  * fallback), "gram_guard" (s_min/s_max below which the Gram route is not
  * taken, default 1e-4), "power_iters_max" and "resid_tol" (adaptive power
@@ -132,9 +133,11 @@ int ipeps_env_get_draws(ipeps_env* env, int64_t* out);
  * (sum), power iterations (largest in one projector), largest final
  * a-posteriori residual / s_1, checks made, discarded-subspace solves in the
  * adjoint, their CG iterations (sum), largest final relative CG residual,
- * solves abandoned for a non-positive-definite operator (thin terms used)].
- * reset != 0 clears them after the read. */
-int ipeps_env_projector_stats(ipeps_env* env, double out[9], int reset);
+ * solves abandoned for a non-positive-definite operator (thin terms used),
+ * range-basis orthonormalisations, those that took CholeskyQR2's second
+ * pass, those that fell back to Householder QR].  reset != 0 clears them
+ * after the read. */
+int ipeps_env_projector_stats(ipeps_env* env, double out[12], int reset);
 /* which: 0..3 = C1..C4, 4..7 = E1..E4, 8 = A.  dims has room for 5. */
 int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd);
 /* Copy out / in (host data in the env's host dtype: F64 or C128). */
@@ -254,7 +257,8 @@ int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double guard, int64_t 
  * (src/utility/autodiff.jl:22-25; the reference has no separate routine).
  * method 0: Householder QR; 1: CholeskyQR2 on rocSOLVER potrf / rocBLAS trsm;
  * 2: CholeskyQR2 on the one-workgroup blocked Cholesky + triangular inverse
- * (F64, n <= 160).  Methods 1 and 2 fall back to 0 when a Cholesky fails or
+ * (F64, n <= 320: n > 160 through a 2 x 2 block split).  Methods 1 and 2 fall
+ * back to 0 when a Cholesky fails or
  * the second Gram matrix is farther than 1/2 from I. */
 int ipeps_orth(ipeps_ctx* ctx, int dtype, int method, int64_t m, int64_t n, const void* Y, void* Q);
 /* svd_back (src/utility/autodiff.jl:41-75) for a square N x N matrix with

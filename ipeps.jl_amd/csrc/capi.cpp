@@ -320,7 +320,7 @@ int ipeps_env_get_draws(ipeps_env* env, int64_t* out) {
   });
 }
 
-int ipeps_env_projector_stats(ipeps_env* env, double out[9], int reset) {
+int ipeps_env_projector_stats(ipeps_env* env, double out[12], int reset) {
   return guard([&] {
     IPEPS_CHECK(env && out, "null argument");
     auto& st = env->env.opt.svd.stats;
@@ -336,6 +336,9 @@ int ipeps_env_projector_stats(ipeps_env* env, double out[9], int reset) {
       out[6] = (double)st->cg_iters;
       out[7] = st->cg_resid_max;
       out[8] = (double)st->cg_fallbacks;
+      out[9] = (double)st->orth_calls;
+      out[10] = (double)st->orth_second;
+      out[11] = (double)st->orth_householder;
     }
     if (reset) st->reset();
   });

@@ -1063,6 +1063,25 @@ static void gesvdj_thin(Ctx& c, const Tensor& Ain, Tensor& U, Tensor& S, Tensor&
 }
 void svd_full_raw(Ctx& c, const Tensor& X, Tensor& U, Tensor& s, Tensor& V) { gesvd(c, X, U, s, V, false); }
 
+namespace {
+// counters of the projector being built on this thread (null: none)
+SvdStats*& orth_stats() {
+  thread_local SvdStats* s = nullptr;
+  return s;
+}
+void orth_count(long SvdStats::*field) {
+  if (SvdStats* s = orth_stats()) {
+    std::lock_guard<std::mutex> g(s->mu);
+    ++(s->*field);
+  }
+}
+struct OrthStatsScope {
+  SvdStats* prev;
+  explicit OrthStatsScope(SvdStats* s) : prev(orth_stats()) { orth_stats() = s; }
+  ~OrthStatsScope() { orth_stats() = prev; }
+};
+}  // namespace
+
 static Tensor orth_householder(Ctx& c, const Tensor& Yin) {
   IPEPS_CHECK(Yin.nd() == 2 && Yin.dims[0] >= Yin.dims[1], "orth: tall matrix expected");
   int64_t m = Yin.dims[0], n = Yin.dims[1];
@@ -1125,6 +1144,53 @@ static void trsm_right_upper(Ctx& c, const Tensor& R, Tensor& X) {
   }
 }
 
+// FP64 GEMM on sub-blocks: C = alpha op(A) op(B) + beta C (leading dimensions given).
+static void gemm_f64(Ctx& c, bool tA, const double* A, int64_t lda, bool tB, const double* B, int64_t ldb, double* C,
+                     int64_t ldc, int64_t M, int64_t N, int64_t K, double alpha, double beta) {
+  k::GemmArgs g;
+  g.dt = DType::F64;
+  g.transA = tA;
+  g.transB = tB;
+  g.M = M; g.N = N; g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = A; g.lda = lda;
+  g.B = B; g.ldb = ldb;
+  g.C = C; g.ldc = ldc;
+  k::gemm(c, g);
+}
+
+// L^-1 of the Cholesky factor of an SPD n x n FP64 G (lower triangle read),
+// n <= 2 SCHOL_MAXN_HOST, into Li (ldl; zeros above the diagonal): the
+// one-workgroup kernel directly, or on the two diagonal blocks of a 2 x 2
+// split,
+//   G = [G11, G21^T; G21, G22],  L11 L11^T = G11,  X = G21 L11^-T (= L21),
+//   L22 L22^T = G22 - X X^T,
+//   L^-1 = [L11^-1, 0; -L22^-1 X L11^-1, L22^-1].
+// info1 / info2: the blocks' kernel infos (pivot j + 1 of the block on failure).
+static void chol_inv(Ctx& c, int n, const double* G, int64_t ldg, double* Li, int64_t ldl, int* info1, int* info2) {
+  if (n <= k::SCHOL_MAXN_HOST) {
+    k::small_chol_inv(c, n, G, ldg, Li, ldl, info1);
+    return;
+  }
+  IPEPS_CHECK(n <= 2 * k::SCHOL_MAXN_HOST, "chol_inv: n out of range");
+  const int n2 = n / 2, n1 = n - n2;
+  double* L11i = Li;
+  double* L22i = Li + n1 + (int64_t)n1 * ldl;
+  k::small_chol_inv(c, n1, G, ldg, L11i, ldl, info1);
+  Tensor X = c.empty(DType::F64, {n2, n1});
+  gemm_f64(c, false, G + n1, ldg, true, L11i, ldl, X.ptr<double>(), n2, n2, n1, n1, 1.0, 0.0);
+  Tensor S = c.empty(DType::F64, {n2, n2});
+  HIP_CHECK(hipMemcpy2DAsync(S.data(), n2 * 8, G + n1 + (int64_t)n1 * ldg, ldg * 8, n2 * 8, n2,
+                             hipMemcpyDeviceToDevice, c.stream));
+  gemm_f64(c, false, X.ptr<double>(), n2, true, X.ptr<double>(), n2, S.ptr<double>(), n2, n2, n2, n1, -1.0, 1.0);
+  k::small_chol_inv(c, n2, S.ptr<double>(), n2, L22i, ldl, info2);
+  Tensor T = c.empty(DType::F64, {n2, n1});
+  gemm_f64(c, false, X.ptr<double>(), n2, false, L11i, ldl, T.ptr<double>(), n2, n2, n1, n1, 1.0, 0.0);
+  gemm_f64(c, false, L22i, ldl, false, T.ptr<double>(), n2, Li + n1, ldl, n2, n1, n2, -1.0, 0.0);
+  HIP_CHECK(hipMemset2DAsync(Li + (int64_t)n1 * ldl, ldl * 8, 0, n1 * 8, n2, c.stream));
+}
+
 // CholeskyQR2: Q1 = Y R1^-1 with R1^H R1 = Y^H Y, then the same on Q1.  The
 // second pass is trusted when its Gram is within 1/2 of I (then cond(Q1)^2 <= 3
 // and Q is orthonormal to working precision).  Returns false when a Cholesky
@@ -1135,38 +1201,41 @@ static void trsm_right_upper(Ctx& c, const Tensor& R, Tensor& X) {
 // orth_cholqr2.
 static bool orth_cholqr2_small(Ctx& c, const Tensor& Y, Tensor& Q) {
   const int64_t n = Y.dims[1];
-  Tensor st = c.zeros(DType::F64, {3});  // [info1, info2, |G2 - I|_max]
-  int* info1 = (int*)st.ptr<double>();
-  int* info2 = (int*)(st.ptr<double>() + 1);
+  Tensor st = c.zeros(DType::F64, {5});  // [info of each block, pass 1 / pass 2] x 2, |G2 - I|_max
+  auto info = [&](int i) { return (int*)(st.ptr<double>() + i); };
+  auto failed = [](const double* h, int i0) {
+    int a, b;
+    std::memcpy(&a, &h[i0], sizeof(int));
+    std::memcpy(&b, &h[i0 + 1], sizeof(int));
+    return a != 0 || b != 0;
+  };
   Tensor G = matmul_raw(c, Y, true, false, Y, false, false);
   Tensor Li = c.empty(DType::F64, {n, n});
-  k::small_chol_inv(c, (int)n, G.ptr<double>(), n, Li.ptr<double>(), n, info1);
+  chol_inv(c, (int)n, G.ptr<double>(), n, Li.ptr<double>(), n, info(0), info(1));
   Tensor Q1 = matmul_raw(c, Y, false, false, Li, true, false);
   Tensor G2 = matmul_raw(c, Q1, true, false, Q1, false, false);
-  k::eye_dev(c, DType::F64, n, G2.data(), st.ptr<double>() + 2);
-  double hs[3];
+  k::eye_dev(c, DType::F64, n, G2.data(), st.ptr<double>() + 4);
+  double hs[5];
   HIP_CHECK(hipMemcpyAsync(hs, st.data(), sizeof(hs), hipMemcpyDeviceToHost, c.stream));
   c.sync();
-  int i1, i2;
-  std::memcpy(&i1, &hs[0], sizeof(int));
-  if (i1 != 0 || !(hs[2] <= 0.5)) return false;
-  if (hs[2] <= 1e-12) {
+  if (failed(hs, 0) || !(hs[4] <= 0.5)) return false;
+  if (hs[4] <= 1e-12) {
     Q = Q1;
     return true;
   }
-  k::small_chol_inv(c, (int)n, G2.ptr<double>(), n, Li.ptr<double>(), n, info2);
+  orth_count(&SvdStats::orth_second);
+  chol_inv(c, (int)n, G2.ptr<double>(), n, Li.ptr<double>(), n, info(2), info(3));
   Q = matmul_raw(c, Q1, false, false, Li, true, false);
   HIP_CHECK(hipMemcpyAsync(hs, st.data(), sizeof(hs), hipMemcpyDeviceToHost, c.stream));
   c.sync();
-  std::memcpy(&i2, &hs[1], sizeof(int));
-  return i2 == 0;
+  return !failed(hs, 2);
 }
 
 // The second pass is skipped when the first already left Q orthonormal to
 // |Q^H Q - I|_max <= 1e-12 (well-conditioned Y: cond(Y)^2 u well below that).
 static bool orth_cholqr2(Ctx& c, const Tensor& Y, Tensor& Q, bool small_chol) {
   const int64_t n = Y.dims[1];
-  if (small_chol && Y.dt == DType::F64 && n <= k::SCHOL_MAXN_HOST) return orth_cholqr2_small(c, Y, Q);
+  if (small_chol && Y.dt == DType::F64 && n <= 2 * k::SCHOL_MAXN_HOST) return orth_cholqr2_small(c, Y, Q);
   Tensor st = c.zeros(DType::F64, {3});  // [info1, info2, |G2 - I|_max]
   int* info1 = (int*)st.ptr<double>();
   int* info2 = (int*)(st.ptr<double>() + 1);
@@ -1183,6 +1252,7 @@ static bool orth_cholqr2(Ctx& c, const Tensor& Y, Tensor& Q, bool small_chol) {
   std::memcpy(&i1, &hs[0], sizeof(int));
   if (i1 != 0 || !(hs[2] <= 0.5)) return false;
   if (hs[2] <= 1e-12) return true;
+  orth_count(&SvdStats::orth_second);
   potrf_upper(c, G2, info2);
   trsm_right_upper(c, G2, Q);
   HIP_CHECK(hipMemcpyAsync(hs, st.data(), sizeof(hs), hipMemcpyDeviceToHost, c.stream));
@@ -1194,7 +1264,9 @@ static bool orth_cholqr2(Ctx& c, const Tensor& Y, Tensor& Q, bool small_chol) {
 Tensor orth(Ctx& c, const Tensor& Y, int fast) {
   IPEPS_CHECK(Y.nd() == 2 && Y.dims[0] >= Y.dims[1], "orth: tall matrix expected");
   Tensor Q;
+  orth_count(&SvdStats::orth_calls);
   if (fast && orth_cholqr2(c, Y, Q, fast == 2)) return Q;
+  orth_count(&SvdStats::orth_householder);
   return orth_householder(c, Y);
 }
 
@@ -1426,6 +1498,7 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
       return matmul_raw(c, R2, true, true, matmul_raw(c, R1, true, true, X, false, false), false, false);
     };
     const int fo = o.fast_orth;
+    OrthStatsScope oss(o.stats.get());
     Tensor Q = orth(c, to_dtype(c, apply_M(Om), sdt), fo);
     int q = 0;
     for (; q < o.power_iters; ++q) {

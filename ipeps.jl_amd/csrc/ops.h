@@ -160,9 +160,13 @@ struct SvdStats {
   long cg_iters = 0;       // their CG iterations, summed
   double cg_resid_max = 0.0;  // largest final relative CG residual
   long cg_fallbacks = 0;   // solves abandoned (operator not positive definite): thin term used
+  long orth_calls = 0;     // range-basis orthonormalisations
+  long orth_second = 0;    // ... that needed CholeskyQR2's second pass
+  long orth_householder = 0;  // ... that fell back to Householder QR
   void reset() {
     std::lock_guard<std::mutex> g(mu);
     calls = power_iters = power_max = checks = cg_calls = cg_iters = cg_fallbacks = 0;
+    orth_calls = orth_second = orth_householder = 0;
     resid_max = cg_resid_max = 0.0;
   }
 };
@@ -183,7 +187,8 @@ struct SvdOpts {
   //                  solve passes), falling back to Householder QR when a
   //                  Cholesky fails or the second Gram is farther than 1/2 from I;
   //                  2: the same with the one-workgroup blocked FP64 Cholesky +
-  //                  inverse (F64, k <= 160; otherwise as 1);
+  //                  inverse (F64, k <= 160; a 2 x 2 block split of it for
+  //                  k <= 320; otherwise as 1);
   //   fast_svd  = 1: the small SVD of B = M^H Q from the eigenpairs of B^H B
   //                  plus one Newton-Schulz orthonormalisation of B's singular
   //                  vectors, falling back to one-sided Jacobi (gesvdj) when

@@ -204,11 +204,11 @@ class CTMTensors:
     def projector_stats(self, reset=False):
         """Counters of the randomized projectors since the last reset: dict with
         calls, power_iters, power_max, resid_max, checks, cg_calls, cg_iters,
-        cg_resid_max, cg_fallbacks."""
-        out = np.zeros(9)
+        cg_resid_max, cg_fallbacks, orth_calls, orth_second, orth_householder."""
+        out = np.zeros(12)
         check(lib().ipeps_env_projector_stats(self.h, _dp(out), int(bool(reset))))
         keys = ("calls", "power_iters", "power_max", "resid_max", "checks", "cg_calls", "cg_iters", "cg_resid_max",
-                "cg_fallbacks")
+                "cg_fallbacks", "orth_calls", "orth_second", "orth_householder")
         return {k: (float(v) if k.endswith("max") and "power" not in k else int(v)) for k, v in zip(keys, out)}
 
     def set_env(self, Cs, Es):
