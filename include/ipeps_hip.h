@@ -182,6 +182,20 @@ int ipeps_gs_energy_grad(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const
                          int ad_max_iter, double tol, const char* options, double* E_out, void* grad_out,
                          int* iters_out);
 
+/* run_gs (src/optimize/optim_gs.jl:133-143) and its pullback: the function
+ * whose ChainRulesCore.rrule replaces Zygote's pass through the
+ * differentiated CTMRG run of fg! (optim_gs.jl:72-82).  From env's current
+ * environment, held constant (env is not modified), with A = x and
+ * Ad = conj(x) (x: env's host dtype, A's dims): run_ctm's stopping rule with
+ * min_iter, max_iter = ad_max_iter and tol on the energy, then
+ * E = get_gs_energy (E_out = re, im).  grad_out = dE (dRe(E)/dRe(x) +
+ * i dRe(E)/dIm(x)) for the cotangent dE of Re(E); steps_out = differentiated
+ * steps.  The engine options of env apply (checkpoint=1: step
+ * recomputation in the reverse sweep). */
+int ipeps_run_gs_vjp(ipeps_env* env, const void* x, const double* H1, const double* H2, int64_t hdim, int64_t chi,
+                     int min_iter, int ad_max_iter, double tol, double dE, double* E_out, void* grad_out,
+                     int* steps_out);
+
 /* get_es_grad (src/optimize/optim_es.jl:167-194) for one tangent tensor B:
  * env must be a converged, normalised ground-state environment in IPEPS_C128;
  * the momentum is (px, py) pi as optim_es's arguments (optim_es.jl:124-125).

@@ -45,6 +45,8 @@ SIGNATURES = {
     "ipeps_step_adjoint": (C.c_int, [_P, C.c_int64, _P, _P, C.c_int64, _DP, _P]),
     "ipeps_gs_energy_grad": (C.c_int, [_P, C.c_int, _I64P, _P, _P, _P, C.c_int64, C.c_int64, C.c_int, C.c_int,
                                        C.c_int, C.c_double, C.c_char_p, _DP, _P, _IP]),
+    "ipeps_run_gs_vjp": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   _DP, _P, _IP]),
     "ipeps_es_grad": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, C.c_double, _P, C.c_int64, C.c_int, C.c_int,
                                 C.c_int, C.c_double, _P, _P, _DP, _DP]),
     "ipeps_tcon": (C.c_int, [_P, C.c_int, C.c_int, _I64P, _P, _IP, C.c_int, _I64P, _P, _IP, _I64P, _IP, _P,

@@ -468,6 +468,29 @@ int ipeps_gs_energy_grad(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const
   });
 }
 
+int ipeps_run_gs_vjp(ipeps_env* env, const void* x, const double* H1, const double* H2, int64_t hdim, int64_t chi,
+                     int min_iter, int ad_max_iter, double tol, double dE, double* E_out, void* grad_out,
+                     int* steps_out) {
+  return guard([&] {
+    IPEPS_CHECK(env && x && chi > 0 && ad_max_iter >= 1, "bad arguments");
+    Ctx& c = *env->hold;
+    const DType hdt = host_dt(env->dtype);
+    Tensor tx = upload(c, hdt, hdt, env->env.A.dims, x);
+    auto r = ctm::run_gs_vjp(c, env->env, tx, upload_H(c, H1, hdim), upload_H(c, H2, hdim), chi, min_iter,
+                             ad_max_iter, tol);
+    if (E_out) { E_out[0] = r.E[0]; E_out[1] = r.E[1]; }
+    if (grad_out) {
+      Tensor gr = r.grad;
+      if (dE != 1.0) {
+        gr = op::clone(c, gr);
+        k::scale_cplx(c, gr.dt, gr.numel(), dE, 0.0, gr.data(), gr.data());
+      }
+      download(c, gr, hdt, grad_out);
+    }
+    if (steps_out) *steps_out = r.ad_iters;
+  });
+}
+
 int ipeps_es_grad(ipeps_env* env, const double* H1, const double* H2, int64_t hdim, double px, double py,
                   const void* B, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol, void* gH_out,
                   void* envB_out, double* E4_out, double* Nb_out) {

@@ -619,7 +619,6 @@ static void strip_nodes(Env& e) {
 
 GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options& opt, const Tensor& H1,
                           const Tensor& H2, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol) {
-  GradResult g;
   TapeScope none(nullptr);
   // x = renormalize(A0) outside the AD (src/optimize/optim_gs.jl:55)
   Tensor x = op::renormalize(c, A0);
@@ -630,7 +629,22 @@ GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options&
   e.Ad = op::conj(c, e.A);
   init_ctm1(c, e);
   auto r = run_ctm(c, e, chi, min_iter, max_iter, tol, 0, H1, H2);
+  GradResult g = run_gs_vjp(c, e, e.A, H1, H2, chi, min_iter, ad_max_iter, tol);
   g.conv_iters = r.iters;
+  return g;
+}
+
+GradResult run_gs_vjp(Ctx& c, const Env& env0, const Tensor& xin, const Tensor& H1, const Tensor& H2, int64_t chi,
+                      int min_iter, int ad_max_iter, double tol) {
+  GradResult g;
+  TapeScope none(nullptr);
+  const Options& opt = env0.opt;
+  const DType env_dt = env0.dt;
+  // setproperties(ts, A = x, Ad = conj(x)) on a copy: env0's tensors are
+  // never written (every step replaces them), so it is held constant
+  Env e = env0;
+  e.A = op::cast(c, xin, env_dt);
+  e.Ad = op::conj(c, e.A);
   // Differentiated run (run_gs): the same stopping rule with max_iter =
   // ad_max_iter from the converged environment, which is held constant.
   // Without checkpointing one tape spans every step (memory ~ steps x the

@@ -90,6 +90,12 @@ struct GradResult {
 };
 GradResult gs_energy_grad(Ctx& c, const Tensor& A0, DType env_dt, const Options& opt, const Tensor& H1,
                           const Tensor& H2, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol);
+// run_gs and its pullback (src/optimize/optim_gs.jl:133-143): from env0
+// (held constant, not modified) with A = x, Ad = conj(x), run_ctm's stopping
+// rule with max_iter = ad_max_iter, E = get_gs_energy; grad = dRe(E)/dx.
+// gs_energy_grad is renormalize + init_ctm1 + run_ctm + this.
+GradResult run_gs_vjp(Ctx& c, const Env& env0, const Tensor& x, const Tensor& H1, const Tensor& H2, int64_t chi,
+                      int min_iter, int ad_max_iter, double tol);
 
 // ---------------------------------------------------------------------------
 // Excited states: nested tensors [X, X_B, X_Bd, X_BB]

@@ -315,6 +315,24 @@ def gs_energy_grad(A0, H, cfg, dtype=None, options=None, ctx=None):
     return complex(E[0], E[1]), g, (it[0], it[1])
 
 
+def run_gs_vjp(ts, x, H, cfg, dE=1.0):
+    """run_gs (src/optimize/optim_gs.jl:133-143) and its pullback: from ts's
+    environment, held constant, with A = x, the differentiated steps
+    (min_iter, max_iter = cfg["ad_max_iter"], rg_tol) and the energy.
+    Returns (E complex, dE * dRe(E)/dx, differentiated steps); ts is not
+    modified.  The rrule of run_gs is x -> (E, dE -> this gradient)."""
+    x = np.asarray(x)
+    xf = np.asfortranarray(x, dtype=_host_np(ts.code))
+    (h1, h2), m = _hpair(H)
+    E = np.zeros(2)
+    g = np.empty(x.shape, dtype=_host_np(ts.code), order="F")
+    n = C.c_int()
+    check(lib().ipeps_run_gs_vjp(ts.h, _ptr(xf), _ptr(h1), _ptr(h2), m, int(cfg.get("chi", 30)),
+                                 int(cfg.get("min_iter", 4)), int(cfg.get("ad_max_iter", 4)),
+                                 float(cfg.get("rg_tol", 1e-6)), float(dE), _dp(E), _ptr(g), C.byref(n)))
+    return complex(E[0], E[1]), g, n.value
+
+
 def tcon(xs, ind_xs, ctx=None):
     """Pairwise ncon contraction on the device (src/utility/wrapper.jl:15-23)."""
     if len(xs) != 2:

@@ -6,7 +6,8 @@ Mirrors src/optimize/optim_es.jl and src/graph/expectation.jl:
 * ``optim_es(ts, H, basis, px, py, cfg)``   optim_es.jl:91-165: effH, effN,
   envB columns for every tangent basis vector (GPU per column)
 * ``normalize_gs``, ``substract_gs_energy``, ``get_tangent_basis``
-                                            optim_es.jl:212-258
+                                            optim_es.jl:212-258 (+ get_gauge_basis :294-344)
+* ``prepare_basis(A, H, cfg)``              optim_es.jl:34-73 (no file I/O)
 * ``compute_es(effH, effN, nrmB_cut)``      expectation.jl:13-56: the small
   generalised eigenproblem, host FP64 (LAPACK via scipy), as in the reference
 * ``make_es_path(step)``                    optim_es.jl:1-20
@@ -110,10 +111,33 @@ def substract_gs_energy(ts, H):
     return [np.asarray(h, dtype=np.complex128) - e * np.eye(np.asarray(h).shape[0]) for h in H]
 
 
-def get_tangent_basis(ts):
-    """optim_es.jl:236-258 ("default" basis): nullspace of the single-site
-    environment contracted with Ad, shape (D^4 d, D^4 d - 1)."""
+def get_tangent_basis(ts, basis_t="default"):
+    """optim_es.jl:236-258.  basis_t (the config key of the same name):
+    "default": nullspace of the single-site environment contracted with Ad,
+    (D^4 d, D^4 d - 1); "unit": the identity; "gauge": the Q-operator images
+    of the unit tensors of a (D/2)^4 d site without flux and with a sigma^z,
+    sigma^x, sigma^y flux (get_gauge_basis, optim_es.jl:294-344); "cut" is
+    refused: its diag_n_dm (optim_es.jl:261) is defined only in the
+    reference's test/test_basis_entropy.jl, not in its package."""
     from .ctm import tcon
+    if basis_t == "unit":
+        return np.eye(ts.A.size, dtype=np.complex128)
+    if basis_t == "gauge":
+        from .models import act_Q_op
+        D, d = ts.A.shape[0] // 2, ts.A.shape[4]
+        M = D ** 4 * d
+        cols = []
+        for add in (0, 3, 1, 2):  # basis0 (no flux), basis1 (z), basis2 (x), basis3 (y)
+            for i in range(M):
+                B0 = np.zeros(M)
+                B0[i] = 1
+                cols.append(act_Q_op(B0.reshape(D, D, D, D, d, order="F"), add=add).reshape(-1, order="F"))
+        return np.stack(cols, axis=1).astype(np.complex128)
+    if basis_t == "cut":
+        raise NotImplementedError("basis_t='cut' needs diag_n_dm, which the reference defines only in its test "
+                                  "file test/test_basis_entropy.jl")
+    if basis_t != "default":
+        raise ValueError(f"unknown basis_t {basis_t!r}")
     Cs, Es = ts.Cs, ts.Es
     C1E1 = tcon([Cs[0], Es[0]], [[-1, 1], [1, -2, -3, -4]])
     CEC2 = tcon([C1E1, Cs[1]], [[-1, 1, -3, -4], [1, -2]])
@@ -126,6 +150,28 @@ def get_tangent_basis(ts):
     ndm_Ad = tcon([ndm, Ad], [[-1, -2, -3, -4, 1, 2, 3, 4], [1, 2, 3, 4, -5]])
     row = ndm_Ad.reshape(1, -1, order="F")
     return sla.null_space(row)
+
+
+def prepare_basis(A, H, cfg, dtype="c128", options=None, ctx=None):
+    """prepare_basis (optim_es.jl:34-73) without the file I/O: A ->
+    renormalize(A); converge the ground-state environment with max_iter
+    doubled and rg_tol squared (energy convergence); normalize_gs;
+    substract_gs_energy; get_tangent_basis (cfg "basis_t", default
+    "default").  Returns (basis, ts, shifted H); ts carries cfg's own
+    min/max_iter and rg_tol again, as setproperties(ts, Params = cfg0)."""
+    from .ctm import run_ctm
+    A = np.asarray(A)
+    A = A / np.abs(A).max()
+    conv_cfg = dict(cfg)
+    conv_cfg["max_iter"] = 2 * int(cfg.get("max_iter", 20))
+    conv_cfg["rg_tol"] = float(cfg.get("rg_tol", 1e-6)) ** 2
+    ts = CTMTensors(A, conv_cfg, dtype=dtype, options=options, ctx=ctx)
+    run_ctm(ts, "energy", H)
+    ts.Params = dict(cfg)
+    ts = normalize_gs(ts)
+    Hs = substract_gs_energy(ts, H)
+    basis = get_tangent_basis(ts, cfg.get("basis_t", "default"))
+    return basis, ts, Hs
 
 
 def compute_es(effH, effN, nrmB_cut=1e-3):

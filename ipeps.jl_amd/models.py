@@ -5,6 +5,8 @@
 * ``heisenberg``  src/model/basis.jl:31-35
 * ``honeycomb``   src/model/honeycomb.jl:1-14
 * ``init_hb_gs``  src/model/lee_gs.jl:2-28 (D = 2 loop-gas state)
+* ``act_Q_op``    src/model/lee_gs.jl:54-73 (the Q operator on a site tensor,
+                  the "gauge" tangent basis of optim_es)
 """
 import math
 
@@ -59,6 +61,28 @@ def get_Q_op():
     Q[1, 0, 1] = sigmay
     Q[1, 1, 0] = sigmaz
     return Q
+
+
+def act_Q_op(A0, add=0):
+    """Q (x) Q applied to a site tensor A0 (D0,D0,D0,D0,4) -> (2 D0, ..., 4),
+    with a flux sigma^x / sigma^y / sigma^z inserted on the first Q for
+    add = 1 / 2 / 3 (src/model/lee_gs.jl:54-73)."""
+    Q = get_Q_op()
+    if add == 1:
+        Q1 = np.einsum("xbcpq,xa->abcpq", Q, sigmax)
+    elif add == 2:
+        Q1 = np.einsum("axcpq,xb->abcpq", Q, sigmay)
+    elif add == 3:
+        Q1 = np.einsum("abxpq,xc->abcpq", Q, sigmaz)
+    elif add == 0:
+        Q1 = Q
+    else:
+        raise ValueError(add)
+    QQ = np.einsum("abxeg,cdxfh->abcdefgh", Q1, Q)
+    QQ = QQ.reshape(2, 2, 2, 2, 4, 4, order="F")
+    A = np.einsum("acegix,bdfhx->abcdefghi", QQ, np.asarray(A0))
+    D1, D2 = A.shape[0], A.shape[1]
+    return A.reshape(D1 * D2, D1 * D2, D1 * D2, D1 * D2, A.shape[8], order="F")
 
 
 def init_hb_gs(D=2, dir="XX"):

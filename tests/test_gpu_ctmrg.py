@@ -654,3 +654,32 @@ def test_paired_moves_match_sequential(pkg, dtype):
     assert np.array_equal(g0, g1)
     for a, b in zip(C0 + E0s, C1 + E1s):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dtype,checkpoint", [("f64", 0), ("f64", 1), ("c128", 0)])
+def test_run_gs_vjp_is_the_fg_pullback(pkg, dtype, checkpoint):
+    """ipeps_run_gs_vjp (the rrule of run_gs, src/optimize/optim_gs.jl:133-143)
+    from a converged environment equals the gradient of fg!
+    (ipeps_gs_energy_grad), scales with the cotangent dE, and leaves the
+    environment it starts from unchanged."""
+    M = pkg.models
+    A0 = M.seeded_tensor((2, 2, 2, 2, 2), 6, unit_max=False)
+    if dtype == "c128":
+        A0 = A0 + 0.3j * M.seeded_tensor((2, 2, 2, 2, 2), 7)
+    H = M.ising(2.0)
+    cfg = dict(chi=12, min_iter=4, max_iter=12, ad_max_iter=3, rg_tol=1e-9)
+    opts = dict(svd_mode=0, checkpoint=checkpoint)
+    E, g, it = pkg.gs_energy_grad(A0, H, cfg, dtype=dtype, options=opts)
+    x = A0 / np.abs(A0).max()
+    ts = pkg.CTMTensors(x, cfg, dtype=dtype, options=opts)
+    pkg.run_ctm(ts, "energy", H)
+    Cs0 = [c.copy() for c in ts.Cs]
+    E1, g1, n = pkg.run_gs_vjp(ts, x, H, cfg)
+    assert n == it[1]
+    # the same arithmetic up to the rounding of rocSOLVER on two concurrent streams
+    assert abs(E1 - E) < 1e-13
+    assert np.linalg.norm(g1 - g) < 1e-10 * np.linalg.norm(g)
+    for a, b in zip(ts.Cs, Cs0):
+        np.testing.assert_array_equal(a, b)
+    E2, g2, _ = pkg.run_gs_vjp(ts, x, H, cfg, dE=2.5)
+    assert np.linalg.norm(g2 - 2.5 * g1) < 1e-10 * np.linalg.norm(g1)

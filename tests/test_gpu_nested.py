@@ -158,3 +158,54 @@ def test_optim_es_workers_match_sequential(pkg, randomized):
     par = pkg.optim_es(ts, H, e["basis"], 0.5, 0.5, cfg, columns=cols, workers=3)
     for a, b in zip(seq, par):
         assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()
+
+
+@pytest.mark.parametrize("fixture", ["env_loopgas_D2_X32.npz", "env_ising_D2_X32.npz"])
+def test_tangent_basis_matches_reference_basis(pkg, fixture):
+    """get_tangent_basis (optim_es.jl:236-258, default basis) on the
+    reference's converged environments against the basis they saved with it:
+    the projectors onto the (D^4 d - 1)-dimensional nullspace agree to 1e-10
+    (a nullspace basis is unique only up to a unitary), the basis is
+    orthonormal and annihilated by ndm_Ad."""
+    e = np.load(os.path.join(GOLDEN, fixture))
+    ts = pkg.CTMTensors(e["A"], dtype="c128")
+    ts.set_env([e[f"C{i}"] for i in range(1, 5)], [e[f"E{i}"] for i in range(1, 5)])
+    B = pkg.excited.get_tangent_basis(ts)
+    Bref = e["basis"]
+    assert B.shape == Bref.shape
+    P, Pref = B @ B.conj().T, Bref @ Bref.conj().T
+    print(fixture, np.abs(P - Pref).max())
+    assert np.abs(P - Pref).max() < 1e-10
+    assert np.abs(B.conj().T @ B - np.eye(B.shape[1])).max() < 1e-12
+    assert pkg.excited.get_tangent_basis(ts, "unit").shape == (e["A"].size, e["A"].size)
+    with pytest.raises(NotImplementedError):
+        pkg.excited.get_tangent_basis(ts, "cut")
+
+
+def test_gauge_basis_shape(pkg):
+    """basis_t = "gauge" (get_gauge_basis, optim_es.jl:294-344) on the D=2
+    loop-gas state: four blocks of (D/2)^4 d = 4 Q-operator images."""
+    e = np.load(os.path.join(GOLDEN, "env_loopgas_D2_X32.npz"))
+    ts = pkg.CTMTensors(e["A"], dtype="c128")
+    B = pkg.excited.get_tangent_basis(ts, "gauge")
+    assert B.shape == (64, 16)
+    np.testing.assert_allclose(B[:, 0], pkg.models.act_Q_op(np.eye(4)[0].reshape(1, 1, 1, 1, 4, order="F"))
+                               .reshape(-1, order="F"))
+
+
+def test_prepare_basis(pkg):
+    """prepare_basis (optim_es.jl:34-73): the returned environment is the
+    normalised ground state (get_gs_norm = 1), H is shifted by E_gs / 2, and
+    the basis spans the nullspace of that environment's ndm_Ad."""
+    M = pkg.models
+    A = M.seeded_tensor((2, 2, 2, 2, 2), 5) + 0j
+    H = M.ising(2.0)
+    cfg = dict(chi=10, min_iter=4, max_iter=10, rg_tol=1e-6)
+    basis, ts, Hs = pkg.excited.prepare_basis(A, H, cfg)
+    assert basis.shape == (32, 31)
+    assert abs(pkg.get_gs_norm(ts) - 1) < 1e-10
+    E, _ = pkg.get_gs_energy(ts, H)
+    np.testing.assert_allclose(Hs[0], H[0] - E.real / 2 * np.eye(4), atol=1e-12)
+    B2 = pkg.excited.get_tangent_basis(ts)
+    assert np.abs(basis @ basis.conj().T - B2 @ B2.conj().T).max() < 1e-12
+    assert ts.Params["max_iter"] == 10

@@ -28,3 +28,25 @@ def test_env_options_documented(pkg):
     hdr = open(os.path.join(ROOT, "include", "ipeps_hip.h")).read()
     for k in pkg.ENV_OPTIONS:
         assert f'"{k}"' in hdr, k
+
+
+def test_act_Q_op_reproduces_loop_gas_state(pkg):
+    """act_Q_op (src/model/lee_gs.jl:54-73) on the unit site tensor carrying
+    s111 (x) s111 is the ZZ loop-gas state of init_hb_gs (lee_gs.jl:2-28), so
+    that state lies in the span of the no-flux columns of the gauge basis;
+    the sigma^z-flux image differs from it."""
+    import math
+    M = pkg.models
+    u = 1 / math.sqrt(3)
+    s = np.array([1 + u, u + 1j * u]) / math.sqrt(2 + 2 * u)
+    B0 = np.kron(s, s).reshape(1, 1, 1, 1, 4, order="F")  # (q1, q2) column-major
+    A = M.act_Q_op(B0, add=0)
+    ref = M.init_hb_gs(2, "ZZ")
+    assert A.shape == ref.shape == (2, 2, 2, 2, 4)
+    np.testing.assert_allclose(A, ref, atol=1e-14)
+    cols = np.stack([M.act_Q_op(np.eye(4)[i].reshape(1, 1, 1, 1, 4, order="F"), 0).reshape(-1, order="F")
+                     for i in range(4)], axis=1)
+    x = np.linalg.lstsq(cols, ref.reshape(-1, order="F"), rcond=None)[0]
+    assert np.abs(cols @ x - ref.reshape(-1, order="F")).max() < 1e-13
+    Az = M.act_Q_op(B0, add=3)
+    assert np.abs(Az - ref).max() > 1e-3
