@@ -161,6 +161,15 @@ int ipeps_run_ctm(ipeps_env* env, int64_t chi, int min_iter, int max_iter, doubl
 /* get_gs_energy (src/graph/evaluation.jl:15-29): E and Na as (re, im). */
 int ipeps_gs_energy(ipeps_env* env, const double* H1, const double* H2, int64_t hdim, double* E_out,
                     double* N_out);
+/* get_gs_energy_4x4 (src/graph/evaluation.jl:340-355: energy_norm_4x4
+ * :443-496 with get_op_Ad4 :504-519) for a four-site plaquette Hamiltonian
+ * in the factored form get_local_h returns (src/model/local_h.jl:1-31):
+ * s1 (d,d,r1), s2 (r1,d,d,r2), s3 (r2,d,d,r3), s4 (r3,d,d), ComplexF64,
+ * column-major; s_dims is 4 x 4 int64 (row i: the dims of s_i, trailing
+ * entries ignored).  E_out = energy / norm, N_out = norm, each (re, im);
+ * FP64 arithmetic whatever the environment's dtype. */
+int ipeps_gs_energy_4x4(ipeps_env* env, const int64_t* s_dims, const double* s1, const double* s2,
+                        const double* s3, const double* s4, double* E_out, double* N_out);
 /* get_gs_norm (src/graph/evaluation.jl:47-59): (re, im). */
 int ipeps_gs_norm(ipeps_env* env, double* out);
 

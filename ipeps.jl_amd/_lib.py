@@ -41,6 +41,7 @@ SIGNATURES = {
     "ipeps_run_ctm": (C.c_int, [_P, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_int, _P, _P, C.c_int64,
                                 _IP, _DP, _DP, C.c_int64, _I64P]),
     "ipeps_gs_energy": (C.c_int, [_P, _P, _P, C.c_int64, _DP, _DP]),
+    "ipeps_gs_energy_4x4": (C.c_int, [_P, _I64P, _P, _P, _P, _P, _DP, _DP]),
     "ipeps_gs_norm": (C.c_int, [_P, _DP]),
     "ipeps_step_adjoint": (C.c_int, [_P, C.c_int64, _P, _P, C.c_int64, _DP, _P]),
     "ipeps_gs_energy_grad": (C.c_int, [_P, C.c_int, _I64P, _P, _P, _P, C.c_int64, C.c_int64, C.c_int, C.c_int,

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libipeps_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-SOURCES = ["runtime.cpp", "gemm.hip", "elem.hip", "absorb.hip", "ops.cpp", "ctmrg.cpp", "nested.cpp", "corr.cpp", "capi.cpp"]
+SOURCES = ["runtime.cpp", "gemm.hip", "elem.hip", "absorb.hip", "ops.cpp", "ctmrg.cpp", "nested.cpp", "corr.cpp", "plaquette.cpp", "capi.cpp"]
 
 CFLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -468,6 +468,36 @@ int ipeps_gs_energy_grad(ipeps_ctx* ctx, int dtype, const int64_t dims[5], const
   });
 }
 
+int ipeps_gs_energy_4x4(ipeps_env* env, const int64_t* s_dims, const double* s1, const double* s2,
+                        const double* s3, const double* s4, double* E_out, double* N_out) {
+  return guard([&] {
+    IPEPS_CHECK(env && s_dims && s1 && s2 && s3 && s4, "null argument");
+    Ctx& c = *env->hold;
+    // s_dims: (d, d, r1), (r1, d, d, r2), (r2, d, d, r3), (r3, d, d), as 4 x 4 int64 (unused trailing = 1)
+    const int nds[4] = {3, 4, 4, 3};
+    const double* ptrs[4] = {s1, s2, s3, s4};
+    std::array<Tensor, 4> s;
+    for (int i = 0; i < 4; ++i) {
+      std::vector<int64_t> d(s_dims + 4 * i, s_dims + 4 * i + nds[i]);
+      for (auto x : d) IPEPS_CHECK(x > 0, "gs_energy_4x4: factor dims must be positive");
+      s[i] = upload(c, DType::C128, DType::C128, d, ptrs[i]);
+    }
+    const int64_t dphys = env->env.A.dims[4];
+    IPEPS_CHECK(s[0].dims[0] == dphys && s[0].dims[1] == dphys && s[1].dims[0] == s[0].dims[2] &&
+                    s[2].dims[0] == s[1].dims[3] && s[3].dims[0] == s[2].dims[3] && s[3].dims[1] == dphys,
+                "gs_energy_4x4: factor shapes do not chain (s1 (d,d,r1), s2 (r1,d,d,r2), s3 (r2,d,d,r3), s4 (r3,d,d))");
+    double out[4];
+    ctm::energy_4x4(c, env->env, s, out);
+    const double den = out[2] * out[2] + out[3] * out[3];
+    IPEPS_CHECK(den > 0.0, "gs_energy_4x4: zero norm");
+    if (E_out) {
+      E_out[0] = (out[0] * out[2] + out[1] * out[3]) / den;
+      E_out[1] = (out[1] * out[2] - out[0] * out[3]) / den;
+    }
+    if (N_out) { N_out[0] = out[2]; N_out[1] = out[3]; }
+  });
+}
+
 int ipeps_run_gs_vjp(ipeps_env* env, const void* x, const double* H1, const double* H2, int64_t hdim, int64_t chi,
                      int min_iter, int ad_max_iter, double tol, double dE, double* E_out, void* grad_out,
                      int* steps_out) {

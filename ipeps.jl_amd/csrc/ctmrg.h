@@ -67,6 +67,13 @@ Tensor energy(Ctx& c, const Env& e, const Tensor& H1, const Tensor& H2);
 // <Ad|single-site environment|A> (src/graph/evaluation.jl:47-59, 110-122).
 void gs_norm(Ctx& c, const Env& e, double out[2]);
 
+// 2 x 2 plaquette energy_norm_4x4 with A1..A4 = A (src/graph/evaluation.jl:
+// 340-355, 443-519) for the four-site Hamiltonian factors s1 (d,d,r1),
+// s2 (r1,d,d,r2), s3 (r2,d,d,r3), s4 (r3,d,d) of get_local_h: out = [Re, Im]
+// of energy, then of norm (get_gs_energy_4x4 = energy / norm).  Computed in
+// FP64 (complex when the environment or a factor is).
+void energy_4x4(Ctx& c, const Env& e, const std::array<Tensor, 4>& s, double out[4]);
+
 struct RunResult {
   int iters = 0;
   std::vector<double> diffs;

@@ -95,3 +95,19 @@ def correlation_TM(ts, direction="h", N=10, full=False):
         es = spla.eigs(TM, k=int(N), which="LM", return_eigenvectors=False)
     es1 = np.sort(np.abs(es))[::-1]
     return es1[es1 > 0], es, TM
+
+
+def get_gs_energy_4x4(ts, sl):
+    """get_gs_energy_4x4 (src/graph/evaluation.jl:340-355) on the GPU for the
+    factors sl = get_local_h(h) of a plaquette Hamiltonian: (energy / norm,
+    norm) as complex numbers."""
+    import numpy as np
+    fs = [np.asfortranarray(np.asarray(x), dtype=np.complex128) for x in sl]
+    dims = np.ones((4, 4), dtype=np.int64)
+    for i, f in enumerate(fs):
+        dims[i, : f.ndim] = f.shape
+    E, N = np.zeros(2), np.zeros(2)
+    dflat = np.ascontiguousarray(dims)
+    check(lib().ipeps_gs_energy_4x4(ts.h, dflat.ctypes.data_as(C.POINTER(C.c_int64)), *[_ptr(f) for f in fs],
+                                    E.ctypes.data_as(C.POINTER(C.c_double)), N.ctypes.data_as(C.POINTER(C.c_double))))
+    return complex(E[0], E[1]), complex(N[0], N[1])

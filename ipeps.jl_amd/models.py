@@ -7,6 +7,11 @@
 * ``init_hb_gs``  src/model/lee_gs.jl:2-28 (D = 2 loop-gas state)
 * ``act_Q_op``    src/model/lee_gs.jl:54-73 (the Q operator on a site tensor,
                   the "gauge" tangent basis of optim_es)
+* ``tout_site``, ``ising_h4``, ``honeycomb_h4``, ``hb_h4_ZZ``
+                  src/model/honeycomb.jl:17-68, basis.jl:37-55 (four-site
+                  plaquette Hamiltonians)
+* ``get_local_h`` src/model/local_h.jl:1-31 (a plaquette Hamiltonian as the
+                  product of four local factors, by sequential SVDs)
 """
 import math
 
@@ -110,3 +115,72 @@ def seeded_tensor(shape, seed, unit_max=False, dtype=np.float64):
     if unit_max:
         a = a / np.abs(a).max()
     return np.asfortranarray(a.astype(dtype))
+
+
+def tout_site(a, b):
+    """c[(i, j), (k, l)] = a[i, j] b[k, l] (honeycomb.jl:33-38)."""
+    c = np.einsum("ij,kl->ijkl", a, b)
+    sh = c.shape
+    return c.reshape(sh[0] * sh[1], sh[2] * sh[3], order="F")
+
+
+def ising_h4(h=2.5):
+    """Transverse-field Ising on a 2 x 2 plaquette (basis.jl:37-55)."""
+    II, Ix, xI = tout_site(sI, sI), tout_site(sI, sigmax), tout_site(sigmax, sI)
+    zI, xx = h * tout_site(sigmaz, sI), tout_site(sigmax, sigmax)
+    H = -tout_site(xx, II) - tout_site(Ix, xI) - tout_site(II, xx) - tout_site(xI, Ix) - tout_site(zI, II) \
+        - tout_site(II, zI)
+    d = sI.shape[0]
+    return H.reshape(d * d, d * d, d * d, d * d, order="F") / 2
+
+
+def honeycomb_h4():
+    """The XX-geometry honeycomb plaquette (honeycomb.jl:17-31)."""
+    II, Iy, yI = tout(sI, sI), tout(sI, sigmay), tout(sigmay, sI)
+    Iz, zI, xx = tout(sI, sigmaz), tout(sigmaz, sI), tout(sigmax, sigmax)
+    h1 = tout(tout(Iy, yI), tout(II, II))
+    h2 = tout(tout(II, II), tout(Iy, yI))
+    h3 = tout(tout(Iz, II), tout(zI, II))
+    h4 = tout(tout(II, Iz), tout(II, zI))
+    h5 = tout(tout(II, xx), tout(II, II))
+    h6 = tout(tout(II, II), tout(xx, II))
+    return (-h1 - h3 - h5 - h2 - h4 - h6) / 2
+
+
+def hb_h4_ZZ():
+    """The ZZ-geometry honeycomb plaquette, sites 1-2-3-4 around it (honeycomb.jl:46-68)."""
+    II, Iy, yI = tout(sI, sI), tout(sI, sigmay), tout(sigmay, sI)
+    Ix, xI, zz = tout(sI, sigmax), tout(sigmax, sI), tout(sigmaz, sigmaz)
+    ts = tout_site
+    H = -ts(ts(II, Ix), ts(xI, II)) - ts(ts(Ix, II), ts(II, xI)) - ts(ts(Iy, yI), ts(II, II)) \
+        - ts(ts(II, II), ts(yI, Iy)) - ts(ts(II, zz), ts(II, II)) - ts(ts(II, II), ts(II, zz))
+    d = II.shape[0]
+    return H.reshape(d * d, d * d, d * d, d * d, order="F") / 2
+
+
+def _svd_sqrt(x):
+    # wrap_svd (src/utility/wrapper.jl:25-55, cutoff 1e-12, no cap), then u sqrt(s), sqrt(s) v^H
+    u, s, vh = np.linalg.svd(x, full_matrices=False)
+    n = int((s / s.max() > 1e-12).sum())
+    if s.size > n and abs(s[n] - s[n - 1]) < 1e-10:
+        n += 1
+    r = np.sqrt(s[:n])
+    return u[:, :n] * r, r[:, None] * vh[:n]
+
+
+def get_local_h(h):
+    """local_h.jl:1-31: [s1 (d,d,r1), s2 (r1,d,d,r2), s3 (r2,d,d,r3),
+    s4 (r3,d,d)] with h[(k1 b1), (k2 b2), (k3 b3), (k4 b4)] = s1 s2 s3 s4."""
+    h = np.asarray(h)
+    dims = h.shape
+    d = int(round(math.sqrt(dims[0])))
+    sl = []
+    s1, sd = _svd_sqrt(h.reshape(dims[0], -1, order="F"))
+    sl.append(s1.reshape(d, d, -1, order="F"))
+    for i in range(len(dims) - 2):
+        dl = sd.shape[0]
+        si, sd = _svd_sqrt(sd.reshape(dl * dims[i], -1, order="F"))
+        sl.append(si.reshape(dl, d, d, -1, order="F"))
+    dl = sd.shape[0]
+    sl.append(sd.reshape(dl, d, d, order="F"))
+    return sl

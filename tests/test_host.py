@@ -50,3 +50,18 @@ def test_act_Q_op_reproduces_loop_gas_state(pkg):
     assert np.abs(cols @ x - ref.reshape(-1, order="F")).max() < 1e-13
     Az = M.act_Q_op(B0, add=3)
     assert np.abs(Az - ref).max() > 1e-3
+
+
+def test_plaquette_models_match_oracle(pkg):
+    """The plaquette Hamiltonians and get_local_h of models.py against the
+    oracle's restatement: equal tensors, and factors whose product is h."""
+    import torch
+    import plaquette_oracle as P
+    M = pkg.models
+    np.testing.assert_array_equal(M.ising_h4(2.5), P.ising_h4(2.5).numpy())
+    np.testing.assert_allclose(M.hb_h4_ZZ(), P.hb_h4_ZZ().numpy(), atol=0)
+    for h in (M.ising_h4(2.5), M.hb_h4_ZZ(), M.honeycomb_h4().reshape(16, 16, 16, 16, order="F")):
+        s1, s2, s3, s4 = M.get_local_h(h)
+        d = int(round(h.shape[0] ** 0.5))
+        rec = np.einsum("abi,icdj,jefk,kgh->abcdefgh", s1, s2, s3, s4)
+        np.testing.assert_allclose(rec, h.reshape((d,) * 8, order="F"), atol=1e-12)
