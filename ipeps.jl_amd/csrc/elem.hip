@@ -612,17 +612,22 @@ __device__ inline double dev_from_eye(double2 g, bool d) {
 }
 
 // one workgroup; a NaN anywhere makes the result +inf
+constexpr int EYE_TPB = 1024;
 template <typename T>
-__global__ __launch_bounds__(TPB) void eye_dev_kernel(int64_t n, const T* G, double* out) {
-  __shared__ double red[TPB];
+__global__ __launch_bounds__(EYE_TPB) void eye_dev_kernel(int n, const T* G, double* out) {
+  __shared__ double red[EYE_TPB];
   double m = 0.0;
-  for (int64_t i = threadIdx.x; i < n * n; i += TPB) {
-    double v = dev_from_eye(G[i], (i % n) == (i / n));
-    m = isnan(v) ? INFINITY : fmax(m, v);
+  // column by column (32-bit index math): thread t covers rows t, t + 1024, ...
+  for (int col = 0; col < n; ++col) {
+    const T* g = G + (int64_t)col * n;
+    for (int r = threadIdx.x; r < n; r += EYE_TPB) {
+      const double v = dev_from_eye(g[r], r == col);
+      m = isnan(v) ? INFINITY : fmax(m, v);
+    }
   }
   red[threadIdx.x] = m;
   __syncthreads();
-  for (int s = TPB / 2; s > 0; s >>= 1) {
+  for (int s = EYE_TPB / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
@@ -1000,7 +1005,8 @@ void energy2_back(Ctx& c, DType dt, int64_t m, const void* rho_h, const void* rh
 
 void eye_dev(Ctx& c, DType dt, int64_t n, const void* G, double* out) {
   IPEPS_CHECK(n >= 1, "eye_dev: empty matrix");
-#define EK(T, ...) hipLaunchKernelGGL(eye_dev_kernel<T>, dim3(1), dim3(TPB), 0, c.stream, n, (const T*)G, out)
+  IPEPS_CHECK(n <= 46340, "eye_dev: n too large");
+#define EK(T, ...) hipLaunchKernelGGL(eye_dev_kernel<T>, dim3(1), dim3(EYE_TPB), 0, c.stream, (int)n, (const T*)G, out)
   DISPATCH3(dt, EK, 0);
 #undef EK
   HIP_CHECK(hipGetLastError());

@@ -1231,6 +1231,18 @@ static bool orth_cholqr2_small(Ctx& c, const Tensor& Y, Tensor& Q) {
   return !failed(hs, 2);
 }
 
+// One CholeskyQR pass (FP64, n <= 320) with no host synchronisation: Q = Y L^-T
+// from the Cholesky factor L of Y^T Y; the blocks' infos go to info[0..1]
+// for a later check.  For the intermediate bases of the power iterations,
+// which only carry the span (orthogonality error ~ u cond(Y)^2).
+static Tensor orth_cholqr1_nosync(Ctx& c, const Tensor& Y, int* info) {
+  const int64_t n = Y.dims[1];
+  Tensor G = matmul_raw(c, Y, true, false, Y, false, false);
+  Tensor Li = c.empty(DType::F64, {n, n});
+  chol_inv(c, (int)n, G.ptr<double>(), n, Li.ptr<double>(), n, info, info + 1);
+  return matmul_raw(c, Y, false, false, Li, true, false);
+}
+
 // The second pass is skipped when the first already left Q orthonormal to
 // |Q^H Q - I|_max <= 1e-12 (well-conditioned Y: cond(Y)^2 u well below that).
 static bool orth_cholqr2(Ctx& c, const Tensor& Y, Tensor& Q, bool small_chol) {
@@ -1499,11 +1511,47 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     };
     const int fo = o.fast_orth;
     OrthStatsScope oss(o.stats.get());
-    Tensor Q = orth(c, to_dtype(c, apply_M(Om), sdt), fo);
+    // FP64 bases of at most 320 columns: the power iterations' bases by one
+    // CholeskyQR pass each with no host round trip, their Cholesky infos
+    // checked once at the end; the final basis by CholeskyQR2 with its checks.
+    // If any pass failed the range finder is run again with checked bases.
+    const bool lazy = sdt == DType::F64 && fo == 2 && kk <= 2 * k::SCHOL_MAXN_HOST && o.power_iters > 0;
+    Tensor Q;
     int q = 0;
-    for (; q < o.power_iters; ++q) {
-      Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), fo);
-      Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), fo);
+    bool lazy_ok = false;
+    if (lazy) {
+      Tensor infos = c.zeros(DType::F64, {2 * (2 * o.power_iters + 1)});
+      int slot = 0;
+      auto info = [&]() { return (int*)(infos.ptr<double>() + 2 * (slot++)); };
+      Q = orth_cholqr1_nosync(c, to_dtype(c, apply_M(Om), sdt), info());
+      for (; q < o.power_iters; ++q) {
+        Tensor Z = orth_cholqr1_nosync(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), info());
+        Q = orth_cholqr1_nosync(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), info());
+      }
+      std::vector<double> hi(infos.numel());
+      HIP_CHECK(hipMemcpyAsync(hi.data(), infos.data(), infos.bytes(), hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+      lazy_ok = true;
+      for (double v : hi) {
+        int a, b;
+        std::memcpy(&a, &v, sizeof(int));
+        std::memcpy(&b, (const char*)&v + sizeof(int), sizeof(int));
+        lazy_ok = lazy_ok && a == 0 && b == 0;
+      }
+      if (lazy_ok) {
+        orth_count(&SvdStats::orth_calls);
+        Q = orth(c, Q, fo);  // re-orthonormalise the last basis (checked)
+      } else {
+        orth_count(&SvdStats::orth_householder);
+        q = 0;
+      }
+    }
+    if (!lazy_ok) {
+      Q = orth(c, to_dtype(c, apply_M(Om), sdt), fo);
+      for (; q < o.power_iters; ++q) {
+        Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), fo);
+        Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), fo);
+      }
     }
     const int qmax = std::max(o.power_iters, o.power_iters_max);
     double resid = -1.0;
