@@ -117,7 +117,9 @@ int ipeps_env_destroy(ipeps_env* env);
  * kept triplets and those beyond the computed k: 0 = their s_j -> 0 limit,
  * the thin-SVD terms; > 0 = solved exactly by conjugate gradients, at most
  * that many iterations, to the relative residual complement_tol, default
- * 1e-8).  "draws" sets the position in the environment's
+ * 1e-8), and "warm_start" (0/1, default 0: each move's range finder starts
+ * from that move's previous left Ritz basis, kept in the environment, instead
+ * of a random test matrix; the power iterations then refine it).  "draws" sets the position in the environment's
  * stream of random test matrices (keyed by "seed"; it advances by one per
  * randomized projector): environments with equal seed, draws and tensors
  * build identical projectors whatever their dtype. */
@@ -135,9 +137,15 @@ int ipeps_env_get_draws(ipeps_env* env, int64_t* out);
  * adjoint, their CG iterations (sum), largest final relative CG residual,
  * solves abandoned for a non-positive-definite operator (thin terms used),
  * range-basis orthonormalisations, those that took CholeskyQR2's second
- * pass, those that fell back to Householder QR].  reset != 0 clears them
- * after the read. */
-int ipeps_env_projector_stats(ipeps_env* env, double out[12], int reset);
+ * pass, those that fell back to Householder QR, projectors warm-started].
+ * reset != 0 clears them after the read. */
+int ipeps_env_projector_stats(ipeps_env* env, double out[13], int reset);
+/* Copy the environment state of src into dst (same A shape): C, E
+ * (converted to dst's dtype), the position in the test-matrix stream and the
+ * warm-start bases, so dst continues exactly as src would -- the state a
+ * copied CTMTensors carries (src/ctm/ctm_tensor.jl:3-18 has no projector
+ * state; its projectors are exact SVDs).  A is not copied. */
+int ipeps_env_copy_state(ipeps_env* dst, const ipeps_env* src);
 /* which: 0..3 = C1..C4, 4..7 = E1..E4, 8 = A.  dims has room for 5. */
 int ipeps_env_tensor_dims(ipeps_env* env, int which, int64_t dims[5], int* nd);
 /* Copy out / in (host data in the env's host dtype: F64 or C128). */

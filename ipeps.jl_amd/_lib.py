@@ -33,6 +33,7 @@ SIGNATURES = {
     "ipeps_env_set_option": (C.c_int, [_P, C.c_char_p, C.c_double]),
     "ipeps_env_get_draws": (C.c_int, [_P, _I64P]),
     "ipeps_env_projector_stats": (C.c_int, [_P, _DP, C.c_int]),
+    "ipeps_env_copy_state": (C.c_int, [_P, _P]),
     "ipeps_env_tensor_dims": (C.c_int, [_P, C.c_int, _I64P, _IP]),
     "ipeps_env_get_tensor": (C.c_int, [_P, C.c_int, _P]),
     "ipeps_env_set_tensor": (C.c_int, [_P, C.c_int, C.c_int, _I64P, _P]),

@@ -114,6 +114,7 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
   else if (key == "fast_svd") o.svd.fast_svd = (int)v;
   else if (key == "gram_guard") o.svd.gram_guard = v;
   else if (key == "power_iters_max") o.svd.power_iters_max = (int)v;
+  else if (key == "warm_start") o.svd.warm_start = v != 0.0;
   else if (key == "resid_tol") o.svd.resid_tol = v;
   else if (key == "complement_iters") {
     IPEPS_CHECK(v >= 0, "complement_iters must be >= 0");
@@ -320,7 +321,31 @@ int ipeps_env_get_draws(ipeps_env* env, int64_t* out) {
   });
 }
 
-int ipeps_env_projector_stats(ipeps_env* env, double out[12], int reset) {
+int ipeps_env_copy_state(ipeps_env* dst, const ipeps_env* src) {
+  return guard([&] {
+    IPEPS_CHECK(dst && src, "null argument");
+    IPEPS_CHECK(dst->env.A.dims == src->env.A.dims, "copy_state: A shapes differ");
+    Ctx& c = *dst->hold;
+    auto cp = [&](const Tensor& t, DType dt) {
+      Tensor r = c.empty(t.dt, t.dims);
+      if (t.bytes()) HIP_CHECK(hipMemcpyAsync(r.data(), t.data(), t.bytes(), hipMemcpyDeviceToDevice, c.stream));
+      return op::to_dtype(c, r, dt);
+    };
+    // the source's stream must have finished writing what is copied
+    src->hold->sync();
+    for (int i = 0; i < 4; ++i) {
+      dst->env.C[i] = cp(src->env.C[i], dst->env.dt);
+      dst->env.E[i] = cp(src->env.E[i], dst->env.dt);
+      const DType wdt = dst->env.dt == DType::C128 ? DType::C128 : DType::F64;
+      dst->env.warm[i] = src->env.warm[i].defined() && src->env.warm[i].dt == wdt ? cp(src->env.warm[i], wdt)
+                                                                                   : Tensor();
+    }
+    dst->env.draws = src->env.draws;
+    c.sync();
+  });
+}
+
+int ipeps_env_projector_stats(ipeps_env* env, double out[13], int reset) {
   return guard([&] {
     IPEPS_CHECK(env && out, "null argument");
     auto& st = env->env.opt.svd.stats;
@@ -339,6 +364,7 @@ int ipeps_env_projector_stats(ipeps_env* env, double out[12], int reset) {
       out[9] = (double)st->orth_calls;
       out[10] = (double)st->orth_second;
       out[11] = (double)st->orth_householder;
+      out[12] = (double)st->warm_starts;
     }
     if (reset) st->reset();
   });

@@ -36,6 +36,11 @@ struct Proj {
 Proj get_projector(Ctx& c, const Env& e, const Tensor& R1, const Tensor& R2, int64_t chi, uint64_t draw) {
   op::SvdOpts so = e.opt.svd;
   so.seed = op::draw_seed(e.opt.svd.seed, draw);
+  const uint64_t dir = draw - e.draws;
+  if (so.warm_start && dir < 4) {
+    so.warm_in = &e.warm[dir];
+    so.warm_out = &e.warm[dir];
+  }
   op::PrecScope ps(e.opt.mixed_proj ? (k::Mixed)e.opt.mixed_proj : op::prec().mixed, op::prec().back);
   auto r = op::svd_trunc(c, R1, R2, chi, so);
   Tensor isq = op::inv_sqrt(c, r.S);
@@ -523,13 +528,14 @@ RunResult run_ctm(Ctx& c, Env& e, int64_t chi, int min_iter, int max_iter, doubl
 namespace {
 
 struct Checkpoint {
-  std::array<Tensor, 4> C, E;
+  std::array<Tensor, 4> C, E, warm;
   uint64_t draws = 0;
 };
 
 Checkpoint save(const Env& e) {
   Checkpoint k;
   k.draws = e.draws;
+  k.warm = e.warm;
   for (int i = 0; i < 4; ++i) {
     k.C[i] = e.C[i];
     k.E[i] = e.E[i];
@@ -541,6 +547,7 @@ Checkpoint save(const Env& e) {
 
 void load(Env& e, const Checkpoint& k) {
   e.draws = k.draws;
+  e.warm = k.warm;
   for (int i = 0; i < 4; ++i) {
     e.C[i] = k.C[i];
     e.E[i] = k.E[i];

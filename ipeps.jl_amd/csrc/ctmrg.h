@@ -41,6 +41,10 @@ struct Env {
   // the same matrices whatever their dtype or move schedule, and a
   // recomputed step replays its forward draws.
   mutable uint64_t draws = 0;
+  // Warm start of the randomized projectors (option warm_start): each move's
+  // last left Ritz basis (m x k, FP64 / complex FP64), indexed by direction
+  // (left, right, top, bottom); part of the environment's state.
+  mutable std::array<Tensor, 4> warm;
 };
 
 // Boundary initialisation C_i, E_i from A and Ad (src/ctm/ctm_tensor.jl:97-130).

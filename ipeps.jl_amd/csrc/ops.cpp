@@ -1501,8 +1501,13 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     gesvd(c, to_dtype(c, M, sdt), Uf, Sf, Vf, true);
   } else {
     const int64_t kk = std::min<int64_t>(kmin, ntar + o.oversample);
-    Tensor Om = c.empty(dt, {nc, kk});
-    k::randn(c, dt, nc * kk, o.seed, Om.data());
+    const bool warm = o.warm_start && o.warm_in && o.warm_in->defined() && o.warm_in->dt == sdt &&
+                      o.warm_in->nd() == 2 && o.warm_in->dims[0] == m && o.warm_in->dims[1] == kk;
+    Tensor Om;
+    if (!warm) {
+      Om = c.empty(dt, {nc, kk});
+      k::randn(c, dt, nc * kk, o.seed, Om.data());
+    }
     auto apply_M = [&](const Tensor& X) {  // M X = R1 (R2 X)
       return matmul_raw(c, R1, false, false, matmul_raw(c, R2, false, false, X, false, false), false, false);
     };
@@ -1523,7 +1528,7 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
       Tensor infos = c.zeros(DType::F64, {2 * (2 * o.power_iters + 1)});
       int slot = 0;
       auto info = [&]() { return (int*)(infos.ptr<double>() + 2 * (slot++)); };
-      Q = orth_cholqr1_nosync(c, to_dtype(c, apply_M(Om), sdt), info());
+      Q = warm ? *o.warm_in : orth_cholqr1_nosync(c, to_dtype(c, apply_M(Om), sdt), info());
       for (; q < o.power_iters; ++q) {
         Tensor Z = orth_cholqr1_nosync(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), info());
         Q = orth_cholqr1_nosync(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), info());
@@ -1547,7 +1552,7 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
       }
     }
     if (!lazy_ok) {
-      Q = orth(c, to_dtype(c, apply_M(Om), sdt), fo);
+      Q = warm ? orth(c, *o.warm_in, fo) : orth(c, to_dtype(c, apply_M(Om), sdt), fo);
       for (; q < o.power_iters; ++q) {
         Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), fo);
         Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), fo);
@@ -1599,7 +1604,9 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
       o.stats->power_iters += q;
       o.stats->power_max = std::max<long>(o.stats->power_max, q);
       if (resid >= 0.0) o.stats->resid_max = std::max(o.stats->resid_max, resid);
+      if (warm) ++o.stats->warm_starts;
     }
+    if (o.warm_start && o.warm_out) *o.warm_out = Uf;
   }
   const int64_t ns = Sf.numel();
   out.s_host.resize(ns);

@@ -163,10 +163,11 @@ struct SvdStats {
   long orth_calls = 0;     // range-basis orthonormalisations
   long orth_second = 0;    // ... that needed CholeskyQR2's second pass
   long orth_householder = 0;  // ... that fell back to Householder QR
+  long warm_starts = 0;    // projectors started from the previous one's basis
   void reset() {
     std::lock_guard<std::mutex> g(mu);
     calls = power_iters = power_max = checks = cg_calls = cg_iters = cg_fallbacks = 0;
-    orth_calls = orth_second = orth_householder = 0;
+    orth_calls = orth_second = orth_householder = warm_starts = 0;
     resid_max = cg_resid_max = 0.0;
   }
 };
@@ -211,6 +212,13 @@ struct SvdOpts {
   int complement_iters = 0;
   double complement_tol = 1e-8;
   std::shared_ptr<SvdStats> stats;  // optional counters
+  // Warm start: with warm_start set and warm_in holding an m x k basis in the
+  // solver dtype, the range finder starts from it (the previous projector's
+  // left Ritz vectors of the same move) instead of M Omega; warm_out receives
+  // this projector's left Ritz vectors (m x k) for the next one.
+  int warm_start = 0;
+  const Tensor* warm_in = nullptr;
+  Tensor* warm_out = nullptr;
 };
 
 // Truncated SVD with the reference cutoff rule (src/utility/wrapper.jl:25-55):

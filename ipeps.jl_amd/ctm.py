@@ -115,7 +115,7 @@ def _hpair(H):
 
 ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "eta", "mixed_cutoff", "fast_orth",
                "fast_svd",
-               "gram_guard", "power_iters_max", "resid_tol", "complement_iters", "complement_tol", "canonical_edge", "mixed", "mixed_back", "mixed_proj", "energy_fp64", "energy_mixed",
+               "gram_guard", "power_iters_max", "resid_tol", "complement_iters", "complement_tol", "warm_start", "canonical_edge", "mixed", "mixed_back", "mixed_proj", "energy_fp64", "energy_mixed",
                "energy_mixed_back",
                "checkpoint", "fused_absorb", "paired_moves", "draws")
 
@@ -204,12 +204,18 @@ class CTMTensors:
     def projector_stats(self, reset=False):
         """Counters of the randomized projectors since the last reset: dict with
         calls, power_iters, power_max, resid_max, checks, cg_calls, cg_iters,
-        cg_resid_max, cg_fallbacks, orth_calls, orth_second, orth_householder."""
-        out = np.zeros(12)
+        cg_resid_max, cg_fallbacks, orth_calls, orth_second, orth_householder,
+        warm_starts."""
+        out = np.zeros(13)
         check(lib().ipeps_env_projector_stats(self.h, _dp(out), int(bool(reset))))
         keys = ("calls", "power_iters", "power_max", "resid_max", "checks", "cg_calls", "cg_iters", "cg_resid_max",
-                "cg_fallbacks", "orth_calls", "orth_second", "orth_householder")
+                "cg_fallbacks", "orth_calls", "orth_second", "orth_householder", "warm_starts")
         return {k: (float(v) if k.endswith("max") and "power" not in k else int(v)) for k, v in zip(keys, out)}
+
+    def copy_state_from(self, other):
+        """C, E, the test-matrix stream position and the warm-start bases of
+        ``other`` (same A shape), so this environment continues as it would."""
+        check(lib().ipeps_env_copy_state(self.h, other.h))
 
     def set_env(self, Cs, Es):
         for i in range(4):
