@@ -65,10 +65,19 @@ LIBRARY_PREFIXES = ("rocsolver_", "rocblas_")
 
 
 def dominant_kernel(stats):
-    """Hand-written kernel with the largest total time (library calls, which
-    carry no algorithmic counts, are reported separately)."""
-    own = {k: v for k, v in stats.items() if not k.startswith(LIBRARY_PREFIXES)}
+    """Hand-written throughput kernel with the largest total time.  Library
+    calls carry no algorithmic counts, and the one-workgroup FP64
+    factorisations (Cholesky + inverse, tridiagonalisation: kernels that
+    report no algorithmic bytes) are latency chains with no roofline; both
+    are reported separately (library_ms, latency_bound)."""
+    own = {k: v for k, v in stats.items() if not k.startswith(LIBRARY_PREFIXES) and v[3] > 0}
     return max(own.items(), key=lambda kv: kv[1][1])[0] if own else None
+
+
+def latency_bound(stats, top=4):
+    """The hand-written kernels without algorithmic bytes: {name: [launches, ms]}."""
+    lat = {k: v for k, v in stats.items() if not k.startswith(LIBRARY_PREFIXES) and v[3] <= 0}
+    return {k: [v[0], round(v[1], 3)] for k, v in sorted(lat.items(), key=lambda kv: -kv[1][1])[:top]}
 
 
 def kernel_roofline(stats, name=None):
@@ -373,7 +382,7 @@ def main():
         "model_flops_rate_tflops": F * value / 1e12,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "step_mix": {"timed_kernels_ms": tot_mix, "rocsolver_rocblas_ms": lib_ms,
+        "step_mix": {"timed_kernels_ms": tot_mix, "rocsolver_rocblas_ms": lib_ms, "latency_bound": latency_bound(mix_m),
                      "top": {k: round(v[1], 3) for k, v in sorted(mix_m.items(), key=lambda kv: -kv[1][1])[:8]}},
     }
     if a.tune:
