@@ -214,6 +214,72 @@ def batch_c4(pkg, steps=10, workers=8):
     return out
 
 
+C5_MOMENTA = [(1.0, 0.0), (1.0, 1.0), (0.5, 0.5), (0.0, 0.0), (1.0, 0.0), (0.5, 0.5)]  # units of pi
+
+
+def excited_c5(pkg, args):
+    """Config 5: Ising h=2, A (3,3,3,3,2) from default_rng(4) optimised in FP64
+    (optim_gs, LBFGS), chi = 48; prepare_basis gives D^4 d - 1 = 161 tangent
+    vectors; at each momentum every column is a nested CTMRG + adjoint on the
+    GPU (optim_es on `workers` contexts) and compute_es gives the lowest 4
+    bands.  The production projector (randomized, k = chi + 48, q = 2) against
+    a reference projector at the first --c5-ref (pi/2, pi/2) momenta: by
+    default k = chi + 200, q = 6 with the exact discarded-subspace adjoint,
+    with --c5-ref-full the full SVD (svd_mode=0, reference semantics)."""
+    import numpy as np
+    M = pkg.models
+    H = M.ising(2.0)
+    cfg = dict(chi=48, min_iter=4, max_iter=20, ad_max_iter=4, rg_tol=1e-6)
+    A0 = M.seeded_tensor((3, 3, 3, 3, 2), 4)
+    t0 = time.perf_counter()
+    r = pkg.optim_gs(H, A0, cfg, iterations=args.c5_gs_iters, dtype="f64",
+                     options=dict(svd_mode=1, oversample=48, power_iters=2))
+    t_gs = time.perf_counter() - t0
+    log(f"[c5] ground state: {r['iterations']} LBFGS iterations, E = {r['energy']:.12f}, {t_gs:.1f} s")
+    prod = dict(svd_mode=1, oversample=48, power_iters=2)
+    t0 = time.perf_counter()
+    basis, ts, Hs = pkg.excited.prepare_basis(r["A"].astype(np.complex128), H, cfg, dtype="c128", options=prod)
+    log(f"[c5] basis {basis.shape} in {time.perf_counter() - t0:.1f} s")
+    momenta = C5_MOMENTA[: args.c5_momenta]
+
+    def progress(done, total):
+        if done % 20 == 0 or done == total:
+            log(f"[c5]   {done}/{total} columns")
+
+    t0 = time.perf_counter()
+    bands = pkg.excited.band_structure(ts, Hs, basis, momenta, cfg, workers=args.c5_workers, progress=progress)
+    t_bands = time.perf_counter() - t0
+    for b in bands:
+        log(f"[c5] ({b['px']}, {b['py']}) pi: bands {np.round(b['bands'], 8).tolist()} in {b['seconds']:.1f} s")
+    ref = []
+    if args.c5_ref:
+        # reference semantics: the full SVD of M (svd_mode=0, ~20 s per column), or a projector that resolves
+        # the kept subspace with the discarded-subspace adjoint solved exactly (the default)
+        ref_opts = dict(svd_mode=0) if args.c5_ref_full else dict(svd_mode=1, oversample=200, power_iters=6,
+                                                                    complement_iters=60, complement_tol=1e-10)
+        ts_ref = pkg.CTMTensors(ts.A, cfg, dtype="c128", options=ref_opts)
+        ts_ref.set_env(ts.Cs, ts.Es)
+        idx = [i for i, m in enumerate(momenta) if m == (0.5, 0.5)][: args.c5_ref]
+        for i in idx:
+            rb = pkg.excited.band_structure(ts_ref, Hs, basis, [momenta[i]], cfg, workers=args.c5_workers,
+                                            progress=progress)[0]
+            err = float(np.abs(rb["bands"] - bands[i]["bands"]).max())
+            ref.append(dict(px=rb["px"], py=rb["py"], bands_ref=rb["bands"].tolist(), max_band_err=err,
+                            seconds=rb["seconds"], projector=ref_opts))
+            log(f"[c5] reference at ({rb['px']}, {rb['py']}) pi: {np.round(rb['bands'], 8).tolist()}, "
+                f"max |band error| {err:.2e}, {rb['seconds']:.1f} s")
+    ncol = basis.shape[1] * len(momenta)
+    print(json.dumps({
+        "metric": "config-5 excited-state columns/s (nested CTMRG + adjoint per tangent vector)",
+        "value": ncol / t_bands, "unit": "columns/s", "n_gpus": 1, "higher_is_better": True,
+        "config": {"workload": "Ising h=2, A (3,3,3,3,2) seed 4 optimised in FP64, chi=48, 161 tangent vectors",
+                   "momenta_pi": momenta, "workers": args.c5_workers, "projector": prod},
+        "seconds": {"ground_state": t_gs, "bands": t_bands}, "columns": ncol,
+        "gs_energy": r["energy"], "gs_iterations": r["iterations"],
+        "bands": [dict(px=b["px"], py=b["py"], lowest=b["bands"].tolist()) for b in bands],
+        "reference": ref}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,11 +306,21 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--c5", action="store_true",
+                    help="run the config-5 excited-state band structure instead (one JSON line of its own)")
+    ap.add_argument("--c5-momenta", type=int, default=6)
+    ap.add_argument("--c5-ref", type=int, default=1, help="momenta compared with the full-SVD engine")
+    ap.add_argument("--c5-workers", type=int, default=8)
+    ap.add_argument("--c5-gs-iters", type=int, default=40)
+    ap.add_argument("--c5-ref-full", action="store_true", help="reference with the full SVD (slow)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(independent instances shard with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.c5:
+        excited_c5(load_pkg(), a)
+        return
     w = WORKLOADS[a.workload]
     D, chi, d = w["D"], w["chi"], w["d"]
     pkg = load_pkg()

@@ -11,6 +11,8 @@ Mirrors src/optimize/optim_es.jl and src/graph/expectation.jl:
 * ``compute_es(effH, effN, nrmB_cut)``      expectation.jl:13-56: the small
   generalised eigenproblem, host FP64 (LAPACK via scipy), as in the reference
 * ``make_es_path(step)``                    optim_es.jl:1-20
+* ``band_structure(ts, H, basis, momenta, cfg)`` the bands over a momentum path
+                                            (optim_es + compute_es per point)
 """
 import ctypes as C
 
@@ -42,7 +44,7 @@ def get_es_grad(ts, H, Bi, px, py, cfg):
     return gH.reshape(-1, order="F"), envB.reshape(-1, order="F"), (float(E4[0]), complex(Nb[0], Nb[1]))
 
 
-def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1):
+def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1, progress=None):
     """effH, effN, envB for the tangent basis at momentum (px, py) pi (optim_es.jl:127-153).
 
     The columns are independent nested runs (the reference splits them over
@@ -64,10 +66,15 @@ def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1):
     envB = np.zeros(basis.shape, dtype=np.complex128)
     cols = list(range(n) if columns is None else columns)
 
+    done = [0]
+
     def store(i, gH, gN):
         envB[:, i] = gN
         effH[:, i] = basis.conj().T @ gH / 2
         effN[:, i] = basis.conj().T @ gN
+        done[0] += 1
+        if progress is not None:
+            progress(done[0], len(cols))
 
     if workers <= 1 or len(cols) <= 1:
         for i in cols:
@@ -80,14 +87,19 @@ def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1):
     ctxs = [Context() for _ in range(nw - 1)]
     envs = [ts] + [ts.clone(c) for c in ctxs]
 
+    import threading
+    lock = threading.Lock()
+
     def run(w):
-        return [(i,) + get_es_grad(envs[w], H, basis[:, i], px, py, cfg)[:2] for i in cols[w::nw]]
+        for i in cols[w::nw]:
+            gH, gN = get_es_grad(envs[w], H, basis[:, i], px, py, cfg)[:2]
+            with lock:
+                store(i, gH, gN)
 
     try:
         with ThreadPoolExecutor(nw) as ex:
-            for part in ex.map(run, range(nw)):
-                for i, gH, gN in part:
-                    store(i, gH, gN)
+            for _ in ex.map(run, range(nw)):
+                pass
     finally:
         for e in envs[1:]:
             e.__del__()
@@ -194,6 +206,23 @@ def compute_es(effH, effN, nrmB_cut=1e-3):
     es, vecs = sla.eigh(H2, N2)
     ixs = np.argsort(es.real)
     return es[ixs], vecs[:, ixs], P
+
+
+def band_structure(ts, H, basis, momenta, cfg, n_bands=4, nrmB_cut=1e-3, workers=1, progress=None):
+    """The excited-state pipeline over a list of momenta (px, py in units of
+    pi): at each, the effH / effN columns of every tangent basis vector
+    (optim_es, optim_es.jl:91-165) and the generalised eigenproblem
+    (compute_es, expectation.jl:13-56).  Returns a list of dicts with the
+    momentum, the lowest n_bands energies, effH, effN and the seconds taken."""
+    import time
+    out = []
+    for px, py in momenta:
+        t0 = time.perf_counter()
+        effH, effN, _ = optim_es(ts, H, basis, px, py, cfg, workers=workers, progress=progress)
+        es, _, _ = compute_es(effH, effN, nrmB_cut)
+        out.append(dict(px=float(px), py=float(py), bands=np.real(es[:n_bands]), effH=effH, effN=effN,
+                        seconds=time.perf_counter() - t0))
+    return out
 
 
 def make_es_path(step=0.1):
