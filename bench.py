@@ -222,10 +222,13 @@ def excited_c5(pkg, args):
     (optim_gs, LBFGS), chi = 48; prepare_basis gives D^4 d - 1 = 161 tangent
     vectors; at each momentum every column is a nested CTMRG + adjoint on the
     GPU (optim_es on `workers` contexts) and compute_es gives the lowest 4
-    bands.  The production projector (randomized, k = chi + 48, q = 2) against
-    a reference projector at the first --c5-ref (pi/2, pi/2) momenta: by
-    default k = chi + 200, q = 6 with the exact discarded-subspace adjoint,
-    with --c5-ref-full the full SVD (svd_mode=0, reference semantics)."""
+    bands.  Every column and momentum takes its projectors from one shared
+    path (the plain parts of the nested runs do not depend on B or the
+    momentum), made once per projector setting.  The production projector
+    (randomized, k = chi + 300, q = 8: made once, its cost is that of a few
+    columns) is compared with the full SVD
+    (svd_mode=0, reference semantics) at the first --c5-ref momenta of
+    (pi/2, pi/2), (pi, 0), (pi, pi), (0, 0)."""
     import numpy as np
     M = pkg.models
     H = M.ising(2.0)
@@ -236,10 +239,12 @@ def excited_c5(pkg, args):
                      options=dict(svd_mode=1, oversample=48, power_iters=2))
     t_gs = time.perf_counter() - t0
     log(f"[c5] ground state: {r['iterations']} LBFGS iterations, E = {r['energy']:.12f}, {t_gs:.1f} s")
-    prod = dict(svd_mode=1, oversample=48, power_iters=2)
+    prod = dict(svd_mode=1, oversample=args.c5_oversample, power_iters=args.c5_power_iters)
     t0 = time.perf_counter()
     basis, ts, Hs = pkg.excited.prepare_basis(r["A"].astype(np.complex128), H, cfg, dtype="c128", options=prod)
     log(f"[c5] basis {basis.shape} in {time.perf_counter() - t0:.1f} s")
+    # the nested runs' stopping rule (rg_tol) may be set apart from the ground state's
+    cfg = dict(cfg, rg_tol=args.c5_rg_tol, max_iter=args.c5_max_iter)
     momenta = C5_MOMENTA[: args.c5_momenta]
 
     def progress(done, total):
@@ -253,28 +258,48 @@ def excited_c5(pkg, args):
         log(f"[c5] ({b['px']}, {b['py']}) pi: bands {np.round(b['bands'], 8).tolist()} in {b['seconds']:.1f} s")
     ref = []
     if args.c5_ref:
-        # reference semantics: the full SVD of M (svd_mode=0, ~20 s per column), or a projector that resolves
-        # the kept subspace with the discarded-subspace adjoint solved exactly (the default)
-        ref_opts = dict(svd_mode=0) if args.c5_ref_full else dict(svd_mode=1, oversample=200, power_iters=6,
-                                                                    complement_iters=60, complement_tol=1e-10)
+        # reference semantics: the full SVD of M (svd_mode=0)
+        ref_opts = dict(svd_mode=0)
         ts_ref = pkg.CTMTensors(ts.A, cfg, dtype="c128", options=ref_opts)
         ts_ref.set_env(ts.Cs, ts.Es)
-        idx = [i for i, m in enumerate(momenta) if m == (0.5, 0.5)][: args.c5_ref]
+        order = [(0.5, 0.5), (1.0, 0.0), (1.0, 1.0), (0.0, 0.0)]
+        idx = [momenta.index(m) for m in order if m in momenta][: args.c5_ref]
+        t0 = time.perf_counter()
+        ref_path = pkg.excited.ProjectorPath(ts_ref, cfg)
+        log(f"[c5] full-SVD projector path ({ref_path.moves} moves) in {time.perf_counter() - t0:.1f} s")
         for i in idx:
             rb = pkg.excited.band_structure(ts_ref, Hs, basis, [momenta[i]], cfg, workers=args.c5_workers,
-                                            progress=progress)[0]
-            err = float(np.abs(rb["bands"] - bands[i]["bands"]).max())
+                                            progress=progress, path=ref_path)[0]
+            b = bands[i]
+            err = float(np.abs(rb["bands"] - b["bands"]).max())
+            # agreement of the columns themselves, the norm-space dimension compute_es keeps, and the band
+            # change that a Hermitian perturbation of the reference's effH / effN of the measured size causes
+            eH = float(np.abs(rb["effH"] - b["effH"]).max() / np.abs(rb["effH"]).max())
+            eN = float(np.abs(rb["effN"] - b["effN"]).max() / np.abs(rb["effN"]).max())
+            rng = np.random.default_rng(0)
+
+            def perturbed(X, rel):
+                Z = rng.standard_normal(X.shape) + 1j * rng.standard_normal(X.shape)
+                Z = (Z + Z.conj().T) / 2
+                return X + Z * (rel * np.abs(X).max() / np.abs(Z).max())
+
+            es_p, _, _ = pkg.excited.compute_es(perturbed(rb["effH"], eH), perturbed(rb["effN"], eN))
+            sens = float(np.abs(np.real(es_p[:4]) - rb["bands"]).max())
+            nsel = [int(pkg.excited.compute_es(x["effH"], x["effN"])[2].shape[1]) for x in (b, rb)]
             ref.append(dict(px=rb["px"], py=rb["py"], bands_ref=rb["bands"].tolist(), max_band_err=err,
-                            seconds=rb["seconds"], projector=ref_opts))
+                            effH_rel_err=eH, effN_rel_err=eN, band_change_same_size_perturbation=sens,
+                            kept_norm_dims=nsel, seconds=rb["seconds"], projector=ref_opts))
             log(f"[c5] reference at ({rb['px']}, {rb['py']}) pi: {np.round(rb['bands'], 8).tolist()}, "
-                f"max |band error| {err:.2e}, {rb['seconds']:.1f} s")
+                f"max |band error| {err:.2e}, effH / effN rel {eH:.1e} / {eN:.1e}, band change under a "
+                f"perturbation of that size {sens:.1e}, kept norm dims {nsel}, {rb['seconds']:.1f} s")
+        ref_path.close()
     ncol = basis.shape[1] * len(momenta)
     print(json.dumps({
         "metric": "config-5 excited-state columns/s (nested CTMRG + adjoint per tangent vector)",
         "value": ncol / t_bands, "unit": "columns/s", "n_gpus": 1, "higher_is_better": True,
         "config": {"workload": "Ising h=2, A (3,3,3,3,2) seed 4 optimised in FP64, chi=48, 161 tangent vectors",
                    "momenta_pi": momenta, "workers": args.c5_workers, "projector": prod},
-        "seconds": {"ground_state": t_gs, "bands": t_bands}, "columns": ncol,
+        "nested_cfg": cfg, "seconds": {"ground_state": t_gs, "bands": t_bands}, "columns": ncol,
         "gs_energy": r["energy"], "gs_iterations": r["iterations"],
         "bands": [dict(px=b["px"], py=b["py"], lowest=b["bands"].tolist()) for b in bands],
         "reference": ref}), flush=True)
@@ -312,7 +337,10 @@ def main():
     ap.add_argument("--c5-ref", type=int, default=1, help="momenta compared with the full-SVD engine")
     ap.add_argument("--c5-workers", type=int, default=8)
     ap.add_argument("--c5-gs-iters", type=int, default=40)
-    ap.add_argument("--c5-ref-full", action="store_true", help="reference with the full SVD (slow)")
+    ap.add_argument("--c5-oversample", type=int, default=300)
+    ap.add_argument("--c5-power-iters", type=int, default=8)
+    ap.add_argument("--c5-rg-tol", type=float, default=1e-6, help="nested runs' stopping tolerance")
+    ap.add_argument("--c5-max-iter", type=int, default=20, help="nested runs' step limit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "

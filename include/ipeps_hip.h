@@ -225,6 +225,29 @@ int ipeps_es_grad(ipeps_env* env, const double* H1, const double* H2, int64_t hd
                   const void* B, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol, void* gH_out,
                   void* envB_out, double* E4_out, double* Nb_out);
 
+/* Projector path of nested runs, for optim_es (src/optimize/optim_es.jl:127-153),
+ * whose columns all start from one environment.  A nested product's first
+ * component involves only first components (src/nested_tensor/nested_tensor.jl:
+ * 48-63), so the plain parts of every nested run from env evolve alike,
+ * whatever B and the momentum, and move t (= 4 step + direction) of each run
+ * uses the same projector (src/ctm/ctmrg.jl:122-265).  ipeps_es_path_create
+ * computes the projectors of the first `steps` steps at bond dimension chi
+ * once (a run with B = EmptyT, on env's context); ipeps_es_grad_path is
+ * ipeps_es_grad with the moves the path covers taking their projectors from
+ * it (later moves compute their own).  The path must have been made from an
+ * environment in the same state as env -- equal tensors, test-matrix
+ * position (ipeps_env_get_draws) and options -- and the same chi, else
+ * IPEPS_ERR_ARG; environments on other contexts may use it (ipeps_env_copy_state
+ * copies the state).  Destroy it after the last call that uses it has
+ * returned. */
+typedef struct ipeps_es_path ipeps_es_path;
+int ipeps_es_path_create(ipeps_env* env, int64_t chi, int steps, ipeps_es_path** out);
+int ipeps_es_path_destroy(ipeps_es_path* path);
+int ipeps_es_path_moves(const ipeps_es_path* path, int64_t* moves);
+int ipeps_es_grad_path(ipeps_env* env, const ipeps_es_path* path, const double* H1, const double* H2, int64_t hdim,
+                       double px, double py, const void* B, int64_t chi, int min_iter, int max_iter,
+                       int ad_max_iter, double tol, void* gH_out, void* envB_out, double* E4_out, double* Nb_out);
+
 /* tcon (src/utility/wrapper.jl:15-23): ncon-style pairwise contraction on the
  * device.  Positive labels are summed, negative ones are output legs -1, -2,
  * ...  dimsC receives the output dims (room for 16), C_host the result

@@ -51,6 +51,11 @@ SIGNATURES = {
                                    _DP, _P, _IP]),
     "ipeps_es_grad": (C.c_int, [_P, _P, _P, C.c_int64, C.c_double, C.c_double, _P, C.c_int64, C.c_int, C.c_int,
                                 C.c_int, C.c_double, _P, _P, _DP, _DP]),
+    "ipeps_es_path_create": (C.c_int, [_P, C.c_int64, C.c_int, _P]),
+    "ipeps_es_path_destroy": (C.c_int, [_P]),
+    "ipeps_es_path_moves": (C.c_int, [_P, _I64P]),
+    "ipeps_es_grad_path": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_double, C.c_double, _P, C.c_int64, C.c_int,
+                                     C.c_int, C.c_int, C.c_double, _P, _P, _DP, _DP]),
     "ipeps_tcon": (C.c_int, [_P, C.c_int, C.c_int, _I64P, _P, _IP, C.c_int, _I64P, _P, _IP, _I64P, _IP, _P,
                              C.c_int64]),
     "ipeps_absorb": (C.c_int, [_P, C.c_int, C.c_int, _I64P, _P, _IP, _I64P, _P, _IP, _P, _IP, _P, _P, _P, _P,

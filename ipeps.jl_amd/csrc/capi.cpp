@@ -547,21 +547,70 @@ int ipeps_run_gs_vjp(ipeps_env* env, const void* x, const double* H1, const doub
   });
 }
 
+struct ipeps_es_path {
+  std::shared_ptr<Ctx> hold;  // the context whose pool holds the projectors
+  std::shared_ptr<const ctm::ProjPath> path;
+};
+
+int ipeps_es_path_create(ipeps_env* env, int64_t chi, int steps, ipeps_es_path** out) {
+  return guard([&] {
+    IPEPS_CHECK(env && out && chi > 0 && steps >= 0, "bad arguments");
+    IPEPS_CHECK(env->dtype == IPEPS_C128, "es_path: the environment must be complex (IPEPS_C128)");
+    *out = nullptr;
+    auto p = std::unique_ptr<ipeps_es_path>(new ipeps_es_path());
+    p->hold = env->hold;
+    p->path = ctm::nested_proj_path(*env->hold, env->env, chi, steps);
+    *out = p.release();
+  });
+}
+
+int ipeps_es_path_destroy(ipeps_es_path* p) {
+  return guard([&] {
+    if (!p) return;
+    p->path.reset();
+    delete p;
+  });
+}
+
+int ipeps_es_path_moves(const ipeps_es_path* p, int64_t* moves) {
+  return guard([&] {
+    IPEPS_CHECK(p && moves, "null argument");
+    *moves = (int64_t)p->path->P.size();
+  });
+}
+
+static void es_grad_impl(ipeps_env* env, const ipeps_es_path* path, const double* H1, const double* H2, int64_t hdim,
+                         double px, double py, const void* B, int64_t chi, int min_iter, int max_iter,
+                         int ad_max_iter, double tol, void* gH_out, void* envB_out, double* E4_out, double* Nb_out) {
+  IPEPS_CHECK(env && B && chi > 0, "bad arguments");
+  IPEPS_CHECK(env->dtype == IPEPS_C128, "es_grad: the environment must be complex (IPEPS_C128)");
+  Ctx& c = *env->hold;
+  Tensor tB = upload(c, DType::C128, DType::C128, env->env.A.dims, B);
+  const double pi = 3.14159265358979323846;
+  auto r = ctm::es_grad(c, env->env, tB, upload_H(c, H1, hdim), upload_H(c, H2, hdim), px * pi, py * pi, chi,
+                        min_iter, max_iter, ad_max_iter, tol, path ? path->path : nullptr);
+  if (gH_out) download(c, r.gH, DType::C128, gH_out);
+  if (envB_out) download(c, r.envB, DType::C128, envB_out);
+  if (E4_out) *E4_out = r.E4;
+  if (Nb_out) { Nb_out[0] = r.Nb[0]; Nb_out[1] = r.Nb[1]; }
+}
+
+int ipeps_es_grad_path(ipeps_env* env, const ipeps_es_path* path, const double* H1, const double* H2, int64_t hdim,
+                       double px, double py, const void* B, int64_t chi, int min_iter, int max_iter,
+                       int ad_max_iter, double tol, void* gH_out, void* envB_out, double* E4_out, double* Nb_out) {
+  return guard([&] {
+    IPEPS_CHECK(path, "null path");
+    es_grad_impl(env, path, H1, H2, hdim, px, py, B, chi, min_iter, max_iter, ad_max_iter, tol, gH_out, envB_out,
+                 E4_out, Nb_out);
+  });
+}
+
 int ipeps_es_grad(ipeps_env* env, const double* H1, const double* H2, int64_t hdim, double px, double py,
                   const void* B, int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol, void* gH_out,
                   void* envB_out, double* E4_out, double* Nb_out) {
   return guard([&] {
-    IPEPS_CHECK(env && B && chi > 0, "bad arguments");
-    IPEPS_CHECK(env->dtype == IPEPS_C128, "es_grad: the environment must be complex (IPEPS_C128)");
-    Ctx& c = *env->hold;
-    Tensor tB = upload(c, DType::C128, DType::C128, env->env.A.dims, B);
-    const double pi = 3.14159265358979323846;
-    auto r = ctm::es_grad(c, env->env, tB, upload_H(c, H1, hdim), upload_H(c, H2, hdim), px * pi, py * pi, chi,
-                          min_iter, max_iter, ad_max_iter, tol);
-    if (gH_out) download(c, r.gH, DType::C128, gH_out);
-    if (envB_out) download(c, r.envB, DType::C128, envB_out);
-    if (E4_out) *E4_out = r.E4;
-    if (Nb_out) { Nb_out[0] = r.Nb[0]; Nb_out[1] = r.Nb[1]; }
+    es_grad_impl(env, nullptr, H1, H2, hdim, px, py, B, chi, min_iter, max_iter, ad_max_iter, tol, gH_out, envB_out,
+                 E4_out, Nb_out);
   });
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <array>
+#include <memory>
 #include <vector>
 
 #include "ops.h"
@@ -117,12 +118,38 @@ struct NT {
   bool nested = true;
 };
 
+// Projectors along nested runs.  A product's first component involves only
+// first components (nested_tensor.jl:48-63), so the plain parts of a nested
+// environment evolve as they would with B = EmptyT, whatever B and the
+// momentum are, and move t (= 4 step + dir) of every nested run started from
+// the same environment uses the same projector (ctmrg.jl:122-265).  A path
+// holds the projectors of the first moves, computed once by a run with
+// B = EmptyT; runs that carry it take theirs from it.
+struct ProjPath {
+  Env start;                                // the environment the runs start from
+  int64_t chi = 0;
+  std::vector<std::array<Tensor, 2>> P;     // (P1, P2) of move t
+  std::vector<std::vector<double>> s;       // truncated spectrum of move t
+};
+
 struct NestedEnv {
   Env g;                        // plain (ground-state) parts: A, Ad, C, E
   Tensor B, Bd;                 // tangent tensor and its conjugate
   std::array<Tensor, 4> BC, BdC, BBC, BE, BdE, BBE;
   double px = 0.0, py = 0.0;    // momentum phases (already multiplied by pi)
+  int64_t moves = 0;            // moves made since the start
+  std::shared_ptr<const ProjPath> path;  // projectors of moves < path->P.size()
+  ProjPath* record = nullptr;   // when set, each move's projector is appended to it
 };
+
+// The projectors of the first `steps` steps of every nested run from g at
+// bond dimension chi (a run with B = EmptyT).  The path reads g's tensors
+// and is valid for environments whose state equals g's (path_matches).
+std::shared_ptr<const ProjPath> nested_proj_path(Ctx& c, const Env& g, int64_t chi, int steps);
+// Whether nested runs from env at bond dimension chi may take their
+// projectors from path: same tensors (identical storage or bitwise equal),
+// test-matrix position and engine options.
+bool path_matches(Ctx& c, const ProjPath& path, const Env& env, int64_t chi);
 
 // Nested rg_step: projectors from the plain parts, nested renormalisation and
 // the momentum shift of up_* (src/ctm/ctmrg.jl:66-120, 267-491).
@@ -140,8 +167,11 @@ struct EsGrad {
 };
 // get_es_grad (src/optimize/optim_es.jl:167-194) from a converged, normalised
 // ground-state environment g and one tangent tensor B.
+// With `path` (path_matches(g, chi) must hold) the moves it covers take its
+// projectors instead of recomputing them.
 EsGrad es_grad(Ctx& c, const Env& g, const Tensor& B, const Tensor& H1, const Tensor& H2, double px, double py,
-               int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol);
+               int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol,
+               std::shared_ptr<const ProjPath> path = nullptr);
 
 // One differentiated step from a given environment: forward rg_step on the
 // tape, the energy and its VJP, then the step VJP; the A cotangent is

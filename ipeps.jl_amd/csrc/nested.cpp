@@ -7,6 +7,7 @@
 // the plain component's max-abs and shifted by the momentum phase
 // (ctmrg.jl:377-491).
 #include <cmath>
+#include <cstring>
 
 #include "ctmrg.h"
 
@@ -136,7 +137,18 @@ std::vector<int64_t> rest_last(const NT& x) {
 std::vector<double> move(Ctx& c, NestedEnv& e, int dir, int64_t chi) {
   Tensor P1, P2;
   std::vector<double> s;
-  projector(c, e.g, dir, chi, P1, P2, s);
+  const int64_t t = e.moves++;
+  if (e.path && t < (int64_t)e.path->P.size()) {
+    P1 = e.path->P[t][0];
+    P2 = e.path->P[t][1];
+    s = e.path->s[t];
+  } else {
+    projector(c, e.g, dir, chi, P1, P2, s);
+  }
+  if (e.record) {
+    e.record->P.push_back({P1, P2});
+    e.record->s.push_back(s);
+  }
   const NT P = plain(P1), Pd = plain(P2), A = all_A(e), Ad = all_Ad(e);
   if (dir == 0) {  // proj_left + up_left
     NT C1E1 = ntcon(c, all_C(e, 0), {-2, 1}, all_E(e, 0), {1, -1, -3, -4});
@@ -233,7 +245,7 @@ NT single_dm(Ctx& c, const NestedEnv& e) {
 std::vector<double> nested_rg_step(Ctx& c, NestedEnv& e, int64_t chi) {
   op::prec().mixed = e.g.opt.mixed;
   op::prec().back = e.g.opt.mixed;
-  IPEPS_CHECK(e.g.dt == DType::C128 && e.B.dt == DType::C128, "nested CTMRG runs in complex FP64");
+  IPEPS_CHECK(e.g.dt == DType::C128 && (!e.B.defined() || e.B.dt == DType::C128), "nested CTMRG runs in complex FP64");
   auto s = move(c, e, 0, chi);
   move(c, e, 1, chi);
   move(c, e, 2, chi);
@@ -289,12 +301,67 @@ void all_norm(Ctx& c, const NestedEnv& e, double Nb[2], Tensor& envB) {
   envB = op::scale_cplx(c, ndmA.c[1], ir, ii);
 }
 
-EsGrad es_grad(Ctx& c, const Env& g, const Tensor& B, const Tensor& H1, const Tensor& H2, double px, double py,
-               int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol) {
+namespace {
+
+bool same_opts(const Options& a, const Options& b) {
+  const op::SvdOpts &x = a.svd, &y = b.svd;
+  return x.mode == y.mode && x.oversample == y.oversample && x.power_iters == y.power_iters &&
+         x.full_below == y.full_below && x.seed == y.seed && x.mixed_cutoff == y.mixed_cutoff &&
+         x.fast_orth == y.fast_orth && x.fast_svd == y.fast_svd && x.gram_guard == y.gram_guard &&
+         x.power_iters_max == y.power_iters_max && x.resid_tol == y.resid_tol && x.warm_start == y.warm_start &&
+         a.canonical_edge == b.canonical_edge && a.mixed == b.mixed && a.mixed_back == b.mixed_back &&
+         a.mixed_proj == b.mixed_proj && a.fused_absorb == b.fused_absorb && a.paired_moves == b.paired_moves;
+}
+
+bool same_tensor(Ctx& c, const Tensor& a, const Tensor& b) {
+  if (a.defined() != b.defined()) return false;
+  if (!a.defined()) return true;
+  if (a.dt != b.dt || a.dims != b.dims) return false;
+  if (a.data() == b.data() || a.bytes() == 0) return true;
+  std::vector<char> ha(a.bytes()), hb(b.bytes());
+  HIP_CHECK(hipMemcpyAsync(ha.data(), a.data(), a.bytes(), hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(hb.data(), b.data(), b.bytes(), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  return std::memcmp(ha.data(), hb.data(), ha.size()) == 0;
+}
+
+}  // namespace
+
+std::shared_ptr<const ProjPath> nested_proj_path(Ctx& c, const Env& g, int64_t chi, int steps) {
+  IPEPS_CHECK(chi > 0 && steps >= 0, "nested_proj_path: bad arguments");
+  IPEPS_CHECK(g.dt == DType::C128, "nested CTMRG runs in complex FP64");
+  IPEPS_CHECK(!g.opt.svd.warm_start, "nested_proj_path: warm-started projectors depend on the run's history");
   TapeScope none(nullptr);
+  auto p = std::make_shared<ProjPath>();
+  p->start = g;
+  p->chi = chi;
+  NestedEnv e;
+  e.g = g;
+  e.record = p.get();
+  for (int i = 0; i < steps; ++i) nested_rg_step(c, e, chi);
+  c.sync();  // other contexts' streams read the projectors
+  return p;
+}
+
+bool path_matches(Ctx& c, const ProjPath& path, const Env& env, int64_t chi) {
+  const Env& s = path.start;
+  if (path.chi != chi || s.dt != env.dt || s.draws != env.draws || !same_opts(s.opt, env.opt)) return false;
+  if (!same_tensor(c, s.A, env.A) || !same_tensor(c, s.Ad, env.Ad)) return false;
+  for (int i = 0; i < 4; ++i)
+    if (!same_tensor(c, s.C[i], env.C[i]) || !same_tensor(c, s.E[i], env.E[i])) return false;
+  return true;
+}
+
+EsGrad es_grad(Ctx& c, const Env& g, const Tensor& B, const Tensor& H1, const Tensor& H2, double px, double py,
+               int64_t chi, int min_iter, int max_iter, int ad_max_iter, double tol,
+               std::shared_ptr<const ProjPath> path) {
+  TapeScope none(nullptr);
+  IPEPS_CHECK(!path || path_matches(c, *path, g, chi),
+              "es_grad: the projector path was made from another environment state, chi or options");
   EsGrad r;
   NestedEnv e;
   e.g = g;
+  e.path = std::move(path);
   e.B = B;
   e.Bd = op::conj(c, B);
   e.px = px;

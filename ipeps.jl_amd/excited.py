@@ -23,11 +23,50 @@ from ._lib import C128, check, lib
 from .ctm import CTMTensors, _dp, _hpair, _ptr, get_gs_energy, get_gs_norm
 
 
-def get_es_grad(ts, H, Bi, px, py, cfg):
+class ProjectorPath:
+    """The projectors of the first ``steps`` moves x 4 of every nested run
+    from ``ts`` at bond dimension cfg["chi"] (ipeps_es_path_create).  The plain
+    parts of a nested environment evolve alike whatever B and the momentum
+    (nested_tensor.jl:48-63), so all columns and momenta started from ``ts``
+    share them (ctmrg.jl:122-265); ``steps`` defaults to max_iter +
+    ad_max_iter, the most a column runs.  Valid for ``ts`` and its clones
+    while they stay in the state it was made from."""
+
+    def __init__(self, ts, cfg, steps=None):
+        if ts.code != C128:
+            raise ValueError("ProjectorPath needs a complex (c128) environment")
+        if steps is None:
+            steps = int(cfg.get("max_iter", 20)) + int(cfg.get("ad_max_iter", 4))
+        self.chi = int(cfg.get("chi", 30))
+        h = C.c_void_p()
+        check(lib().ipeps_es_path_create(ts.h, self.chi, int(steps), C.byref(h)))
+        self.h = h
+
+    @property
+    def moves(self):
+        v = C.c_int64()
+        check(lib().ipeps_es_path_moves(self.h, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ipeps_es_path_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def get_es_grad(ts, H, Bi, px, py, cfg, path=None):
     """(gradH, gradN) flattened in Julia (column-major) order, plus (E4, Nb).
 
     ``ts``: converged, normalised complex ground-state ``CTMTensors``;
-    ``H``: the E_gs-shifted two-site terms; (px, py) in units of pi.
+    ``H``: the E_gs-shifted two-site terms; (px, py) in units of pi;
+    ``path``: an optional ``ProjectorPath`` made from ``ts`` (or an
+    environment in the same state).
     """
     if ts.code != C128:
         raise ValueError("get_es_grad needs a complex (c128) environment")
@@ -38,13 +77,17 @@ def get_es_grad(ts, H, Bi, px, py, cfg):
     envB = np.empty(shape, dtype=np.complex128, order="F")
     E4 = np.zeros(1)
     Nb = np.zeros(2)
-    check(lib().ipeps_es_grad(ts.h, _ptr(h1), _ptr(h2), m, float(px), float(py), _ptr(B), int(cfg.get("chi", 30)),
-                              int(cfg.get("min_iter", 4)), int(cfg.get("max_iter", 20)), int(cfg.get("ad_max_iter", 4)),
-                              float(cfg.get("rg_tol", 1e-6)), _ptr(gH), _ptr(envB), _dp(E4), _dp(Nb)))
+    args = (_ptr(h1), _ptr(h2), m, float(px), float(py), _ptr(B), int(cfg.get("chi", 30)),
+            int(cfg.get("min_iter", 4)), int(cfg.get("max_iter", 20)), int(cfg.get("ad_max_iter", 4)),
+            float(cfg.get("rg_tol", 1e-6)), _ptr(gH), _ptr(envB), _dp(E4), _dp(Nb))
+    if path is None:
+        check(lib().ipeps_es_grad(ts.h, *args))
+    else:
+        check(lib().ipeps_es_grad_path(ts.h, path.h, *args))
     return gH.reshape(-1, order="F"), envB.reshape(-1, order="F"), (float(E4[0]), complex(Nb[0], Nb[1]))
 
 
-def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1, progress=None):
+def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1, progress=None, path=True):
     """effH, effN, envB for the tangent basis at momentum (px, py) pi (optim_es.jl:127-153).
 
     The columns are independent nested runs (the reference splits them over
@@ -58,7 +101,24 @@ def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1, progress=None):
     on this path: this library's own kernels use no atomics and read no
     unwritten memory (the GPU tests pass with every fresh allocation filled
     with NaN, IPEPS_POOL_POISON=1).
+
+    ``path``: True (default) computes the columns' shared projectors once
+    (``ProjectorPath``), False recomputes them in every column, or a
+    ``ProjectorPath`` made from ``ts``.
     """
+    own_path = path is True
+    if own_path:
+        path = ProjectorPath(ts, cfg)
+    elif path is False:
+        path = None
+    try:
+        return _optim_es(ts, H, basis, px, py, cfg, columns, workers, progress, path)
+    finally:
+        if own_path:
+            path.close()
+
+
+def _optim_es(ts, H, basis, px, py, cfg, columns, workers, progress, path):
     basis = np.asarray(basis, dtype=np.complex128)
     n = basis.shape[1]
     effH = np.zeros((n, n), dtype=np.complex128)
@@ -78,7 +138,7 @@ def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1, progress=None):
 
     if workers <= 1 or len(cols) <= 1:
         for i in cols:
-            gH, gN, _ = get_es_grad(ts, H, basis[:, i], px, py, cfg)
+            gH, gN, _ = get_es_grad(ts, H, basis[:, i], px, py, cfg, path)
             store(i, gH, gN)
         return effH, effN, envB
     from concurrent.futures import ThreadPoolExecutor
@@ -92,7 +152,7 @@ def optim_es(ts, H, basis, px, py, cfg, columns=None, workers=1, progress=None):
 
     def run(w):
         for i in cols[w::nw]:
-            gH, gN = get_es_grad(envs[w], H, basis[:, i], px, py, cfg)[:2]
+            gH, gN = get_es_grad(envs[w], H, basis[:, i], px, py, cfg, path)[:2]
             with lock:
                 store(i, gH, gN)
 
@@ -208,20 +268,29 @@ def compute_es(effH, effN, nrmB_cut=1e-3):
     return es[ixs], vecs[:, ixs], P
 
 
-def band_structure(ts, H, basis, momenta, cfg, n_bands=4, nrmB_cut=1e-3, workers=1, progress=None):
+def band_structure(ts, H, basis, momenta, cfg, n_bands=4, nrmB_cut=1e-3, workers=1, progress=None, path=True):
     """The excited-state pipeline over a list of momenta (px, py in units of
     pi): at each, the effH / effN columns of every tangent basis vector
     (optim_es, optim_es.jl:91-165) and the generalised eigenproblem
     (compute_es, expectation.jl:13-56).  Returns a list of dicts with the
-    momentum, the lowest n_bands energies, effH, effN and the seconds taken."""
+    momentum, the lowest n_bands energies, effH, effN and the seconds taken
+    (the first momentum's include making the shared projector path, which
+    every momentum uses: ``path`` as in optim_es)."""
     import time
     out = []
-    for px, py in momenta:
-        t0 = time.perf_counter()
-        effH, effN, _ = optim_es(ts, H, basis, px, py, cfg, workers=workers, progress=progress)
-        es, _, _ = compute_es(effH, effN, nrmB_cut)
-        out.append(dict(px=float(px), py=float(py), bands=np.real(es[:n_bands]), effH=effH, effN=effN,
-                        seconds=time.perf_counter() - t0))
+    own_path = path is True
+    try:
+        for px, py in momenta:
+            t0 = time.perf_counter()
+            if path is True:
+                path = ProjectorPath(ts, cfg)
+            effH, effN, _ = optim_es(ts, H, basis, px, py, cfg, workers=workers, progress=progress, path=path)
+            es, _, _ = compute_es(effH, effN, nrmB_cut)
+            out.append(dict(px=float(px), py=float(py), bands=np.real(es[:n_bands]), effH=effH, effN=effN,
+                            seconds=time.perf_counter() - t0))
+    finally:
+        if own_path and isinstance(path, ProjectorPath):
+            path.close()
     return out
 
 

@@ -160,6 +160,57 @@ def test_optim_es_workers_match_sequential(pkg, randomized):
         assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()
 
 
+@pytest.mark.parametrize("randomized", [False, True])
+def test_projector_path_matches_recomputed(pkg, randomized):
+    """Columns that take their projectors from a shared path (made once with
+    B = EmptyT) match columns that compute every projector (G5 Ising
+    environment, (pi/2, pi/2) and (pi, 0)), also when the path covers fewer
+    moves than a column makes, and on a clone in another context.  A path
+    made from another environment state is refused."""
+    IpepsError = pkg.IpepsError
+    e = np.load(os.path.join(GOLDEN, "env_ising_D2_X32.npz"))
+    cfg = dict(O.DEFAULT_CFG)
+    opts = dict(svd_mode=1, oversample=8) if randomized else {}
+    ts = pkg.CTMTensors(e["A"], cfg, options=opts)
+    ts.set_env([e[f"C{i}"] for i in range(1, 5)], [e[f"E{i}"] for i in range(1, 5)])
+    if randomized:
+        ts.set_option("draws", 11)
+    H = pkg.excited.substract_gs_energy(ts, pkg.models.ising(2.0))
+    basis = e["basis"]
+    full = pkg.excited.ProjectorPath(ts, cfg)
+    assert full.moves == 4 * (cfg["max_iter"] + cfg["ad_max_iter"])
+    short = pkg.excited.ProjectorPath(ts, cfg, steps=2)
+    assert short.moves == 8
+    ctx = pkg.Context()
+    tc = ts.clone(ctx)
+    try:
+        for (px, py), i in (((0.5, 0.5), 3), ((1.0, 0.0), 12)):
+            ref = pkg.get_es_grad(ts, H, basis[:, i], px, py, cfg)
+            for env, path in ((ts, full), (ts, short), (tc, full)):
+                got = pkg.get_es_grad(env, H, basis[:, i], px, py, cfg, path)
+                for a, b in zip(ref[:2], got[:2]):
+                    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+                assert abs(ref[2][0] - got[2][0]) <= 1e-12 * abs(ref[2][0])
+        # another state: the test-matrix position, or the tensors
+        other = ts.clone()
+        other.set_option("draws", 12)
+        with pytest.raises(IpepsError):
+            pkg.get_es_grad(other, H, basis[:, 0], 0.5, 0.5, cfg, full)
+        other = ts.clone()
+        Cs = ts.Cs
+        Cs[0] = Cs[0] * (1 + 1e-15)
+        other.set_env(Cs, ts.Es)
+        with pytest.raises(IpepsError):
+            pkg.get_es_grad(other, H, basis[:, 0], 0.5, 0.5, cfg, full)
+        with pytest.raises(IpepsError):
+            pkg.get_es_grad(ts, H, basis[:, 0], 0.5, 0.5, dict(cfg, chi=cfg["chi"] - 1), full)
+    finally:
+        full.close()
+        short.close()
+        del tc
+        ctx.close()
+
+
 @pytest.mark.parametrize("fixture", ["env_loopgas_D2_X32.npz", "env_ising_D2_X32.npz"])
 def test_tangent_basis_matches_reference_basis(pkg, fixture):
     """get_tangent_basis (optim_es.jl:236-258, default basis) on the
