@@ -1311,12 +1311,193 @@ __global__ __launch_bounds__(TRI_THREADS) void sytrd_kernel(int n, const double*
   }
 }
 
-// column c of Q = H(0) ... H(c-1) e_c (one wave; rows lane + 64 t)
+// A packed lower triangle holding A(i0.., i0..) of an n x n matrix:
+// A(r, c), r >= c >= i0, at P[tri_off(n - i0, c - i0) + r - c].
+struct TriView {
+  double* P;
+  int np, i0;
+  // column c: A(r, c) = col(c)[r] for r >= c
+  __device__ double* col(int c) const { return P + tri_off(np, c - i0) - c; }
+};
+
+constexpr int TRB_NWAVE = TRI_THREADS / 64;
+constexpr int TRB_RW = (TRI_BIG_MAXN + 63) / 64;  // rows per lane
+constexpr int TRB_U = 4;                          // columns per wave in flight
+
+struct TrbShared {
+  double v[TRI_BIG_MAXN], w[TRI_BIG_MAXN], tp[TRI_BIG_MAXN];
+  double part[TRB_NWAVE][TRI_BIG_MAXN];  // per-wave lower-part partials of p
+  double red[TRB_NWAVE];
+  double sc[2];                          // tau, scale
+};
+
+// Step i of the reduction (tridiag_impl.h) on the triangle A; the reflector
+// goes to column i of A and of V (column-major n x n, zero elsewhere).
+__device__ void trb_step(const TriView& A, int n, int i, TrbShared& sh, double* __restrict__ d,
+                         double* __restrict__ e, double* __restrict__ tau, double* __restrict__ V) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m = n - 1 - i;
+  double* ci = A.col(i);
+  // (a) reflector of x = A(i+1 .., i)
+  if (wave == 0) {
+    double x2 = 0.0;
+    for (int r = i + 2 + lane; r < n; r += 64) {
+      const double x = ci[r];
+      x2 += x * x;
+    }
+    x2 = wave_sum_d(x2);
+    if (lane == 0) {
+      const double alpha = ci[i + 1];
+      double t = 0.0, beta = alpha, scale = 0.0;
+      if (x2 > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + x2), alpha);
+        t = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+      }
+      sh.sc[0] = t;
+      sh.sc[1] = scale;
+      d[i] = ci[i];
+      e[i] = beta;
+      tau[i] = t;
+    }
+  }
+  __syncthreads();
+  const double t = sh.sc[0], scale = sh.sc[1];
+  for (int r = tid; r < m; r += TRI_THREADS) {
+    const double vr = r == 0 ? 1.0 : ci[i + 1 + r] * scale;
+    sh.v[r] = vr;
+    ci[i + 1 + r] = vr;
+    V[(i + 1 + r) + (int64_t)i * n] = vr;
+  }
+  __syncthreads();
+  if (t == 0.0) return;  // H(i) = I (uniform)
+  // (b) p = tau A' v, A' = A(i+1 .., i+1 ..) with local indices: each
+  //     column c read once, TRB_U columns per wave in flight
+  double pp[TRB_RW];
+#pragma unroll
+  for (int q = 0; q < TRB_RW; ++q) pp[q] = 0.0;
+  for (int c0 = wave; c0 < m; c0 += TRB_U * TRB_NWAVE) {
+    double a[TRB_U][TRB_RW];
+#pragma unroll
+    for (int u = 0; u < TRB_U; ++u) {
+      const int c = c0 + u * TRB_NWAVE;
+      const double* col = c < m ? A.col(i + 1 + c) + (i + 1) : nullptr;
+#pragma unroll
+      for (int q = 0; q < TRB_RW; ++q) {
+        const int r = lane + 64 * q;
+        a[u][q] = (c < m && r >= c && r < m) ? col[r] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TRB_U; ++u) {
+      const int c = c0 + u * TRB_NWAVE;
+      if (c >= m) break;  // uniform
+      const double vc = sh.v[c];
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < TRB_RW; ++q) {
+        const int r = lane + 64 * q;
+        pp[q] += a[u][q] * vc;
+        if (r > c && r < m) s += a[u][q] * sh.v[r];
+      }
+      s = wave_sum_d(s);
+      if (lane == 0) sh.tp[c] = s;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < TRB_RW; ++q) {
+    const int r = lane + 64 * q;
+    if (r < m) sh.part[wave][r] = pp[q];
+  }
+  __syncthreads();
+  // (c) p and p . v
+  double prod = 0.0;
+  if (tid < m) {
+    double pr = sh.tp[tid];
+#pragma unroll
+    for (int q = 0; q < TRB_NWAVE; ++q) pr += sh.part[q][tid];
+    pr *= t;
+    sh.w[tid] = pr;
+    prod = pr * sh.v[tid];
+  }
+  prod = wave_sum_d(prod);
+  if (lane == 0) sh.red[wave] = prod;
+  __syncthreads();
+  double dot = 0.0;
+#pragma unroll
+  for (int q = 0; q < TRB_NWAVE; ++q) dot += sh.red[q];
+  // (d) w = p - (tau / 2)(p . v) v
+  if (tid < m) sh.w[tid] += -0.5 * t * dot * sh.v[tid];
+  __syncthreads();
+  // (e) A' -= v w^T + w v^T (lower): loads of TRB_U columns, then their stores
+  for (int c0 = wave; c0 < m; c0 += TRB_U * TRB_NWAVE) {
+    double a[TRB_U][TRB_RW];
+#pragma unroll
+    for (int u = 0; u < TRB_U; ++u) {
+      const int c = c0 + u * TRB_NWAVE;
+      const double* col = c < m ? A.col(i + 1 + c) + (i + 1) : nullptr;
+#pragma unroll
+      for (int q = 0; q < TRB_RW; ++q) {
+        const int r = lane + 64 * q;
+        a[u][q] = (c < m && r >= c && r < m) ? col[r] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TRB_U; ++u) {
+      const int c = c0 + u * TRB_NWAVE;
+      if (c >= m) break;  // uniform
+      double* col = A.col(i + 1 + c) + (i + 1);
+      const double vc = sh.v[c], wc = sh.w[c];
+#pragma unroll
+      for (int q = 0; q < TRB_RW; ++q) {
+        const int r = lane + 64 * q;
+        if (r >= c && r < m) col[r] = a[u][q] - (sh.v[r] * wc + sh.w[r] * vc);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// n <= 320: the triangle in the workspace W (n (n + 1) / 2 doubles) while the
+// trailing block exceeds TRI_LDS_M, then in LDS.
+__global__ __launch_bounds__(TRI_THREADS) void sytrd_big_kernel(int n, const double* __restrict__ G, int64_t ldg,
+                                                                double* __restrict__ d, double* __restrict__ e,
+                                                                double* __restrict__ tau, double* __restrict__ V,
+                                                                double* __restrict__ W) {
+  __shared__ double PL[(TRI_LDS_M + 1) * (TRI_LDS_M + 2) / 2];
+  __shared__ TrbShared sh;
+  const int tid = threadIdx.x;
+  for (int c = 0; c < n; ++c)
+    for (int r = c + tid; r < n; r += TRI_THREADS) W[tri_off(n, c) + r - c] = G[r + c * ldg];
+  for (int idx = tid; idx < n * n; idx += TRI_THREADS) V[idx] = 0.0;
+  __syncthreads();
+  // steps i < i_lds run on W; step i needs A(i.., i..), n - i <= TRI_LDS_M + 1 values a side in LDS
+  const int i_lds = max(0, n - 1 - TRI_LDS_M);
+  int i = 0;
+  const TriView gw{W, n, 0};
+  for (; i < i_lds && i + 2 < n; ++i) trb_step(gw, n, i, sh, d, e, tau, V);
+  const int nl = n - i;
+  for (int c = i; c < n; ++c)
+    for (int r = c + tid; r < n; r += TRI_THREADS) PL[tri_off(nl, c - i) + r - c] = W[tri_off(n, c) + r - c];
+  __syncthreads();
+  const TriView lv{PL, nl, i};
+  for (; i + 2 < n; ++i) trb_step(lv, n, i, sh, d, e, tau, V);
+  if (tid == 0) {
+    if (n >= 2) {
+      d[n - 2] = lv.col(n - 2)[n - 2];
+      e[n - 2] = lv.col(n - 2)[n - 1];
+      tau[n - 2] = 0.0;
+    }
+    d[n - 1] = lv.col(n - 1)[n - 1];
+  }
+}
+
+// column c of Q = H(0) ... H(c-1) e_c (one wave; rows lane + 64 t, t < R)
+template <int R>
 __global__ __launch_bounds__(256) void orgtr_kernel(int n, const double* __restrict__ V, const double* __restrict__ tau,
                                                     double* __restrict__ Q, int64_t ldq) {
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (c >= n) return;
-  constexpr int R = (TRI_MAXN + 63) / 64;
   double q[R];
 #pragma unroll
   for (int t = 0; t < R; ++t) q[t] = (lane + 64 * t == c) ? 1.0 : 0.0;
@@ -1342,18 +1523,30 @@ __global__ __launch_bounds__(256) void orgtr_kernel(int n, const double* __restr
 }
 
 void sytrd_q(Ctx& c, int n, const double* G, int64_t ldg, double* d, double* e, double* Q, int64_t ldq) {
-  IPEPS_CHECK(n >= 1 && n <= TRI_MAXN, "sytrd_q: n out of range");
-  std::shared_ptr<Buf> ws(new Buf{c.pool->alloc((size_t)(n * n + n) * 8), (size_t)(n * n + n) * 8, c.pool.get()});
+  static_assert(TRI_BIG_MAXN == TRI_MAXN_HOST, "sytrd_q size limit");
+  IPEPS_CHECK(n >= 1 && n <= TRI_BIG_MAXN, "sytrd_q: n out of range");
+  const bool big = n > TRI_MAXN;
+  const size_t nws = (size_t)n * n + n + (big ? (size_t)n * (n + 1) / 2 : 0);
+  std::shared_ptr<Buf> ws(new Buf{c.pool->alloc(nws * 8), nws * 8, c.pool.get()});
   double* V = (double*)ws->p;
   double* tau = V + (size_t)n * n;
-  {
+  if (big) {
+    double* W = tau + n;
+    TimedLaunch tl(c, "sytrd_big_kernel", 4.0 / 3.0 * n * n * n, 0.0);
+    hipLaunchKernelGGL(sytrd_big_kernel, dim3(1), dim3(TRI_THREADS), 0, c.stream, n, G, ldg, d, e, tau, V, W);
+  } else {
     TimedLaunch tl(c, "sytrd_kernel", 4.0 / 3.0 * n * n * n, 0.0);
     hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(TRI_THREADS), 0, c.stream, n, G, ldg, d, e, tau, V);
   }
   HIP_CHECK(hipGetLastError());
   {
     TimedLaunch tl(c, "orgtr_kernel", 4.0 / 3.0 * n * n * n, 0.0);
-    hipLaunchKernelGGL(orgtr_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream, n, V, tau, Q, ldq);
+    if (big)
+      hipLaunchKernelGGL(orgtr_kernel<(TRI_BIG_MAXN + 63) / 64>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream,
+                         n, V, tau, Q, ldq);
+    else
+      hipLaunchKernelGGL(orgtr_kernel<(TRI_MAXN + 63) / 64>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream, n,
+                         V, tau, Q, ldq);
   }
   HIP_CHECK(hipGetLastError());
 }

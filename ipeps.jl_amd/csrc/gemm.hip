@@ -644,17 +644,23 @@ __device__ inline double2 axpby_(double al, double2 s, double be, double2 c) {
 __device__ inline double axpby_(double al, double s, double be, double c) { return al * s + be * c; }
 __device__ inline float axpby_(double al, float s, double be, float c) { return (float)(al * s + be * c); }
 
+// One output element per thread (MN < 2^32: 32-bit row / column split); the
+// S slice loads are independent and issued together.
 template <typename T>
-__global__ __launch_bounds__(256) void splitk_reduce(int64_t M, int64_t N, int64_t S, const T* __restrict__ W,
+__global__ __launch_bounds__(256) void splitk_reduce(unsigned M, int64_t MN, int S, const T* __restrict__ W,
                                                      double alpha, double beta, T* __restrict__ C, int64_t ldc) {
-  const int64_t MN = M * N;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < MN; t += (int64_t)gridDim.x * 256) {
-    T acc = zero_of<T>();
-    for (int64_t s = 0; s < S; ++s) acc = add_(acc, W[s * MN + t]);
-    int64_t m = t % M, n = t / M;
-    T* dst = C + m + n * ldc;
-    *dst = beta != 0.0 ? axpby_(alpha, acc, beta, *dst) : axpby_(alpha, acc, 0.0, zero_of<T>());
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= MN) return;
+  const unsigned n = (unsigned)t / M, m = (unsigned)t - n * M;
+  T acc = zero_of<T>();
+  int s = 0;
+  for (; s + 4 <= S; s += 4) {
+    const T w0 = W[s * MN + t], w1 = W[(s + 1) * MN + t], w2 = W[(s + 2) * MN + t], w3 = W[(s + 3) * MN + t];
+    acc = add_(add_(add_(add_(acc, w0), w1), w2), w3);
   }
+  for (; s < S; ++s) acc = add_(acc, W[s * MN + t]);
+  T* dst = C + m + (int64_t)n * ldc;
+  *dst = beta != 0.0 ? axpby_(alpha, acc, beta, *dst) : axpby_(alpha, acc, 0.0, zero_of<T>());
 }
 
 // W2[y][e] = sum of W[s][e] over s in [y F, min(S, (y+1) F)): folds many
@@ -1087,7 +1093,8 @@ static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
     const char* rname = g.dt == DType::F64 ? "splitk_reduce<double>" : g.dt == DType::C128 ? "splitk_reduce<double2>"
                                                                                           : "splitk_reduce<float>";
     TimedLaunch tr(c, timed ? rname : "", 0.0, (double)(S + 1) * MN * es);
-    unsigned rb = (unsigned)std::min<int64_t>((MN + 255) / 256, 4096);
+    IPEPS_CHECK(MN < ((int64_t)1 << 32), "gemm: split-K output too large");
+    const unsigned rb = (unsigned)((MN + 255) / 256);
     std::shared_ptr<Buf> ws2;
     if (S > 32) {
       const int64_t F = 16, G = (S + F - 1) / F;
@@ -1108,14 +1115,14 @@ static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
       S = G;
     }
     if (g.dt == DType::F64)
-      hipLaunchKernelGGL(splitk_reduce<double>, dim3(rb), dim3(256), 0, c.stream, g.M, g.N, S, (const double*)ws->p,
-                         g.alpha, g.beta, (double*)g.C, g.ldc);
+      hipLaunchKernelGGL(splitk_reduce<double>, dim3(rb), dim3(256), 0, c.stream, (unsigned)g.M, MN, (int)S,
+                         (const double*)ws->p, g.alpha, g.beta, (double*)g.C, g.ldc);
     else if (g.dt == DType::C128)
-      hipLaunchKernelGGL(splitk_reduce<double2>, dim3(rb), dim3(256), 0, c.stream, g.M, g.N, S,
+      hipLaunchKernelGGL(splitk_reduce<double2>, dim3(rb), dim3(256), 0, c.stream, (unsigned)g.M, MN, (int)S,
                          (const double2*)ws->p, g.alpha, g.beta, (double2*)g.C, g.ldc);
     else
-      hipLaunchKernelGGL(splitk_reduce<float>, dim3(rb), dim3(256), 0, c.stream, g.M, g.N, S, (const float*)ws->p,
-                         g.alpha, g.beta, (float*)g.C, g.ldc);
+      hipLaunchKernelGGL(splitk_reduce<float>, dim3(rb), dim3(256), 0, c.stream, (unsigned)g.M, MN, (int)S,
+                         (const float*)ws->p, g.alpha, g.beta, (float*)g.C, g.ldc);
     HIP_CHECK(hipGetLastError());
     return;
   }

@@ -195,9 +195,9 @@ constexpr int SCHOL_MAXN_HOST = 160;  // largest n of small_chol_inv
 // j + 1 if pivot j is not positive.
 void small_chol_inv(Ctx& c, int n, const double* G, int64_t ldg, double* Linv, int64_t ldl, int* info);
 
-constexpr int TRI_MAXN_HOST = 160;  // largest n of sytrd_q
+constexpr int TRI_MAXN_HOST = 320;  // largest n of sytrd_q
 // Householder tridiagonalisation Q^T G Q = tridiag(d, e) of a symmetric n x n
-// FP64 matrix (n <= 160; lower triangle read): d (n), e (n - 1) and the
+// FP64 matrix (n <= 320; lower triangle read): d (n), e (n - 1) and the
 // orthogonal Q (n x n, leading dimension ldq), the input rocSOLVER's stedc
 // takes for the eigenvectors of G.
 void sytrd_q(Ctx& c, int n, const double* G, int64_t ldg, double* d, double* e, double* Q, int64_t ldq);

@@ -1,7 +1,8 @@
 // Householder reduction of a small real symmetric matrix to tridiagonal form
-// in one workgroup (FP64, n <= 160), and the formation of the orthogonal
+// in one workgroup (FP64, n <= 320), and the formation of the orthogonal
 // factor: the storage scheme and index helpers of the device kernels in
-// elem.hip (sytrd_kernel, orgtr_kernel).
+// elem.hip (sytrd_kernel for n <= 160, sytrd_big_kernel for n <= 320,
+// orgtr_kernel).
 //
 // Lower storage, LAPACK dsytd2 'L' convention: for i = 0 .. n-3 the reflector
 // H(i) = I - tau_i v v^T (v_0 = 1) maps A(i+1:n, i) to e_i e_1 and is applied
@@ -11,6 +12,13 @@
 // The lower triangle lives in LDS packed by columns: (r, c), r >= c, at
 // tri_off(n, c) + r - c.  The reflectors go to global memory as a
 // column-major n x n array V (V(i+1, i) = 1, V(r, i) = v_{r-i-1}) and tau.
+//
+// sytrd_big_kernel keeps the triangle in a global-memory workspace (the same
+// packing, read and written through the L2 of the workgroup's XCD) while the
+// trailing block is larger than TRI_LDS_M, then moves A(i.., i..) into LDS
+// (packed with its own dimension) for the remaining steps.  Its step reads
+// each column of the trailing block once: lane rows accumulate the lower
+// part of p and a wave sum gives the column's transposed part.
 //
 // Q = H(0) H(1) ... H(n-3) acts on rows / columns 1 .. n-1 (Q(0, 0) = 1).
 // Column c of Q is independent of the others: starting from e_c, apply
@@ -28,6 +36,8 @@ namespace ipeps {
 namespace k {
 
 constexpr int TRI_MAXN = 160;
+constexpr int TRI_BIG_MAXN = 320;
+constexpr int TRI_LDS_M = 144;  // largest trailing block sytrd_big_kernel reduces from LDS
 constexpr int TRI_THREADS = 1024;
 
 IPEPS_HD inline int tri_off(int n, int c) { return c * n - c * (c - 1) / 2; }
