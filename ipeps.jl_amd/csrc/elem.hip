@@ -1046,7 +1046,7 @@ __device__ inline double readlane_d(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// (a) diagonal tile K by wave 0: lane i < 32 holds row i; the tile is replaced
+// (a) diagonal tile K by wave 0: lane i < 16 holds row i; the tile is replaced
 // by the inverse of its Cholesky factor (zeros above the diagonal).  Returns
 // the 1-based failing pivot within the tile, 0 on success (uniform).
 __device__ inline int bch_diag(double* tile, int lane) {
@@ -1085,63 +1085,107 @@ __device__ inline int bch_diag(double* tile, int lane) {
   return fail;
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// acc + A B over one 16 x 16 x 16 tile product (four 16x16x4 FP64 MFMAs);
+// a(r, k) / b(k, c) give the operands: lane l supplies A[l % 16][4 s + l / 16]
+// and B[4 s + l / 16][l % 16] at k step s.  Output lane l, element e:
+// C[l / 16 + 4 e][l % 16].
+template <class FA, class FB>
+__device__ inline v4d bch_mm(FA a, FB b, v4d acc, int lane) {
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a(r, 4 * s + kq), b(4 * s + kq, r), acc, 0, 0, 0);
+  return acc;
+}
+
+__device__ inline void bch_store(double* tile, v4d acc, int lane) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) tile[(lane & 15) * BCH_LD + (lane >> 4) + 4 * e] = acc[e];
+}
+
 __global__ __launch_bounds__(BCH_THREADS) void small_chol_inv_kernel(int n, const double* __restrict__ G, int64_t ldg,
                                                                      double* __restrict__ Linv, int64_t ldl,
                                                                      int* __restrict__ info) {
+  constexpr int NW = BCH_THREADS / 64;
   __shared__ double T[BCH_NTILES * BCH_TE];
-  __shared__ double W[BCH_TE];
   __shared__ int fail;
-  const int tid = threadIdx.x, nt = BCH_THREADS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int NB = (n + BCH_T - 1) / BCH_T;
-  constexpr int TT = BCH_T * BCH_T;
+  auto tile = [&](int I, int J) { return T + bch_tix(I, J) * BCH_TE; };
   if (tid == 0) fail = 0;
-  bch_load(n, NB, G, ldg, T, tid, nt);
+  bch_load(n, NB, G, ldg, T, tid, BCH_THREADS);
   __syncthreads();
   for (int K = 0; K < NB; ++K) {
-    if (tid < 64) {
-      const int f = bch_diag(T + bch_tix(K, K) * BCH_TE, tid);
-      if (tid == 0 && f) fail = K * BCH_T + f;
+    if (wave == 0) {
+      const int f = bch_diag(tile(K, K), lane);
+      if (lane == 0 && f) fail = K * BCH_T + f;
     }
     __syncthreads();
     if (fail) break;
-    // (b) panel: results held in registers until every thread has read A_IK
-    const int npan = (NB - K - 1) * TT;
-    constexpr int PMAX = (BCH_NBMAX - 1) * TT / BCH_THREADS;
-    double pv[PMAX];
-    int pidx[PMAX];
-#pragma unroll
-    for (int q = 0; q < PMAX; ++q) {
-      const int e = tid + q * nt;
-      pidx[q] = -1;
-      if (e < npan) {
-        int I, r, j;
-        pv[q] = bch_panel(T, NB, K, e, &I, &r, &j);
-        pidx[q] = bch_at(I, K, r, j);
-      }
+    // (b) panel: L_IK = A_IK Dinv_K^T, a wave per tile (it reads only its own tile and Dinv_K)
+    const double* D = tile(K, K);
+    for (int I = K + 1 + wave; I < NB; I += NW) {
+      double* t = tile(I, K);
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      acc = bch_mm([&](int r, int k) { return t[k * BCH_LD + r]; }, [&](int k, int c) { return D[k * BCH_LD + c]; },
+                   acc, lane);
+      bch_store(t, acc, lane);
     }
     __syncthreads();
+    // (c) trailing tiles (I, J), K < J <= I: A_IJ - L_IK L_JK^T, a wave per tile
+    const int m = NB - K - 1, ntr = m * (m + 1) / 2;
+    for (int q = wave; q < ntr; q += NW) {
+      int a = 0;
+      while ((a + 1) * (a + 2) / 2 <= q) ++a;
+      const int I = K + 1 + a, J = K + 1 + (q - a * (a + 1) / 2);
+      double* t = tile(I, J);
+      const double *LI = tile(I, K), *LJ = tile(J, K);
+      v4d acc;
 #pragma unroll
-    for (int q = 0; q < PMAX; ++q)
-      if (pidx[q] >= 0) T[pidx[q]] = pv[q];
-    __syncthreads();
-    // (c) trailing tiles (disjoint from the panel it reads)
-    const int m = NB - K - 1, ntr = m * (m + 1) / 2 * TT;
-    for (int e = tid; e < ntr; e += nt) bch_update(T, NB, K, e);
+      for (int e = 0; e < 4; ++e) acc[e] = t[(lane & 15) * BCH_LD + (lane >> 4) + 4 * e];
+      acc = bch_mm([&](int r, int k) { return -LI[k * BCH_LD + r]; }, [&](int k, int c) { return LJ[k * BCH_LD + c]; },
+                   acc, lane);
+      bch_store(t, acc, lane);
+    }
     __syncthreads();
   }
   if (fail) {
     if (tid == 0) *info = fail <= n ? fail : n;
     return;
   }
+  // inverse, tile columns J = NB-2 .. 0; wave w computes X_IJ for the rows
+  // I = J + 1 + w + NW u (u < IPW), held until column J of L is read
+  constexpr int IPW = (BCH_NBMAX - 1 + NW - 1) / NW;
   for (int J = NB - 2; J >= 0; --J) {
-    for (int I = NB - 1; I > J; --I) {
-      for (int e = tid; e < TT; e += nt) W[(e / BCH_T) * BCH_LD + e % BCH_T] = bch_isum(T, I, J, e % BCH_T, e / BCH_T);
-      __syncthreads();
-      for (int e = tid; e < TT; e += nt) T[bch_at(I, J, e % BCH_T, e / BCH_T)] = bch_iprod(T, W, J, e % BCH_T, e / BCH_T);
-      __syncthreads();
+    v4d x[IPW];
+#pragma unroll
+    for (int u = 0; u < IPW; ++u) {
+      const int I = J + 1 + wave + NW * u;
+      x[u] = (v4d){0.0, 0.0, 0.0, 0.0};
+      if (I >= NB) continue;  // uniform
+      v4d wt = {0.0, 0.0, 0.0, 0.0};  // W^T = sum_M L_MJ^T X_IM^T
+      for (int M = J + 1; M <= I; ++M) {
+        const double *LM = tile(M, J), *XI = tile(I, M);
+        wt = bch_mm([&](int r, int k) { return LM[r * BCH_LD + k]; }, [&](int k, int c) { return XI[k * BCH_LD + c]; },
+                    wt, lane);
+      }
+      // X_IJ = -W X_JJ: the A operand at k step s is -wt[s]
+      const double* XJ = tile(J, J);
+      const int kq = lane >> 4, c = lane & 15;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        x[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-wt[s], XJ[c * BCH_LD + 4 * s + kq], x[u], 0, 0, 0);
     }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < IPW; ++u) {
+      const int I = J + 1 + wave + NW * u;
+      if (I < NB) bch_store(tile(I, J), x[u], lane);
+    }
+    __syncthreads();
   }
-  for (int e = tid; e < n * n; e += nt) {
+  for (int e = tid; e < n * n; e += BCH_THREADS) {
     const int i = e % n, j = e / n;
     Linv[i + j * ldl] = i >= j ? T[bch_at(i / BCH_T, j / BCH_T, i % BCH_T, j % BCH_T)] : 0.0;
   }

@@ -565,11 +565,12 @@ def test_mixed_energy_contractions_fixed_env(pkg, emode):
     assert abs(Em.real - E64.real) < (1e-5 if emode == "bf16x3" else 1e-7)
 
 
-@pytest.mark.parametrize("n", [1, 31, 33, 96, 160])
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 31, 33, 96, 144, 160, 161, 288, 320])
 def test_orth_methods(pkg, n):
     """Range bases of a tall Y (5184 x n, cond(Y) ~ 1e4): Householder (0),
     CholeskyQR2 on rocSOLVER (1) and on the blocked one-workgroup Cholesky
-    (2, tile edges and the n = 160 limit covered).  Q^T Q = I to 1e-13, Q
+    (2: 16 x 16 tile edges, the n = 160 limit of one kernel and the 2 x 2
+    block split up to 320 covered).  Q^T Q = I to 1e-13, Q
     spans Y, and 1 and 2 agree to 1e-10 (both are the Q of the QR with a
     positive-diagonal R; 0 agrees up to column signs)."""
     rng = np.random.default_rng(n)
