@@ -331,6 +331,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--shape-stats", default=None,
+                    help="after the timed mixed steps, time one more step + adjoint per kernel and GEMM shape "
+                         "(HIP events) and write {name: [launches, ms, flops, bytes]} to this JSON file")
     ap.add_argument("--c5", action="store_true",
                     help="run the config-5 excited-state band structure instead (one JSON line of its own)")
     ap.add_argument("--c5-momenta", type=int, default=6)
@@ -388,6 +391,14 @@ def main():
         stats = ctx.kernel_stats(reset=True)
         ctx.set_timing(False)
         stats = {k: v for k, v in stats.items() if k == dom}
+        if a.shape_stats and dtype == "mixed":
+            # one more unit with every launch timed per kernel and GEMM shape
+            ctx.set_timing(2)
+            pkg.step_adjoint(ts, chi, H)
+            ctx.sync()
+            with open(a.shape_stats, "w") as f:
+                json.dump(ctx.kernel_stats(reset=True), f)
+            ctx.set_timing(False)
         return t, E, g, stats, ts, mix
 
     mixed_opts = dict(mixed=a.mixed, mixed_back=a.mixed_back, mixed_proj=a.mixed_proj)

@@ -6,7 +6,9 @@
 //  * F32   : FP32 storage, operands split into NSPLIT bf16 terms when staged
 //            into LDS (hi = bf16(x), mid = bf16(x - hi), ...); products with
 //            i + j < NSPLIT go through v_mfma_f32_16x16x32_bf16 with FP32
-//            accumulation.
+//            accumulation.  At full FP32 accuracy (the "6" setting) the
+//            operands are staged as FP32 and multiplied by
+//            v_mfma_f32_16x16x4f32 (gemm_f32x_kernel).
 //
 // Block: 256 threads = 4 waves of 64 in a 2 x 2 grid, each wave owns a
 // 32 x 32 sub-tile = 2 x 2 MFMA 16 x 16 output tiles.
@@ -123,10 +125,6 @@ struct TileLoader {
   }
 };
 
-// The same kernel on FP32 data with FP32 MFMA products (v_mfma_f32_16x16x4f32,
-// whose output layout differs: lane l, element r holds C[4 (l/16) + r][l % 16])
-// serves the FP32 products asked at full FP32 accuracy (the bf16x6 setting):
-// one FP32 product per term instead of six bf16 ones and no bf16 planes.
 template <typename T>
 struct RealMfma;
 template <>
@@ -135,14 +133,6 @@ struct RealMfma<double> {
   static __device__ inline acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
   static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
-template <>
-struct RealMfma<float> {
-  typedef v4f32 acc_t;
-  static __device__ inline acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-template <>
-__device__ inline float zero_t<float>() { return 0.f; }
 
 template <typename T, bool AT, bool BT>
 __global__ __launch_bounds__(256) void gemm_real_kernel(GemmP p) {
@@ -629,6 +619,174 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
 }
 
 // ---------------------------------------------------------------------------
+// FP32 storage, FP32 MFMA products (the full-accuracy "6" setting): the block
+// structure of gemm_f32_kernel with the operand tiles staged in LDS as FP32,
+// k-contiguous rows of 32 (+4 pad: the 16-B reads of 16 consecutive rows hit
+// distinct banks).  Each v_mfma_f32_16x16x4f32 sums four k; lane group
+// g = lane / 16 takes physical k = 16 h + 4 g + kk at MFMA kk of half h, the
+// same for A and B, so one 16-B LDS read per operand fragment feeds four
+// MFMAs.  Block tile TBM x TBN (128 x 64, 128 x 96, 128 x 128 or 64 x 192),
+// 256 threads = 4 waves in a 2 x 2 grid.
+// ---------------------------------------------------------------------------
+// One operand: TR rows x 32 k per K step, item (r, g) = 8 values k = 8g .. 8g+7
+// of row r; MODE 0 row-contiguous (X[r + k ld]), 1 k-contiguous (X[k + r ld]).
+// Items past TR x 4 (TR not a multiple of 64) load a valid row and are not staged.
+template <int MODE, int TR>
+struct F32xOperand {
+  static constexpr int NI = (TR * 4 + 255) / 256;
+  int r[NI], g[NI];
+  bool live[NI];
+  int64_t rc[NI];
+  float v[NI][8];
+  __device__ inline void init(int tid, int64_t r0, int64_t R) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const int q = tid + 256 * i;
+      live[i] = q < TR * 4;
+      const int qq = live[i] ? q : 0;
+      if (MODE == 0) { r[i] = qq % TR; g[i] = qq / TR; }
+      else { g[i] = qq & 3; r[i] = qq >> 2; }
+      rc[i] = min(r0 + r[i], R - 1);
+    }
+  }
+  __device__ inline void fetch(const float* X, int64_t ld, int64_t k0, bool vec) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) load8<MODE == 1>(X, ld, rc[i], k0 + 8 * g[i], vec, v[i]);
+  }
+  __device__ inline void fetch_tail(const float* X, int64_t ld, int64_t k0, int64_t Keff) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) load8_tail<MODE == 1>(X, ld, rc[i], k0 + 8 * g[i], Keff, v[i]);
+  }
+  __device__ inline void zero_tail(int64_t k0, int64_t Keff) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) zero8(v[i], k0 + 8 * g[i], Keff);
+  }
+  template <int LDK>
+  __device__ inline void stage(float* dst) const {
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+      if (live[i]) {
+        float* d = dst + r[i] * LDK + 8 * g[i];
+        *reinterpret_cast<float4*>(d) = make_float4(v[i][0], v[i][1], v[i][2], v[i][3]);
+        *reinterpret_cast<float4*>(d + 4) = make_float4(v[i][4], v[i][5], v[i][6], v[i][7]);
+      }
+  }
+};
+
+template <int MA, int MB, int TBM, int TBN>
+__global__ __launch_bounds__(256) void gemm_f32x_kernel(GemmP p) {
+  constexpr int BK = 32, LDK = BK + 4;
+  constexpr int WTM = TBM / 2, WTN = TBN / 2;
+  constexpr int TI = WTM / 16, TJ = WTN / 16;
+  __shared__ __attribute__((aligned(16))) float As[2][TBM][LDK];
+  __shared__ __attribute__((aligned(16))) float Bs[2][TBN][LDK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const BlockXYZ bk = xcd_block();
+  const int64_t m0 = bk.x * TBM, n0 = bk.y * TBN;
+  const float* A = (const float*)p.A + bk.z * p.sA;
+  const float* B = (const float*)p.B + bk.z * p.sB;
+  float* C = (float*)p.C + bk.z * p.sC;
+  // split-K: block z covers k in [z*kchunk, min(K, (z+1)*kchunk)) (A/B offsets via sA/sB)
+  const int64_t Keff = p.kchunk ? min(p.kchunk, p.K - bk.z * p.kchunk) : p.K;
+  const bool vecA = p.vecA != 0, vecB = p.vecB != 0;
+
+  F32xOperand<MA, TBM> oa;
+  F32xOperand<MB, TBN> ob;
+  oa.init(tid, m0, p.M);
+  ob.init(tid, n0, p.N);
+  auto stage = [&](int buf) {
+    oa.template stage<LDK>(&As[buf][0][0]);
+    ob.template stage<LDK>(&Bs[buf][0][0]);
+  };
+
+  v4f32 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = (v4f32){0.f, 0.f, 0.f, 0.f};
+
+  const int64_t nk = (Keff + BK - 1) / BK, nfull = Keff / BK;
+  if (nfull > 0) {
+    oa.fetch(A, p.lda, 0, vecA);
+    ob.fetch(B, p.ldb, 0, vecB);
+  } else {
+    oa.fetch_tail(A, p.lda, 0, Keff);
+    ob.fetch_tail(B, p.ldb, 0, Keff);
+    oa.zero_tail(0, Keff);
+    ob.zero_tail(0, Keff);
+  }
+  stage(0);
+  __syncthreads();
+  const int lr = lane & 15, kq = 4 * (lane >> 4);
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int cur = (int)(kt & 1);
+    const int64_t kn = (kt + 1) * BK;
+    const bool more = kt + 1 < nk, full = kt + 1 < nfull;
+    if (full) {
+      oa.fetch(A, p.lda, kn, vecA);
+      ob.fetch(B, p.ldb, kn, vecB);
+    } else if (more) {
+      oa.fetch_tail(A, p.lda, kn, Keff);
+      ob.fetch_tail(B, p.ldb, kn, Keff);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float4 a[TI], b[TJ];
+#pragma unroll
+      for (int i = 0; i < TI; ++i) a[i] = *reinterpret_cast<const float4*>(&As[cur][wm * WTM + 16 * i + lr][16 * h + kq]);
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) b[j] = *reinterpret_cast<const float4*>(&Bs[cur][wn * WTN + 16 * j + lr][16 * h + kq]);
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].x, b[j].x, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].y, b[j].y, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].z, b[j].z, acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].w, b[j].w, acc[i][j], 0, 0, 0);
+        }
+    }
+    if (more) {
+      if (!full) {
+        oa.zero_tail(kn, Keff);
+        ob.zero_tail(kn, Keff);
+      }
+      stage(cur ^ 1);
+    }
+    __syncthreads();
+  }
+
+  const float al = (float)p.alpha, be = (float)p.beta;
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+      const int64_t col = n0 + wn * WTN + 16 * j + lr;
+      const int64_t row0 = m0 + wm * WTM + 16 * i + 4 * (lane >> 4);
+      if (col >= p.N || row0 >= p.M) continue;
+      float* dst = C + row0 + col * p.ldc;
+      if (p.vecC && row0 + 3 < p.M) {
+        float4 v = make_float4(al * acc[i][j][0], al * acc[i][j][1], al * acc[i][j][2], al * acc[i][j][3]);
+        if (p.beta != 0.0) {
+          const float4 o = *reinterpret_cast<const float4*>(dst);
+          v.x += be * o.x; v.y += be * o.y; v.z += be * o.z; v.w += be * o.w;
+        }
+        *reinterpret_cast<float4*>(dst) = v;
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (row0 + r < p.M) {
+            float v = al * acc[i][j][r];
+            if (p.beta != 0.0) v += be * dst[r];
+            dst[r] = v;
+          }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // split-K reduction: C = alpha * sum_s W_s + beta * C
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -738,13 +896,23 @@ __global__ __launch_bounds__(256) void split_operand_rows_kernel(const float* __
 // streamed from HBM once; a 256-row tile measured slower on the skinny
 // chi-sized products: fewer, single-wave-per-SIMD blocks); else 128 x 128.
 static int f32_tile_n(int64_t N) { return N <= 64 ? 64 : (N > 128 && N <= 192) ? 192 : 128; }
+// FP32-MFMA kernel: the column tile (64, 96, 128, 192) that pads N least, the
+// wider one on a tie
+static int f32x_tile_n(int64_t N) {
+  int best = 64;
+  int64_t waste = (N + 63) / 64 * 64 - N;
+  for (int tn : {96, 128, 192}) {
+    const int64_t w = (N + tn - 1) / tn * tn - N;
+    if (w <= waste) { best = tn; waste = w; }
+  }
+  return best;
+}
 static int f32_tile_m(int64_t N, Mixed) { return f32_tile_n(N) == 192 ? 64 : 128; }
 
 static bool al16(const void* ptr) { return ((uintptr_t)ptr & 15u) == 0; }
 
 // FP32 data at full FP32 accuracy (the bf16x6 setting) runs on FP32 MFMA
-// (gemm_real_kernel<float>, 64 x 64 tiles); the other FP32 settings on the
-// split-bf16 kernel.
+// (gemm_f32x_kernel); the other FP32 settings on the split-bf16 kernel.
 static bool real_f32(const GemmArgs& g) { return g.dt == DType::F32 && g.mixed == Mixed::BF16X6; }
 static bool split_f32(const GemmArgs& g) { return g.dt == DType::F32 && g.mixed != Mixed::BF16X6; }
 
@@ -766,10 +934,22 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid, int ma,
     else if (!at && bt) hipLaunchKernelGGL((gemm_real_kernel<double, false, true>), grid, block, 0, c.stream, p);
     else hipLaunchKernelGGL((gemm_real_kernel<double, true, true>), grid, block, 0, c.stream, p);
   } else if (real_f32(g)) {
-    if (!at && !bt) hipLaunchKernelGGL((gemm_real_kernel<float, false, false>), grid, block, 0, c.stream, p);
-    else if (at && !bt) hipLaunchKernelGGL((gemm_real_kernel<float, true, false>), grid, block, 0, c.stream, p);
-    else if (!at && bt) hipLaunchKernelGGL((gemm_real_kernel<float, false, true>), grid, block, 0, c.stream, p);
-    else hipLaunchKernelGGL((gemm_real_kernel<float, true, true>), grid, block, 0, c.stream, p);
+#define IPEPS_F32X_MB(MA, TM, TN)                                                                       \
+  do {                                                                                                  \
+    if (mb == 0) hipLaunchKernelGGL((gemm_f32x_kernel<MA, 0, TM, TN>), grid, block, 0, c.stream, p);     \
+    else hipLaunchKernelGGL((gemm_f32x_kernel<MA, 1, TM, TN>), grid, block, 0, c.stream, p);             \
+  } while (0)
+#define IPEPS_F32X(TM, TN)                      \
+  do {                                          \
+    if (ma == 0) IPEPS_F32X_MB(0, TM, TN);      \
+    else IPEPS_F32X_MB(1, TM, TN);              \
+  } while (0)
+    if (tn == 64) IPEPS_F32X(128, 64);
+    else if (tn == 96) IPEPS_F32X(128, 96);
+    else if (tn == 192) IPEPS_F32X(64, 192);
+    else IPEPS_F32X(128, 128);
+#undef IPEPS_F32X
+#undef IPEPS_F32X_MB
   } else if (g.dt == DType::C128) {
     if (!at && !bt) hipLaunchKernelGGL((gemm_c128_kernel<false, false>), grid, block, 0, c.stream, p);
     else if (at && !bt) hipLaunchKernelGGL((gemm_c128_kernel<true, false>), grid, block, 0, c.stream, p);
@@ -797,7 +977,7 @@ static void launch(Ctx& c, const GemmArgs& g, const GemmP& p, dim3 grid, int ma,
     switch (g.mixed) {
       case Mixed::BF16: IPEPS_F32_LAUNCH(1, 64); break;
       case Mixed::BF16X3: IPEPS_F32_LAUNCH(2, 64); break;
-      case Mixed::BF16X6: throw Error(IPEPS_ERR_INTERNAL, "gemm: FP32 products run on gemm_real_kernel<float>");
+      case Mixed::BF16X6: throw Error(IPEPS_ERR_INTERNAL, "gemm: FP32 products run on gemm_f32x_kernel");
     }
 #undef IPEPS_F32_LAUNCH
 #undef IPEPS_F32_LAUNCH_T
@@ -973,21 +1153,26 @@ struct Plan {
   int64_t S = 1;
 };
 
-static int64_t tile_m(const GemmArgs& g, const Plan& pl) { return split_f32(g) ? (pl.tn == 192 ? 64 : 128) : BM; }
-static int64_t tile_n(const GemmArgs& g, const Plan& pl) { return split_f32(g) ? pl.tn : BN; }
+static bool tiled_f32(const GemmArgs& g) { return g.dt == DType::F32; }
+static int64_t tile_m(const GemmArgs& g, const Plan& pl) { return tiled_f32(g) ? (pl.tn == 192 ? 64 : 128) : BM; }
+static int64_t tile_n(const GemmArgs& g, const Plan& pl) { return tiled_f32(g) ? pl.tn : BN; }
 static int64_t out_tiles(const GemmArgs& g, const Plan& pl) {
   return ((g.M + tile_m(g, pl) - 1) / tile_m(g, pl)) * ((g.N + tile_n(g, pl) - 1) / tile_n(g, pl)) * g.batch;
 }
 // Split-K applies to one long-K product (batch 1, K >= 1024) whose output
 // tiles cannot fill the 256 CUs; slices are at least kmin long.
 static bool splitk_allowed(const GemmArgs& g, int64_t tiles) { return g.batch == 1 && tiles < 600 && g.K >= 1024; }
-static int64_t splitk_kmin(const GemmArgs& g) { return split_f32(g) ? 256 : 64; }
+static int64_t splitk_kmin(const GemmArgs& g) { return tiled_f32(g) ? 256 : 64; }
+// blocks the split-K rule aims at: the FP32-MFMA kernel's 128-row tiles at two
+// per CU; the split-bf16 kernel ~1024; FP64 / C128 (64 x 64 tiles) ~2048
+static int64_t splitk_target(const GemmArgs& g) { return real_f32(g) ? 512 : split_f32(g) ? 1024 : 2048; }
 // A stored plan is applied only where it is valid for the shape: FP32 tiles of
 // 64 / 128 / 192 columns, and split-K (S >= 2) only where splitk_allowed holds
 // (execute's grid.z indexes the K chunks then, so a batched product would
 // lose every batch but the first).
 static bool plan_valid(const GemmArgs& g, const Plan& pl) {
   if (split_f32(g) && !(pl.tn == 64 || pl.tn == 128 || pl.tn == 192)) return false;
+  if (real_f32(g) && !(pl.tn == 64 || pl.tn == 96 || pl.tn == 128 || pl.tn == 192)) return false;
   return pl.S == 1 || (pl.S >= 2 && splitk_allowed(g, out_tiles(g, pl)));
 }
 
@@ -996,10 +1181,10 @@ static bool plan_valid(const GemmArgs& g, const Plan& pl) {
 // K step, chunks >= 64).
 static Plan rule_plan(const GemmArgs& g) {
   Plan pl;
-  pl.tn = split_f32(g) ? f32_tile_n(g.N) : BN;
+  pl.tn = split_f32(g) ? f32_tile_n(g.N) : real_f32(g) ? f32x_tile_n(g.N) : BN;
   const int64_t tiles = out_tiles(g, pl);
   if (splitk_allowed(g, tiles)) {
-    const int64_t target = split_f32(g) ? 1024 : 2048;
+    const int64_t target = splitk_target(g);
     pl.S = std::max<int64_t>(1, std::min<int64_t>((target + tiles - 1) / tiles, g.K / splitk_kmin(g)));
   }
   return pl;
@@ -1169,6 +1354,9 @@ std::vector<Plan> candidates(const GemmArgs& g) {
   if (split_f32(g)) {
     tns = {64, 128};
     if (g.N > 128) tns.push_back(192);
+  } else if (real_f32(g)) {
+    tns = {64, 96, 128};
+    if (g.N > 128) tns.push_back(192);
   } else {
     tns = {(int)BN};
   }
@@ -1183,7 +1371,7 @@ std::vector<Plan> candidates(const GemmArgs& g) {
     std::vector<int64_t> ss;
     if (splitk_allowed(g, tiles)) {
       const int64_t smax = g.K / splitk_kmin(g);
-      const int64_t target = split_f32(g) ? 1024 : 2048;
+      const int64_t target = splitk_target(g);
       const int64_t r = std::max<int64_t>(1, std::min<int64_t>((target + tiles - 1) / tiles, smax));
       if (r <= 4) ss.push_back(1);
       for (int64_t s : {r / 2, r, 2 * r})
@@ -1286,7 +1474,7 @@ void tune_load(const std::string& path) {
     char key[256];
     int tn;
     long long S;
-    if (sscanf(line, "%255s %d %lld", key, &tn, &S) != 3 || !(tn == 64 || tn == 128 || tn == 192) || S < 1 ||
+    if (sscanf(line, "%255s %d %lld", key, &tn, &S) != 3 || !(tn == 64 || tn == 96 || tn == 128 || tn == 192) || S < 1 ||
         S > 65535) {
       fclose(f);
       throw Error(IPEPS_ERR_ARG, "tune_load: malformed line " + std::to_string(ln) + " of " + path);
@@ -1299,6 +1487,14 @@ void tune_load(const std::string& path) {
         kM < 1 || kN < 1 || kK < 1 || kbatch < 1) {
       fclose(f);
       throw Error(IPEPS_ERR_ARG, "tune_load: malformed key on line " + std::to_string(ln) + " of " + path);
+    }
+    // column tiles: FP64 / C128 kernels 64; split-bf16 FP32 64 / 128 / 192;
+    // FP32-MFMA also 96
+    const bool tile_ok = kdt == (int)DType::F32 ? (tn != 96 || kmx == (int)Mixed::BF16X6) : tn == (int)BN;
+    if (!tile_ok) {
+      fclose(f);
+      throw Error(IPEPS_ERR_ARG, "tune_load: line " + std::to_string(ln) + " of " + path + " gives tile " +
+                                     std::to_string(tn) + " to a kernel without it");
     }
     if (S >= 2 && (kbatch != 1 || kK < 1024)) {
       fclose(f);

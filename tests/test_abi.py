@@ -94,9 +94,15 @@ def test_tune_table_file_handling(pkg, tmp_path):
     p.write_text("0.3.00.10.10.10.1.0.000 96 1\n")
     with pytest.raises(pkg.IpepsError):
         pkg.tune_load(str(p))
-    p.write_text("# header\n0.3.00.10.10.2048.1.0.000 192 4\n")
+    # column tiles by kernel family (key = dtype.mixed...: dtype 0 FP64, 2 FP32; mixed 6 = FP32 MFMA)
+    for bad in ("0.3.00.10.10.2048.1.0.000 128 1", "2.3.00.10.10.2048.1.0.000 96 1"):
+        p.write_text(bad + "\n")
+        with pytest.raises(pkg.IpepsError):
+            pkg.tune_load(str(p))
+    p.write_text("# header\n2.3.00.10.10.2048.1.0.000 192 4\n2.6.00.10.10.2048.1.0.000 96 2\n"
+                 "0.3.00.10.10.2048.1.0.000 64 4\n")
     pkg.tune_load(str(p))
-    assert pkg.tune_stats()["entries"] == 1
+    assert pkg.tune_stats()["entries"] == 3
     pkg.tune_clear()
     with pytest.raises(pkg.IpepsError):
         pkg.tune_load(str(tmp_path / "missing.txt"))
