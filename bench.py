@@ -316,6 +316,10 @@ def main():
                     help="bf16 products of the VJPs (0: same as --mixed)")
     ap.add_argument("--mixed-proj", type=int, default=6, choices=[0, 1, 3, 6],
                     help="bf16 products inside the projector (0: same as --mixed; 6: FP32 products)")
+    ap.add_argument("--early-mixed", type=int, default=0, choices=[0, 1, 3, 6],
+                    help="products of the range finder's early applications of M (all but the last --late-apps "
+                         "before B = M^H Q; 0: same as --mixed-proj)")
+    ap.add_argument("--late-apps", type=int, default=2)
     ap.add_argument("--tune", type=int, default=2, choices=[0, 1, 2],
                     help="GEMM launch plans: 0 shape rules, 1 autotune now (unseen shapes are timed during the "
                          "warm-up steps), 2 replay the tuning table --tune-file (shapes missing from it: rules)")
@@ -401,7 +405,8 @@ def main():
             ctx.set_timing(False)
         return t, E, g, stats, ts, mix
 
-    mixed_opts = dict(mixed=a.mixed, mixed_back=a.mixed_back, mixed_proj=a.mixed_proj)
+    mixed_opts = dict(mixed=a.mixed, mixed_back=a.mixed_back, mixed_proj=a.mixed_proj, early_mixed=a.early_mixed,
+                      late_apps=a.late_apps)
     if a.tune == 2 and not os.path.exists(a.tune_file):
         log(f"[bench] no tuning table at {a.tune_file}: tuning now")
         a.tune = 1

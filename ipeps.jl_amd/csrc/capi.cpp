@@ -120,6 +120,14 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
     IPEPS_CHECK(v >= 0, "complement_iters must be >= 0");
     o.svd.complement_iters = (int)v;
   } else if (key == "complement_tol") o.svd.complement_tol = v;
+  else if (key == "early_mixed") {
+    int m = (int)v;
+    IPEPS_CHECK(m == 0 || m == 1 || m == 3 || m == 6, "early_mixed must be 0, 1, 3 or 6");
+    o.svd.early_mixed = m;
+  } else if (key == "late_apps") {
+    IPEPS_CHECK(v >= 0, "late_apps must be >= 0");
+    o.svd.late_apps = (int)v;
+  }
   else if (key == "canonical_edge") o.canonical_edge = v != 0.0;
   else if (key == "energy_fp64") o.energy_fp64 = v != 0.0;
   else if (key == "checkpoint") o.checkpoint = v != 0.0;

@@ -78,13 +78,15 @@ Tensor edge_update(Ctx& c, const Env& e, const Tensor& E, const L& ia, const L& 
   Tensor P2r = op::reshape(c, Pd, {chip, chi_e, D, D});
   Tensor X1 = tcon(c, P2r, {-1, 1, -2, -3}, E, {1, -4, -5, -6});  // (i, ap2, adp2, E2, Eket, Ebra)
   // A and Ad absorbed over (ap2, Eket) / (adp2, Ebra) and the physical leg:
-  // one site absorption (fused kernel on the mixed engine)
+  // one site absorption (fused kernel on the mixed engine), its output with
+  // the legs P1 contracts last, so that X3 enters the last product as a
+  // plain M x K matrix
   L la(5), ld(5);
-  la[a_p2] = 10; la[a_E] = 11; la[a_p1] = -3; la[a_open] = -4; la[a_ph] = 3;
-  ld[d_p2] = 20; ld[d_E] = 21; ld[d_ph] = 3; ld[d_p1] = -5; ld[d_open] = -6;
-  Tensor X3 = op::absorb(c, X1, {-1, 10, 20, -2, 11, 21}, A, la, Ad, ld);  // (i, E2, Ap1, Aopen, Adp1, Adopen)
+  la[a_p2] = 10; la[a_E] = 11; la[a_p1] = -5; la[a_open] = -2; la[a_ph] = 3;
+  ld[d_p2] = 20; ld[d_E] = 21; ld[d_ph] = 3; ld[d_p1] = -6; ld[d_open] = -3;
+  Tensor X3 = op::absorb(c, X1, {-1, 10, 20, -4, 11, 21}, A, la, Ad, ld);  // (i, Aopen, Adopen, E2, Ap1, Adp1)
   Tensor P1r = op::reshape(c, P, {E.dims[1], D, D, chin});
-  return tcon(c, X3, {-1, 30, 31, -3, 32, -4}, P1r, {30, 31, 32, -2});
+  return tcon(c, X3, {-1, -3, -4, 30, 31, 32}, P1r, {30, 31, 32, -2});
 }
 
 void finish(Ctx& c, Env& e, int ci, const Tensor& C_, int cj, const Tensor& Cb, int ek, const Tensor& E_) {
@@ -96,13 +98,16 @@ void finish(Ctx& c, Env& e, int ci, const Tensor& C_, int cj, const Tensor& Cb, 
 std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) {
   const Tensor &C1 = e.C[0], &C4 = e.C[3], &E1 = e.E[0], &E3 = e.E[2], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
   // get_projector_left (src/ctm/ctmrg.jl:137-160)
-  Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});
-  Tensor CEE4 = tcon(c, C1E1, {1, -2, -5, -3}, E4, {1, -1, -6, -4});
-  Tensor LU = op::absorb(c, CEE4, {-1, -4, 11, 12, 21, 22}, A, {21, 22, -2, -5, 3}, Ad, {11, 12, -3, -6, 3});
-  Tensor R1 = tr2(c, mat(c, LU, 3));
-  Tensor C4E3 = tcon(c, C4, {-1, 1}, E3, {1, -2, -3, -4});
-  Tensor CEE4b = tcon(c, C4E3, {1, -2, -6, -4}, E4, {-1, 1, -5, -3});
-  Tensor LB = op::absorb(c, CEE4b, {-1, -4, 11, 12, 21, 22}, A, {-2, 21, 22, -5, 3}, Ad, {-3, 11, 12, -6, 3});
+  // The (C E) E products come out in the GEMM's own leg order (no permute)
+  // with the row leg that is stride 1 in the absorbed corner first; the
+  // absorption's output order makes R1 without a transpose.
+  Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});                  // (c1d, e1r, e1k, e1b)
+  Tensor CEE4 = tcon(c, C1E1, {1, -1, -2, -3}, E4, {1, -4, -5, -6});        // (e1r, e1k, e1b, e4d, e4k, e4b)
+  Tensor LU = op::absorb(c, CEE4, {-1, 21, 11, -4, 22, 12}, A, {21, 22, -5, -2, 3}, Ad, {11, 12, -6, -3, 3});
+  Tensor R1 = mat(c, LU, 3);
+  Tensor C4E3 = tcon(c, C4, {-1, 1}, E3, {1, -2, -3, -4});                  // (c4u, e3r, e3k, e3b)
+  Tensor CEE4b = tcon(c, C4E3, {1, -4, -5, -6}, E4, {-1, 1, -2, -3});       // (e4u, e4k, e4b, e3r, e3k, e3b)
+  Tensor LB = op::absorb(c, CEE4b, {-1, 21, 11, -4, 22, 12}, A, {-2, 21, 22, -5, 3}, Ad, {-3, 11, 12, -6, 3});
   Tensor R2 = mat(c, LB, 3);
   Proj p = get_projector(c, e, R1, R2, chi, draw);
   if (only) {
@@ -123,13 +128,13 @@ std::vector<double> left_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* on
 void right_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) {
   const Tensor &C2 = e.C[1], &C3 = e.C[2], &E1 = e.E[0], &E2 = e.E[1], &E3 = e.E[2], &A = e.A, &Ad = e.Ad;
   // get_projector_right (src/ctm/ctmrg.jl:176-198)
-  Tensor C2E1 = tcon(c, C2, {1, -2}, E1, {-1, 1, -3, -4});
-  Tensor CEE2 = tcon(c, C2E1, {-1, 1, -5, -3}, E2, {1, -2, -6, -4});
-  Tensor RU = op::absorb(c, CEE2, {-1, -4, 11, 12, 21, 22}, A, {21, -2, -5, 22, 3}, Ad, {11, -3, -6, 12, 3});
+  Tensor C2E1 = tcon(c, C2, {1, -1}, E1, {-2, 1, -3, -4});                  // (c2d, e1l, e1k, e1b)
+  Tensor CEE2 = tcon(c, C2E1, {1, -1, -2, -3}, E2, {1, -4, -5, -6});        // (e1l, e1k, e1b, e2d, e2k, e2b)
+  Tensor RU = op::absorb(c, CEE2, {-1, 21, 11, -4, 22, 12}, A, {21, -2, -5, 22, 3}, Ad, {11, -3, -6, 12, 3});
   Tensor R1 = mat(c, RU, 3);
-  Tensor C3E3 = tcon(c, C3, {-1, 1}, E3, {-2, 1, -3, -4});
-  Tensor CEE2b = tcon(c, C3E3, {1, -2, -5, -3}, E2, {-1, 1, -6, -4});
-  Tensor RB = op::absorb(c, CEE2b, {-1, -4, 11, 12, 21, 22}, A, {-2, -5, 21, 22, 3}, Ad, {-3, -6, 11, 12, 3});
+  Tensor C3E3 = tcon(c, C3, {-1, 1}, E3, {-2, 1, -3, -4});                  // (c3u, e3l, e3k, e3b)
+  Tensor CEE2b = tcon(c, C3E3, {1, -4, -5, -6}, E2, {-1, 1, -2, -3});       // (e2u, e2k, e2b, e3l, e3k, e3b)
+  Tensor RB = op::absorb(c, CEE2b, {-1, 22, 12, -4, 21, 11}, A, {-2, -5, 21, 22, 3}, Ad, {-3, -6, 11, 12, 3});
   Tensor R2 = mat(c, RB, 3);
   Proj p = get_projector(c, e, R1, R2, chi, draw);
   if (only) {
@@ -149,13 +154,13 @@ void right_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) 
 void top_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) {
   const Tensor &C1 = e.C[0], &C2 = e.C[1], &E1 = e.E[0], &E2 = e.E[1], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
   // get_projector_top (src/ctm/ctmrg.jl:214-236)
-  Tensor C1E4 = tcon(c, C1, {1, -2}, E4, {1, -1, -3, -4});
-  Tensor CEE1 = tcon(c, C1E4, {-1, 1, -6, -4}, E1, {1, -2, -5, -3});
-  Tensor UL = op::absorb(c, CEE1, {-1, -4, 11, 12, 21, 22}, A, {21, 22, -2, -5, 3}, Ad, {11, 12, -3, -6, 3});
+  Tensor C1E4 = tcon(c, C1, {1, -1}, E4, {1, -2, -3, -4});                  // (c1r, e4d, e4k, e4b)
+  Tensor CEE1 = tcon(c, C1E4, {1, -1, -2, -3}, E1, {1, -4, -5, -6});        // (e4d, e4k, e4b, e1r, e1k, e1b)
+  Tensor UL = op::absorb(c, CEE1, {-1, 22, 12, -4, 21, 11}, A, {21, 22, -2, -5, 3}, Ad, {11, 12, -3, -6, 3});
   Tensor R1 = mat(c, UL, 3);
-  Tensor C2E2 = tcon(c, C2, {-1, 1}, E2, {1, -2, -3, -4});
-  Tensor CEE1b = tcon(c, C2E2, {1, -2, -6, -4}, E1, {-1, 1, -5, -3});
-  Tensor UR = op::absorb(c, CEE1b, {-1, -4, 11, 12, 21, 22}, A, {21, -2, -5, 22, 3}, Ad, {11, -3, -6, 12, 3});
+  Tensor C2E2 = tcon(c, C2, {-1, 1}, E2, {1, -2, -3, -4});                  // (c2l, e2d, e2k, e2b)
+  Tensor CEE1b = tcon(c, C2E2, {1, -4, -5, -6}, E1, {-1, 1, -2, -3});       // (e1l, e1k, e1b, e2d, e2k, e2b)
+  Tensor UR = op::absorb(c, CEE1b, {-1, 21, 11, -4, 22, 12}, A, {21, -2, -5, 22, 3}, Ad, {11, -3, -6, 12, 3});
   Tensor R2 = mat(c, UR, 3);
   Proj p = get_projector(c, e, R1, R2, chi, draw);
   if (only) {
@@ -175,14 +180,14 @@ void top_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) {
 void bottom_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr) {
   const Tensor &C3 = e.C[2], &C4 = e.C[3], &E2 = e.E[1], &E3 = e.E[2], &E4 = e.E[3], &A = e.A, &Ad = e.Ad;
   // get_projector_bottom (src/ctm/ctmrg.jl:238-252... via 226-250)
-  Tensor C4E4 = tcon(c, C4, {1, -2}, E4, {-1, 1, -3, -4});
-  Tensor CEE3 = tcon(c, C4E4, {-1, 1, -5, -3}, E3, {1, -2, -6, -4});
-  Tensor BL = op::absorb(c, CEE3, {-1, -4, 11, 12, 21, 22}, A, {-2, 21, 22, -5, 3}, Ad, {-3, 11, 12, -6, 3});
+  Tensor C4E4 = tcon(c, C4, {1, -1}, E4, {-2, 1, -3, -4});                  // (c4r, e4u, e4k, e4b)
+  Tensor CEE3 = tcon(c, C4E4, {1, -1, -2, -3}, E3, {1, -4, -5, -6});        // (e4u, e4k, e4b, e3r, e3k, e3b)
+  Tensor BL = op::absorb(c, CEE3, {-1, 21, 11, -4, 22, 12}, A, {-2, 21, 22, -5, 3}, Ad, {-3, 11, 12, -6, 3});
   Tensor R1 = mat(c, BL, 3);
-  Tensor C3E2 = tcon(c, C3, {1, -2}, E2, {-1, 1, -3, -4});
-  Tensor CEE3b = tcon(c, C3E2, {-1, 1, -6, -4}, E3, {-2, 1, -5, -3});
-  Tensor BR = op::absorb(c, CEE3b, {-1, -4, 11, 12, 21, 22}, A, {-2, -5, 21, 22, 3}, Ad, {-3, -6, 11, 12, 3});
-  Tensor R2 = tr2(c, mat(c, BR, 3));
+  Tensor C3E2 = tcon(c, C3, {1, -1}, E2, {-2, 1, -3, -4});                  // (c3l, e2u, e2k, e2b)
+  Tensor CEE3b = tcon(c, C3E2, {1, -4, -5, -6}, E3, {-1, 1, -2, -3});       // (e3l, e3k, e3b, e2u, e2k, e2b)
+  Tensor BR = op::absorb(c, CEE3b, {-1, 21, 11, -4, 22, 12}, A, {-5, -2, 21, 22, 3}, Ad, {-6, -3, 11, 12, 3});
+  Tensor R2 = mat(c, BR, 3);
   Proj p = get_projector(c, e, R1, R2, chi, draw);
   if (only) {
     *only = p;

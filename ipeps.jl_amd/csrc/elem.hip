@@ -1369,29 +1369,44 @@ constexpr int TRB_RW = (TRI_BIG_MAXN + 63) / 64;  // rows per lane
 constexpr int TRB_U = 4;                          // columns per wave in flight
 
 struct TrbShared {
-  double v[TRI_BIG_MAXN], w[TRI_BIG_MAXN], tp[TRI_BIG_MAXN];
+  double v[2][TRI_BIG_MAXN], w[2][TRI_BIG_MAXN];  // (v, w) of steps j (buffer j & 1) and j - 1
+  double tp[TRI_BIG_MAXN];
   double part[TRB_NWAVE][TRI_BIG_MAXN];  // per-wave lower-part partials of p
   double red[TRB_NWAVE];
   double sc[2];                          // tau, scale
 };
 
-// Step i of the reduction (tridiag_impl.h) on the triangle A; the reflector
-// goes to column i of A and of V (column-major n x n, zero elsewhere).
-__device__ void trb_step(const TriView& A, int n, int i, TrbShared& sh, double* __restrict__ d,
-                         double* __restrict__ e, double* __restrict__ tau, double* __restrict__ V) {
+// Reflector of step j from column j of A.  With `upd`, column j still lacks
+// step j - 1's update A <- A - v w^T - w v^T (vp, wp: step j - 1's vectors,
+// local index = row - j), applied here first by wave 0; each lane re-reads
+// only the rows it wrote.  The reflector goes to column j of A, of V and to
+// vj (local index = row - j - 1); returns tau_j through sh.sc.
+__device__ void trb_reflector(const TriView& A, int n, int j, bool upd, const double* vp, const double* wp,
+                              double* vj, TrbShared& sh, double* __restrict__ d, double* __restrict__ e,
+                              double* __restrict__ tau, double* __restrict__ V) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m = n - 1 - i;
-  double* ci = A.col(i);
-  // (a) reflector of x = A(i+1 .., i)
+  const int m = n - 1 - j;
+  double* cj = A.col(j);
   if (wave == 0) {
+    const double vp0 = upd ? vp[0] : 0.0, wp0 = upd ? wp[0] : 0.0;
     double x2 = 0.0;
-    for (int r = i + 2 + lane; r < n; r += 64) {
-      const double x = ci[r];
+    for (int r = j + 2 + lane; r < n; r += 64) {
+      double x = cj[r];
+      if (upd) {
+        x = x - (vp[r - j] * wp0 + wp[r - j] * vp0);
+        cj[r] = x;
+      }
       x2 += x * x;
     }
     x2 = wave_sum_d(x2);
     if (lane == 0) {
-      const double alpha = ci[i + 1];
+      double a0 = cj[j], alpha = cj[j + 1];
+      if (upd) {
+        a0 = a0 - (vp[0] * wp0 + wp[0] * vp0);
+        alpha = alpha - (vp[1] * wp0 + wp[1] * vp0);
+        cj[j] = a0;
+        cj[j + 1] = alpha;
+      }
       double t = 0.0, beta = alpha, scale = 0.0;
       if (x2 > 0.0) {
         beta = -copysign(sqrt(alpha * alpha + x2), alpha);
@@ -1400,23 +1415,34 @@ __device__ void trb_step(const TriView& A, int n, int i, TrbShared& sh, double* 
       }
       sh.sc[0] = t;
       sh.sc[1] = scale;
-      d[i] = ci[i];
-      e[i] = beta;
-      tau[i] = t;
+      d[j] = a0;
+      e[j] = beta;
+      tau[j] = t;
     }
   }
   __syncthreads();
-  const double t = sh.sc[0], scale = sh.sc[1];
+  const double scale = sh.sc[1];
   for (int r = tid; r < m; r += TRI_THREADS) {
-    const double vr = r == 0 ? 1.0 : ci[i + 1 + r] * scale;
-    sh.v[r] = vr;
-    ci[i + 1 + r] = vr;
-    V[(i + 1 + r) + (int64_t)i * n] = vr;
+    const double vr = r == 0 ? 1.0 : cj[j + 1 + r] * scale;
+    vj[r] = vr;
+    cj[j + 1 + r] = vr;
+    V[(j + 1 + r) + (int64_t)j * n] = vr;
   }
   __syncthreads();
-  if (t == 0.0) return;  // H(i) = I (uniform)
-  // (b) p = tau A' v, A' = A(i+1 .., i+1 ..) with local indices: each
-  //     column c read once, TRB_U columns per wave in flight
+}
+
+// One pass over the trailing block A' = A(j+1 .., j+1 ..) of step j (local
+// indices r, c < m = n - 1 - j): with `upd`, each element first takes step
+// j - 1's update (vp, wp at local index + 1) and is written back; then
+// p = tau_j A' v_j accumulates from the updated value (lane rows the lower
+// part, a wave sum per column its transposed part), and
+// w_j = p - (tau_j / 2)(p . v_j) v_j goes to wj.  Each column of A' is read
+// and written once per step; TRB_U columns per wave in flight.
+__device__ void trb_pass(const TriView& A, int n, int j, bool upd, const double* vp, const double* wp,
+                         const double* vj, double* wj, TrbShared& sh) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m = n - 1 - j;
+  const double t = sh.sc[0];
   double pp[TRB_RW];
 #pragma unroll
   for (int q = 0; q < TRB_RW; ++q) pp[q] = 0.0;
@@ -1425,7 +1451,7 @@ __device__ void trb_step(const TriView& A, int n, int i, TrbShared& sh, double* 
 #pragma unroll
     for (int u = 0; u < TRB_U; ++u) {
       const int c = c0 + u * TRB_NWAVE;
-      const double* col = c < m ? A.col(i + 1 + c) + (i + 1) : nullptr;
+      const double* col = c < m ? A.col(j + 1 + c) + (j + 1) : nullptr;
 #pragma unroll
       for (int q = 0; q < TRB_RW; ++q) {
         const int r = lane + 64 * q;
@@ -1436,13 +1462,25 @@ __device__ void trb_step(const TriView& A, int n, int i, TrbShared& sh, double* 
     for (int u = 0; u < TRB_U; ++u) {
       const int c = c0 + u * TRB_NWAVE;
       if (c >= m) break;  // uniform
-      const double vc = sh.v[c];
+      if (upd) {
+        double* col = A.col(j + 1 + c) + (j + 1);
+        const double vc = vp[c + 1], wc = wp[c + 1];
+#pragma unroll
+        for (int q = 0; q < TRB_RW; ++q) {
+          const int r = lane + 64 * q;
+          if (r >= c && r < m) {
+            a[u][q] = a[u][q] - (vp[r + 1] * wc + wp[r + 1] * vc);
+            col[r] = a[u][q];
+          }
+        }
+      }
+      const double vc = vj[c];
       double s = 0.0;
 #pragma unroll
       for (int q = 0; q < TRB_RW; ++q) {
         const int r = lane + 64 * q;
         pp[q] += a[u][q] * vc;
-        if (r > c && r < m) s += a[u][q] * sh.v[r];
+        if (r > c && r < m) s += a[u][q] * vj[r];
       }
       s = wave_sum_d(s);
       if (lane == 0) sh.tp[c] = s;
@@ -1454,15 +1492,15 @@ __device__ void trb_step(const TriView& A, int n, int i, TrbShared& sh, double* 
     if (r < m) sh.part[wave][r] = pp[q];
   }
   __syncthreads();
-  // (c) p and p . v
+  // p and p . v
   double prod = 0.0;
   if (tid < m) {
     double pr = sh.tp[tid];
 #pragma unroll
     for (int q = 0; q < TRB_NWAVE; ++q) pr += sh.part[q][tid];
     pr *= t;
-    sh.w[tid] = pr;
-    prod = pr * sh.v[tid];
+    wj[tid] = pr;
+    prod = pr * vj[tid];
   }
   prod = wave_sum_d(prod);
   if (lane == 0) sh.red[wave] = prod;
@@ -1470,69 +1508,61 @@ __device__ void trb_step(const TriView& A, int n, int i, TrbShared& sh, double* 
   double dot = 0.0;
 #pragma unroll
   for (int q = 0; q < TRB_NWAVE; ++q) dot += sh.red[q];
-  // (d) w = p - (tau / 2)(p . v) v
-  if (tid < m) sh.w[tid] += -0.5 * t * dot * sh.v[tid];
-  __syncthreads();
-  // (e) A' -= v w^T + w v^T (lower): loads of TRB_U columns, then their stores
-  for (int c0 = wave; c0 < m; c0 += TRB_U * TRB_NWAVE) {
-    double a[TRB_U][TRB_RW];
-#pragma unroll
-    for (int u = 0; u < TRB_U; ++u) {
-      const int c = c0 + u * TRB_NWAVE;
-      const double* col = c < m ? A.col(i + 1 + c) + (i + 1) : nullptr;
-#pragma unroll
-      for (int q = 0; q < TRB_RW; ++q) {
-        const int r = lane + 64 * q;
-        a[u][q] = (c < m && r >= c && r < m) ? col[r] : 0.0;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < TRB_U; ++u) {
-      const int c = c0 + u * TRB_NWAVE;
-      if (c >= m) break;  // uniform
-      double* col = A.col(i + 1 + c) + (i + 1);
-      const double vc = sh.v[c], wc = sh.w[c];
-#pragma unroll
-      for (int q = 0; q < TRB_RW; ++q) {
-        const int r = lane + 64 * q;
-        if (r >= c && r < m) col[r] = a[u][q] - (sh.v[r] * wc + sh.w[r] * vc);
-      }
-    }
-  }
+  // w = p - (tau / 2)(p . v) v
+  if (tid < m) wj[tid] += -0.5 * t * dot * vj[tid];
   __syncthreads();
 }
 
 // n <= 320: the triangle in the workspace W (n (n + 1) / 2 doubles) while the
-// trailing block exceeds TRI_LDS_M, then in LDS.
+// block A(j.., j..) a step touches exceeds TRI_LDS_M + 1 a side, then in LDS.
+// Step j's update of the trailing block is deferred into step j + 1's pass
+// (column j + 1 first, by the wave that forms reflector j + 1).
 __global__ __launch_bounds__(TRI_THREADS) void sytrd_big_kernel(int n, const double* __restrict__ G, int64_t ldg,
                                                                 double* __restrict__ d, double* __restrict__ e,
                                                                 double* __restrict__ tau, double* __restrict__ V,
                                                                 double* __restrict__ W) {
-  __shared__ double PL[(TRI_LDS_M + 1) * (TRI_LDS_M + 2) / 2];
+  constexpr int NL = TRI_LDS_M + 1;
+  __shared__ double PL[NL * (NL + 1) / 2];
   __shared__ TrbShared sh;
   const int tid = threadIdx.x;
   for (int c = 0; c < n; ++c)
     for (int r = c + tid; r < n; r += TRI_THREADS) W[tri_off(n, c) + r - c] = G[r + c * ldg];
   for (int idx = tid; idx < n * n; idx += TRI_THREADS) V[idx] = 0.0;
   __syncthreads();
-  // steps i < i_lds run on W; step i needs A(i.., i..), n - i <= TRI_LDS_M + 1 values a side in LDS
-  const int i_lds = max(0, n - 1 - TRI_LDS_M);
-  int i = 0;
-  const TriView gw{W, n, 0};
-  for (; i < i_lds && i + 2 < n; ++i) trb_step(gw, n, i, sh, d, e, tau, V);
-  const int nl = n - i;
-  for (int c = i; c < n; ++c)
-    for (int r = c + tid; r < n; r += TRI_THREADS) PL[tri_off(nl, c - i) + r - c] = W[tri_off(n, c) + r - c];
-  __syncthreads();
-  const TriView lv{PL, nl, i};
-  for (; i + 2 < n; ++i) trb_step(lv, n, i, sh, d, e, tau, V);
+  TriView A{W, n, 0};
+  bool in_lds = false;
+  int j = 0;
+  for (; j + 2 < n; ++j) {
+    if (!in_lds && n - j <= NL) {
+      const int nl = n - j;
+      for (int c = j; c < n; ++c)
+        for (int r = c + tid; r < n; r += TRI_THREADS) PL[tri_off(nl, c - j) + r - c] = W[tri_off(n, c) + r - c];
+      __syncthreads();
+      A = TriView{PL, nl, j};
+      in_lds = true;
+    }
+    const int b = j & 1;
+    const bool upd = j > 0;
+    trb_reflector(A, n, j, upd, sh.v[b ^ 1], sh.w[b ^ 1], sh.v[b], sh, d, e, tau, V);
+    trb_pass(A, n, j, upd, sh.v[b ^ 1], sh.w[b ^ 1], sh.v[b], sh.w[b], sh);
+  }
   if (tid == 0) {
     if (n >= 2) {
-      d[n - 2] = lv.col(n - 2)[n - 2];
-      e[n - 2] = lv.col(n - 2)[n - 1];
+      double a00 = A.col(n - 2)[n - 2], a10 = A.col(n - 2)[n - 1], a11 = A.col(n - 1)[n - 1];
+      if (j > 0) {  // step j - 1 = n - 3's update of A(n-2.., n-2..), local index = row - (n - 2)
+        const double* vp = sh.v[(j - 1) & 1];
+        const double* wp = sh.w[(j - 1) & 1];
+        a00 = a00 - (vp[0] * wp[0] + wp[0] * vp[0]);
+        a10 = a10 - (vp[1] * wp[0] + wp[1] * vp[0]);
+        a11 = a11 - (vp[1] * wp[1] + wp[1] * vp[1]);
+      }
+      d[n - 2] = a00;
+      e[n - 2] = a10;
       tau[n - 2] = 0.0;
+      d[n - 1] = a11;
+    } else {
+      d[0] = A.col(0)[0];
     }
-    d[n - 1] = lv.col(n - 1)[n - 1];
   }
 }
 

@@ -309,6 +309,7 @@ bool same_opts(const Options& a, const Options& b) {
          x.full_below == y.full_below && x.seed == y.seed && x.mixed_cutoff == y.mixed_cutoff &&
          x.fast_orth == y.fast_orth && x.fast_svd == y.fast_svd && x.gram_guard == y.gram_guard &&
          x.power_iters_max == y.power_iters_max && x.resid_tol == y.resid_tol && x.warm_start == y.warm_start &&
+         x.early_mixed == y.early_mixed && x.late_apps == y.late_apps &&
          a.canonical_edge == b.canonical_edge && a.mixed == b.mixed && a.mixed_back == b.mixed_back &&
          a.mixed_proj == b.mixed_proj && a.fused_absorb == b.fused_absorb && a.paired_moves == b.paired_moves;
 }

@@ -1514,6 +1514,23 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     auto apply_MH = [&](const Tensor& X) {  // M^H X = R2^H (R1^H X)
       return matmul_raw(c, R2, true, true, matmul_raw(c, R1, true, true, X, false, false), false, false);
     };
+    // Applications 0 .. 2 q of the range finder (before B = M^H Q): the first
+    // 2 q + 1 - late_apps with early_mixed products on FP32 engines.
+    int app = 0;
+    const int n_apps = 2 * o.power_iters + 1;
+    auto app_prec = [&]() {
+      const bool early = dt == DType::F32 && o.early_mixed != 0 && app < n_apps - o.late_apps;
+      ++app;
+      return early ? (k::Mixed)o.early_mixed : prec().mixed;
+    };
+    auto range_M = [&](const Tensor& X) {
+      PrecScope ps(app_prec(), prec().back);
+      return apply_M(X);
+    };
+    auto range_MH = [&](const Tensor& X) {
+      PrecScope ps(app_prec(), prec().back);
+      return apply_MH(X);
+    };
     const int fo = o.fast_orth;
     OrthStatsScope oss(o.stats.get());
     // FP64 bases of at most 320 columns: the power iterations' bases by one
@@ -1528,10 +1545,11 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
       Tensor infos = c.zeros(DType::F64, {2 * (2 * o.power_iters + 1)});
       int slot = 0;
       auto info = [&]() { return (int*)(infos.ptr<double>() + 2 * (slot++)); };
-      Q = warm ? *o.warm_in : orth_cholqr1_nosync(c, to_dtype(c, apply_M(Om), sdt), info());
+      Q = warm ? *o.warm_in : orth_cholqr1_nosync(c, to_dtype(c, range_M(Om), sdt), info());
+      if (warm) ++app;
       for (; q < o.power_iters; ++q) {
-        Tensor Z = orth_cholqr1_nosync(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), info());
-        Q = orth_cholqr1_nosync(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), info());
+        Tensor Z = orth_cholqr1_nosync(c, to_dtype(c, range_MH(to_dtype(c, Q, dt)), sdt), info());
+        Q = orth_cholqr1_nosync(c, to_dtype(c, range_M(to_dtype(c, Z, dt)), sdt), info());
       }
       std::vector<double> hi(infos.numel());
       HIP_CHECK(hipMemcpyAsync(hi.data(), infos.data(), infos.bytes(), hipMemcpyDeviceToHost, c.stream));
@@ -1552,10 +1570,12 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
       }
     }
     if (!lazy_ok) {
-      Q = warm ? orth(c, *o.warm_in, fo) : orth(c, to_dtype(c, apply_M(Om), sdt), fo);
+      app = 0;
+      Q = warm ? orth(c, *o.warm_in, fo) : orth(c, to_dtype(c, range_M(Om), sdt), fo);
+      if (warm) ++app;
       for (; q < o.power_iters; ++q) {
-        Tensor Z = orth(c, to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt), fo);
-        Q = orth(c, to_dtype(c, apply_M(to_dtype(c, Z, dt)), sdt), fo);
+        Tensor Z = orth(c, to_dtype(c, range_MH(to_dtype(c, Q, dt)), sdt), fo);
+        Q = orth(c, to_dtype(c, range_M(to_dtype(c, Z, dt)), sdt), fo);
       }
     }
     const int qmax = std::max(o.power_iters, o.power_iters_max);

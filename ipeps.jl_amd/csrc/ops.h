@@ -211,6 +211,14 @@ struct SvdOpts {
   // residual complement_tol.
   int complement_iters = 0;
   double complement_tol = 1e-8;
+  // Range-finder products of FP32 engines: the first applications of M / M^H
+  // (all but the last `late_apps` before B = M^H Q) run with `early_mixed`
+  // products (1, 3 or 6; 0 = the projector's setting).  Subspace iteration
+  // damps an error made in an early application by (s_{k+1} / s_n)^2 per
+  // later application pair, so only the last applications and B need the
+  // projector's accuracy.
+  int early_mixed = 0;
+  int late_apps = 2;
   std::shared_ptr<SvdStats> stats;  // optional counters
   // Warm start: with warm_start set and warm_in holding an m x k basis in the
   // solver dtype, the range finder starts from it (the previous projector's

@@ -15,10 +15,13 @@
 //
 // sytrd_big_kernel keeps the triangle in a global-memory workspace (the same
 // packing, read and written through the L2 of the workgroup's XCD) while the
-// trailing block is larger than TRI_LDS_M, then moves A(i.., i..) into LDS
-// (packed with its own dimension) for the remaining steps.  Its step reads
-// each column of the trailing block once: lane rows accumulate the lower
-// part of p and a wave sum gives the column's transposed part.
+// block a step touches is larger than TRI_LDS_M + 1, then moves A(j.., j..)
+// into LDS (packed with its own dimension) for the remaining steps.  Step
+// j - 1's rank-2 update is deferred into step j: column j is updated by the
+// wave that forms reflector j, and the rest of the trailing block in the
+// same pass that forms p = tau A v, so each element is read and written once
+// per step.  Lane rows accumulate the lower part of p and a wave sum gives
+// each column's transposed part.
 //
 // Q = H(0) H(1) ... H(n-3) acts on rows / columns 1 .. n-1 (Q(0, 0) = 1).
 // Column c of Q is independent of the others: starting from e_c, apply
@@ -37,7 +40,7 @@ namespace k {
 
 constexpr int TRI_MAXN = 160;
 constexpr int TRI_BIG_MAXN = 320;
-constexpr int TRI_LDS_M = 144;  // largest trailing block sytrd_big_kernel reduces from LDS
+constexpr int TRI_LDS_M = 159;  // sytrd_big_kernel works in LDS once A(j.., j..) is at most TRI_LDS_M + 1 a side
 constexpr int TRI_THREADS = 1024;
 
 IPEPS_HD inline int tri_off(int n, int c) { return c * n - c * (c - 1) / 2; }
