@@ -194,23 +194,31 @@ def cpu_baseline_sample(D, chi, seed):
 def batch_c4(pkg, steps=10, workers=8):
     """Uneven 64-instance workload (D in {2,3,4}, chi in {16..96}): mixed and
     FP64 engines advance all instances `steps` rg_steps after one warm-up
-    step; aggregate instance-steps/s and |dE/site| mixed vs FP64."""
+    step, eagerly and in graph mode (each instance's step captured into a HIP
+    graph once its shapes settle, then replayed; captures happen inside the
+    timed steps); aggregate instance-steps/s and |dE/site| mixed vs FP64
+    (eager FP64 as the reference)."""
     import numpy as np
     ins = pkg.c4_instances()
     As, chis, Hs = [x[0] for x in ins], [x[1] for x in ins], [x[2] for x in ins]
     out = {}
     Es = {}
-    for dt in ("mixed", "f64"):
-        b = pkg.BatchedCTM(As, chis, Hs, dtype=dt, workers=workers, options=dict(svd_mode=1))
+    for dt, graph in (("mixed", 1), ("mixed", 0), ("f64", 1), ("f64", 0)):
+        key = dt + ("_graph" if graph else "")
+        b = pkg.BatchedCTM(As, chis, Hs, dtype=dt, workers=workers, options=dict(svd_mode=1, graph=graph))
         b.run(1)
         E, sec = b.run(steps)
-        Es[dt] = E
-        out[dt] = dict(instance_steps_per_s=steps * len(ins) / sec, seconds=sec)
+        Es[key] = E
+        out[key] = dict(instance_steps_per_s=steps * len(ins) / sec, seconds=sec)
+        if graph:
+            out[key]["graph_counters"] = b.stats()
         b.close()
-        log(f"[bench] c4 batch {dt}: {steps} steps x {len(ins)} instances in {sec:.2f} s")
-    d = np.abs(Es["mixed"].real - Es["f64"].real)
-    out.update(instances=len(ins), steps=1 + steps, workers=workers,
-               dE_per_site_median=float(np.median(d)), dE_per_site_max=float(d.max()))
+        log(f"[bench] c4 batch {key}: {steps} steps x {len(ins)} instances in {sec:.2f} s"
+            + (f" {out[key]['graph_counters']}" if graph else ""))
+    for key in ("mixed", "mixed_graph", "f64_graph"):
+        d = np.abs(Es[key].real - Es["f64"].real)
+        out[key].update(dE_per_site_vs_f64_median=float(np.median(d)), dE_per_site_vs_f64_max=float(d.max()))
+    out.update(instances=len(ins), steps=1 + steps, workers=workers)
     return out
 
 

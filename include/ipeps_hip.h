@@ -108,7 +108,14 @@ int ipeps_env_destroy(ipeps_env* env);
  * Cholesky + inverse when k <= 160, on its 2 x 2 block split when k <= 320,
  * else as 1),
  * "fast_svd" (0/1: small SVD from Gram eigenpairs with one-sided Jacobi
- * fallback), "gram_guard" (s_min/s_max below which the Gram route is not
+ * fallback; 2: the Gram eigenpairs from the one-workgroup Jacobi kernel,
+ * k <= 112), "early_mixed" / "late_apps" (MIXED: products of all but the
+ * last late_apps applications of M before B = M^H Q at early_mixed = 1, 3
+ * or 6 instead of the projector's setting; 0 = off), "graph_safe" (0/1: each
+ * rg_step in the form a captured graph runs -- no host round trips,
+ * truncation at the largest rank the cutoff rule admits, Jacobi small SVD --
+ * with its deferred checks validated after the step, an error if one fails;
+ * needs paired_moves=0), "gram_guard" (s_min/s_max below which the Gram route is not
  * taken, default 1e-4), "power_iters_max" and "resid_tol" (adaptive power
  * iterations: after "power_iters", while the a-posteriori residual
  * max_i ||M v_i - s_i u_i|| / s_1 of the kept triplets exceeds resid_tol and
@@ -303,7 +310,9 @@ int ipeps_svd(ipeps_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A, vo
  * (B = M^H Q; src/utility/autodiff.jl:22-25 is svd1, which this replaces for
  * tall operands).  method 0: one-sided Jacobi (rocSOLVER gesvdj); 1: Gram
  * eigenpairs B^H B with one Newton-Schulz orthonormalisation of U, taken
- * only when S_min/S_max >= guard, else method 0.  *method_used reports it. */
+ * only when S_min/S_max >= guard, else method 0; 2: as 1 with the Gram
+ * eigenpairs from the one-workgroup Jacobi kernel (FP64, n <= 112: the form
+ * a captured graph holds).  *method_used reports it. */
 int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double guard, int64_t m, int64_t n, const void* A,
                    void* U, double* S, void* V, int* method_used);
 /* Orthonormal basis Q (m x n) of the columns of a tall Y (m >= n): the range
@@ -339,6 +348,17 @@ int ipeps_tm_dims(ipeps_env* env, int direction, int64_t* rows, int64_t* cols);
 int ipeps_tm_apply(ipeps_env* env, int direction, int64_t nvec, const void* X, void* Y);
 int ipeps_tm_dense(ipeps_env* env, int direction, void* TM_out);
 
+/* `steps` rg_steps (src/ctm/ctmrg.jl:57-64) of the environment as replays
+ * of a HIP graph: the step is captured on the first call (shapes must be
+ * stable, chi saturated; paired_moves=0; randomized projector with
+ * chi + oversample <= 112) in the graph-safe form that "graph_safe" names,
+ * and each replay's deferred checks are validated; a replay that fails them
+ * is undone and the step run eagerly.  The graph is dropped when an option,
+ * A or a tensor is set, and recaptured when the environment's buffers have
+ * moved (an eager step).  out (may be NULL) = [captures, replays kept,
+ * replays undone, eager steps].  No counterpart in the reference. */
+int ipeps_env_run_graph(ipeps_env* env, int64_t chi, int steps, int64_t out[4]);
+
 /* Batched independent environments (many instances of CTMTensors +
  * rg_step, src/ctm/ctm_tensor.jl:63-69 and src/ctm/ctmrg.jl:57-64; the
  * reference runs such instances one after another, e.g. the per-basis-vector
@@ -348,7 +368,16 @@ int ipeps_tm_dense(ipeps_env* env, int direction, void* TM_out);
  * d^2); init_ctm1 runs for each.  `workers` contexts (one HIP stream and host
  * thread each, 1..64) share the instances by a longest-processing-time
  * assignment on the per-step cost.  options: "key=value;..." as
- * ipeps_env_set_option, applied to every instance. */
+ * ipeps_env_set_option, applied to every instance; with "graph=1" each
+ * instance's rg_step is captured into a HIP graph once its shapes are stable
+ * (chi saturated) and replayed from then on: the step in graph-safe form (no
+ * host round trip; zero fills and copies as kernels; the one-workgroup
+ * Jacobi eigensolver for the projector's small SVD; the truncation at the
+ * largest rank the cutoff rule admits) with its checks (factorisation infos,
+ * the cutoff rule replayed on the spectrum) validated after each replay; a
+ * replay that fails them is undone and run eagerly.  Needs paired_moves=0
+ * (the batch default) and the randomized projector with chi + oversample <=
+ * 128; other instances stay eager. */
 int ipeps_batch_create(int device, int n, int dtype, const int64_t* dims, const void* const* A, const int64_t* chi,
                        const double* const* H1, const double* const* H2, const int64_t* hdim, int workers,
                        const char* options, ipeps_batch** out);
@@ -360,6 +389,10 @@ int ipeps_batch_run(ipeps_batch* b, int steps, double* E_out, double* seconds_ou
 /* Copy out tensor `which` (as ipeps_env_get_tensor) of instance `inst`;
  * dims (room for 5) / nd receive its shape; host may be NULL. */
 int ipeps_batch_get_tensor(ipeps_batch* b, int inst, int which, int64_t* dims, int* nd, void* host);
+/* Counters of graph mode since creation: out = [steps captured, replays
+ * kept, replays undone (and run eagerly), eager steps].  No counterpart in
+ * the reference (its instances run eagerly). */
+int ipeps_batch_stats(ipeps_batch* b, int64_t out[4]);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,8 @@ SIGNATURES = {
     "ipeps_batch_destroy": (C.c_int, [_P]),
     "ipeps_batch_run": (C.c_int, [_P, C.c_int, _DP, _DP]),
     "ipeps_batch_get_tensor": (C.c_int, [_P, C.c_int, C.c_int, _I64P, _IP, _P]),
+    "ipeps_batch_stats": (C.c_int, [_P, _I64P]),
+    "ipeps_env_run_graph": (C.c_int, [_P, C.c_int64, C.c_int, _I64P]),
 }
 
 _lib = None

@@ -83,6 +83,12 @@ class BatchedCTM:
                                     sec.ctypes.data_as(C.POINTER(C.c_double))))
         return E[0::2] + 1j * E[1::2], float(sec[0])
 
+    def stats(self):
+        """Graph-mode counters: dict(captured, replayed, undone, eager) steps."""
+        out = (C.c_int64 * 4)()
+        check(lib().ipeps_batch_stats(self.h, out))
+        return dict(captured=out[0], replayed=out[1], undone=out[2], eager=out[3])
+
     def tensor(self, inst, which):
         dims = (C.c_int64 * 5)()
         nd = C.c_int()

@@ -1,5 +1,6 @@
 // extern "C" entry points (declared in include/ipeps_hip.h).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -25,6 +26,9 @@ struct ipeps_env {
   std::shared_ptr<Ctx> hold;  // declared first: outlives the tensors in env
   int dtype = IPEPS_F64;
   ctm::Env env;
+  // captured rg_step (ipeps_env_run_graph); dropped whenever the options, A or
+  // a tensor are set, and recaptured when the environment's buffers moved
+  std::unique_ptr<ctm::StepGraph> graph;
 };
 
 // Independent environments advanced together by a pool of workers.  Each
@@ -41,16 +45,31 @@ struct ipeps_batch {
     Tensor H1, H2;
     double E[2] = {0, 0};
     int64_t steps = 0;
+    std::unique_ptr<ctm::StepGraph> graph;  // declared after env: destroyed first
+    int fails = 0;                          // consecutive replays whose checks failed
+    bool no_graph = false;                  // capture refused or replays kept failing
   };
   int device = 0;
   int dtype = IPEPS_F64;
   std::vector<std::shared_ptr<Ctx>> workers;  // declared before insts: outlives their tensors
   std::vector<Inst> insts;
+  bool graphs = false;
+  // captures, replays kept, replays undone (then run eagerly), eager steps
+  std::atomic<long> n_capture{0}, n_replay{0}, n_undo{0}, n_eager{0};
 };
 
 namespace {
 
 thread_local std::string g_err;
+
+std::vector<int64_t> dims_of_graph(const ctm::StepGraph& g) {
+  std::vector<int64_t> d;
+  for (int i = 0; i < 4; ++i) {
+    d.insert(d.end(), g.C[i].dims.begin(), g.C[i].dims.end());
+    d.insert(d.end(), g.E[i].dims.begin(), g.E[i].dims.end());
+  }
+  return d;
+}
 
 template <typename F>
 int guard(F&& f) {
@@ -133,6 +152,8 @@ void set_opt(ctm::Options& o, const std::string& key, double v) {
   else if (key == "checkpoint") o.checkpoint = v != 0.0;
   else if (key == "fused_absorb") o.fused_absorb = v != 0.0;
   else if (key == "paired_moves") o.paired_moves = v != 0.0;
+  else if (key == "graph_safe") o.graph_safe = v != 0.0;
+  else if (key == "graph") o.graph = v != 0.0;
   else if (key == "energy_mixed") {
     int m = (int)v;
     IPEPS_CHECK(m == 1 || m == 3 || m == 6, "energy_mixed must be 1, 3 or 6");
@@ -313,6 +334,7 @@ int ipeps_env_destroy(ipeps_env* env) {
 int ipeps_env_set_option(ipeps_env* env, const char* key, double value) {
   return guard([&] {
     IPEPS_CHECK(env && key, "null argument");
+    env->graph.reset();
     if (std::string(key) == "draws") {
       IPEPS_CHECK(value >= 0.0, "draws must be >= 0");
       env->env.draws = (uint64_t)value;
@@ -399,14 +421,53 @@ int ipeps_env_set_tensor(ipeps_env* env, int which, int nd, const int64_t* dims,
     IPEPS_CHECK(env && dims && nd > 0 && nd <= 5, "bad arguments");
     std::vector<int64_t> d(dims, dims + nd);
     Ctx& c = *env->hold;
+    env->graph.reset();
     Tensor t = upload(c, host_dt(env->dtype), dev_dt(env->dtype), d, host);
     env_slot(env->env, which) = t;
     if (which == 8) env->env.Ad = op::conj(c, t);
   });
 }
 
+int ipeps_env_run_graph(ipeps_env* env, int64_t chi, int steps, int64_t out[4]) {
+  return guard([&] {
+    IPEPS_CHECK(env && chi > 0 && steps >= 0, "bad arguments");
+    Ctx& c = *env->hold;
+    HIP_CHECK(hipSetDevice(c.device));
+    TapeScope ts(nullptr);
+    auto& e = env->env;
+    int64_t st[4] = {0, 0, 0, 0};
+    auto current = [&]() {
+      const auto* g = env->graph.get();
+      if (!g || g->chi != chi) return false;
+      for (int i = 0; i < 4; ++i)
+        if (g->C[i].data() != e.C[i].data() || g->E[i].data() != e.E[i].data()) return false;
+      return true;
+    };
+    for (int s = 0; s < steps; ++s) {
+      if (!current()) {
+        env->graph.reset();
+        env->graph = ctm::capture_step(c, e, chi);
+        ++st[0];
+      }
+      ctm::replay_step(*env->graph, e);
+      c.sync();
+      if (ctm::finish_step(*env->graph, e, nullptr)) {
+        ++st[1];
+        continue;
+      }
+      ++st[2];
+      ctm::rg_step(c, e, chi);
+      ++st[3];
+      ctm::adopt_step(*env->graph, e);
+    }
+    if (out)
+      for (int i = 0; i < 4; ++i) out[i] = st[i];
+  });
+}
+
 int ipeps_env_set_A(ipeps_env* env, const void* A) {
   return guard([&] {
+    if (env) env->graph.reset();
     IPEPS_CHECK(env, "null env");
     Ctx& c = *env->hold;
     Tensor t = upload(c, host_dt(env->dtype), dev_dt(env->dtype), env->env.A.dims, A);
@@ -765,7 +826,8 @@ int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double gram_guard, int
   return guard([&] {
     IPEPS_CHECK(ctx && m >= n && n > 0 && A && U && S && V, "bad arguments (m >= n > 0 required)");
     IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_C128, "svd_thin: F64 or C128");
-    IPEPS_CHECK(method == 0 || method == 1, "svd_thin: method must be 0 or 1");
+    IPEPS_CHECK(method >= 0 && method <= 2, "svd_thin: method must be 0, 1 or 2");
+    IPEPS_CHECK(method != 2 || (dtype == IPEPS_F64 && n <= k::JAC_MAXN_HOST), "svd_thin: method 2 needs F64, n <= 112");
     Ctx& c = *ctx->c;
     DType dt = host_dt(dtype);
     Tensor tA = upload(c, dt, dt, {m, n}, A);
@@ -881,9 +943,11 @@ int ipeps_batch_create(int device, int n, int dtype, const int64_t* dims, const 
     opt.mixed = k::Mixed::BF16X3;
     opt.paired_moves = false;  // the worker pool already runs instances concurrently
     parse_opts(opt, options);
+    IPEPS_CHECK(!opt.graph || !opt.paired_moves, "graph=1 needs paired_moves=0");
     std::unique_ptr<ipeps_batch> b(new ipeps_batch());
     b->device = device;
     b->dtype = dtype;
+    b->graphs = opt.graph;
     b->insts.resize(n);
     // longest-processing-time assignment by the per-step cost estimate
     // chi^3 D^4 + (chi D^2)^2 (chi + 16) of the corner and projector terms
@@ -940,12 +1004,61 @@ int ipeps_batch_run(ipeps_batch* b, int steps, double* E_out, double* seconds_ou
         HIP_CHECK(hipSetDevice(b->device));
         Ctx& c = *b->workers[w];
         TapeScope ts(nullptr);
-        for (int s = 0; s < steps; ++s)
+        auto dims_of = [](const ctm::Env& e) {
+          std::vector<int64_t> d;
+          for (int i = 0; i < 4; ++i) {
+            d.insert(d.end(), e.C[i].dims.begin(), e.C[i].dims.end());
+            d.insert(d.end(), e.E[i].dims.begin(), e.E[i].dims.end());
+          }
+          return d;
+        };
+        for (int s = 0; s < steps; ++s) {
+          // captured steps are enqueued first (no host round trip), then the
+          // eager ones; an eager step with unchanged shapes captures the
+          // instance's step for the following rounds
+          std::vector<ipeps_batch::Inst*> replayed;
           for (auto& in : b->insts)
-            if (in.worker == w) {
+            if (in.worker == w && in.graph) {
+              ctm::replay_step(*in.graph, in.env);
+              replayed.push_back(&in);
+            }
+          for (auto& in : b->insts)
+            if (in.worker == w && !in.graph) {
+              const auto before = dims_of(in.env);
               ctm::rg_step(c, in.env, in.chi);
               ++in.steps;
+              ++b->n_eager;
+              if (b->graphs && !in.no_graph && dims_of(in.env) == before) {
+                try {
+                  in.graph = ctm::capture_step(c, in.env, in.chi);
+                  ++b->n_capture;
+                } catch (const Error&) {
+                  in.no_graph = true;  // this instance's step does not fit a graph: eager from here on
+                }
+              }
             }
+          c.sync();
+          for (auto* in : replayed) {
+            if (ctm::finish_step(*in->graph, in->env, nullptr)) {
+              ++in->steps;
+              ++b->n_replay;
+              in->fails = 0;
+              continue;
+            }
+            // checks failed (the cutoff rule would keep another rank, a
+            // factorisation failed): the step again, eagerly
+            ++b->n_undo;
+            ctm::rg_step(c, in->env, in->chi);
+            ++in->steps;
+            ++b->n_eager;
+            if (++in->fails >= 3 || dims_of(in->env) != dims_of_graph(*in->graph)) {
+              in->graph.reset();
+              in->no_graph = true;
+            } else {
+              ctm::adopt_step(*in->graph, in->env);
+            }
+          }
+        }
         for (auto& in : b->insts)
           if (in.worker == w) {
             Tensor e = ctm::energy(c, in.env, in.H1, in.H2);
@@ -972,6 +1085,16 @@ int ipeps_batch_run(ipeps_batch* b, int steps, double* E_out, double* seconds_ou
         E_out[2 * i] = b->insts[i].E[0];
         E_out[2 * i + 1] = b->insts[i].E[1];
       }
+  });
+}
+
+int ipeps_batch_stats(ipeps_batch* b, int64_t out[4]) {
+  return guard([&] {
+    IPEPS_CHECK(b && out, "bad arguments");
+    out[0] = b->n_capture;
+    out[1] = b->n_replay;
+    out[2] = b->n_undo;
+    out[3] = b->n_eager;
   });
 }
 

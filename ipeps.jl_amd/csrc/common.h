@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <memory>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -82,6 +83,14 @@ class Pool {
   // joined; holds nest.
   void hold();
   void unhold();
+  // Graph capture: between begin_graph() and end_graph() every block handed
+  // out joins the graph (a block released during the capture is reused only
+  // by the capture, in stream order); end_graph() returns the graph's blocks,
+  // which stay out of the free lists until release_graph(), since a captured
+  // graph replays into the same addresses.
+  void begin_graph();
+  std::vector<std::pair<size_t, void*>> end_graph();
+  void release_graph(const std::vector<std::pair<size_t, void*>>& blocks);
   size_t live_bytes() const { return live_; }
   size_t peak_bytes() const { return peak_; }
 
@@ -100,6 +109,10 @@ class Pool {
   std::multimap<size_t, void*> free_;
   std::vector<std::pair<size_t, void*>> parked_;  // released under hold()
   int holds_ = 0;
+  bool capturing_ = false;
+  std::multimap<size_t, void*> cap_free_;               // released during the capture
+  std::vector<std::pair<size_t, void*>> cap_blocks_;    // handed out during the capture
+  std::set<void*> owned_;                               // blocks of live graphs
   size_t live_ = 0, peak_ = 0, cached_ = 0;
 };
 
@@ -162,6 +175,23 @@ struct Ctx {
   std::map<std::string, KStat> kstats;
   void resolve_timing();
 
+  // Graph-safe mode (set while a step is captured into a HIP graph, or
+  // eagerly to run exactly what such a graph runs): no host round trips, zero
+  // fills and device copies as kernels, truncation at the largest admissible
+  // rank, and every check the host would make deferred: the values it needs
+  // are copied into `checks` (FP64, from checks_pos on) and described by a
+  // CheckRec, for validate_checks() after the step.
+  struct CheckRec {
+    int kind;            // CheckKind
+    int64_t off, len;    // doubles in `checks`
+    int64_t a = 0, b = 0;
+    double x = 0.0;
+  };
+  bool graph_safe = false;
+  Tensor checks;  // from this context's pool: ctx_destroy releases it before the pool
+  int64_t checks_pos = 0;
+  std::vector<CheckRec> check_recs;
+
   // Second lane for the independent moves of a CTMRG step (own stream,
   // allocator and solver handle), created on first use.
   std::unique_ptr<Ctx> lane;
@@ -173,6 +203,28 @@ struct Ctx {
 
 Ctx* ctx_create(int device);
 void ctx_destroy(Ctx* c);
+
+// Deferred checks of graph-safe mode.
+enum CheckKind : int {
+  CHECK_INFO_PAIRS = 1,  // doubles each holding two ints (Cholesky infos): all zero
+  CHECK_CHOLQR2 = 2,     // [info pair, info pair, |G2 - I|_max, -, -]: infos zero, |G2 - I| <= 1/2
+  CHECK_INFO = 3,        // one double holding an int: zero
+  CHECK_CUTOFF = 4,      // spectrum (len values): the cutoff rule with chi = a, rel = x keeps b
+  CHECK_GUARD = 5,       // spectrum, descending: s_0 > 0 and s_last >= x s_0
+};
+// Starts a graph-safe step on c: checks buffer of `cap` doubles, no records.
+void checks_begin(Ctx& c, int64_t cap);
+// Appends `len` FP64 device values at src as a record of `kind`.
+void record_check(Ctx& c, int kind, const void* src, int64_t len, int64_t a = 0, int64_t b = 0, double x = 0.0);
+// Validates host copies of the checks buffer against the records; on failure
+// `why` says which record failed.
+bool validate_checks(const std::vector<Ctx::CheckRec>& recs, const double* host, std::string* why);
+// Device copies and zero fills that become kernels in graph-safe mode
+// (hipMemcpyAsync / hipMemsetAsync otherwise).
+void dev_copy(Ctx& c, void* dst, const void* src, size_t bytes);
+void dev_copy2d(Ctx& c, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows);
+void dev_zero(Ctx& c, void* p, size_t bytes);
+void dev_zero2d(Ctx& c, void* p, size_t pitch, size_t width, size_t rows);
 
 // RAII bracket of one launch for the timing instrumentation.
 struct TimedLaunch {

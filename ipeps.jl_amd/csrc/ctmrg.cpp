@@ -41,8 +41,16 @@ Proj get_projector(Ctx& c, const Env& e, const Tensor& R1, const Tensor& R2, int
     so.warm_in = &e.warm[dir];
     so.warm_out = &e.warm[dir];
   }
+  if (c.graph_safe && e.draws_dev.defined()) {
+    // captured: the seed comes from the device counter at replay time
+    so.draws_dev = (const uint64_t*)e.draws_dev.data();
+    so.seed_base = e.opt.svd.seed;
+    so.draw_off = draw - e.draws;
+  }
+  if (c.graph_safe && dir < 4) so.fixed_n = e.kept[dir];
   op::PrecScope ps(e.opt.mixed_proj ? (k::Mixed)e.opt.mixed_proj : op::prec().mixed, op::prec().back);
   auto r = op::svd_trunc(c, R1, R2, chi, so);
+  if (dir < 4) e.kept[dir] = r.n;
   Tensor isq = op::inv_sqrt(c, r.S);
   Proj p;
   p.P1 = op::scale_vec(c, op::matmul(c, R2, r.V), isq, false);
@@ -389,7 +397,179 @@ void init_ctm1(Ctx& c, Env& e) {
 }
 
 
+static std::vector<double> rg_step_impl(Ctx& c, Env& e, int64_t chi);
+
+// Graph-safe mode on c for a scope (timing off: a capture records no events).
+struct GraphSafeScope {
+  Ctx& c;
+  bool timing;
+  explicit GraphSafeScope(Ctx& cc, int64_t cap) : c(cc), timing(cc.timing) {
+    checks_begin(c, cap);
+    c.timing = false;
+    c.graph_safe = true;
+  }
+  ~GraphSafeScope() {
+    c.graph_safe = false;
+    c.timing = timing;
+  }
+};
+
+constexpr int64_t STEP_CHECKS_CAP = 1 << 14;
+
 std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi) {
+  if (!e.opt.graph_safe || c.graph_safe) return rg_step_impl(c, e, chi);
+  IPEPS_CHECK(!e.opt.paired_moves, "graph_safe needs paired_moves=0");
+  std::vector<double> s;
+  std::vector<Ctx::CheckRec> recs;
+  std::vector<double> h;
+  {
+    GraphSafeScope gs(c, STEP_CHECKS_CAP);
+    s = rg_step_impl(c, e, chi);
+    recs = c.check_recs;
+    h.resize(c.checks_pos);
+    if (!h.empty()) HIP_CHECK(hipMemcpyAsync(h.data(), c.checks.data(), h.size() * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  }
+  std::string why;
+  if (!validate_checks(recs, h.data(), &why)) throw Error(IPEPS_ERR_STATE, "graph-safe step: " + why);
+  return s;
+}
+
+StepGraph::~StepGraph() {
+  if (c) (void)hipStreamSynchronize(c->stream);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (c && !blocks.empty()) c->pool->release_graph(blocks);
+  if (host_checks) (void)hipHostFree(host_checks);
+}
+
+std::unique_ptr<StepGraph> capture_step(Ctx& c, Env& e, int64_t chi) {
+  IPEPS_CHECK(!e.opt.paired_moves, "step capture needs paired_moves=0");
+  IPEPS_CHECK(e.opt.svd.mode == 1 && !e.opt.svd.warm_start, "step capture needs the randomized projector, no warm start");
+  IPEPS_CHECK(!op::recording(), "step capture: no tape");
+  {
+    // the step once, eagerly, in graph-safe form on a copy of the handles:
+    // a path that would synchronise the stream (the full SVD, host checks)
+    // throws here rather than inside the capture, the shapes must come back
+    // unchanged and the deferred checks must pass
+    Env probe = e;
+    std::vector<Ctx::CheckRec> recs;
+    std::vector<double> h;
+    {
+      GraphSafeScope gs(c, STEP_CHECKS_CAP);
+      rg_step_impl(c, probe, chi);
+      for (int i = 0; i < 4; ++i)
+        if (probe.C[i].dims != e.C[i].dims || probe.E[i].dims != e.E[i].dims)
+          throw Error(IPEPS_ERR_STATE, "step capture: shapes not stable");
+      recs = c.check_recs;
+      h.resize(c.checks_pos);
+      if (!h.empty()) HIP_CHECK(hipMemcpyAsync(h.data(), c.checks.data(), h.size() * 8, hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+    }
+    std::string why;
+    if (!validate_checks(recs, h.data(), &why)) throw Error(IPEPS_ERR_STATE, "step capture: " + why);
+  }
+  std::unique_ptr<StepGraph> g(new StepGraph());
+  g->c = &c;
+  g->chi = chi;
+  if (!e.draws_dev.defined()) e.draws_dev = c.empty(DType::F64, {1});
+  HIP_CHECK(hipMemcpyAsync(e.draws_dev.data(), &e.draws, 8, hipMemcpyHostToDevice, c.stream));
+  for (int i = 0; i < 4; ++i) {
+    g->bakC[i] = c.empty(e.C[i].dt, e.C[i].dims);
+    g->bakE[i] = c.empty(e.E[i].dt, e.E[i].dims);
+  }
+  c.sync();
+  const std::array<Tensor, 4> inC = e.C, inE = e.E;
+  g->C = inC;
+  g->E = inE;
+  const uint64_t d0 = e.draws;
+  c.pool->begin_graph();
+  bool capturing = false;
+  try {
+    GraphSafeScope gs(c, STEP_CHECKS_CAP);
+    HIP_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+    capturing = true;
+    for (int i = 0; i < 4; ++i) {
+      dev_copy(c, g->bakC[i].data(), inC[i].data(), inC[i].bytes());
+      dev_copy(c, g->bakE[i].data(), inE[i].data(), inE[i].bytes());
+    }
+    rg_step_impl(c, e, chi);
+    for (int i = 0; i < 4; ++i) {
+      IPEPS_CHECK(e.C[i].dims == inC[i].dims && e.E[i].dims == inE[i].dims, "step capture: shapes not stable");
+      dev_copy(c, inC[i].data(), e.C[i].data(), inC[i].bytes());
+      dev_copy(c, inE[i].data(), e.E[i].data(), inE[i].bytes());
+    }
+    k::counter_add(c, (uint64_t*)e.draws_dev.data(), 4);
+    e.C = inC;
+    e.E = inE;
+    e.draws = d0;
+    HIP_CHECK(hipStreamEndCapture(c.stream, &g->graph));
+    capturing = false;
+    g->checks = c.checks;
+    g->recs = c.check_recs;
+    g->nchecks = c.checks_pos;
+    c.checks = Tensor();
+  } catch (...) {
+    if (capturing) {
+      hipGraph_t dead = nullptr;
+      (void)hipStreamEndCapture(c.stream, &dead);
+      if (dead) (void)hipGraphDestroy(dead);
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(c.stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+        dead = nullptr;
+        (void)hipStreamEndCapture(c.stream, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+      }
+    }
+    (void)hipGetLastError();
+    e.C = inC;
+    e.E = inE;
+    e.draws = d0;
+    c.pool->release_graph(c.pool->end_graph());
+    throw;
+  }
+  g->blocks = c.pool->end_graph();
+  HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  HIP_CHECK(hipHostMalloc((void**)&g->host_checks, std::max<int64_t>(1, g->nchecks) * 8, hipHostMallocDefault));
+  return g;
+}
+
+void replay_step(StepGraph& g, Env& e) {
+  Ctx& c = *g.c;
+  HIP_CHECK(hipGraphLaunch(g.exec, c.stream));
+  if (g.nchecks)
+    HIP_CHECK(hipMemcpyAsync(g.host_checks, g.checks.data(), g.nchecks * 8, hipMemcpyDeviceToHost, c.stream));
+  e.draws += 4;
+}
+
+bool finish_step(StepGraph& g, Env& e, std::string* why) {
+  if (validate_checks(g.recs, g.host_checks, why)) return true;
+  Ctx& c = *g.c;
+  for (int i = 0; i < 4; ++i) {
+    dev_copy(c, e.C[i].data(), g.bakC[i].data(), e.C[i].bytes());
+    dev_copy(c, e.E[i].data(), g.bakE[i].data(), e.E[i].bytes());
+  }
+  e.draws -= 4;
+  HIP_CHECK(hipMemcpyAsync(e.draws_dev.data(), &e.draws, 8, hipMemcpyHostToDevice, c.stream));
+  c.sync();
+  return false;
+}
+
+void adopt_step(StepGraph& g, Env& e) {
+  Ctx& c = *g.c;
+  for (int i = 0; i < 4; ++i)
+    IPEPS_CHECK(e.C[i].dims == g.C[i].dims && e.E[i].dims == g.E[i].dims, "adopt_step: shapes changed");
+  for (int i = 0; i < 4; ++i) {
+    if (e.C[i].data() != g.C[i].data()) dev_copy(c, g.C[i].data(), e.C[i].data(), e.C[i].bytes());
+    if (e.E[i].data() != g.E[i].data()) dev_copy(c, g.E[i].data(), e.E[i].data(), e.E[i].bytes());
+    e.C[i] = g.C[i];
+    e.E[i] = g.E[i];
+  }
+  HIP_CHECK(hipMemcpyAsync(e.draws_dev.data(), &e.draws, 8, hipMemcpyHostToDevice, c.stream));
+  c.sync();
+}
+
+static std::vector<double> rg_step_impl(Ctx& c, Env& e, int64_t chi) {
   op::prec().mixed = e.opt.mixed;
   op::prec().back = e.opt.mixed_back ? (k::Mixed)e.opt.mixed_back : e.opt.mixed;
   op::prec().fused_absorb = e.opt.fused_absorb;

@@ -26,6 +26,12 @@ struct Options {
   bool checkpoint = false;     // recompute each step in the reverse sweep instead of keeping its tape
   bool fused_absorb = true;    // fused FP32-MFMA site absorption in the corners (mixed engine, 4 <= D <= 6)
   bool paired_moves = true;    // left || right and top || bottom on two streams / host threads
+  // rg_step in graph-safe mode (what a captured step runs), its deferred
+  // checks validated after each step (an error if one fails)
+  bool graph_safe = false;
+  // batches only: each instance's rg_step captured into a HIP graph once its
+  // shapes are stable and replayed (see StepGraph)
+  bool graph = false;
 };
 
 // Plain ground-state environment (src/ctm/ctm_tensor.jl:3-18).  Storage dtype
@@ -46,6 +52,13 @@ struct Env {
   // last left Ritz basis (m x k, FP64 / complex FP64), indexed by direction
   // (left, right, top, bottom); part of the environment's state.
   mutable std::array<Tensor, 4> warm;
+  // Device copy of `draws` (one uint64 in an F64 tensor) while a captured
+  // step is attached: the graph forms its test-matrix seeds from it.
+  mutable Tensor draws_dev;
+  // Rank each direction's projector kept in the last step (left, right, top,
+  // bottom): the rank a graph-safe step truncates at (the cutoff rule is
+  // replayed on its spectrum afterwards).
+  mutable std::array<int64_t, 4> kept{{0, 0, 0, 0}};
 };
 
 // Boundary initialisation C_i, E_i from A and Ad (src/ctm/ctm_tensor.jl:97-130).
@@ -58,6 +71,39 @@ void projector(Ctx& c, const Env& e, int dir, int64_t chi, Tensor& P1, Tensor& P
 // One CTMRG step left -> right -> top -> bottom (src/ctm/ctmrg.jl:57-64).
 // Returns the left move's truncated spectrum.
 std::vector<double> rg_step(Ctx& c, Env& e, int64_t chi);
+
+// A captured rg_step of one environment (HIP graph).  The graph reads the
+// environment's C / E buffers, saves them into backups, runs the step in
+// graph-safe mode and writes the new C / E back into the same buffers, so a
+// replay advances the environment in place; the step's deferred checks land
+// in `checks` and are copied to pinned host memory after each replay.  A
+// replay whose checks fail is undone from the backups, and the step is then
+// run eagerly (the caller's decision).  Capture needs stable shapes (chi
+// saturated), unpaired moves and the randomized projector with k <= 112.
+struct StepGraph {
+  Ctx* c = nullptr;
+  int64_t chi = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<std::pair<size_t, void*>> blocks;  // pool blocks the graph replays into
+  std::array<Tensor, 4> C, E;        // the environment buffers the graph reads and rewrites
+  std::array<Tensor, 4> bakC, bakE;
+  Tensor checks;
+  std::vector<Ctx::CheckRec> recs;
+  double* host_checks = nullptr;  // pinned
+  int64_t nchecks = 0;
+  ~StepGraph();
+};
+// Captures one step of e (nothing runs; e is unchanged).
+std::unique_ptr<StepGraph> capture_step(Ctx& c, Env& e, int64_t chi);
+// Enqueues one replay (e.draws advances).
+void replay_step(StepGraph& g, Env& e);
+// After the stream has finished a replay: validates its checks; on failure
+// restores the pre-step environment, returns false and says why.
+bool finish_step(StepGraph& g, Env& e, std::string* why);
+// After an eager step of e with a graph attached: the new C / E go back into
+// the graph's buffers (dims must match) and the device draw counter follows.
+void adopt_step(StepGraph& g, Env& e);
 
 // Two-site density matrices rho_h, rho_v (d^2 x d^2)
 // (src/graph/evaluation.jl:124-248, ground-state branch).

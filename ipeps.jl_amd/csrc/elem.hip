@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.h"
 #include "permute_impl.h"
@@ -248,6 +249,54 @@ template <>
 __global__ void randn_kernel<double2>(int64_t n, uint64_t seed, double2* out) {
   for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB)
     out[i] = make_double2(normal_at(seed, 2 * i), normal_at(seed, 2 * i + 1));
+}
+
+// seed of draw i of the stream keyed by base (the host's draw_seed)
+__device__ inline uint64_t draw_seed_dev(uint64_t base, uint64_t i) {
+  uint64_t z = base + (i + 1) * 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+template <typename T>
+__global__ void randn_drawn_kernel(int64_t n, uint64_t base, const uint64_t* __restrict__ draws, uint64_t off,
+                                   T* out) {
+  const uint64_t seed = draw_seed_dev(base, draws[0] + off);
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TPB) {
+    if constexpr (std::is_same<T, double2>::value) out[i] = make_double2(normal_at(seed, 2 * i), normal_at(seed, 2 * i + 1));
+    else out[i] = (T)normal_at(seed, i);
+  }
+}
+__global__ void counter_add_kernel(uint64_t* c, uint64_t add) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) c[0] += add;
+}
+
+// byte rows: 16-byte words when every address and length is a multiple of 16
+__global__ void copy_rows16_kernel(uint4* __restrict__ dst, size_t dpitch, const uint4* __restrict__ src, size_t spitch,
+                                   size_t wwords, size_t rows) {
+  const size_t n = wwords * rows;
+  for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
+    const size_t r = i / wwords, w = i % wwords;
+    dst[r * dpitch + w] = src[r * spitch + w];
+  }
+}
+__global__ void copy_rows1_kernel(unsigned char* __restrict__ dst, size_t dpitch, const unsigned char* __restrict__ src,
+                                  size_t spitch, size_t width, size_t rows) {
+  const size_t n = width * rows;
+  for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) {
+    const size_t r = i / width, w = i % width;
+    dst[r * dpitch + w] = src[r * spitch + w];
+  }
+}
+__global__ void zero_rows16_kernel(uint4* __restrict__ p, size_t pitch, size_t wwords, size_t rows) {
+  const size_t n = wwords * rows;
+  for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB)
+    p[(i / wwords) * pitch + i % wwords] = make_uint4(0u, 0u, 0u, 0u);
+}
+__global__ void zero_rows1_kernel(unsigned char* __restrict__ p, size_t pitch, size_t width, size_t rows) {
+  const size_t n = width * rows;
+  for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB)
+    p[(i / width) * pitch + i % width] = 0;
 }
 
 // ---- reductions ----------------------------------------------------------
@@ -728,7 +777,7 @@ void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::v
   const int cj = conj && dt == DType::C128;
   if (od.size() <= 1) {  // plain copy
     if (cj) conj_copy(c, dt, n, in, out);
-    else HIP_CHECK(hipMemcpyAsync(out, in, (size_t)n * dtype_size(dt), hipMemcpyDeviceToDevice, c.stream));
+    else dev_copy(c, out, in, (size_t)n * dtype_size(dt));
     return;
   }
   if (n < ((int64_t)1 << 31)) {
@@ -785,9 +834,7 @@ IPEPS_CHECK(rp.rb * rp.chunk <= (uint32_t)PERM_ROWS_MAX, "permute: rows block to
   HIP_CHECK(hipGetLastError());
 }
 
-void fill_zero(Ctx& c, void* p, size_t bytes) {
-  if (bytes) HIP_CHECK(hipMemsetAsync(p, 0, bytes, c.stream));
-}
+void fill_zero(Ctx& c, void* p, size_t bytes) { dev_zero(c, p, bytes); }
 
 void axpy(Ctx& c, DType dt, int64_t n, double a, const void* x, void* y) {
   if (n <= 0) return;
@@ -876,6 +923,61 @@ void randn(Ctx& c, DType dt, int64_t n, uint64_t seed, void* out) {
 #define RK(T, ...) hipLaunchKernelGGL(randn_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, seed, (T*)out)
   DISPATCH3(dt, RK, 0);
 #undef RK
+  HIP_CHECK(hipGetLastError());
+}
+
+void randn_drawn(Ctx& c, DType dt, int64_t n, uint64_t base, const uint64_t* draws, uint64_t off, void* out) {
+  if (n <= 0) return;
+#define RK(T, ...) hipLaunchKernelGGL(randn_drawn_kernel<T>, dim3(blocks_for(n, 4)), dim3(TPB), 0, c.stream, n, base, draws, off, (T*)out)
+  DISPATCH3(dt, RK, 0);
+#undef RK
+  HIP_CHECK(hipGetLastError());
+}
+
+void counter_add(Ctx& c, uint64_t* counter, uint64_t add) {
+  hipLaunchKernelGGL(counter_add_kernel, dim3(1), dim3(64), 0, c.stream, counter, add);
+  HIP_CHECK(hipGetLastError());
+}
+
+static bool al16z(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+void copy_bytes_2d(Ctx& c, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows) {
+  if (!width || !rows) return;
+  const size_t n = width * rows;
+  if (al16z(dst) && al16z(src) && dpitch % 16 == 0 && spitch % 16 == 0 && width % 16 == 0) {
+    const size_t nw = n / 16;
+    hipLaunchKernelGGL(copy_rows16_kernel, dim3((unsigned)std::min<size_t>((nw + TPB - 1) / TPB, 65536)), dim3(TPB), 0,
+                       c.stream, (uint4*)dst, dpitch / 16, (const uint4*)src, spitch / 16, width / 16, rows);
+  } else {
+    hipLaunchKernelGGL(copy_rows1_kernel, dim3((unsigned)std::min<size_t>((n + TPB - 1) / TPB, 65536)), dim3(TPB), 0,
+                       c.stream, (unsigned char*)dst, dpitch, (const unsigned char*)src, spitch, width, rows);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+void copy_bytes(Ctx& c, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  if (bytes % 16 == 0 && al16z(dst) && al16z(src)) {
+    // as 1024-byte rows (any multiple of 16 that divides bytes)
+    size_t w = 1024;
+    while (bytes % w) w >>= 1;
+    copy_bytes_2d(c, dst, w, src, w, w, bytes / w);
+  } else {
+    copy_bytes_2d(c, dst, bytes, src, bytes, bytes, 1);
+  }
+}
+
+void zero_bytes_2d(Ctx& c, void* p, size_t pitch, size_t width, size_t rows) {
+  if (!width || !rows) return;
+  const size_t n = width * rows;
+  if (al16z(p) && pitch % 16 == 0 && width % 16 == 0) {
+    const size_t nw = n / 16;
+    hipLaunchKernelGGL(zero_rows16_kernel, dim3((unsigned)std::min<size_t>((nw + TPB - 1) / TPB, 65536)), dim3(TPB), 0,
+                       c.stream, (uint4*)p, pitch / 16, width / 16, rows);
+  } else {
+    hipLaunchKernelGGL(zero_rows1_kernel, dim3((unsigned)std::min<size_t>((n + TPB - 1) / TPB, 65536)), dim3(TPB), 0,
+                       c.stream, (unsigned char*)p, pitch, width, rows);
+  }
   HIP_CHECK(hipGetLastError());
 }
 
@@ -1596,6 +1698,143 @@ __global__ __launch_bounds__(256) void orgtr_kernel(int n, const double* __restr
   }
 }
 
+// ---------------------------------------------------------------------------
+// Cyclic two-sided Jacobi for a symmetric n x n FP64 matrix, n <= 112, in one
+// workgroup with everything in LDS: the lower triangle of A packed by columns
+// (tri_off) and V = the product of the rotations.  A sweep is m - 1 rounds
+// (m = n rounded up to even; index n is a dummy when n is odd) of the
+// round-robin ordering: in round r the players at positions i and m - 1 - i
+// form pair i, position 0 fixed and the others rotating, so the m / 2 pairs of
+// a round are disjoint and their rotations J(p, q) commute.  A round computes
+// each pair's rotation (Golub & Van Loan, sym.schur2: tau = (a_qq - a_pp) /
+// 2 a_pq, t = sign(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2),
+// s = t c), skipped when |a_pq| <= tol sqrt(|a_pp a_qq|), then updates every
+// 2 x 2 block of A below the diagonal (rows of pair a, columns of pair b,
+// a > b; the block above is its transpose) as B <- J_a^T B J_b, the diagonal
+// blocks by the closed forms a_pp - t a_pq, a_qq + t a_pq and an exact zero,
+// and V <- V J.  The sweeps stop after one with no rotation.
+// ---------------------------------------------------------------------------
+constexpr int JAC_MAXN = 112;
+constexpr int JAC_THREADS = 1024;
+constexpr int JAC_MAXBLK = (JAC_MAXN / 2) * (JAC_MAXN / 2 - 1) / 2;
+constexpr int JAC_MAXSWEEP = 40;
+constexpr double JAC_TOL = 1e-14;
+
+__device__ inline int jac_player(int pos, int r, int m) { return pos == 0 ? 0 : 1 + (pos - 1 + r) % (m - 1); }
+// packed lower-triangle index of A(i, j), either order
+__device__ inline int jac_at(int n, int i, int j) { return i >= j ? tri_off(n, j) + i - j : tri_off(n, i) + j - i; }
+
+__global__ __launch_bounds__(JAC_THREADS) void jacobi_eig_kernel(int n, const double* __restrict__ G, int64_t ldg,
+                                                                 double* __restrict__ lam, double* __restrict__ Q,
+                                                                 int64_t ldq, int* __restrict__ info) {
+  __shared__ double A[JAC_MAXN * (JAC_MAXN + 1) / 2];
+  __shared__ double V[JAC_MAXN * JAC_MAXN];  // V(i, j) at V[i + j n]
+  __shared__ double cs[JAC_MAXN / 2], sn[JAC_MAXN / 2], tt[JAC_MAXN / 2];
+  __shared__ int pp[JAC_MAXN / 2], pq[JAC_MAXN / 2];
+  __shared__ int rot_any;
+  __shared__ unsigned char blk_a[JAC_MAXBLK], blk_b[JAC_MAXBLK];  // block it -> pairs (a > b)
+  const int tid = threadIdx.x;
+  const int m = n + (n & 1), np = m / 2;
+  for (int j = 0; j < n; ++j)
+    for (int i = j + tid; i < n; i += JAC_THREADS) A[tri_off(n, j) + i - j] = G[i + (int64_t)j * ldg];
+  for (int idx = tid; idx < n * n; idx += JAC_THREADS) V[idx] = (idx % n) == (idx / n) ? 1.0 : 0.0;
+  const int nblk = np * (np - 1) / 2;  // blocks a > b
+  for (int a = 1 + tid; a < np; a += JAC_THREADS)
+    for (int b = 0; b < a; ++b) {
+      blk_a[a * (a - 1) / 2 + b] = (unsigned char)a;
+      blk_b[a * (a - 1) / 2 + b] = (unsigned char)b;
+    }
+  __syncthreads();
+  bool converged = n <= 1;
+  for (int sweep = 0; sweep < JAC_MAXSWEEP && !converged; ++sweep) {
+    if (tid == 0) rot_any = 0;
+    __syncthreads();
+    for (int r = 0; r < m - 1; ++r) {
+      if (tid < np) {
+        const int a = jac_player(tid, r, m), b = jac_player(m - 1 - tid, r, m);
+        const int p = min(a, b), q = max(a, b);
+        double c = 1.0, s = 0.0, t = 0.0;
+        if (q < n) {
+          const double apq = A[jac_at(n, q, p)], app = A[tri_off(n, p)], aqq = A[tri_off(n, q)];
+          if (apq != 0.0 && fabs(apq) > JAC_TOL * sqrt(fabs(app * aqq))) {
+            const double tau = (aqq - app) / (2.0 * apq);
+            t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+            rot_any = 1;
+          }
+        }
+        cs[tid] = c;
+        sn[tid] = s;
+        tt[tid] = t;
+        pp[tid] = p;
+        pq[tid] = q;
+      }
+      __syncthreads();
+      // blocks below the diagonal: item -> (ia > ib)
+      for (int it = tid; it < nblk + np; it += JAC_THREADS) {
+        if (it >= nblk) {  // diagonal block of pair it - nblk
+          const int ia = it - nblk;
+          const int p0 = pp[ia], q0 = pq[ia];
+          if (q0 < n && tt[ia] != 0.0) {
+            const int ipq = jac_at(n, q0, p0);
+            const double apq = A[ipq];
+            A[tri_off(n, p0)] -= tt[ia] * apq;
+            A[tri_off(n, q0)] += tt[ia] * apq;
+            A[ipq] = 0.0;
+          }
+          continue;
+        }
+        const int ia = blk_a[it], ib = blk_b[it];
+        const double ca = cs[ia], sa = sn[ia], cb = cs[ib], sb = sn[ib];
+        if (sa == 0.0 && sb == 0.0) continue;
+        const int p0 = pp[ia], q0 = pq[ia], p1 = pp[ib], q1 = pq[ib];
+        const bool rq = q0 < n, cq = q1 < n;  // rows / columns with a real q
+        const int i00 = jac_at(n, p0, p1);
+        const int i01 = cq ? jac_at(n, p0, q1) : 0;
+        const int i10 = rq ? jac_at(n, q0, p1) : 0;
+        const int i11 = rq && cq ? jac_at(n, q0, q1) : 0;
+        const double b00 = A[i00];
+        const double b01 = cq ? A[i01] : 0.0;
+        const double b10 = rq ? A[i10] : 0.0;
+        const double b11 = rq && cq ? A[i11] : 0.0;
+        // X = B J_b, then J_a^T X
+        const double x00 = cb * b00 - sb * b01, x01 = sb * b00 + cb * b01;
+        const double x10 = cb * b10 - sb * b11, x11 = sb * b10 + cb * b11;
+        A[i00] = ca * x00 - sa * x10;
+        if (cq) A[i01] = ca * x01 - sa * x11;
+        if (rq) A[i10] = sa * x00 + ca * x10;
+        if (rq && cq) A[i11] = sa * x01 + ca * x11;
+      }
+      // V <- V J: row i, pair b
+      for (int it = tid; it < n * np; it += JAC_THREADS) {
+        const int i = it % n, ib = it / n;
+        const int p1 = pp[ib], q1 = pq[ib];
+        if (q1 >= n || tt[ib] == 0.0) continue;
+        const double cb = cs[ib], sb = sn[ib];
+        const double vp = V[i + p1 * n], vq = V[i + q1 * n];
+        V[i + p1 * n] = cb * vp - sb * vq;
+        V[i + q1 * n] = sb * vp + cb * vq;
+      }
+      __syncthreads();
+    }
+    converged = rot_any == 0;
+    __syncthreads();
+  }
+  // ascending order: rank of d_i among the diagonal (ties by index)
+  for (int i = tid; i < n; i += JAC_THREADS) {
+    const double di = A[tri_off(n, i)];
+    int rk = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = A[tri_off(n, j)];
+      rk += (dj < di || (dj == di && j < i)) ? 1 : 0;
+    }
+    lam[rk] = di;
+    for (int rr = 0; rr < n; ++rr) Q[rr + (int64_t)rk * ldq] = V[rr + i * n];
+  }
+  if (tid == 0) *info = converged ? 0 : 1;
+}
+
 void sytrd_q(Ctx& c, int n, const double* G, int64_t ldg, double* d, double* e, double* Q, int64_t ldq) {
   static_assert(TRI_BIG_MAXN == TRI_MAXN_HOST, "sytrd_q size limit");
   IPEPS_CHECK(n >= 1 && n <= TRI_BIG_MAXN, "sytrd_q: n out of range");
@@ -1622,6 +1861,14 @@ void sytrd_q(Ctx& c, int n, const double* G, int64_t ldg, double* d, double* e, 
       hipLaunchKernelGGL(orgtr_kernel<(TRI_MAXN + 63) / 64>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream, n,
                          V, tau, Q, ldq);
   }
+  HIP_CHECK(hipGetLastError());
+}
+
+void jacobi_eig(Ctx& c, int n, const double* G, int64_t ldg, double* lam, double* Q, int64_t ldq, int* info) {
+  static_assert(JAC_MAXN == JAC_MAXN_HOST, "jacobi_eig size limit");
+  IPEPS_CHECK(n >= 1 && n <= JAC_MAXN, "jacobi_eig: n out of range");
+  TimedLaunch tl(c, "jacobi_eig_kernel", 0.0, 0.0);
+  hipLaunchKernelGGL(jacobi_eig_kernel, dim3(1), dim3(JAC_THREADS), 0, c.stream, n, G, ldg, lam, Q, ldq, info);
   HIP_CHECK(hipGetLastError());
 }
 

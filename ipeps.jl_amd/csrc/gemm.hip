@@ -1551,7 +1551,8 @@ void gemm(Ctx& c, const GemmArgs& g) {
       }
     }
     const double fl = 2.0 * (double)g.M * (double)g.N * (double)g.K * (double)g.batch;
-    if (!found && mode == 1 && fl >= min_flops) {
+    // (never while a graph is captured: tuning times the candidates)
+    if (!found && mode == 1 && fl >= min_flops && !c.graph_safe) {
       Plan best = tune(c, g);
       std::lock_guard<std::mutex> lk(T.mu);
       // two lanes that tuned one shape together keep the first entry

@@ -74,6 +74,16 @@ void col_xpby_ratio(Ctx& c, DType dt, int64_t rows, int64_t cols, const double* 
                     void* P);
 // Gaussian N x k test matrix from a counter-based generator (deterministic).
 void randn(Ctx& c, DType dt, int64_t n, uint64_t seed, void* out);
+// The same with the seed formed on the device: seed = draw_seed(base, *draws +
+// off) (ops.h), draws a device counter (what a captured graph replays with).
+void randn_drawn(Ctx& c, DType dt, int64_t n, uint64_t base, const uint64_t* draws, uint64_t off, void* out);
+// *counter += add (one thread).
+void counter_add(Ctx& c, uint64_t* counter, uint64_t add);
+// Byte copies and zero fills as kernels (graph-safe forms of hipMemcpyAsync /
+// hipMemsetAsync): rows of `width` bytes, `rows` of them, pitches in bytes.
+void copy_bytes(Ctx& c, void* dst, const void* src, size_t bytes);
+void copy_bytes_2d(Ctx& c, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows);
+void zero_bytes_2d(Ctx& c, void* p, size_t pitch, size_t width, size_t rows);
 
 // Reductions into device scalars.
 //   maxabs: out[0] = max |x|, out_idx[0] = first column-major index of it.
@@ -201,6 +211,15 @@ constexpr int TRI_MAXN_HOST = 320;  // largest n of sytrd_q
 // orthogonal Q (n x n, leading dimension ldq), the input rocSOLVER's stedc
 // takes for the eigenvectors of G.
 void sytrd_q(Ctx& c, int n, const double* G, int64_t ldg, double* d, double* e, double* Q, int64_t ldq);
+
+constexpr int JAC_MAXN_HOST = 112;  // largest n of jacobi_eig
+// Eigenpairs G = Q diag(lam) Q^T of a symmetric n x n FP64 matrix (n <= 112;
+// lower triangle read) in one workgroup by cyclic two-sided Jacobi: lam
+// ascending, Q (n x n, leading dimension ldq) the matching eigenvectors;
+// info (device int) = 0, or 1 if the off-diagonal part did not fall below the
+// relative threshold within the sweep limit.  Kernels only (no library call,
+// no memset): the routine a captured graph can hold.
+void jacobi_eig(Ctx& c, int n, const double* G, int64_t ldg, double* lam, double* Q, int64_t ldq, int* info);
 
 }  // namespace k
 }  // namespace ipeps

@@ -59,7 +59,7 @@ Tensor leaf(Tensor t) {
 
 Tensor clone(Ctx& c, const Tensor& t) {
   Tensor r = c.empty(t.dt, t.dims);
-  if (t.bytes()) HIP_CHECK(hipMemcpyAsync(r.data(), t.data(), t.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  dev_copy(c, r.data(), t.data(), t.bytes());
   return r;
 }
 
@@ -1181,14 +1181,13 @@ static void chol_inv(Ctx& c, int n, const double* G, int64_t ldg, double* Li, in
   Tensor X = c.empty(DType::F64, {n2, n1});
   gemm_f64(c, false, G + n1, ldg, true, L11i, ldl, X.ptr<double>(), n2, n2, n1, n1, 1.0, 0.0);
   Tensor S = c.empty(DType::F64, {n2, n2});
-  HIP_CHECK(hipMemcpy2DAsync(S.data(), n2 * 8, G + n1 + (int64_t)n1 * ldg, ldg * 8, n2 * 8, n2,
-                             hipMemcpyDeviceToDevice, c.stream));
+  dev_copy2d(c, S.data(), n2 * 8, G + n1 + (int64_t)n1 * ldg, ldg * 8, n2 * 8, n2);
   gemm_f64(c, false, X.ptr<double>(), n2, true, X.ptr<double>(), n2, S.ptr<double>(), n2, n2, n2, n1, -1.0, 1.0);
   k::small_chol_inv(c, n2, S.ptr<double>(), n2, L22i, ldl, info2);
   Tensor T = c.empty(DType::F64, {n2, n1});
   gemm_f64(c, false, X.ptr<double>(), n2, false, L11i, ldl, T.ptr<double>(), n2, n2, n1, n1, 1.0, 0.0);
   gemm_f64(c, false, L22i, ldl, false, T.ptr<double>(), n2, Li + n1, ldl, n2, n1, n2, -1.0, 0.0);
-  HIP_CHECK(hipMemset2DAsync(Li + (int64_t)n1 * ldl, ldl * 8, 0, n1 * 8, n2, c.stream));
+  dev_zero2d(c, Li + (int64_t)n1 * ldl, ldl * 8, n1 * 8, n2);
 }
 
 // CholeskyQR2: Q1 = Y R1^-1 with R1^H R1 = Y^H Y, then the same on Q1.  The
@@ -1215,6 +1214,13 @@ static bool orth_cholqr2_small(Ctx& c, const Tensor& Y, Tensor& Q) {
   Tensor Q1 = matmul_raw(c, Y, false, false, Li, true, false);
   Tensor G2 = matmul_raw(c, Q1, true, false, Q1, false, false);
   k::eye_dev(c, DType::F64, n, G2.data(), st.ptr<double>() + 4);
+  if (c.graph_safe) {
+    // both passes, checked after the step
+    chol_inv(c, (int)n, G2.ptr<double>(), n, Li.ptr<double>(), n, info(2), info(3));
+    Q = matmul_raw(c, Q1, false, false, Li, true, false);
+    record_check(c, CHECK_CHOLQR2, st.data(), 5);
+    return true;
+  }
   double hs[5];
   HIP_CHECK(hipMemcpyAsync(hs, st.data(), sizeof(hs), hipMemcpyDeviceToHost, c.stream));
   c.sync();
@@ -1287,14 +1293,22 @@ Tensor orth(Ctx& c, const Tensor& Y, int fast) {
 // by one Newton-Schulz step W <- W (3I - W^H W)/2.  Eigenvalues carry an
 // absolute error ~ u lambda_max, so the route is taken only when
 // s_min / s_max >= guard; returns false otherwise (outputs undefined).
-static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s, Tensor& X) {
+static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s, Tensor& X, bool jacobi) {
   const int64_t m = B.dims[0], n = B.dims[1];
   Tensor G = matmul_raw(c, B, true, true, B, false, false);
   Tensor lam = c.empty(DType::F64, {n});
   Tensor E = c.empty(DType::F64, {n});
   Tensor st = c.zeros(DType::F64, {1});
   rocblas_handle h = (rocblas_handle)c.solver;
-  if (B.dt == DType::F64 && n <= k::TRI_MAXN_HOST) {
+  if (jacobi || c.graph_safe) {
+    // kernels only: the one-workgroup Jacobi eigensolver (graph-safe mode
+    // defers its check)
+    IPEPS_CHECK(B.dt == DType::F64 && n <= k::JAC_MAXN_HOST, "Jacobi small SVD: FP64 and k <= 112 only");
+    Tensor Q = c.empty(DType::F64, {n, n});
+    k::jacobi_eig(c, (int)n, G.ptr<double>(), n, lam.ptr<double>(), Q.ptr<double>(), n, (int*)st.data());
+    if (c.graph_safe) record_check(c, CHECK_INFO, st.data(), 1);
+    G = Q;
+  } else if (B.dt == DType::F64 && n <= k::TRI_MAXN_HOST) {
     // Householder tridiagonalisation and Q on the device (one workgroup + one
     // wave per column), then rocSOLVER's divide and conquer on (d, e, Q)
     Tensor Q = c.empty(DType::F64, {n, n});
@@ -1322,6 +1336,9 @@ static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s
   s = c.empty(DType::F64, {n});
   Tensor sinv = c.empty(DType::F64, {n});
   k::eig_to_svd(c, B.dt, n, lam.ptr<double>(), G.data(), s.ptr<double>(), X.data(), sinv.ptr<double>());
+  if (c.graph_safe) {
+    if (guard > 0.0) record_check(c, CHECK_GUARD, s.data(), n, 0, 0, guard);
+  } else {
   std::vector<double> sh(n);
   double hinfo = 0.0;
   HIP_CHECK(hipMemcpyAsync(sh.data(), s.data(), n * 8, hipMemcpyDeviceToHost, c.stream));
@@ -1330,6 +1347,7 @@ static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s
   int info;
   std::memcpy(&info, &hinfo, sizeof(int));
   if (info != 0 || !(sh[0] > 0.0) || !(sh[n - 1] >= guard * sh[0])) return false;
+  }
   Tensor BX = matmul_raw(c, B, false, false, X, false, false);
   Tensor W0 = c.empty(B.dt, {m, n});
   k::scale_cols(c, B.dt, m, n, BX.data(), sinv.ptr<double>(), W0.data(), false);
@@ -1341,7 +1359,8 @@ static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s
 
 int svd_tall(Ctx& c, const Tensor& B, int method, double guard, Tensor& W, Tensor& s, Tensor& X) {
   IPEPS_CHECK(B.nd() == 2 && B.dims[0] >= B.dims[1], "svd_tall: tall matrix expected");
-  if (method == 1 && gram_svd(c, B, guard, W, s, X)) return 1;
+  if (method >= 1 && gram_svd(c, B, guard, W, s, X, method == 2)) return method;
+  IPEPS_CHECK(!c.graph_safe, "graph-safe small SVD: the Gram route only");
   gesvdj_thin(c, B, W, s, X);
   return 0;
 }
@@ -1363,13 +1382,13 @@ static int64_t cutoff_rule(const std::vector<double>& s, int64_t chi, double rel
 static Tensor first_cols(Ctx& c, const Tensor& X, int64_t n) {
   if (X.dims[1] == n) return X;
   Tensor r = c.empty(X.dt, {X.dims[0], n});
-  HIP_CHECK(hipMemcpyAsync(r.data(), X.data(), r.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  dev_copy(c, r.data(), X.data(), r.bytes());
   return r;
 }
 
 static Tensor first_n(Ctx& c, const Tensor& s, int64_t n) {
   Tensor r = c.empty(s.dt, {n});
-  HIP_CHECK(hipMemcpyAsync(r.data(), s.data(), r.bytes(), hipMemcpyDeviceToDevice, c.stream));
+  dev_copy(c, r.data(), s.data(), r.bytes());
   return r;
 }
 
@@ -1506,7 +1525,8 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     Tensor Om;
     if (!warm) {
       Om = c.empty(dt, {nc, kk});
-      k::randn(c, dt, nc * kk, o.seed, Om.data());
+      if (o.draws_dev) k::randn_drawn(c, dt, nc * kk, o.seed_base, o.draws_dev, o.draw_off, Om.data());
+      else k::randn(c, dt, nc * kk, o.seed, Om.data());
     }
     auto apply_M = [&](const Tensor& X) {  // M X = R1 (R2 X)
       return matmul_raw(c, R1, false, false, matmul_raw(c, R2, false, false, X, false, false), false, false);
@@ -1551,15 +1571,19 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
         Tensor Z = orth_cholqr1_nosync(c, to_dtype(c, range_MH(to_dtype(c, Q, dt)), sdt), info());
         Q = orth_cholqr1_nosync(c, to_dtype(c, range_M(to_dtype(c, Z, dt)), sdt), info());
       }
-      std::vector<double> hi(infos.numel());
-      HIP_CHECK(hipMemcpyAsync(hi.data(), infos.data(), infos.bytes(), hipMemcpyDeviceToHost, c.stream));
-      c.sync();
       lazy_ok = true;
-      for (double v : hi) {
-        int a, b;
-        std::memcpy(&a, &v, sizeof(int));
-        std::memcpy(&b, (const char*)&v + sizeof(int), sizeof(int));
-        lazy_ok = lazy_ok && a == 0 && b == 0;
+      if (c.graph_safe) {
+        record_check(c, CHECK_INFO_PAIRS, infos.data(), infos.numel());
+      } else {
+        std::vector<double> hi(infos.numel());
+        HIP_CHECK(hipMemcpyAsync(hi.data(), infos.data(), infos.bytes(), hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        for (double v : hi) {
+          int a, b;
+          std::memcpy(&a, &v, sizeof(int));
+          std::memcpy(&b, (const char*)&v + sizeof(int), sizeof(int));
+          lazy_ok = lazy_ok && a == 0 && b == 0;
+        }
       }
       if (lazy_ok) {
         orth_count(&SvdStats::orth_calls);
@@ -1579,12 +1603,19 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
       }
     }
     const int qmax = std::max(o.power_iters, o.power_iters_max);
+    IPEPS_CHECK(!c.graph_safe || o.resid_tol <= 0.0 || qmax <= o.power_iters,
+                "graph-safe projector: no adaptive power iterations");
+    IPEPS_CHECK(!c.graph_safe || !warm, "graph-safe projector: no warm start");
     double resid = -1.0;
     for (;;) {
       Tensor Bt = to_dtype(c, apply_MH(to_dtype(c, Q, dt)), sdt);  // nc x kk = M^H Q
       Tensor W, s, X;
       // Bt = W diag(s) X^H  ->  M ~ (Q X) diag(s) W^H
-      svd_tall(c, Bt, o.fast_svd ? 1 : 0, o.gram_guard, W, s, X);
+      // FP32 data carry ~1e-7 relative rounding in M, far above the Gram
+      // route's u (s_1 / s_i)^2 for every triplet above the mixed cutoff: the
+      // guard applies to FP64 engines only in graph-safe mode
+      const double guard = (c.graph_safe && dt == DType::F32) ? 0.0 : o.gram_guard;
+      svd_tall(c, Bt, c.graph_safe ? 2 : o.fast_svd, guard, W, s, X);
       Uf = matmul_raw(c, Q, false, false, X, false, false);
       Sf = s;
       Vf = W;
@@ -1629,13 +1660,24 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
     if (o.warm_start && o.warm_out) *o.warm_out = Uf;
   }
   const int64_t ns = Sf.numel();
-  out.s_host.resize(ns);
-  HIP_CHECK(hipMemcpyAsync(out.s_host.data(), Sf.data(), ns * 8, hipMemcpyDeviceToHost, c.stream));
-  c.sync();
   // FP32 engines keep only triplets above their noise floor (s / s_max >
   // mixed_cutoff): P = R2 V S^-1/2 amplifies the rounding error of smaller ones.
   const double rel = dt == DType::F32 ? std::max(1e-12, o.mixed_cutoff) : 1e-12;
-  const int64_t n = cutoff_rule(out.s_host, chi_eff, rel);
+  int64_t n;
+  if (c.graph_safe) {
+    // the rank of the last eager step, else the largest the rule admits
+    // without its degeneracy step; the rule itself is replayed on the
+    // spectrum after the step
+    IPEPS_CHECK(!full, "graph-safe projector: randomized mode only");
+    n = std::min<int64_t>(o.fixed_n > 0 ? o.fixed_n : chi_eff, ns);
+    record_check(c, CHECK_CUTOFF, Sf.data(), ns, chi_eff, n, rel);
+    out.s_host.assign(ns, 0.0);
+  } else {
+    out.s_host.resize(ns);
+    HIP_CHECK(hipMemcpyAsync(out.s_host.data(), Sf.data(), ns * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    n = cutoff_rule(out.s_host, chi_eff, rel);
+  }
   IPEPS_CHECK(n >= 1 && n <= ns, "svd_trunc: empty truncation");
   out.n = n;
   Tensor Un = first_cols(c, Uf, n), Vn = first_cols(c, Vf, n), Sn = first_n(c, Sf, n);

@@ -194,7 +194,9 @@ struct SvdOpts {
   //                  plus one Newton-Schulz orthonormalisation of B's singular
   //                  vectors, falling back to one-sided Jacobi (gesvdj) when
   //                  s_min / s_max < gram_guard (the Gram eigenvalues then lose
-  //                  more than ~1e-8 relative accuracy).
+  //                  more than ~1e-8 relative accuracy);
+  //                  2: the same with the Gram eigenpairs from the
+  //                  one-workgroup Jacobi kernel (F64, k <= 112).
   int fast_orth = 2;
   int fast_svd = 1;
   double gram_guard = 1e-4;
@@ -219,6 +221,13 @@ struct SvdOpts {
   // projector's accuracy.
   int early_mixed = 0;
   int late_apps = 2;
+  // Test matrix seed formed on the device from a draw counter (graph
+  // capture): draw_seed(seed_base, *draws_dev + draw_off) instead of `seed`.
+  const uint64_t* draws_dev = nullptr;
+  uint64_t seed_base = 0, draw_off = 0;
+  // Graph-safe mode: the rank to keep (> 0; the cutoff rule is checked
+  // against it after the step), else the largest the rule admits.
+  int64_t fixed_n = 0;
   std::shared_ptr<SvdStats> stats;  // optional counters
   // Warm start: with warm_start set and warm_in holding an m x k basis in the
   // solver dtype, the range finder starts from it (the previous projector's

@@ -1,7 +1,10 @@
 // Context, caching allocator and host/device copies.
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <sstream>
 
 #include "common.h"
@@ -18,6 +21,16 @@ Pool::~Pool() {
 void* Pool::alloc(size_t bytes) {
   std::lock_guard<std::mutex> g(mu_);
   size_t b = round(bytes);
+  if (capturing_) {
+    auto ic = cap_free_.find(b);
+    if (ic != cap_free_.end()) {
+      void* p = ic->second;
+      cap_free_.erase(ic);
+      live_ += b;
+      if (live_ > peak_) peak_ = live_;
+      return p;
+    }
+  }
   auto it = free_.find(b);
   void* p = nullptr;
   if (it != free_.end()) {
@@ -35,6 +48,11 @@ void* Pool::alloc(size_t bytes) {
   }
   live_ += b;
   if (live_ > peak_) peak_ = live_;
+  if (capturing_) {
+    cap_blocks_.emplace_back(b, p);
+    owned_.insert(p);
+    return p;
+  }
   static const bool poison = [] {
     const char* v = std::getenv("IPEPS_POOL_POISON");
     return v && v[0] == '1';
@@ -46,6 +64,11 @@ void* Pool::alloc(size_t bytes) {
 void Pool::release(void* p, size_t bytes) {
   std::lock_guard<std::mutex> g(mu_);
   size_t b = round(bytes);
+  if (owned_.count(p)) {  // a graph's block: reusable only by the capture
+    live_ -= b;
+    if (capturing_) cap_free_.emplace(b, p);
+    return;
+  }
   if (holds_ > 0)
     parked_.emplace_back(b, p);
   else
@@ -64,6 +87,32 @@ void Pool::unhold() {
   if (holds_ > 0 && --holds_ == 0) {
     for (auto& kv : parked_) free_.emplace(kv.first, kv.second);
     parked_.clear();
+  }
+}
+
+void Pool::begin_graph() {
+  std::lock_guard<std::mutex> g(mu_);
+  if (capturing_) throw Error(IPEPS_ERR_STATE, "Pool: graph capture already in progress");
+  capturing_ = true;
+  cap_free_.clear();
+  cap_blocks_.clear();
+}
+
+std::vector<std::pair<size_t, void*>> Pool::end_graph() {
+  std::lock_guard<std::mutex> g(mu_);
+  capturing_ = false;
+  cap_free_.clear();
+  std::vector<std::pair<size_t, void*>> out;
+  out.swap(cap_blocks_);
+  return out;
+}
+
+void Pool::release_graph(const std::vector<std::pair<size_t, void*>>& blocks) {
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& kv : blocks) {
+    owned_.erase(kv.second);
+    free_.emplace(kv.first, kv.second);
+    cached_ += kv.first;
   }
 }
 
@@ -91,8 +140,91 @@ Tensor Ctx::empty(DType dt, const std::vector<int64_t>& dims) {
 
 Tensor Ctx::zeros(DType dt, const std::vector<int64_t>& dims) {
   Tensor t = empty(dt, dims);
-  if (t.bytes()) HIP_CHECK(hipMemsetAsync(t.data(), 0, t.bytes(), stream));
+  if (t.bytes()) dev_zero(*this, t.data(), t.bytes());
   return t;
+}
+
+void dev_copy(Ctx& c, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  if (c.graph_safe) k::copy_bytes(c, dst, src, bytes);
+  else HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream));
+}
+
+void dev_copy2d(Ctx& c, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows) {
+  if (!width || !rows) return;
+  if (c.graph_safe) k::copy_bytes_2d(c, dst, dpitch, src, spitch, width, rows);
+  else HIP_CHECK(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyDeviceToDevice, c.stream));
+}
+
+void dev_zero(Ctx& c, void* p, size_t bytes) {
+  if (!bytes) return;
+  if (c.graph_safe) k::zero_bytes_2d(c, p, bytes, bytes, 1);
+  else HIP_CHECK(hipMemsetAsync(p, 0, bytes, c.stream));
+}
+
+void dev_zero2d(Ctx& c, void* p, size_t pitch, size_t width, size_t rows) {
+  if (!width || !rows) return;
+  if (c.graph_safe) k::zero_bytes_2d(c, p, pitch, width, rows);
+  else HIP_CHECK(hipMemset2DAsync(p, pitch, 0, width, rows, c.stream));
+}
+
+void checks_begin(Ctx& c, int64_t cap) {
+  if (!c.checks.defined() || c.checks.numel() < cap) c.checks = c.empty(DType::F64, {cap});
+  c.checks_pos = 0;
+  c.check_recs.clear();
+}
+
+void record_check(Ctx& c, int kind, const void* src, int64_t len, int64_t a, int64_t b, double x) {
+  IPEPS_CHECK(c.checks.defined() && c.checks_pos + len <= c.checks.numel(), "graph-safe checks buffer full");
+  dev_copy(c, c.checks.ptr<double>() + c.checks_pos, src, (size_t)len * 8);
+  c.check_recs.push_back(Ctx::CheckRec{kind, c.checks_pos, len, a, b, x});
+  c.checks_pos += len;
+}
+
+static int64_t cutoff_keep(const double* s, int64_t n, int64_t chi, double rel) {
+  // src/utility/wrapper.jl:37-55
+  double smax = 0.0;
+  for (int64_t i = 0; i < n; ++i) smax = std::max(smax, s[i]);
+  int64_t above = 0;
+  for (int64_t i = 0; i < n; ++i) above += (s[i] / smax > rel);
+  int64_t keep = std::min<int64_t>(chi, above);
+  if (n > keep && keep >= 1 && std::fabs(s[keep] - s[keep - 1]) < 1e-10) keep += 1;
+  return keep;
+}
+
+bool validate_checks(const std::vector<Ctx::CheckRec>& recs, const double* h, std::string* why) {
+  auto ints_zero = [](const double* v, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+      int a, b;
+      std::memcpy(&a, &v[i], sizeof(int));
+      std::memcpy(&b, (const char*)&v[i] + sizeof(int), sizeof(int));
+      if (a != 0 || b != 0) return false;
+    }
+    return true;
+  };
+  for (size_t r = 0; r < recs.size(); ++r) {
+    const auto& c = recs[r];
+    const double* v = h + c.off;
+    bool ok = true;
+    switch (c.kind) {
+      case CHECK_INFO_PAIRS: ok = ints_zero(v, c.len); break;
+      case CHECK_CHOLQR2: ok = ints_zero(v, 2) && v[4] <= 0.5; break;
+      case CHECK_INFO: {
+        int a;
+        std::memcpy(&a, v, sizeof(int));
+        ok = a == 0;
+        break;
+      }
+      case CHECK_CUTOFF: ok = cutoff_keep(v, c.len, c.a, c.x) == c.b; break;
+      case CHECK_GUARD: ok = c.len >= 1 && v[0] > 0.0 && v[c.len - 1] >= c.x * v[0]; break;
+      default: ok = false;
+    }
+    if (!ok) {
+      if (why) *why = "check " + std::to_string(r) + " (kind " + std::to_string(c.kind) + ") failed";
+      return false;
+    }
+  }
+  return true;
 }
 
 void Ctx::resolve_timing() {
@@ -153,6 +285,9 @@ void ctx_destroy(Ctx* c) {
   }
   (void)hipStreamSynchronize(c->stream);
   if (c->solver) rocblas_destroy_handle((rocblas_handle)c->solver);
+  // tensors the context itself holds go back to its pool before the pool goes
+  c->checks = Tensor();
+  c->check_recs.clear();
   c->pool.reset();
   (void)hipStreamDestroy(c->stream);
   delete c;

@@ -117,7 +117,7 @@ ENV_OPTIONS = ("svd_mode", "oversample", "power_iters", "full_below", "seed", "e
                "fast_svd",
                "gram_guard", "power_iters_max", "resid_tol", "complement_iters", "complement_tol", "warm_start", "canonical_edge", "mixed", "mixed_back", "mixed_proj", "energy_fp64", "energy_mixed",
                "energy_mixed_back",
-               "checkpoint", "fused_absorb", "paired_moves", "draws")
+               "checkpoint", "fused_absorb", "paired_moves", "draws", "early_mixed", "late_apps", "graph_safe")
 
 
 class CTMTensors:
@@ -229,6 +229,14 @@ def rg_step(ts, chi):
     n = C.c_int64()
     check(lib().ipeps_rg_step(ts.h, int(chi), _dp(s), s.size, C.byref(n)))
     return ts, s[: n.value].copy()
+
+
+def run_graph(ts, chi, steps):
+    """``steps`` rg_steps as replays of a captured HIP graph of the step
+    (ipeps_env_run_graph); returns dict(captured, replayed, undone, eager)."""
+    out = (C.c_int64 * 4)()
+    check(lib().ipeps_env_run_graph(ts.h, int(chi), int(steps), out))
+    return dict(captured=out[0], replayed=out[1], undone=out[2], eager=out[3])
 
 
 def run_ctm(ts, conv_fun="energy", H=None):
@@ -474,7 +482,8 @@ def orth(Y, method=1, ctx=None):
 def svd_thin(A, method=1, guard=1e-4, ctx=None):
     """Thin SVD of a tall matrix on the device: (U, S, V, method_used) with
     A = U diag(S) V^H.  method 0: one-sided Jacobi; 1: Gram eigenpairs when
-    S_min/S_max >= guard (else 0)."""
+    S_min/S_max >= guard (else 0); 2: as 1 with the Gram eigenpairs from the
+    one-workgroup Jacobi kernel (real, n <= 112)."""
     ctx = ctx or default_context()
     A = np.asarray(A)
     code = C128 if np.iscomplexobj(A) else F64
@@ -515,4 +524,4 @@ def svd_back(U, S, V, dU=None, dS=None, dV=None, eta=1e-12, ctx=None):
 
 __all__ = ["absorb", "Context", "CTMTensors", "rg_step", "run_ctm", "get_gs_energy", "get_gs_norm", "compute_gs_energy",
            "step_adjoint", "gs_energy_grad", "tcon", "gemm", "svd1", "svd_back", "default_context", "IpepsError",
-           "ENV_OPTIONS", "tune", "tune_stats", "tune_clear", "tune_save", "tune_load"]
+           "ENV_OPTIONS", "tune", "tune_stats", "tune_clear", "tune_save", "tune_load", "run_graph"]
