@@ -191,7 +191,7 @@ def cpu_baseline_sample(D, chi, seed):
                 energy_forward_s=en_f, energy_adjoint_s=en_b)
 
 
-def batch_c4(pkg, steps=10, workers=8):
+def batch_c4(pkg, steps=10, workers=16):
     """Uneven 64-instance workload (D in {2,3,4}, chi in {16..96}): mixed and
     FP64 engines advance all instances `steps` rg_steps after one warm-up
     step, eagerly and in graph mode (each instance's step captured into a HIP

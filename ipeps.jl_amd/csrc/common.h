@@ -16,6 +16,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <initializer_list>
 #include <vector>
 
 namespace ipeps {
@@ -189,6 +190,20 @@ struct Ctx {
   };
   bool graph_safe = false;
   Tensor checks;  // from this context's pool: ctx_destroy releases it before the pool
+
+  // bf16 planes of FP32 GEMM operands that several products re-read (the
+  // half-system matrices inside a projector), kept while a SplitCacheScope
+  // registers the source tensors: products of a registered operand reuse
+  // its planes instead of splitting it again.
+  struct SplitEntry {
+    const void* p;
+    int64_t ld, R, K, kp;
+    bool kv;
+    int ns;
+    std::shared_ptr<Buf> planes;
+  };
+  std::vector<std::pair<const char*, size_t>> split_src;  // registered [p, p + bytes)
+  std::vector<SplitEntry> split_cache;
   int64_t checks_pos = 0;
   std::vector<CheckRec> check_recs;
 
@@ -203,6 +218,22 @@ struct Ctx {
 
 Ctx* ctx_create(int device);
 void ctx_destroy(Ctx* c);
+
+// Registers tensors whose bf16 operand planes the context may keep for the
+// scope (the tensors must stay alive and unchanged meanwhile); the planes go
+// when the scope ends.  Nested scopes add to the registration.
+struct SplitCacheScope {
+  Ctx& c;
+  size_t n_src, n_cache;
+  SplitCacheScope(Ctx& cc, std::initializer_list<const Tensor*> ts) : c(cc), n_src(cc.split_src.size()), n_cache(cc.split_cache.size()) {
+    for (const Tensor* t : ts)
+      if (t && t->defined() && t->dt == DType::F32) c.split_src.emplace_back((const char*)t->data(), t->bytes());
+  }
+  ~SplitCacheScope() {
+    c.split_src.resize(n_src);
+    c.split_cache.resize(n_cache);
+  }
+};
 
 // Deferred checks of graph-safe mode.
 enum CheckKind : int {

@@ -49,6 +49,9 @@ Proj get_projector(Ctx& c, const Env& e, const Tensor& R1, const Tensor& R2, int
   }
   if (c.graph_safe && dir < 4) so.fixed_n = e.kept[dir];
   op::PrecScope ps(e.opt.mixed_proj ? (k::Mixed)e.opt.mixed_proj : op::prec().mixed, op::prec().back);
+  // R1 and R2 enter every product of the range finder and P1 / P2: their
+  // bf16 planes are split once
+  SplitCacheScope scs(c, {&R1, &R2});
   auto r = op::svd_trunc(c, R1, R2, chi, so);
   if (dir < 4) e.kept[dir] = r.n;
   Tensor isq = op::inv_sqrt(c, r.S);

@@ -991,8 +991,28 @@ static int nsplit_of(Mixed m) { return m == Mixed::BF16 ? 1 : 2; }
 
 // Pre-split op(X) (R rows x K; kv: X is k-contiguous, X[k + r ld]) into
 // NSPLIT bf16 planes of R x Kp, Kp = K rounded up to 32; returns the buffer.
+static std::shared_ptr<Buf> split_operand_new(Ctx& c, const float* X, int64_t ld, bool kv, int64_t R, int64_t K,
+                                              int ns, int64_t* Kp_out);
+// The same through the context's cache of registered operands (SplitCacheScope).
 static std::shared_ptr<Buf> split_operand(Ctx& c, const float* X, int64_t ld, bool kv, int64_t R, int64_t K,
                                           int ns, int64_t* Kp_out) {
+  const char* px = (const char*)X;
+  bool reg = false;
+  for (const auto& r : c.split_src)
+    if (px >= r.first && px < r.first + r.second) reg = true;
+  if (!reg) return split_operand_new(c, X, ld, kv, R, K, ns, Kp_out);
+  for (const auto& e : c.split_cache)
+    if (e.p == X && e.ld == ld && e.kv == kv && e.R == R && e.K == K && e.ns == ns) {
+      *Kp_out = e.kp;
+      return e.planes;
+    }
+  auto planes = split_operand_new(c, X, ld, kv, R, K, ns, Kp_out);
+  c.split_cache.push_back(Ctx::SplitEntry{X, ld, R, K, *Kp_out, kv, ns, planes});
+  return planes;
+}
+
+static std::shared_ptr<Buf> split_operand_new(Ctx& c, const float* X, int64_t ld, bool kv, int64_t R, int64_t K,
+                                              int ns, int64_t* Kp_out) {
   const int64_t Kp = (K + 31) / 32 * 32;
   const size_t bytes = (size_t)ns * R * Kp * 2;
   std::shared_ptr<Buf> buf(new Buf{c.pool->alloc(bytes), bytes, c.pool.get()});

@@ -287,6 +287,7 @@ void ctx_destroy(Ctx* c) {
   if (c->solver) rocblas_destroy_handle((rocblas_handle)c->solver);
   // tensors the context itself holds go back to its pool before the pool goes
   c->checks = Tensor();
+  c->split_cache.clear();
   c->check_recs.clear();
   c->pool.reset();
   (void)hipStreamDestroy(c->stream);
