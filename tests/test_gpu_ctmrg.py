@@ -595,6 +595,25 @@ def test_orth_blocked_cholesky_kernel_runs(pkg):
     assert "small_chol_inv_kernel" in st and "rocsolver_potrf" not in st
 
 
+@pytest.mark.parametrize("n", [96, 200])
+def test_orth_cholesky_failure_falls_back(pkg, n):
+    """A rank-deficient Y (two equal columns): the one-workgroup Cholesky of
+    its Gram reports a non-positive pivot (or the second pass is far from
+    I), and orth falls back to Householder QR: Q orthonormal, Q Q^T Y = Y.
+    n = 200 runs the 2 x 2 block split of the Cholesky."""
+    rng = np.random.default_rng(3)
+    Y = rng.standard_normal((1500, n))
+    Y[:, 5] = Y[:, 2]
+    ctx = pkg.Context()
+    ctx.set_timing(1)
+    Q = pkg.orth(Y, method=2, ctx=ctx)
+    st = ctx.kernel_stats(reset=True)
+    ctx.close()
+    assert "small_chol_inv_kernel" in st and "rocsolver_geqrf_orgqr" in st
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-12
+    assert np.abs(Q @ (Q.T @ Y) - Y).max() < 1e-12
+
+
 def test_small_cholesky_route_matches_rocsolver(pkg):
     """fast_orth=2 (CholeskyQR2 on the one-workgroup FP64 Cholesky + triangular
     inverse) against fast_orth=1 (rocSOLVER potrf + trsm) on the same test
