@@ -784,7 +784,22 @@ void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::v
     PermRowsP rp;
     PermTileP tp;
     const int esize = (int)dtype_size(dt);
-    const int kind = plan_permute((int)od.size(), od.data(), is.data(), esize, rp, tp);
+    // A stride-1 fastest axis whose extent and the other input strides are
+    // multiples of a 16-byte word: the same permutation of 16-byte elements
+    // (bit copies; conjugation never applies to real data)
+    bool words = false;
+    std::vector<int64_t> odw = od, isw = is;
+    if (!cj && is[0] == 1 && esize < 16 && ((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0) {
+      const int64_t g = 16 / esize;
+      words = od[0] % g == 0;
+      for (size_t a = 1; a < is.size(); ++a) words = words && is[a] % g == 0;
+      if (words) {
+        odw[0] /= g;
+        for (size_t a = 1; a < is.size(); ++a) isw[a] /= g;
+      }
+    }
+    const int kind = words ? plan_permute((int)odw.size(), odw.data(), isw.data(), 16, rp, tp)
+                           : plan_permute((int)od.size(), od.data(), is.data(), esize, rp, tp);
     if (kind >= 0) {
       std::string kname = std::string(kind == 0 ? "permute_rows_kernel<" : "permute_tiled_kernel<") + dtype_name(dt) + ">";
       if (c.timing_detail) {
@@ -798,7 +813,8 @@ void permute(Ctx& c, DType dt, const std::vector<int64_t>& in_dims, const std::v
         const unsigned nb = (unsigned)(((int64_t)rp.nrows + rp.rb - 1) / rp.rb * rp.nchunk);
 IPEPS_CHECK(rp.rb * rp.chunk <= (uint32_t)PERM_ROWS_MAX, "permute: rows block too large");
 #define RK(T, ...) hipLaunchKernelGGL(permute_rows_kernel<T>, dim3(nb), dim3(TPB), 0, c.stream, rp, (const T*)in, (T*)out, cj)
-        DISPATCH3(dt, RK, 0);
+        if (words) RK(float4, 0);
+        else DISPATCH3(dt, RK, 0);
 #undef RK
       } else {
         dim3 grid(tp.ntx * tp.nty, std::min<uint32_t>(tp.nrest_total, 65535u));

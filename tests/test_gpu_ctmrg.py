@@ -71,6 +71,17 @@ def test_permutes_bit_exact(pkg, cplx):
             ref = np.transpose(X, perm)[..., None]
             assert got.shape == ref.shape, (shp, perm)
             assert np.array_equal(got, ref), (shp, perm)
+    # the fastest axis kept first with even extent and strides: the rows kernel on 16-byte words (real
+    # data), and odd extents that fall back to elements
+    for shp, perm in (((144, 6, 6, 6), (0, 2, 1, 3)), ((144, 6, 6, 6), (0, 3, 2, 1)), ((6, 144, 3), (0, 2, 1)),
+                      ((2, 5000, 3), (0, 2, 1)), ((7, 6, 4), (0, 2, 1)), ((36, 36, 2, 2), (0, 2, 1, 3))):
+        X = rng.standard_normal(shp)
+        if cplx:
+            X = X + 1j * rng.standard_normal(shp)
+        perm = np.array(perm)
+        la = [-(int(np.where(perm == a)[0][0]) + 1) for a in range(len(shp))]
+        got = pkg.tcon([X, np.ones(1, dtype=X.dtype)], [la, [-(len(shp) + 1)]])
+        assert np.array_equal(got, np.transpose(X, perm)[..., None]), (shp, perm)
 
 
 def test_large_gemm(pkg):
