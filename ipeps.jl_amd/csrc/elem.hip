@@ -1496,36 +1496,40 @@ struct TrbShared {
 
 // Reflector of step j from column j of A.  With `upd`, column j still lacks
 // step j - 1's update A <- A - v w^T - w v^T (vp, wp: step j - 1's vectors,
-// local index = row - j), applied here first by wave 0; each lane re-reads
-// only the rows it wrote.  The reflector goes to column j of A, of V and to
-// vj (local index = row - j - 1); returns tau_j through sh.sc.
+// local index = row - j), applied here.  Wave 0 does the whole step: its
+// lanes hold the column's rows in registers (the update, the norm, the
+// scaling), lane 0 forms tau and the scale, and the reflector goes to vj
+// (local index = row - j - 1) and to column j of V; column j of A is not read
+// again, so it is not written.  tau_j is returned through sh.sc.
 __device__ void trb_reflector(const TriView& A, int n, int j, bool upd, const double* vp, const double* wp,
                               double* vj, TrbShared& sh, double* __restrict__ d, double* __restrict__ e,
                               double* __restrict__ tau, double* __restrict__ V) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m = n - 1 - j;
-  double* cj = A.col(j);
+  const double* cj = A.col(j);
   if (wave == 0) {
     const double vp0 = upd ? vp[0] : 0.0, wp0 = upd ? wp[0] : 0.0;
+    double xs[TRB_RW];
     double x2 = 0.0;
-    for (int r = j + 2 + lane; r < n; r += 64) {
-      double x = cj[r];
-      if (upd) {
-        x = x - (vp[r - j] * wp0 + wp[r - j] * vp0);
-        cj[r] = x;
+#pragma unroll
+    for (int q = 0; q < TRB_RW; ++q) {
+      const int r = j + 2 + lane + 64 * q;
+      double x = 0.0;
+      if (r < n) {
+        x = cj[r];
+        if (upd) x = x - (vp[r - j] * wp0 + wp[r - j] * vp0);
+        x2 += x * x;
       }
-      x2 += x * x;
+      xs[q] = x;
     }
     x2 = wave_sum_d(x2);
+    double scale = 0.0;
     if (lane == 0) {
       double a0 = cj[j], alpha = cj[j + 1];
       if (upd) {
         a0 = a0 - (vp[0] * wp0 + wp[0] * vp0);
         alpha = alpha - (vp[1] * wp0 + wp[1] * vp0);
-        cj[j] = a0;
-        cj[j + 1] = alpha;
       }
-      double t = 0.0, beta = alpha, scale = 0.0;
+      double t = 0.0, beta = alpha;
       if (x2 > 0.0) {
         beta = -copysign(sqrt(alpha * alpha + x2), alpha);
         t = (beta - alpha) / beta;
@@ -1536,15 +1540,19 @@ __device__ void trb_reflector(const TriView& A, int n, int j, bool upd, const do
       d[j] = a0;
       e[j] = beta;
       tau[j] = t;
+      vj[0] = 1.0;
+      V[(j + 1) + (int64_t)j * n] = 1.0;
     }
-  }
-  __syncthreads();
-  const double scale = sh.sc[1];
-  for (int r = tid; r < m; r += TRI_THREADS) {
-    const double vr = r == 0 ? 1.0 : cj[j + 1 + r] * scale;
-    vj[r] = vr;
-    cj[j + 1 + r] = vr;
-    V[(j + 1 + r) + (int64_t)j * n] = vr;
+    scale = __shfl(scale, 0);
+#pragma unroll
+    for (int q = 0; q < TRB_RW; ++q) {
+      const int r = j + 2 + lane + 64 * q;
+      if (r < n) {
+        const double vr = xs[q] * scale;
+        vj[r - j - 1] = vr;
+        V[r + (int64_t)j * n] = vr;
+      }
+    }
   }
   __syncthreads();
 }
