@@ -324,10 +324,10 @@ def main():
                     help="bf16 products of the VJPs (0: same as --mixed)")
     ap.add_argument("--mixed-proj", type=int, default=6, choices=[0, 1, 3, 6],
                     help="bf16 products inside the projector (0: same as --mixed; 6: FP32 products)")
-    ap.add_argument("--early-mixed", type=int, default=0, choices=[0, 1, 3, 6],
+    ap.add_argument("--early-mixed", type=int, default=1, choices=[0, 1, 3, 6],
                     help="products of the range finder's early applications of M (all but the last --late-apps "
                          "before B = M^H Q; 0: same as --mixed-proj)")
-    ap.add_argument("--late-apps", type=int, default=2)
+    ap.add_argument("--late-apps", type=int, default=1)
     ap.add_argument("--tune", type=int, default=2, choices=[0, 1, 2],
                     help="GEMM launch plans: 0 shape rules, 1 autotune now (unseen shapes are timed during the "
                          "warm-up steps), 2 replay the tuning table --tune-file (shapes missing from it: rules)")
@@ -504,6 +504,7 @@ def main():
                           + (f" complement CG {a.complement_iters}" if a.complement_iters else ""),
                    "mixed_products": a.mixed, "mixed_back": a.mixed_back or a.mixed,
                    "mixed_proj": a.mixed_proj or a.mixed,
+                   "early_mixed": a.early_mixed, "late_apps": a.late_apps,
                    "gemm_plans": {0: "shape rules", 1: "autotuned in this run",
                                   2: "tuning table " + os.path.relpath(a.tune_file, ROOT)}[a.tune]},
         "algorithmic_flops_per_step": F,

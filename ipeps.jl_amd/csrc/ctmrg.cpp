@@ -246,9 +246,15 @@ Tensor corner_lr(Ctx& c, const Tensor& C3, const Tensor& E3, const Tensor& E2, c
   return tcon(c, Y2, {-1, 1, -2, 2, -3, -4, -7}, Ad, {-5, -6, 1, 2, -8});
 }
 
-// rho from two halves (x1, x2, a, ad, p, p') sharing their first four legs
+// rho from two halves (x1, x2, a, ad, p, p') sharing their first four legs.
+// The trace over the chi^2 D^2 shared legs is a sum with heavy cancellation
+// (the bond energy is a small difference of rho's entries): FP32 halves are
+// cast to FP64 for it.
 Tensor close_dm(Ctx& c, const Tensor& h1, const Tensor& h2) {
-  Tensor r = tcon(c, h1, {1, 2, 3, 4, -1, -3}, h2, {1, 2, 3, 4, -2, -4});
+  const bool widen = h1.dt == DType::F32;
+  const Tensor a = widen ? op::cast(c, h1, DType::F64) : h1;
+  const Tensor b = widen ? op::cast(c, h2, DType::F64) : h2;
+  Tensor r = tcon(c, a, {1, 2, 3, 4, -1, -3}, b, {1, 2, 3, 4, -2, -4});
   int64_t d = h1.dims[4];
   return op::reshape(c, r, {d * d, d * d});
 }

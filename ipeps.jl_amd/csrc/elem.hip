@@ -1333,6 +1333,38 @@ __device__ inline double wave_sum_d(double x) {
   return x;
 }
 
+// x moved across lanes by one DPP control (both 32-bit halves); rows outside
+// ROWS receive 0.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_move_d(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Sum over the 64 lanes of a wave, valid in lane 63 only: quad permutes and
+// row rotates sum each 16-lane row, row_bcast:15 adds row 0 into row 1 and row
+// 2 into row 3, row_bcast:31 adds row 1 into rows 2 and 3.  VALU moves only
+// (no LDS permutes); every lane of the wave must be active.
+__device__ inline double wave_sum_dpp63(double x) {
+  x += dpp_move_d<0xB1, 0xF>(x);   // quad_perm [1, 0, 3, 2]
+  x += dpp_move_d<0x4E, 0xF>(x);   // quad_perm [2, 3, 0, 1]
+  x += dpp_move_d<0x124, 0xF>(x);  // row_ror:4
+  x += dpp_move_d<0x128, 0xF>(x);  // row_ror:8
+  x += dpp_move_d<0x142, 0xA>(x);  // row_bcast:15
+  x += dpp_move_d<0x143, 0xC>(x);  // row_bcast:31
+  return x;
+}
+
+// the value of lane l (wave-uniform l)
+__device__ inline double lane_value_d(double x, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 __global__ __launch_bounds__(TRI_THREADS) void sytrd_kernel(int n, const double* __restrict__ G, int64_t ldg,
                                                             double* __restrict__ d, double* __restrict__ e,
                                                             double* __restrict__ tau, double* __restrict__ V) {
@@ -1521,7 +1553,7 @@ __device__ void trb_reflector(const TriView& A, int n, int j, bool upd, const do
       }
       xs[q] = x;
     }
-    x2 = wave_sum_d(x2);
+    x2 = lane_value_d(wave_sum_dpp63(x2), 63);
     double scale = 0.0;
     if (lane == 0) {
       double a0 = cj[j], alpha = cj[j + 1];
@@ -1543,7 +1575,7 @@ __device__ void trb_reflector(const TriView& A, int n, int j, bool upd, const do
       vj[0] = 1.0;
       V[(j + 1) + (int64_t)j * n] = 1.0;
     }
-    scale = __shfl(scale, 0);
+    scale = lane_value_d(scale, 0);
 #pragma unroll
     for (int q = 0; q < TRB_RW; ++q) {
       const int r = j + 2 + lane + 64 * q;
@@ -1608,8 +1640,8 @@ __device__ void trb_pass(const TriView& A, int n, int j, bool upd, const double*
         pp[q] += a[u][q] * vc;
         if (r > c && r < m) s += a[u][q] * vj[r];
       }
-      s = wave_sum_d(s);
-      if (lane == 0) sh.tp[c] = s;
+      s = wave_sum_dpp63(s);
+      if (lane == 63) sh.tp[c] = s;
     }
   }
 #pragma unroll
@@ -1628,8 +1660,8 @@ __device__ void trb_pass(const TriView& A, int n, int j, bool upd, const double*
     wj[tid] = pr;
     prod = pr * vj[tid];
   }
-  prod = wave_sum_d(prod);
-  if (lane == 0) sh.red[wave] = prod;
+  prod = wave_sum_dpp63(prod);
+  if (lane == 63) sh.red[wave] = prod;
   __syncthreads();
   double dot = 0.0;
 #pragma unroll
