@@ -1365,190 +1365,42 @@ __device__ inline double lane_value_d(double x, int l) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-__global__ __launch_bounds__(TRI_THREADS) void sytrd_kernel(int n, const double* __restrict__ G, int64_t ldg,
+// Householder tridiagonalisation in one workgroup (n <= TRI_BIG_MAXN); the
+// scheme is described in tridiag_impl.h.
+constexpr int SQ_NWAVE = TRI_THREADS / 64;
+constexpr int SQ_RW = (TRI_BIG_MAXN + 63) / 64;  // 64-row blocks of a column (global phase)
+constexpr int SQ_LM = TRI_SQ_LDS;                // LDS phase: trailing square at most SQ_LM a side
+constexpr int SQ_U = 4;                          // columns per wave in flight
+
+struct SqShared {
+  double v[2][TRI_BIG_MAXN], w[2][TRI_BIG_MAXN];
+  double part_l[SQ_NWAVE][SQ_LM];
+  double red[SQ_NWAVE];
+  double sc[2];
+  double junk[64];
+};
+
+// reflector of step j from column j (rows j .. n-1) of the square, stored at
+// col[row] for the column's base pointer; step j-1's update applied on the fly
+template <typename P>
+__device__ __attribute__((always_inline)) void sq_reflector(P col, int n, int j, bool upd, const double* vp,
+                                                            const double* wp, double* vj, SqShared& sh,
                                                             double* __restrict__ d, double* __restrict__ e,
                                                             double* __restrict__ tau, double* __restrict__ V) {
-  constexpr int NWAVE = TRI_THREADS / 64;
-  constexpr int RW = (TRI_MAXN + 63) / 64;   // rows per lane
-  constexpr int NBLK = (TRI_MAXN + 31) / 32;  // 32-row blocks of a column
-  __shared__ double P[TRI_MAXN * (TRI_MAXN + 1) / 2];
-  __shared__ double v[TRI_MAXN], w[TRI_MAXN];
-  __shared__ double part[NWAVE][TRI_MAXN];  // per-wave partials of sum_{c <= r} A'(r, c) v_c
-  __shared__ double tpart[NBLK][TRI_MAXN];  // per-block partials of sum_{r > c} A'(r, c) v_r
-  __shared__ double red[NWAVE];
-  __shared__ double dl[TRI_MAXN], el[TRI_MAXN], tl[TRI_MAXN];
-  __shared__ double sc[3];  // tau, scale
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int c = 0; c < n; ++c)
-    for (int r = c + tid; r < n; r += TRI_THREADS) P[tri_off(n, c) + r - c] = G[r + c * ldg];
-  __syncthreads();
-  for (int i = 0; i + 2 < n; ++i) {
-    const int m = n - 1 - i;
-    const int ci = tri_off(n, i);  // A(i + 1 + r, i) = P[ci + 1 + r]
-    // (a) reflector of x = A(i+1 .., i)
-    if (wave == 0) {
-      double x2 = 0.0;
-      for (int r = 1 + lane; r < m; r += 64) {
-        const double x = P[ci + 1 + r];
-        x2 += x * x;
-      }
-      x2 = wave_sum_d(x2);
-      if (lane == 0) {
-        const double alpha = P[ci + 1];
-        double t = 0.0, beta = alpha, scale = 0.0;
-        if (x2 > 0.0) {
-          beta = -copysign(sqrt(alpha * alpha + x2), alpha);
-          t = (beta - alpha) / beta;
-          scale = 1.0 / (alpha - beta);
-        }
-        sc[0] = t;
-        sc[1] = scale;
-        dl[i] = P[ci];
-        el[i] = beta;
-        tl[i] = t;
-      }
-    }
-    __syncthreads();
-    const double t = sc[0], scale = sc[1];
-    // v (v_0 = 1), kept in column i of P for the reflector output
-    for (int r = tid; r < m; r += TRI_THREADS) {
-      const double vr = r == 0 ? 1.0 : P[ci + 1 + r] * scale;
-      v[r] = vr;
-      P[ci + 1 + r] = vr;
-    }
-    __syncthreads();
-    if (t == 0.0) continue;  // H(i) = I (uniform)
-    // (b) p = tau A' v from the lower triangle of A' = A(i+1 .., i+1 ..):
-    //     lower part: waves over columns c, lanes down the rows r >= c;
-    //     transposed part: (column c, 32-row block) items, a thread each.
-    double pp[RW];
-#pragma unroll
-    for (int q = 0; q < RW; ++q) pp[q] = 0.0;
-    for (int c = wave; c < m; c += NWAVE) {
-      const double vc = v[c];
-      const int base = tri_off(n, i + 1 + c) - c;  // A'(r, c) = P[base + r], r >= c
-#pragma unroll
-      for (int q = 0; q < RW; ++q) {
-        const int r = lane + 64 * q;
-        if (r >= c && r < m) pp[q] += P[base + r] * vc;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < RW; ++q) {
-      const int r = lane + 64 * q;
-      if (r < m) part[wave][r] = pp[q];
-    }
-    for (int it = tid; it < NBLK * m; it += TRI_THREADS) {
-      const int c = it % m, blk = it / m;
-      const int r0 = max(c + 1, 32 * blk), r1 = min(m, 32 * blk + 32);
-      const int base = tri_off(n, i + 1 + c) - c;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four chains of loads in flight
-      int r = r0;
-      for (; r + 3 < r1; r += 4) {
-        s0 += P[base + r] * v[r];
-        s1 += P[base + r + 1] * v[r + 1];
-        s2 += P[base + r + 2] * v[r + 2];
-        s3 += P[base + r + 3] * v[r + 3];
-      }
-      for (; r < r1; ++r) s0 += P[base + r] * v[r];
-      tpart[blk][c] = (s0 + s1) + (s2 + s3);
-    }
-    __syncthreads();
-    // (c) p and p . v; every thread then sums the wave totals itself
-    double prod = 0.0;
-    if (tid < m) {
-      double pr = 0.0;
-#pragma unroll
-      for (int q = 0; q < NWAVE; ++q) pr += part[q][tid];
-#pragma unroll
-      for (int q = 0; q < NBLK; ++q) pr += tpart[q][tid];
-      pr *= t;
-      w[tid] = pr;
-      prod = pr * v[tid];
-    }
-    prod = wave_sum_d(prod);
-    if (lane == 0) red[wave] = prod;
-    __syncthreads();
-    double dot = 0.0;
-#pragma unroll
-    for (int q = 0; q < NWAVE; ++q) dot += red[q];
-    // (d) w = p - (tau / 2)(p . v) v
-    if (tid < m) w[tid] += -0.5 * t * dot * v[tid];
-    __syncthreads();
-    // (e) A' -= v w^T + w v^T (lower), one wave per column
-    for (int c = wave; c < m; c += NWAVE) {
-      const double vc = v[c], wc = w[c];
-      const int base = tri_off(n, i + 1 + c) - c;
-      for (int r = c + lane; r < m; r += 64) P[base + r] -= v[r] * wc + w[r] * vc;
-    }
-    __syncthreads();
-  }
-  // outputs: d, e, tau and the reflectors V (column-major n x n)
-  for (int q = tid; q < n; q += TRI_THREADS) {
-    if (q + 2 < n) {
-      d[q] = dl[q];
-      e[q] = el[q];
-      tau[q] = tl[q];
-    } else if (q + 2 == n) {
-      d[q] = P[tri_off(n, q)];
-      e[q] = P[tri_off(n, q) + 1];
-      tau[q] = 0.0;
-    } else {
-      d[q] = P[tri_off(n, q)];
-    }
-  }
-  for (int idx = tid; idx < n * n; idx += TRI_THREADS) {
-    const int r = idx % n, c = idx / n;
-    double val = 0.0;
-    if (c + 2 < n && r > c) val = P[tri_off(n, c) + r - c];  // v with v_0 = 1 at r = c + 1
-    V[idx] = val;
-  }
-}
-
-// A packed lower triangle holding A(i0.., i0..) of an n x n matrix:
-// A(r, c), r >= c >= i0, at P[tri_off(n - i0, c - i0) + r - c].
-struct TriView {
-  double* P;
-  int np, i0;
-  // column c: A(r, c) = col(c)[r] for r >= c
-  __device__ double* col(int c) const { return P + tri_off(np, c - i0) - c; }
-};
-
-constexpr int TRB_NWAVE = TRI_THREADS / 64;
-constexpr int TRB_RW = (TRI_BIG_MAXN + 63) / 64;  // rows per lane
-constexpr int TRB_U = 4;                          // columns per wave in flight
-
-struct TrbShared {
-  double v[2][TRI_BIG_MAXN], w[2][TRI_BIG_MAXN];  // (v, w) of steps j (buffer j & 1) and j - 1
-  double tp[TRI_BIG_MAXN];
-  double part[TRB_NWAVE][TRI_BIG_MAXN];  // per-wave lower-part partials of p
-  double red[TRB_NWAVE];
-  double sc[2];                          // tau, scale
-};
-
-// Reflector of step j from column j of A.  With `upd`, column j still lacks
-// step j - 1's update A <- A - v w^T - w v^T (vp, wp: step j - 1's vectors,
-// local index = row - j), applied here.  Wave 0 does the whole step: its
-// lanes hold the column's rows in registers (the update, the norm, the
-// scaling), lane 0 forms tau and the scale, and the reflector goes to vj
-// (local index = row - j - 1) and to column j of V; column j of A is not read
-// again, so it is not written.  tau_j is returned through sh.sc.
-__device__ void trb_reflector(const TriView& A, int n, int j, bool upd, const double* vp, const double* wp,
-                              double* vj, TrbShared& sh, double* __restrict__ d, double* __restrict__ e,
-                              double* __restrict__ tau, double* __restrict__ V) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const double* cj = A.col(j);
+  // col[i] = A(j + i, j), i < n - j
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nj = n - j;
   if (wave == 0) {
     const double vp0 = upd ? vp[0] : 0.0, wp0 = upd ? wp[0] : 0.0;
-    double xs[TRB_RW];
+    double xs[SQ_RW];
     double x2 = 0.0;
 #pragma unroll
-    for (int q = 0; q < TRB_RW; ++q) {
-      const int r = j + 2 + lane + 64 * q;
+    for (int q = 0; q < SQ_RW; ++q) {
+      const int i = 2 + lane + 64 * q;
       double x = 0.0;
-      if (r < n) {
-        x = cj[r];
-        if (upd) x = x - (vp[r - j] * wp0 + wp[r - j] * vp0);
+      if (i < nj) {
+        x = col[i];
+        if (upd) x = x - (vp[i] * wp0 + wp[i] * vp0);
         x2 += x * x;
       }
       xs[q] = x;
@@ -1556,7 +1408,7 @@ __device__ void trb_reflector(const TriView& A, int n, int j, bool upd, const do
     x2 = lane_value_d(wave_sum_dpp63(x2), 63);
     double scale = 0.0;
     if (lane == 0) {
-      double a0 = cj[j], alpha = cj[j + 1];
+      double a0 = col[0], alpha = col[1];
       if (upd) {
         a0 = a0 - (vp[0] * wp0 + wp[0] * vp0);
         alpha = alpha - (vp[1] * wp0 + wp[1] * vp0);
@@ -1568,7 +1420,6 @@ __device__ void trb_reflector(const TriView& A, int n, int j, bool upd, const do
         scale = 1.0 / (alpha - beta);
       }
       sh.sc[0] = t;
-      sh.sc[1] = scale;
       d[j] = a0;
       e[j] = beta;
       tau[j] = t;
@@ -1577,137 +1428,233 @@ __device__ void trb_reflector(const TriView& A, int n, int j, bool upd, const do
     }
     scale = lane_value_d(scale, 0);
 #pragma unroll
-    for (int q = 0; q < TRB_RW; ++q) {
-      const int r = j + 2 + lane + 64 * q;
-      if (r < n) {
+    for (int q = 0; q < SQ_RW; ++q) {
+      const int i = 2 + lane + 64 * q;
+      if (i < nj) {
         const double vr = xs[q] * scale;
-        vj[r - j - 1] = vr;
-        V[r + (int64_t)j * n] = vr;
+        vj[i - 1] = vr;
+        V[(j + i) + (int64_t)j * n] = vr;
       }
     }
   }
   __syncthreads();
 }
 
-// One pass over the trailing block A' = A(j+1 .., j+1 ..) of step j (local
-// indices r, c < m = n - 1 - j): with `upd`, each element first takes step
-// j - 1's update (vp, wp at local index + 1) and is written back; then
-// p = tau_j A' v_j accumulates from the updated value (lane rows the lower
-// part, a wave sum per column its transposed part), and
-// w_j = p - (tau_j / 2)(p . v_j) v_j goes to wj.  Each column of A' is read
-// and written once per step; TRB_U columns per wave in flight.
-__device__ void trb_pass(const TriView& A, int n, int j, bool upd, const double* vp, const double* wp,
-                         const double* vj, double* wj, TrbShared& sh) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m = n - 1 - j;
+// pass of step j over the trailing square (local rows / columns 0 .. m-1,
+// m = n-1-j; element (r, c) at base[r + c * ld]); QB = 64-row blocks per
+// column (uniform), RW its compile-time bound.  Rows r >= m of the last block
+// are padding: with JUNK they go to junk[lane] instead (the square's columns
+// have no padding rows), else the storage has 64 padding rows per column.
+template <int RW, bool JUNK, typename P>
+__device__ __attribute__((always_inline)) void sq_pass(P base, int64_t ld, int m, bool upd, const double* vp,
+                                                       const double* wp, const double* vj, double* wj,
+                                                       double* part, int ldp, P junk, SqShared& sh) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int QB = (m + 63) >> 6;
   const double t = sh.sc[0];
-  double pp[TRB_RW];
+  double pp[RW], vpr[RW], wpr[RW];
+  bool pad[RW];
 #pragma unroll
-  for (int q = 0; q < TRB_RW; ++q) pp[q] = 0.0;
-  for (int c0 = wave; c0 < m; c0 += TRB_U * TRB_NWAVE) {
-    double a[TRB_U][TRB_RW];
+  for (int q = 0; q < RW; ++q) {
+    pp[q] = 0.0;
+    const int r = lane + 64 * q;
+    pad[q] = r >= m;
+    vpr[q] = (upd && q < QB) ? vp[min(r + 1, m)] : 0.0;
+    wpr[q] = (upd && q < QB) ? wp[min(r + 1, m)] : 0.0;
+  }
+  for (int c0 = wave; c0 < m; c0 += SQ_U * SQ_NWAVE) {
+    double a[SQ_U][RW];
 #pragma unroll
-    for (int u = 0; u < TRB_U; ++u) {
-      const int c = c0 + u * TRB_NWAVE;
-      const double* col = c < m ? A.col(j + 1 + c) + (j + 1) : nullptr;
+    for (int u = 0; u < SQ_U; ++u) {
+      const int c = min(c0 + u * SQ_NWAVE, m - 1);  // past the end: a repeat of the last column, not used
+      P col = base + (int64_t)c * ld + lane;
 #pragma unroll
-      for (int q = 0; q < TRB_RW; ++q) {
-        const int r = lane + 64 * q;
-        a[u][q] = (c < m && r >= c && r < m) ? col[r] : 0.0;
+      for (int q = 0; q < RW; ++q) {
+        P p = col + 64 * q;
+        if (JUNK && pad[q]) p = junk;
+        a[u][q] = q < QB ? *p : 0.0;
       }
     }
 #pragma unroll
-    for (int u = 0; u < TRB_U; ++u) {
-      const int c = c0 + u * TRB_NWAVE;
+    for (int u = 0; u < SQ_U; ++u) {
+      const int c = c0 + u * SQ_NWAVE;
       if (c >= m) break;  // uniform
+      P col = base + (int64_t)c * ld + lane;
       if (upd) {
-        double* col = A.col(j + 1 + c) + (j + 1);
         const double vc = vp[c + 1], wc = wp[c + 1];
 #pragma unroll
-        for (int q = 0; q < TRB_RW; ++q) {
-          const int r = lane + 64 * q;
-          if (r >= c && r < m) {
-            a[u][q] = a[u][q] - (vp[r + 1] * wc + wp[r + 1] * vc);
-            col[r] = a[u][q];
+        for (int q = 0; q < RW; ++q)
+          if (q < QB) {
+            a[u][q] = a[u][q] - (vpr[q] * wc + wpr[q] * vc);
+            P p = col + 64 * q;
+            if (JUNK && pad[q]) p = junk;
+            *p = a[u][q];
           }
-        }
       }
       const double vc = vj[c];
-      double s = 0.0;
 #pragma unroll
-      for (int q = 0; q < TRB_RW; ++q) {
-        const int r = lane + 64 * q;
-        pp[q] += a[u][q] * vc;
-        if (r > c && r < m) s += a[u][q] * vj[r];
-      }
-      s = wave_sum_dpp63(s);
-      if (lane == 63) sh.tp[c] = s;
+      for (int q = 0; q < RW; ++q) pp[q] += a[u][q] * vc;
     }
   }
 #pragma unroll
-  for (int q = 0; q < TRB_RW; ++q) {
+  for (int q = 0; q < RW; ++q) {
     const int r = lane + 64 * q;
-    if (r < m) sh.part[wave][r] = pp[q];
+    if (q < QB && r < m) part[wave * ldp + r] = pp[q];
   }
   __syncthreads();
-  // p and p . v
   double prod = 0.0;
-  if (tid < m) {
-    double pr = sh.tp[tid];
+  for (int r = tid; r < m; r += TRI_THREADS) {
+    double pr = 0.0;
 #pragma unroll
-    for (int q = 0; q < TRB_NWAVE; ++q) pr += sh.part[q][tid];
+    for (int w = 0; w < SQ_NWAVE; ++w) pr += part[w * ldp + r];
     pr *= t;
-    wj[tid] = pr;
-    prod = pr * vj[tid];
+    wj[r] = pr;
+    prod += pr * vj[r];
   }
   prod = wave_sum_dpp63(prod);
   if (lane == 63) sh.red[wave] = prod;
   __syncthreads();
   double dot = 0.0;
 #pragma unroll
-  for (int q = 0; q < TRB_NWAVE; ++q) dot += sh.red[q];
-  // w = p - (tau / 2)(p . v) v
-  if (tid < m) wj[tid] += -0.5 * t * dot * vj[tid];
+  for (int w = 0; w < SQ_NWAVE; ++w) dot += sh.red[w];
+  for (int r = tid; r < m; r += TRI_THREADS) wj[r] += -0.5 * t * dot * vj[r];
   __syncthreads();
 }
 
-// n <= 320: the triangle in the workspace W (n (n + 1) / 2 doubles) while the
-// block A(j.., j..) a step touches exceeds TRI_LDS_M + 1 a side, then in LDS.
-// Step j's update of the trailing block is deferred into step j + 1's pass
-// (column j + 1 first, by the wave that forms reflector j + 1).
-__global__ __launch_bounds__(TRI_THREADS) void sytrd_big_kernel(int n, const double* __restrict__ G, int64_t ldg,
-                                                                double* __restrict__ d, double* __restrict__ e,
-                                                                double* __restrict__ tau, double* __restrict__ V,
-                                                                double* __restrict__ W) {
-  constexpr int NL = TRI_LDS_M + 1;
-  __shared__ double PL[NL * (NL + 1) / 2];
-  __shared__ TrbShared sh;
+// pass of step j over the lower triangle of the trailing square kept in
+// global memory with 64 padding rows per column (element (r, c) at
+// base[r + c * ld], local r, c < m): rows r >= c of column c, the 64-row
+// blocks wholly above the diagonal skipped (uniform), the diagonal block's
+// rows above it read and written back unchanged-in-effect (their values are
+// never used: only the lower triangle is kept) and masked out of the sums.
+// p = tau A v takes the lower part per lane row and each column's transposed
+// part by a wave sum.
+template <int RW, typename P>
+__device__ __attribute__((always_inline)) void sq_pass_lower(P base, int64_t ld, int m, bool upd, const double* vp,
+                                                             const double* wp, const double* vj, double* wj,
+                                                             double* part, int ldp, double* tp, SqShared& sh) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const double t = sh.sc[0];
+  double pp[RW], vpr[RW], wpr[RW], vjr[RW];
+#pragma unroll
+  for (int q = 0; q < RW; ++q) {
+    pp[q] = 0.0;
+    const int r = lane + 64 * q;
+    vpr[q] = upd ? vp[min(r + 1, m)] : 0.0;
+    wpr[q] = upd ? wp[min(r + 1, m)] : 0.0;
+    vjr[q] = r < m ? vj[r] : 0.0;
+  }
+  const int QB = (m + 63) >> 6;
+  for (int c0 = wave; c0 < m; c0 += SQ_U * SQ_NWAVE) {
+    double a[SQ_U][RW];
+#pragma unroll
+    for (int u = 0; u < SQ_U; ++u) {
+      const int c = min(c0 + u * SQ_NWAVE, m - 1);
+      P col = base + (int64_t)c * ld + lane;
+#pragma unroll
+      for (int q = 0; q < RW; ++q) {
+        a[u][q] = 0.0;
+        if (q < QB && 64 * q + 63 >= c) a[u][q] = col[64 * q];  // uniform
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SQ_U; ++u) {
+      const int c = c0 + u * SQ_NWAVE;
+      if (c >= m) break;  // uniform
+      P col = base + (int64_t)c * ld + lane;
+      const double vc = vj[c];
+      double s = 0.0;
+      const double vpc = upd ? vp[c + 1] : 0.0, wpc = upd ? wp[c + 1] : 0.0;
+#pragma unroll
+      for (int q = 0; q < RW; ++q) {
+        if (q >= QB || 64 * q + 63 < c) continue;  // uniform
+        const int r = lane + 64 * q;
+        if (upd) {
+          a[u][q] = a[u][q] - (vpr[q] * wpc + wpr[q] * vpc);
+          col[64 * q] = a[u][q];
+        }
+        const double x = (r >= c) ? a[u][q] : 0.0;
+        pp[q] += x * vc;
+        s += (r > c ? x : 0.0) * vjr[q];
+      }
+      s = wave_sum_dpp63(s);
+      if (lane == 63) tp[c] = s;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < RW; ++q) {
+    const int r = lane + 64 * q;
+    if (q < QB && r < m) part[wave * ldp + r] = pp[q];
+  }
+  __syncthreads();
+  double prod = 0.0;
+  for (int r = tid; r < m; r += TRI_THREADS) {
+    double pr = tp[r];
+#pragma unroll
+    for (int w = 0; w < SQ_NWAVE; ++w) pr += part[w * ldp + r];
+    pr *= t;
+    wj[r] = pr;
+    prod += pr * vj[r];
+  }
+  prod = wave_sum_dpp63(prod);
+  if (lane == 63) sh.red[wave] = prod;
+  __syncthreads();
+  double dot = 0.0;
+#pragma unroll
+  for (int w = 0; w < SQ_NWAVE; ++w) dot += sh.red[w];
+  for (int r = tid; r < m; r += TRI_THREADS) wj[r] += -0.5 * t * dot * vj[r];
+  __syncthreads();
+}
+
+// G (lower triangle read) -> d, e, tau, V (reflectors, column-major n x n).
+// W: for n > TRI_SQ_LDS, (n + 64) x n doubles, the square with ld = n + 64
+// (rows n .. n+63 padding); unused otherwise.
+__global__ __launch_bounds__(TRI_THREADS) void sytrd_kernel(int n, const double* __restrict__ G, int64_t ldg,
+                                                          double* __restrict__ d, double* __restrict__ e,
+                                                          double* __restrict__ tau, double* __restrict__ V,
+                                                          double* __restrict__ W) {
+  __shared__ double SQ[SQ_LM * SQ_LM];
+  __shared__ SqShared sh;
   const int tid = threadIdx.x;
-  for (int c = 0; c < n; ++c)
-    for (int r = c + tid; r < n; r += TRI_THREADS) W[tri_off(n, c) + r - c] = G[r + c * ldg];
+  const int64_t ldw = n + 64;
+  if (n > SQ_LM)
+    for (int64_t idx = tid; idx < (int64_t)n * n; idx += TRI_THREADS) {
+      const int r = (int)(idx % n), c = (int)(idx / n);
+      W[r + c * ldw] = r >= c ? G[r + c * ldg] : G[c + r * ldg];
+    }
   for (int idx = tid; idx < n * n; idx += TRI_THREADS) V[idx] = 0.0;
   __syncthreads();
-  TriView A{W, n, 0};
-  bool in_lds = false;
   int j = 0;
-  for (; j + 2 < n; ++j) {
-    if (!in_lds && n - j <= NL) {
-      const int nl = n - j;
-      for (int c = j; c < n; ++c)
-        for (int r = c + tid; r < n; r += TRI_THREADS) PL[tri_off(nl, c - j) + r - c] = W[tri_off(n, c) + r - c];
-      __syncthreads();
-      A = TriView{PL, nl, j};
-      in_lds = true;
-    }
+  for (; j + 2 < n && n - j > SQ_LM; ++j) {
     const int b = j & 1;
     const bool upd = j > 0;
-    trb_reflector(A, n, j, upd, sh.v[b ^ 1], sh.w[b ^ 1], sh.v[b], sh, d, e, tau, V);
-    trb_pass(A, n, j, upd, sh.v[b ^ 1], sh.w[b ^ 1], sh.v[b], sh.w[b], sh);
+    sq_reflector(W + j + (int64_t)j * ldw, n, j, upd, sh.v[b ^ 1], sh.w[b ^ 1], sh.v[b], sh, d, e, tau, V);
+    sq_pass_lower<SQ_RW>(W + (j + 1) + (int64_t)(j + 1) * ldw, ldw, n - 1 - j, upd, sh.v[b ^ 1], sh.w[b ^ 1],
+                         sh.v[b], sh.w[b], SQ, TRI_BIG_MAXN, SQ + SQ_NWAVE * TRI_BIG_MAXN, sh);
+  }
+  // the trailing square A(j.., j..) into LDS, local (r, c) at SQ[r + c * SQ_LM]
+  const int nl = n - j;
+  for (int idx = tid; idx < nl * nl; idx += TRI_THREADS) {
+    const int r = idx % nl, c = idx / nl;
+    if (n <= SQ_LM)  // j = 0: straight from G
+      SQ[r + c * SQ_LM] = r >= c ? G[r + c * ldg] : G[c + r * ldg];
+    else
+      SQ[r + c * SQ_LM] = r >= c ? W[(j + r) + (int64_t)(j + c) * ldw] : W[(j + c) + (int64_t)(j + r) * ldw];
+  }
+  __syncthreads();
+  for (; j + 2 < n; ++j) {
+    const int b = j & 1;
+    const bool upd = j > 0;
+    const int o = j - (n - nl);  // local index of row / column j
+    sq_reflector(SQ + o + o * SQ_LM, n, j, upd, sh.v[b ^ 1], sh.w[b ^ 1], sh.v[b], sh, d, e, tau, V);
+    sq_pass<2, true>(SQ + (o + 1) + (o + 1) * SQ_LM, SQ_LM, n - 1 - j, upd, sh.v[b ^ 1], sh.w[b ^ 1], sh.v[b],
+                     sh.w[b], &sh.part_l[0][0], SQ_LM, sh.junk + (threadIdx.x & 63), sh);
   }
   if (tid == 0) {
     if (n >= 2) {
-      double a00 = A.col(n - 2)[n - 2], a10 = A.col(n - 2)[n - 1], a11 = A.col(n - 1)[n - 1];
-      if (j > 0) {  // step j - 1 = n - 3's update of A(n-2.., n-2..), local index = row - (n - 2)
+      const int o = (n - 2) - (n - nl);
+      double a00 = SQ[o + o * SQ_LM], a10 = SQ[(o + 1) + o * SQ_LM], a11 = SQ[(o + 1) + (o + 1) * SQ_LM];
+      if (j > 0) {
         const double* vp = sh.v[(j - 1) & 1];
         const double* wp = sh.w[(j - 1) & 1];
         a00 = a00 - (vp[0] * wp[0] + wp[0] * vp[0]);
@@ -1719,7 +1666,7 @@ __global__ __launch_bounds__(TRI_THREADS) void sytrd_big_kernel(int n, const dou
       tau[n - 2] = 0.0;
       d[n - 1] = a11;
     } else {
-      d[0] = A.col(0)[0];
+      d[0] = SQ[0];
     }
   }
 }
@@ -1895,17 +1842,14 @@ void sytrd_q(Ctx& c, int n, const double* G, int64_t ldg, double* d, double* e, 
   static_assert(TRI_BIG_MAXN == TRI_MAXN_HOST, "sytrd_q size limit");
   IPEPS_CHECK(n >= 1 && n <= TRI_BIG_MAXN, "sytrd_q: n out of range");
   const bool big = n > TRI_MAXN;
-  const size_t nws = (size_t)n * n + n + (big ? (size_t)n * (n + 1) / 2 : 0);
+  const size_t nws = (size_t)n * n + n + (n > TRI_SQ_LDS ? (size_t)(n + 64) * n : 0);
   std::shared_ptr<Buf> ws(new Buf{c.pool->alloc(nws * 8), nws * 8, c.pool.get()});
   double* V = (double*)ws->p;
   double* tau = V + (size_t)n * n;
-  if (big) {
-    double* W = tau + n;
-    TimedLaunch tl(c, "sytrd_big_kernel", 4.0 / 3.0 * n * n * n, 0.0);
-    hipLaunchKernelGGL(sytrd_big_kernel, dim3(1), dim3(TRI_THREADS), 0, c.stream, n, G, ldg, d, e, tau, V, W);
-  } else {
+  double* W = n > TRI_SQ_LDS ? tau + n : nullptr;
+  {
     TimedLaunch tl(c, "sytrd_kernel", 4.0 / 3.0 * n * n * n, 0.0);
-    hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(TRI_THREADS), 0, c.stream, n, G, ldg, d, e, tau, V);
+    hipLaunchKernelGGL(sytrd_kernel, dim3(1), dim3(TRI_THREADS), 0, c.stream, n, G, ldg, d, e, tau, V, W);
   }
   HIP_CHECK(hipGetLastError());
   {

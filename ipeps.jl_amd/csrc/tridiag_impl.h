@@ -1,27 +1,27 @@
 // Householder reduction of a small real symmetric matrix to tridiagonal form
 // in one workgroup (FP64, n <= 320), and the formation of the orthogonal
-// factor: the storage scheme and index helpers of the device kernels in
-// elem.hip (sytrd_kernel for n <= 160, sytrd_big_kernel for n <= 320,
-// orgtr_kernel).
+// factor: constants and index helpers of the device kernels in elem.hip
+// (sytrd_kernel, orgtr_kernel; the packed-triangle helpers serve the Jacobi
+// eigensolver there).
 //
-// Lower storage, LAPACK dsytd2 'L' convention: for i = 0 .. n-3 the reflector
+// LAPACK dsytd2 'L' convention: for i = 0 .. n-3 the reflector
 // H(i) = I - tau_i v v^T (v_0 = 1) maps A(i+1:n, i) to e_i e_1 and is applied
 // to the trailing block from both sides,
-//   p = tau A v,  w = p - (tau / 2)(p . v) v,  A <- A - v w^T - w v^T,
-// the product and the update column by column (one wave per column).
-// The lower triangle lives in LDS packed by columns: (r, c), r >= c, at
-// tri_off(n, c) + r - c.  The reflectors go to global memory as a
-// column-major n x n array V (V(i+1, i) = 1, V(r, i) = v_{r-i-1}) and tau.
+//   p = tau A v,  w = p - (tau / 2)(p . v) v,  A <- A - v w^T - w v^T.
+// The reflectors go to global memory as a column-major n x n array V
+// (V(i+1, i) = 1, V(r, i) = v_{r-i-1}) and tau.
 //
-// sytrd_big_kernel keeps the triangle in a global-memory workspace (the same
-// packing, read and written through the L2 of the workgroup's XCD) while the
-// block a step touches is larger than TRI_LDS_M + 1, then moves A(j.., j..)
-// into LDS (packed with its own dimension) for the remaining steps.  Step
-// j - 1's rank-2 update is deferred into step j: column j is updated by the
-// wave that forms reflector j, and the rest of the trailing block in the
-// same pass that forms p = tau A v, so each element is read and written once
-// per step.  Lane rows accumulate the lower part of p and a wave sum gives
-// each column's transposed part.
+// sytrd_kernel stores the matrix as a square.  While the trailing block is
+// larger than TRI_SQ_LDS a side it lives in a global-memory workspace (ld =
+// n + 64: each column has 64 padding rows, so a lane's 64-row block never
+// needs a bounds test), read and written through the L2 of the workgroup's
+// XCD, and only its lower triangle is kept: lane rows accumulate the lower
+// part of p, a wave sum per column its transposed part.  Then the trailing
+// square (both triangles, TRI_SQ_LDS a side at most) moves to LDS, where p is
+// a plain row-per-lane product with no per-column wave sums.  Step j - 1's
+// rank-2 update is deferred into step j: column j is updated by the wave that
+// forms reflector j, the rest of the trailing block in the same pass that
+// forms p = tau A v, so each element is read and written once per step.
 //
 // Q = H(0) H(1) ... H(n-3) acts on rows / columns 1 .. n-1 (Q(0, 0) = 1).
 // Column c of Q is independent of the others: starting from e_c, apply
@@ -38,9 +38,9 @@
 namespace ipeps {
 namespace k {
 
-constexpr int TRI_MAXN = 160;
+constexpr int TRI_MAXN = 160;  // orgtr_kernel: 3 or 5 row blocks per lane
 constexpr int TRI_BIG_MAXN = 320;
-constexpr int TRI_LDS_M = 159;  // sytrd_big_kernel works in LDS once A(j.., j..) is at most TRI_LDS_M + 1 a side
+constexpr int TRI_SQ_LDS = 128;  // sytrd_kernel works in LDS once the trailing square is at most this a side
 constexpr int TRI_THREADS = 1024;
 
 IPEPS_HD inline int tri_off(int n, int c) { return c * n - c * (c - 1) / 2; }

@@ -135,10 +135,10 @@ def test_svd_thin_methods(pkg, cplx, decay):
         assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-12
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 33, 64, 160, 161, 224, 288, 320])
+@pytest.mark.parametrize("n", [1, 2, 3, 33, 64, 127, 128, 129, 130, 160, 161, 224, 288, 320])
 def test_svd_thin_gram_sizes(pkg, n):
     """Gram route (device Householder tridiagonalisation + Q, rocSOLVER stedc)
-    at the sizes it serves (k <= 160 in LDS, k <= 320 starting from a global
+    at the sizes it serves (k <= 128 in LDS, k <= 320 starting from a global
     workspace; tile, wave and phase edges): singular values to 1e-13 of S_max
     against numpy, A = U S V^T, orthonormal U and V, and the kernels that ran."""
     rng = np.random.default_rng(n)
@@ -151,7 +151,7 @@ def test_svd_thin_gram_sizes(pkg, n):
     st = ctx.kernel_stats(reset=True)
     ctx.close()
     assert used == 1
-    assert ("sytrd_kernel" if n <= 160 else "sytrd_big_kernel") in st
+    assert "sytrd_kernel" in st
     assert "rocsolver_stedc" in st and "rocsolver_syevd" not in st
     assert np.abs(S - s0).max() < 1e-13
     assert np.abs((U * S) @ V.T - A).max() < 1e-12
