@@ -230,20 +230,21 @@ void bottom_rg(Ctx& c, Env& e, int64_t chi, uint64_t draw, Proj* only = nullptr)
 // E1/E3 (left, right, ket, bra), E2/E4 (up, down, ket, bra);
 // A (up, left, down, right, phys).
 
-// (e1r, e4d, A_d, A_r, Ad_d, Ad_r, p, p')
+// (e1r, e4d, A_d, A_r, p, Ad_d, Ad_r, p'): the last product's own output
+// order (its first operand's free legs, then the second's), so no permute
 Tensor corner_ul(Ctx& c, const Tensor& C1, const Tensor& E1, const Tensor& E4, const Tensor& A, const Tensor& Ad) {
   Tensor C1E1 = tcon(c, C1, {-1, 1}, E1, {1, -2, -3, -4});
   Tensor X1 = tcon(c, C1E1, {1, -1, -2, -3}, E4, {1, -4, -5, -6});
   Tensor X2 = tcon(c, X1, {-1, 1, -2, -3, 2, -4}, A, {1, 2, -5, -6, -7});
-  return tcon(c, X2, {-1, 1, -2, 2, -3, -4, -7}, Ad, {1, 2, -5, -6, -8});
+  return tcon(c, X2, {-1, 1, -2, 2, -3, -4, -5}, Ad, {1, 2, -6, -7, -8});
 }
 
-// (e3l, e2u, A_u, A_l, Ad_u, Ad_l, p, p')
+// (e3l, e2u, A_u, A_l, p, Ad_u, Ad_l, p'), likewise
 Tensor corner_lr(Ctx& c, const Tensor& C3, const Tensor& E3, const Tensor& E2, const Tensor& A, const Tensor& Ad) {
   Tensor C3E3 = tcon(c, C3, {-1, 1}, E3, {-2, 1, -3, -4});
   Tensor Y1 = tcon(c, C3E3, {1, -1, -2, -3}, E2, {-4, 1, -5, -6});
   Tensor Y2 = tcon(c, Y1, {-1, 1, -2, -3, 2, -4}, A, {-5, -6, 1, 2, -7});
-  return tcon(c, Y2, {-1, 1, -2, 2, -3, -4, -7}, Ad, {-5, -6, 1, 2, -8});
+  return tcon(c, Y2, {-1, 1, -2, 2, -3, -4, -5}, Ad, {-6, -7, 1, 2, -8});
 }
 
 // rho from two halves (x1, x2, a, ad, p, p') sharing their first four legs.
@@ -343,20 +344,20 @@ void get_dms(Ctx& c, const Env& e, Tensor& roh, Tensor& rov) {
     Tensor UL = corner_ul(cc, C[0], E[0], E[3], e.A, e.Ad);
     // left half of rho_h: UL closed with C4 E3 -> (e1r, e3r, A_r, Ad_r, p, p')
     Tensor C4E3 = tcon(cc, C[3], {-1, 1}, E[2], {1, -2, -3, -4});
-    Lh = tcon(cc, UL, {-1, 1, 2, -3, 3, -4, -5, -6}, C4E3, {1, -2, 2, 3});
+    Lh = tcon(cc, UL, {-1, 1, 2, -3, -5, 3, -4, -6}, C4E3, {1, -2, 2, 3});
     // upper half of rho_v: UL closed with C2 E2 -> (e4d, e2d, A_d, Ad_d, p, p')
     Tensor C2E2 = tcon(cc, C[1], {-1, 1}, E[1], {1, -2, -3, -4});
-    Uv = tcon(cc, UL, {1, -1, -3, 2, -4, 3, -5, -6}, C2E2, {1, -2, 2, 3});
+    Uv = tcon(cc, UL, {1, -1, -3, 2, -5, -4, 3, -6}, C2E2, {1, -2, 2, 3});
   };
   // the lower-right corner and the halves it closes
   auto lr_side = [&](Ctx& cc) {
     Tensor LR = corner_lr(cc, C[2], E[2], E[1], e.A, e.Ad);
     // right half of rho_h: LR closed with C2 E1 -> (e1l, e3l, A_l, Ad_l, p, p')
     Tensor C2E1 = tcon(cc, C[1], {1, -2}, E[0], {-1, 1, -3, -4});
-    Rh = tcon(cc, LR, {-2, 1, 2, -3, 3, -4, -5, -6}, C2E1, {-1, 1, 2, 3});
+    Rh = tcon(cc, LR, {-2, 1, 2, -3, -5, 3, -4, -6}, C2E1, {-1, 1, 2, 3});
     // lower half of rho_v: LR closed with C4 E4 -> (e4u, e2u, A_u, Ad_u, p, p')
     Tensor C4E4 = tcon(cc, C[3], {1, -2}, E[3], {-1, 1, -3, -4});
-    Bv = tcon(cc, LR, {1, -2, -3, 2, -4, 3, -5, -6}, C4E4, {-1, 1, 2, 3});
+    Bv = tcon(cc, LR, {1, -2, -3, 2, -5, -4, 3, -6}, C4E4, {-1, 1, 2, 3});
   };
   if (e.opt.paired_moves) {
     // the two sides share only their inputs: second lane for the lower right
