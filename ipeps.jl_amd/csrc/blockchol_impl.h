@@ -10,7 +10,8 @@
 //
 // Cholesky, right-looking over tile columns K:
 //  (a) one wave factors the diagonal tile in registers (lane i holds row i,
-//      columns broadcast with readlane) and replaces it by its inverse Dinv_K;
+//      columns broadcast with readlane; 1 / sqrt of each pivot by v_rsq_f64
+//      and Newton steps) and replaces it by its inverse Dinv_K;
 //  (b) the panel below: L_IK = A_IK Dinv_K^T, a wave per tile;
 //  (c) the trailing tiles: A_IJ -= L_IK L_JK^T for K < J <= I, a wave per tile.
 // Inverse X = L^-1, over tile columns J = NB-2 .. 0, a wave per row I > J:
@@ -41,23 +42,6 @@ constexpr int BCH_THREADS = 512;
 
 IPEPS_HD inline int bch_tix(int I, int J) { return I * (I + 1) / 2 + J; }
 IPEPS_HD inline int bch_at(int I, int J, int r, int c) { return bch_tix(I, J) * BCH_TE + c * BCH_LD + r; }
-
-// load the lower tiles of G (padding: identity)
-IPEPS_HD inline void bch_load(int n, int NB, const double* G, int64_t ld, double* T, int tid, int nt) {
-  const int nel = NB * (NB + 1) / 2 * BCH_T * BCH_T;
-  for (int e = tid; e < nel; e += nt) {
-    const int t = e / (BCH_T * BCH_T), rc = e % (BCH_T * BCH_T);
-    int I = 0;
-    while ((I + 1) * (I + 2) / 2 <= t) ++I;
-    const int J = t - I * (I + 1) / 2;
-    const int r = rc % BCH_T, c = rc / BCH_T;
-    const int gi = I * BCH_T + r, gj = J * BCH_T + c;
-    double v;
-    if (gi < n && gj < n) v = G[gi + gj * ld];
-    else v = gi == gj ? 1.0 : 0.0;
-    T[bch_at(I, J, r, c)] = v;
-  }
-}
 
 }  // namespace k
 }  // namespace ipeps

@@ -1164,12 +1164,23 @@ __device__ inline double readlane_d(double v, int l) {
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
+// 1 / sqrt(p), p > 0: v_rsq_f64 and two Newton steps
+__device__ inline double rsqrt_nr(double p) {
+  double r = __builtin_amdgcn_rsq(p);
+  const double h = 0.5 * p;
+  r = r * (1.5 - h * r * r);
+  r = r * (1.5 - h * r * r);
+  return r;
+}
+
 // (a) diagonal tile K by wave 0: lane i < 16 holds row i; the tile is replaced
-// by the inverse of its Cholesky factor (zeros above the diagonal).  Returns
-// the 1-based failing pivot within the tile, 0 on success (uniform).
+// by the inverse of its Cholesky factor (zeros above the diagonal), its rows
+// formed from the right with 1 / L[m][m] from the pivots' reciprocal square
+// roots (no divisions) and L's rows read back from the tile as broadcasts.
+// Returns the 1-based failing pivot within the tile, 0 on success (uniform).
 __device__ inline int bch_diag(double* tile, int lane) {
   const int i = lane & (BCH_T - 1);
-  double c[BCH_T];
+  double c[BCH_T], dinv[BCH_T];
 #pragma unroll
   for (int j = 0; j < BCH_T; ++j) c[j] = tile[j * BCH_LD + i];
   int fail = 0;
@@ -1179,26 +1190,34 @@ __device__ inline int bch_diag(double* tile, int lane) {
     if (!(pk > 0.0) || !(pk < 1.7e308)) {
       if (!fail) fail = k + 1;
     }
-    const double sk = sqrt(fabs(pk)), rk = 1.0 / sk;
+    const double ap = fabs(pk) > 0.0 ? fabs(pk) : 1.0;
+    const double rk = rsqrt_nr(ap), sk = ap * rk;
+    dinv[k] = rk;
     c[k] = i == k ? sk : c[k] * rk;  // L[i][k] (rows below k; rows above hold garbage)
 #pragma unroll
     for (int j = k + 1; j < BCH_T; ++j) c[j] -= c[k] * readlane_d(c[k], j);
   }
-  // row i of L^-1: x[j] = -(sum_{m=j+1..i} x[m] L[m][j]) / L[j][j], x[i] = 1 / L[i][i]
-  double x[BCH_T];
-#pragma unroll
-  for (int j = 0; j < BCH_T; ++j) x[j] = 0.0;
-#pragma unroll
-  for (int j = BCH_T - 1; j >= 0; --j) {
-    double sum = 0.0;
-#pragma unroll
-    for (int m = j + 1; m < BCH_T; ++m) sum += x[m] * readlane_d(c[j], m);
-    const double ljj = readlane_d(c[j], j);
-    x[j] = i == j ? 1.0 / ljj : (i > j ? -sum / ljj : 0.0);
-  }
+  // L to the tile (lane i: row i), then row i of L^-1 by columns from the
+  // right: x[m] = (i == m ? 1 : -acc[m]) / L[m][m] (0 above the diagonal),
+  // then acc[j] += x[m] L[m][j] for every j < m (independent updates)
   if (lane < BCH_T) {
 #pragma unroll
-    for (int j = 0; j < BCH_T; ++j) tile[j * BCH_LD + i] = x[j];
+    for (int j = 0; j < BCH_T; ++j) tile[j * BCH_LD + i] = c[j];
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  double acc[BCH_T];
+#pragma unroll
+  for (int j = 0; j < BCH_T; ++j) acc[j] = 0.0;
+#pragma unroll
+  for (int m = BCH_T - 1; m >= 0; --m) {
+    const double xm = i == m ? dinv[m] : (i > m ? -acc[m] * dinv[m] : 0.0);
+#pragma unroll
+    for (int j = 0; j < m; ++j) acc[j] += xm * tile[j * BCH_LD + m];  // L[m][j], a broadcast read
+    // column m of L is read no more (only columns j < m are, below): X[i][m] goes there
+    __builtin_amdgcn_wave_barrier();
+    if (lane < BCH_T) tile[m * BCH_LD + i] = xm;
+    __builtin_amdgcn_sched_barrier(0);
   }
   return fail;
 }
@@ -1232,7 +1251,14 @@ __global__ __launch_bounds__(BCH_THREADS) void small_chol_inv_kernel(int n, cons
   const int NB = (n + BCH_T - 1) / BCH_T;
   auto tile = [&](int I, int J) { return T + bch_tix(I, J) * BCH_TE; };
   if (tid == 0) fail = 0;
-  bch_load(n, NB, G, ldg, T, tid, BCH_THREADS);
+  // lower tiles of G, column j by wave, rows by lane (padding: identity)
+  for (int j = wave; j < NB * BCH_T; j += NW) {
+    const int J = j >> 4, cc = j & (BCH_T - 1);
+    for (int i = (J << 4) + lane; i < NB * BCH_T; i += 64) {
+      const double v = (i < n && j < n) ? G[i + (int64_t)j * ldg] : (i == j ? 1.0 : 0.0);
+      T[bch_at(i >> 4, J, i & (BCH_T - 1), cc)] = v;
+    }
+  }
   __syncthreads();
   for (int K = 0; K < NB; ++K) {
     if (wave == 0) {
@@ -1303,9 +1329,10 @@ __global__ __launch_bounds__(BCH_THREADS) void small_chol_inv_kernel(int n, cons
     }
     __syncthreads();
   }
-  for (int e = tid; e < n * n; e += BCH_THREADS) {
-    const int i = e % n, j = e / n;
-    Linv[i + j * ldl] = i >= j ? T[bch_at(i / BCH_T, j / BCH_T, i % BCH_T, j % BCH_T)] : 0.0;
+  for (int j = wave; j < n; j += NW) {
+    const int J = j >> 4, cc = j & (BCH_T - 1);
+    for (int i = lane; i < n; i += 64)
+      Linv[i + (int64_t)j * ldl] = i >= j ? T[bch_at(i >> 4, J, i & (BCH_T - 1), cc)] : 0.0;
   }
   if (tid == 0) *info = 0;
 }

@@ -46,8 +46,12 @@ def test_complement_solve_gives_full_svd_back(pkg):
         assert dE < 1e-12, key
         if key[2]:
             assert st["cg_calls"] == 4 and st["cg_fallbacks"] == 0 and st["cg_resid_max"] < 1e-10, st
+    # (12, 20): the kept subspace is resolved to ~1e-6 only (the CG result moves
+    # by ~30% with the rounding of the range bases); (12, 60) resolves it
     thin, cg = res[(12, 20, 0)][1], res[(12, 20, 80)][1]
-    assert cg < 2e-6 and thin > 50 * cg, (thin, cg)
+    assert cg < 2e-6 and thin > 30 * cg, (thin, cg)
+    thin, cg = res[(12, 60, 0)][1], res[(12, 60, 80)][1]
+    assert cg < 1e-7 and thin > 10 * cg, (thin, cg)
 
 
 def test_adaptive_power_iterations(pkg):
