@@ -42,6 +42,9 @@ def test_tcon_matches_einsum(pkg, cplx):
         ((65, 33), [-1, 1], (33, 129), [1, -2]),                # ragged tiles
         ((4, 1, 3), [-1, -2, 1], (3, 1), [1, -3]),              # unit dims
         ((6,), [1], (6, 5), [1, -1]),                           # vector
+        ((20, 20), [1, -1], (20, 20, 3, 2), [-2, 1, -3, -4]),   # batched over E's last legs (middle leg summed)
+        ((20, 20, 3, 2), [-1, 1, -3, -4], (20, 20), [1, -2]),   # batched, the batched operand first
+        ((20, 3, 20, 2), [-1, -3, 1, -4], (20, 17), [1, -2]),   # needs a permute: plain plan
     ]
     for sa, la, sb, lb in cases:
         A, B = r(*sa), r(*sb)
