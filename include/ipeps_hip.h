@@ -108,12 +108,16 @@ int ipeps_env_destroy(ipeps_env* env);
  * Cholesky + inverse when k <= 160, on its 2 x 2 block split when k <= 320,
  * else as 1),
  * "fast_svd" (0/1: small SVD from Gram eigenpairs with one-sided Jacobi
- * fallback; 2: the Gram eigenpairs from the one-workgroup Jacobi kernel,
- * k <= 112), "early_mixed" / "late_apps" (MIXED: products of all but the
+ * fallback, for k <= 320 the eigenpairs by device tridiagonalisation,
+ * bisection and inverse iteration; 2: the Gram eigenpairs from the
+ * one-workgroup Jacobi kernel, k <= 112; 3: as 1 with rocSOLVER stedc on the
+ * tridiagonal form), "early_mixed" / "late_apps" (MIXED: products of all but the
  * last late_apps applications of M before B = M^H Q at early_mixed = 1, 3
  * or 6 instead of the projector's setting; 0 = off), "graph_safe" (0/1: each
  * rg_step in the form a captured graph runs -- no host round trips,
- * truncation at the largest rank the cutoff rule admits, Jacobi small SVD --
+ * truncation at the largest rank the cutoff rule admits, the small SVD's
+ * Gram eigenpairs by the device tridiagonalisation, bisection and inverse
+ * iteration (k <= 320), or the Jacobi kernel with fast_svd=2 --
  * with its deferred checks validated after the step, an error if one fails;
  * needs paired_moves=0), "gram_guard" (s_min/s_max below which the Gram route is not
  * taken, default 1e-4), "power_iters_max" and "resid_tol" (adaptive power
@@ -144,9 +148,11 @@ int ipeps_env_get_draws(ipeps_env* env, int64_t* out);
  * adjoint, their CG iterations (sum), largest final relative CG residual,
  * solves abandoned for a non-positive-definite operator (thin terms used),
  * range-basis orthonormalisations, those that took CholeskyQR2's second
- * pass, those that fell back to Householder QR, projectors warm-started].
+ * pass, those that fell back to Householder QR, projectors warm-started,
+ * tridiagonal eigenpairs redone by rocSOLVER stedc after the inverse-iteration
+ * vectors failed their orthogonality check].
  * reset != 0 clears them after the read. */
-int ipeps_env_projector_stats(ipeps_env* env, double out[13], int reset);
+int ipeps_env_projector_stats(ipeps_env* env, double out[14], int reset);
 /* Copy the environment state of src into dst (same A shape): C, E
  * (converted to dst's dtype), the position in the test-matrix stream and the
  * warm-start bases, so dst continues exactly as src would -- the state a
@@ -312,7 +318,10 @@ int ipeps_svd(ipeps_ctx* ctx, int dtype, int64_t m, int64_t n, const void* A, vo
  * eigenpairs B^H B with one Newton-Schulz orthonormalisation of U, taken
  * only when S_min/S_max >= guard, else method 0; 2: as 1 with the Gram
  * eigenpairs from the one-workgroup Jacobi kernel (FP64, n <= 112: the form
- * a captured graph holds).  *method_used reports it. */
+ * a captured graph holds); 3: as 1 with the Gram eigenpairs' tridiagonal
+ * form solved by rocSOLVER stedc (method 1 solves it by bisection and inverse
+ * iteration on the device for n <= 320, stedc only as the fallback).
+ * *method_used reports it. */
 int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double guard, int64_t m, int64_t n, const void* A,
                    void* U, double* S, void* V, int* method_used);
 /* Orthonormal basis Q (m x n) of the columns of a tall Y (m >= n): the range
@@ -371,13 +380,14 @@ int ipeps_env_run_graph(ipeps_env* env, int64_t chi, int steps, int64_t out[4]);
  * ipeps_env_set_option, applied to every instance; with "graph=1" each
  * instance's rg_step is captured into a HIP graph once its shapes are stable
  * (chi saturated) and replayed from then on: the step in graph-safe form (no
- * host round trip; zero fills and copies as kernels; the one-workgroup
- * Jacobi eigensolver for the projector's small SVD; the truncation at the
+ * host round trip; zero fills and copies as kernels; the projector's small
+ * SVD from the Gram eigenpairs by the device tridiagonalisation, bisection
+ * and inverse iteration, their orthogonality check deferred; the truncation at the
  * largest rank the cutoff rule admits) with its checks (factorisation infos,
  * the cutoff rule replayed on the spectrum) validated after each replay; a
  * replay that fails them is undone and run eagerly.  Needs paired_moves=0
  * (the batch default) and the randomized projector with chi + oversample <=
- * 128; other instances stay eager. */
+ * 320; other instances stay eager. */
 int ipeps_batch_create(int device, int n, int dtype, const int64_t* dims, const void* const* A, const int64_t* chi,
                        const double* const* H1, const double* const* H2, const int64_t* hdim, int workers,
                        const char* options, ipeps_batch** out);

@@ -375,7 +375,7 @@ int ipeps_env_copy_state(ipeps_env* dst, const ipeps_env* src) {
   });
 }
 
-int ipeps_env_projector_stats(ipeps_env* env, double out[13], int reset) {
+int ipeps_env_projector_stats(ipeps_env* env, double out[14], int reset) {
   return guard([&] {
     IPEPS_CHECK(env && out, "null argument");
     auto& st = env->env.opt.svd.stats;
@@ -395,6 +395,7 @@ int ipeps_env_projector_stats(ipeps_env* env, double out[13], int reset) {
       out[10] = (double)st->orth_second;
       out[11] = (double)st->orth_householder;
       out[12] = (double)st->warm_starts;
+      out[13] = (double)st->tri_fallbacks;
     }
     if (reset) st->reset();
   });
@@ -826,7 +827,7 @@ int ipeps_svd_thin(ipeps_ctx* ctx, int dtype, int method, double gram_guard, int
   return guard([&] {
     IPEPS_CHECK(ctx && m >= n && n > 0 && A && U && S && V, "bad arguments (m >= n > 0 required)");
     IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_C128, "svd_thin: F64 or C128");
-    IPEPS_CHECK(method >= 0 && method <= 2, "svd_thin: method must be 0, 1 or 2");
+    IPEPS_CHECK(method >= 0 && method <= 3, "svd_thin: method must be 0, 1, 2 or 3");
     IPEPS_CHECK(method != 2 || (dtype == IPEPS_F64 && n <= k::JAC_MAXN_HOST), "svd_thin: method 2 needs F64, n <= 112");
     Ctx& c = *ctx->c;
     DType dt = host_dt(dtype);

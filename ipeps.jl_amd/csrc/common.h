@@ -242,6 +242,7 @@ enum CheckKind : int {
   CHECK_INFO = 3,        // one double holding an int: zero
   CHECK_CUTOFF = 4,      // spectrum (len values): the cutoff rule with chi = a, rel = x keeps b
   CHECK_GUARD = 5,       // spectrum, descending: s_0 > 0 and s_last >= x s_0
+  CHECK_AT_MOST = 6,     // one value: at most x (NaN fails)
 };
 // Starts a graph-safe step on c: checks buffer of `cap` doubles, no records.
 void checks_begin(Ctx& c, int64_t cap);

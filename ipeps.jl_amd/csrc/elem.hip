@@ -8,6 +8,7 @@
 #include "permute_impl.h"
 #include "blockchol_impl.h"
 #include "tridiag_impl.h"
+#include "trieig_impl.h"
 
 namespace ipeps {
 namespace k {
@@ -666,12 +667,25 @@ template <typename T>
 __global__ __launch_bounds__(EYE_TPB) void eye_dev_kernel(int n, const T* G, double* out) {
   __shared__ double red[EYE_TPB];
   double m = 0.0;
-  // column by column (32-bit index math): thread t covers rows t, t + 1024, ...
-  for (int col = 0; col < n; ++col) {
-    const T* g = G + (int64_t)col * n;
-    for (int r = threadIdx.x; r < n; r += EYE_TPB) {
-      const double v = dev_from_eye(g[r], r == col);
-      m = isnan(v) ? INFINITY : fmax(m, v);
+  // flat over the n x n elements (n <= 46340, so n^2 fits 31 bits): thread t
+  // covers i = t, t + 1024, ...; eight loads issued before their maxima
+  const int nn = n * n;
+  constexpr int U = 8;
+  for (int base = 0; base < nn; base += U * EYE_TPB) {
+    T g[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * EYE_TPB + (int)threadIdx.x;
+      g[u] = G[i < nn ? i : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * EYE_TPB + (int)threadIdx.x;
+      if (i < nn) {
+        const int col = i / n, r = i - col * n;
+        const double v = dev_from_eye(g[u], r == col);
+        m = isnan(v) ? INFINITY : fmax(m, v);
+      }
     }
   }
   red[threadIdx.x] = m;
@@ -1887,6 +1901,109 @@ void sytrd_q(Ctx& c, int n, const double* G, int64_t ldg, double* d, double* e, 
     else
       hipLaunchKernelGGL(orgtr_kernel<(TRI_MAXN + 63) / 64>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream, n,
                          V, tau, Q, ldq);
+  }
+  HIP_CHECK(hipGetLastError());
+}
+
+// ---- tridiagonal eigenpairs: bisection + inverse iteration (trieig_impl.h) ----
+// One wave per eigenvalue: each multisection step counts at 64 TRIEIG_PTS
+// interior points of the current interval (TRIEIG_PTS per lane, their
+// recurrences interleaved) and keeps the two points around the first one
+// whose count exceeds j.  Every wave computes the Gershgorin bounds
+// itself from the LDS copies of d and e.  Stops at the rounding of the end
+// points (relative accuracy, tri_converged).
+constexpr int TRIB_WAVES = 4;
+__global__ __launch_bounds__(64 * TRIB_WAVES) void tri_bisect_kernel(int n, const double* __restrict__ d,
+                                                                     const double* __restrict__ e,
+                                                                     double* __restrict__ lam) {
+  __shared__ double sd[TRIEIG_MAXN], se[TRIEIG_MAXN], se2[TRIEIG_MAXN];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    sd[i] = d[i];
+    const double v = i < n - 1 ? e[i] : 0.0;
+    se[i] = v;
+    se2[i] = v * v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * TRIB_WAVES + (int)(threadIdx.x >> 6);
+  if (j >= n) return;  // whole waves
+  double lo, hi, tnorm, pivmin;
+  tri_bounds(n, sd, se, lo, hi, tnorm, pivmin);
+  const double atol = tri_atol(pivmin);
+  constexpr int NP = 64 * TRIEIG_PTS;
+  for (int it = 0; it < TRIEIG_MAXIT && !tri_converged(lo, hi, atol); ++it) {
+    double s[TRIEIG_PTS];
+    int cnt[TRIEIG_PTS];
+#pragma unroll
+    for (int k = 0; k < TRIEIG_PTS; ++k) s[k] = tri_point(lo, hi, k * 64 + lane, NP);
+    tri_sturm<TRIEIG_PTS>(n, sd, se2, s, pivmin, cnt);
+    int f = NP;  // first point (k * 64 + lane order) whose count exceeds j; wave-uniform
+#pragma unroll
+    for (int k = TRIEIG_PTS - 1; k >= 0; --k) {
+      const unsigned long long above = __ballot(cnt[k] > j);
+      if (above) f = k * 64 + __builtin_ctzll(above);
+    }
+    const double nlo = f > 0 ? tri_point(lo, hi, f - 1, NP) : lo;
+    const double nhi = f < NP ? tri_point(lo, hi, f, NP) : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  if (lane == 0) lam[j] = 0.5 * (lo + hi);
+}
+
+// One thread per eigenvalue: TRIEIG_SOLVES inverse-iteration solves from a
+// pseudo-random start, each normalised (the last with the sign that makes the
+// vector's product with the start positive), the factor rows in w (4 n x n, row i
+// of eigenvalue j at w[(4 i + f) n + j]: a wave's accesses are contiguous);
+// the unit eigenvector j goes to row j of Zt (Zt[j + i n] = z_j[i]).  Small
+// blocks spread the latency-bound sweeps over more CUs.
+constexpr int TRIV_TPB = 32;
+__global__ __launch_bounds__(TRIV_TPB) void tri_invit_kernel(int n, const double* __restrict__ d,
+                                                             const double* __restrict__ e,
+                                                             const double* __restrict__ lam, double* Zt, double* w) {
+  __shared__ double sd[TRIEIG_MAXN], se[TRIEIG_MAXN];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    sd[i] = d[i];
+    se[i] = i < n - 1 ? e[i] : 0.0;
+  }
+  __syncthreads();
+  const int j = blockIdx.x * TRIV_TPB + (int)threadIdx.x;
+  if (j >= n) return;
+  double gl, gu, tnorm, pivmin;
+  tri_bounds(n, sd, se, gl, gu, tnorm, pivmin);
+  const double pert = fmax(2.220446049250313e-16 * tnorm, 1e-300);
+  const double l = lam[j];
+  double* x = Zt + j;
+  double* wj = w + j;
+  double ss = tri_invit_solve(n, j, sd, se, l, pert, 1.0, nullptr, n, x, n, wj, n);
+  for (int it = 1; it < TRIEIG_SOLVES; ++it)
+    ss = tri_invit_solve(n, j, sd, se, l, pert, 1.0 / sqrt(ss), x, n, x, n, wj, n);
+  const double sc = tri_sign_scale(n, j, x, n, ss);
+  for (int i0 = 0; i0 < n; i0 += TRIEIG_BLK) {
+    double v[TRIEIG_BLK];
+#pragma unroll
+    for (int t = 0; t < TRIEIG_BLK; ++t) v[t] = i0 + t < n ? x[(int64_t)(i0 + t) * n] : 0.0;
+#pragma unroll
+    for (int t = 0; t < TRIEIG_BLK; ++t)
+      if (i0 + t < n) x[(int64_t)(i0 + t) * n] = v[t] * sc;
+  }
+}
+
+void tri_eig(Ctx& c, int n, const double* d, const double* e, double* lam, double* Zt) {
+  static_assert(TRIEIG_MAXN == TRIEIG_MAXN_HOST, "tri_eig size limit");
+  IPEPS_CHECK(n >= 1 && n <= TRIEIG_MAXN, "tri_eig: n out of range");
+  const size_t nws = (size_t)4 * n * n;
+  std::shared_ptr<Buf> ws(new Buf{c.pool->alloc(nws * 8), nws * 8, c.pool.get()});
+  {
+    TimedLaunch tl(c, "tri_bisect_kernel", 0.0, 0.0);
+    hipLaunchKernelGGL(tri_bisect_kernel, dim3((unsigned)((n + TRIB_WAVES - 1) / TRIB_WAVES)), dim3(64 * TRIB_WAVES),
+                       0, c.stream, n, d, e, lam);
+  }
+  HIP_CHECK(hipGetLastError());
+  {
+    TimedLaunch tl(c, "tri_invit_kernel", 0.0, 0.0);
+    hipLaunchKernelGGL(tri_invit_kernel, dim3((unsigned)((n + TRIV_TPB - 1) / TRIV_TPB)), dim3(TRIV_TPB), 0, c.stream,
+                       n, d, e, lam, Zt, (double*)ws->p);
   }
   HIP_CHECK(hipGetLastError());
 }

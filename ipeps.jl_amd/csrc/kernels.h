@@ -212,6 +212,15 @@ constexpr int TRI_MAXN_HOST = 320;  // largest n of sytrd_q
 // takes for the eigenvectors of G.
 void sytrd_q(Ctx& c, int n, const double* G, int64_t ldg, double* d, double* e, double* Q, int64_t ldq);
 
+constexpr int TRIEIG_MAXN_HOST = 320;  // largest n of tri_eig
+// Eigenpairs of the symmetric tridiagonal matrix with diagonal d (n) and
+// off-diagonal e (n - 1): lam ascending by bisection (one wave per
+// eigenvalue), unit eigenvectors by inverse iteration (one thread per
+// eigenvalue) as the rows of Zt (n x n, Zt[j + i n] = component i of vector
+// j).  Vectors of (nearly) equal eigenvalues are independent but not
+// orthogonal: the caller orthonormalises and checks the set.  Kernels only.
+void tri_eig(Ctx& c, int n, const double* d, const double* e, double* lam, double* Zt);
+
 constexpr int JAC_MAXN_HOST = 112;  // largest n of jacobi_eig
 // Eigenpairs G = Q diag(lam) Q^T of a symmetric n x n FP64 matrix (n <= 112;
 // lower triangle read) in one workgroup by cyclic two-sided Jacobi: lam

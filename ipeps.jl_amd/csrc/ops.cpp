@@ -1293,14 +1293,23 @@ Tensor orth(Ctx& c, const Tensor& Y, int fast) {
 // by one Newton-Schulz step W <- W (3I - W^H W)/2.  Eigenvalues carry an
 // absolute error ~ u lambda_max, so the route is taken only when
 // s_min / s_max >= guard; returns false otherwise (outputs undefined).
-static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s, Tensor& X, bool jacobi) {
+static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s, Tensor& X, bool jacobi,
+                     bool stedc) {
   const int64_t m = B.dims[0], n = B.dims[1];
   Tensor G = matmul_raw(c, B, true, true, B, false, false);
   Tensor lam = c.empty(DType::F64, {n});
   Tensor E = c.empty(DType::F64, {n});
-  Tensor st = c.zeros(DType::F64, {1});
+  Tensor st = c.zeros(DType::F64, {2});  // [info, |Z Z^T - I|_max of the tridiagonal solver's vectors]
+  Tensor lam_used = lam;
+  Tensor Qt;  // the tridiagonal form's Q while the rocSOLVER fallback may still need it
+  auto run_stedc = [&](Tensor& Q) {
+    TimedLaunch tl(c, "rocsolver_stedc", 0.0, 0.0);
+    solver_check(rocsolver_dstedc((rocblas_handle)c.solver, rocblas_evect_original, (rocblas_int)n, lam.ptr<double>(),
+                                  E.ptr<double>(), Q.ptr<double>(), (rocblas_int)n, (rocblas_int*)st.data()),
+                 "rocsolver_dstedc");
+  };
   rocblas_handle h = (rocblas_handle)c.solver;
-  if (jacobi || c.graph_safe) {
+  if (jacobi) {
     // kernels only: the one-workgroup Jacobi eigensolver (graph-safe mode
     // defers its check)
     IPEPS_CHECK(B.dt == DType::F64 && n <= k::JAC_MAXN_HOST, "Jacobi small SVD: FP64 and k <= 112 only");
@@ -1310,15 +1319,37 @@ static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s
     G = Q;
   } else if (B.dt == DType::F64 && n <= k::TRI_MAXN_HOST) {
     // Householder tridiagonalisation and Q on the device (one workgroup + one
-    // wave per column), then rocSOLVER's divide and conquer on (d, e, Q)
+    // wave per column), then the tridiagonal eigenpairs by bisection and
+    // inverse iteration (kernels) or by rocSOLVER's divide and conquer
     Tensor Q = c.empty(DType::F64, {n, n});
     k::sytrd_q(c, (int)n, G.ptr<double>(), n, lam.ptr<double>(), E.ptr<double>(), Q.ptr<double>(), n);
-    TimedLaunch tl(c, "rocsolver_stedc", 0.0, 0.0);
-    solver_check(rocsolver_dstedc(h, rocblas_evect_original, (rocblas_int)n, lam.ptr<double>(), E.ptr<double>(),
-                                  Q.ptr<double>(), (rocblas_int)n, (rocblas_int*)st.data()),
-                 "rocsolver_dstedc");
-    G = Q;
+    IPEPS_CHECK(!(stedc && c.graph_safe), "graph-safe small SVD: no rocSOLVER stedc");
+    if (stedc) {
+      run_stedc(Q);
+      G = Q;
+    } else {
+      // T = Z diag(lam) Z^T with Z's columns the rows of Zt; two Newton-Schulz
+      // steps Zt <- (3I - Zt Zt^T) Zt / 2 make the rows orthonormal (the first
+      // Gram's distance from I, e, goes to st[1]: after the steps the distance
+      // is ~ e^4, so e <= 1e-3 is accepted; otherwise rocSOLVER redoes it)
+      lam_used = c.empty(DType::F64, {n});
+      Tensor Zt = c.empty(DType::F64, {n, n});
+      k::tri_eig(c, (int)n, lam.ptr<double>(), E.ptr<double>(), lam_used.ptr<double>(), Zt.ptr<double>());
+      Tensor S = matmul_raw(c, Zt, false, false, Zt, true, false);
+      k::eye_dev(c, DType::F64, n, S.data(), st.ptr<double>() + 1);
+      k::ns_coeff(c, DType::F64, n, S.data());
+      Tensor Z1 = matmul_raw(c, S, false, false, Zt, false, false);
+      Tensor S2 = matmul_raw(c, Z1, false, false, Z1, true, false);
+      k::ns_coeff(c, DType::F64, n, S2.data());
+      Tensor Z2 = matmul_raw(c, S2, false, false, Z1, false, false);
+      G = matmul_raw(c, Q, false, false, Z2, true, false);
+      Qt = Q;
+      // graph-safe: no fallback inside a captured step; a rejected set fails
+      // the step's deferred checks (the replay is undone and run eagerly)
+      if (c.graph_safe) record_check(c, CHECK_AT_MOST, st.ptr<double>() + 1, 1, 0, 0, 1e-3);
+    }
   } else {
+    IPEPS_CHECK(!c.graph_safe, "graph-safe small SVD: F64 and k <= 320 only");
     TimedLaunch tl(c, "rocsolver_syevd", 0.0, 0.0);
     if (B.dt == DType::F64)
       solver_check(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, G.ptr<double>(),
@@ -1335,17 +1366,27 @@ static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s
   X = c.empty(B.dt, {n, n});
   s = c.empty(DType::F64, {n});
   Tensor sinv = c.empty(DType::F64, {n});
-  k::eig_to_svd(c, B.dt, n, lam.ptr<double>(), G.data(), s.ptr<double>(), X.data(), sinv.ptr<double>());
+  k::eig_to_svd(c, B.dt, n, lam_used.ptr<double>(), G.data(), s.ptr<double>(), X.data(), sinv.ptr<double>());
   if (c.graph_safe) {
     if (guard > 0.0) record_check(c, CHECK_GUARD, s.data(), n, 0, 0, guard);
   } else {
   std::vector<double> sh(n);
-  double hinfo = 0.0;
+  double hst[2] = {0.0, 0.0};
   HIP_CHECK(hipMemcpyAsync(sh.data(), s.data(), n * 8, hipMemcpyDeviceToHost, c.stream));
-  HIP_CHECK(hipMemcpyAsync(&hinfo, st.data(), 8, hipMemcpyDeviceToHost, c.stream));
+  HIP_CHECK(hipMemcpyAsync(hst, st.data(), 16, hipMemcpyDeviceToHost, c.stream));
   c.sync();
+  if (Qt.data() && !(hst[1] <= 1e-3)) {
+    // the inverse-iteration vectors were not independent enough: rocSOLVER's
+    // divide and conquer on the same (d, e, Q)
+    orth_count(&SvdStats::tri_fallbacks);
+    run_stedc(Qt);
+    k::eig_to_svd(c, B.dt, n, lam.ptr<double>(), Qt.data(), s.ptr<double>(), X.data(), sinv.ptr<double>());
+    HIP_CHECK(hipMemcpyAsync(sh.data(), s.data(), n * 8, hipMemcpyDeviceToHost, c.stream));
+    HIP_CHECK(hipMemcpyAsync(hst, st.data(), 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  }
   int info;
-  std::memcpy(&info, &hinfo, sizeof(int));
+  std::memcpy(&info, &hst[0], sizeof(int));
   if (info != 0 || !(sh[0] > 0.0) || !(sh[n - 1] >= guard * sh[0])) return false;
   }
   Tensor BX = matmul_raw(c, B, false, false, X, false, false);
@@ -1359,7 +1400,7 @@ static bool gram_svd(Ctx& c, const Tensor& B, double guard, Tensor& W, Tensor& s
 
 int svd_tall(Ctx& c, const Tensor& B, int method, double guard, Tensor& W, Tensor& s, Tensor& X) {
   IPEPS_CHECK(B.nd() == 2 && B.dims[0] >= B.dims[1], "svd_tall: tall matrix expected");
-  if (method >= 1 && gram_svd(c, B, guard, W, s, X, method == 2)) return method;
+  if (method >= 1 && gram_svd(c, B, guard, W, s, X, method == 2, method == 3)) return method;
   IPEPS_CHECK(!c.graph_safe, "graph-safe small SVD: the Gram route only");
   gesvdj_thin(c, B, W, s, X);
   return 0;
@@ -1615,7 +1656,9 @@ SVDResult svd_trunc(Ctx& c, const Tensor& R1, const Tensor& R2, int64_t chi, con
       // route's u (s_1 / s_i)^2 for every triplet above the mixed cutoff: the
       // guard applies to FP64 engines only in graph-safe mode
       const double guard = (c.graph_safe && dt == DType::F32) ? 0.0 : o.gram_guard;
-      svd_tall(c, Bt, c.graph_safe ? 2 : o.fast_svd, guard, W, s, X);
+      // graph-safe steps take the tridiagonal route with its check deferred
+      // (fast_svd = 2 keeps the one-workgroup Jacobi eigensolver)
+      svd_tall(c, Bt, c.graph_safe ? (o.fast_svd == 2 ? 2 : 1) : o.fast_svd, guard, W, s, X);
       Uf = matmul_raw(c, Q, false, false, X, false, false);
       Sf = s;
       Vf = W;

@@ -164,10 +164,11 @@ struct SvdStats {
   long orth_second = 0;    // ... that needed CholeskyQR2's second pass
   long orth_householder = 0;  // ... that fell back to Householder QR
   long warm_starts = 0;    // projectors started from the previous one's basis
+  long tri_fallbacks = 0;  // tridiagonal eigenpairs redone by rocSOLVER stedc (inverse iteration rejected)
   void reset() {
     std::lock_guard<std::mutex> g(mu);
     calls = power_iters = power_max = checks = cg_calls = cg_iters = cg_fallbacks = 0;
-    orth_calls = orth_second = orth_householder = warm_starts = 0;
+    orth_calls = orth_second = orth_householder = warm_starts = tri_fallbacks = 0;
     resid_max = cg_resid_max = 0.0;
   }
 };
@@ -195,8 +196,13 @@ struct SvdOpts {
   //                  vectors, falling back to one-sided Jacobi (gesvdj) when
   //                  s_min / s_max < gram_guard (the Gram eigenvalues then lose
   //                  more than ~1e-8 relative accuracy);
+  //                  the Gram eigenpairs (F64, k <= 320) by the device
+  //                  tridiagonalisation, bisection and inverse iteration with
+  //                  Newton-Schulz orthonormalisation, rocSOLVER stedc when
+  //                  the vectors fail their check;
   //                  2: the same with the Gram eigenpairs from the
-  //                  one-workgroup Jacobi kernel (F64, k <= 112).
+  //                  one-workgroup Jacobi kernel (F64, k <= 112);
+  //                  3: as 1 with rocSOLVER stedc for every tridiagonal form.
   int fast_orth = 2;
   int fast_svd = 1;
   double gram_guard = 1e-4;

@@ -217,6 +217,7 @@ bool validate_checks(const std::vector<Ctx::CheckRec>& recs, const double* h, st
       }
       case CHECK_CUTOFF: ok = cutoff_keep(v, c.len, c.a, c.x) == c.b; break;
       case CHECK_GUARD: ok = c.len >= 1 && v[0] > 0.0 && v[c.len - 1] >= c.x * v[0]; break;
+      case CHECK_AT_MOST: ok = c.len >= 1 && v[0] <= c.x; break;
       default: ok = false;
     }
     if (!ok) {

@@ -205,11 +205,12 @@ class CTMTensors:
         """Counters of the randomized projectors since the last reset: dict with
         calls, power_iters, power_max, resid_max, checks, cg_calls, cg_iters,
         cg_resid_max, cg_fallbacks, orth_calls, orth_second, orth_householder,
-        warm_starts."""
-        out = np.zeros(13)
+        warm_starts, tri_fallbacks."""
+        out = np.zeros(14)
         check(lib().ipeps_env_projector_stats(self.h, _dp(out), int(bool(reset))))
         keys = ("calls", "power_iters", "power_max", "resid_max", "checks", "cg_calls", "cg_iters", "cg_resid_max",
-                "cg_fallbacks", "orth_calls", "orth_second", "orth_householder", "warm_starts")
+                "cg_fallbacks", "orth_calls", "orth_second", "orth_householder", "warm_starts",
+                "tri_fallbacks")
         return {k: (float(v) if k.endswith("max") and "power" not in k else int(v)) for k, v in zip(keys, out)}
 
     def copy_state_from(self, other):
@@ -482,8 +483,10 @@ def orth(Y, method=1, ctx=None):
 def svd_thin(A, method=1, guard=1e-4, ctx=None):
     """Thin SVD of a tall matrix on the device: (U, S, V, method_used) with
     A = U diag(S) V^H.  method 0: one-sided Jacobi; 1: Gram eigenpairs when
-    S_min/S_max >= guard (else 0); 2: as 1 with the Gram eigenpairs from the
-    one-workgroup Jacobi kernel (real, n <= 112)."""
+    S_min/S_max >= guard (else 0), for real n <= 320 by device
+    tridiagonalisation, bisection and inverse iteration; 2: as 1 with the Gram
+    eigenpairs from the one-workgroup Jacobi kernel (real, n <= 112); 3: as 1
+    with rocSOLVER stedc on the tridiagonal form."""
     ctx = ctx or default_context()
     A = np.asarray(A)
     code = C128 if np.iscomplexobj(A) else F64

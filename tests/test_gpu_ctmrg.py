@@ -138,24 +138,79 @@ def test_svd_thin_methods(pkg, cplx, decay):
         assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-12
 
 
+@pytest.mark.parametrize("method", [1, 3])
 @pytest.mark.parametrize("n", [1, 2, 3, 33, 64, 127, 128, 129, 130, 160, 161, 224, 288, 320])
-def test_svd_thin_gram_sizes(pkg, n):
-    """Gram route (device Householder tridiagonalisation + Q, rocSOLVER stedc)
-    at the sizes it serves (k <= 128 in LDS, k <= 320 starting from a global
-    workspace; tile, wave and phase edges): singular values to 1e-13 of S_max
-    against numpy, A = U S V^T, orthonormal U and V, and the kernels that ran."""
+def test_svd_thin_gram_sizes(pkg, n, method):
+    """Gram route (device Householder tridiagonalisation + Q, then the
+    tridiagonal eigenpairs by bisection + inverse iteration (method 1) or by
+    rocSOLVER stedc (method 3)) at the sizes it serves (k <= 128 in LDS,
+    k <= 320 starting from a global workspace; tile, wave and phase edges):
+    singular values to 1e-13 of S_max against numpy, A = U S V^T, orthonormal
+    U and V, and the kernels that ran."""
     rng = np.random.default_rng(n)
     m = 2000
     s0 = np.logspace(0, -2, n)
     A = (np.linalg.qr(rng.standard_normal((m, n)))[0] * s0) @ np.linalg.qr(rng.standard_normal((n, n)))[0].T
     ctx = pkg.Context()
     ctx.set_timing(1)
-    U, S, V, used = pkg.svd_thin(A, method=1, ctx=ctx)
+    U, S, V, used = pkg.svd_thin(A, method=method, ctx=ctx)
     st = ctx.kernel_stats(reset=True)
     ctx.close()
+    assert used == method
+    assert "sytrd_kernel" in st and "rocsolver_syevd" not in st
+    if method == 1:
+        assert "tri_bisect_kernel" in st and "tri_invit_kernel" in st and "rocsolver_stedc" not in st
+    else:
+        assert "rocsolver_stedc" in st and "tri_invit_kernel" not in st
+    assert np.abs(S - s0).max() < 1e-13
+    assert np.abs((U * S) @ V.T - A).max() < 1e-12
+    assert np.abs(U.T @ U - np.eye(n)).max() < 1e-12
+    assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_svd_thin_tridiagonal_solvers_agree_graded(pkg, seed):
+    """The projector's shape (k = 288 of a 5184-row B, one dominant value, a
+    decade-spanning middle and a flat tail across the cut at 144): the device
+    bisection + inverse iteration and rocSOLVER stedc give the same values
+    (1e-13 of S_max), the same kept subspace and the same truncated
+    V S^-1 U^T (to the rounding of the flat tail's relative gaps)."""
+    rng = np.random.default_rng(seed)
+    m, n, cut = 5184, 288, 144
+    s0 = np.empty(n)
+    s0[0] = 1.0
+    s0[1:cut] = np.geomspace(0.04, 2.75e-4, cut - 1)
+    s0[cut:] = np.geomspace(2.75e-4 * 0.991, 2.75e-4 * 0.59, n - cut)
+    A = (np.linalg.qr(rng.standard_normal((m, n)))[0] * s0) @ np.linalg.qr(rng.standard_normal((n, n)))[0].T
+    ctx = pkg.Context()
+    ctx.set_timing(1)
+    U1, S1, V1, u1 = pkg.svd_thin(A, method=1, ctx=ctx)
+    st = ctx.kernel_stats(reset=True)
+    U3, S3, V3, u3 = pkg.svd_thin(A, method=3, ctx=ctx)
+    ctx.close()
+    assert (u1, u3) == (1, 3)
+    assert "rocsolver_stedc" not in st
+    assert np.abs(S1 - S3).max() < 1e-13 and np.abs(S1 - s0).max() < 1e-13
+    cos_min = np.linalg.svd(U1[:, :cut].T @ U3[:, :cut], compute_uv=False).min()
+    assert 1 - cos_min < 1e-12
+    P1 = V1[:, :cut] @ np.diag(1 / S1[:cut]) @ U1[:, :cut].T
+    P3 = V3[:, :cut] @ np.diag(1 / S3[:cut]) @ U3[:, :cut].T
+    assert np.abs(P1 - P3).max() < 1e-8 * np.abs(P3).max()
+    assert np.abs(U1.T @ U1 - np.eye(n)).max() < 1e-12
+    assert np.abs(V1.T @ V1 - np.eye(n)).max() < 1e-12
+
+
+def test_svd_thin_repeated_values_fall_back(pkg):
+    """Exactly repeated singular values: the inverse-iteration vectors of a
+    repeated Gram eigenvalue need not be independent, so the route checks them
+    and hands the tridiagonal form to rocSOLVER stedc when they are not; either
+    way the result is an SVD to working precision."""
+    rng = np.random.default_rng(9)
+    m, n = 600, 48
+    s0 = np.repeat(np.logspace(0, -2, 12), 4)
+    A = (np.linalg.qr(rng.standard_normal((m, n)))[0] * s0) @ np.linalg.qr(rng.standard_normal((n, n)))[0].T
+    U, S, V, used = pkg.svd_thin(A, method=1)
     assert used == 1
-    assert "sytrd_kernel" in st
-    assert "rocsolver_stedc" in st and "rocsolver_syevd" not in st
     assert np.abs(S - s0).max() < 1e-13
     assert np.abs((U * S) @ V.T - A).max() < 1e-12
     assert np.abs(U.T @ U - np.eye(n)).max() < 1e-12
@@ -501,8 +556,9 @@ def test_fast_solver_routes_match_exact(pkg, dtype):
     # the Gram route runs (calls whose tail falls below the guard go to gesvdj);
     # CholeskyQR2 on the blocked one-workgroup Cholesky for FP64, rocSOLVER for C128
     chol = "small_chol_inv_kernel" if dtype == "f64" else "rocsolver_potrf"
-    eig = "rocsolver_stedc" if dtype == "f64" else "rocsolver_syevd"
+    eig = "tri_invit_kernel" if dtype == "f64" else "rocsolver_syevd"
     assert eig in st and chol in st
+    assert "rocsolver_stedc" not in st
     assert "rocsolver_geqrf_orgqr" not in st
     assert "rocsolver_gesvdj" in st_exact and "rocsolver_geqrf_orgqr" in st_exact
     print("dE", abs(Ef - Ee), "grad rel", np.abs(gf - ge).max() / np.abs(ge).max())
