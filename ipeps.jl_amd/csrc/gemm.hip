@@ -1564,6 +1564,13 @@ void gemm(Ctx& c, const GemmArgs& g) {
     {
       std::lock_guard<std::mutex> lk(T.mu);
       auto it = T.table.find(key);
+      if (it == T.table.end() && g.beta != 0.0) {
+        // an accumulating product (beta != 0) takes the plan of the same
+        // product without accumulation when only that one is in the table
+        GemmArgs g0 = g;
+        g0.beta = 0.0;
+        it = T.table.find(tune_key(g0));
+      }
       if (it != T.table.end() && plan_valid(g, it->second)) {
         pl = it->second;
         found = true;

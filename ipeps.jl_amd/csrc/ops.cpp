@@ -102,6 +102,26 @@ void acc(Ctx& c, const std::shared_ptr<Node>& n, const Tensor& g) {
   k::axpy(c, g.dt, g.numel(), 1.0, g.data(), n->grad.data());
 }
 
+// The cotangent n already holds when a contribution may be added to it in
+// place (same segment of a pair, its buffer not shared), else null.
+static Tensor* acc_target(const std::shared_ptr<Node>& n) {
+  if (!n || !n->grad.defined() || n->grad.buf.use_count() > 1) return nullptr;
+  if (Deferral* d = deferral())
+    if (n->seg != d->seg) return nullptr;
+  return &n->grad;
+}
+
+// acc(c, n, tcon_raw(...)) with the product added to n's cotangent by its GEMM
+// when acc would add it there anyway (no temporary, no separate sum pass).
+static void acc_tcon(Ctx& c, const std::shared_ptr<Node>& n, const Tensor& A, const std::vector<int>& la, bool conjA,
+                     const Tensor& B, const std::vector<int>& lb, bool conjB) {
+  Tensor* into = acc_target(n);
+  const void* held = into ? into->data() : nullptr;
+  Tensor r = tcon_raw(c, A, la, conjA, B, lb, conjB, into);
+  if (held && r.data() == held) return;
+  acc(c, n, r);
+}
+
 static void back_node(Ctx& c, Node& n) {
   if (!n.back) return;
   bool any = n.grad.defined();
@@ -366,7 +386,7 @@ Tensor matmul_raw(Ctx& c, const Tensor& A, bool tA, bool cA, const Tensor& B, bo
 }
 
 Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA, const Tensor& B,
-                const std::vector<int>& lb, bool conjB) {
+                const std::vector<int>& lb, bool conjB, Tensor* into) {
   const int na = A.nd(), nb = B.nd();
   IPEPS_CHECK((int)la.size() == na && (int)lb.size() == nb, "tcon: label count does not match tensor rank");
   IPEPS_CHECK(A.dt == B.dt, "tcon: dtype mismatch");
@@ -474,7 +494,10 @@ Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA,
   };
   if (!best.ct) { push(best.FA, 0); push(best.FB, 1); }
   else { push(best.FB, 1); push(best.FA, 0); }
-  Tensor R = c.empty(A.dt, rdims);
+  // accumulate into *into when the GEMM writes the result's own layout
+  const bool acc_in = into && into->defined() && into->dt == A.dt && into->numel() == nC && nC > 0 && K > 0 &&
+                      layout_is_identity(res, odims);
+  Tensor R = acc_in ? *into : c.empty(A.dt, rdims);
   if (nC > 0) {
     k::GemmArgs g;
     g.dt = A.dt;
@@ -491,9 +514,11 @@ Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA,
       g.B = Am.data(); g.transB = !a_kxm; g.ldb = std::max<int64_t>(1, a_kxm ? K : M); g.conjB = cjA;
       g.C = R.data(); g.ldc = std::max<int64_t>(1, N);
     }
+    if (acc_in) g.beta = 1.0;
     if (K == 0) k::fill_zero(c, R.data(), R.bytes());
     else k::gemm(c, g);
   }
+  if (acc_in) return *into;
   if (layout_is_identity(res, odims)) return ipeps::reshape(R, odims);
   // R has legs res[0..]; output wants leg kk at position kk
   std::vector<int> perm(nout);
@@ -542,8 +567,8 @@ Tensor tcon(Ctx& c, const Tensor& A, const std::vector<int>& la, const Tensor& B
     const k::Mixed mx = prec().back;
     C.node = make_node([=](Ctx& c, Node& n) {
       PrecScope ps(mx);
-      if (gA) acc(c, nA, tcon_raw(c, n.grad, lc_a, false, Bk, lb_a, true));
-      if (gB) acc(c, nB, tcon_raw(c, Ak, la_b, true, n.grad, lc_b, false));
+      if (gA) acc_tcon(c, nA, n.grad, lc_a, false, Bk, lb_a, true);
+      if (gB) acc_tcon(c, nB, Ak, la_b, true, n.grad, lc_b, false);
     });
   }
   return C;

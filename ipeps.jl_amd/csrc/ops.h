@@ -97,8 +97,11 @@ std::shared_ptr<Node> make_node(std::function<void(Ctx&, Node&)> back);
 // ---- raw (non-recording) helpers -----------------------------------------
 Tensor clone(Ctx& c, const Tensor& t);
 Tensor permute_raw(Ctx& c, const Tensor& t, const std::vector<int>& perm, bool conj = false);
+// With `into` (same dtype and element count as the result): when the product
+// comes out of its GEMM in the result's own leg order, it is added to *into by
+// the GEMM (beta = 1) and *into is returned; otherwise a new tensor as usual.
 Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA, const Tensor& B,
-                const std::vector<int>& lb, bool conjB);
+                const std::vector<int>& lb, bool conjB, Tensor* into = nullptr);
 Tensor to_dtype(Ctx& c, const Tensor& t, DType dt);
 // C = op(A) op(B) for matrices (column-major), raw.
 Tensor matmul_raw(Ctx& c, const Tensor& A, bool tA, bool cA, const Tensor& B, bool tB, bool cB);
