@@ -1041,104 +1041,136 @@ static std::shared_ptr<Buf> split_operand_new(Ctx& c, const float* X, int64_t ld
 // M = chi^2 D^2 d, N, K in {D^2, D^2 d}).  Such a product moves 4 (M K + M N)
 // bytes for 2 M N K flops, well below the ridge point of any MFMA type, so it
 // is HBM-bound: FP32 MFMA (v_mfma_f32_16x16x4f32, FP32 products) costs nothing
-// extra and needs no bf16 planes.  Persistent blocks hold op(B) in LDS for the
-// whole launch and walk 64-row tiles of op(A): the tile is read with
-// consecutive threads along the contiguous leg into LDS, each wave computes 16
-// rows x all N columns, and the tile of C is staged in LDS and written one
-// 64-row column segment at a time.
+// extra and needs no bf16 planes.  A streaming kernel: op(B) sits in LDS for
+// the whole launch (zero beyond K and N); each wave of the persistent blocks
+// walks 32-row groups of op(A) and reads them straight into MFMA operand
+// registers, the next group's loads in flight during the current group's
+// MFMAs; the accumulators go straight to C (four consecutive rows per lane,
+// 16-B stores where C allows).  The K axis is taken in chunks of 16: lane group
+// g of MFMA s in chunk h holds k = 16 h + 4 g + s, so a k-contiguous op(A)
+// gives each lane one 16-B load per chunk and row; op(B) is read with the same
+// k mapping.
 // ---------------------------------------------------------------------------
-constexpr int LK_BM = 64, LK_KMAX = 96, LK_NMAX = 96;
+constexpr int LK_KMAX = 96, LK_NMAX = 96, LK_KC = LK_KMAX / 16, LK_NT = LK_NMAX / 16;
+constexpr int LK_TPB = 256, LK_RG = 32;  // threads per block, rows per wave group
 
-// LDS (dynamic, sized by the launch): As[k LDA + m] for max(K4, 16 NT) rows
-// (reused as Cs[n LDA + m] for the output), then Bs[k LDB + n] with
-// LDB = 16 NT + 1, zero beyond K and N.
-__host__ __device__ inline int lowk_arows(int K, int N) { return max((K + 3) / 4 * 4, (N + 15) / 16 * 16); }
 __host__ __device__ inline int lowk_lds_floats(int K, int N) {
-  return lowk_arows(K, N) * (LK_BM + 1) + (K + 3) / 4 * 4 * ((N + 15) / 16 * 16 + 1);
+  return (K + 15) / 16 * 16 * ((N + 15) / 16 * 16 + 1);
 }
 
-__global__ __launch_bounds__(256) void gemm_lowk_f32_kernel(GemmP p, int transA, int transB) {
-  constexpr int LDA = LK_BM + 1, NTMAX = LK_NMAX / 16;
+// The 32-row group's A fragments: a[t][h][s] = op(A)[row 16 t + lr][k = 16 h + 4 lg + s]
+// of rows m0 .. m0+31 (rows past M read row M - 1; k past K read a valid
+// element, which meets a zero row of op(B)).
+template <int TA>
+__device__ inline void lowk_fetch(const float* __restrict__ A, int lda, int M, int K, int KC, int m0, int lr, int lg,
+                                  bool vec, float (&a)[2][LK_KC][4]) {
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int m = min(m0 + 16 * t + lr, M - 1);
+#pragma unroll
+    for (int h = 0; h < LK_KC; ++h) {
+      if (h >= KC) break;
+      const int k0 = 16 * h + 4 * lg;
+      if (TA) {  // k contiguous: A[k + m lda]
+        if (vec && k0 + 3 < K) {
+          const float4 v = *reinterpret_cast<const float4*>(A + k0 + (int64_t)m * lda);
+          a[t][h][0] = v.x; a[t][h][1] = v.y; a[t][h][2] = v.z; a[t][h][3] = v.w;
+        } else {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) a[t][h][s] = A[min(k0 + s, K - 1) + (int64_t)m * lda];
+        }
+      } else {  // m contiguous: A[m + k lda]
+#pragma unroll
+        for (int s = 0; s < 4; ++s) a[t][h][s] = A[m + (int64_t)min(k0 + s, K - 1) * lda];
+      }
+    }
+  }
+}
+
+template <int TA>
+__global__ __launch_bounds__(LK_TPB) void gemm_lowk_f32_kernel(GemmP p, int transB) {
   extern __shared__ float lk_smem[];
   const int M = (int)p.M, N = (int)p.N, K = (int)p.K;
-  const int K4 = (K + 3) / 4 * 4, NT = (N + 15) / 16, LDB = 16 * NT + 1;
-  float* As = lk_smem;
-  float* Bs = lk_smem + lowk_arows(K, N) * LDA;
+  const int KC = (K + 15) / 16, NT = (N + 15) / 16, LDB = 16 * NT + 1;
+  float* Bs = lk_smem;  // Bs[k LDB + n], k < 16 KC
   const float* A = (const float*)p.A;
   const float* B = (const float*)p.B;
   float* C = (float*)p.C;
   const int lda = (int)p.lda, ldb = (int)p.ldb, ldc = (int)p.ldc;
   const float alpha = (float)p.alpha, beta = (float)p.beta;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lg = lane >> 4;
-  for (int e = tid; e < K4 * LDB; e += 256) {
+  for (int e = tid; e < 16 * KC * LDB; e += LK_TPB) {
     const int k = e / LDB, n = e - k * LDB;
     float v = 0.f;
     if (k < K && n < N) v = transB ? B[n + (int64_t)k * ldb] : B[k + (int64_t)n * ldb];
     Bs[e] = v;
   }
-  const int ntile = (M + LK_BM - 1) / LK_BM;
-  // a tile of op(A) as PER values per thread: element e = tid + 256 j is
-  // (k = e / BM, m = e % BM) for A (m contiguous) and (m = e / K4, k = e % K4)
-  // for A^T (k contiguous); every load is issued before any is used, and the
-  // next tile's loads are in flight during the current tile's MFMAs
-  constexpr int PER = LK_KMAX * LK_BM / 256;
-  float v[PER];
-  auto fetch = [&](int t) {
-    const int m0 = t * LK_BM, mrows = min(LK_BM, M - m0);
+  __syncthreads();
+  const bool vecA = TA && ((uintptr_t)A & 15) == 0 && lda % 4 == 0;
+  const bool vecC = ((uintptr_t)C & 15) == 0 && ldc % 4 == 0;
+  const int ngroups = (M + LK_RG - 1) / LK_RG;
+  const int gstride = (int)gridDim.x * (LK_TPB / 64);
+  int g = (int)blockIdx.x * (LK_TPB / 64) + wave;
+  float a[2][LK_KC][4], an[2][LK_KC][4];
+  if (g < ngroups) lowk_fetch<TA>(A, lda, M, K, KC, g * LK_RG, lr, lg, vecA, a);
+  for (; g < ngroups; g += gstride) {
+    const int gn = g + gstride;
+    if (gn < ngroups) lowk_fetch<TA>(A, lda, M, K, KC, gn * LK_RG, lr, lg, vecA, an);
+    v4f32 acc[2][LK_NT];
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const int e = tid + 256 * j;
-      int k, m;
-      if (!transA) { k = e / LK_BM; m = e - k * LK_BM; }
-      else { m = e / K4; k = e - m * K4; }
-      const bool ok = e < K4 * LK_BM && k < K && m < mrows;
-      const int64_t idx = !transA ? (int64_t)(m0 + m) + (int64_t)k * lda : (int64_t)k + (int64_t)(m0 + m) * lda;
-      const float x = A[ok ? idx : 0];
-      v[j] = ok ? x : 0.f;
-    }
-  };
-  if ((int)blockIdx.x < ntile) fetch(blockIdx.x);
-  for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
-    const int m0 = t * LK_BM, mrows = min(LK_BM, M - m0);
-    __syncthreads();  // Bs written (first tile); the previous tile's Cs consumed
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-      const int e = tid + 256 * j;
-      if (e < K4 * LK_BM) {
-        int k, m;
-        if (!transA) { k = e / LK_BM; m = e - k * LK_BM; }
-        else { m = e / K4; k = e - m * K4; }
-        As[k * LDA + m] = v[j];
+      for (int j = 0; j < LK_NT; ++j) acc[t][j] = (v4f32){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int h = 0; h < LK_KC; ++h) {
+      if (h >= KC) break;
+#pragma unroll
+      for (int sidx = 0; sidx < 4; ++sidx) {
+        const float* brow = Bs + (16 * h + 4 * lg + sidx) * LDB + lr;
+#pragma unroll
+        for (int j = 0; j < LK_NT; ++j) {
+          if (j >= NT) break;
+          const float b = brow[16 * j];
+#pragma unroll
+          for (int t = 0; t < 2; ++t)
+            acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t][h][sidx], b, acc[t][j], 0, 0, 0);
+        }
       }
     }
-    __syncthreads();
-    if (t + (int)gridDim.x < ntile) fetch(t + gridDim.x);
-    v4f32 acc[NTMAX];
+    // accumulator element i of lane l: row 16 t + 4 lg + i, column 16 j + lr
+    const int m0 = g * LK_RG;
 #pragma unroll
-    for (int j = 0; j < NTMAX; ++j) acc[j] = (v4f32){0.f, 0.f, 0.f, 0.f};
-    const int mw = 16 * wave + lr;
-    for (int k0 = 0; k0 < K4; k0 += 4) {
-      const float a = As[(k0 + lg) * LDA + mw];
+    for (int t = 0; t < 2; ++t) {
+      const int row0 = m0 + 16 * t + 4 * lg;
+      if (row0 >= M) continue;
 #pragma unroll
-      for (int j = 0; j < NTMAX; ++j)
-        if (j < NT) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bs[(k0 + lg) * LDB + 16 * j + lr], acc[j], 0, 0, 0);
-    }
-    __syncthreads();  // every wave is done reading As
-    // accumulator element i of lane l: row 16 wave + 4 lg + i, column 16 j + lr
+      for (int j = 0; j < LK_NT; ++j) {
+        const int col = 16 * j + lr;
+        if (j >= NT || col >= N) continue;
+        float* dst = C + row0 + (int64_t)col * ldc;
+        if (vecC && row0 + 3 < M) {
+          float4 v = make_float4(alpha * acc[t][j][0], alpha * acc[t][j][1], alpha * acc[t][j][2], alpha * acc[t][j][3]);
+          if (beta != 0.f) {
+            const float4 o = *reinterpret_cast<const float4*>(dst);
+            v.x += beta * o.x; v.y += beta * o.y; v.z += beta * o.z; v.w += beta * o.w;
+          }
+          *reinterpret_cast<float4*>(dst) = v;
+        } else {
 #pragma unroll
-    for (int j = 0; j < NTMAX; ++j)
-      if (j < NT)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) As[(16 * j + lr) * LDA + 16 * wave + 4 * lg + i] = acc[j][i];
-    __syncthreads();
-    for (int e = tid; e < N * LK_BM; e += 256) {
-      const int n = e / LK_BM, m = e - n * LK_BM;
-      if (m < mrows) {
-        float* c = C + (int64_t)(m0 + m) + (int64_t)n * ldc;
-        const float o = alpha * As[n * LDA + m];
-        *c = beta != 0.f ? o + beta * *c : o;
+          for (int i = 0; i < 4; ++i)
+            if (row0 + i < M) {
+              const float o = alpha * acc[t][j][i];
+              dst[i] = beta != 0.f ? o + beta * dst[i] : o;
+            }
+        }
       }
     }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int h = 0; h < LK_KC; ++h)
+#pragma unroll
+        for (int sidx = 0; sidx < 4; ++sidx) a[t][h][sidx] = an[t][h][sidx];
   }
 }
 
@@ -1152,17 +1184,21 @@ static void gemm_lowk(Ctx& c, const GemmArgs& g) {
   GemmP p{};
   p.M = g.M; p.N = g.N; p.K = g.K; p.alpha = g.alpha; p.beta = g.beta;
   p.A = g.A; p.lda = g.lda; p.B = g.B; p.ldb = g.ldb; p.C = g.C; p.ldc = g.ldc;
-  const int64_t ntile = (g.M + LK_BM - 1) / LK_BM;
+  const int64_t ngroups = (g.M + LK_RG - 1) / LK_RG;
   const int lds = lowk_lds_floats((int)g.K, (int)g.N) * 4;
-  // persistent blocks: up to three per CU (139 VGPRs), as many as the LDS holds
-  const int per_cu = std::max(1, std::min(3, (160 * 1024) / lds));
-  const unsigned grid = (unsigned)std::min<int64_t>(ntile, 256 * per_cu);
+  // persistent blocks, LK_TPB / 64 row groups each per round: up to two per CU
+  const int per_cu = std::max(1, std::min(2, (160 * 1024) / lds));
+  const unsigned grid =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>((ngroups + LK_TPB / 64 - 1) / (LK_TPB / 64), 256 * per_cu));
   std::string name = "gemm_lowk_f32_kernel";
   if (c.timing_detail)
     name += " M=" + std::to_string(g.M) + " N=" + std::to_string(g.N) + " K=" + std::to_string(g.K);
   TimedLaunch tl(c, name, 2.0 * (double)g.M * g.N * g.K,
                  ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N * (g.beta != 0.0 ? 2 : 1)) * 4.0);
-  hipLaunchKernelGGL(gemm_lowk_f32_kernel, dim3(grid), dim3(256), lds, c.stream, p, (int)g.transA, (int)g.transB);
+  if (g.transA)
+    hipLaunchKernelGGL(gemm_lowk_f32_kernel<1>, dim3(grid), dim3(LK_TPB), lds, c.stream, p, (int)g.transB);
+  else
+    hipLaunchKernelGGL(gemm_lowk_f32_kernel<0>, dim3(grid), dim3(LK_TPB), lds, c.stream, p, (int)g.transB);
   HIP_CHECK(hipGetLastError());
 }
 
