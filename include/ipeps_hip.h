@@ -264,7 +264,9 @@ int ipeps_es_grad_path(ipeps_env* env, const ipeps_es_path* path, const double* 
 /* tcon (src/utility/wrapper.jl:15-23): ncon-style pairwise contraction on the
  * device.  Positive labels are summed, negative ones are output legs -1, -2,
  * ...  dimsC receives the output dims (room for 16), C_host the result
- * (capacity C_cap elements). */
+ * (capacity C_cap elements).  dtype IPEPS_F64 / IPEPS_C128, or IPEPS_MIXED
+ * for float host data stored and multiplied in FP32 (the mixed engine's
+ * storage; FP32 products). */
 int ipeps_tcon(ipeps_ctx* ctx, int dtype, int ndA, const int64_t* dimsA, const void* A, const int* la, int ndB,
                const int64_t* dimsB, const void* B, const int* lb, int64_t* dimsC, int* ndC, void* C_host,
                int64_t C_cap);

@@ -689,12 +689,22 @@ int ipeps_tcon(ipeps_ctx* ctx, int dtype, int ndA, const int64_t* dimsA, const v
                int64_t C_cap) {
   return guard([&] {
     IPEPS_CHECK(ctx && dimsA && dimsB && la && lb && ndA >= 0 && ndB >= 0, "bad arguments");
-    IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_C128, "tcon: F64 or C128");
+    IPEPS_CHECK(dtype == IPEPS_F64 || dtype == IPEPS_C128 || dtype == IPEPS_MIXED, "tcon: F64, C128 or MIXED");
     Ctx& c = *ctx->c;
-    DType dt = host_dt(dtype);
+    // MIXED: float host data, FP32 storage and FP32 products (the "6" setting)
+    const DType dt = dtype == IPEPS_MIXED ? DType::F32 : host_dt(dtype);
     Tensor tA = upload(c, dt, dt, std::vector<int64_t>(dimsA, dimsA + ndA), A);
     Tensor tB = upload(c, dt, dt, std::vector<int64_t>(dimsB, dimsB + ndB), B);
-    Tensor C = op::tcon_raw(c, tA, std::vector<int>(la, la + ndA), false, tB, std::vector<int>(lb, lb + ndB), false);
+    op::Prec saved = op::prec();
+    if (dtype == IPEPS_MIXED) op::prec().mixed = k::Mixed::BF16X6;
+    Tensor C;
+    try {
+      C = op::tcon_raw(c, tA, std::vector<int>(la, la + ndA), false, tB, std::vector<int>(lb, lb + ndB), false);
+    } catch (...) {
+      op::prec() = saved;
+      throw;
+    }
+    op::prec() = saved;
     IPEPS_CHECK(C.nd() <= 16, "tcon: output rank > 16");
     if (ndC) *ndC = C.nd();
     if (dimsC) for (int i = 0; i < C.nd(); ++i) dimsC[i] = C.dims[i];

@@ -348,15 +348,19 @@ def run_gs_vjp(ts, x, H, cfg, dE=1.0):
     return complex(E[0], E[1]), g, n.value
 
 
-def tcon(xs, ind_xs, ctx=None):
-    """Pairwise ncon contraction on the device (src/utility/wrapper.jl:15-23)."""
+def tcon(xs, ind_xs, ctx=None, fp32=False):
+    """Pairwise ncon contraction on the device (src/utility/wrapper.jl:15-23).
+    fp32=True: real data stored and multiplied in FP32 (the mixed engine's
+    storage; FP32 products), float32 result."""
     if len(xs) != 2:
         raise ValueError("tcon contracts two tensors")
     ctx = ctx or default_context()
     A, B = (np.asarray(x) for x in xs)
     cplx = np.iscomplexobj(A) or np.iscomplexobj(B)
-    code = C128 if cplx else F64
-    dt = _host_np(code)
+    if fp32 and cplx:
+        raise ValueError("tcon: fp32 takes real data")
+    code = MIXED if fp32 else (C128 if cplx else F64)
+    dt = np.float32 if fp32 else _host_np(code)
     Af, Bf = np.asfortranarray(A, dtype=dt), np.asfortranarray(B, dtype=dt)
     la = (C.c_int * max(1, A.ndim))(*ind_xs[0])
     lb = (C.c_int * max(1, B.ndim))(*ind_xs[1])

@@ -54,6 +54,26 @@ def test_tcon_matches_einsum(pkg, cplx):
         np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(ref).max()))
 
 
+def test_permutes_bit_exact_fp32(pkg):
+    """FP32 leg permutations (the mixed engine's storage) through tcon with a
+    one-element partner, bit for bit against numpy: the tiled kernel on 16-byte
+    words (extents and strides multiples of four: the chi x chi x D^2 edge
+    transposes and the corner layouts of the energy network) and on elements
+    (odd extents), and the rows kernel."""
+    rng = np.random.default_rng(12)
+    cases = [((144, 144, 36), (1, 0, 2)), ((36, 144, 144), (2, 0, 1)), ((64, 12, 8, 4), (2, 3, 0, 1)),
+             ((8, 12, 6, 2, 4), (4, 1, 3, 0, 2)), ((68, 4, 132), (2, 1, 0)), ((65, 64, 3), (1, 0, 2)),
+             ((36, 36, 2), (1, 0, 2)), ((6, 36, 12), (1, 0, 2)), ((144, 6, 6, 6), (0, 2, 1, 3)),
+             ((70, 3, 129), (2, 0, 1)), ((4, 4, 4), (2, 1, 0))]
+    for shp, perm in cases:
+        X = rng.standard_normal(shp).astype(np.float32)
+        la = [-(int(np.where(np.array(perm) == a)[0][0]) + 1) for a in range(len(shp))]
+        got = pkg.tcon([X, np.ones(1, dtype=np.float32)], [la, [-(len(shp) + 1)]], fp32=True)
+        ref = np.transpose(X, perm)[..., None]
+        assert got.dtype == np.float32 and got.shape == ref.shape, (shp, perm)
+        assert np.array_equal(got, ref), (shp, perm)
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 def test_permutes_bit_exact(pkg, cplx):
     """Leg permutations (rows and LDS-tiled permute kernels) through tcon with a
