@@ -49,6 +49,10 @@ struct GemmP {
   int vecA, vecB, vecC;
   // FP32 kernel, pre-split operands: elements per bf16 plane
   int64_t planeA, planeB;
+  // output offsets (batch 1, no split-K): element (m, n) goes to
+  // C[offM[m] + offN[n]] instead of C[m + n ldc] (a permuted result layout)
+  const int* offM;
+  const int* offN;
 };
 
 constexpr int BM = 64, BN = 64;
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(256) void gemm_real_kernel(GemmP p) {
         int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
         if (row < p.M && col < p.N) {
           double v = p.alpha * (double)acc[i][j][r];
-          T* dst = C + row + col * p.ldc;
+          T* dst = p.offM ? C + p.offM[row] + p.offN[col] : C + row + col * p.ldc;
           if (p.beta != 0.0) v += p.beta * (double)*dst;
           *dst = (T)v;
         }
@@ -598,6 +602,18 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(GemmP p) {
       const int64_t col = n0 + wn * WTN + 16 * j + lr;
       const int64_t row0 = m0 + wm * WTM + 16 * i + 4 * (lane >> 4);
       if (col >= p.N || row0 >= p.M) continue;
+      if (p.offM) {
+        const int on = p.offN[col];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (row0 + r < p.M) {
+            float* d = C + p.offM[row0 + r] + on;
+            float v = al * acc[i][j][r];
+            if (p.beta != 0.0) v += be * *d;
+            *d = v;
+          }
+        continue;
+      }
       float* dst = C + row0 + col * p.ldc;
       if (p.vecC && row0 + 3 < p.M) {
         float4 v = make_float4(al * acc[i][j][0], al * acc[i][j][1], al * acc[i][j][2], al * acc[i][j][3]);
@@ -766,6 +782,18 @@ __global__ __launch_bounds__(256) void gemm_f32x_kernel(GemmP p) {
       const int64_t col = n0 + wn * WTN + 16 * j + lr;
       const int64_t row0 = m0 + wm * WTM + 16 * i + 4 * (lane >> 4);
       if (col >= p.N || row0 >= p.M) continue;
+      if (p.offM) {
+        const int on = p.offN[col];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (row0 + r < p.M) {
+            float* d = C + p.offM[row0 + r] + on;
+            float v = al * acc[i][j][r];
+            if (p.beta != 0.0) v += be * *d;
+            *d = v;
+          }
+        continue;
+      }
       float* dst = C + row0 + col * p.ldc;
       if (p.vecC && row0 + 3 < p.M) {
         float4 v = make_float4(al * acc[i][j][0], al * acc[i][j][1], al * acc[i][j][2], al * acc[i][j][3]);
@@ -806,7 +834,8 @@ __device__ inline float axpby_(double al, float s, double be, float c) { return 
 // S slice loads are independent and issued together.
 template <typename T>
 __global__ __launch_bounds__(256) void splitk_reduce(unsigned M, int64_t MN, int S, const T* __restrict__ W,
-                                                     double alpha, double beta, T* __restrict__ C, int64_t ldc) {
+                                                     double alpha, double beta, T* __restrict__ C, int64_t ldc,
+                                                     const int* __restrict__ offM, const int* __restrict__ offN) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= MN) return;
   const unsigned n = (unsigned)t / M, m = (unsigned)t - n * M;
@@ -817,7 +846,7 @@ __global__ __launch_bounds__(256) void splitk_reduce(unsigned M, int64_t MN, int
     acc = add_(add_(add_(add_(acc, w0), w1), w2), w3);
   }
   for (; s < S; ++s) acc = add_(acc, W[s * MN + t]);
-  T* dst = C + m + (int64_t)n * ldc;
+  T* dst = offM ? C + offM[m] + offN[n] : C + m + (int64_t)n * ldc;
   *dst = beta != 0.0 ? axpby_(alpha, acc, beta, *dst) : axpby_(alpha, acc, 0.0, zero_of<T>());
 }
 
@@ -1147,6 +1176,17 @@ __global__ __launch_bounds__(LK_TPB) void gemm_lowk_f32_kernel(GemmP p, int tran
       for (int j = 0; j < LK_NT; ++j) {
         const int col = 16 * j + lr;
         if (j >= NT || col >= N) continue;
+        if (p.offM) {
+          const int on = p.offN[col];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (row0 + i < M) {
+              float* d = C + p.offM[row0 + i] + on;
+              const float o = alpha * acc[t][j][i];
+              *d = beta != 0.f ? o + beta * *d : o;
+            }
+          continue;
+        }
         float* dst = C + row0 + (int64_t)col * ldc;
         if (vecC && row0 + 3 < M) {
           float4 v = make_float4(alpha * acc[t][j][0], alpha * acc[t][j][1], alpha * acc[t][j][2], alpha * acc[t][j][3]);
@@ -1184,6 +1224,7 @@ static void gemm_lowk(Ctx& c, const GemmArgs& g) {
   GemmP p{};
   p.M = g.M; p.N = g.N; p.K = g.K; p.alpha = g.alpha; p.beta = g.beta;
   p.A = g.A; p.lda = g.lda; p.B = g.B; p.ldb = g.ldb; p.C = g.C; p.ldc = g.ldc;
+  p.offM = g.offM; p.offN = g.offN;
   const int64_t ngroups = (g.M + LK_RG - 1) / LK_RG;
   const int lds = lowk_lds_floats((int)g.K, (int)g.N) * 4;
   // persistent blocks, LK_TPB / 64 row groups each per round: up to two per CU
@@ -1255,6 +1296,8 @@ static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
   p.C = g.C; p.ldc = g.ldc; p.sC = g.sC;
   p.conjA = g.conjA; p.conjB = g.conjB;
   p.kchunk = 0;
+  p.offM = g.offM;
+  p.offN = g.offN;
   const int64_t tbm = tile_m(g, pl), tbn = tile_n(g, pl);
   set_vec_flags(g, p);
   p.planeA = p.planeB = 0;
@@ -1317,6 +1360,7 @@ static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
     GemmP q = p;
     q.alpha = 1.0;
     q.beta = 0.0;
+    q.offM = q.offN = nullptr;  // the reduction applies the output offsets
     q.C = ws->p;
     q.ldc = g.M;
     q.sC = g.M * g.N;
@@ -1357,13 +1401,13 @@ static void execute(Ctx& c, const GemmArgs& g, const Plan& pl, bool timed) {
     }
     if (g.dt == DType::F64)
       hipLaunchKernelGGL(splitk_reduce<double>, dim3(rb), dim3(256), 0, c.stream, (unsigned)g.M, MN, (int)S,
-                         (const double*)ws->p, g.alpha, g.beta, (double*)g.C, g.ldc);
+                         (const double*)ws->p, g.alpha, g.beta, (double*)g.C, g.ldc, g.offM, g.offN);
     else if (g.dt == DType::C128)
       hipLaunchKernelGGL(splitk_reduce<double2>, dim3(rb), dim3(256), 0, c.stream, (unsigned)g.M, MN, (int)S,
-                         (const double2*)ws->p, g.alpha, g.beta, (double2*)g.C, g.ldc);
+                         (const double2*)ws->p, g.alpha, g.beta, (double2*)g.C, g.ldc, g.offM, g.offN);
     else
       hipLaunchKernelGGL(splitk_reduce<float>, dim3(rb), dim3(256), 0, c.stream, (unsigned)g.M, MN, (int)S,
-                         (const float*)ws->p, g.alpha, g.beta, (float*)g.C, g.ldc);
+                         (const float*)ws->p, g.alpha, g.beta, (float*)g.C, g.ldc, g.offM, g.offN);
     HIP_CHECK(hipGetLastError());
     return;
   }
@@ -1568,8 +1612,43 @@ void tune_load(const std::string& path) {
   for (auto& kv : t) tuner().table[kv.first] = kv.second;
 }
 
+namespace {
+struct MixedRadixP {
+  int nd;
+  int64_t ext[16], stride[16];
+};
+__global__ __launch_bounds__(256) void mixed_radix_offsets_kernel(MixedRadixP p, int64_t count, int* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) {
+    int64_t r = i, off = 0;
+    for (int a = 0; a < p.nd; ++a) {
+      const int64_t q = r / p.ext[a];
+      off += (r - q * p.ext[a]) * p.stride[a];
+      r = q;
+    }
+    out[i] = (int)off;
+  }
+}
+}  // namespace
+
+void mixed_radix_offsets(Ctx& c, int nd, const int64_t* ext, const int64_t* stride, int64_t count, int* out) {
+  IPEPS_CHECK(nd >= 0 && nd <= 16, "mixed_radix_offsets: at most 16 axes");
+  if (count <= 0) return;
+  MixedRadixP p{};
+  p.nd = nd;
+  for (int a = 0; a < nd; ++a) {
+    p.ext[a] = ext[a];
+    p.stride[a] = stride[a];
+  }
+  const unsigned grid = (unsigned)std::min<int64_t>((count + 255) / 256, 65535);
+  TimedLaunch tl(c, "mixed_radix_offsets_kernel", 0.0, 4.0 * (double)count);
+  hipLaunchKernelGGL(mixed_radix_offsets_kernel, dim3(grid), dim3(256), 0, c.stream, p, count, out);
+  HIP_CHECK(hipGetLastError());
+}
+
 void gemm(Ctx& c, const GemmArgs& g) {
   if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return;
+  IPEPS_CHECK(!g.offM || ((g.dt == DType::F32 || g.dt == DType::F64) && g.batch == 1 && g.K > 0 && g.offN),
+              "gemm: output offsets need F32 / F64, batch 1 and K > 0");
   if (g.K <= 0) {
     IPEPS_CHECK(g.beta == 0.0 || g.beta == 1.0, "gemm: K=0 with beta not in {0,1}");
     if (g.beta == 0.0) {

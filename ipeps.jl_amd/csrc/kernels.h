@@ -35,8 +35,16 @@ struct GemmArgs {
   int64_t ldc = 0, sC = 0;
   int64_t batch = 1;
   Mixed mixed = Mixed::BF16X3;
+  // output offsets (F32 / F64, batch 1): element (m, n) of the product goes to
+  // C[offM[m] + offN[n]] (device int arrays of M and N entries) instead of
+  // C[m + n ldc]: a product written straight into a permuted layout
+  const int* offM = nullptr;
+  const int* offN = nullptr;
 };
 void gemm(Ctx& c, const GemmArgs& g);
+// offsets[i] = sum_a digit_a(i) stride[a] for i < count, the digits of i in
+// the mixed radix ext[0..nd-1] (first fastest); int32 offsets (< 2^31)
+void mixed_radix_offsets(Ctx& c, int nd, const int64_t* ext, const int64_t* stride, int64_t count, int* out);
 // GEMM autotuner (process-wide table of launch plans keyed by dtype, split,
 // transposes, shape, beta and operand alignment).  mode 0: shape rules,
 // 1: time the candidate plans of each unseen shape with at least min_flops

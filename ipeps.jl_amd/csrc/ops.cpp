@@ -494,10 +494,24 @@ Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA,
   };
   if (!best.ct) { push(best.FA, 0); push(best.FB, 1); }
   else { push(best.FB, 1); push(best.FA, 0); }
+  // A result whose legs come out of the GEMM in another order is written by
+  // the GEMM straight into the output layout through row / column offset
+  // tables (real data, the output's fastest leg the GEMM's fastest row leg
+  // and at least 16 long, so a wave's stores stay in 64-B runs), instead of a
+  // separate permute pass.
+  const bool ident = layout_is_identity(res, odims);
+  int64_t nrow_legs = 0;  // R's row legs: res[0 .. nrow_legs)
+  {
+    int64_t p = 1;
+    while (nrow_legs < (int64_t)res.size() && p < (best.ct ? N : M)) p *= odims[res[nrow_legs++]];
+  }
+  const bool direct = !ident && nC > 0 && K > 0 && (A.dt == DType::F32 || A.dt == DType::F64) && nrow_legs >= 1 &&
+                      res[0] == 0 && odims[0] >= 16 && nC < ((int64_t)1 << 31);
   // accumulate into *into when the GEMM writes the result's own layout
   const bool acc_in = into && into->defined() && into->dt == A.dt && into->numel() == nC && nC > 0 && K > 0 &&
-                      layout_is_identity(res, odims);
-  Tensor R = acc_in ? *into : c.empty(A.dt, rdims);
+                      (ident || direct);
+  Tensor R = acc_in ? *into : c.empty(A.dt, (ident || !direct) ? rdims : odims);
+  std::shared_ptr<Buf> offs;
   if (nC > 0) {
     k::GemmArgs g;
     g.dt = A.dt;
@@ -515,11 +529,35 @@ Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA,
       g.C = R.data(); g.ldc = std::max<int64_t>(1, N);
     }
     if (acc_in) g.beta = 1.0;
+    if (direct) {
+      // output strides (column-major odims) of R's legs
+      std::vector<int64_t> ost(nout);
+      int64_t st = 1;
+      for (int kk = 0; kk < nout; ++kk) {
+        ost[kk] = st;
+        st *= odims[kk];
+      }
+      std::vector<int64_t> rext, rstr, cext, cstr;
+      for (int64_t i = 0; i < (int64_t)res.size(); ++i) {
+        (i < nrow_legs ? rext : cext).push_back(odims[res[i]]);
+        (i < nrow_legs ? rstr : cstr).push_back(ost[res[i]]);
+      }
+      const int64_t gm = g.M, gn = g.N;
+      IPEPS_CHECK((int64_t)std::accumulate(rext.begin(), rext.end(), (int64_t)1, std::multiplies<int64_t>()) == gm,
+                  "tcon: output row legs do not match the GEMM rows");
+      const size_t ob = (size_t)(gm + gn) * sizeof(int);
+      offs.reset(new Buf{c.pool->alloc(ob), ob, c.pool.get()});
+      int* om = (int*)offs->p;
+      k::mixed_radix_offsets(c, (int)rext.size(), rext.data(), rstr.data(), gm, om);
+      k::mixed_radix_offsets(c, (int)cext.size(), cext.data(), cstr.data(), gn, om + gm);
+      g.offM = om;
+      g.offN = om + gm;
+    }
     if (K == 0) k::fill_zero(c, R.data(), R.bytes());
     else k::gemm(c, g);
   }
   if (acc_in) return *into;
-  if (layout_is_identity(res, odims)) return ipeps::reshape(R, odims);
+  if (ident || direct) return ipeps::reshape(R, odims);
   // R has legs res[0..]; output wants leg kk at position kk
   std::vector<int> perm(nout);
   for (int kk = 0; kk < nout; ++kk) {
