@@ -328,9 +328,10 @@ def main():
                     help="products of the range finder's early applications of M (all but the last --late-apps "
                          "before B = M^H Q; 0: same as --mixed-proj)")
     ap.add_argument("--late-apps", type=int, default=1)
-    ap.add_argument("--tune", type=int, default=2, choices=[0, 1, 2],
+    ap.add_argument("--tune", type=int, default=2, choices=[0, 1, 2, 3],
                     help="GEMM launch plans: 0 shape rules, 1 autotune now (unseen shapes are timed during the "
-                         "warm-up steps), 2 replay the tuning table --tune-file (shapes missing from it: rules)")
+                         "warm-up steps), 2 replay the tuning table --tune-file (shapes missing from it: rules), "
+                         "3 replay the table and autotune the shapes it lacks (with --save-tune: extend it)")
     ap.add_argument("--tune-file", default=TUNE_TABLE)
     ap.add_argument("--save-tune", default=None, help="write the tuning table after the run")
     # k = chi + 144 = 2 chi and 4 power iterations resolve the kept subspace of
@@ -421,6 +422,10 @@ def main():
     if a.tune == 2:
         pkg.tune_load(a.tune_file)
         pkg.tune(2)
+    elif a.tune == 3:
+        # the committed table, and the shapes it lacks tuned in the run
+        pkg.tune_load(a.tune_file)
+        pkg.tune(1, 1e8)
     elif a.tune == 1:
         pkg.tune(1, 1e8)
     log(f"[bench] workload {a.workload}: D={D} chi={chi} N={chi*D*D}; mixed {mixed_opts}")
@@ -506,7 +511,9 @@ def main():
                    "mixed_proj": a.mixed_proj or a.mixed,
                    "early_mixed": a.early_mixed, "late_apps": a.late_apps,
                    "gemm_plans": {0: "shape rules", 1: "autotuned in this run",
-                                  2: "tuning table " + os.path.relpath(a.tune_file, ROOT)}[a.tune]},
+                                  2: "tuning table " + os.path.relpath(a.tune_file, ROOT),
+                                  3: "tuning table " + os.path.relpath(a.tune_file, ROOT) + " + autotuned in this run"}[
+                                      a.tune]},
         "algorithmic_flops_per_step": F,
         "model_flops_rate_tflops": F * value / 1e12,
         "roofline": roof,

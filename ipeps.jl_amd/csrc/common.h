@@ -204,6 +204,11 @@ struct Ctx {
   };
   std::vector<std::pair<const char*, size_t>> split_src;  // registered [p, p + bytes)
   std::vector<SplitEntry> split_cache;
+  // Output offset tables of products written straight into a permuted
+  // layout, keyed by the legs' extents and strides (eager steps only; at most
+  // OUT_OFFSETS_MAX layouts, then the cache starts over).
+  static constexpr size_t OUT_OFFSETS_MAX = 256;
+  std::map<std::vector<int64_t>, std::shared_ptr<Buf>> out_offsets;
   int64_t checks_pos = 0;
   std::vector<CheckRec> check_recs;
 

@@ -459,7 +459,13 @@ Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA,
         };
         if (!ct) { push(FA, 0); push(FB, 1); }
         else { push(FB, 1); push(FA, 0); }
-        int64_t costC = layout_is_identity(res, odims) ? 0 : nC;
+        // a result the GEMM can write straight into the output layout (real
+        // data, its fastest row leg the output's fastest leg, >= 16 long:
+        // see below) costs its offset tables, not a permute pass
+        const bool real = A.dt == DType::F32 || A.dt == DType::F64;
+        const int64_t rows = ct ? N : M;
+        const bool directable = real && !res.empty() && res[0] == 0 && odims[0] >= 16 && rows > 1;
+        int64_t costC = layout_is_identity(res, odims) ? 0 : directable ? nC / 8 : nC;
         int64_t cost = costA + costB + costC;
         if (cost < best.cost) best = Choice{cost, CA, CB, FA, FB, (bool)ct};
       }
@@ -545,13 +551,32 @@ Tensor tcon_raw(Ctx& c, const Tensor& A, const std::vector<int>& la, bool conjA,
       const int64_t gm = g.M, gn = g.N;
       IPEPS_CHECK((int64_t)std::accumulate(rext.begin(), rext.end(), (int64_t)1, std::multiplies<int64_t>()) == gm,
                   "tcon: output row legs do not match the GEMM rows");
-      const size_t ob = (size_t)(gm + gn) * sizeof(int);
-      offs.reset(new Buf{c.pool->alloc(ob), ob, c.pool.get()});
-      int* om = (int*)offs->p;
-      k::mixed_radix_offsets(c, (int)rext.size(), rext.data(), rstr.data(), gm, om);
-      k::mixed_radix_offsets(c, (int)cext.size(), cext.data(), cstr.data(), gn, om + gm);
-      g.offM = om;
-      g.offN = om + gm;
+      // the tables depend only on the legs' extents and output strides: kept
+      // per context (a step reuses each layout many times)
+      std::vector<int64_t> key;
+      key.push_back((int64_t)rext.size());
+      key.insert(key.end(), rext.begin(), rext.end());
+      key.insert(key.end(), rstr.begin(), rstr.end());
+      key.insert(key.end(), cext.begin(), cext.end());
+      key.insert(key.end(), cstr.begin(), cstr.end());
+      // (graph-safe steps build their own: a block handed out during a
+      // capture belongs to the graph, and its kernels run only at replay)
+      auto it = c.graph_safe ? c.out_offsets.end() : c.out_offsets.find(key);
+      if (it != c.out_offsets.end()) {
+        offs = it->second;
+      } else {
+        const size_t ob = (size_t)(gm + gn) * sizeof(int);
+        offs.reset(new Buf{c.pool->alloc(ob), ob, c.pool.get()});
+        int* om = (int*)offs->p;
+        k::mixed_radix_offsets(c, (int)rext.size(), rext.data(), rstr.data(), gm, om);
+        k::mixed_radix_offsets(c, (int)cext.size(), cext.data(), cstr.data(), gn, om + gm);
+        if (!c.graph_safe) {
+          if (c.out_offsets.size() >= Ctx::OUT_OFFSETS_MAX) c.out_offsets.clear();
+          c.out_offsets.emplace(key, offs);
+        }
+      }
+      g.offM = (const int*)offs->p;
+      g.offN = (const int*)offs->p + gm;
     }
     if (K == 0) k::fill_zero(c, R.data(), R.bytes());
     else k::gemm(c, g);

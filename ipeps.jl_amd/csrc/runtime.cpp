@@ -289,6 +289,7 @@ void ctx_destroy(Ctx* c) {
   // tensors the context itself holds go back to its pool before the pool goes
   c->checks = Tensor();
   c->split_cache.clear();
+  c->out_offsets.clear();
   c->check_recs.clear();
   c->pool.reset();
   (void)hipStreamDestroy(c->stream);
